@@ -1,0 +1,241 @@
+/*
+ * softbox_hip.h -- C ABI of libsoftbox_hip.so, the MI355X (gfx950) mechanics hot path of SoftBox.
+ *
+ * This boundary replaces the TornadoVM TaskGraph dispatch of SoftBox's per-timestep rigid-rod
+ * Systems.  Each System entry point takes the same arrays, in the same order, as the Java
+ * System method it replaces (cited beside it), so a java.lang.foreign downcall can stand in
+ * for `.task(name, System::method, args...)`:
+ *   - big per-body / per-link arrays are DEVICE pointers (allocated with sbx_alloc or any HIP
+ *     allocator on the context's device), planar SoA with plane stride n exactly as the
+ *     reference stores (RigidRodBody.java:26-57, FilamentStore.java:29-87);
+ *   - the tiny scalar arrays the reference re-uploads every execution (`params`, `counts`,
+ *     `chainParams`, `gridParams`, ...) are HOST pointers read at call time (the
+ *     EVERY_EXECUTION transfer becomes kernel arguments; no device copy);
+ *   - array lengths the Java code takes from getSize() are explicit int arguments.
+ * All calls are asynchronous on the context's stream; sbx_pull and sbx_sync synchronise.
+ *
+ * Errors: every int-returning function returns SBX_OK (0) or a negative SBX_ERR_*.  Argument
+ * errors are reported and leave the context usable; HIP/runtime failures are sticky (every
+ * later call returns the same code until sbx_clear_error).  sbx_last_error gives the text.
+ * Nothing aborts the host process.  One host thread drives one context.
+ */
+#ifndef SOFTBOX_HIP_H
+#define SOFTBOX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SBX_OK 0
+#define SBX_ERR_ARG -1      /* invalid argument (size, null pointer, out-of-range parameter) */
+#define SBX_ERR_HIP -2      /* HIP runtime failure (sticky) */
+#define SBX_ERR_NOMEM -3    /* device allocation failed (sticky) */
+#define SBX_ERR_STATE -4    /* call not valid in the current state */
+
+#define SBX_RNG_WANG 0      /* reference stream: Wang hash keyed (i*1000003)^(step*999983)^(seed*7919) */
+#define SBX_RNG_PHILOX 1    /* counter-based Philox4x32-10 keyed (seed, stream) / (body, step, block) */
+
+#define SBX_MAX_CAND 256    /* SpatialGrid.MAX_CAND (SpatialGrid.java:48) */
+#define SBX_XL_STRIDE 12    /* CrosslinkerStore.STRIDE (CrosslinkerStore.java:33) */
+#define SBX_STRAIN_WIN 10   /* CrosslinkerStore.STRAIN_WIN (CrosslinkerStore.java:64) */
+#define SBX_TORQUE_WIN 5    /* CrosslinkerStore.TORQUE_WIN (CrosslinkerStore.java:118) */
+
+typedef struct sbx_ctx sbx_ctx;
+typedef struct sbx_net sbx_net;
+
+/* ------------------------------------------------------------------ context and memory
+ * Replaces TornadoExecutionPlan / DataTransferMode plumbing: FIRST_EXECUTION ->
+ * sbx_push once, EVERY_EXECUTION -> host scalar arrays passed per call, UNDER_DEMAND -> sbx_pull. */
+int sbx_create(sbx_ctx **out, int device, void *hip_stream); /* hip_stream NULL: library-owned stream */
+void sbx_destroy(sbx_ctx *c);
+const char *sbx_last_error(const sbx_ctx *c);
+void sbx_clear_error(sbx_ctx *c);
+int sbx_sync(sbx_ctx *c);
+int sbx_alloc(sbx_ctx *c, void **dptr, size_t bytes);
+int sbx_free(sbx_ctx *c, void *dptr);
+int sbx_push(sbx_ctx *c, void *dst_dev, const void *src_host, size_t bytes);   /* transferToDevice */
+int sbx_pull(sbx_ctx *c, void *dst_host, const void *src_dev, size_t bytes);   /* transferToHost (synchronous) */
+int sbx_copy(sbx_ctx *c, void *dst_dev, const void *src_dev, size_t bytes);
+int sbx_fill_i32(sbx_ctx *c, int *dst_dev, int value, size_t count);           /* kernel fill (graph-safe) */
+const char *sbx_version(void);
+
+/* ------------------------------------------------------------------ per-body systems */
+/* ChainBendingForceSystem.zeroAccumulators (ChainBendingForceSystem.java:38-45) */
+int sbx_zero_accumulators(sbx_ctx *c, float *forceSum, float *torqueSum, int n);
+
+/* BrownianForceSystem.brownianForce (BrownianForceSystem.java:183-245); params[1] = sqrt(2kT/dt),
+ * counts[1] = step, counts[2] = seed.  rng/stream select the random stream (stream: replica key). */
+int sbx_brownian_force(sbx_ctx *c, float *randForce, float *randTorque, const float *bTransGam, const float *bRotGam,
+                       const float *brownTransScale, const float *brownRotScale, const float *params,
+                       const int *counts, int n, int rng, uint32_t stream);
+
+/* RigidRodLangevinIntegrationSystem.integrate (RigidRodLangevinIntegrationSystem.java:33-112); params[0] = dt */
+int sbx_integrate(sbx_ctx *c, float *coord, float *uVec, float *yVec, const float *forceSum, const float *torqueSum,
+                  const float *randForce, const float *randTorque, const float *bTransGam, const float *bRotGam,
+                  const float *params, const int *counts, int n);
+
+/* DerivedGeometrySystem.derive (DerivedGeometrySystem.java:140-186) */
+int sbx_derive(sbx_ctx *c, const float *coord, const float *uVec, float *yVec, float *zVec, float *end1, float *end2,
+               const float *segLength, const int *counts, int n);
+
+/* MechanicsFusion.forceBuild (MechanicsFusion.java:52-114) = zeroAccumulators + brownianForce */
+int sbx_force_build(sbx_ctx *c, float *forceSum, float *torqueSum, float *randForce, float *randTorque,
+                    const float *bTransGam, const float *bRotGam, const float *brownTransScale,
+                    const float *brownRotScale, const float *params, const int *counts, int n, int rng,
+                    uint32_t stream);
+
+/* MechanicsFusion.integDerive (MechanicsFusion.java:121-201) */
+int sbx_integ_derive(sbx_ctx *c, float *coord, float *uVec, float *yVec, float *zVec, const float *segLength,
+                     float *end1, float *end2, const float *forceSum, const float *torqueSum, const float *randForce,
+                     const float *randTorque, const float *bTransGam, const float *bRotGam, const float *params,
+                     int n);
+
+/* MechanicsFusion.integDeriveConfined (MechanicsFusion.java:209-338); boxParams[7] = dt, counts[1] = step */
+int sbx_integ_derive_confined(sbx_ctx *c, float *coord, float *uVec, float *yVec, float *zVec,
+                              const float *segLength, float *end1, float *end2, float *forceSum, float *torqueSum,
+                              const float *randForce, const float *randTorque, const float *bTransGam,
+                              const float *bRotGam, const float *boxParams, const int *counts, int n);
+
+/* ContainmentSystem.confine (ContainmentSystem.java:58-126) */
+int sbx_confine(sbx_ctx *c, const float *coord, const float *uVec, const float *segLength, const float *bTransGam,
+                float *forceSum, float *torqueSum, const float *boxParams, const int *counts, int n);
+
+/* ChainBendingForceSystem.chainForces (ChainBendingForceSystem.java:105-331) */
+int sbx_chain_forces(sbx_ctx *c, const float *coord, const float *uVec, const float *segLength,
+                     const int *end2NbrSlot, const int *end2NbrSide, const int *end1NbrSlot, const int *end1NbrSide,
+                     const float *bTransGam, const float *bRotGam, float *forceSum, float *torqueSum,
+                     const float *chainParams, const int *counts, int n);
+
+/* ------------------------------------------------------------------ spatial grid
+ * gridParams (host): [0..2] xMin yMin zMin, [3] cellSize, [4] invCellSize, [5] cutoff
+ * gridDims (host):   [0..2] nX nY nZ, [3] totalCells;  counts (host): [1] = S active bodies
+ * periodic: 0 = reference clamped grid; 1 = periodic extension (cells wrap, minimum image,
+ * boxLen = nBins * cellSize per axis, needs nBins >= 3). */
+
+/* FilamentStore.publishToBodyView (FilamentStore.java:275-295); viewParams[0] = actin radius, cap = view capacity */
+int sbx_publish_filaments(sbx_ctx *c, const float *coord, const float *segLength, float *center,
+                          float *boundingRadius, int *ownerStore, int *ownerSlot, const float *viewParams,
+                          int n, int cap);
+
+/* SpatialGrid.bodyCell (SpatialGrid.java:51-75) */
+int sbx_body_cell(sbx_ctx *c, const float *center, const float *gridParams, const int *gridDims, const int *counts,
+                  int *bodyCell, int cap, int periodic);
+
+/* The whole stable counting sort of SpatialGrid.gridChunkZero/Histogram/Reduce + gridScanLocal/Chunks/Add +
+ * gridChunkScatter (SpatialGrid.java:99-160, 237-331): gridCellOffsets[totalCells+1] and gridCellContents[cap],
+ * bodies ascending within a cell, terminator clamped to contentsCap (:155-158). */
+int sbx_grid_build(sbx_ctx *c, const int *bodyCell, const int *counts, const int *gridDims, int *gridCellOffsets,
+                   int *gridCellContents, int contentsCap);
+
+/* SpatialGrid.broadPhase (SpatialGrid.java:340-406); candPartner[cap*maxCand], candCount[cap] (may exceed maxCand) */
+int sbx_broad_phase(sbx_ctx *c, const float *center, const float *boundingRadius, const int *bodyCell,
+                    const int *gridCellOffsets, const int *gridCellContents, const int *gridDims,
+                    const float *gridParams, const int *counts, int *candPartner, int *candCount, int cap,
+                    int maxCand, int periodic);
+
+/* ------------------------------------------------------------------ CSR inverse and gathers */
+/* CrossBridgeSystem.csrHistogram/csrScan/csrScatter and csrChunk* (CrossBridgeSystem.java:190-298): stable
+ * counting sort of items by key (key < 0 skipped): offsets[nKeys+1], list[nItems] ascending item index. */
+int sbx_csr_build(sbx_ctx *c, const int *key, int nItems, int nKeys, int *offsets, int *list);
+
+/* CrosslinkerSystem.countActiveLinks (CrosslinkerSystem.java:694-706) */
+int sbx_count_active_links(sbx_ctx *c, const int *linkState, const int *linkFilA, const int *linkFilB,
+                           int *activeLinkCount, int C, int nSeg);
+
+/* CrosslinkerSystem.unbind (CrosslinkerSystem.java:231-281); counts: [0] nLinks [1] step [2] seed */
+int sbx_link_unbind(sbx_ctx *c, const float *filCoord, const float *filUVec, const float *filEnd1,
+                    const int *linkFilA, const int *linkFilB, const float *loc1, const float *loc2, int *linkState,
+                    float *strainHist, int *strainPlace, const float *offParams, const int *counts, int nSeg,
+                    int rng, uint32_t stream);
+
+/* CrosslinkerSystem.linkForces (CrosslinkerSystem.java:51-124) */
+int sbx_link_forces(sbx_ctx *c, const float *filCoord, const float *filUVec, const float *filEnd1,
+                    const float *filBTransGam, const int *linkFilA, const int *linkFilB, const float *loc1,
+                    const float *loc2, const int *filLinkCt, float *xlinkData, const float *xlParams, int nSeg,
+                    int nLinks);
+
+/* CrosslinkerSystem.linkTorsion (CrosslinkerSystem.java:767-808) */
+int sbx_link_torsion(sbx_ctx *c, const float *filUVec, const int *linkFilA, const int *linkFilB,
+                     const int *linkState, const int *linkOrientSame, float *torqueMagHist, int *torqueMagPlace,
+                     float *xlinkData, const float *torsionParams, int nSeg, int nLinks);
+
+/* CrosslinkerSystem.segGatherA / segGatherB (CrosslinkerSystem.java:129-173); side 0 = A, 1 = B */
+int sbx_seg_gather(sbx_ctx *c, const int *segOffsets, const int *segLinkIdx, const float *xlinkData,
+                   const int *linkState, float *filForceSum, float *filTorqueSum, int nSeg, int side);
+
+/* ------------------------------------------------------------------ whole network step
+ * Replaces the filament block of DenseContractileHarness.buildPlan / cpuStep (DenseContractileHarness.java:
+ * 350-429, 460-606) as ONE library call per step: [publish + grid + broad phase] -> [crosslink unbind ->
+ * count -> force + torsion] -> per-rod fused zero + Brownian + chain + gather A + gather B + confine +
+ * integrate + derive.  The store arrays stay caller-owned device memory in the reference layout. */
+typedef struct sbx_filaments {
+    int n;
+    float *coord, *uVec, *yVec;               /* canonical pose (updated in place; see sbx_net_publish) */
+    float *zVec, *end1, *end2;                /* derived, written every step */
+    float *segLength, *bTransGam, *bRotGam, *brownTransScale, *brownRotScale;
+    float *forceSum, *torqueSum;              /* optional: NULL keeps the accumulated F/T in registers */
+    int *end1NbrSlot, *end1NbrSide, *end2NbrSlot, *end2NbrSide;
+} sbx_filaments;
+
+typedef struct sbx_crosslinks {
+    int nLinks;                                /* pool capacity C; 0 = no crosslinks */
+    int *linkFilA, *linkFilB;
+    float *loc1, *loc2;
+    int *linkState;
+    float *strainHist;                         /* C * SBX_STRAIN_WIN */
+    int *strainPlace;
+    int *linkOrientSame;
+    float *torqueMagHist;                      /* C * SBX_TORQUE_WIN */
+    int *torqueMagPlace;
+    float *xlinkData;                          /* C * SBX_XL_STRIDE */
+    int *activeLinkCount;                      /* nSeg */
+} sbx_crosslinks;
+
+typedef struct sbx_net_params {
+    float params[2];        /* FilamentStore.params: dt, sqrt(2kT/dt) */
+    float chainParams[7];   /* FilamentStore.chainParams (FilamentStore.java:83-87) */
+    int chainOn;
+    float xlParams[3];      /* CrosslinkerStore.xlParams: restLength, fracMoveBase, dt */
+    float offParams[5];     /* CrosslinkerStore.offParams: const, coeff, exp, dt, restLength */
+    float torsionParams[2]; /* CrosslinkerStore.torsionParams: spring, active */
+    int unbindOn;
+    float boxParams[8];     /* MechanicsFusion boxParams (MechanicsFusion.java:29-31) */
+    int confineOn;
+    int gridOn;             /* 1: grid + broad phase every step (per-step rebuild) */
+    int periodic;
+    int maxCand;            /* per-body candidate slice (SpatialGrid.MAX_CAND = 256) */
+    float gridParams[6];
+    int gridDims[4];
+    float boxLen[3];
+    float actinRadius;      /* viewParams[0] of publishToBodyView */
+    int seed;
+    int rng;
+    uint32_t stream;
+} sbx_net_params;
+
+int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *xl, const sbx_net_params *p,
+                   sbx_net **out);
+void sbx_net_destroy(sbx_net *net);
+/* advance nSteps steps; step counters step0 .. step0+nSteps-1 (counts[1] of the reference) */
+int sbx_net_step(sbx_net *net, int step0, int nSteps);
+/* make the caller's coord/uVec/yVec hold the current pose (the step ping-pongs pose buffers) */
+int sbx_net_publish(sbx_net *net);
+/* rebuild the crosslink CSR after the caller changed linkFilA/linkFilB (formation) */
+int sbx_net_topology_changed(sbx_net *net);
+/* device pointers of the latest broad-phase output and grid CSR */
+int sbx_net_candidates(sbx_net *net, int **candPartner, int **candCount, int **gridCellOffsets,
+                       int **gridCellContents, int **bodyCell);
+/* per-kernel timing with HIP events on the context stream: on=1 records every launch of sbx_net_step */
+int sbx_net_set_profiling(sbx_net *net, int on);
+/* accumulated [kernel count] names / total ms / launches; returns number of kernels written (<= cap) */
+int sbx_net_kernel_times(sbx_net *net, const char **names, double *total_ms, long long *launches, int cap);
+/* 1: capture one step as a hipGraph and replay it (re-captured when nSteps patterns change) */
+int sbx_net_use_graph(sbx_net *net, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,152 @@
+"""ctypes binding of the CPU oracle (liboracle_softbox.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg as the checker.  The product path (softbox_amd) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_softbox.so")
+
+vp = C.c_void_p
+i32 = C.c_int
+u32 = C.c_uint32
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle_softbox.so"], check=True)
+    return LIB_PATH
+
+
+class SboNet(C.Structure):
+    _fields_ = (
+        [("nRod", i32)]
+        + [(k, vp) for k in ("coord", "uVec", "yVec", "zVec", "end1", "end2", "segLength", "bTransGam", "bRotGam",
+                             "forceSum", "torqueSum", "randForce", "randTorque", "brownTransScale", "brownRotScale",
+                             "end1NbrSlot", "end1NbrSide", "end2NbrSlot", "end2NbrSide")]
+        + [("params", C.c_float * 2), ("chainParams", C.c_float * 7), ("chainOn", i32), ("nLinks", i32)]
+        + [(k, vp) for k in ("linkFilA", "linkFilB", "linkState", "strainPlace", "linkOrientSame", "torqueMagPlace",
+                             "activeLinkCount", "loc1", "loc2", "strainHist", "torqueMagHist", "xlinkData")]
+        + [("xlParams", C.c_float * 3), ("offParams", C.c_float * 5), ("torsionParams", C.c_float * 2),
+           ("unbindOn", i32)]
+        + [(k, vp) for k in ("segCountA", "segOffsetsA", "segIdxA", "segCountB", "segOffsetsB", "segIdxB")]
+        + [("confineOn", i32), ("boxParams", C.c_float * 8), ("gridOn", i32), ("periodic", i32), ("maxCand", i32),
+           ("gridParams", C.c_float * 6), ("boxLen", C.c_float * 3), ("actinRadius", C.c_float),
+           ("gridDims", C.c_int * 4)]
+        + [(k, vp) for k in ("center", "boundingRadius", "bodyCell", "cellCount", "gridCellOffsets",
+                             "gridCellContents", "candPartner", "candCount")]
+        + [("seed", i32), ("rng", i32), ("stream", u32)]
+    )
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = C.CDLL(LIB_PATH)
+        lib.sbo_const.restype = C.c_double
+        lib.sbo_wang_hash.restype = u32
+        lib.sbo_wang_hash.argtypes = [u32]
+        lib.sbo_event_bits.restype = u32
+        lib.sbo_event_bits.argtypes = [i32, u32, i32, i32, u32, u32]
+        lib.sbo_uniform.restype = C.c_float
+        lib.sbo_uniform.argtypes = [u32]
+        for name in ("sbo_accurate_acos", "sbo_accurate_asin"):
+            getattr(lib, name).restype = C.c_double
+            getattr(lib, name).argtypes = [C.c_double]
+        lib.sbo_angle_from_sin_cos.restype = C.c_double
+        lib.sbo_angle_from_sin_cos.argtypes = [C.c_double, C.c_double]
+        lib.sbo_rod_drag_si.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+        lib.sbo_sphere_drag_si.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
+        lib.sbo_network_step.argtypes = [C.POINTER(SboNet), i32]
+        lib.sbo_brownian_bits.argtypes = [i32, u32, i32, i32, u32, C.POINTER(u32)]
+        lib.sbo_philox4x32_10.argtypes = [C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+        _lib = lib
+    return _lib
+
+
+def P(a: np.ndarray | None) -> C.c_void_p:
+    if a is None:
+        return C.c_void_p(0)
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+def call(name: str, *args):
+    """Call an oracle function with numpy arrays (passed as pointers) and python scalars."""
+    lib = load()
+    conv = []
+    for a in args:
+        if isinstance(a, np.ndarray):
+            conv.append(P(a))
+        elif a is None:
+            conv.append(C.c_void_p(0))
+        elif isinstance(a, float):
+            conv.append(C.c_float(a))
+        elif isinstance(a, (np.float32,)):
+            conv.append(C.c_float(float(a)))
+        elif isinstance(a, (int, np.integer)):
+            conv.append(C.c_int(int(a)) if -2**31 <= int(a) < 2**31 else C.c_uint32(int(a) & 0xffffffff))
+        else:
+            conv.append(a)
+    return getattr(lib, name)(*conv)
+
+
+class OracleNet:
+    """sbo_network_step over host copies of a Scene (DenseContractileHarness.cpuStep filament block)."""
+
+    def __init__(self, scene):
+        fs, xl, p = scene.fs.copy(), (scene.xl.copy() if scene.xl is not None else None), scene.p
+        self.fs, self.xl, self.p = fs, xl, p
+        n = fs.n
+        s = SboNet()
+        s.nRod = n
+        for k in ("coord", "uVec", "yVec", "zVec", "end1", "end2", "segLength", "bTransGam", "bRotGam",
+                  "forceSum", "torqueSum", "randForce", "randTorque", "brownTransScale", "brownRotScale",
+                  "end1NbrSlot", "end1NbrSide", "end2NbrSlot", "end2NbrSide"):
+            setattr(s, k, P(getattr(fs, k)).value)
+        for k in ("params", "chainParams", "xlParams", "offParams", "torsionParams", "boxParams", "gridParams",
+                  "boxLen", "gridDims"):
+            arr = getattr(s, k)
+            for i, v in enumerate(p[k]):
+                arr[i] = v
+        for k in ("chainOn", "unbindOn", "confineOn", "gridOn", "periodic", "maxCand", "seed", "rng"):
+            setattr(s, k, int(p[k]))
+        s.stream = int(p["stream"])
+        s.actinRadius = p["actinRadius"]
+        self.keep = []
+        if xl is not None:
+            s.nLinks = xl.nLinks
+            for k in ("linkFilA", "linkFilB", "linkState", "strainPlace", "linkOrientSame", "torqueMagPlace",
+                      "activeLinkCount", "loc1", "loc2", "strainHist", "torqueMagHist", "xlinkData"):
+                setattr(s, k, P(getattr(xl, k)).value)
+            self.csr = {k: np.zeros(n + 1 if "Offsets" in k else (n if "Count" in k else max(1, xl.nLinks)), np.int32)
+                        for k in ("segCountA", "segOffsetsA", "segIdxA", "segCountB", "segOffsetsB", "segIdxB")}
+            for k, a in self.csr.items():
+                setattr(s, k, P(a).value)
+        if p["gridOn"]:
+            cells = int(p["gridDims"][3])
+            mc = int(p["maxCand"])
+            self.grid = dict(center=np.zeros(3 * n, np.float32), boundingRadius=np.zeros(n, np.float32),
+                             bodyCell=np.zeros(n, np.int32), cellCount=np.zeros(cells, np.int32),
+                             gridCellOffsets=np.zeros(cells + 1, np.int32), gridCellContents=np.zeros(n, np.int32),
+                             candPartner=np.full(n * mc, -1, np.int32), candCount=np.zeros(n, np.int32))
+            for k, a in self.grid.items():
+                setattr(s, k, P(a).value)
+        self.s = s
+        self.lib = load()
+
+    def step(self, step0: int, n: int = 1) -> None:
+        for t in range(step0, step0 + n):
+            self.lib.sbo_network_step(C.byref(self.s), t)

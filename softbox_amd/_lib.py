@@ -1,0 +1,148 @@
+"""ctypes binding of libsoftbox_hip.so (the C ABI declared in include/softbox_hip.h).
+
+The library is the product path: if it is missing or cannot be loaded this module raises,
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsoftbox_hip.so")
+
+SBX_OK = 0
+SBX_ERR_ARG = -1
+SBX_ERR_HIP = -2
+SBX_ERR_NOMEM = -3
+SBX_ERR_STATE = -4
+SBX_RNG_WANG = 0
+SBX_RNG_PHILOX = 1
+SBX_MAX_CAND = 256
+SBX_XL_STRIDE = 12
+SBX_STRAIN_WIN = 10
+SBX_TORQUE_WIN = 5
+
+vp = C.c_void_p
+i32 = C.c_int
+u32 = C.c_uint32
+f32p = C.POINTER(C.c_float)
+i32p = C.POINTER(C.c_int)
+
+
+class SbxFilaments(C.Structure):
+    _fields_ = [("n", i32)] + [
+        (name, vp)
+        for name in (
+            "coord", "uVec", "yVec", "zVec", "end1", "end2", "segLength", "bTransGam", "bRotGam",
+            "brownTransScale", "brownRotScale", "forceSum", "torqueSum",
+            "end1NbrSlot", "end1NbrSide", "end2NbrSlot", "end2NbrSide",
+        )
+    ]
+
+
+class SbxCrosslinks(C.Structure):
+    _fields_ = [("nLinks", i32)] + [
+        (name, vp)
+        for name in (
+            "linkFilA", "linkFilB", "loc1", "loc2", "linkState", "strainHist", "strainPlace",
+            "linkOrientSame", "torqueMagHist", "torqueMagPlace", "xlinkData", "activeLinkCount",
+        )
+    ]
+
+
+class SbxNetParams(C.Structure):
+    _fields_ = [
+        ("params", C.c_float * 2),
+        ("chainParams", C.c_float * 7),
+        ("chainOn", i32),
+        ("xlParams", C.c_float * 3),
+        ("offParams", C.c_float * 5),
+        ("torsionParams", C.c_float * 2),
+        ("unbindOn", i32),
+        ("boxParams", C.c_float * 8),
+        ("confineOn", i32),
+        ("gridOn", i32),
+        ("periodic", i32),
+        ("maxCand", i32),
+        ("gridParams", C.c_float * 6),
+        ("gridDims", C.c_int * 4),
+        ("boxLen", C.c_float * 3),
+        ("actinRadius", C.c_float),
+        ("seed", i32),
+        ("rng", i32),
+        ("stream", u32),
+    ]
+
+
+# name -> argtypes (restype int unless listed in _RESTYPES)
+_SIGS = {
+    "sbx_create": [C.POINTER(vp), i32, vp],
+    "sbx_destroy": [vp],
+    "sbx_last_error": [vp],
+    "sbx_clear_error": [vp],
+    "sbx_sync": [vp],
+    "sbx_alloc": [vp, C.POINTER(vp), C.c_size_t],
+    "sbx_free": [vp, vp],
+    "sbx_push": [vp, vp, vp, C.c_size_t],
+    "sbx_pull": [vp, vp, vp, C.c_size_t],
+    "sbx_copy": [vp, vp, vp, C.c_size_t],
+    "sbx_fill_i32": [vp, vp, i32, C.c_size_t],
+    "sbx_version": [],
+    "sbx_zero_accumulators": [vp, vp, vp, i32],
+    "sbx_brownian_force": [vp, vp, vp, vp, vp, vp, vp, f32p, i32p, i32, i32, u32],
+    "sbx_integrate": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, f32p, i32p, i32],
+    "sbx_derive": [vp, vp, vp, vp, vp, vp, vp, vp, i32p, i32],
+    "sbx_force_build": [vp, vp, vp, vp, vp, vp, vp, vp, vp, f32p, i32p, i32, i32, u32],
+    "sbx_integ_derive": [vp] + [vp] * 13 + [f32p, i32],
+    "sbx_integ_derive_confined": [vp] + [vp] * 13 + [f32p, i32p, i32],
+    "sbx_confine": [vp, vp, vp, vp, vp, vp, vp, f32p, i32p, i32],
+    "sbx_chain_forces": [vp] + [vp] * 11 + [f32p, i32p, i32],
+    "sbx_publish_filaments": [vp, vp, vp, vp, vp, vp, vp, f32p, i32, i32],
+    "sbx_body_cell": [vp, vp, f32p, i32p, i32p, vp, i32, i32],
+    "sbx_grid_build": [vp, vp, i32p, i32p, vp, vp, i32],
+    "sbx_broad_phase": [vp, vp, vp, vp, vp, vp, i32p, f32p, i32p, vp, vp, i32, i32, i32],
+    "sbx_csr_build": [vp, vp, i32, i32, vp, vp],
+    "sbx_count_active_links": [vp, vp, vp, vp, vp, i32, i32],
+    "sbx_link_unbind": [vp] + [vp] * 10 + [f32p, i32p, i32, i32, u32],
+    "sbx_link_forces": [vp] + [vp] * 10 + [f32p, i32, i32],
+    "sbx_link_torsion": [vp] + [vp] * 8 + [f32p, i32, i32],
+    "sbx_seg_gather": [vp, vp, vp, vp, vp, vp, vp, i32, i32],
+    "sbx_net_create": [vp, C.POINTER(SbxFilaments), C.POINTER(SbxCrosslinks), C.POINTER(SbxNetParams), C.POINTER(vp)],
+    "sbx_net_destroy": [vp],
+    "sbx_net_step": [vp, i32, i32],
+    "sbx_net_publish": [vp],
+    "sbx_net_topology_changed": [vp],
+    "sbx_net_candidates": [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)],
+    "sbx_net_set_profiling": [vp, i32],
+    "sbx_net_kernel_times": [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_longlong), i32],
+    "sbx_net_use_graph": [vp, i32],
+}
+_RESTYPES = {"sbx_destroy": None, "sbx_last_error": C.c_char_p, "sbx_clear_error": None,
+             "sbx_version": C.c_char_p, "sbx_net_destroy": None}
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+class SoftboxError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"softbox_hip error {code}: {message}")
+        self.code = code
+
+
+def load() -> C.CDLL:
+    """Load libsoftbox_hip.so (built in-tree); raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is not built: run `python -m softbox_amd.build` (the HIP library has no fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, i32)
+    _lib = lib
+    return lib

@@ -1,0 +1,413 @@
+// sbx_device.h -- device-side arithmetic shared by every SoftBox mechanics kernel (gfx950).
+//
+// Every helper restates one piece of a reference System method with the same float/double
+// types and the same operation order; the library is compiled with -ffp-contract=off and
+// IEEE division/sqrt so float results match the reference's strict Java arithmetic.
+// Planar SoA throughout: component a of body i lives at [a*n + i] (RigidRodBody.java:26-57).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SBX_PI_D 3.141592653589793
+
+namespace sbx {
+
+enum { RNG_WANG = 0, RNG_PHILOX = 1 };
+
+// Thomas Wang's 32-bit integer mixer as used by BrownianForceSystem.wangHash
+// (BrownianForceSystem.java:174-181); Java int wrap-around == uint32 arithmetic.
+__device__ __forceinline__ uint32_t wang_hash(uint32_t s) {
+    s = (s ^ 61u) ^ (s >> 16);
+    s *= 9u;
+    s = s ^ (s >> 4);
+    s *= 0x27d4eb2du;
+    s = s ^ (s >> 15);
+    return s;
+}
+
+// Philox4x32-10 (Salmon et al., SC'11).  mul_hi/mul_lo map to v_mul_hi_u32 / v_mul_lo_u32.
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                              uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Reference key (BrownianForceSystem.java:203): (i*1000003) ^ (step*999983) ^ (seed*7919)
+__device__ __forceinline__ uint32_t wang_base(uint32_t i, int32_t step, int32_t seed) {
+    return (i * 1000003u) ^ ((uint32_t)step * 999983u) ^ ((uint32_t)seed * 7919u);
+}
+
+// The six raw draws of one body-step.  WANG = BrownianForceSystem.java:204-209 salts;
+// PHILOX = counter (body, step, block, 0), key (seed, stream).
+__device__ __forceinline__ void brownian_bits(int rng, uint32_t body, int32_t step, int32_t seed, uint32_t stream,
+                                              uint32_t h[6]) {
+    if (rng == RNG_WANG) {
+        uint32_t base = wang_base(body, step, seed);
+        h[0] = wang_hash(base);
+        h[1] = wang_hash(base ^ 0x9e3779b9u);
+        h[2] = wang_hash(base ^ 0x85ebca6bu);
+        h[3] = wang_hash(base ^ 0xc2b2ae35u);
+        h[4] = wang_hash(base ^ 0x517cc1b7u);
+        h[5] = wang_hash(base ^ 0x1f0a7ed5u);
+    } else {
+        uint32_t o0[4], o1[4];
+        philox4x32_10(body, (uint32_t)step, 0u, 0u, (uint32_t)seed, stream, o0);
+        philox4x32_10(body, (uint32_t)step, 1u, 0u, (uint32_t)seed, stream, o1);
+        h[0] = o0[0]; h[1] = o0[1]; h[2] = o0[2]; h[3] = o0[3];
+        h[4] = o1[0]; h[5] = o1[1];
+    }
+}
+
+// Event draw keyed by (item, step, seed, salt): WANG = CrosslinkerSystem.java:277 form.
+__device__ __forceinline__ uint32_t event_bits(int rng, uint32_t item, int32_t step, int32_t seed, uint32_t stream,
+                                               uint32_t salt) {
+    if (rng == RNG_WANG) return wang_hash(wang_base(item, step, seed) ^ salt);
+    uint32_t o[4];
+    philox4x32_10(item, (uint32_t)step, salt, 2u, (uint32_t)seed, stream, o);
+    return o[0];
+}
+
+// (h >>> 1) / 2147483647f  (BrownianForceSystem.java:211)
+__device__ __forceinline__ float uniform01(uint32_t h) { return (float)(int32_t)(h >> 1) / 2147483647.0f; }
+
+__device__ __forceinline__ float fmax_java(float a, float b) { return a >= b ? a : b; }
+
+// Box-Muller noise of one body-step: gf = cos branch (force), gt = sin branch (torque),
+// BrownianForceSystem.java:211-232.  log/cos/sin are evaluated in double and cast, as Java's
+// (float) Math.log((double) u) does.
+__device__ __forceinline__ void brownian_noise(int rng, uint32_t body, int32_t step, int32_t seed, uint32_t stream,
+                                               float gf[3], float gt[3]) {
+    uint32_t h[6];
+    brownian_bits(rng, body, step, seed, stream, h);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float ua = fmax_java(1.0e-7f, uniform01(h[2 * a]));
+        float ub = uniform01(h[2 * a + 1]);
+        float r = (float)sqrt((double)(-2.0f * (float)log((double)ua)));
+        float th = 2.0f * 3.14159265f * ub;
+        double s, c;
+        sincos((double)th, &s, &c);
+        gf[a] = r * (float)c;
+        gt[a] = r * (float)s;
+    }
+}
+
+// randForce/randTorque amplitude (BrownianForceSystem.java:234-243): ((tS*bfm)*sqrt(g))*gauss
+__device__ __forceinline__ float brown_amp(float scale, float bfm, float gam, float g) {
+    return scale * bfm * (float)sqrt((double)gam) * g;
+}
+
+// ---------------------------------------------------------------------------------------
+// Integrate + derive of one body (RigidRodLangevinIntegrationSystem.java:49-111 followed by
+// DerivedGeometrySystem.java:152-185; the fused order of MechanicsFusion.integDerive,
+// MechanicsFusion.java:129-200).  In: pose, lab-frame F/T, body-frame random F/T, drag.
+// Out: new coord, u, y (re-orthogonalised), z, end1, end2.
+struct Pose {
+    float c[3], u[3], y[3];
+};
+struct Derived {
+    float z[3], e1[3], e2[3];
+};
+
+__device__ __forceinline__ void integrate_derive(Pose &p, const float f[3], const float t[3], const float rf[3],
+                                                 const float rt[3], const float gT[3], const float gR[3], float dt,
+                                                 float segLen, Derived &d) {
+    float ux = p.u[0], uyc = p.u[1], uzc = p.u[2];
+    float yx = p.y[0], yyc = p.y[1], yzc = p.y[2];
+    float zx = uyc * yzc - uzc * yyc;
+    float zy = uzc * yx - ux * yzc;
+    float zz = ux * yyc - uyc * yx;
+    float zlen = 1.0f / (float)sqrt((double)(zx * zx + zy * zy + zz * zz));
+    zx *= zlen; zy *= zlen; zz *= zlen;
+    float fx = f[0], fy = f[1], fz = f[2], tx = t[0], ty = t[1], tz = t[2];
+    float bfx = ux * fx + uyc * fy + uzc * fz + rf[0];
+    float bfy = yx * fx + yyc * fy + yzc * fz + rf[1];
+    float bfz = zx * fx + zy * fy + zz * fz + rf[2];
+    float btx = ux * tx + uyc * ty + uzc * tz + rt[0];
+    float bty = yx * tx + yyc * ty + yzc * tz + rt[1];
+    float btz = zx * tx + zy * ty + zz * tz + rt[2];
+    float bvx = 1.0e6f * bfx / gT[0];
+    float bvy = 1.0e6f * bfy / gT[1];
+    float bvz = 1.0e6f * bfz / gT[2];
+    float bwx = btx / gR[0];
+    float bwy = bty / gR[1];
+    float bwz = btz / gR[2];
+    float vx = ux * bvx + yx * bvy + zx * bvz;
+    float vy = uyc * bvx + yyc * bvy + zy * bvz;
+    float vz = uzc * bvx + yzc * bvy + zz * bvz;
+    p.c[0] = p.c[0] + dt * vx;
+    p.c[1] = p.c[1] + dt * vy;
+    p.c[2] = p.c[2] + dt * vz;
+    float uTransInZ = -bwy * dt;
+    float uTransInY = bwz * dt;
+    float nuX = ux + yx * uTransInY + zx * uTransInZ;
+    float nuY = uyc + yyc * uTransInY + zy * uTransInZ;
+    float nuZ = uzc + yzc * uTransInY + zz * uTransInZ;
+    float nuInv = 1.0f / (float)sqrt((double)(nuX * nuX + nuY * nuY + nuZ * nuZ));
+    float nux = nuX * nuInv, nuy = nuY * nuInv, nuz = nuZ * nuInv;
+    float yTransInX = -uTransInY;
+    float yTransInZ = bwx * dt;
+    float nyX = ux * yTransInX + yx + zx * yTransInZ;
+    float nyY = uyc * yTransInX + yyc + zy * yTransInZ;
+    float nyZ = uzc * yTransInX + yzc + zz * yTransInZ;
+    float nyInv = 1.0f / (float)sqrt((double)(nyX * nyX + nyY * nyY + nyZ * nyZ));
+    float ny0 = nyX * nyInv, ny1 = nyY * nyInv, ny2 = nyZ * nyInv;
+    // derive
+    float dzx = nuy * ny2 - nuz * ny1;
+    float dzy = nuz * ny0 - nux * ny2;
+    float dzz = nux * ny1 - nuy * ny0;
+    float zmag2 = dzx * dzx + dzy * dzy + dzz * dzz;
+    if (zmag2 > 0.0f) {
+        float inv = 1.0f / (float)sqrt((double)zmag2);
+        dzx *= inv; dzy *= inv; dzz *= inv;
+    }
+    d.z[0] = dzx; d.z[1] = dzy; d.z[2] = dzz;
+    p.u[0] = nux; p.u[1] = nuy; p.u[2] = nuz;
+    p.y[0] = dzy * nuz - dzz * nuy;
+    p.y[1] = dzz * nux - dzx * nuz;
+    p.y[2] = dzx * nuy - dzy * nux;
+    float halfLen = segLen * 0.5f;
+    d.e1[0] = p.c[0] - halfLen * nux;
+    d.e1[1] = p.c[1] - halfLen * nuy;
+    d.e1[2] = p.c[2] - halfLen * nuz;
+    d.e2[0] = p.c[0] + halfLen * nux;
+    d.e2[1] = p.c[1] + halfLen * nuy;
+    d.e2[2] = p.c[2] + halfLen * nuz;
+}
+
+// ContainmentSystem.wallPush (ContainmentSystem.java:130-136)
+__device__ __forceinline__ double wall_push(double d, double halfDim) {
+    double s = (d > 0.0) ? 1.0 : ((d < 0.0) ? -1.0 : 0.0);
+    double push = s * halfDim - d;
+    double ps = (push > 0.0) ? 1.0 : ((push < 0.0) ? -1.0 : 0.0);
+    if (ps == s) return 0.0;
+    return push;
+}
+
+// Confine one body (ContainmentSystem.java:76-124); returns true when it touched a wall and
+// adds into f/t as (float)((double)f + df).  half* are the inset half-extents.
+__device__ __forceinline__ bool confine_body(const float c[3], const float u[3], float segLen, float gTx, double tau,
+                                             double halfX, double halfY, double halfZ, double coeff, float f[3],
+                                             float t[3]) {
+    double cx = c[0], cy = c[1], cz = c[2];
+    double ux = u[0], uy = u[1], uz = u[2];
+    double half = 0.5 * (double)segLen;
+    double bTGx = gTx;
+    double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
+    bool touched = false;
+    for (int e = 0; e < 2; e++) {
+        double sgn = (e == 0) ? -1.0 : 1.0;
+        double dx = cx + sgn * half * ux;
+        double dy = cy + sgn * half * uy;
+        double dz = cz + sgn * half * uz;
+        double fux = wall_push(dx, halfX);
+        double fuy = wall_push(dy, halfY);
+        double fuz = wall_push(dz, halfZ);
+        double delta2 = fux * fux + fuy * fuy + fuz * fuz;
+        if (delta2 > 0.0) {
+            double delta = sqrt(delta2);
+            double inv = 1.0 / delta;
+            double lux = fux * inv, luy = fuy * inv, luz = fuz * inv;
+            double mag = coeff * 1.0e-6 * delta * bTGx / tau;
+            double Fx = mag * lux, Fy = mag * luy, Fz = mag * luz;
+            double rx = sgn * half * ux * 1.0e-6;
+            double ry = sgn * half * uy * 1.0e-6;
+            double rz = sgn * half * uz * 1.0e-6;
+            fx += Fx; fy += Fy; fz += Fz;
+            tx += ry * Fz - rz * Fy;
+            ty += rz * Fx - rx * Fz;
+            tz += rx * Fy - ry * Fx;
+            touched = true;
+        }
+    }
+    if (touched) {
+        f[0] = (float)((double)f[0] + fx);
+        f[1] = (float)((double)f[1] + fy);
+        f[2] = (float)((double)f[2] + fz);
+        t[0] = (float)((double)t[0] + tx);
+        t[1] = (float)((double)t[1] + ty);
+        t[2] = (float)((double)t[2] + tz);
+    }
+    return touched;
+}
+
+// ---------------------------------------------------------------------------------------
+// Chain (ChainBendingForceSystem.java:47-331)
+__device__ __forceinline__ double accurate_acos(double x) {
+    if (x > 1.0) x = 1.0;
+    if (x < -1.0) x = -1.0;
+    double y;
+    if (x > 0.95) {
+        double t = 1.0 - x;
+        if (t < 0.0) t = 0.0;
+        y = sqrt(2.0 * t);
+    } else if (x < -0.95) {
+        double t = 1.0 + x;
+        if (t < 0.0) t = 0.0;
+        y = 3.141592653589793 - sqrt(2.0 * t);
+    } else {
+        double ax = (x < 0.0) ? -x : x;
+        double p = (-0.0187293 * ax + 0.0742610) * ax - 0.2121144;
+        p = (p * ax + 1.5707963);
+        p = p * sqrt(1.0 - ax);
+        y = (x < 0.0) ? (3.141592653589793 - p) : p;
+    }
+    double s, c;
+    sincos(y, &s, &c);
+    if (s > 1.0e-12 || s < -1.0e-12) y = y + (c - x) / s;
+    sincos(y, &s, &c);
+    if (s > 1.0e-12 || s < -1.0e-12) y = y + (c - x) / s;
+    return y;
+}
+
+__device__ __forceinline__ double accurate_asin(double s) {
+    double s2 = s * s;
+    double y = s * (1.0 + s2 * (0.16666666666666666 + s2 * (0.075 + s2 * 0.044642857142857)));
+    double sy, cy;
+    sincos(y, &sy, &cy);
+    if (cy > 1.0e-12) y = y + (s - sy) / cy;
+    sincos(y, &sy, &cy);
+    if (cy > 1.0e-12) y = y + (s - sy) / cy;
+    return y;
+}
+
+__device__ __forceinline__ double angle_from_sin_cos(double sin2, double c) {
+    double s = sqrt(sin2);
+    double ac = (c < 0.0) ? -c : c;
+    if (s <= ac) {
+        double base = accurate_asin(s);
+        return (c >= 0.0) ? base : (SBX_PI_D - base);
+    }
+    return accurate_acos(c);
+}
+
+struct ChainParams {
+    double dt, fracMove, fracR, fracMoveTorq, springActive, spring, monoR;
+};
+
+// One joint of chainForces: endSign +1 = end2 block (:152-235), -1 = end1 block (:237-321).
+// Neighbour data (nc, nu, nlen, nBTGx, nBTGy, nBRGy) is passed in so callers can stage it.
+__device__ __forceinline__ void chain_joint(int i, int nbr, int side, double endSign, double cx, double cy, double cz,
+                                            double ux, double uy, double uz, double len, double rBTGx, double rBTGy,
+                                            double rBRGy, double ncx, double ncy, double ncz, double nux, double nuy,
+                                            double nuz, double nlen, double nBTGx, double nBTGy, double nBRGy,
+                                            const ChainParams &cp, double f[3], double t[3]) {
+    const double DEG2RAD = SBX_PI_D / 180.0;
+    const double RAD2DEG = 180.0 / SBX_PI_D;
+    double halfLen_um = 0.5 * len;
+    double lSqSelf = 1.0e-12 * len * len;
+    double nHalfLen_um = 0.5 * nlen;
+    double lSqN = 1.0e-12 * nlen * nlen;
+    double ex, ey, ez, selfLpx, selfLpy, selfLpz;
+    if (endSign > 0) {
+        ex = cx + halfLen_um * ux; ey = cy + halfLen_um * uy; ez = cz + halfLen_um * uz;
+        selfLpx = ex - cp.monoR * ux; selfLpy = ey - cp.monoR * uy; selfLpz = ez - cp.monoR * uz;
+    } else {
+        ex = cx - halfLen_um * ux; ey = cy - halfLen_um * uy; ez = cz - halfLen_um * uz;
+        selfLpx = ex + cp.monoR * ux; selfLpy = ey + cp.monoR * uy; selfLpz = ez + cp.monoR * uz;
+    }
+    double nbrTipx, nbrTipy, nbrTipz, nbrLpx, nbrLpy, nbrLpz;
+    if (side == 0) {
+        nbrTipx = ncx - nHalfLen_um * nux; nbrTipy = ncy - nHalfLen_um * nuy; nbrTipz = ncz - nHalfLen_um * nuz;
+        nbrLpx = nbrTipx + cp.monoR * nux; nbrLpy = nbrTipy + cp.monoR * nuy; nbrLpz = nbrTipz + cp.monoR * nuz;
+    } else {
+        nbrTipx = ncx + nHalfLen_um * nux; nbrTipy = ncy + nHalfLen_um * nuy; nbrTipz = ncz + nHalfLen_um * nuz;
+        nbrLpx = nbrTipx - cp.monoR * nux; nbrLpy = nbrTipy - cp.monoR * nuy; nbrLpz = nbrTipz - cp.monoR * nuz;
+    }
+    double linkPtX, linkPtY, linkPtZ, ptAtEndX, ptAtEndY, ptAtEndZ;
+    if (i < nbr) {
+        linkPtX = selfLpx; linkPtY = selfLpy; linkPtZ = selfLpz;
+        ptAtEndX = nbrTipx; ptAtEndY = nbrTipy; ptAtEndZ = nbrTipz;
+    } else {
+        linkPtX = nbrLpx; linkPtY = nbrLpy; linkPtZ = nbrLpz;
+        ptAtEndX = ex; ptAtEndY = ey; ptAtEndZ = ez;
+    }
+    double dx = ptAtEndX - linkPtX, dy = ptAtEndY - linkPtY, dz = ptAtEndZ - linkPtZ;
+    double strainDist = sqrt(dx * dx + dy * dy + dz * dz);
+    double invStrain = (strainDist > 0.0) ? (1.0 / strainDist) : 0.0;
+    double luX = dx * invStrain, luY = dy * invStrain, luZ = dz * invStrain;
+    double cosB1 = ux * luX + uy * luY + uz * luZ;
+    if (cosB1 > 1.0) cosB1 = 1.0;
+    if (cosB1 < -1.0) cosB1 = -1.0;
+    double cosA1 = sin(accurate_acos(cosB1));
+    double cosA1_2 = cosA1 * cosA1;
+    double moveC1 = cosB1 * cosB1 / rBTGx + cosA1_2 / rBTGy + lSqSelf * cosA1_2 / (4.0 * rBRGy);
+    double cosB2 = nux * luX + nuy * luY + nuz * luZ;
+    if (cosB2 > 1.0) cosB2 = 1.0;
+    if (cosB2 < -1.0) cosB2 = -1.0;
+    double cosA2 = sin(accurate_acos(cosB2));
+    double cosA2_2 = cosA2 * cosA2;
+    double moveC2 = cosB2 * cosB2 / nBTGx + cosA2_2 / nBTGy + lSqN * cosA2_2 / (4.0 * nBRGy);
+    double denom = cp.dt * (moveC1 + moveC2);
+    double forceMag = (denom > 0.0) ? (cp.fracMove * 1.0e-6 * strainDist / denom) : 0.0;
+    double fSign = (i < nbr) ? 1.0 : -1.0;
+    double Fx = fSign * forceMag * luX, Fy = fSign * forceMag * luY, Fz = fSign * forceMag * luZ;
+    f[0] += Fx; f[1] += Fy; f[2] += Fz;
+    double Rscale = (endSign > 0) ? (0.5e-6 * len * cp.fracR) : (-0.5e-6 * len * cp.fracR);
+    double Rx = Rscale * ux, Ry = Rscale * uy, Rz = Rscale * uz;
+    t[0] += Ry * Fz - Rz * Fy; t[1] += Rz * Fx - Rx * Fz; t[2] += Rx * Fy - Ry * Fx;
+    double nuxE, nuyE, nuzE;
+    if (side == 0) { nuxE = nux; nuyE = nuy; nuzE = nuz; } else { nuxE = -nux; nuyE = -nuy; nuzE = -nuz; }
+    double mux, muy, muz;
+    if (endSign > 0) { mux = ux; muy = uy; muz = uz; } else { mux = -ux; muy = -uy; muz = -uz; }
+    double tvx = muy * nuzE - muz * nuyE, tvy = muz * nuxE - mux * nuzE, tvz = mux * nuyE - muy * nuxE;
+    double tvMag2 = tvx * tvx + tvy * tvy + tvz * tvz;
+    if (tvMag2 > 1.0e-30) {
+        double invMag = 1.0 / sqrt(tvMag2);
+        tvx *= invMag; tvy *= invMag; tvz *= invMag;
+        double dotV = mux * nuxE + muy * nuyE + muz * nuzE;
+        if (dotV > 1.0) dotV = 1.0;
+        if (dotV < -1.0) dotV = -1.0;
+        double angTween = angle_from_sin_cos(tvMag2, dotV) * RAD2DEG;
+        double torsionMag;
+        if (cp.springActive > 0.5) {
+            torsionMag = cp.fracMoveTorq * cp.spring * angTween;
+        } else {
+            double invBRG = 1.0 / rBRGy + 1.0 / nBRGy;
+            torsionMag = cp.fracMoveTorq * DEG2RAD * angTween / (invBRG * cp.dt);
+        }
+        t[0] += tvx * torsionMag; t[1] += tvy * torsionMag; t[2] += tvz * torsionMag;
+    }
+}
+
+// Whole chain contribution of body i reading planar arrays (stride n).
+__device__ __forceinline__ void chain_body(int n, int i, const float *__restrict__ coord, const float *__restrict__ uVec,
+                                           const float *__restrict__ segLength, int e2Slot, int e2Side, int e1Slot,
+                                           int e1Side, const float *__restrict__ bTransGam,
+                                           const float *__restrict__ bRotGam, const ChainParams &cp, double f[3],
+                                           double t[3]) {
+    double cx = coord[i], cy = coord[n + i], cz = coord[2 * n + i];
+    double ux = uVec[i], uy = uVec[n + i], uz = uVec[2 * n + i];
+    double len = segLength[i];
+    double rBTGx = bTransGam[i], rBTGy = bTransGam[n + i], rBRGy = bRotGam[n + i];
+    if (e2Slot >= 0) {
+        int j = e2Slot;
+        chain_joint(i, j, e2Side, +1.0, cx, cy, cz, ux, uy, uz, len, rBTGx, rBTGy, rBRGy, coord[j], coord[n + j],
+                    coord[2 * n + j], uVec[j], uVec[n + j], uVec[2 * n + j], segLength[j], bTransGam[j],
+                    bTransGam[n + j], bRotGam[n + j], cp, f, t);
+    }
+    if (e1Slot >= 0) {
+        int j = e1Slot;
+        chain_joint(i, j, e1Side, -1.0, cx, cy, cz, ux, uy, uz, len, rBTGx, rBTGy, rBRGy, coord[j], coord[n + j],
+                    coord[2 * n + j], uVec[j], uVec[n + j], uVec[2 * n + j], segLength[j], bTransGam[j],
+                    bTransGam[n + j], bRotGam[n + j], cp, f, t);
+    }
+}
+
+// CrosslinkerSystem.fastAcos (CrosslinkerSystem.java:372-376)
+__device__ __forceinline__ double fast_acos(double dot) {
+    if (dot > 0.95) { double t = 1.0 - dot; if (t < 0) t = 0; return sqrt(2.0 * t); }
+    if (dot < -0.95) { double t = 1.0 + dot; if (t < 0) t = 0; return SBX_PI_D - sqrt(2.0 * t); }
+    return accurate_acos(dot);
+}
+
+}  // namespace sbx

@@ -1,0 +1,65 @@
+// sbx_internal.h -- host-side context, error plumbing and launch helpers of libsoftbox_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/softbox_hip.h"
+
+namespace sbx {
+// Scratch of the stable counting sort (keys -> CSR offsets + ascending item list).
+struct SortScratch {
+    int *counts = nullptr, *cursor = nullptr, *tmp = nullptr, *blockSums = nullptr;
+    long long keyCap = 0, itemCap = 0, blockCap = 0;
+};
+struct ChainParams;
+}  // namespace sbx
+
+struct sbx_ctx {
+    int device;
+    hipStream_t stream;
+    int owns_stream;
+    int sticky;            // first HIP failure code, kept until sbx_clear_error
+    char err[512];
+    int cu_count;
+    sbx::SortScratch api_sort;  // scratch of the stand-alone sbx_grid_build / sbx_csr_build entry points
+};
+
+namespace sbx {
+
+int fail(sbx_ctx *c, int code, const char *fmt, ...);
+int check_hip(sbx_ctx *c, hipError_t e, const char *what);
+int check_launch(sbx_ctx *c, const char *what);
+int set_device(sbx_ctx *c);
+
+inline unsigned blocks_for(long long n, int threads) {
+    long long b = (n + threads - 1) / threads;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+int sort_scratch_reserve(sbx_ctx *c, SortScratch &s, long long nKeys, long long nItems);
+void sort_scratch_free(SortScratch &s);
+// Stable counting sort of items by integer key (key < 0 skipped) into offsets[nKeys+1] and
+// list[...]; offsets[nKeys] is the total clamped to contentsCap; positions >= contentsCap are dropped.
+int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
+                  int *list);
+// exclusive scan of in[n] into out[n+1] (out[n] = total, clamped to clampTotal when >= 0)
+int exclusive_scan(sbx_ctx *c, SortScratch &s, const int *in, int *out, int n, int clampTotal);
+int fill_i32(sbx_ctx *c, int *dst, int value, long long count);
+
+ChainParams chain_params(const float *cp);
+
+}  // namespace sbx
+
+#define SBX_CHECK_CTX(c)                         \
+    do {                                         \
+        if (!(c)) return SBX_ERR_ARG;            \
+        if ((c)->sticky) return (c)->sticky;     \
+    } while (0)
+
+#define SBX_REQUIRE(c, cond, ...)                                     \
+    do {                                                              \
+        if (!(cond)) return sbx::fail((c), SBX_ERR_ARG, __VA_ARGS__); \
+    } while (0)

@@ -1,0 +1,90 @@
+"""The whole network mechanics step on the GPU (sbx_net_* of the C ABI).
+
+Stands in for a harness's `buildPlan` + per-step `plan.execute()` (DenseContractileHarness.java:
+442-606): build once from device stores, then `step(step0, n)` advances n steps.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .context import Context
+from .stores import DeviceCrosslinks, DeviceFilaments
+
+
+class Network:
+    def __init__(self, ctx: Context, scene, keep_forces: bool = True):
+        self.ctx = ctx
+        self.scene = scene
+        self.fil = DeviceFilaments(ctx, scene.fs, keep_forces=keep_forces)
+        self.xl = DeviceCrosslinks(ctx, scene.xl) if scene.xl is not None else None
+        self._fs = self.fil.struct()
+        self._xs = self.xl.struct() if self.xl is not None else None
+        self._p = scene.net_params()
+        h = C.c_void_p()
+        ctx.check(ctx.lib.sbx_net_create(ctx.h, C.byref(self._fs), C.byref(self._xs) if self._xs else None,
+                                         C.byref(self._p), C.byref(h)))
+        self.h = h
+
+    def step(self, step0: int, n: int = 1) -> None:
+        self.ctx.check(self.ctx.lib.sbx_net_step(self.h, step0, n))
+
+    def publish(self) -> None:
+        self.ctx.check(self.ctx.lib.sbx_net_publish(self.h))
+
+    def use_graph(self, on: bool = True) -> None:
+        self.ctx.check(self.ctx.lib.sbx_net_use_graph(self.h, 1 if on else 0))
+
+    def set_profiling(self, on: bool) -> None:
+        self.ctx.check(self.ctx.lib.sbx_net_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_times(self) -> dict:
+        cap = 16
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        cnt = (C.c_longlong * cap)()
+        k = self.ctx.check(self.ctx.lib.sbx_net_kernel_times(self.h, names, ms, cnt, cap))
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(k)}
+
+    def candidates(self, n: int, max_cand: int):
+        """Host copies of (candPartner[n*maxCand], candCount[n], offsets, contents, bodyCell)."""
+        import torch
+
+        ptrs = [C.c_void_p() for _ in range(5)]
+        self.ctx.check(self.ctx.lib.sbx_net_candidates(self.h, *[C.byref(p) for p in ptrs]))
+        self.ctx.sync()
+        cells = int(self._p.gridDims[3])
+        sizes = [n * max_cand, n, cells + 1, n, n]
+        out = []
+        for p, sz in zip(ptrs, sizes):
+            if not p.value:
+                out.append(None)
+                continue
+            a = np.empty(sz, np.int32)
+            self.ctx.check(self.ctx.lib.sbx_pull(self.ctx.h, a.ctypes.data_as(C.c_void_p), p, a.nbytes))
+            out.append(a)
+        return out
+
+    def pull(self, fs=None, xl=None):
+        """Synchronise and copy state back into host stores (pose published first)."""
+        self.publish()
+        self.ctx.sync()
+        if fs is not None:
+            self.fil.pull(fs, ("coord", "uVec", "yVec", "zVec", "end1", "end2")
+                          + (("forceSum", "torqueSum") if self.fil.forceSum is not None else ()))
+        if xl is not None and self.xl is not None:
+            self.xl.pull(xl)
+        return fs, xl
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.ctx.lib.sbx_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,248 @@
+"""Seeded synthetic workloads in the reference store layout.
+
+free_rods      -- free Brownian rigid rods (the FDT workload of DiffusionHarness.java:742-851):
+                  N rods of length L, centres uniform in a cube, orientations uniform on S^2.
+network        -- crosslinked filament network (DenseContractileHarness.java:180-341 filament part):
+                  straight chains of rigid rods with end2->next (side 0) / end1->prev (side 1)
+                  topology (DiffusionHarness.java:546-547), crosslinks pre-placed at segment
+                  midpoints between bundled or crossing filaments at the link rest length.
+
+Both return a Scene: host stores + the scalar parameter block of the network step.  `periodic=True`
+selects the periodic-cube extension (coordinates unwrapped, grid cells wrapped, minimum image in
+the broad phase); `periodic=False` is the reference's clamped grid with the containment box
+centred at the origin (ContainmentSystem.java:50-53).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import _lib
+from .physics import ACTIN_MONO_RADIUS, ACTIN_RADIUS, brownian_force_mag, set_rod_drag
+from .stores import CrosslinkerStore, FilamentStore
+
+KT_WORKLOAD = 4.11e-21   # kT = 4.11 pN nm as the workloads state it (reference Constants.kT = 4.11644e-21)
+ETA_WATER = 1.0e-3       # Pa s
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    fs: FilamentStore
+    xl: CrosslinkerStore | None
+    p: dict                     # sbx_net_params fields
+    box: float
+    seed: int
+    n_fil: int = 0
+    segs_per_fil: int = 1
+
+    def net_params(self) -> _lib.SbxNetParams:
+        s = _lib.SbxNetParams()
+        for k, v in self.p.items():
+            f = getattr(s, k)
+            if hasattr(f, "__len__"):
+                for i, x in enumerate(v):
+                    f[i] = x
+            else:
+                setattr(s, k, v)
+        return s
+
+    def copy(self) -> "Scene":
+        return dataclasses.replace(self, fs=self.fs.copy(), xl=None if self.xl is None else self.xl.copy(),
+                                   p={k: (list(v) if isinstance(v, (list, tuple, np.ndarray)) else v)
+                                      for k, v in self.p.items()})
+
+
+def _unit_vectors(rng: np.random.Generator, n: int) -> np.ndarray:
+    z = 2.0 * rng.random(n) - 1.0
+    phi = 2.0 * math.pi * rng.random(n)
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z])
+
+
+def _perp(u: np.ndarray) -> np.ndarray:
+    """Unit vector perpendicular to u by Gram-Schmidt against z (or x when u is near z)
+    (DiffusionHarness.java:692-702)."""
+    a = np.zeros_like(u)
+    near_z = np.abs(u[2]) >= 0.9
+    a[2, ~near_z] = 1.0
+    a[0, near_z] = 1.0
+    dot = (a * u).sum(0)
+    y = a - dot * u
+    return y / np.linalg.norm(y, axis=0)
+
+
+def grid_for(box: float, bound_r: float, cutoff: float, periodic: bool, min_cell: float = 0.0,
+             margin: float = 1.0) -> dict:
+    """Grid parameters.  Cell >= 2*maxBoundingRadius + cutoff makes the 27-cell stencil complete
+    (SpatialGrid.java:19-27); periodic grids tile the box exactly."""
+    need = max(2.0 * bound_r + cutoff, min_cell)
+    if periodic:
+        nb = max(3, int(math.floor(box / need)))
+        cell = box / nb
+        lo = 0.0
+        box_len = [box, box, box]
+    else:
+        half = 0.5 * box + margin
+        nb = 1 + int(math.ceil(2 * half / need))
+        cell = need
+        lo = -half
+        box_len = [0.0, 0.0, 0.0]
+    return dict(gridParams=[lo, lo, lo, cell, 1.0 / cell, cutoff], gridDims=[nb, nb, nb, nb ** 3],
+                boxLen=box_len, periodic=1 if periodic else 0)
+
+
+def _base_params(dt: float, kT: float) -> dict:
+    return dict(params=[dt, brownian_force_mag(dt, kT)], chainParams=[dt, 0.5, 0.1, 0.265, 0.0, 1.0e-20,
+                                                                       ACTIN_MONO_RADIUS],
+                chainOn=0, xlParams=[0.0, 0.0, 1.0], offParams=[0.0] * 5, torsionParams=[0.0, 0.0], unbindOn=0,
+                boxParams=[1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, dt], confineOn=0, gridOn=0, periodic=0,
+                maxCand=_lib.SBX_MAX_CAND, gridParams=[0.0] * 6, gridDims=[1, 1, 1, 1], boxLen=[0.0] * 3,
+                actinRadius=ACTIN_RADIUS, seed=0, rng=_lib.SBX_RNG_PHILOX, stream=0)
+
+
+def free_rods(n: int = 1024, length: float = 1.0, box: float = 10.0, seed: int = 0, dt: float = 1.0e-5,
+              periodic: bool = True, eta: float = ETA_WATER, kT: float = KT_WORKLOAD, grid: bool = True,
+              cell: float = 1.0, rng: int = _lib.SBX_RNG_PHILOX) -> Scene:
+    """Free Brownian rods, centres uniform in the cube, orientations uniform on S^2."""
+    g = np.random.default_rng(seed)
+    fs = FilamentStore(n)
+    u = _unit_vectors(g, n)
+    y = _perp(u)
+    lo = 0.0 if periodic else -0.5 * box
+    c = lo + box * g.random((3, n))
+    fs.plane("coord")[:] = c
+    fs.plane("uVec")[:] = u
+    fs.plane("yVec")[:] = y
+    set_rod_drag(fs, length, ACTIN_RADIUS, eta, kT)
+    fs.brownTransScale[:] = 1.0
+    fs.brownRotScale[:] = 1.0
+    fs.set_params(dt, brownian_force_mag(dt, kT))
+    fs.set_chain_params(dt)
+    derive_host(fs)
+    p = _base_params(dt, kT)
+    p.update(seed=seed, rng=rng)
+    if grid:
+        p.update(grid_for(box, 0.5 * length + ACTIN_RADIUS, 0.0, periodic, min_cell=cell), gridOn=1)
+    if not periodic:
+        p.update(confineOn=1, boxParams=[1.0e-4, box, box, box, ACTIN_RADIUS, 0.5, 1.0, dt])
+    return Scene("free_rods", fs, None, p, box, seed, n_fil=n, segs_per_fil=1)
+
+
+def network(n_fil: int = 20000, segs: int = 10, seg_len: float = 0.1, box: float = 20.0, n_links: int = 50000,
+            seed: int = 0, dt: float = 1.0e-5, periodic: bool = True, eta: float = ETA_WATER,
+            kT: float = KT_WORKLOAD, rest: float = 0.035, grab: float = 0.04, bell=(0.0, 0.1, 4.26),
+            torsion: float = 1.0e-19, unbind: bool = True, grid: bool = True, rng: int = _lib.SBX_RNG_PHILOX,
+            confine: bool | None = None, seed_frac: float = 0.2) -> Scene:
+    """Crosslinked network of straight `segs`-rod filaments.
+
+    A fraction `seed_frac` of the filaments are placed uniformly (centre in the cube, direction
+    uniform on S^2); every other filament attaches to a random earlier filament either parallel at
+    distance `rest` (a bundle, several links between facing segments) or crossing it with its
+    common perpendicular of length `rest` (one link).  Every link therefore starts at its rest length,
+    so the Bell off-rate starts at k0 and the network persists (uniformly random filaments at this
+    density have their nearest neighbour ~0.15 um away, which would start every link at strain ~4).
+    Links sit at segment midpoints with orientSame = 1 (DenseContractileHarness.java:330-333).
+    """
+    g = np.random.default_rng(seed)
+    n = n_fil * segs
+    fs = FilamentStore(n)
+    lo = 0.0 if periodic else -0.5 * box
+    kk = np.arange(segs) - 0.5 * (segs - 1)
+    fc = np.zeros((3, n_fil))
+    fu = np.zeros((3, n_fil))
+    n_seed = max(1, min(n_fil, int(round(seed_frac * n_fil))))
+    fc[:, :n_seed] = lo + box * g.random((3, n_seed))
+    fu[:, :n_seed] = _unit_vectors(g, n_seed)
+    links = []
+    attached = n_fil - n_seed
+    r = n_links / attached if attached > 0 else 0.0
+    k_par = int(min(segs, max(2, math.ceil(r) + 1)))
+    p_par = min(1.0, max(0.0, (r - 1.0) / (k_par - 1.0))) if k_par > 1 else 1.0
+    for f in range(n_seed, n_fil):
+        par = int(g.integers(0, f))
+        up = fu[:, par]
+        if g.random() < p_par:
+            w = _unit_vectors(g, 1)[:, 0]
+            w -= (w @ up) * up
+            w /= np.linalg.norm(w)
+            fu[:, f] = up
+            fc[:, f] = fc[:, par] + rest * w
+            for k in g.choice(segs, size=k_par, replace=False):
+                links.append((par * segs + int(k), f * segs + int(k)))
+        else:
+            uf = _unit_vectors(g, 1)[:, 0]
+            w = np.cross(up, uf)
+            if np.linalg.norm(w) < 1e-6:
+                continue
+            w /= np.linalg.norm(w)
+            ks, kt = int(g.integers(0, segs)), int(g.integers(0, segs))
+            mid = fc[:, par] + kk[ks] * seg_len * up
+            fu[:, f] = uf
+            fc[:, f] = mid + rest * w - kk[kt] * seg_len * uf
+            links.append((par * segs + ks, f * segs + kt))
+    fy = _perp(fu)
+    cen = fc[:, :, None] + fu[:, :, None] * (kk[None, None, :] * seg_len)
+    fs.plane("coord")[:] = cen.reshape(3, n)
+    fs.plane("uVec")[:] = np.repeat(fu, segs, axis=1)
+    fs.plane("yVec")[:] = np.repeat(fy, segs, axis=1)
+    idx = np.arange(n)
+    pos = idx % segs
+    fs.end2NbrSlot[:] = np.where(pos < segs - 1, idx + 1, -1)
+    fs.end2NbrSide[:] = np.where(pos < segs - 1, 0, -1)
+    fs.end1NbrSlot[:] = np.where(pos > 0, idx - 1, -1)
+    fs.end1NbrSide[:] = np.where(pos > 0, 1, -1)
+    set_rod_drag(fs, seg_len, ACTIN_RADIUS, eta, kT)
+    fs.brownTransScale[:] = 1.0
+    fs.brownRotScale[:] = 1.0
+    fs.set_params(dt, brownian_force_mag(dt, kT))
+    fs.set_chain_params(dt)
+    derive_host(fs)
+    xl = None
+    if n_links > 0:
+        xl = CrosslinkerStore(n_links, n)
+        order = g.permutation(len(links))[:n_links]
+        for m, li in enumerate(order):
+            a, b = links[li]
+            xl.set_link(m, a, 0.5 * float(fs.segLength[a]), b, 0.5 * float(fs.segLength[b]))
+            xl.linkOrientSame[m] = 1
+        for m in range(len(order), n_links):     # pool slots beyond the placed links stay FREE
+            xl.linkState[m] = -1
+        xl.set_params(rest, 0.4, dt)
+        xl.set_off_params(bell[0], bell[1], bell[2], dt, rest)
+        xl.set_torsion_params(torsion, torsion > 0.0)
+        xl.compute_fil_link_ct()
+    p = _base_params(dt, kT)
+    p.update(seed=seed, rng=rng, chainOn=1)
+    if xl is not None:
+        p.update(xlParams=list(xl.xlParams), offParams=list(xl.offParams), torsionParams=list(xl.torsionParams),
+                 unbindOn=1 if unbind else 0)
+    if grid:
+        p.update(grid_for(box, 0.5 * seg_len + ACTIN_RADIUS, grab, periodic), gridOn=1)
+    if confine if confine is not None else not periodic:
+        p.update(confineOn=1, boxParams=[1.0e-4, box, box, box, ACTIN_RADIUS, 0.5, 1.0, dt])
+    return Scene("network", fs, xl, p, box, seed, n_fil=n_fil, segs_per_fil=segs)
+
+
+def derive_host(fs: FilamentStore) -> None:
+    """Initial derived geometry in float32 with the reference's operation order
+    (DerivedGeometrySystem.java:156-185), so the first step reads consistent end1/end2."""
+    f = np.float32
+    u = fs.plane("uVec")
+    y = fs.plane("yVec")
+    c = fs.plane("coord")
+    ux, uy, uz = u[0].astype(f), u[1].astype(f), u[2].astype(f)
+    yx, yy, yz = y[0].astype(f), y[1].astype(f), y[2].astype(f)
+    zx = uy * yz - uz * yy
+    zy = uz * yx - ux * yz
+    zz = ux * yy - uy * yx
+    m2 = zx * zx + zy * zy + zz * zz
+    inv = np.where(m2 > 0, f(1.0) / np.sqrt(m2, dtype=f), f(1.0)).astype(f)
+    zx, zy, zz = zx * inv, zy * inv, zz * inv
+    fs.plane("zVec")[:] = np.stack([zx, zy, zz])
+    fs.plane("yVec")[:] = np.stack([zy * uz - zz * uy, zz * ux - zx * uz, zx * uy - zy * ux])
+    half = fs.segLength * f(0.5)
+    fs.plane("end1")[:] = np.stack([c[0] - half * ux, c[1] - half * uy, c[2] - half * uz])
+    fs.plane("end2")[:] = np.stack([c[0] + half * ux, c[1] + half * uy, c[2] + half * uz])

@@ -1,0 +1,148 @@
+"""Whole-step parity: sbx_net_step (fused HIP step) against the oracle's sbo_network_step.
+
+The oracle runs the reference's per-System sequence (DenseContractileHarness.cpuStep filament block:
+publish, grid, broad phase, zero + Brownian, chain, unbind, count, link force, torsion, CSR A/B +
+gathers, confine, integrate, derive).  The HIP step fuses it into a few kernels; every output must
+match bit-for-bit: pose, derived geometry, accumulated forces, crosslink lifecycle and rings, the
+grid CSR and the candidate slices.
+"""
+import numpy as np
+import pytest
+
+from _common import assert_same, small_network
+from oracle import ffi
+from softbox_amd import scenes
+from softbox_amd.systems import RNG_PHILOX, RNG_WANG
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from softbox_amd.context import Context
+
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _compare(net, orc, sc, what):
+    fs = sc.fs.copy()
+    xl = sc.xl.copy() if sc.xl is not None else None
+    net.pull(fs, xl)
+    for k in ("coord", "uVec", "yVec", "zVec", "end1", "end2", "forceSum", "torqueSum"):
+        assert_same(getattr(fs, k), getattr(orc.fs, k), f"{what}: {k}")
+    if xl is not None:
+        for k in ("linkState", "strainHist", "strainPlace", "torqueMagHist", "torqueMagPlace", "xlinkData",
+                  "activeLinkCount"):
+            assert_same(getattr(xl, k), getattr(orc.xl, k), f"{what}: {k}")
+    if sc.p["gridOn"]:
+        n, mc = sc.fs.n, sc.p["maxCand"]
+        cand, cc, off, cont, cell = net.candidates(n, mc)
+        assert_same(cell, orc.grid["bodyCell"], f"{what}: bodyCell")
+        assert_same(off, orc.grid["gridCellOffsets"], f"{what}: gridCellOffsets")
+        assert_same(cont, orc.grid["gridCellContents"], f"{what}: gridCellContents")
+        assert_same(cc, orc.grid["candCount"], f"{what}: candCount")
+        cc_clip = np.minimum(cc, mc)
+        mask = (np.arange(mc)[None, :] < cc_clip[:, None]).ravel()
+        assert_same(cand[mask], orc.grid["candPartner"][mask], f"{what}: candPartner")
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("rng", [RNG_PHILOX, RNG_WANG])
+def test_network_steps_match_oracle(ctx, periodic, rng):
+    from softbox_amd.network import Network
+
+    sc = small_network(seed=5, n_fil=150, links=400, periodic=periodic, rng=rng)
+    orc = ffi.OracleNet(sc)
+    net = Network(ctx, sc)
+    steps = 25
+    for t in range(steps):
+        orc.step(t, 1)
+    net.step(0, steps)
+    _compare(net, orc, sc, f"periodic={periodic} rng={rng} after {steps} steps")
+    net.close()
+
+
+def test_network_box_mode_with_walls(ctx):
+    """Reference parity mode: clamped grid + containment box; rods start partly outside the walls."""
+    from softbox_amd.network import Network
+
+    sc = small_network(seed=6, n_fil=120, links=300, periodic=False)
+    sc.p["boxParams"] = [1.0e-4, 0.8 * sc.box, 0.8 * sc.box, 0.8 * sc.box, 0.0035, 0.5, 3.0, sc.p["params"][0]]
+    orc = ffi.OracleNet(sc)
+    net = Network(ctx, sc)
+    for t in range(7):
+        orc.step(t, 1)
+    net.step(0, 7)
+    _compare(net, orc, sc, "box mode")
+    net.close()
+
+
+def test_graph_replay_equals_eager(ctx):
+    """A hipGraph-captured run must equal eager launches, replay after replay."""
+    from softbox_amd.network import Network
+
+    sc = small_network(seed=7, n_fil=150, links=400)
+    a = Network(ctx, sc.copy())
+    b = Network(ctx, sc.copy())
+    b.use_graph(True)
+    a.step(0, 5)
+    b.step(0, 5)            # capture + 2 replays + 1 eager step
+    a.step(5, 6)
+    b.step(5, 6)            # replays starting from the other pose buffer
+    fa, xa = a.pull(sc.fs.copy(), sc.xl.copy())
+    fb, xb = b.pull(sc.fs.copy(), sc.xl.copy())
+    for k in ("coord", "uVec", "yVec", "end1", "forceSum"):
+        assert_same(getattr(fb, k), getattr(fa, k), f"graph vs eager {k}")
+    assert_same(xb.linkState, xa.linkState, "graph vs eager linkState")
+    a.close()
+    b.close()
+
+
+def test_free_rods_fdt(ctx):
+    """Config-1 workload at full size: per-step body-frame displacement variances give D_par and
+    D_perp = kT/gamma exactly (closed forms), and <u(t).u(0)> = exp(-2 D_rot t) gives D_rot."""
+    from softbox_amd.network import Network
+    from softbox_amd.physics import rod_drag_si
+
+    sc = scenes.free_rods(n=1024, seed=0)
+    net = Network(ctx, sc)
+    fs = sc.fs.copy()
+    dt = float(sc.p["params"][0])
+    kT = scenes.KT_WORKLOAD
+    g = rod_drag_si(1.0, 0.0035, scenes.ETA_WATER)
+    Dpar, Dperp, Drot = kT / g[0] * 1e12, kT / g[1] * 1e12, kT / g[4]
+    # closed forms for this workload (um^2/s, rad^2/s)
+    assert abs(Dpar - 3.11485) < 2e-4 and abs(Dperp - 1.89757) < 2e-4 and abs(Drot - 16.8759) < 2e-3
+    u0 = sc.fs.plane("uVec").copy()
+    par, perp = [], []
+    prev_c, prev_u, prev_y = sc.fs.plane("coord").copy(), u0.copy(), sc.fs.plane("yVec").copy()
+    for t in range(60):
+        net.step(t, 1)
+        net.pull(fs)
+        c = fs.plane("coord").astype(np.float64)
+        d = c - prev_c
+        par.append((d * prev_u).sum(0))
+        perp.append((d * prev_y).sum(0))
+        prev_c = c.copy()
+        prev_u = fs.plane("uVec").astype(np.float64).copy()
+        prev_y = fs.plane("yVec").astype(np.float64).copy()
+    steps = 1000
+    net.step(60, steps - 60)
+    # single-step increments projected on the pre-step body frame
+    par = np.concatenate(par)
+    perp = np.concatenate(perp)
+    n = par.size
+    Dp = par.var() / (2 * dt)
+    Dq = perp.var() / (2 * dt)
+    tol = 4.0 * np.sqrt(2.0 / n)  # 4 standard errors of a variance estimate
+    assert abs(Dp / Dpar - 1) < tol, (Dp, Dpar, tol)
+    assert abs(Dq / Dperp - 1) < tol, (Dq, Dperp, tol)
+    # rotational: C(t) = <u(t).u(0)>
+    net.pull(fs)
+    C = (fs.plane("uVec").astype(np.float64) * u0).sum(0).mean()
+    T = steps * dt
+    Dr = -np.log(C) / (2 * T)
+    assert abs(Dr / Drot - 1) < 0.2, (Dr, Drot, C)  # ~3 standard errors at 1024 rods
+    net.close()
