@@ -982,20 +982,20 @@ void sbo_link_brute_gather(int nSeg, int nLinks, const int *linkFilA, const int 
 }
 
 /* ------------------------------------------------------------------ whole step */
-/* Filament block of DenseContractileHarness.cpuStep (DenseContractileHarness.java:350-429) with no
- * motors: [publish + grid + broad phase] -> zero + Brownian -> chain -> [unbind -> countActiveLinks ->
- * linkForces -> linkTorsion -> CSR A -> gatherA -> CSR B -> gatherB] -> confine -> integrate -> derive. */
-void sbo_network_step(sbo_net *net, int32_t step) {
+void sbo_network_grid(sbo_net *net) {
     int n = net->nRod;
-    if (net->gridOn) {
-        sbo_publish_filaments(n, n, net->coord, net->segLength, net->actinRadius, net->center, net->boundingRadius);
-        sbo_body_cell(n, n, net->center, net->gridParams, net->gridDims, net->periodic, net->bodyCell);
-        sbo_grid_build_serial(n, n, net->bodyCell, net->gridDims, net->cellCount, net->gridCellOffsets,
-                              net->gridCellContents);
-        sbo_broad_phase(n, n, net->center, net->boundingRadius, net->bodyCell, net->gridCellOffsets,
-                        net->gridCellContents, net->gridDims, net->gridParams, net->periodic, net->boxLen,
-                        net->maxCand, net->candPartner, net->candCount);
-    }
+    sbo_publish_filaments(n, n, net->coord, net->segLength, net->actinRadius, net->center, net->boundingRadius);
+    sbo_body_cell(n, n, net->center, net->gridParams, net->gridDims, net->periodic, net->bodyCell);
+    sbo_grid_build_serial(n, n, net->bodyCell, net->gridDims, net->cellCount, net->gridCellOffsets,
+                          net->gridCellContents);
+    sbo_broad_phase(n, n, net->center, net->boundingRadius, net->bodyCell, net->gridCellOffsets,
+                    net->gridCellContents, net->gridDims, net->gridParams, net->periodic, net->boxLen, net->maxCand,
+                    net->candPartner, net->candCount);
+}
+
+/* zero + Brownian + chain + crosslink unbind/count/force/torsion/gathers (accumulates forceSum) */
+void sbo_network_forces(sbo_net *net, int32_t step) {
+    int n = net->nRod;
     sbo_zero_accumulators(n, net->forceSum, net->torqueSum);
     sbo_brownian_force(n, net->randForce, net->randTorque, net->bTransGam, net->bRotGam, net->brownTransScale,
                        net->brownRotScale, net->params[1], step, net->seed, net->rng, net->stream, 0u);
@@ -1021,10 +1021,24 @@ void sbo_network_step(sbo_net *net, int32_t step) {
         sbo_seg_gather_b(n, net->segOffsetsB, net->segIdxB, net->xlinkData, net->linkState, net->forceSum,
                          net->torqueSum);
     }
+}
+
+/* confine + integrate + derive */
+void sbo_network_finish(sbo_net *net, int32_t step) {
+    int n = net->nRod;
     if (net->confineOn)
         sbo_confine(n, net->coord, net->uVec, net->segLength, net->bTransGam, net->forceSum, net->torqueSum,
                     net->boxParams, step);
     sbo_integrate(n, net->coord, net->uVec, net->yVec, net->forceSum, net->torqueSum, net->randForce,
                   net->randTorque, net->bTransGam, net->bRotGam, net->params[0]);
     sbo_derive(n, net->coord, net->uVec, net->yVec, net->zVec, net->end1, net->end2, net->segLength);
+}
+
+/* Filament block of DenseContractileHarness.cpuStep (DenseContractileHarness.java:350-429) with no
+ * motors: [publish + grid + broad phase] -> zero + Brownian -> chain -> [unbind -> countActiveLinks ->
+ * linkForces -> linkTorsion -> CSR A -> gatherA -> CSR B -> gatherB] -> confine -> integrate -> derive. */
+void sbo_network_step(sbo_net *net, int32_t step) {
+    if (net->gridOn) sbo_network_grid(net);
+    sbo_network_forces(net, step);
+    sbo_network_finish(net, step);
 }

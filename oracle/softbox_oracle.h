@@ -13,10 +13,11 @@
  * follows is given beside it.  All arrays are planar SoA (x-plane | y-plane |
  * z-plane, plane stride = n) exactly as in RigidRodBody.java:26-57.
  *
- * Parity pins (what checks each function beyond retyping):
- *   drag              closed forms recorded in JOURNAL.md:2527-2532 (tests/test_oracle_drag.py)
- *   wang hash/uniform integer semantics checked against an independent 64-bit
- *                     Python model + statistical moments (tests/test_oracle_rng.py)
+ * Parity pins (what checks each function beyond retyping; tests/test_oracle_pins.py,
+ * tests/test_oracle_grid_links.py, tests/test_oracle_motor.py):
+ *   drag              values recorded in JOURNAL.md:2527-2532 + closed forms
+ *   wang hash/uniform Java int semantics walked by hand (unsigned shifts, 32-bit wrap), the
+ *                     uniform mapping's edge values, Box-Muller moments (mean 0, var 1, kurtosis 3)
  *   philox4x32-10     Random123 known-answer vectors (Salmon et al., SC'11)
  *   brownian/integrate FDT closed forms D = kT/gamma within 3 sigma; zero-noise
  *                     invariance; closed-form single-step rotation
@@ -123,6 +124,52 @@ void sbo_seg_gather_b(int nSeg, const int *segOffsets, const int *segLinkIdx, co
 void sbo_link_brute_gather(int nSeg, int nLinks, const int *linkFilA, const int *linkFilB, const float *xlinkData,
                            const int *linkState, float *bForceSum, float *bTorqueSum);    /* :178-198 */
 
+/* ---- myosin side (softbox_oracle_motor.c) ----
+ * motor body: sub-body 3m rod, 3m+1 lever, 3m+2 head (MotorStore.java:45-60), stride nB = 3*nMotors */
+void sbo_publish_head_from_body(int nM, const float *bodyCoord, const float *bodyUVec, const float *bodySegLength,
+                                float *head, float *uVec, float *rodUVec);                 /* MotorStore.java:300-314 */
+void sbo_publish_motors(int nM, int cap, int base, const float *head, const float *reach, float *center,
+                        float *boundingRadius, int *ownerStore, int *ownerSlot);           /* MotorStore.java:326-347 */
+void sbo_grid_reachable(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
+                        const float *segEnd1, const float *segEnd2, const float *gridParams, const int *gridDims,
+                        const int *gridCellOffsets, const int *gridCellContents, const int *ownerStore,
+                        const int *ownerSlot, int maxCand, int *reachSeg, int *reachCount,
+                        const float *kinParams);                                          /* BindingDetectionSystem.java:143-228 */
+void sbo_brute_reachable(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
+                         const float *segEnd1, const float *segEnd2, int maxCand, int *reachSeg, int *reachCount,
+                         const float *kinParams);                                         /* :260-283 */
+void sbo_bind_nearest(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
+                      const float *segEnd1, const float *segEnd2, const int *reachSeg, const int *reachCount,
+                      int maxCand, int *boundSeg, float *bindArc, const float *kinParams); /* :356-389 */
+void sbo_catch_slip_release(int nM, int *boundSeg, const float *forceDotFil, const float *forceMag, int *cooldown,
+                            int *stats, int *capStats, const float *kinParams, int32_t step, int32_t seed, int rng,
+                            uint32_t stream);                                             /* NucleotideCycleSystem.java:77-135 */
+void sbo_nucleotide_cycle(int nM, int *nucleotideState, const int *boundSeg, const float *forceDotHist,
+                          const float *nucParams, int32_t step, int32_t seed, int rng, uint32_t stream); /* :33-70 */
+void sbo_bond_forces(int nM, int nSeg, const float *motorCoord, const float *motorUVec, const float *motorYVec,
+                     const float *motorBRotGam, const float *filCoord, const float *filUVec, const float *filYVec,
+                     const float *filBRotGam, const float *filSegLength, const int *boundSeg, const float *bindArc,
+                     const int *nucleotideState, float *bondData, const float *xbParams); /* CrossBridgeSystem.java:58-142 */
+void sbo_apply_head_force(int nM, const float *bondData, float *bodyForceSum, float *bodyTorqueSum); /* :145-157 */
+void sbo_register_force_dot(int nM, const float *bondData, const int *boundSeg, float *forceDotFil, float *forceMag,
+                            float *forceDotHist, int *forceDotPlace);                    /* :166-187 */
+void sbo_motor_seg_gather(int nSeg, const int *segMotorOffsets, const int *segMotorMyo, const float *bondData,
+                          float *filForceSum, float *filTorqueSum);                       /* :302-320 */
+void sbo_motor_joints(int nB, const float *coord, const float *uVec, const float *segLength, const float *bTransGam,
+                      const float *bRotGam, float *forceSum, float *torqueSum, const int *nucleotideState,
+                      const float *jointParams);                                          /* MotorJointSystem.java:72-198 */
+void sbo_dimer_couple(int nB, int nD, const float *coord, const float *uVec, const float *segLength,
+                      const float *bTransGam, const float *bRotGam, float *forceSum, float *torqueSum,
+                      const int *motorA, const int *motorB, const int *parallel, const float *dimerParams,
+                      const int *boundSeg);                                               /* DimerCouplingSystem.java:83-184 */
+void sbo_mini_tether(int nMB, int nBb, int nD, const float *mCoord, const float *mUVec, const float *mSegLength,
+                     const float *mBTransGam, const float *mBRotGam, float *mForceSum, float *mTorqueSum,
+                     const float *bbCoord, const float *bbUVec, const float *bbInvDragY, const int *headBackboneSlot,
+                     const int *motorA, const float *attachAxial, float *miniData,
+                     const float *miniParams);                                            /* MiniFilamentSystem.java:65-132 */
+void sbo_backbone_gather(int nBb, const int *bbDimerOffsets, const int *bbDimerList, const float *miniData,
+                         float *bbForceSum, float *bbTorqueSum);                          /* :137-155 */
+
 /* ---- one whole network step (DenseContractileHarness.java:350-429 filament block, no motors) ---- */
 typedef struct sbo_net {
     int nRod;
@@ -156,6 +203,39 @@ typedef struct sbo_net {
 } sbo_net;
 
 void sbo_network_step(sbo_net *net, int32_t step);
+void sbo_network_grid(sbo_net *net);                   /* publish + body cell + grid + broad phase */
+void sbo_network_forces(sbo_net *net, int32_t step);   /* zero, Brownian, chain, crosslinks + gathers */
+void sbo_network_finish(sbo_net *net, int32_t step);   /* confine, integrate, derive */
+
+/* ---- one whole contractile step: motors + minifilament backbones + the filament network ----
+ * DenseContractileHarness.cpuStep (DenseContractileHarness.java:350-429) without turnover.  The body
+ * view holds filaments in slots [0, nRod) and motor heads in [nRod, nRod + nM) (base = nRod). */
+typedef struct sbo_motors {
+    int nM;                                   /* motors; sub-bodies nB = 3*nM */
+    float *coord, *uVec, *yVec, *zVec, *end1, *end2, *segLength, *bTransGam, *bRotGam;
+    float *forceSum, *torqueSum, *randForce, *randTorque, *brownTransScale, *brownRotScale;
+    float *head, *hUVec, *rodUVec, *reach;    /* published binding interface, stride nM */
+    int *boundSeg, *cooldown, *stats, *capStats, *nucleotideState, *forceDotPlace;
+    float *bindArc, *forceDotFil, *forceMag, *forceDotHist;
+    float bodyParams[2], jointParams[11], kinParams[14], nucParams[8], xbParams[6];
+    int *reachSeg, *reachCount;               /* nM*maxCand, nM */
+    float *bondData;                          /* 13*nM */
+    int *segMotorCount, *segMotorOffsets, *segMotorMyo;
+    int nD;                                   /* dimers */
+    int *motorA, *motorB, *parallel;
+    float dimerParams[5];
+    int nBb, nMD;                             /* backbones, tethered dimers */
+    float *bbCoord, *bbUVec, *bbYVec, *bbZVec, *bbEnd1, *bbEnd2, *bbSegLength, *bbTransGam, *bbRotGam;
+    float *bbForceSum, *bbTorqueSum, *bbRandForce, *bbRandTorque, *bbBts, *bbBrs, *bbInvDragY;
+    int *headBackboneSlot, *tetherMotor;
+    float *attachAxial, *miniData;
+    int *bbDimerCount, *bbDimerOffsets, *bbDimerList;
+    float miniParams[3], bbBodyParams[2];
+    int32_t seedMotor, seedBackbone;
+    int *ownerStore, *ownerSlot;              /* body view back-pointers, cap = nRod + nM */
+} sbo_motors;
+
+void sbo_contractile_step(sbo_net *net, sbo_motors *mo, int32_t step);
 
 #ifdef __cplusplus
 }

@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -27,6 +28,10 @@ class Context:
         if rc != _lib.SBX_OK:
             raise _lib.SoftboxError(rc, "sbx_create failed")
         self.h = h
+        self._children = weakref.WeakSet()   # networks built on this context: closed before it
+
+    def adopt(self, obj) -> None:
+        self._children.add(obj)
 
     # ---------------------------------------------------------------- errors
     def check(self, rc: int) -> int:
@@ -42,6 +47,8 @@ class Context:
         self.check(self.lib.sbx_sync(self.h))
 
     def close(self) -> None:
+        for ch in list(getattr(self, "_children", ())):
+            ch.close()
         if getattr(self, "h", None):
             self.lib.sbx_destroy(self.h)
             self.h = None

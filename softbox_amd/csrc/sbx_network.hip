@@ -529,6 +529,11 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
             done++;
         }
         while (nSteps - done >= G) {
+            if (done != advanced) {  // a replay reads its first step from the device counter: bring it up to date
+                hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, c->stream, net->dStep, done - advanced);
+                if ((r = check_launch(c, "k_add_step"))) return r;
+                advanced = done;
+            }
             if ((r = check_hip(c, hipGraphLaunch(net->graphExec, c->stream), "graph launch"))) return r;
             done += G;
             advanced += G;  // G is even: net->cur is unchanged by a replay

@@ -27,6 +27,7 @@ class Network:
         ctx.check(ctx.lib.sbx_net_create(ctx.h, C.byref(self._fs), C.byref(self._xs) if self._xs else None,
                                          C.byref(self._p), C.byref(h)))
         self.h = h
+        ctx.adopt(self)
 
     def step(self, step0: int, n: int = 1) -> None:
         self.ctx.check(self.ctx.lib.sbx_net_step(self.h, step0, n))
@@ -79,9 +80,9 @@ class Network:
         return fs, xl
 
     def close(self) -> None:
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             self.ctx.lib.sbx_net_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):  # pragma: no cover
         try:

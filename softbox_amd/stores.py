@@ -179,3 +179,101 @@ class DeviceCrosslinks:
         for k in names or self.NAMES:
             getattr(xl, k)[:] = self.ctx.to_host(getattr(self, k))
         return xl
+
+
+NUC_NONE, NUC_ATP, NUC_ADPPI, NUC_ADP = 0, 1, 2, 3
+FREE_BINDABLE, FREE_COOLDOWN = -1, -2
+XB_STRIDE = 13
+MINI_STRIDE = 6
+
+
+class MotorStore:
+    """MotorStore.java:32-263: a 3-sub-body myosin per motor (3m rod, 3m+1 lever, 3m+2 head) as a
+    rigid-rod body of n = 3*nMotors, the published binding interface and the kinetic state."""
+
+    ROD_LEN, LEVER_LEN, HEAD_LEN = 0.080, 0.008, 0.020      # um (MotorStore.java:62)
+    ROD_R, LEVER_R, HEAD_R = 0.003, 0.002, 0.010
+
+    def __init__(self, nMotors: int):
+        self.nM = nMotors
+        self.body = FilamentStore(3 * nMotors)               # the shared rigid-rod layout (no chain topology)
+        self.head = np.zeros(3 * nMotors, np.float32)
+        self.uVec = np.zeros(3 * nMotors, np.float32)
+        self.rodUVec = np.zeros(3 * nMotors, np.float32)
+        self.reach = np.zeros(nMotors, np.float32)
+        self.boundSeg = np.full(nMotors, FREE_BINDABLE, np.int32)
+        self.bindArc = np.zeros(nMotors, np.float32)
+        self.cooldown = np.zeros(nMotors, np.int32)
+        self.stats = np.zeros(2 * nMotors, np.int32)
+        self.capStats = np.zeros(nMotors, np.int32)
+        self.nucleotideState = np.full(nMotors, NUC_NONE, np.int32)
+        self.forceDotFil = np.zeros(nMotors, np.float32)
+        self.forceMag = np.zeros(nMotors, np.float32)
+        self.forceDotHist = np.zeros(10 * nMotors, np.float32)
+        self.forceDotPlace = np.zeros(nMotors, np.int32)
+        self.bondData = np.zeros(XB_STRIDE * nMotors, np.float32)
+        self.bodyParams = np.zeros(2, np.float32)
+        self.jointParams = np.zeros(11, np.float32)
+        self.kinParams = np.zeros(14, np.float32)
+        self.nucParams = np.zeros(8, np.float32)
+        self.xbParams = np.zeros(6, np.float32)
+
+    def set_joint_params(self, dt, stall_pn=6.0):                       # MotorStore.java:244-250
+        self.jointParams[:] = np.array([dt, 0.4, 0.4, 0.4, 0.0, 0.4, 0.4, 0.0, 96.0, 0.4, stall_pn], np.float32)
+
+    def set_kin_params(self, myoColTol, alignTol, dt, kT):               # MotorStore.java:190-221
+        self.kinParams[:] = np.array([100.0, 0.92, 0.08, 2.5e-9, 0.4e-9, kT, dt, myoColTol, alignTol, 0.0,
+                                      np.ceil(1.0e-5 / dt), 12.0e-12, 0.0, 1.0], np.float32)
+
+    def set_nuc_params(self, dt):                                       # MotorStore.java:255-263
+        self.nucParams[:] = np.array([dt, 2.0e4, 100.0, 100.0, 1.0e4, 0.0, 1.0e3, 1.0e3], np.float32)
+
+    def copy(self) -> "MotorStore":
+        o = MotorStore.__new__(MotorStore)
+        for k, v in self.__dict__.items():
+            setattr(o, k, v.copy() if hasattr(v, "copy") else v)
+        return o
+
+
+class DimerStore:
+    """DimerStore.java:26-66: disjoint 1:1 motor pairing (motorA = 2d, motorB = 2d + 1)."""
+
+    def __init__(self, nDimers: int):
+        self.nD = nDimers
+        self.motorA = np.zeros(nDimers, np.int32)
+        self.motorB = np.zeros(nDimers, np.int32)
+        self.parallel = np.ones(nDimers, np.int32)
+        self.dimerParams = np.zeros(5, np.float32)
+
+    def set_dimer_params(self, dt):                                     # DimerStore.java:59-65
+        self.dimerParams[:] = np.array([dt, 0.2, 0.4, 160.0, MotorStore.ROD_LEN], np.float32)
+
+    def copy(self):
+        o = DimerStore(self.nD)
+        for k, v in self.__dict__.items():
+            setattr(o, k, v.copy() if hasattr(v, "copy") else v)
+        return o
+
+
+class MiniFilamentStore:
+    """MiniFilamentStore.java:29-135: backbone rigid rods + per-dimer tether data."""
+
+    BACKBONE_LEN, BACKBONE_R, HEAD_ZONE = 0.180, 0.005, 0.05
+
+    def __init__(self, nBackbones: int, nDimers: int):
+        self.nBb = nBackbones
+        self.nMD = nDimers
+        self.backbone = FilamentStore(nBackbones)
+        self.headBackboneSlot = np.zeros(nDimers, np.int32)
+        self.motorA = np.zeros(nDimers, np.int32)
+        self.attachAxial = np.zeros(nDimers, np.float32)
+        self.miniData = np.zeros(MINI_STRIDE * nDimers, np.float32)
+        self.bbInvDragY = np.zeros(2 * nBackbones, np.float32)
+        self.miniParams = np.zeros(3, np.float32)
+        self.bbBodyParams = np.zeros(2, np.float32)
+
+    def copy(self):
+        o = MiniFilamentStore.__new__(MiniFilamentStore)
+        for k, v in self.__dict__.items():
+            setattr(o, k, v.copy() if hasattr(v, "copy") else v)
+        return o

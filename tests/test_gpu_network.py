@@ -80,22 +80,25 @@ def test_network_box_mode_with_walls(ctx):
 
 
 def test_graph_replay_equals_eager(ctx):
-    """A hipGraph-captured run must equal eager launches, replay after replay."""
+    """A hipGraph-captured run must equal eager launches, replay after replay, whatever mix of odd
+    and even step counts (eager alignment steps before replays, remainders after)."""
     from softbox_amd.network import Network
 
     sc = small_network(seed=7, n_fil=150, links=400)
     a = Network(ctx, sc.copy())
     b = Network(ctx, sc.copy())
     b.use_graph(True)
-    a.step(0, 5)
-    b.step(0, 5)            # capture + 2 replays + 1 eager step
-    a.step(5, 6)
-    b.step(5, 6)            # replays starting from the other pose buffer
+    t = 0
+    for n in (5, 6, 3, 4, 1, 7):
+        a.step(t, n)
+        b.step(t, n)
+        t += n
     fa, xa = a.pull(sc.fs.copy(), sc.xl.copy())
     fb, xb = b.pull(sc.fs.copy(), sc.xl.copy())
     for k in ("coord", "uVec", "yVec", "end1", "forceSum"):
         assert_same(getattr(fb, k), getattr(fa, k), f"graph vs eager {k}")
     assert_same(xb.linkState, xa.linkState, "graph vs eager linkState")
+    assert_same(xb.strainHist, xa.strainHist, "graph vs eager strain ring")
     a.close()
     b.close()
 
