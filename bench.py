@@ -86,7 +86,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the sharded multi-GPU design is described in DESIGN.md", file=sys.stderr)
+              "the multi-GPU design (spatial slabs + halo exchange) is documentation only", file=sys.stderr)
         return 2
 
     import torch

@@ -47,7 +47,7 @@ typedef struct sbx_ctx sbx_ctx;
 typedef struct sbx_net sbx_net;
 
 /* ------------------------------------------------------------------ context and memory
- * Replaces TornadoExecutionPlan / DataTransferMode plumbing: FIRST_EXECUTION ->
+ * Replaces the TornadoExecutionPlan / DataTransferMode plumbing: FIRST_EXECUTION ->
  * sbx_push once, EVERY_EXECUTION -> host scalar arrays passed per call, UNDER_DEMAND -> sbx_pull. */
 int sbx_create(sbx_ctx **out, int device, void *hip_stream); /* hip_stream NULL: library-owned stream */
 void sbx_destroy(sbx_ctx *c);
@@ -166,6 +166,65 @@ int sbx_link_torsion(sbx_ctx *c, const float *filUVec, const int *linkFilA, cons
 int sbx_seg_gather(sbx_ctx *c, const int *segOffsets, const int *segLinkIdx, const float *xlinkData,
                    const int *linkState, float *filForceSum, float *filTorqueSum, int nSeg, int side);
 
+/* ------------------------------------------------------------------ myosin side
+ * Motor body: sub-body 3m rod, 3m+1 lever, 3m+2 head (MotorStore.java:45-60), planar stride nB = 3*nM.
+ * Published per-motor arrays (head, uVec, rodUVec) have stride nM.  counts: [0] nM [1] step [2] seed [3] nSeg. */
+
+/* MotorStore.publishHeadFromBody (MotorStore.java:300-314) */
+int sbx_publish_head_from_body(sbx_ctx *c, const float *bodyCoord, const float *bodyUVec, const float *bodySegLength,
+                               float *head, float *uVec, float *rodUVec, const int *counts, int nM);
+/* MotorStore.publishToBodyView (MotorStore.java:326-347); publishParams[0] = first view slot */
+int sbx_publish_motors(sbx_ctx *c, const float *head, const float *reach, float *center, float *boundingRadius,
+                       int *ownerStore, int *ownerSlot, const int *publishParams, const int *counts, int nM, int cap);
+/* BindingDetectionSystem.gridReachable (BindingDetectionSystem.java:143-228); kinParams[7] reach, [8] alignTol */
+int sbx_grid_reachable(sbx_ctx *c, const float *head, const float *uVec, const float *rodUVec, const float *segEnd1,
+                       const float *segEnd2, const float *gridParams, const int *gridDims,
+                       const int *gridCellOffsets, const int *gridCellContents, const int *ownerStore,
+                       const int *ownerSlot, int *motorReachSeg, int *motorReachCount, const float *kinParams,
+                       const int *counts, int nSeg, int maxCand);
+/* BindingDetectionSystem.bindNearest (BindingDetectionSystem.java:356-389): binding events */
+int sbx_bind_nearest(sbx_ctx *c, const float *head, const float *uVec, const float *rodUVec, const float *segEnd1,
+                     const float *segEnd2, const int *motorCandSeg, const int *motorCandCount, int *boundSeg,
+                     float *bindArc, const float *kinParams, const int *counts, int nSeg, int maxCand);
+/* NucleotideCycleSystem.catchSlipRelease (NucleotideCycleSystem.java:77-135): unbinding events */
+int sbx_catch_slip_release(sbx_ctx *c, int *boundSeg, const float *forceDotFil, const float *forceMag,
+                           int *cooldown, int *stats, int *capStats, const float *kinParams, const int *counts,
+                           int nM, int rng, uint32_t stream);
+/* NucleotideCycleSystem.cycle (NucleotideCycleSystem.java:33-70) */
+int sbx_nucleotide_cycle(sbx_ctx *c, int *nucleotideState, const int *boundSeg, const float *forceDotHist,
+                         const float *nucParams, const int *counts, int nM, int rng, uint32_t stream);
+/* CrossBridgeSystem.bondForces (CrossBridgeSystem.java:58-142); bondData stride 13 */
+int sbx_bond_forces(sbx_ctx *c, const float *motorCoord, const float *motorUVec, const float *motorYVec,
+                    const float *motorBRotGam, const float *filCoord, const float *filUVec, const float *filYVec,
+                    const float *filBRotGam, const float *filSegLength, const int *boundSeg, const float *bindArc,
+                    const int *nucleotideState, float *bondData, const float *xbParams, int nM, int nSeg);
+/* CrossBridgeSystem.applyHeadForce (CrossBridgeSystem.java:145-157) */
+int sbx_apply_head_force(sbx_ctx *c, const float *bondData, float *bodyForceSum, float *bodyTorqueSum, int nM);
+/* CrossBridgeSystem.registerForceDot (CrossBridgeSystem.java:166-187) */
+int sbx_register_force_dot(sbx_ctx *c, const float *bondData, const int *boundSeg, float *forceDotFil,
+                           float *forceMag, float *forceDotHist, int *forceDotPlace, int nM);
+/* CrossBridgeSystem.segGather (CrossBridgeSystem.java:302-320); CSR from sbx_csr_build keyed by boundSeg */
+int sbx_motor_seg_gather(sbx_ctx *c, const int *segMotorOffsets, const int *segMotorMyo, const float *bondData,
+                         float *filForceSum, float *filTorqueSum, int nSeg);
+/* MotorJointSystem.joints (MotorJointSystem.java:72-198) */
+int sbx_motor_joints(sbx_ctx *c, const float *coord, const float *uVec, const float *segLength,
+                     const float *bTransGam, const float *bRotGam, float *forceSum, float *torqueSum,
+                     const int *nucleotideState, const float *jointParams, int nB);
+/* DimerCouplingSystem.couple (DimerCouplingSystem.java:83-184); needs a disjoint pairing (each motor in <= 1 dimer) */
+int sbx_dimer_couple(sbx_ctx *c, const float *coord, const float *uVec, const float *segLength,
+                     const float *bTransGam, const float *bRotGam, float *forceSum, float *torqueSum,
+                     const int *motorA, const int *motorB, const int *parallel, const float *dimerParams,
+                     const int *boundSeg, int nB, int nD);
+/* MiniFilamentSystem.tether (MiniFilamentSystem.java:65-132) */
+int sbx_mini_tether(sbx_ctx *c, const float *mCoord, const float *mUVec, const float *mSegLength,
+                    const float *mBTransGam, const float *mBRotGam, float *mForceSum, float *mTorqueSum,
+                    const float *bbCoord, const float *bbUVec, const float *bbInvDragY, const int *headBackboneSlot,
+                    const int *motorA, const float *attachAxial, float *miniData, const float *miniParams, int nMB,
+                    int nBb, int nD);
+/* MiniFilamentSystem.backboneGather (MiniFilamentSystem.java:137-155) */
+int sbx_backbone_gather(sbx_ctx *c, const int *bbDimerOffsets, const int *bbDimerList, const float *miniData,
+                        float *bbForceSum, float *bbTorqueSum, int nBb);
+
 /* ------------------------------------------------------------------ whole network step
  * Replaces the filament block of DenseContractileHarness.buildPlan / cpuStep (DenseContractileHarness.java:
  * 350-429, 460-606) as ONE library call per step: [publish + grid + broad phase] -> [crosslink unbind ->
@@ -215,6 +274,45 @@ typedef struct sbx_net_params {
     int rng;
     uint32_t stream;
 } sbx_net_params;
+
+/* Myosin stage of the whole step (DenseContractileHarness.cpuStep, DenseContractileHarness.java:350-395):
+ * publish heads -> grid over filaments + heads -> per motor: gridReachable + catchSlipRelease +
+ * bindNearest + cycle -> per sub-body joints -> per dimer couple -> per dimer tether -> per backbone
+ * gather + confine + Brownian + integrate + derive -> per motor bond + head force -> per sub-body
+ * Brownian + integrate + derive (+ registerForceDot) -> filament block with the motor -> segment gather.
+ * Requires the clamped grid (periodic = 0) with gridOn; the filament-pair broad phase is not run. */
+typedef struct sbx_motors {
+    int nM;                                    /* motors; body sub-bodies nB = 3*nM */
+    float *coord, *uVec, *yVec, *zVec, *end1, *end2, *segLength, *bTransGam, *bRotGam;
+    float *brownTransScale, *brownRotScale;
+    float *forceSum, *torqueSum;               /* body accumulators (required) */
+    float *head, *hUVec, *rodUVec, *reach;     /* published binding interface, stride nM */
+    int *boundSeg, *cooldown, *stats, *capStats, *nucleotideState, *forceDotPlace;
+    float *bindArc, *forceDotFil, *forceMag, *forceDotHist, *bondData;
+    int *reachSeg, *reachCount;                /* nM*maxCand, nM */
+    int nD;                                    /* dimers (disjoint motor pairs) */
+    int *motorA, *motorB, *parallel;
+    int nBb, nMD;                              /* backbones, tethered dimers */
+    float *bbCoord, *bbUVec, *bbYVec, *bbZVec, *bbEnd1, *bbEnd2, *bbSegLength, *bbTransGam, *bbRotGam;
+    float *bbBrownTransScale, *bbBrownRotScale, *bbInvDragY;
+    float *bbForceSum, *bbTorqueSum;           /* optional: NULL keeps backbone F/T in registers */
+    int *headBackboneSlot, *tetherMotor;
+    float *attachAxial, *miniData;
+} sbx_motors;
+
+typedef struct sbx_motor_params {
+    float bodyParams[2];    /* MotorStore.bodyParams: dt, sqrt(2kT/dt) */
+    float jointParams[11];  /* MotorStore.jointParams */
+    float kinParams[14];    /* MotorStore.kinParams */
+    float nucParams[8];     /* MotorStore.nucParams */
+    float xbParams[6];      /* CrossBridgeSystem xbParams */
+    float dimerParams[5];   /* DimerStore.dimerParams */
+    float miniParams[3];    /* MiniFilamentStore.miniParams */
+    float bbBodyParams[2];  /* MiniFilamentStore.bbBodyParams */
+    int seedMotor, seedBackbone;
+} sbx_motor_params;
+
+int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_params *p);
 
 int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *xl, const sbx_net_params *p,
                    sbx_net **out);

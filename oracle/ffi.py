@@ -150,3 +150,97 @@ class OracleNet:
     def step(self, step0: int, n: int = 1) -> None:
         for t in range(step0, step0 + n):
             self.lib.sbo_network_step(C.byref(self.s), t)
+
+
+class SboMotors(C.Structure):
+    _fields_ = (
+        [("nM", i32)]
+        + [(k, vp) for k in ("coord", "uVec", "yVec", "zVec", "end1", "end2", "segLength", "bTransGam", "bRotGam",
+                             "forceSum", "torqueSum", "randForce", "randTorque", "brownTransScale", "brownRotScale",
+                             "head", "hUVec", "rodUVec", "reach", "boundSeg", "cooldown", "stats", "capStats",
+                             "nucleotideState", "forceDotPlace", "bindArc", "forceDotFil", "forceMag", "forceDotHist")]
+        + [("bodyParams", C.c_float * 2), ("jointParams", C.c_float * 11), ("kinParams", C.c_float * 14),
+           ("nucParams", C.c_float * 8), ("xbParams", C.c_float * 6)]
+        + [(k, vp) for k in ("reachSeg", "reachCount", "bondData", "segMotorCount", "segMotorOffsets", "segMotorMyo")]
+        + [("nD", i32)]
+        + [(k, vp) for k in ("motorA", "motorB", "parallel")]
+        + [("dimerParams", C.c_float * 5), ("nBb", i32), ("nMD", i32)]
+        + [(k, vp) for k in ("bbCoord", "bbUVec", "bbYVec", "bbZVec", "bbEnd1", "bbEnd2", "bbSegLength", "bbTransGam",
+                             "bbRotGam", "bbForceSum", "bbTorqueSum", "bbRandForce", "bbRandTorque", "bbBts", "bbBrs",
+                             "bbInvDragY", "headBackboneSlot", "tetherMotor", "attachAxial", "miniData",
+                             "bbDimerCount", "bbDimerOffsets", "bbDimerList")]
+        + [("miniParams", C.c_float * 3), ("bbBodyParams", C.c_float * 2), ("seedMotor", i32),
+           ("seedBackbone", i32), ("ownerStore", vp), ("ownerSlot", vp)]
+    )
+
+
+class OracleContractile(OracleNet):
+    """sbo_contractile_step over host copies of a ContractileScene (DenseContractileHarness.cpuStep)."""
+
+    def __init__(self, scene):
+        super().__init__(scene)
+        mot, dim, mini = scene.mot.copy(), scene.dim.copy(), scene.mini.copy()
+        self.mot, self.dim, self.mini = mot, dim, mini
+        n, nM = self.fs.n, mot.nM
+        cap = n + nM
+        cells = int(self.p["gridDims"][3])
+        mc = int(self.p["maxCand"])
+        # grid arrays cover filaments + motor heads
+        self.grid = dict(center=np.zeros(3 * cap, np.float32), boundingRadius=np.zeros(cap, np.float32),
+                         bodyCell=np.zeros(cap, np.int32), cellCount=np.zeros(cells, np.int32),
+                         gridCellOffsets=np.zeros(cells + 1, np.int32), gridCellContents=np.zeros(cap, np.int32),
+                         candPartner=np.full(1, -1, np.int32), candCount=np.zeros(1, np.int32))
+        for k, a in self.grid.items():
+            setattr(self.s, k, P(a).value)
+        m = SboMotors()
+        m.nM = nM
+        body = mot.body
+        for k in ("coord", "uVec", "yVec", "zVec", "end1", "end2", "segLength", "bTransGam", "bRotGam", "forceSum",
+                  "torqueSum", "randForce", "randTorque", "brownTransScale", "brownRotScale"):
+            setattr(m, k, P(getattr(body, k)).value)
+        m.hUVec = P(mot.uVec).value
+        for k in ("head", "rodUVec", "reach", "boundSeg", "cooldown", "stats", "capStats", "nucleotideState",
+                  "forceDotPlace", "bindArc", "forceDotFil", "forceMag", "forceDotHist", "bondData"):
+            setattr(m, k, P(getattr(mot, k)).value)
+        for k in ("bodyParams", "jointParams", "kinParams", "nucParams", "xbParams"):
+            arr = getattr(m, k)
+            for i, v in enumerate(getattr(mot, k)):
+                arr[i] = v
+        self.mreach = dict(reachSeg=np.full(nM * mc, -1, np.int32), reachCount=np.zeros(nM, np.int32),
+                           segMotorCount=np.zeros(n, np.int32), segMotorOffsets=np.zeros(n + 1, np.int32),
+                           segMotorMyo=np.zeros(max(1, nM), np.int32), ownerStore=np.zeros(cap, np.int32),
+                           ownerSlot=np.zeros(cap, np.int32), bbDimerCount=np.zeros(mini.nBb, np.int32),
+                           bbDimerOffsets=np.zeros(mini.nBb + 1, np.int32),
+                           bbDimerList=np.zeros(max(1, mini.nMD), np.int32))
+        for k, a in self.mreach.items():
+            setattr(m, k, P(a).value)
+        m.nD = dim.nD
+        for k in ("motorA", "motorB", "parallel"):
+            setattr(m, k, P(getattr(dim, k)).value)
+        for i, v in enumerate(dim.dimerParams):
+            m.dimerParams[i] = v
+        m.nBb, m.nMD = mini.nBb, mini.nMD
+        bb = mini.backbone
+        for src, dst in (("coord", "bbCoord"), ("uVec", "bbUVec"), ("yVec", "bbYVec"), ("zVec", "bbZVec"),
+                         ("end1", "bbEnd1"), ("end2", "bbEnd2"), ("segLength", "bbSegLength"),
+                         ("bTransGam", "bbTransGam"), ("bRotGam", "bbRotGam"), ("forceSum", "bbForceSum"),
+                         ("torqueSum", "bbTorqueSum"), ("randForce", "bbRandForce"), ("randTorque", "bbRandTorque"),
+                         ("brownTransScale", "bbBts"), ("brownRotScale", "bbBrs")):
+            setattr(m, dst, P(getattr(bb, src)).value)
+        m.bbInvDragY = P(mini.bbInvDragY).value
+        m.headBackboneSlot = P(mini.headBackboneSlot).value
+        m.tetherMotor = P(mini.motorA).value
+        m.attachAxial = P(mini.attachAxial).value
+        m.miniData = P(mini.miniData).value
+        for i, v in enumerate(mini.miniParams):
+            m.miniParams[i] = v
+        for i, v in enumerate(mini.bbBodyParams):
+            m.bbBodyParams[i] = v
+        m.seedMotor = int(self.p["seedMotor"])
+        m.seedBackbone = int(self.p["seedBackbone"])
+        self.m = m
+        self.lib.sbo_contractile_step.argtypes = [C.POINTER(SboNet), C.POINTER(SboMotors), i32]
+
+    def step(self, step0: int, n: int = 1) -> None:
+        for t in range(step0, step0 + n):
+            self.lib.sbo_contractile_step(C.byref(self.s), C.byref(self.m), t)

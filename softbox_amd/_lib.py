@@ -74,6 +74,39 @@ class SbxNetParams(C.Structure):
     ]
 
 
+class SbxMotors(C.Structure):
+    _fields_ = (
+        [("nM", i32)]
+        + [(k, vp) for k in ("coord", "uVec", "yVec", "zVec", "end1", "end2", "segLength", "bTransGam", "bRotGam",
+                             "brownTransScale", "brownRotScale", "forceSum", "torqueSum", "head", "hUVec", "rodUVec",
+                             "reach", "boundSeg", "cooldown", "stats", "capStats", "nucleotideState",
+                             "forceDotPlace", "bindArc", "forceDotFil", "forceMag", "forceDotHist", "bondData",
+                             "reachSeg", "reachCount")]
+        + [("nD", i32)]
+        + [(k, vp) for k in ("motorA", "motorB", "parallel")]
+        + [("nBb", i32), ("nMD", i32)]
+        + [(k, vp) for k in ("bbCoord", "bbUVec", "bbYVec", "bbZVec", "bbEnd1", "bbEnd2", "bbSegLength",
+                             "bbTransGam", "bbRotGam", "bbBrownTransScale", "bbBrownRotScale", "bbInvDragY",
+                             "bbForceSum", "bbTorqueSum", "headBackboneSlot", "tetherMotor", "attachAxial",
+                             "miniData")]
+    )
+
+
+class SbxMotorParams(C.Structure):
+    _fields_ = [
+        ("bodyParams", C.c_float * 2),
+        ("jointParams", C.c_float * 11),
+        ("kinParams", C.c_float * 14),
+        ("nucParams", C.c_float * 8),
+        ("xbParams", C.c_float * 6),
+        ("dimerParams", C.c_float * 5),
+        ("miniParams", C.c_float * 3),
+        ("bbBodyParams", C.c_float * 2),
+        ("seedMotor", i32),
+        ("seedBackbone", i32),
+    ]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPES)
 _SIGS = {
     "sbx_create": [C.POINTER(vp), i32, vp],
@@ -107,8 +140,23 @@ _SIGS = {
     "sbx_link_forces": [vp] + [vp] * 10 + [f32p, i32, i32],
     "sbx_link_torsion": [vp] + [vp] * 8 + [f32p, i32, i32],
     "sbx_seg_gather": [vp, vp, vp, vp, vp, vp, vp, i32, i32],
+    "sbx_publish_head_from_body": [vp, vp, vp, vp, vp, vp, vp, i32p, i32],
+    "sbx_publish_motors": [vp, vp, vp, vp, vp, vp, vp, i32p, i32p, i32, i32],
+    "sbx_grid_reachable": [vp] + [vp] * 5 + [f32p, i32p] + [vp] * 6 + [f32p, i32p, i32, i32],
+    "sbx_bind_nearest": [vp] + [vp] * 9 + [f32p, i32p, i32, i32],
+    "sbx_catch_slip_release": [vp] + [vp] * 6 + [f32p, i32p, i32, i32, u32],
+    "sbx_nucleotide_cycle": [vp, vp, vp, vp, f32p, i32p, i32, i32, u32],
+    "sbx_bond_forces": [vp] + [vp] * 13 + [f32p, i32, i32],
+    "sbx_apply_head_force": [vp, vp, vp, vp, i32],
+    "sbx_register_force_dot": [vp] + [vp] * 6 + [i32],
+    "sbx_motor_seg_gather": [vp] + [vp] * 5 + [i32],
+    "sbx_motor_joints": [vp] + [vp] * 8 + [f32p, i32],
+    "sbx_dimer_couple": [vp] + [vp] * 11 + [f32p, vp, i32, i32],
+    "sbx_mini_tether": [vp] + [vp] * 14 + [f32p, i32, i32, i32],
+    "sbx_backbone_gather": [vp] + [vp] * 5 + [i32],
     "sbx_net_create": [vp, C.POINTER(SbxFilaments), C.POINTER(SbxCrosslinks), C.POINTER(SbxNetParams), C.POINTER(vp)],
     "sbx_net_destroy": [vp],
+    "sbx_net_attach_motors": [vp, C.POINTER(SbxMotors), C.POINTER(SbxMotorParams)],
     "sbx_net_step": [vp, i32, i32],
     "sbx_net_publish": [vp],
     "sbx_net_topology_changed": [vp],

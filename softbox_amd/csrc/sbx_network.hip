@@ -19,6 +19,7 @@
 #include "sbx_grid.h"
 #include "sbx_internal.h"
 #include "sbx_links.h"
+#include "sbx_motor.h"
 
 using namespace sbx;
 
@@ -59,6 +60,20 @@ struct sbx_net {
     std::vector<Timer> timers;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> eventPool;
+    // myosin stage
+    int hasMotors = 0;
+    sbx_motors mo{};
+    sbx_motor_params mp{};
+    int *ownerStore = nullptr, *ownerSlot = nullptr;   // body view back-pointers over cap = n + nM
+    int *mOff = nullptr, *mList = nullptr;             // segment -> bound motors CSR (rebuilt every step)
+    int *bbOff = nullptr, *bbList = nullptr;           // backbone -> tethered dimers CSR (static)
+    KinParams kp{};
+    NucParams np{};
+    XbParams xp{};
+    JointParams jp{};
+    DimerParams dp{};
+    MiniParams mip{};
+    ReachGrid rg{};
     // graph
     int useGraph = 0;
     int graphSteps = 0;  // non-zero while a capture is being recorded
@@ -69,9 +84,14 @@ struct sbx_net {
 
 namespace {
 
-enum { T_BIN = 0, T_SORT, T_PAIRS, T_UNBIND, T_COUNT, T_LINK, T_ROD, T_COUNT_TIMERS };
-const char *kTimerNames[T_COUNT_TIMERS] = {"k_body_cell", "grid_sort", "k_broad_phase", "k_net_unbind",
-                                           "k_net_count",  "k_net_link", "k_rod"};
+enum {
+    T_BIN = 0, T_SORT, T_PAIRS, T_UNBIND, T_COUNT, T_LINK, T_ROD, T_MBIND, T_MJOINT, T_DIMER, T_TETHER, T_BACKBONE,
+    T_BOND, T_MINTEG, T_MCSR, T_COUNT_TIMERS
+};
+const char *kTimerNames[T_COUNT_TIMERS] = {"k_body_cell", "grid_sort",     "k_broad_phase", "k_net_unbind",
+                                           "k_net_count", "k_net_link",    "k_rod",         "k_motor_bind",
+                                           "k_motor_joints", "k_net_dimer", "k_net_tether", "k_backbone",
+                                           "k_bond_head", "k_motor_integ", "motor_csr"};
 
 struct RodArgs {
     const float *coord, *uVec, *yVec;
@@ -81,6 +101,9 @@ struct RodArgs {
     const int *e1Slot, *e1Side, *e2Slot, *e2Side;
     const int *offA, *idxA, *offB, *idxB, *linkState;
     const float *xlinkData;
+    const int *mOff, *mList;
+    const float *bondData;
+    int hasMotors;
     const int *dStep;
     int stepOff;
     int n, chainOn, hasLinks, confineOn, checkInt, rng, seed;
@@ -121,6 +144,15 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         for (int k = 0; k < 3; k++) {
             f[k] = (float)(f[k] + fb[k]);
             t[k] = (float)(t[k] + tb[k]);
+        }
+    }
+    if (a.hasMotors) {  // CrossBridgeSystem.segGather (CrossBridgeSystem.java:302-320)
+        double fm[3] = {0, 0, 0}, tm[3] = {0, 0, 0};
+        csr_payload_sum(i, a.mOff, a.mList, a.bondData, XB_STRIDE, 6, fm, tm);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            f[k] = (float)(f[k] + fm[k]);
+            t[k] = (float)(t[k] + tm[k]);
         }
     }
     Pose p;
@@ -235,6 +267,260 @@ __global__ void __launch_bounds__(256) k_bounding_radius(const float *__restrict
     if (i < n) boundR[i] = 0.5f * segLength[i] + actinRadius;  // FilamentStore.java:291
 }
 
+// ---------------------------------------------------------------- myosin stage kernels
+// body cell over the view: filaments [0, n) bin their centre, motors [n, n + nM) their head tip
+__global__ void __launch_bounds__(256) k_body_cell_view(const float *__restrict__ coord, int n,
+                                                        const float *__restrict__ head, int nM, GridGeom g,
+                                                        int *__restrict__ bodyCell) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n + nM) return;
+    if (i < n) bodyCell[i] = cell_of(coord[i], coord[n + i], coord[2 * n + i], g);
+    else {
+        int m = i - n;
+        bodyCell[i] = cell_of(head[m], head[nM + m], head[2 * nM + m], g);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_owner_view(int *__restrict__ ownerStore, int *__restrict__ ownerSlot, int n,
+                                                    int nM) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n + nM) return;
+    ownerStore[i] = i < n ? 0 : 1;
+    ownerSlot[i] = i < n ? i : i - n;
+}
+
+struct MotorArgs {
+    sbx_motors m;
+    const float *e1, *e2;          // filament end1/end2 (derived last step)
+    const int *off, *cont, *ownerStore, *ownerSlot;
+    ReachGrid rg;
+    KinParams kp;
+    NucParams np;
+    int maxCand, nSeg, seed, rng;
+    uint32_t stream;
+    const int *dStep;
+    int stepOff;
+};
+
+// publishHeadFromBody (MotorStore.java:300-314) for the grid + binding of this step
+__global__ void __launch_bounds__(256) k_pub_heads(sbx_motors m) {
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= m.nM) return;
+    int nM = m.nM, nB = 3 * nM, h = 3 * k + 2, r = 3 * k;
+    float hl = m.segLength[h];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        m.head[a * nM + k] = m.coord[a * nB + h] + 0.5f * hl * m.uVec[a * nB + h];
+        m.hUVec[a * nM + k] = m.uVec[a * nB + h];
+        m.rodUVec[a * nM + k] = m.uVec[a * nB + r];
+    }
+}
+
+// gridReachable + catchSlipRelease + bindNearest + cycle for one motor (each reads only its own kinetic
+// state and the step-start filament geometry, so the per-motor fusion equals the reference's four passes)
+__global__ void __launch_bounds__(256) k_motor_bind(MotorArgs a) {
+    int k = blockIdx.x * 256 + threadIdx.x;
+    const sbx_motors &m = a.m;
+    if (k >= m.nM) return;
+    int step = *a.dStep + a.stepOff;
+    int cnt = grid_reachable_one(k, m.nM, a.nSeg, m.head, m.hUVec, m.rodUVec, a.e1, a.e2, a.rg, a.off, a.cont,
+                                 a.ownerStore, a.ownerSlot, m.reachSeg, a.maxCand, a.kp.myoColTol, a.kp.alignTol);
+    m.reachCount[k] = cnt;
+    int bs = m.boundSeg[k], cool = m.cooldown[k];
+    catch_slip_one(k, bs, cool, m.stats, m.capStats, m.forceDotFil[k], m.forceMag[k], a.kp, step, a.seed, a.rng,
+                   a.stream);
+    float arc = m.bindArc[k];
+    bind_nearest_one(k, m.nM, a.nSeg, m.head, m.hUVec, m.rodUVec, a.e1, a.e2, m.reachSeg, cnt, a.maxCand, bs, arc,
+                     a.kp.myoColTol, a.kp.alignTol);
+    m.boundSeg[k] = bs;
+    m.cooldown[k] = cool;
+    m.bindArc[k] = arc;
+    m.nucleotideState[k] = nucleotide_one(k, m.nucleotideState[k], bs >= 0, m.forceDotHist, a.np, step, a.seed,
+                                          a.rng, a.stream);
+}
+
+// zero + MotorJointSystem.joints for one sub-body: F = (float)(0 + joints)
+__global__ void __launch_bounds__(256) k_motor_joints0(sbx_motors m, JointParams jp) {
+    int s = blockIdx.x * 256 + threadIdx.x;
+    int nB = 3 * m.nM;
+    if (s >= nB) return;
+    double f[3] = {0, 0, 0}, t[3] = {0, 0, 0};
+    joints_one(s, nB, m.coord, m.uVec, m.segLength, m.bTransGam, m.bRotGam, m.nucleotideState[s / 3], jp, f, t);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        m.forceSum[a * nB + s] = (float)(0.0f + f[a]);
+        m.torqueSum[a * nB + s] = (float)(0.0f + t[a]);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_net_dimer(sbx_motors m, DimerParams dp) {
+    int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= m.nD) return;
+    int mA = m.motorA[d], mB = m.motorB[d];
+    bool both = m.boundSeg[mA] >= 0 && m.boundSeg[mB] >= 0;
+    dimer_one(mA, mB, m.parallel[d] != 0, both, 3 * m.nM, m.coord, m.uVec, m.segLength, m.bTransGam, m.bRotGam,
+              m.forceSum, m.torqueSum, dp);
+}
+
+__global__ void __launch_bounds__(256) k_net_tether(sbx_motors m, MiniParams mp) {
+    int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= m.nMD) return;
+    tether_one(d, 3 * m.nM, m.nBb, m.headBackboneSlot[d], 3 * m.tetherMotor[d], m.attachAxial[d], m.coord, m.uVec,
+               m.segLength, m.bTransGam, m.bRotGam, m.forceSum, m.torqueSum, m.bbCoord, m.bbUVec, m.bbInvDragY,
+               m.miniData, mp);
+}
+
+struct BackboneArgs {
+    sbx_motors m;
+    const int *bbOff, *bbList;
+    float dt, bfm;
+    int confineOn, checkInt, seed, rng;
+    uint32_t stream;
+    double tau, halfX, halfY, halfZ, coeff;
+    const int *dStep;
+    int stepOff;
+};
+
+// one backbone: zero -> backboneGather -> confine -> Brownian -> integrate -> derive (in place)
+__global__ void __launch_bounds__(256) k_backbone(BackboneArgs a) {
+    int b = blockIdx.x * 256 + threadIdx.x;
+    const sbx_motors &m = a.m;
+    int nBb = m.nBb;
+    if (b >= nBb) return;
+    int step = *a.dStep + a.stepOff;
+    double fd[3] = {0, 0, 0}, td[3] = {0, 0, 0};
+    csr_payload_sum(b, a.bbOff, a.bbList, m.miniData, MINI_STRIDE, 0, fd, td);
+    float f[3], t[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        f[k] = (float)(0.0f + fd[k]);
+        t[k] = (float)(0.0f + td[k]);
+    }
+    Pose p;
+    float gT[3], gR[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        p.c[k] = m.bbCoord[k * nBb + b];
+        p.u[k] = m.bbUVec[k * nBb + b];
+        p.y[k] = m.bbYVec[k * nBb + b];
+        gT[k] = m.bbTransGam[k * nBb + b];
+        gR[k] = m.bbRotGam[k * nBb + b];
+    }
+    float len = m.bbSegLength[b];
+    if (a.confineOn && !(a.checkInt > 1 && (step % a.checkInt) != 0))
+        confine_body(p.c, p.u, len, gT[0], a.tau, a.halfX, a.halfY, a.halfZ, a.coeff, f, t);
+    if (m.bbForceSum) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            m.bbForceSum[k * nBb + b] = f[k];
+            m.bbTorqueSum[k * nBb + b] = t[k];
+        }
+    }
+    float gf[3], gt[3], rf[3], rt[3];
+    brownian_noise(a.rng, (uint32_t)b, step, a.seed, a.stream, gf, gt);
+    float tS = m.bbBrownTransScale[b], rS = m.bbBrownRotScale[b];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        rf[k] = brown_amp(tS, a.bfm, gT[k], gf[k]);
+        rt[k] = brown_amp(rS, a.bfm, gR[k], gt[k]);
+    }
+    Derived d;
+    integrate_derive(p, f, t, rf, rt, gT, gR, a.dt, len, d);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        m.bbCoord[k * nBb + b] = p.c[k];
+        m.bbUVec[k * nBb + b] = p.u[k];
+        m.bbYVec[k * nBb + b] = p.y[k];
+        m.bbZVec[k * nBb + b] = d.z[k];
+        m.bbEnd1[k * nBb + b] = d.e1[k];
+        m.bbEnd2[k * nBb + b] = d.e2[k];
+    }
+}
+
+// CrossBridgeSystem.bondForces + applyHeadForce for one motor (reads the step-start filament pose)
+__global__ void __launch_bounds__(256) k_bond_head(sbx_motors m, const float *__restrict__ fc,
+                                                   const float *__restrict__ fu, const float *__restrict__ fy,
+                                                   const float *__restrict__ fbRG, const float *__restrict__ fLen,
+                                                   XbParams xp, int nSeg) {
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= m.nM) return;
+    float out[13];
+#pragma unroll
+    for (int j = 0; j < 13; j++) out[j] = 0.0f;
+    int s = m.boundSeg[k];
+    if (s >= 0)
+        bond_one(k, m.nM, nSeg, s, m.coord, m.uVec, m.yVec, m.bRotGam, fc, fu, fy, fbRG, fLen, m.bindArc[k],
+                 m.nucleotideState[k], xp, out);
+    float *d = m.bondData + (size_t)k * XB_STRIDE;
+#pragma unroll
+    for (int j = 0; j < 13; j++) d[j] = out[j];
+    int nB = 3 * m.nM, h = 3 * k + 2;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        m.forceSum[a * nB + h] = m.forceSum[a * nB + h] + out[a];
+        m.torqueSum[a * nB + h] = m.torqueSum[a * nB + h] + out[3 + a];
+    }
+}
+
+// Brownian + integrate + derive of one motor sub-body (in place); the head thread also runs
+// CrossBridgeSystem.registerForceDot for its motor (CrossBridgeSystem.java:166-187)
+__global__ void __launch_bounds__(256) k_motor_integ(sbx_motors m, float dt, float bfm, int seed, int rng,
+                                                     uint32_t stream, const int *__restrict__ dStep, int stepOff) {
+    int s = blockIdx.x * 256 + threadIdx.x;
+    int nB = 3 * m.nM;
+    if (s >= nB) return;
+    int step = *dStep + stepOff;
+    Pose p;
+    float f[3], t[3], gT[3], gR[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        p.c[k] = m.coord[k * nB + s];
+        p.u[k] = m.uVec[k * nB + s];
+        p.y[k] = m.yVec[k * nB + s];
+        f[k] = m.forceSum[k * nB + s];
+        t[k] = m.torqueSum[k * nB + s];
+        gT[k] = m.bTransGam[k * nB + s];
+        gR[k] = m.bRotGam[k * nB + s];
+    }
+    float gf[3], gt[3], rf[3], rt[3];
+    brownian_noise(rng, (uint32_t)s, step, seed, stream, gf, gt);
+    float tS = m.brownTransScale[s], rS = m.brownRotScale[s];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        rf[k] = brown_amp(tS, bfm, gT[k], gf[k]);
+        rt[k] = brown_amp(rS, bfm, gR[k], gt[k]);
+    }
+    Derived d;
+    integrate_derive(p, f, t, rf, rt, gT, gR, dt, m.segLength[s], d);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        m.coord[k * nB + s] = p.c[k];
+        m.uVec[k * nB + s] = p.u[k];
+        m.yVec[k * nB + s] = p.y[k];
+        m.zVec[k * nB + s] = d.z[k];
+        m.end1[k * nB + s] = d.e1[k];
+        m.end2[k * nB + s] = d.e2[k];
+    }
+    if (s % 3 == 2) {
+        int mi = s / 3;
+        if (m.boundSeg[mi] >= 0) {
+            const float *bd = m.bondData + (size_t)mi * XB_STRIDE;
+            float fd = bd[12];
+            m.forceDotFil[mi] = fd;
+            float fx = bd[0], fy = bd[1], fz = bd[2];
+            m.forceMag[mi] = (float)sqrt((double)(fx * fx + fy * fy + fz * fz));
+            int pl = m.forceDotPlace[mi];
+            m.forceDotHist[mi * 10 + pl] = fd;
+            m.forceDotPlace[mi] = (pl + 1) % 10;
+        } else {
+            m.forceDotFil[mi] = 0.0f;
+            m.forceMag[mi] = 0.0f;
+            m.forceDotPlace[mi] = 0;
+            for (int k = 0; k < 10; k++) m.forceDotHist[mi * 10 + k] = 0.0f;
+        }
+    }
+}
+
 hipEvent_t take_event(sbx_net *net) {
     if (!net->eventPool.empty()) {
         hipEvent_t e = net->eventPool.back();
@@ -281,6 +567,83 @@ int drain_timers(sbx_net *net) {
     return SBX_OK;
 }
 
+// Binding and minifilament mechanics of one step (DenseContractileHarness.cpuStep,
+// DenseContractileHarness.java:350-395), before the filament block.  Every kernel reads the
+// step-start filament pose (the current ping-pong buffer); motor and backbone bodies integrate in
+// place after the last kernel that reads their step-start pose (k_bond_head).
+int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const float *yVec, int stepOff) {
+    sbx_ctx *c = net->ctx;
+    hipStream_t s = c->stream;
+    const int n = net->n;
+    const sbx_motors &m = net->mo;
+    const int nM = m.nM, nB = 3 * nM, cap = n + nM;
+    int r;
+    {
+        Scope sc(net, T_BIN);
+        hipLaunchKernelGGL(k_pub_heads, dim3(blocks_for(nM, 256)), dim3(256), 0, s, m);
+        hipLaunchKernelGGL(k_body_cell_view, dim3(blocks_for(cap, 256)), dim3(256), 0, s, coord, n, m.head, nM,
+                           net->geom, net->bodyCell);
+        if ((r = check_launch(c, "k_pub_heads/k_body_cell_view"))) return r;
+    }
+    {
+        Scope sc(net, T_SORT);
+        if ((r = counting_sort(c, net->sort, net->bodyCell, cap, net->p.gridDims[3], cap, net->cellOffsets,
+                               net->cellContents)))
+            return r;
+    }
+    {
+        Scope sc(net, T_MBIND);
+        MotorArgs a;
+        a.m = m;
+        a.e1 = net->fil.end1; a.e2 = net->fil.end2;
+        a.off = net->cellOffsets; a.cont = net->cellContents;
+        a.ownerStore = net->ownerStore; a.ownerSlot = net->ownerSlot;
+        a.rg = net->rg; a.kp = net->kp; a.np = net->np;
+        a.maxCand = net->p.maxCand; a.nSeg = n; a.seed = net->mp.seedMotor; a.rng = net->p.rng;
+        a.stream = net->p.stream; a.dStep = net->dStep; a.stepOff = stepOff;
+        hipLaunchKernelGGL(k_motor_bind, dim3(blocks_for(nM, 256)), dim3(256), 0, s, a);
+    }
+    {
+        Scope sc(net, T_MJOINT);
+        hipLaunchKernelGGL(k_motor_joints0, dim3(blocks_for(nB, 256)), dim3(256), 0, s, m, net->jp);
+    }
+    if (m.nD > 0) {
+        Scope sc(net, T_DIMER);
+        hipLaunchKernelGGL(k_net_dimer, dim3(blocks_for(m.nD, 256)), dim3(256), 0, s, m, net->dp);
+    }
+    if (m.nMD > 0) {
+        Scope sc(net, T_TETHER);
+        hipLaunchKernelGGL(k_net_tether, dim3(blocks_for(m.nMD, 256)), dim3(256), 0, s, m, net->mip);
+    }
+    if (m.nBb > 0) {
+        Scope sc(net, T_BACKBONE);
+        BackboneArgs b;
+        b.m = m;
+        b.bbOff = net->bbOff; b.bbList = net->bbList;
+        b.dt = net->mp.bbBodyParams[0]; b.bfm = net->mp.bbBodyParams[1];
+        b.confineOn = net->p.confineOn; b.checkInt = (int)net->p.boxParams[6];
+        b.seed = net->mp.seedBackbone; b.rng = net->p.rng; b.stream = net->p.stream;
+        b.tau = net->p.boxParams[0];
+        b.halfX = 0.5 * net->p.boxParams[1] - net->p.boxParams[4];
+        b.halfY = 0.5 * net->p.boxParams[2] - net->p.boxParams[4];
+        b.halfZ = 0.5 * net->p.boxParams[3] - net->p.boxParams[4];
+        b.coeff = net->p.boxParams[5];
+        b.dStep = net->dStep; b.stepOff = stepOff;
+        hipLaunchKernelGGL(k_backbone, dim3(blocks_for(m.nBb, 256)), dim3(256), 0, s, b);
+    }
+    {
+        Scope sc(net, T_BOND);
+        hipLaunchKernelGGL(k_bond_head, dim3(blocks_for(nM, 256)), dim3(256), 0, s, m, coord, uVec, yVec,
+                           net->fil.bRotGam, net->fil.segLength, net->xp, n);
+    }
+    {
+        Scope sc(net, T_MINTEG);
+        hipLaunchKernelGGL(k_motor_integ, dim3(blocks_for(nB, 256)), dim3(256), 0, s, m, net->mp.bodyParams[0],
+                           net->mp.bodyParams[1], net->mp.seedMotor, net->p.rng, net->p.stream, net->dStep, stepOff);
+    }
+    return check_launch(c, "myosin stage");
+}
+
 // one step's launches; stepOff is added to the device step counter
 int enqueue_step(sbx_net *net, int stepOff) {
     sbx_ctx *c = net->ctx;
@@ -288,7 +651,9 @@ int enqueue_step(sbx_net *net, int stepOff) {
     const int n = net->n, L = net->L;
     const float *coord = net->coordB[net->cur], *uVec = net->uB[net->cur], *yVec = net->yB[net->cur];
     int r;
-    if (net->p.gridOn) {
+    if (net->hasMotors) {
+        if ((r = enqueue_myosin(net, coord, uVec, yVec, stepOff))) return r;
+    } else if (net->p.gridOn) {
         {
             Scope sc(net, T_BIN);
             r = launch_body_cell(c, coord, n, n, net->geom, net->bodyCell);
@@ -343,6 +708,13 @@ int enqueue_step(sbx_net *net, int stepOff) {
     a.e2Slot = net->fil.end2NbrSlot; a.e2Side = net->fil.end2NbrSide;
     a.offA = net->offA; a.idxA = net->idxA; a.offB = net->offB; a.idxB = net->idxB;
     a.linkState = net->xl.linkState; a.xlinkData = net->xl.xlinkData;
+    a.hasMotors = net->hasMotors;
+    a.mOff = net->mOff; a.mList = net->mList; a.bondData = net->mo.bondData;
+    if (net->hasMotors) {
+        Scope sc(net, T_MCSR);
+        if ((r = counting_sort(c, net->sort, net->mo.boundSeg, net->mo.nM, n, net->mo.nM, net->mOff, net->mList)))
+            return r;
+    }
     a.dStep = net->dStep; a.stepOff = stepOff;
     a.n = n; a.chainOn = net->p.chainOn; a.hasLinks = L > 0;
     a.confineOn = net->p.confineOn; a.checkInt = (int)net->p.boxParams[6];
@@ -374,8 +746,10 @@ void release(sbx_net *net) {
     if (net->ctx) hipStreamSynchronize(net->ctx->stream);
     if (net->graphExec) hipGraphExecDestroy(net->graphExec);
     if (net->graph) hipGraphDestroy(net->graph);
-    void *bufs[] = {net->ownPose, net->boundR, net->bodyCell, net->cellOffsets, net->cellContents, net->candPartner,
-                    net->candCount, net->offA, net->idxA, net->offB, net->idxB, net->ownCount, net->dStep};
+    void *bufs[] = {net->ownPose,  net->boundR,    net->bodyCell, net->cellOffsets, net->cellContents,
+                    net->candPartner, net->candCount, net->offA,  net->idxA,        net->offB,
+                    net->idxB,     net->ownCount,  net->dStep,    net->ownerStore,  net->ownerSlot,
+                    net->mOff,     net->mList,     net->bbOff,    net->bbList};
     for (void *b : bufs)
         if (b) hipFree(b);
     sort_scratch_free(net->sort);
@@ -485,6 +859,108 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
 }
 
 void sbx_net_destroy(sbx_net *net) { release(net); }
+
+int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_params *p) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, m && p, "net_attach_motors: null argument");
+    SBX_REQUIRE(c, !net->hasMotors, "net_attach_motors: motors already attached");
+    SBX_REQUIRE(c, net->p.gridOn && !net->p.periodic,
+                "net_attach_motors: the binding grid needs gridOn with the clamped (non-periodic) grid");
+    const int n = net->n, nM = m->nM;
+    SBX_REQUIRE(c, nM > 0, "net_attach_motors: need nM > 0");
+    SBX_REQUIRE(c, (long long)n + nM < (1LL << 31), "net_attach_motors: too many bodies");
+    SBX_REQUIRE(c, m->coord && m->uVec && m->yVec && m->zVec && m->end1 && m->end2 && m->segLength && m->bTransGam &&
+                       m->bRotGam && m->brownTransScale && m->brownRotScale && m->forceSum && m->torqueSum,
+                "net_attach_motors: null motor body array");
+    SBX_REQUIRE(c, m->head && m->hUVec && m->rodUVec && m->boundSeg && m->cooldown && m->stats && m->capStats &&
+                       m->nucleotideState && m->forceDotPlace && m->bindArc && m->forceDotFil && m->forceMag &&
+                       m->forceDotHist && m->bondData && m->reachSeg && m->reachCount,
+                "net_attach_motors: null motor kinetic array");
+    SBX_REQUIRE(c, m->nD >= 0 && m->nBb >= 0 && m->nMD >= 0, "net_attach_motors: negative counts");
+    if (m->nD > 0) SBX_REQUIRE(c, m->motorA && m->motorB && m->parallel, "net_attach_motors: null dimer array");
+    if (m->nMD > 0) {
+        SBX_REQUIRE(c, m->nBb > 0, "net_attach_motors: tethered dimers without backbones");
+        SBX_REQUIRE(c, m->headBackboneSlot && m->tetherMotor && m->attachAxial && m->miniData,
+                    "net_attach_motors: null tether array");
+    }
+    if (m->nBb > 0)
+        SBX_REQUIRE(c, m->bbCoord && m->bbUVec && m->bbYVec && m->bbZVec && m->bbEnd1 && m->bbEnd2 &&
+                           m->bbSegLength && m->bbTransGam && m->bbRotGam && m->bbBrownTransScale &&
+                           m->bbBrownRotScale && m->bbInvDragY,
+                    "net_attach_motors: null backbone array");
+    SBX_REQUIRE(c, (m->bbForceSum == nullptr) == (m->bbTorqueSum == nullptr),
+                "net_attach_motors: bbForceSum and bbTorqueSum must be both set or both null");
+    SBX_REQUIRE(c, p->jointParams[0] != 0.0f && p->xbParams[3] != 0.0f && p->kinParams[5] != 0.0f,
+                "net_attach_motors: zero dt or kT in the motor parameters");
+    if (m->nD > 0) SBX_REQUIRE(c, p->dimerParams[0] != 0.0f, "net_attach_motors: zero dimer dt");
+    if (m->nMD > 0) SBX_REQUIRE(c, p->miniParams[0] != 0.0f, "net_attach_motors: zero tether dt");
+    int r = set_device(c);
+    if (r) return r;
+    const int cap = n + nM;
+    // the grid now bins filaments + heads: regrow the per-body grid arrays
+    int *bodyCell = nullptr, *cellContents = nullptr, *ownerStore = nullptr, *ownerSlot = nullptr, *mOff = nullptr,
+        *mList = nullptr, *bbOff = nullptr, *bbList = nullptr;
+    do {
+        if ((r = alloc(c, &bodyCell, cap))) break;
+        if ((r = alloc(c, &cellContents, cap))) break;
+        if ((r = alloc(c, &ownerStore, cap))) break;
+        if ((r = alloc(c, &ownerSlot, cap))) break;
+        if ((r = alloc(c, &mOff, (size_t)n + 1))) break;
+        if ((r = alloc(c, &mList, nM))) break;
+        if ((r = alloc(c, &bbOff, (size_t)m->nBb + 1))) break;
+        if ((r = alloc(c, &bbList, m->nMD))) break;
+        long long keys = net->p.gridDims[3];
+        if (keys < n) keys = n;
+        if (keys < m->nBb) keys = m->nBb;
+        if ((r = sort_scratch_reserve(c, net->sort, keys, cap))) break;
+    } while (0);
+    if (r) {
+        void *bufs[] = {bodyCell, cellContents, ownerStore, ownerSlot, mOff, mList, bbOff, bbList};
+        for (void *b : bufs)
+            if (b) hipFree(b);
+        return r;
+    }
+    hipStreamSynchronize(c->stream);
+    hipFree(net->bodyCell);
+    hipFree(net->cellContents);
+    net->bodyCell = bodyCell;
+    net->cellContents = cellContents;
+    net->ownerStore = ownerStore;
+    net->ownerSlot = ownerSlot;
+    net->mOff = mOff;
+    net->mList = mList;
+    net->bbOff = bbOff;
+    net->bbList = bbList;
+    net->mo = *m;
+    net->mp = *p;
+    net->kp = kin_params(p->kinParams);
+    net->np = nuc_params(p->nucParams);
+    net->xp = xb_params(p->xbParams);
+    net->jp = joint_params(p->jointParams);
+    net->dp = dimer_params(p->dimerParams);
+    net->mip = mini_params(p->miniParams);
+    const float *gp = net->p.gridParams;
+    net->rg = {gp[0], gp[1], gp[2], gp[4], net->p.gridDims[0], net->p.gridDims[1], net->p.gridDims[2]};
+    net->hasMotors = 1;
+    hipLaunchKernelGGL(k_owner_view, dim3(blocks_for(cap, 256)), dim3(256), 0, c->stream, ownerStore, ownerSlot, n,
+                       nM);
+    if ((r = check_launch(c, "k_owner_view"))) return r;
+    // tethers are static (MiniFilamentSystem.csrBuild keys headBackboneSlot): build the backbone CSR once
+    if (m->nBb > 0 && (r = counting_sort(c, net->sort, m->headBackboneSlot, m->nMD, m->nBb, m->nMD, bbOff, bbList)))
+        return r;
+    // a captured step no longer matches the launch sequence
+    if (net->graphExec) {
+        hipGraphExecDestroy(net->graphExec);
+        net->graphExec = nullptr;
+    }
+    if (net->graph) {
+        hipGraphDestroy(net->graph);
+        net->graph = nullptr;
+    }
+    return sbx_sync(c);
+}
 
 int sbx_net_step(sbx_net *net, int step0, int nSteps) {
     if (!net) return SBX_ERR_ARG;

@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _lib
 from .context import Context
-from .stores import DeviceCrosslinks, DeviceFilaments
+from .stores import DeviceCrosslinks, DeviceFilaments, DeviceMotors
 
 
 class Network:
@@ -28,6 +28,13 @@ class Network:
                                          C.byref(self._p), C.byref(h)))
         self.h = h
         ctx.adopt(self)
+        self.mot = None
+        if getattr(scene, "mot", None) is not None:
+            self.mot = DeviceMotors(ctx, scene.mot, scene.dim, scene.mini, int(scene.p["maxCand"]),
+                                    seeds=(scene.p["seedMotor"], scene.p["seedBackbone"]))
+            self._ms = self.mot.struct()
+            self._mp = self.mot.params()
+            ctx.check(ctx.lib.sbx_net_attach_motors(self.h, C.byref(self._ms), C.byref(self._mp)))
 
     def step(self, step0: int, n: int = 1) -> None:
         self.ctx.check(self.ctx.lib.sbx_net_step(self.h, step0, n))
@@ -78,6 +85,11 @@ class Network:
         if xl is not None and self.xl is not None:
             self.xl.pull(xl)
         return fs, xl
+
+    def pull_motors(self, mot=None, mini=None):
+        """Synchronise and copy the motor and backbone state back into host stores."""
+        self.ctx.sync()
+        self.mot.pull(mot, mini)
 
     def close(self) -> None:
         if getattr(self, "h", None) and getattr(self.ctx, "h", None):

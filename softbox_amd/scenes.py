@@ -40,7 +40,10 @@ class Scene:
 
     def net_params(self) -> _lib.SbxNetParams:
         s = _lib.SbxNetParams()
+        fields = {name for name, _ in _lib.SbxNetParams._fields_}
         for k, v in self.p.items():
+            if k not in fields:
+                continue
             f = getattr(s, k)
             if hasattr(f, "__len__"):
                 for i, x in enumerate(v):
@@ -246,3 +249,136 @@ def derive_host(fs: FilamentStore) -> None:
     half = fs.segLength * f(0.5)
     fs.plane("end1")[:] = np.stack([c[0] - half * ux, c[1] - half * uy, c[2] - half * uz])
     fs.plane("end2")[:] = np.stack([c[0] + half * ux, c[1] + half * uy, c[2] + half * uz])
+
+
+@dataclasses.dataclass
+class ContractileScene(Scene):
+    mot: object = None
+    dim: object = None
+    mini: object = None
+
+
+def contractile(n_fil: int = 50000, segs: int = 10, seg_len: float = 0.1, box: float = 30.0,
+                n_mini: int = 2000, dimers_each_end: int = 4, n_links: int = 100000, seed: int = 0,
+                dt: float = 1.0e-5, eta: float = ETA_WATER, kT: float = KT_WORKLOAD, stall_pn: float = 4.0,
+                rng: int = _lib.SBX_RNG_PHILOX, mini_near_fil: bool = False, unbind: bool = True,
+                xlink_rest: float = 0.035, bell=(0.0, 0.1, 4.26)) -> ContractileScene:
+    """Actomyosin network (DenseContractileHarness.buildScene, DenseContractileHarness.java:112-293):
+    the crosslinked filament network above, plus minifilaments of 2*dimers_each_end parallel dimers
+    (4 per end -> 16 heads) splayed around a 0.18 um backbone (ContractileAssayHarness.placeDimerAlong,
+    ContractileAssayHarness.java:982-1017), in the reference's clamped box with containment walls.
+    mini_near_fil=True places each backbone beside a random filament segment (small test scenes)."""
+    from .physics import rod_drag_si
+    from .stores import DimerStore, MiniFilamentStore, MotorStore
+
+    base = network(n_fil=n_fil, segs=segs, seg_len=seg_len, box=box, n_links=n_links, seed=seed, dt=dt,
+                   periodic=False, eta=eta, kT=kT, rest=xlink_rest, bell=bell, unbind=unbind, grid=False, rng=rng,
+                   confine=True)
+    g = np.random.default_rng(seed + 77)
+    nD = n_mini * 2 * dimers_each_end
+    nM = 2 * nD
+    mot = MotorStore(nM)
+    dim = DimerStore(nD)
+    mini = MiniFilamentStore(n_mini, nD)
+    MYO_COL_TOL, ALIGN_TOL = 0.025, -0.4
+    SIN80, COS80 = math.sin(math.radians(80.0)), math.cos(math.radians(80.0))
+    rl, ll, hl = MotorStore.ROD_LEN, MotorStore.LEVER_LEN, MotorStore.HEAD_LEN
+    b = mot.body
+    bb = mini.backbone
+    fc = base.fs.plane("coord")
+    fu = base.fs.plane("uVec")
+
+    def place_arm(m, rc, d, p, e2, splay):
+        rod, lever, head = 3 * m, 3 * m + 1, 3 * m + 2
+        b.plane("coord")[:, rod] = rc
+        b.plane("uVec")[:, rod] = d
+        b.plane("yVec")[:, rod] = p
+        lu = COS80 * d + splay * SIN80 * p
+        nrm = np.cross(d, p)
+        b.plane("coord")[:, lever] = e2 + 0.5 * ll * lu
+        b.plane("uVec")[:, lever] = lu
+        b.plane("yVec")[:, lever] = nrm
+        le2 = e2 + ll * lu
+        b.plane("coord")[:, head] = le2 + 0.5 * hl * lu
+        b.plane("uVec")[:, head] = lu
+        b.plane("yVec")[:, head] = nrm
+        b.brownTransScale[[rod, head]] = 1.0
+        b.brownRotScale[[rod, head]] = 0.3
+        b.brownTransScale[lever] = 0.0
+        b.brownRotScale[lever] = 0.0
+
+    d_i = 0
+    half = 0.5 * box
+    for mfi in range(n_mini):
+        if mini_near_fil:
+            s = int(g.integers(0, base.fs.n))
+            u = fu[:, s].astype(np.float64)
+            side = _unit_vectors(g, 1)[:, 0]
+            side -= (side @ u) * u
+            side /= np.linalg.norm(side)
+            c = fc[:, s].astype(np.float64) + 0.06 * side
+        else:
+            c = (g.random(3) - 0.5) * (box - 1.0)
+            u = _unit_vectors(g, 1)[:, 0]
+        y = _perp(u[:, None])[:, 0]
+        bb.plane("coord")[:, mfi] = c
+        bb.plane("uVec")[:, mfi] = u
+        bb.plane("yVec")[:, mfi] = y
+        bb.brownTransScale[mfi] = 1.0
+        bb.brownRotScale[mfi] = 0.3
+        for e in range(2):
+            dirn = -1.0 if e == 0 else 1.0
+            for j in range(dimers_each_end):
+                mag = MiniFilamentStore.BACKBONE_LEN / 2.0 - (j + 0.5) / dimers_each_end * MiniFilamentStore.HEAD_ZONE
+                ax = dirn * mag
+                e1 = c + ax * u
+                phi = (j + 0.5) / dimers_each_end * math.pi
+                p = math.cos(phi) * y + math.sin(phi) * np.cross(u, y)
+                d = dirn * u
+                d = d / np.linalg.norm(d)
+                p = p / np.linalg.norm(p)
+                mA, mB = 2 * d_i, 2 * d_i + 1
+                rc = e1 + 0.5 * rl * d
+                e2 = e1 + rl * d
+                place_arm(mA, rc, d, p, e2, +1)
+                place_arm(mB, rc, d, p, e2, -1)
+                dim.motorA[d_i], dim.motorB[d_i], dim.parallel[d_i] = mA, mB, 1
+                mini.headBackboneSlot[d_i], mini.motorA[d_i], mini.attachAxial[d_i] = mfi, mA, np.float32(ax)
+                mot.reach[mA] = mot.reach[mB] = MYO_COL_TOL
+                d_i += 1
+    # drag (DragTensorSystem.run(MotorStore), DragTensorSystem.java:105-122; MiniFilamentStore.initBackboneDrag)
+    from .physics import set_rod_drag
+
+    idx = np.arange(nM)
+    for role, (L, R) in enumerate([(rl, MotorStore.ROD_R), (ll, MotorStore.LEVER_R)]):
+        set_rod_drag(b, L, R, eta, kT, idx=3 * idx + role)
+    gsph = (6 * math.pi * eta * MotorStore.HEAD_R * 1e-6, 8 * math.pi * eta * (MotorStore.HEAD_R * 1e-6) ** 3)
+    hidx = 3 * idx + 2
+    b.segLength[hidx] = np.float32(hl)
+    for a in range(3):
+        b.plane("bTransGam")[a][hidx] = np.float32(gsph[0])
+        b.plane("bRotGam")[a][hidx] = np.float32(gsph[1])
+    set_rod_drag(bb, MiniFilamentStore.BACKBONE_LEN, MiniFilamentStore.BACKBONE_R, eta, kT)
+    gb = rod_drag_si(MiniFilamentStore.BACKBONE_LEN, MiniFilamentStore.BACKBONE_R, eta)
+    mini.bbInvDragY[:n_mini] = np.float32(1.0 / gb[1])
+    mini.bbInvDragY[n_mini:] = np.float32(1.0 / gb[4])
+    derive_host(b)
+    derive_host(bb)
+    bfm = brownian_force_mag(dt, kT)
+    mot.bodyParams[:] = np.array([dt, bfm], np.float32)
+    mot.set_joint_params(dt, stall_pn)
+    mot.set_kin_params(MYO_COL_TOL, ALIGN_TOL, dt, kT)
+    mot.kinParams[12] = 1.0                        # v1 12 pN break-force cap on (DenseContractileHarness.java:206)
+    mot.set_nuc_params(dt)
+    mot.xbParams[:] = np.array([1.0e-9, 90.0, 0.4, dt, hl, 0.0], np.float32)
+    dim.set_dimer_params(dt)
+    mini.miniParams[:] = np.array([dt, 0.07, 0.01], np.float32)
+    mini.bbBodyParams[:] = np.array([dt, bfm], np.float32)
+    p = dict(base.p)
+    seg_bound = 0.5 * seg_len + ACTIN_RADIUS
+    cutoff = MYO_COL_TOL + 0.5 * rl
+    p.update(grid_for(box, seg_bound, cutoff, False), gridOn=1)
+    p.update(confineOn=1, boxParams=[1.0e-4, box, box, box, ACTIN_RADIUS, 0.5, 1.0, dt])
+    p.update(seedMotor=seed + 0x0CDE5, seedBackbone=seed + 0x5C2F11)
+    return ContractileScene("contractile", base.fs, base.xl, p, box, seed, n_fil=n_fil, segs_per_fil=segs,
+                            mot=mot, dim=dim, mini=mini)

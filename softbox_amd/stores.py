@@ -277,3 +277,82 @@ class MiniFilamentStore:
         for k, v in self.__dict__.items():
             setattr(o, k, v.copy() if hasattr(v, "copy") else v)
         return o
+
+
+class DeviceMotors:
+    """Device mirror of a MotorStore + DimerStore + MiniFilamentStore; `struct()` gives the
+    sbx_motors of the C ABI and `params()` the sbx_motor_params."""
+
+    _BODY = ("coord", "uVec", "yVec", "zVec", "end1", "end2", "segLength", "bTransGam", "bRotGam",
+             "brownTransScale", "brownRotScale", "forceSum", "torqueSum")
+    _KIN = ("head", "rodUVec", "reach", "boundSeg", "cooldown", "stats", "capStats", "nucleotideState",
+            "forceDotPlace", "bindArc", "forceDotFil", "forceMag", "forceDotHist", "bondData")
+    _BB = (("coord", "bbCoord"), ("uVec", "bbUVec"), ("yVec", "bbYVec"), ("zVec", "bbZVec"), ("end1", "bbEnd1"),
+           ("end2", "bbEnd2"), ("segLength", "bbSegLength"), ("bTransGam", "bbTransGam"), ("bRotGam", "bbRotGam"),
+           ("brownTransScale", "bbBrownTransScale"), ("brownRotScale", "bbBrownRotScale"))
+
+    def __init__(self, ctx, mot: MotorStore, dim: DimerStore, mini: MiniFilamentStore, max_cand: int,
+                 seeds=(0, 0), keep_bb_forces: bool = True):
+        self.ctx = ctx
+        self.nM, self.nD, self.nBb, self.nMD = mot.nM, dim.nD, mini.nBb, mini.nMD
+        self.max_cand = max_cand
+        self.t = {}
+        for k in self._BODY:
+            self.t[k] = ctx.to_device(getattr(mot.body, k))
+        self.t["hUVec"] = ctx.to_device(mot.uVec)
+        for k in self._KIN:
+            self.t[k] = ctx.to_device(getattr(mot, k))
+        self.t["reachSeg"] = ctx.to_device(np.full(max(1, mot.nM * max_cand), -1, np.int32))
+        self.t["reachCount"] = ctx.to_device(np.zeros(mot.nM, np.int32))
+        for k in ("motorA", "motorB", "parallel"):
+            self.t[k] = ctx.to_device(getattr(dim, k)) if dim.nD else None
+        bb = mini.backbone
+        for src, dst in self._BB:
+            self.t[dst] = ctx.to_device(getattr(bb, src)) if mini.nBb else None
+        self.t["bbInvDragY"] = ctx.to_device(mini.bbInvDragY) if mini.nBb else None
+        self.t["bbForceSum"] = ctx.to_device(bb.forceSum) if (mini.nBb and keep_bb_forces) else None
+        self.t["bbTorqueSum"] = ctx.to_device(bb.torqueSum) if (mini.nBb and keep_bb_forces) else None
+        self.t["headBackboneSlot"] = ctx.to_device(mini.headBackboneSlot) if mini.nMD else None
+        self.t["tetherMotor"] = ctx.to_device(mini.motorA) if mini.nMD else None
+        self.t["attachAxial"] = ctx.to_device(mini.attachAxial) if mini.nMD else None
+        self.t["miniData"] = ctx.to_device(mini.miniData) if mini.nMD else None
+        p = _lib.SbxMotorParams()
+        for k, src in (("bodyParams", mot.bodyParams), ("jointParams", mot.jointParams), ("kinParams", mot.kinParams),
+                       ("nucParams", mot.nucParams), ("xbParams", mot.xbParams), ("dimerParams", dim.dimerParams),
+                       ("miniParams", mini.miniParams), ("bbBodyParams", mini.bbBodyParams)):
+            arr = getattr(p, k)
+            for i, v in enumerate(src):
+                arr[i] = float(v)
+        p.seedMotor, p.seedBackbone = int(seeds[0]), int(seeds[1])
+        self._params = p
+
+    def struct(self) -> _lib.SbxMotors:
+        from .context import ptr
+
+        s = _lib.SbxMotors()
+        s.nM, s.nD, s.nBb, s.nMD = self.nM, self.nD, self.nBb, self.nMD
+        for name, _ in _lib.SbxMotors._fields_:
+            if name in self.t:
+                v = self.t[name]
+                setattr(s, name, ptr(v).value if v is not None else None)
+        return s
+
+    def params(self) -> _lib.SbxMotorParams:
+        return self._params
+
+    def pull(self, mot: MotorStore = None, mini: MiniFilamentStore = None) -> None:
+        h = self.ctx.to_host
+        if mot is not None:
+            for k in self._BODY:
+                getattr(mot.body, k)[:] = h(self.t[k])
+            mot.uVec[:] = h(self.t["hUVec"])
+            for k in self._KIN:
+                getattr(mot, k)[:] = h(self.t[k])
+        if mini is not None and self.nBb:
+            for src, dst in self._BB:
+                getattr(mini.backbone, src)[:] = h(self.t[dst])
+            if self.t["bbForceSum"] is not None:
+                mini.backbone.forceSum[:] = h(self.t["bbForceSum"])
+                mini.backbone.torqueSum[:] = h(self.t["bbTorqueSum"])
+            if self.nMD:
+                mini.miniData[:] = h(self.t["miniData"])
