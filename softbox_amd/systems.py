@@ -172,14 +172,6 @@ class SpatialGrid:
                                                  ptr(candPartner), ptr(candCount), _n3(center), maxCand, periodic))
 
 
-class CrossBridgeSystem:
-    """CrossBridgeSystem.java:189-298 (CSR inverse: histogram + scan + scatter in one call)."""
-
-    @staticmethod
-    def csrBuild(ctx: Context, key, nKeys: int, offsets, items):
-        return ctx.check(ctx.lib.sbx_csr_build(ctx.h, ptr(key), key.numel(), nKeys, ptr(offsets), ptr(items)))
-
-
 class CrosslinkerSystem:
     """CrosslinkerSystem.java:47-808 (mechanics part)"""
 
@@ -229,3 +221,152 @@ class CrosslinkerSystem:
         return ctx.check(ctx.lib.sbx_seg_gather(ctx.h, ptr(segOffsets), ptr(segLinkIdx), ptr(xlinkData),
                                                 ptr(linkState), ptr(filForceSum), ptr(filTorqueSum),
                                                 _n3(filForceSum), 1))
+
+
+class MotorStoreKernels:
+    """MotorStore.java:300-347: the published binding interface of the motor bodies."""
+
+    @staticmethod
+    def publishHeadFromBody(ctx: Context, bodyCoord, bodyUVec, bodySegLength, head, uVec, rodUVec, counts):
+        cn, cnp = _i(counts)
+        return ctx.check(ctx.lib.sbx_publish_head_from_body(ctx.h, ptr(bodyCoord), ptr(bodyUVec),
+                                                            ptr(bodySegLength), ptr(head), ptr(uVec), ptr(rodUVec),
+                                                            cnp, int(cn[0])))
+
+    @staticmethod
+    def publishToBodyView(ctx: Context, head, reach, center, boundingRadius, ownerStore, ownerSlot, publishParams,
+                          counts):
+        pp, ppp = _i(publishParams)
+        cn, cnp = _i(counts)
+        return ctx.check(ctx.lib.sbx_publish_motors(ctx.h, ptr(head), ptr(reach), ptr(center), ptr(boundingRadius),
+                                                    ptr(ownerStore), ptr(ownerSlot), ppp, cnp, int(cn[0]),
+                                                    boundingRadius.numel()))
+
+
+class BindingDetectionSystem:
+    """BindingDetectionSystem.java:143-389: grid-filtered reach tests and nearest-segment binding."""
+
+    @staticmethod
+    def gridReachable(ctx: Context, head, uVec, rodUVec, segEnd1, segEnd2, gridParams, gridDims, gridCellOffsets,
+                      gridCellContents, ownerStore, ownerSlot, motorReachSeg, motorReachCount, kinParams, counts):
+        gp, gpp = _f(gridParams)
+        gd, gdp = _i(gridDims)
+        kp, kpp = _f(kinParams)
+        cn, cnp = _i(counts)
+        nM = int(cn[0])
+        return ctx.check(ctx.lib.sbx_grid_reachable(ctx.h, ptr(head), ptr(uVec), ptr(rodUVec), ptr(segEnd1),
+                                                    ptr(segEnd2), gpp, gdp, ptr(gridCellOffsets),
+                                                    ptr(gridCellContents), ptr(ownerStore), ptr(ownerSlot),
+                                                    ptr(motorReachSeg), ptr(motorReachCount), kpp, cnp,
+                                                    _n3(segEnd1), motorReachSeg.numel() // max(1, nM)))
+
+    @staticmethod
+    def bindNearest(ctx: Context, head, uVec, rodUVec, segEnd1, segEnd2, motorCandSeg, motorCandCount, boundSeg,
+                    bindArc, kinParams, counts):
+        kp, kpp = _f(kinParams)
+        cn, cnp = _i(counts)
+        nM = int(cn[0])
+        return ctx.check(ctx.lib.sbx_bind_nearest(ctx.h, ptr(head), ptr(uVec), ptr(rodUVec), ptr(segEnd1),
+                                                  ptr(segEnd2), ptr(motorCandSeg), ptr(motorCandCount),
+                                                  ptr(boundSeg), ptr(bindArc), kpp, cnp, _n3(segEnd1),
+                                                  motorCandSeg.numel() // max(1, nM)))
+
+
+class NucleotideCycleSystem:
+    """NucleotideCycleSystem.java:33-135: catch-slip unbinding and the ATPase cycle."""
+
+    @staticmethod
+    def catchSlipRelease(ctx: Context, boundSeg, forceDotFil, forceMag, cooldown, stats, capStats, kinParams, counts,
+                         rng: int = RNG_PHILOX, stream: int = 0):
+        kp, kpp = _f(kinParams)
+        cn, cnp = _i(counts)
+        return ctx.check(ctx.lib.sbx_catch_slip_release(ctx.h, ptr(boundSeg), ptr(forceDotFil), ptr(forceMag),
+                                                        ptr(cooldown), ptr(stats), ptr(capStats), kpp, cnp,
+                                                        boundSeg.numel(), rng, stream))
+
+    @staticmethod
+    def cycle(ctx: Context, nucleotideState, boundSeg, forceDotHist, nucParams, counts, rng: int = RNG_PHILOX,
+              stream: int = 0):
+        npar, npp = _f(nucParams)
+        cn, cnp = _i(counts)
+        return ctx.check(ctx.lib.sbx_nucleotide_cycle(ctx.h, ptr(nucleotideState), ptr(boundSeg), ptr(forceDotHist),
+                                                      npp, cnp, nucleotideState.numel(), rng, stream))
+
+
+class CrossBridgeSystem:
+    """CrossBridgeSystem.java:58-320: cross-bridge bond forces, head force, force-dot ring, segment CSR + gather.
+    csrBuild (CrossBridgeSystem.java:189-298) is the CSR inverse: histogram + scan + scatter in one call."""
+
+    @staticmethod
+    def csrBuild(ctx: Context, key, nKeys: int, offsets, items):
+        return ctx.check(ctx.lib.sbx_csr_build(ctx.h, ptr(key), key.numel(), nKeys, ptr(offsets), ptr(items)))
+
+    @staticmethod
+    def bondForces(ctx: Context, motorCoord, motorUVec, motorYVec, motorBRotGam, filCoord, filUVec, filYVec,
+                   filBRotGam, filSegLength, boundSeg, bindArc, nucleotideState, bondData, xbParams):
+        xp, xpp = _f(xbParams)
+        return ctx.check(ctx.lib.sbx_bond_forces(ctx.h, ptr(motorCoord), ptr(motorUVec), ptr(motorYVec),
+                                                 ptr(motorBRotGam), ptr(filCoord), ptr(filUVec), ptr(filYVec),
+                                                 ptr(filBRotGam), ptr(filSegLength), ptr(boundSeg), ptr(bindArc),
+                                                 ptr(nucleotideState), ptr(bondData), xpp, boundSeg.numel(),
+                                                 filSegLength.numel()))
+
+    @staticmethod
+    def applyHeadForce(ctx: Context, bondData, bodyForceSum, bodyTorqueSum):
+        return ctx.check(ctx.lib.sbx_apply_head_force(ctx.h, ptr(bondData), ptr(bodyForceSum), ptr(bodyTorqueSum),
+                                                      _n3(bodyForceSum) // 3))
+
+    @staticmethod
+    def registerForceDot(ctx: Context, bondData, boundSeg, forceDotFil, forceMag, forceDotHist, forceDotPlace):
+        return ctx.check(ctx.lib.sbx_register_force_dot(ctx.h, ptr(bondData), ptr(boundSeg), ptr(forceDotFil),
+                                                        ptr(forceMag), ptr(forceDotHist), ptr(forceDotPlace),
+                                                        boundSeg.numel()))
+
+    @staticmethod
+    def segGather(ctx: Context, segMotorOffsets, segMotorMyo, bondData, filForceSum, filTorqueSum):
+        return ctx.check(ctx.lib.sbx_motor_seg_gather(ctx.h, ptr(segMotorOffsets), ptr(segMotorMyo), ptr(bondData),
+                                                      ptr(filForceSum), ptr(filTorqueSum), _n3(filForceSum)))
+
+
+class MotorJointSystem:
+    """MotorJointSystem.java:61-198: rod-lever and lever-head joints."""
+
+    @staticmethod
+    def joints(ctx: Context, coord, uVec, segLength, bTransGam, bRotGam, forceSum, torqueSum, nucleotideState,
+               jointParams):
+        jp, jpp = _f(jointParams)
+        return ctx.check(ctx.lib.sbx_motor_joints(ctx.h, ptr(coord), ptr(uVec), ptr(segLength), ptr(bTransGam),
+                                                  ptr(bRotGam), ptr(forceSum), ptr(torqueSum), ptr(nucleotideState),
+                                                  jpp, _n3(coord)))
+
+
+class DimerCouplingSystem:
+    """DimerCouplingSystem.java:83-184: rod-end coupling and lever torsion of a dimer."""
+
+    @staticmethod
+    def couple(ctx: Context, coord, uVec, segLength, bTransGam, bRotGam, forceSum, torqueSum, motorA, motorB,
+               parallel, dimerParams, boundSeg):
+        dp, dpp = _f(dimerParams)
+        return ctx.check(ctx.lib.sbx_dimer_couple(ctx.h, ptr(coord), ptr(uVec), ptr(segLength), ptr(bTransGam),
+                                                  ptr(bRotGam), ptr(forceSum), ptr(torqueSum), ptr(motorA),
+                                                  ptr(motorB), ptr(parallel), dpp, ptr(boundSeg), _n3(coord),
+                                                  motorA.numel()))
+
+
+class MiniFilamentSystem:
+    """MiniFilamentSystem.java:65-155: dimer-to-backbone tethers and the backbone gather."""
+
+    @staticmethod
+    def tether(ctx: Context, mCoord, mUVec, mSegLength, mBTransGam, mBRotGam, mForceSum, mTorqueSum, bbCoord, bbUVec,
+               bbInvDragY, headBackboneSlot, motorA, attachAxial, miniData, miniParams):
+        mp, mpp = _f(miniParams)
+        return ctx.check(ctx.lib.sbx_mini_tether(ctx.h, ptr(mCoord), ptr(mUVec), ptr(mSegLength), ptr(mBTransGam),
+                                                 ptr(mBRotGam), ptr(mForceSum), ptr(mTorqueSum), ptr(bbCoord),
+                                                 ptr(bbUVec), ptr(bbInvDragY), ptr(headBackboneSlot), ptr(motorA),
+                                                 ptr(attachAxial), ptr(miniData), mpp, _n3(mCoord), _n3(bbCoord),
+                                                 headBackboneSlot.numel()))
+
+    @staticmethod
+    def backboneGather(ctx: Context, bbDimerOffsets, bbDimerList, miniData, bbForceSum, bbTorqueSum):
+        return ctx.check(ctx.lib.sbx_backbone_gather(ctx.h, ptr(bbDimerOffsets), ptr(bbDimerList), ptr(miniData),
+                                                     ptr(bbForceSum), ptr(bbTorqueSum), _n3(bbForceSum)))

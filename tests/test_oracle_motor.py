@@ -1,0 +1,321 @@
+"""Pins of the oracle's myosin restatement (CPU only).
+
+Each test checks the oracle against something that does not restate its own formula: the textbook
+point-segment distance, brute force over all segments, the closed-form catch-slip and nucleotide
+rates (statistically, over many independent draws), the rest-angle fixed points of the joints,
+and Newton's third law of every two-body force law.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ffi
+from softbox_amd import scenes
+
+KT = scenes.KT_WORKLOAD
+DT = 1.0e-5
+TOL, ALIGN = 0.025, -0.4
+
+
+def kin(kOff=100.0, aC=0.92, aS=0.08, xC=2.5e-9, xS=0.4e-9, refr=0.0, brk=12.0e-12, cap=0.0, block=1.0, dt=DT):
+    return np.array([kOff, aC, aS, xC, xS, KT, dt, TOL, ALIGN, 0.0, refr, brk, cap, block], np.float32)
+
+
+def planar(rows):
+    """[(x, y, z), ...] -> planar SoA float32 (x plane | y plane | z plane)."""
+    a = np.asarray(rows, np.float64)
+    return np.ascontiguousarray(a.T.ravel(), dtype=np.float32)
+
+
+def reach(head, hu, ru, e1, e2, max_cand=8):
+    nM, nSeg = len(head), len(e1)
+    rs, rc = np.full(nM * max_cand, -1, np.int32), np.zeros(nM, np.int32)
+    ffi.call("sbo_brute_reachable", nM, nSeg, planar(head), planar(hu), planar(ru), planar(e1), planar(e2), max_cand,
+             rs, rc, kin())
+    return rs.reshape(nM, max_cand), rc
+
+
+def test_reach_test_geometry():
+    """Reachable iff the head's closest point on the segment lies inside it (0 <= alpha <= 1), within
+    myoColTol, with the head direction within alignTol and the rod pointing along the filament."""
+    e1, e2 = [(0.0, 0.0, 0.0)], [(0.1, 0.0, 0.0)]
+    x = (1.0, 0.0, 0.0)
+    cases = [  # head, head u, rod u, expected reachable
+        ((0.05, 0.020, 0.0), x, x, True),
+        ((0.05, 0.0, -0.0249), x, x, True),
+        ((0.05, 0.030, 0.0), x, x, False),              # too far
+        ((-0.01, 0.01, 0.0), x, x, False),              # alpha < 0
+        ((0.105, 0.01, 0.0), x, x, False),              # alpha > 1
+        ((0.05, 0.01, 0.0), (-1.0, 0.0, 0.0), x, False),  # head anti-aligned (dot -1 < -0.4)
+        ((0.05, 0.01, 0.0), (-0.3, math.sqrt(1 - 0.09), 0.0), x, True),   # dot -0.3 >= -0.4
+        ((0.05, 0.01, 0.0), x, (-1.0, 0.0, 0.0), False),  # rod against the filament
+    ]
+    for head, hu, ru, want in cases:
+        _, rc = reach([head], [hu], [ru], e1, e2)
+        assert bool(rc[0]) == want, (head, hu, ru)
+
+
+def test_bind_arc_and_nearest():
+    """bindArc is the head's projection onto the segment axis from end1; the nearest segment wins
+    and an exact tie keeps the first candidate in list order."""
+    segs_e1 = [(0.0, 0.01, 0.0), (0.0, -0.02, 0.0), (0.0, -0.01, 0.0)]
+    segs_e2 = [(0.1, 0.01, 0.0), (0.1, -0.02, 0.0), (0.1, -0.01, 0.0)]
+    head = [(0.037, 0.0, 0.0)]
+    x = [(1.0, 0.0, 0.0)]
+    for order, want in (([1, 0, 2], 0), ([1, 2, 0], 2), ([2, 0], 2), ([1], 1)):
+        mc = 8
+        rs = np.full(mc, -1, np.int32)
+        rs[: len(order)] = order
+        rc = np.array([len(order)], np.int32)
+        bs = np.array([-1], np.int32)
+        arc = np.zeros(1, np.float32)
+        ffi.call("sbo_bind_nearest", 1, 3, planar(head), planar(x), planar(x), planar(segs_e1), planar(segs_e2), rs,
+                 rc, mc, bs, arc, kin())
+        assert bs[0] == want, order
+        assert abs(arc[0] - 0.037) < 1e-7
+    # a motor that is not FREE_BINDABLE never binds
+    for st in (-2, 4):
+        bs = np.array([st], np.int32)
+        ffi.call("sbo_bind_nearest", 1, 3, planar(head), planar(x), planar(x), planar(segs_e1), planar(segs_e2),
+                 np.array([0] + [-1] * 7, np.int32), np.array([1], np.int32), 8, bs, np.zeros(1, np.float32), kin())
+        assert bs[0] == st
+
+
+def test_grid_reachable_equals_brute_force():
+    """The grid walk finds exactly the brute-force reachable set for every motor (order may differ)."""
+    sc = scenes.contractile(n_fil=60, segs=10, box=2.5, n_mini=24, dimers_each_end=4, n_links=150, seed=11,
+                            mini_near_fil=True)
+    orc = ffi.OracleContractile(sc)
+    hits = 0
+    for t in range(12):
+        # binding reads the step-start segment ends; the published heads stay as published
+        e1, e2 = orc.fs.end1.copy(), orc.fs.end2.copy()
+        orc.step(t, 1)
+        mot, nM, n = orc.mot, orc.mot.nM, orc.fs.n
+        mc = int(orc.p["maxCand"])
+        gs, gc = orc.mreach["reachSeg"].reshape(nM, mc), orc.mreach["reachCount"]
+        bs, bc = np.full(nM * mc, -1, np.int32), np.zeros(nM, np.int32)
+        ffi.call("sbo_brute_reachable", nM, n, mot.head, mot.uVec, mot.rodUVec, e1, e2, mc, bs, bc, mot.kinParams)
+        assert np.array_equal(gc, bc), f"step {t}: reach counts differ"
+        bs = bs.reshape(nM, mc)
+        for m in np.flatnonzero(gc):
+            assert sorted(gs[m, : gc[m]]) == sorted(bs[m, : bc[m]])
+            hits += 1
+    assert hits > 0
+
+
+def _release_fraction(F, n=200000, kOff=2.0e3, step=3, seed=99, rng=1):
+    bs = np.zeros(n, np.int32)
+    fd = np.full(n, F, np.float32)
+    fm = np.full(n, abs(F), np.float32)
+    cool, stats, cap = np.zeros(n, np.int32), np.zeros(2 * n, np.int32), np.zeros(n, np.int32)
+    ffi.call("sbo_catch_slip_release", n, bs, fd, fm, cool, stats, cap, kin(kOff=kOff), step, seed, rng, 0)
+    assert stats[0::2].sum() == n                     # every bound motor counts a bound step
+    return float((bs < 0).mean()), stats
+
+
+@pytest.mark.parametrize("rng", [0, 1])
+def test_catch_slip_rate_matches_closed_form(rng):
+    """P(release in one step) = kOff (aC exp(-F xC/kT) + aS exp(F xS/kT)) dt: checked at F = 0, at the
+    catch-bond minimum F* = kT/(xC+xS) ln(aC xC / (aS xS)), at 2F* and at -F* (5 sigma)."""
+    kOff, aC, aS, xC, xS = 2.0e3, 0.92, 0.08, 2.5e-9, 0.4e-9
+    fstar = KT / (xC + xS) * math.log(aC * xC / (aS * xS))
+    n = 200000
+    for F in (0.0, fstar, 2.0 * fstar, -fstar):
+        p = kOff * (aC * math.exp(-F * xC / KT) + aS * math.exp(F * xS / KT)) * DT
+        got, stats = _release_fraction(F, n=n, kOff=kOff, rng=rng)
+        sigma = math.sqrt(p * (1 - p) / n)
+        assert abs(got - p) < 5 * sigma + 1e-6, (F, got, p)
+        assert stats[1::2].sum() == round(got * n)
+    # the catch bond: release is slowest near F*
+    assert _release_fraction(fstar, rng=rng)[0] < _release_fraction(0.0, rng=rng)[0]
+    assert _release_fraction(fstar, rng=rng)[0] < _release_fraction(2.5 * fstar, rng=rng)[0]
+
+
+def test_break_cap_and_refractory():
+    n = 1000
+    bs = np.zeros(n, np.int32)
+    fm = np.full(n, 13.0e-12, np.float32)                # above the 12 pN cap
+    cool, stats, cap = np.zeros(n, np.int32), np.zeros(2 * n, np.int32), np.zeros(n, np.int32)
+    ffi.call("sbo_catch_slip_release", n, bs, np.zeros(n, np.float32), fm, cool, stats, cap,
+             kin(cap=1.0, refr=3.0), 0, 1, 1, 0)
+    assert (bs == -2).all() and (cool == 3).all() and (cap == 1).all() and stats[1::2].sum() == 0
+    # cooldown counts down and frees the motor after refractorySteps steps
+    for k in range(3):
+        ffi.call("sbo_catch_slip_release", n, bs, np.zeros(n, np.float32), fm, cool, stats, cap,
+                 kin(cap=1.0, refr=3.0), 1 + k, 1, 1, 0)
+        assert ((bs == -2) if k < 2 else (bs == -1)).all()
+    # blockProb 0: never refractory
+    bs = np.zeros(n, np.int32)
+    ffi.call("sbo_catch_slip_release", n, bs, np.zeros(n, np.float32), fm, cool, stats, cap,
+             kin(cap=1.0, refr=3.0, block=0.0), 0, 1, 1, 0)
+    assert (bs == -1).all()
+
+
+def test_nucleotide_rates_closed_form():
+    """P(transition) = rate * dt per state, bound and free rates, ADP release gated on the mean
+    force-dot history being <= 0."""
+    n = 200000
+    npar = np.array([DT, 2.0e4, 5.0e3, 1.0e3, 1.0e4, 0.0, 3.0e4, 1.0e3], np.float32)
+    cases = [  # state, bound, hist value, rate
+        (0, False, 0.0, 2.0e4), (0, True, 0.0, 2.0e4), (1, True, 0.0, 5.0e3), (1, False, 0.0, 1.0e3),
+        (2, True, 0.0, 1.0e4), (2, False, 0.0, 0.0), (3, True, -1e-12, 3.0e4), (3, False, 0.0, 1.0e3),
+        (3, True, +1e-12, 0.0)]
+    for st, bound, hv, rate in cases:
+        nuc = np.full(n, st, np.int32)
+        bsg = np.full(n, 5 if bound else -1, np.int32)
+        hist = np.full(10 * n, hv, np.float32)
+        ffi.call("sbo_nucleotide_cycle", n, nuc, bsg, hist, npar, 7, 31, 1, 0)
+        p = rate * DT
+        got = float((nuc != st).mean())
+        sigma = math.sqrt(max(p * (1 - p), 1e-12) / n)
+        assert abs(got - p) < 5 * sigma + 1e-6, (st, bound, hv, got, p)
+        assert set(np.unique(nuc)) <= {st, (st + 1) % 4}
+
+
+def _one_motor(rod_u, lever_u, head_u, joint_gap=0.0):
+    """Rod centred at the origin; lever and head chained end to end (optionally with a gap)."""
+    rl, ll, hl = 0.080, 0.008, 0.020
+    ru, lu, hu = (np.asarray(v, np.float64) / np.linalg.norm(v) for v in (rod_u, lever_u, head_u))
+    rc = np.zeros(3)
+    lc = rc + 0.5 * rl * ru + 0.5 * ll * lu + joint_gap * np.array([0.0, 0.0, 1.0])
+    hc = lc + 0.5 * ll * lu + 0.5 * hl * hu
+    coord = planar([rc, lc, hc])
+    uVec = planar([ru, lu, hu])
+    segLength = np.array([rl, ll, hl], np.float32)
+    sc = scenes.contractile(n_fil=2, segs=2, box=2.0, n_mini=1, dimers_each_end=1, n_links=0, seed=0)
+    b = sc.mot.body
+    tg = np.ascontiguousarray(np.stack([b.plane("bTransGam")[:, k] for k in range(3)], 1).ravel(), np.float32)
+    rg = np.ascontiguousarray(np.stack([b.plane("bRotGam")[:, k] for k in range(3)], 1).ravel(), np.float32)
+    return coord, uVec, segLength, tg, rg, sc.mot.jointParams
+
+
+def _joints(state, nuc):
+    coord, uVec, segLength, tg, rg, jp = state
+    F, T = np.zeros(9, np.float32), np.zeros(9, np.float32)
+    ffi.call("sbo_motor_joints", 3, coord, uVec, segLength, tg, rg, F, T, np.array([nuc], np.int32), jp)
+    return F, T
+
+
+def test_joint_rest_angles_are_fixed_points():
+    """Connected joints at the rest angles (rod-lever 96 deg; lever-head 60 deg, or 0 deg in ADP.Pi)
+    carry no force and no torque; moving away from them does."""
+    def at(deg_rl, deg_lh):
+        a = math.radians(deg_rl)
+        lever = (math.cos(a), math.sin(a), 0.0)
+        b = a + math.radians(deg_lh)
+        head = (math.cos(b), math.sin(b), 0.0)
+        return lever, head
+
+    lever, head = at(96.0, 60.0)
+    F, T = _joints(_one_motor((1, 0, 0), lever, head), nuc=1)
+    Fb, Tb = _joints(_one_motor((1, 0, 0), *at(80.0, 40.0)), nuc=1)
+    assert np.abs(Tb).max() > 0 and np.abs(T).max() < 1e-4 * np.abs(Tb).max()
+    Fg, _ = _joints(_one_motor((1, 0, 0), lever, head, joint_gap=0.002), nuc=1)
+    assert np.abs(Fg).max() > 0 and np.abs(F).max() < 1e-4 * np.abs(Fg).max()
+    lever, head = at(96.0, 0.0)
+    F2, T2 = _joints(_one_motor((1, 0, 0), lever, head), nuc=2)    # ADP.Pi: lever-head rest 0 deg
+    assert np.abs(T2).max() < 1e-4 * np.abs(Tb).max() and np.abs(F2).max() < 1e-4 * np.abs(Fg).max()
+    # the torsions are restoring: rod-lever at 80 deg (< 96 rest; its torsion switched on here) turns
+    # the lever away from the rod; lever-head at 40 deg (< 60 rest) turns the head away from the lever
+    lever, head = at(80.0, 40.0)
+    st = list(_one_motor((1, 0, 0), lever, head))
+    st[5] = st[5].copy()
+    st[5][7] = 0.4
+    _, Ts = _joints(tuple(st), nuc=1)
+    Tp = Ts.reshape(3, 3)
+    axis = np.cross((1.0, 0.0, 0.0), lever)
+    assert Tp[:, 0] @ axis < 0
+    axis2 = np.cross(lever, head)
+    assert Tp[:, 2] @ axis2 > 0
+    assert (Tp[:, 1] @ axis) > 0 or (Tp[:, 1] @ axis2) < 0
+    # internal joint forces sum to zero over the three sub-bodies
+    assert abs(float(Fg.reshape(3, 3).sum(1).max())) <= 1e-6 * np.abs(Fg).max()
+
+
+def test_bond_force_hooke_and_third_law():
+    """Cross-bridge: F = myoSpring * (attach point - head tip); the segment gets -F (no bias); the
+    angular-spring torques cancel between head and segment."""
+    sc = scenes.contractile(n_fil=4, segs=3, box=2.0, n_mini=1, dimers_each_end=1, n_links=0, seed=2)
+    fs, mot = sc.fs, sc.mot
+    nM, n = mot.nM, fs.n
+    b = mot.body
+    bs = np.full(nM, -1, np.int32)
+    bs[0] = 1
+    # head y-vector parallel to the segment's: the twist spring is idle, only the bend spring acts
+    for a in range(3):
+        b.yVec[a * 3 * nM + 2] = fs.yVec[a * fs.n + 1]
+    arc = np.full(nM, 0.03, np.float32)
+    nuc = np.full(nM, 1, np.int32)
+    bond = np.zeros(13 * nM, np.float32)
+    ffi.call("sbo_bond_forces", nM, n, b.coord, b.uVec, b.yVec, b.bRotGam, fs.coord, fs.uVec, fs.yVec, fs.bRotGam,
+             fs.segLength, bs, arc, nuc, bond, mot.xbParams)
+    d = bond[:13].astype(np.float64)
+    nB = 3 * nM
+    h = 2
+    hc = np.array([b.coord[h], b.coord[nB + h], b.coord[2 * nB + h]], np.float64)
+    hu = np.array([b.uVec[h], b.uVec[nB + h], b.uVec[2 * nB + h]], np.float64)
+    tip = hc + 0.5 * b.segLength[h] * hu
+    s = 1
+    sc_ = np.array([fs.coord[s], fs.coord[n + s], fs.coord[2 * n + s]], np.float64)
+    su = np.array([fs.uVec[s], fs.uVec[n + s], fs.uVec[2 * n + s]], np.float64)
+    ap = sc_ + (0.03 - 0.5 * fs.segLength[s]) * su
+    F = float(mot.xbParams[0]) * (ap - tip)
+    assert np.allclose(d[0:3], F, rtol=1e-5, atol=0)
+    assert np.array_equal(bond[6:9], -bond[0:3])
+    TH = np.cross((tip - hc) * 1e-6, F)
+    TS = np.cross((ap - sc_) * 1e-6, -F)
+    assert np.allclose(d[3:6] + d[9:12], TH + TS, rtol=1e-4, atol=1e-6 * np.abs(d[3:6]).max())
+    assert abs(d[12] - F @ su) <= 1e-5 * abs(F @ su)
+    # the bend spring is restoring about su x hu toward the 120 deg rest angle
+    ang = math.degrees(math.acos(float(np.clip(su @ hu, -1, 1))))
+    axis = np.cross(su, hu)
+    bend_on_head = d[3:6] - TH
+    assert abs(ang - 120.0) > 1.0
+    assert np.sign(bend_on_head @ axis) == -np.sign(ang - 120.0)
+    assert np.all(bond[13:] == 0)                      # unbound motor: zero payload
+
+
+def test_dimer_and_tether_third_law():
+    sc = scenes.contractile(n_fil=4, segs=3, box=2.0, n_mini=2, dimers_each_end=2, n_links=0, seed=4)
+    mot, dim, mini = sc.mot, sc.dim, sc.mini
+    nM, nB = mot.nM, 3 * mot.nM
+    b = mot.body
+    rng = np.random.default_rng(0)
+    coord = (b.coord + rng.standard_normal(b.coord.size).astype(np.float32) * 0.002).astype(np.float32)
+    F, T = np.zeros(3 * nB, np.float32), np.zeros(3 * nB, np.float32)
+    ffi.call("sbo_dimer_couple", nB, dim.nD, coord, b.uVec, b.segLength, b.bTransGam, b.bRotGam, F, T, dim.motorA,
+             dim.motorB, dim.parallel, dim.dimerParams, np.full(nM, -1, np.int32))
+    Fp = F.reshape(3, nB)
+    for d in range(dim.nD):
+        a, bb = 3 * dim.motorA[d], 3 * dim.motorB[d]
+        assert np.abs(Fp[:, a]).max() > 0
+        assert np.allclose(Fp[:, a], -Fp[:, bb], rtol=1e-6, atol=0)
+    F, T = np.zeros(3 * nB, np.float32), np.zeros(3 * nB, np.float32)
+    md = np.zeros_like(mini.miniData)
+    bbc = (mini.backbone.coord + 0.003).astype(np.float32)
+    ffi.call("sbo_mini_tether", nB, mini.nBb, mini.nMD, coord, b.uVec, b.segLength, b.bTransGam, b.bRotGam, F, T,
+             bbc, mini.backbone.uVec, mini.bbInvDragY, mini.headBackboneSlot, mini.motorA, mini.attachAxial, md,
+             mini.miniParams)
+    Fp = F.reshape(3, nB)
+    for d in range(mini.nMD):
+        rod = 3 * mini.motorA[d]
+        assert np.abs(md[6 * d: 6 * d + 3]).max() > 0
+        assert np.array_equal(md[6 * d: 6 * d + 3], -Fp[:, rod])
+
+
+def test_force_dot_ring():
+    """registerForceDot keeps the last 10 force-dot values in a ring and clears it on unbinding."""
+    nM = 1
+    bond = np.zeros(13, np.float32)
+    fdf, fm = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    hist, place = np.zeros(10, np.float32), np.zeros(1, np.int32)
+    for k in range(13):
+        bond[12] = k + 1
+        bond[0:3] = (3.0, 4.0, 0.0)
+        ffi.call("sbo_register_force_dot", nM, bond, np.array([2], np.int32), fdf, fm, hist, place)
+    assert place[0] == 13 % 10 and fdf[0] == 13 and fm[0] == 5.0
+    assert sorted(hist.tolist()) == list(range(4, 14))
+    ffi.call("sbo_register_force_dot", nM, bond, np.array([-1], np.int32), fdf, fm, hist, place)
+    assert place[0] == 0 and fdf[0] == 0 and fm[0] == 0 and not hist.any()
