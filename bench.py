@@ -30,28 +30,31 @@ METRIC = "rod-steps/sec + HBM GB/s % peak at 1 GPU; learned-skin vs per-step reb
 
 
 def kernel_bytes(sc, name: str) -> float | None:
-    """Algorithmic (compulsory) HBM bytes of one launch of a network kernel on this workload."""
+    """Algorithmic (compulsory) HBM bytes of one launch of a network kernel on this workload:
+    every input read once and every output written once, at its natural width."""
     n = sc.fs.n
     L = sc.xl.nLinks if sc.xl is not None else 0
     cells = int(sc.p["gridDims"][3])
     if name == "k_rod":
-        # read pose 36 + segLength 4 + gamma 24 + Brownian scales 8 + topology 16 + CSR offsets 8,
-        # write pose 36 + zVec/end1/end2 36  -> 168 B/rod; per link end: index 4 + state 4 + payload 24
-        return 168.0 * n + 32.0 * 2 * L
+        # read pose 36 + segLength 4 + gamma 24 + Brownian scales 8 + topology 16 + CSR offsets 16,
+        # write pose 36 + zVec/end1/end2 36 + active-link count 4 + next-step cell 4 -> 184 B/rod;
+        # per link end: index 4 + state 4 + payload 24
+        return 184.0 * n + 32.0 * 2 * L
     if name == "k_net_link":
         # ids 8 + locs 8 + state 4 + orient 4 + torque ring 20+4 + endpoints A/B (end1, u, coord, gamma) 2x40
-        # + counts 8; write payload 48 + ring slot 4 + place 4
-        return 192.0 * L
+        # + both segments' CSR bounds 2x16; write payload 48 + ring slot 4 + place 4
+        return 224.0 * L
     if name == "k_net_unbind":
         return (8 + 8 + 4 + 40 + 4 + 2 * 24 + 40 + 4) * float(L)
-    if name == "k_net_count":
-        return 16.0 * n + 8.0 * 2 * L
     if name == "k_body_cell":
         return 16.0 * n
     if name == "grid_sort":
-        return 4.0 * n * 6 + 4.0 * cells * 6
+        # per body: count (key 4, slot 4, counts/tile atomics 8), scatter (key, offset, slot 12, tmp 4,
+        # counts 4), place (key 4, offsets 8, segment 4, list 4); per cell: counts 4 + offsets 4
+        return 56.0 * n + 8.0 * cells
     if name == "k_broad_phase":
-        return (4 + 16 + 4 + 4) * float(n) + 8.0 * cells
+        # per body: contents 4 + cell 4 + centre 12 + radius 4 + count 4; per cell: offsets 4
+        return 28.0 * n + 4.0 * cells
     return None
 
 
@@ -83,6 +86,9 @@ def main() -> int:
     ap.add_argument("--n-fil", type=int, default=20000)
     ap.add_argument("--n-links", type=int, default=50000)
     ap.add_argument("--box", type=float, default=20.0)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="before the warm-up, one device-to-device copy of this many MiB (a plain copy kernel "
+                         "of known size that calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -101,6 +107,15 @@ def main() -> int:
     print(f"[bench] scene: {n} rods, {sc.xl.nLinks} links, grid {sc.p['gridDims'][:3]} "
           f"({time.perf_counter() - t_build:.1f} s)", file=sys.stderr, flush=True)
     ctx = Context(0)
+    if args.calib_copy_mb > 0:
+        nb = args.calib_copy_mb << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        ctx.check(ctx.lib.sbx_copy(ctx.h, dst.data_ptr(), src.data_ptr(), nb))
+        ctx.sync()
+        del src, dst
+        print(f"[bench] calibration copy of {args.calib_copy_mb} MiB done", file=sys.stderr, flush=True)
     net = Network(ctx, sc, keep_forces=False)
     if args.graph:
         net.use_graph(True)

@@ -151,7 +151,7 @@ _SIGS = {
     "sbx_register_force_dot": [vp] + [vp] * 6 + [i32],
     "sbx_motor_seg_gather": [vp] + [vp] * 5 + [i32],
     "sbx_motor_joints": [vp] + [vp] * 8 + [f32p, i32],
-    "sbx_dimer_couple": [vp] + [vp] * 11 + [f32p, vp, i32, i32],
+    "sbx_dimer_couple": [vp] + [vp] * 10 + [f32p, vp, i32, i32],
     "sbx_mini_tether": [vp] + [vp] * 14 + [f32p, i32, i32, i32],
     "sbx_backbone_gather": [vp] + [vp] * 5 + [i32],
     "sbx_net_create": [vp, C.POINTER(SbxFilaments), C.POINTER(SbxCrosslinks), C.POINTER(SbxNetParams), C.POINTER(vp)],

@@ -3,10 +3,11 @@
 // The counting sort is the one primitive behind SpatialGrid's cell CSR (SpatialGrid.java:99-331)
 // and CrossBridgeSystem's CSR inverse (CrossBridgeSystem.java:190-298).  The reference builds it
 // with per-chunk private histogram rows so that the within-key order is ascending item index.  On
-// gfx950 the same unique result is produced by: atomic key histogram -> wave64 shuffle scan ->
-// atomic (unordered) scatter -> per-item rank placement (rank = number of same-key items with a
-// smaller index, read back from the unordered segment).  The totals and the final order do not
-// depend on atomic ordering, so the CSR is bit-identical to the serial reference build.
+// gfx950 the same unique result is produced by: atomic key histogram (which also hands each item a
+// provisional slot) -> one-pass tiled scan -> scatter to the provisional slots -> per-item rank
+// placement (rank = number of same-key items with a smaller index, read back from the unordered
+// segment).  The totals and the final order do not depend on atomic ordering, so the CSR is
+// bit-identical to the serial reference build.
 #include "sbx_internal.h"
 
 #include <new>
@@ -49,9 +50,9 @@ int fill_i32(sbx_ctx *c, int *dst, int value, long long count) {
 }
 
 // ------------------------------------------------------------------ scan
-// Tile = 256 threads x 4 items.  Wave64 inclusive scan by __shfl_up, then wave totals in LDS.
+// Tile = 256 threads x 8 items.  Wave64 inclusive scan by __shfl_up, then wave totals in LDS.
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
 __device__ __forceinline__ int wave_inclusive_scan(int v) {
@@ -61,6 +62,12 @@ __device__ __forceinline__ int wave_inclusive_scan(int v) {
         int o = __shfl_up(v, d, 64);
         if (lane >= d) v += o;
     }
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
 }
 
@@ -83,155 +90,147 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *total) {
     return waveOff + inc - v;
 }
 
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const int *__restrict__ in, int n,
-                                                              int *__restrict__ blockSums) {
-    long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
-    int s = 0;
-#pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        long long idx = base + k;
-        if (idx < n) s += in[idx];
-    }
-    int total;
-    block_exclusive_scan(s, &total);
-    if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
-}
-
-// single block: exclusive scan of blockSums[nb] in place, blockSums[nb] = grand total
-__global__ void __launch_bounds__(1024) k_scan_blocksums(int *__restrict__ blockSums, int nb) {
-    __shared__ int part[1024];
-    int per = (nb + 1023) / 1024;
-    int start = threadIdx.x * per;
-    int s = 0;
-    for (int k = 0; k < per; k++) {
-        int idx = start + k;
-        if (idx < nb) s += blockSums[idx];
-    }
-    part[threadIdx.x] = s;
+// Exclusive scan of counts[n] into out[n+1] in ONE pass.  tileSums[t] (sum of tile t of counts)
+// was accumulated by the producer of counts, so each tile's prefix is the sum of the tile sums
+// before it -- a block-wide reduction over at most n/2048 L2-resident ints -- and no tile waits
+// on another.  out[n] = grand total, clamped to clampTotal when >= 0.
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int *__restrict__ in, int n,
+                                                             const int *__restrict__ tileSums, int nTiles,
+                                                             int *__restrict__ out, int clampTotal) {
+    __shared__ int red[SCAN_THREADS / 64];
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // prefix of this tile (and, for the last tile, the grand total)
+    int lim = (tile == nTiles - 1) ? nTiles : tile;
+    int part = 0;
+    for (int t = threadIdx.x; t < lim; t += SCAN_THREADS) part += (t < tile ? tileSums[t] : 0);
+    int lastPart = 0;
+    if (tile == nTiles - 1)
+        for (int t = threadIdx.x; t < nTiles; t += SCAN_THREADS) lastPart += tileSums[t];
+    part = wave_sum(part);
+    if (lane == 0) red[wave] = part;
     __syncthreads();
-    // Hillis-Steele over 1024 partials (runs once per sort; nb is small)
-    for (int d = 1; d < 1024; d <<= 1) {
-        int v = (threadIdx.x >= (unsigned)d) ? part[threadIdx.x - d] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    int acc = part[threadIdx.x] - s;  // exclusive prefix of this thread's chunk
-    for (int k = 0; k < per; k++) {
-        int idx = start + k;
-        if (idx < nb) {
-            int v = blockSums[idx];
-            blockSums[idx] = acc;
-            acc += v;
-        }
-    }
-    if (threadIdx.x == 1023) blockSums[nb] = part[1023];
-}
-
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const int *__restrict__ in, int n,
-                                                             const int *__restrict__ blockSums,
-                                                             int *__restrict__ out, int nb, int clampTotal) {
-    long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
+    int tilePrefix = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_THREADS / 64; w++) tilePrefix += red[w];
+    __syncthreads();
+    long long base = (long long)tile * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS];
-    int s = 0;
+    int sum = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
         long long idx = base + k;
         v[k] = (idx < n) ? in[idx] : 0;
-        s += v[k];
+        sum += v[k];
     }
     int total;
-    int pre = block_exclusive_scan(s, &total) + blockSums[blockIdx.x];
+    int pre = block_exclusive_scan(sum, &total) + tilePrefix;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
         long long idx = base + k;
         if (idx < n) out[idx] = pre;
         pre += v[k];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        int grand = blockSums[nb];
-        out[n] = (clampTotal >= 0 && grand > clampTotal) ? clampTotal : grand;
+    if (tile == nTiles - 1) {
+        lastPart = wave_sum(lastPart);
+        if (lane == 0) red[wave] = lastPart;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int grand = 0;
+            for (int w = 0; w < SCAN_THREADS / 64; w++) grand += red[w];
+            out[n] = (clampTotal >= 0 && grand > clampTotal) ? clampTotal : grand;
+        }
     }
 }
 
 int sort_scratch_reserve(sbx_ctx *c, SortScratch &s, long long nKeys, long long nItems) {
-    long long nb = (nKeys + SCAN_TILE - 1) / SCAN_TILE + 1;
-    if (nKeys > s.keyCap) {
+    long long nt = (nKeys + SCAN_TILE - 1) / SCAN_TILE + 1;
+    if (nKeys > s.keyCap || nt > s.tileCap) {
         if (s.counts) hipFree(s.counts);
-        if (s.cursor) hipFree(s.cursor);
-        s.counts = s.cursor = nullptr;
+        if (s.tileSums) hipFree(s.tileSums);
+        s.counts = s.tileSums = nullptr;
+        s.keyCap = s.tileCap = 0;
         int r = check_hip(c, hipMalloc(&s.counts, sizeof(int) * (size_t)(nKeys + 1)), "hipMalloc counts");
         if (r) return r;
-        r = check_hip(c, hipMalloc(&s.cursor, sizeof(int) * (size_t)(nKeys + 1)), "hipMalloc cursor");
+        r = check_hip(c, hipMalloc(&s.tileSums, sizeof(int) * (size_t)(nt + 1)), "hipMalloc tileSums");
         if (r) return r;
         s.keyCap = nKeys;
+        s.tileCap = nt;
+        s.clean = false;  // zeroed by the next sort
     }
     if (nItems > s.itemCap) {
         if (s.tmp) hipFree(s.tmp);
-        s.tmp = nullptr;
+        if (s.slot) hipFree(s.slot);
+        s.tmp = s.slot = nullptr;
+        s.itemCap = 0;
         int r = check_hip(c, hipMalloc(&s.tmp, sizeof(int) * (size_t)(nItems + 1)), "hipMalloc tmp");
+        if (r) return r;
+        r = check_hip(c, hipMalloc(&s.slot, sizeof(int) * (size_t)(nItems + 1)), "hipMalloc slot");
         if (r) return r;
         s.itemCap = nItems;
     }
-    if (nb > s.blockCap) {
-        if (s.blockSums) hipFree(s.blockSums);
-        s.blockSums = nullptr;
-        int r = check_hip(c, hipMalloc(&s.blockSums, sizeof(int) * (size_t)(nb + 1)), "hipMalloc blockSums");
+    if (!s.clean) {
+        int r = fill_i32(c, s.counts, 0, s.keyCap + 1);
+        if (!r) r = fill_i32(c, s.tileSums, 0, s.tileCap + 1);
         if (r) return r;
-        s.blockCap = nb;
+        s.clean = true;
     }
     return SBX_OK;
 }
 
 void sort_scratch_free(SortScratch &s) {
     if (s.counts) hipFree(s.counts);
-    if (s.cursor) hipFree(s.cursor);
+    if (s.tileSums) hipFree(s.tileSums);
     if (s.tmp) hipFree(s.tmp);
-    if (s.blockSums) hipFree(s.blockSums);
+    if (s.slot) hipFree(s.slot);
     s = SortScratch();
 }
 
-int exclusive_scan(sbx_ctx *c, SortScratch &s, const int *in, int *out, int n, int clampTotal) {
-    int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (nb < 1) nb = 1;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, c->stream, in, n, s.blockSums);
-    hipLaunchKernelGGL(k_scan_blocksums, dim3(1), dim3(1024), 0, c->stream, s.blockSums, nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, c->stream, in, n, s.blockSums, out, nb,
-                       clampTotal);
-    return check_launch(c, "exclusive_scan");
-}
-
 // ------------------------------------------------------------------ counting sort
-__global__ void __launch_bounds__(256) k_key_count(const int *__restrict__ key, int nItems, int *__restrict__ counts) {
+// Scratch invariant: counts[] and tileSums[] are all zero between sorts (zeroed once at
+// allocation; each sort re-zeroes exactly the entries it dirtied), so a sort issues no fill
+// over the key range -- which for a sparse grid is ~10x larger than the item count.
+//
+// 1. k_key_count:   slot[m] = atomicAdd(counts[key]) (a provisional, unordered position) and the
+//                   key's tile sum;
+// 2. k_scan_tiles:  offsets = exclusive scan of counts (tile prefixes from the tile sums);
+// 3. k_key_scatter: tmp[offsets[key] + slot[m]] = m; counts[key] = 0; tileSums zeroed;
+// 4. k_key_place:   rank of m among its key's segment (items with a smaller index) -> list.
+// The result is independent of the atomic order: ascending item index within each key.
+__global__ void __launch_bounds__(256) k_key_count(const int *__restrict__ key, int nItems, int nKeys,
+                                                   int *__restrict__ counts, int *__restrict__ tileSums,
+                                                   int *__restrict__ slot) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nItems) return;
     int k = key[m];
-    if (k >= 0) atomicAdd(&counts[k], 1);
+    if (k < 0 || k >= nKeys) return;
+    slot[m] = atomicAdd(&counts[k], 1);
+    atomicAdd(&tileSums[k / SCAN_TILE], 1);
 }
 
-__global__ void __launch_bounds__(256) k_key_scatter(const int *__restrict__ key, int nItems,
-                                                     const int *__restrict__ offsets, int *__restrict__ cursor,
+__global__ void __launch_bounds__(256) k_key_scatter(const int *__restrict__ key, int nItems, int nKeys,
+                                                     const int *__restrict__ offsets, const int *__restrict__ slot,
+                                                     int *__restrict__ counts, int *__restrict__ tileSums, int nTiles,
                                                      int *__restrict__ tmp, int contentsCap) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < nTiles) tileSums[m] = 0;
     if (m >= nItems) return;
     int k = key[m];
-    if (k < 0) return;
-    int pos = offsets[k] + atomicAdd(&cursor[k], 1);
+    if (k < 0 || k >= nKeys) return;
+    counts[k] = 0;
+    int pos = offsets[k] + slot[m];
     if (pos < contentsCap) tmp[pos] = m;
 }
 
-// rank placement: the key's segment in tmp holds exactly its items (any order); count smaller ids.
-__global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, int nItems,
-                                                   const int *__restrict__ offsets, const int *__restrict__ counts,
-                                                   const int *__restrict__ tmp, int *__restrict__ list,
-                                                   int contentsCap) {
+__global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, int nItems, int nKeys,
+                                                   const int *__restrict__ offsets, const int *__restrict__ tmp,
+                                                   int *__restrict__ list, int contentsCap) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nItems) return;
     int k = key[m];
-    if (k < 0) return;
+    if (k < 0 || k >= nKeys) return;
     int start = offsets[k];
-    int cnt = counts[k];
-    int end = start + cnt;
+    int end = offsets[k + 1];
     if (end > contentsCap) end = contentsCap;
     int rank = 0;
     for (int p = start; p < end; p++) rank += (tmp[p] < m) ? 1 : 0;
@@ -243,25 +242,23 @@ int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nK
                   int *list) {
     int r = sort_scratch_reserve(c, s, nKeys, nItems);
     if (r) return r;
-    r = fill_i32(c, s.counts, 0, nKeys);
-    if (r) return r;
-    r = fill_i32(c, s.cursor, 0, nKeys);
-    if (r) return r;
+    int nt = (nKeys + SCAN_TILE - 1) / SCAN_TILE;
+    if (nt < 1) nt = 1;
     if (nItems > 0) {
-        hipLaunchKernelGGL(k_key_count, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems,
-                           s.counts);
-        r = check_launch(c, "k_key_count");
-        if (r) return r;
+        hipLaunchKernelGGL(k_key_count, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems, nKeys,
+                           s.counts, s.tileSums, s.slot);
     }
-    r = exclusive_scan(c, s, s.counts, offsets, nKeys, contentsCap);
-    if (r) return r;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, c->stream, s.counts, nKeys, s.tileSums, nt,
+                       offsets, contentsCap);
+    int span = nItems > nt ? nItems : nt;
+    hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(span, 256)), dim3(256), 0, c->stream, key, nItems, nKeys, offsets,
+                       s.slot, s.counts, s.tileSums, nt, s.tmp, contentsCap);
     if (nItems > 0) {
-        hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems,
-                           offsets, s.cursor, s.tmp, contentsCap);
-        hipLaunchKernelGGL(k_key_place, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems,
-                           offsets, s.counts, s.tmp, list, contentsCap);
-        r = check_launch(c, "counting_sort scatter/place");
+        hipLaunchKernelGGL(k_key_place, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems, nKeys,
+                           offsets, s.tmp, list, contentsCap);
     }
+    r = check_launch(c, "counting_sort");
+    if (r) s.clean = false;
     return r;
 }
 

@@ -48,61 +48,88 @@ __device__ __forceinline__ int cell_of(float x, float y, float z, const GridGeom
 
 __device__ __forceinline__ float min_image(float d, float L) { return d - L * rintf(d / L); }
 
+// Candidate test of body i against body j (SpatialGrid.java:385-399): inclusive sphere predicate
+// on (minimum-image) centre distance; appends j to i's slice while it has room.
+__device__ __forceinline__ void bp_visit(int i, int j, float icx, float icy, float icz, float ri,
+                                         const float *__restrict__ center, const float *__restrict__ boundingRadius,
+                                         const GridGeom &g, int cap, int *__restrict__ slice, int maxCand, int &cnt) {
+    if (j <= i) return;
+    float dx = icx - center[j];
+    float dy = icy - center[cap + j];
+    float dz = icz - center[2 * cap + j];
+    if (g.periodic) {
+        dx = min_image(dx, g.Lx);
+        dy = min_image(dy, g.Ly);
+        dz = min_image(dz, g.Lz);
+    }
+    float distSq = dx * dx + dy * dy + dz * dz;
+    float reach = ri + boundingRadius[j] + g.cutoff;
+    if (distSq <= reach * reach) {
+        if (cnt < maxCand) slice[cnt] = j;
+        cnt++;
+    }
+}
+
 // SpatialGrid.broadPhase body loop (SpatialGrid.java:362-404): 27-cell stencil in (z, y, x)
 // order, within-cell ascending, j > i, inclusive sphere predicate; returns the realised count
 // (may exceed maxCand: surplus dropped, the count keeps going, :397-398).
+// The three x-cells of one stencil row are adjacent in the cell-sorted contents, so a row is one
+// contents range [offsets[first], offsets[last + 1]) -- two ranges when a periodic row wraps.  All
+// 18 range bounds are loaded up front (independent loads in flight together) before the ranges are
+// walked in stencil order, which visits exactly the reference's sequence of candidates.
 __device__ __forceinline__ int broad_phase_body(int i, int c, const float *__restrict__ center,
                                                 const float *__restrict__ boundingRadius,
                                                 const int *__restrict__ cellOffsets,
                                                 const int *__restrict__ cellContents, const GridGeom &g, int cap,
                                                 int *__restrict__ candPartner, int maxCand) {
-    int nX = g.nX, nY = g.nY, nZ = g.nZ, nXY = nX * nY;
-    int cz = c / nXY;
-    int rem = c - cz * nXY;
-    int cy = rem / nX;
-    int cx = rem - cy * nX;
-    float icx = center[i], icy = center[cap + i], icz = center[2 * cap + i];
-    float ri = boundingRadius[i];
-    int x0 = cx - 1, x1 = cx + 1, y0 = cy - 1, y1 = cy + 1, z0 = cz - 1, z1 = cz + 1;
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ, nXY = nX * nY;
+    const int cz = c / nXY;
+    const int rem = c - cz * nXY;
+    const int cy = rem / nX;
+    const int cx = rem - cy * nX;
+    const float icx = center[i], icy = center[cap + i], icz = center[2 * cap + i];
+    const float ri = boundingRadius[i];
+    // x extent of every row: [xa0, xa1] then (periodic wrap only) [xb0, xb1]
+    int xa0, xa1, xb0 = 0, xb1 = -1;
     if (!g.periodic) {
-        if (x0 < 0) x0 = 0;
-        if (x1 >= nX) x1 = nX - 1;
-        if (y0 < 0) y0 = 0;
-        if (y1 >= nY) y1 = nY - 1;
-        if (z0 < 0) z0 = 0;
-        if (z1 >= nZ) z1 = nZ - 1;
+        xa0 = cx > 0 ? cx - 1 : 0;
+        xa1 = cx + 1 < nX ? cx + 1 : nX - 1;
+    } else if (cx == 0) {
+        xa0 = nX - 1; xa1 = nX - 1; xb0 = 0; xb1 = 1;
+    } else if (cx == nX - 1) {
+        xa0 = nX - 2; xa1 = nX - 1; xb0 = 0; xb1 = 0;
+    } else {
+        xa0 = cx - 1; xa1 = cx + 1;
     }
-    int cnt = 0;
-    size_t slice = (size_t)i * (size_t)maxCand;
-    for (int zz = z0; zz <= z1; zz++) {
-        int wz = g.periodic ? (zz + nZ) % nZ : zz;
-        for (int yy = y0; yy <= y1; yy++) {
-            int wy = g.periodic ? (yy + nY) % nY : yy;
-            for (int xx = x0; xx <= x1; xx++) {
-                int wx = g.periodic ? (xx + nX) % nX : xx;
-                int cc = wx + wy * nX + wz * nXY;
-                int start = cellOffsets[cc];
-                int end = cellOffsets[cc + 1];
-                for (int idx = start; idx < end; idx++) {
-                    int j = cellContents[idx];
-                    if (j <= i) continue;
-                    float dx = icx - center[j];
-                    float dy = icy - center[cap + j];
-                    float dz = icz - center[2 * cap + j];
-                    if (g.periodic) {
-                        dx = min_image(dx, g.Lx);
-                        dy = min_image(dy, g.Ly);
-                        dz = min_image(dz, g.Lz);
-                    }
-                    float distSq = dx * dx + dy * dy + dz * dz;
-                    float reach = ri + boundingRadius[j] + g.cutoff;
-                    if (distSq <= reach * reach) {
-                        if (cnt < maxCand) candPartner[slice + cnt] = j;
-                        cnt++;
-                    }
-                }
+    const bool twoSeg = xb1 >= xb0;
+    int lo[18], hi[18];
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        int zz = cz + r / 3 - 1, yy = cy + r % 3 - 1;
+        bool ok = true;
+        if (g.periodic) {
+            zz = zz < 0 ? zz + nZ : (zz >= nZ ? zz - nZ : zz);
+            yy = yy < 0 ? yy + nY : (yy >= nY ? yy - nY : yy);
+        } else {
+            ok = zz >= 0 && zz < nZ && yy >= 0 && yy < nY;
+        }
+        int base = yy * nX + zz * nXY;
+        lo[2 * r] = 0; hi[2 * r] = 0; lo[2 * r + 1] = 0; hi[2 * r + 1] = 0;
+        if (ok) {
+            lo[2 * r] = cellOffsets[base + xa0];
+            hi[2 * r] = cellOffsets[base + xa1 + 1];
+            if (twoSeg) {
+                lo[2 * r + 1] = cellOffsets[base + xb0];
+                hi[2 * r + 1] = cellOffsets[base + xb1 + 1];
             }
         }
+    }
+    int cnt = 0;
+    int *slice = candPartner + (size_t)i * (size_t)maxCand;
+#pragma unroll
+    for (int s = 0; s < 18; s++) {
+        for (int idx = lo[s]; idx < hi[s]; idx++)
+            bp_visit(i, cellContents[idx], icx, icy, icz, ri, center, boundingRadius, g, cap, slice, maxCand, cnt);
     }
     return cnt;
 }

@@ -10,9 +10,11 @@
 
 namespace sbx {
 // Scratch of the stable counting sort (keys -> CSR offsets + ascending item list).
+// counts[] and tileSums[] are kept all-zero between sorts (`clean`), so no per-sort fill is needed.
 struct SortScratch {
-    int *counts = nullptr, *cursor = nullptr, *tmp = nullptr, *blockSums = nullptr;
-    long long keyCap = 0, itemCap = 0, blockCap = 0;
+    int *counts = nullptr, *tileSums = nullptr, *tmp = nullptr, *slot = nullptr;
+    long long keyCap = 0, itemCap = 0, tileCap = 0;
+    bool clean = false;
 };
 struct ChainParams;
 }  // namespace sbx
@@ -45,8 +47,6 @@ void sort_scratch_free(SortScratch &s);
 // list[...]; offsets[nKeys] is the total clamped to contentsCap; positions >= contentsCap are dropped.
 int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
                   int *list);
-// exclusive scan of in[n] into out[n+1] (out[n] = total, clamped to clampTotal when >= 0)
-int exclusive_scan(sbx_ctx *c, SortScratch &s, const int *in, int *out, int n, int clampTotal);
 int fill_i32(sbx_ctx *c, int *dst, int value, long long count);
 
 ChainParams chain_params(const float *cp);

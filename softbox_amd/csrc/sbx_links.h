@@ -135,15 +135,27 @@ __device__ __forceinline__ void link_torsion_one(int k, int nSeg, int a, int b, 
 // One side of segGatherA/B (CrosslinkerSystem.java:132-147): FP64 sum in CSR order, skip dead links.
 __device__ __forceinline__ void gather_side(int s, const int *__restrict__ offsets, const int *__restrict__ idx,
                                             const float *__restrict__ xlinkData, const int *__restrict__ linkState,
-                                            int off, double f[3], double t[3]) {
+                                            int off, double f[3], double t[3], int &active) {
     int start = offsets[s], end = offsets[s + 1];
     for (int k = start; k < end; k++) {
         int li = idx[k];
         if (linkState[li] < 0) continue;
+        active++;
         const float *d = xlinkData + (size_t)li * XL_STRIDE + off;
         f[0] += d[0]; f[1] += d[1]; f[2] += d[2];
         t[0] += d[3]; t[1] += d[4]; t[2] += d[5];
     }
+}
+
+// CrosslinkerSystem.countActiveLinks for one segment (CrosslinkerSystem.java:694-706): ACTIVE links
+// keyed to s on either side, read from the two static CSR ranges.
+__device__ __forceinline__ int active_links(int s, const int *__restrict__ offA, const int *__restrict__ idxA,
+                                            const int *__restrict__ offB, const int *__restrict__ idxB,
+                                            const int *__restrict__ linkState) {
+    int cnt = 0;
+    for (int k = offA[s]; k < offA[s + 1]; k++) cnt += linkState[idxA[k]] >= 0 ? 1 : 0;
+    for (int k = offB[s]; k < offB[s + 1]; k++) cnt += linkState[idxB[k]] >= 0 ? 1 : 0;
+    return cnt;
 }
 
 LinkParams link_params(const float *xlParams, const float *offParams, const float *torsionParams);

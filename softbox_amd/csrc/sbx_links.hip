@@ -110,7 +110,8 @@ __global__ void __launch_bounds__(256) k_seg_gather(const int *__restrict__ offs
     int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= nSeg) return;
     double f[3] = {0, 0, 0}, t[3] = {0, 0, 0};
-    gather_side(s, offsets, idx, xlinkData, linkState, off, f, t);
+    int active = 0;
+    gather_side(s, offsets, idx, xlinkData, linkState, off, f, t, active);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         forceSum[a * nSeg + s] = (float)(forceSum[a * nSeg + s] + f[a]);

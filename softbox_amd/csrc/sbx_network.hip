@@ -2,12 +2,14 @@
 //
 // Replaces the filament block of DenseContractileHarness.buildPlan/cpuStep
 // (DenseContractileHarness.java:350-429, 460-606), which the reference runs as ~30 TaskGraph tasks:
-//   [grid]  k_body_cell -> counting sort (fill, count, 3-pass scan, scatter, place) -> k_broad_phase
-//   [links] k_net_unbind -> k_net_count -> k_net_link (force + torsion payload)
-//   [rods]  k_rod: zero + Brownian + chain + gather A + gather B + confine + integrate + derive
+//   [grid]  counting sort (count, one-pass scan, scatter, place) -> k_broad_phase
+//   [links] k_net_unbind -> k_net_link (active counts from the CSR + force + torsion payload)
+//   [rods]  k_rod: zero + Brownian + chain + gather A + gather B + confine + integrate + derive,
+//           then the active-link count and the cell of the new pose (the next step's binning)
 // k_rod keeps forceSum/torqueSum and the Brownian draw in registers and ping-pongs the pose buffers
 // (the chain reads neighbours' pre-step pose while each rod writes its own new pose).  The step
 // counter lives in device memory so a run of steps can be captured once as a hipGraph and replayed.
+// k_body_cell runs only when the pose did not come from k_rod (creation, after sbx_net_publish).
 // The crosslink CSR (keyed by linkFilA / linkFilB) depends only on the link topology, so it is built
 // at creation and on sbx_net_topology_changed instead of every step; linkState (which unbinding
 // changes every step) is applied by the gather guard, as in CrosslinkerSystem.segGatherA/B.
@@ -49,8 +51,9 @@ struct sbx_net {
     int *bodyCell = nullptr, *cellOffsets = nullptr, *cellContents = nullptr, *candPartner = nullptr,
         *candCount = nullptr;
     SortScratch sort;
+    int cellsValid = 0;  // bodyCell holds the cells of the current pose (written by the last k_rod)
     // links
-    int *offA = nullptr, *idxA = nullptr, *offB = nullptr, *idxB = nullptr, *ownCount = nullptr;
+    int *offA = nullptr, *idxA = nullptr, *offB = nullptr, *idxB = nullptr;
     LinkParams lp{};
     ChainParams cp{};
     // step counter
@@ -104,6 +107,9 @@ struct RodArgs {
     const int *mOff, *mList;
     const float *bondData;
     int hasMotors;
+    int *activeLinkCount;   // written from the gather's active-link tally (may be null)
+    int *nextCell;          // bodyCell of the new pose for the next step's grid (null: not fused)
+    GridGeom geom;
     const int *dStep;
     int stepOff;
     int n, chainOn, hasLinks, confineOn, checkInt, rng, seed;
@@ -132,19 +138,21 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
     }
     if (a.hasLinks) {
         double fa[3] = {0, 0, 0}, ta[3] = {0, 0, 0};
-        gather_side(i, a.offA, a.idxA, a.xlinkData, a.linkState, 0, fa, ta);
+        int active = 0;
+        gather_side(i, a.offA, a.idxA, a.xlinkData, a.linkState, 0, fa, ta, active);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             f[k] = (float)(f[k] + fa[k]);
             t[k] = (float)(t[k] + ta[k]);
         }
         double fb[3] = {0, 0, 0}, tb[3] = {0, 0, 0};
-        gather_side(i, a.offB, a.idxB, a.xlinkData, a.linkState, 6, fb, tb);
+        gather_side(i, a.offB, a.idxB, a.xlinkData, a.linkState, 6, fb, tb, active);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             f[k] = (float)(f[k] + fb[k]);
             t[k] = (float)(t[k] + tb[k]);
         }
+        if (a.activeLinkCount) a.activeLinkCount[i] = active;
     }
     if (a.hasMotors) {  // CrossBridgeSystem.segGather (CrossBridgeSystem.java:302-320)
         double fm[3] = {0, 0, 0}, tm[3] = {0, 0, 0};
@@ -194,6 +202,7 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         a.end1[k * n + i] = d.e1[k];
         a.end2[k * n + i] = d.e2[k];
     }
+    if (a.nextCell) a.nextCell[i] = cell_of(p.c[0], p.c[1], p.c[2], a.geom);
 }
 
 __global__ void __launch_bounds__(256) k_net_unbind(const float *__restrict__ uVec, const float *__restrict__ end1,
@@ -213,26 +222,13 @@ __global__ void __launch_bounds__(256) k_net_unbind(const float *__restrict__ uV
     if (ns != st) linkState[k] = ns;
 }
 
-// activeLinkCount[s] = ACTIVE links keyed to s on either side (CrosslinkerSystem.countActiveLinks,
-// CrosslinkerSystem.java:694-706), read from the two CSR ranges: no atomics.
-__global__ void __launch_bounds__(256) k_net_count(const int *__restrict__ offA, const int *__restrict__ idxA,
-                                                   const int *__restrict__ offB, const int *__restrict__ idxB,
-                                                   const int *__restrict__ linkState,
-                                                   int *__restrict__ activeLinkCount, int nSeg) {
-    int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= nSeg) return;
-    int cnt = 0;
-    for (int k = offA[s]; k < offA[s + 1]; k++) cnt += linkState[idxA[k]] >= 0 ? 1 : 0;
-    for (int k = offB[s]; k < offB[s + 1]; k++) cnt += linkState[idxB[k]] >= 0 ? 1 : 0;
-    activeLinkCount[s] = cnt;
-}
-
 // linkForces + linkTorsion for one link (CrosslinkerSystem.java:62-123, 96-128)
 __global__ void __launch_bounds__(256) k_net_link(const float *__restrict__ coord, const float *__restrict__ uVec,
                                                   const float *__restrict__ end1, const float *__restrict__ bTransGam,
                                                   const int *__restrict__ linkFilA, const int *__restrict__ linkFilB,
                                                   const float *__restrict__ loc1, const float *__restrict__ loc2,
-                                                  const int *__restrict__ activeLinkCount,
+                                                  const int *__restrict__ offA, const int *__restrict__ idxA,
+                                                  const int *__restrict__ offB, const int *__restrict__ idxB,
                                                   const int *__restrict__ linkState,
                                                   const int *__restrict__ linkOrientSame,
                                                   float *__restrict__ torqueMagHist, int *__restrict__ torqueMagPlace,
@@ -244,8 +240,9 @@ __global__ void __launch_bounds__(256) k_net_link(const float *__restrict__ coor
     for (int j = 0; j < 12; j++) pay[j] = 0.0f;
     int a = linkFilA[k], b = linkFilB[k];
     if (a >= 0 && b >= 0) {
-        link_force_one(nSeg, a, b, loc1[k], loc2[k], coord, uVec, end1, bTransGam, activeLinkCount[a],
-                       activeLinkCount[b], lp, pay);
+        int ctA = active_links(a, offA, idxA, offB, idxB, linkState);
+        int ctB = active_links(b, offA, idxA, offB, idxB, linkState);
+        link_force_one(nSeg, a, b, loc1[k], loc2[k], coord, uVec, end1, bTransGam, ctA, ctB, lp, pay);
         if (lp.torsionOn && linkState[k] >= 0)
             link_torsion_one(k, nSeg, a, b, linkOrientSame[k] != 0, uVec, torqueMagHist, torqueMagPlace, lp, pay);
     }
@@ -654,10 +651,11 @@ int enqueue_step(sbx_net *net, int stepOff) {
     if (net->hasMotors) {
         if ((r = enqueue_myosin(net, coord, uVec, yVec, stepOff))) return r;
     } else if (net->p.gridOn) {
-        {
+        if (!net->cellsValid) {  // otherwise the previous k_rod binned the current pose
             Scope sc(net, T_BIN);
             r = launch_body_cell(c, coord, n, n, net->geom, net->bodyCell);
             if (r) return r;
+            net->cellsValid = 1;
         }
         {
             Scope sc(net, T_SORT);
@@ -672,7 +670,6 @@ int enqueue_step(sbx_net *net, int stepOff) {
             if (r) return r;
         }
     }
-    int *activeCount = net->xl.activeLinkCount ? net->xl.activeLinkCount : net->ownCount;
     if (L > 0) {
         if (net->p.unbindOn) {
             Scope sc(net, T_UNBIND);
@@ -682,15 +679,11 @@ int enqueue_step(sbx_net *net, int stepOff) {
                                net->p.rng, net->p.stream, n, L);
         }
         {
-            Scope sc(net, T_COUNT);
-            hipLaunchKernelGGL(k_net_count, dim3(blocks_for(n, 256)), dim3(256), 0, s, net->offA, net->idxA,
-                               net->offB, net->idxB, net->xl.linkState, activeCount, n);
-        }
-        {
             Scope sc(net, T_LINK);
             hipLaunchKernelGGL(k_net_link, dim3(blocks_for(L, 256)), dim3(256), 0, s, coord, uVec, net->fil.end1,
                                net->fil.bTransGam, net->xl.linkFilA, net->xl.linkFilB, net->xl.loc1, net->xl.loc2,
-                               activeCount, net->xl.linkState, net->xl.linkOrientSame, net->xl.torqueMagHist,
+                               net->offA, net->idxA, net->offB, net->idxB, net->xl.linkState,
+                               net->xl.linkOrientSame, net->xl.torqueMagHist,
                                net->xl.torqueMagPlace, net->xl.xlinkData, net->lp, n, L);
         }
         r = check_launch(c, "link kernels");
@@ -709,6 +702,9 @@ int enqueue_step(sbx_net *net, int stepOff) {
     a.offA = net->offA; a.idxA = net->idxA; a.offB = net->offB; a.idxB = net->idxB;
     a.linkState = net->xl.linkState; a.xlinkData = net->xl.xlinkData;
     a.hasMotors = net->hasMotors;
+    a.activeLinkCount = net->xl.activeLinkCount;
+    a.geom = net->geom;
+    a.nextCell = (net->p.gridOn && !net->hasMotors) ? net->bodyCell : nullptr;
     a.mOff = net->mOff; a.mList = net->mList; a.bondData = net->mo.bondData;
     if (net->hasMotors) {
         Scope sc(net, T_MCSR);
@@ -748,7 +744,7 @@ void release(sbx_net *net) {
     if (net->graph) hipGraphDestroy(net->graph);
     void *bufs[] = {net->ownPose,  net->boundR,    net->bodyCell, net->cellOffsets, net->cellContents,
                     net->candPartner, net->candCount, net->offA,  net->idxA,        net->offB,
-                    net->idxB,     net->ownCount,  net->dStep,    net->ownerStore,  net->ownerSlot,
+                    net->idxB,     net->dStep,     net->ownerStore, net->ownerSlot,
                     net->mOff,     net->mList,     net->bbOff,    net->bbList};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -841,7 +837,6 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
             if ((r = alloc(c, &net->offB, (size_t)n + 1))) break;
             if ((r = alloc(c, &net->idxA, L))) break;
             if ((r = alloc(c, &net->idxB, L))) break;
-            if (!xl->activeLinkCount && (r = alloc(c, &net->ownCount, n))) break;
             if ((r = build_link_csr(net))) break;
         }
         // reserve the sort scratch now: nothing may allocate inside a captured step
@@ -975,6 +970,10 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
     // Eager launches use stepOff = done - advanced, where `advanced` counts the steps already added
     // to the device counter by graph replays (each replay ends with k_add_step(G)).
     int done = 0, advanced = 0;
+    if (net->p.gridOn && !net->hasMotors && !net->cellsValid) {  // bin the caller's pose once, eagerly
+        if ((r = launch_body_cell(c, net->coordB[net->cur], net->n, net->n, net->geom, net->bodyCell))) return r;
+        net->cellsValid = 1;
+    }
     if (net->useGraph && !net->profiling) {
         // one captured graph = 2 steps (the pose ping-pong returns to its start buffers) + step advance
         const int G = 2;
@@ -1035,6 +1034,7 @@ int sbx_net_publish(sbx_net *net) {
     if (!r) r = check_hip(c, hipMemcpyAsync(net->yB[0], net->yB[1], bytes, hipMemcpyDeviceToDevice, c->stream),
                           "publish yVec");
     if (!r) net->cur = 0;
+    net->cellsValid = 0;  // the caller may now edit its pose arrays
     return r;
 }
 
