@@ -642,7 +642,8 @@ void sbo_grid_build_serial(int S, int contentsCap, const int *bodyCell, const in
 }
 
 static float min_image(float d, float L) {
-    return d - L * rintf(d / L);
+    float inv = 1.0f / L;
+    return d - L * rintf(d * inv);
 }
 
 /* SpatialGrid.broadPhase, SpatialGrid.java:340-406.  Box mode is the reference verbatim.

@@ -90,23 +90,36 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *total) {
     return waveOff + inc - v;
 }
 
-// Exclusive scan of counts[n] into out[n+1] in ONE pass.  tileSums[t] (sum of tile t of counts)
-// was accumulated by the producer of counts, so each tile's prefix is the sum of the tile sums
-// before it -- a block-wide reduction over at most n/2048 L2-resident ints -- and no tile waits
-// on another.  out[n] = grand total, clamped to clampTotal when >= 0.
+// Exclusive scan of counts[n] into out[n+1] given tileSums[t] (the sum of tile t): each tile's
+// prefix is the sum of the tile sums before it -- a block-wide reduction over at most n/2048
+// L2-resident ints -- so no tile waits on another.  out[n] = grand total, clamped to clampTotal
+// when >= 0.
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int *__restrict__ in, int n,
                                                              const int *__restrict__ tileSums, int nTiles,
                                                              int *__restrict__ out, int clampTotal) {
     __shared__ int red[SCAN_THREADS / 64];
     const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // this thread's 8 consecutive items (two 16-byte loads when the tile is full), issued before the
+    // tile-prefix reduction so both sets of loads are in flight together
+    long long base = (long long)tile * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    if ((long long)(tile + 1) * SCAN_TILE <= n) {
+        const int4 *p = reinterpret_cast<const int4 *>(in + base);
+        int4 a = p[0], b = p[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) v[k] = (base + k < n) ? in[base + k] : 0;
+    }
     // prefix of this tile (and, for the last tile, the grand total)
-    int lim = (tile == nTiles - 1) ? nTiles : tile;
-    int part = 0;
-    for (int t = threadIdx.x; t < lim; t += SCAN_THREADS) part += (t < tile ? tileSums[t] : 0);
-    int lastPart = 0;
-    if (tile == nTiles - 1)
-        for (int t = threadIdx.x; t < nTiles; t += SCAN_THREADS) lastPart += tileSums[t];
+    int part = 0, lastPart = 0;
+    for (int t = threadIdx.x; t < nTiles; t += SCAN_THREADS) {
+        int ts = tileSums[t];
+        part += t < tile ? ts : 0;
+        lastPart += ts;
+    }
     part = wave_sum(part);
     if (lane == 0) red[wave] = part;
     __syncthreads();
@@ -114,15 +127,9 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int *__restri
 #pragma unroll
     for (int w = 0; w < SCAN_THREADS / 64; w++) tilePrefix += red[w];
     __syncthreads();
-    long long base = (long long)tile * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
-    int v[SCAN_ITEMS];
     int sum = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        long long idx = base + k;
-        v[k] = (idx < n) ? in[idx] : 0;
-        sum += v[k];
-    }
+    for (int k = 0; k < SCAN_ITEMS; k++) sum += v[k];
     int total;
     int pre = block_exclusive_scan(sum, &total) + tilePrefix;
 #pragma unroll
@@ -187,33 +194,53 @@ void sort_scratch_free(SortScratch &s) {
 }
 
 // ------------------------------------------------------------------ counting sort
-// Scratch invariant: counts[] and tileSums[] are all zero between sorts (zeroed once at
-// allocation; each sort re-zeroes exactly the entries it dirtied), so a sort issues no fill
-// over the key range -- which for a sparse grid is ~10x larger than the item count.
+// Scratch invariant: counts[] is all zero between sorts (zeroed once at allocation; each sort
+// re-zeroes exactly the entries it dirtied), so a sort issues no fill over the key range -- which
+// for a sparse grid is ~10x larger than the item count.
 //
-// 1. k_key_count:   slot[m] = atomicAdd(counts[key]) (a provisional, unordered position) and the
-//                   key's tile sum;
-// 2. k_scan_tiles:  offsets = exclusive scan of counts (tile prefixes from the tile sums);
-// 3. k_key_scatter: tmp[offsets[key] + slot[m]] = m; counts[key] = 0; tileSums zeroed;
-// 4. k_key_place:   rank of m among its key's segment (items with a smaller index) -> list.
+// 1. k_key_count:   slot[m] = atomicAdd(counts[key]) (a provisional, unordered position);
+// 2. k_tile_reduce: tileSums = per-tile sums of counts;
+// 3. k_scan_tiles:  offsets = exclusive scan of counts (tile prefixes from the tile sums);
+// 4. k_key_scatter: tmp[offsets[key] + slot[m]] = m; counts[key] = 0;
+// 5. k_key_place:   rank of m among its key's segment (items with a smaller index) -> list.
 // The result is independent of the atomic order: ascending item index within each key.
 __global__ void __launch_bounds__(256) k_key_count(const int *__restrict__ key, int nItems, int nKeys,
-                                                   int *__restrict__ counts, int *__restrict__ tileSums,
-                                                   int *__restrict__ slot) {
+                                                   int *__restrict__ counts, int *__restrict__ slot) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nItems) return;
     int k = key[m];
     if (k < 0 || k >= nKeys) return;
     slot[m] = atomicAdd(&counts[k], 1);
-    atomicAdd(&tileSums[k / SCAN_TILE], 1);
+}
+
+// tileSums[t] = sum of tile t of counts (one block per tile; no atomics: device-scope atomics on a
+// few hot addresses serialise at the memory side)
+__global__ void __launch_bounds__(SCAN_THREADS) k_tile_reduce(const int *__restrict__ in, int n,
+                                                              int *__restrict__ tileSums) {
+    __shared__ int red[SCAN_THREADS / 64];
+    long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
+    int sum = 0;
+    if ((long long)(blockIdx.x + 1) * SCAN_TILE <= n) {
+        const int4 *p = reinterpret_cast<const int4 *>(in + base);
+        int4 a = p[0], b = p[1];
+        sum = a.x + a.y + a.z + a.w + b.x + b.y + b.z + b.w;
+    } else {
+        for (int k = 0; k < SCAN_ITEMS; k++) sum += (base + k < n) ? in[base + k] : 0;
+    }
+    sum = wave_sum(sum);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < SCAN_THREADS / 64; w++) t += red[w];
+        tileSums[blockIdx.x] = t;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_key_scatter(const int *__restrict__ key, int nItems, int nKeys,
                                                      const int *__restrict__ offsets, const int *__restrict__ slot,
-                                                     int *__restrict__ counts, int *__restrict__ tileSums, int nTiles,
-                                                     int *__restrict__ tmp, int contentsCap) {
+                                                     int *__restrict__ counts, int *__restrict__ tmp, int contentsCap) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < nTiles) tileSums[m] = 0;
     if (m >= nItems) return;
     int k = key[m];
     if (k < 0 || k >= nKeys) return;
@@ -246,14 +273,14 @@ int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nK
     if (nt < 1) nt = 1;
     if (nItems > 0) {
         hipLaunchKernelGGL(k_key_count, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems, nKeys,
-                           s.counts, s.tileSums, s.slot);
+                           s.counts, s.slot);
     }
+    hipLaunchKernelGGL(k_tile_reduce, dim3(nt), dim3(SCAN_THREADS), 0, c->stream, s.counts, nKeys, s.tileSums);
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, c->stream, s.counts, nKeys, s.tileSums, nt,
                        offsets, contentsCap);
-    int span = nItems > nt ? nItems : nt;
-    hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(span, 256)), dim3(256), 0, c->stream, key, nItems, nKeys, offsets,
-                       s.slot, s.counts, s.tileSums, nt, s.tmp, contentsCap);
     if (nItems > 0) {
+        hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems, nKeys,
+                           offsets, s.slot, s.counts, s.tmp, contentsCap);
         hipLaunchKernelGGL(k_key_place, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems, nKeys,
                            offsets, s.tmp, list, contentsCap);
     }

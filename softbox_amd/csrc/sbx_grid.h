@@ -12,6 +12,7 @@ struct GridGeom {
     int nX, nY, nZ;
     int periodic;
     float Lx, Ly, Lz;  // periodic box lengths (minimum image)
+    float iLx, iLy, iLz;  // 1.0f / L (host IEEE division)
 };
 
 // Java (int) of a float followed by the reference clamp to [0, nBins-1] (SpatialGrid.java:67-72);
@@ -46,29 +47,11 @@ __device__ __forceinline__ int cell_of(float x, float y, float z, const GridGeom
     return ix + iy * g.nX + iz * g.nX * g.nY;
 }
 
-__device__ __forceinline__ float min_image(float d, float L) { return d - L * rintf(d / L); }
+// minimum image of a coordinate difference: d - L * rint(d * (1/L)), rint = round half to even
+__device__ __forceinline__ float min_image(float d, float L, float invL) { return d - L * rintf(d * invL); }
 
-// Candidate test of body i against body j (SpatialGrid.java:385-399): inclusive sphere predicate
-// on (minimum-image) centre distance; appends j to i's slice while it has room.
-__device__ __forceinline__ void bp_visit(int i, int j, float icx, float icy, float icz, float ri,
-                                         const float *__restrict__ center, const float *__restrict__ boundingRadius,
-                                         const GridGeom &g, int cap, int *__restrict__ slice, int maxCand, int &cnt) {
-    if (j <= i) return;
-    float dx = icx - center[j];
-    float dy = icy - center[cap + j];
-    float dz = icz - center[2 * cap + j];
-    if (g.periodic) {
-        dx = min_image(dx, g.Lx);
-        dy = min_image(dy, g.Ly);
-        dz = min_image(dz, g.Lz);
-    }
-    float distSq = dx * dx + dy * dy + dz * dz;
-    float reach = ri + boundingRadius[j] + g.cutoff;
-    if (distSq <= reach * reach) {
-        if (cnt < maxCand) slice[cnt] = j;
-        cnt++;
-    }
-}
+constexpr int BP_THREADS = 256;
+constexpr int BP_BATCH = 4;
 
 // SpatialGrid.broadPhase body loop (SpatialGrid.java:362-404): 27-cell stencil in (z, y, x)
 // order, within-cell ascending, j > i, inclusive sphere predicate; returns the realised count
@@ -77,11 +60,12 @@ __device__ __forceinline__ void bp_visit(int i, int j, float icx, float icy, flo
 // contents range [offsets[first], offsets[last + 1]) -- two ranges when a periodic row wraps.  All
 // 18 range bounds are loaded up front (independent loads in flight together) before the ranges are
 // walked in stencil order, which visits exactly the reference's sequence of candidates.
+// segLds: 18 int2 of LDS per thread of the block (BP_THREADS x 18).
 __device__ __forceinline__ int broad_phase_body(int i, int c, const float *__restrict__ center,
                                                 const float *__restrict__ boundingRadius,
                                                 const int *__restrict__ cellOffsets,
                                                 const int *__restrict__ cellContents, const GridGeom &g, int cap,
-                                                int *__restrict__ candPartner, int maxCand) {
+                                                int *__restrict__ candPartner, int maxCand, int2 *segLds) {
     const int nX = g.nX, nY = g.nY, nZ = g.nZ, nXY = nX * nY;
     const int cz = c / nXY;
     const int rem = c - cz * nXY;
@@ -102,7 +86,9 @@ __device__ __forceinline__ int broad_phase_body(int i, int c, const float *__res
         xa0 = cx - 1; xa1 = cx + 1;
     }
     const bool twoSeg = xb1 >= xb0;
-    int lo[18], hi[18];
+    // Range bounds go to this thread's LDS row so the walk below can index them dynamically.
+    int2 *seg = segLds + threadIdx.x * 18;
+    int total = 0;
 #pragma unroll
     for (int r = 0; r < 9; r++) {
         int zz = cz + r / 3 - 1, yy = cy + r % 3 - 1;
@@ -114,22 +100,62 @@ __device__ __forceinline__ int broad_phase_body(int i, int c, const float *__res
             ok = zz >= 0 && zz < nZ && yy >= 0 && yy < nY;
         }
         int base = yy * nX + zz * nXY;
-        lo[2 * r] = 0; hi[2 * r] = 0; lo[2 * r + 1] = 0; hi[2 * r + 1] = 0;
+        int2 s0 = make_int2(0, 0), s1 = make_int2(0, 0);
         if (ok) {
-            lo[2 * r] = cellOffsets[base + xa0];
-            hi[2 * r] = cellOffsets[base + xa1 + 1];
-            if (twoSeg) {
-                lo[2 * r + 1] = cellOffsets[base + xb0];
-                hi[2 * r + 1] = cellOffsets[base + xb1 + 1];
-            }
+            s0 = make_int2(cellOffsets[base + xa0], cellOffsets[base + xa1 + 1]);
+            if (twoSeg) s1 = make_int2(cellOffsets[base + xb0], cellOffsets[base + xb1 + 1]);
         }
+        seg[2 * r] = s0;
+        seg[2 * r + 1] = s1;
+        total += (s0.y - s0.x) + (s1.y - s1.x);
     }
+    // One flat loop over all candidates (the wave iterates max over lanes of the candidate total,
+    // not the sum over the 18 ranges of each range's maximum), BP_BATCH candidates at a time so
+    // their contents and centre loads are in flight together; tests and appends stay in order.
     int cnt = 0;
     int *slice = candPartner + (size_t)i * (size_t)maxCand;
+    int s = 0, idx = seg[0].x, end = seg[0].y;
+    for (int q = 0; q < total; q += BP_BATCH) {
+        int jj[BP_BATCH];
 #pragma unroll
-    for (int s = 0; s < 18; s++) {
-        for (int idx = lo[s]; idx < hi[s]; idx++)
-            bp_visit(i, cellContents[idx], icx, icy, icz, ri, center, boundingRadius, g, cap, slice, maxCand, cnt);
+        for (int u = 0; u < BP_BATCH; u++) {
+            jj[u] = -1;
+            if (q + u < total) {
+                while (idx >= end && s < 17) {
+                    s++;
+                    int2 e = seg[s];
+                    idx = e.x;
+                    end = e.y;
+                }
+                jj[u] = cellContents[idx];
+                idx++;
+            }
+        }
+        float px[BP_BATCH], py[BP_BATCH], pz[BP_BATCH], pr[BP_BATCH];
+#pragma unroll
+        for (int u = 0; u < BP_BATCH; u++) {
+            int j = jj[u] > i ? jj[u] : i;  // j <= i is skipped below; read a valid slot meanwhile
+            px[u] = center[j];
+            py[u] = center[cap + j];
+            pz[u] = center[2 * cap + j];
+            pr[u] = boundingRadius[j];
+        }
+#pragma unroll
+        for (int u = 0; u < BP_BATCH; u++) {
+            if (jj[u] <= i) continue;
+            float dx = icx - px[u], dy = icy - py[u], dz = icz - pz[u];
+            if (g.periodic) {
+                dx = min_image(dx, g.Lx, g.iLx);
+                dy = min_image(dy, g.Ly, g.iLy);
+                dz = min_image(dz, g.Lz, g.iLz);
+            }
+            float distSq = dx * dx + dy * dy + dz * dz;
+            float reach = ri + pr[u] + g.cutoff;
+            if (distSq <= reach * reach) {
+                if (cnt < maxCand) slice[cnt] = jj[u];
+                cnt++;
+            }
+        }
     }
     return cnt;
 }

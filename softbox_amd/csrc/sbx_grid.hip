@@ -32,24 +32,27 @@ __global__ void __launch_bounds__(256) k_body_cell(const float *__restrict__ cen
     bodyCell[i] = cell_of(center[i], center[cap + i], center[2 * cap + i], g);
 }
 
-__global__ void __launch_bounds__(256) k_broad_phase(const float *__restrict__ center,
+__global__ void __launch_bounds__(BP_THREADS) k_broad_phase(const float *__restrict__ center,
                                                      const float *__restrict__ boundingRadius,
                                                      const int *__restrict__ bodyCell,
                                                      const int *__restrict__ cellOffsets,
                                                      const int *__restrict__ cellContents, GridGeom g, int cap, int S,
                                                      int nSorted, int *__restrict__ candPartner,
                                                      int *__restrict__ candCount, int maxCand) {
-    int t = blockIdx.x * 256 + threadIdx.x;
+    // XCD-aware: each XCD walks one contiguous run of the cell-sorted bodies (a spatial slab), so
+    // its L2 holds that slab's cell offsets and centres instead of the whole grid
+    int t = xcd_block(blockIdx.x, gridDim.x) * BP_THREADS + threadIdx.x;
     if (t >= cap) return;
     if (t >= nSorted) {  // padding slots [S, cap): count cleared, nothing emitted
         candCount[t] = 0;
         return;
     }
+    __shared__ int2 segLds[BP_THREADS * 18];
     int i = cellContents[t];  // cell-sorted order: neighbouring lanes query neighbouring cells
     int c = bodyCell[i];
     if (c < 0) { candCount[i] = 0; return; }
     candCount[i] = broad_phase_body(i, c, center, boundingRadius, cellOffsets, cellContents, g, cap, candPartner,
-                                    maxCand);
+                                    maxCand, segLds);
 }
 
 GridGeom grid_geom(const float *gridParams, const int *gridDims, int periodic, const float *boxLen) {
@@ -64,8 +67,10 @@ GridGeom grid_geom(const float *gridParams, const int *gridDims, int periodic, c
         else {
             g.Lx = gridParams[3] * g.nX; g.Ly = gridParams[3] * g.nY; g.Lz = gridParams[3] * g.nZ;
         }
+        g.iLx = 1.0f / g.Lx; g.iLy = 1.0f / g.Ly; g.iLz = 1.0f / g.Lz;
     } else {
         g.Lx = g.Ly = g.Lz = 0.0f;
+        g.iLx = g.iLy = g.iLz = 0.0f;
     }
     return g;
 }
@@ -92,7 +97,8 @@ int launch_broad_phase(sbx_ctx *c, const float *center, const float *boundingRad
                        const int *cellOffsets, const int *cellContents, const GridGeom &g, int cap, int S,
                        int *candPartner, int *candCount, int maxCand) {
     // bodies with a cell are exactly the first S (counting sort places each live body once)
-    hipLaunchKernelGGL(k_broad_phase, dim3(blocks_for(cap, 256)), dim3(256), 0, c->stream, center, boundingRadius,
+    hipLaunchKernelGGL(k_broad_phase, dim3(blocks_for(cap, BP_THREADS)), dim3(BP_THREADS), 0, c->stream, center,
+                       boundingRadius,
                        bodyCell, cellOffsets, cellContents, g, cap, S, S, candPartner, candCount, maxCand);
     return check_launch(c, "k_broad_phase");
 }

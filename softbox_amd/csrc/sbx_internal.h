@@ -10,7 +10,7 @@
 
 namespace sbx {
 // Scratch of the stable counting sort (keys -> CSR offsets + ascending item list).
-// counts[] and tileSums[] are kept all-zero between sorts (`clean`), so no per-sort fill is needed.
+// counts[] is kept all-zero between sorts (`clean`), so no per-sort fill is needed.
 struct SortScratch {
     int *counts = nullptr, *tileSums = nullptr, *tmp = nullptr, *slot = nullptr;
     long long keyCap = 0, itemCap = 0, tileCap = 0;
@@ -35,6 +35,16 @@ int fail(sbx_ctx *c, int code, const char *fmt, ...);
 int check_hip(sbx_ctx *c, hipError_t e, const char *what);
 int check_launch(sbx_ctx *c, const char *what);
 int set_device(sbx_ctx *c);
+
+// gfx950 dispatches workgroup b to XCD b % 8, and each XCD has its own L2.  Kernels whose
+// consecutive blocks touch neighbouring data renumber their blocks so that each XCD gets one
+// contiguous run of logical blocks (a bijection of [0, nb)).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    int xcd = b % kXcds, k = b / kXcds;
+    int per = nb / kXcds, rem = nb % kXcds;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
 
 inline unsigned blocks_for(long long n, int threads) {
     long long b = (n + threads - 1) / threads;

@@ -16,29 +16,57 @@ struct LinkParams {
     int torsionOn;
 };
 
+// What the link laws read of one segment: centre, axis, end1 and the parallel drag.
+struct LinkEnd {
+    float c[3], u[3], e1[3], gpar;
+};
+
+// planar store arrays (the reference layout, stride nSeg)
+__device__ __forceinline__ LinkEnd link_end_planar(int s, int nSeg, const float *__restrict__ coord,
+                                                   const float *__restrict__ uVec, const float *__restrict__ end1,
+                                                   const float *__restrict__ bTransGam) {
+    LinkEnd e;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        e.c[k] = coord ? coord[k * nSeg + s] : 0.0f;
+        e.u[k] = uVec[k * nSeg + s];
+        e.e1[k] = end1[k * nSeg + s];
+    }
+    e.gpar = bTransGam ? bTransGam[s] : 0.0f;
+    return e;
+}
+
+// packed mirror written by the fused step: 3 float4 per segment {c, gpar} {u, 0} {end1, 0}, so one
+// random endpoint gather touches 48 contiguous bytes instead of 10 planar streams
+constexpr int LINK_PACK = 3;
+__device__ __forceinline__ LinkEnd link_end_packed(int s, const float4 *__restrict__ pack) {
+    float4 a = pack[LINK_PACK * s], b = pack[LINK_PACK * s + 1], c = pack[LINK_PACK * s + 2];
+    LinkEnd e;
+    e.c[0] = a.x; e.c[1] = a.y; e.c[2] = a.z; e.gpar = a.w;
+    e.u[0] = b.x; e.u[1] = b.y; e.u[2] = b.z;
+    e.e1[0] = c.x; e.e1[1] = c.y; e.e1[2] = c.z;
+    return e;
+}
+
 // attachment points p1 = end1(A) + loc1*u(A), p2 = end1(B) + loc2*u(B) (CrosslinkerSystem.java:72-83)
-__device__ __forceinline__ void link_points(int nSeg, int a, int b, float l1f, float l2f,
-                                            const float *__restrict__ uVec, const float *__restrict__ end1,
-                                            double p1[3], double p2[3]) {
-    double l1 = l1f;
-    p1[0] = end1[a] + l1 * (double)uVec[a];
-    p1[1] = end1[nSeg + a] + l1 * (double)uVec[nSeg + a];
-    p1[2] = end1[2 * nSeg + a] + l1 * (double)uVec[2 * nSeg + a];
-    double l2 = l2f;
-    p2[0] = end1[b] + l2 * (double)uVec[b];
-    p2[1] = end1[nSeg + b] + l2 * (double)uVec[nSeg + b];
-    p2[2] = end1[2 * nSeg + b] + l2 * (double)uVec[2 * nSeg + b];
+__device__ __forceinline__ void link_points(const LinkEnd &A, const LinkEnd &B, float l1f, float l2f, double p1[3],
+                                            double p2[3]) {
+    double l1 = l1f, l2 = l2f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        p1[k] = A.e1[k] + l1 * (double)A.u[k];
+        p2[k] = B.e1[k] + l2 * (double)B.u[k];
+    }
 }
 
 // CrosslinkerSystem.unbind body (CrosslinkerSystem.java:245-279) for one ACTIVE link; returns the
 // new linkState (unchanged or -1) and updates the strain ring in place.
-__device__ __forceinline__ int link_unbind_one(int k, int nSeg, int a, int b, float l1, float l2,
-                                               const float *__restrict__ uVec, const float *__restrict__ end1,
+__device__ __forceinline__ int link_unbind_one(int k, const LinkEnd &A, const LinkEnd &B, float l1, float l2,
                                                float *__restrict__ strainHist, int *__restrict__ strainPlace,
                                                const LinkParams &lp, int step, int seed, int rng, uint32_t stream,
                                                int state) {
     double p1[3], p2[3];
-    link_points(nSeg, a, b, l1, l2, uVec, end1, p1, p2);
+    link_points(A, B, l1, l2, p1, p2);
     double lvx = p2[0] - p1[0], lvy = p2[1] - p1[1], lvz = p2[2] - p1[2];
     double linkLength = sqrt(lvx * lvx + lvy * lvy + lvz * lvz);
     double stretch = linkLength - lp.offRest;
@@ -60,12 +88,10 @@ __device__ __forceinline__ int link_unbind_one(int k, int nSeg, int a, int b, fl
 }
 
 // CrosslinkerSystem.linkForces body (CrosslinkerSystem.java:72-122) -> 12-float payload
-__device__ __forceinline__ void link_force_one(int nSeg, int a, int b, float l1, float l2,
-                                               const float *__restrict__ coord, const float *__restrict__ uVec,
-                                               const float *__restrict__ end1, const float *__restrict__ bTransGam,
-                                               int ctA, int ctB, const LinkParams &lp, float out[12]) {
+__device__ __forceinline__ void link_force_one(const LinkEnd &A, const LinkEnd &B, float l1, float l2, int ctA,
+                                               int ctB, const LinkParams &lp, float out[12]) {
     double p1[3], p2[3];
-    link_points(nSeg, a, b, l1, l2, uVec, end1, p1, p2);
+    link_points(A, B, l1, l2, p1, p2);
     double lvx = p2[0] - p1[0], lvy = p2[1] - p1[1], lvz = p2[2] - p1[2];
     double linkLength = sqrt(lvx * lvx + lvy * lvy + lvz * lvz);
     double stretch = linkLength - lp.restLength;
@@ -73,20 +99,20 @@ __device__ __forceinline__ void link_force_one(int nSeg, int a, int b, float l1,
     int maxLinks = ctA > ctB ? ctA : ctB;
     if (maxLinks < 1) maxLinks = 1;
     double fracMove = lp.fracMoveBase / maxLinks;
-    double g1 = bTransGam[a];
-    double g2 = bTransGam[b];
+    double g1 = A.gpar;
+    double g2 = B.gpar;
     double curForceMag = (fracMove * 1.0e-6 * stretch / lp.dt) / (1.0 / g1 + 1.0 / g2);
     double Fx = curForceMag * lvx, Fy = curForceMag * lvy, Fz = curForceMag * lvz;
-    double rax = (p1[0] - coord[a]) * 1e-6;
-    double ray = (p1[1] - coord[nSeg + a]) * 1e-6;
-    double raz = (p1[2] - coord[2 * nSeg + a]) * 1e-6;
+    double rax = (p1[0] - A.c[0]) * 1e-6;
+    double ray = (p1[1] - A.c[1]) * 1e-6;
+    double raz = (p1[2] - A.c[2]) * 1e-6;
     double TAx = ray * Fz - raz * Fy;
     double TAy = raz * Fx - rax * Fz;
     double TAz = rax * Fy - ray * Fx;
     double nFx = -Fx, nFy = -Fy, nFz = -Fz;
-    double rbx = (p2[0] - coord[b]) * 1e-6;
-    double rby = (p2[1] - coord[nSeg + b]) * 1e-6;
-    double rbz = (p2[2] - coord[2 * nSeg + b]) * 1e-6;
+    double rbx = (p2[0] - B.c[0]) * 1e-6;
+    double rby = (p2[1] - B.c[1]) * 1e-6;
+    double rbz = (p2[2] - B.c[2]) * 1e-6;
     double TBx = rby * nFz - rbz * nFy;
     double TBy = rbz * nFx - rbx * nFz;
     double TBz = rbx * nFy - rby * nFx;
@@ -98,12 +124,11 @@ __device__ __forceinline__ void link_force_one(int nSeg, int a, int b, float l1,
 
 // CrosslinkerSystem.linkTorsion body (CrosslinkerSystem.java:97-127) for one ACTIVE link with valid
 // keys; adds into the payload torque slots exactly as (float)(payload + T).
-__device__ __forceinline__ void link_torsion_one(int k, int nSeg, int a, int b, int same,
-                                                 const float *__restrict__ uVec, float *__restrict__ torqueMagHist,
-                                                 int *__restrict__ torqueMagPlace, const LinkParams &lp,
-                                                 float pay[12]) {
-    double uax = uVec[a], uay = uVec[nSeg + a], uaz = uVec[2 * nSeg + a];
-    double ubx = uVec[b], uby = uVec[nSeg + b], ubz = uVec[2 * nSeg + b];
+__device__ __forceinline__ void link_torsion_one(int k, const LinkEnd &A, const LinkEnd &B, int same,
+                                                 float *__restrict__ torqueMagHist, int *__restrict__ torqueMagPlace,
+                                                 const LinkParams &lp, float pay[12]) {
+    double uax = A.u[0], uay = A.u[1], uaz = A.u[2];
+    double ubx = B.u[0], uby = B.u[1], ubz = B.u[2];
     double vbx = same ? ubx : -ubx, vby = same ? uby : -uby, vbz = same ? ubz : -ubz;
     double tx = uay * vbz - uaz * vby, ty = uaz * vbx - uax * vbz, tz = uax * vby - uay * vbx;
     double m2 = tx * tx + ty * ty + tz * tz;

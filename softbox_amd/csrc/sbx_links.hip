@@ -55,8 +55,9 @@ __global__ void __launch_bounds__(256) k_link_unbind(const float *__restrict__ u
     if (k >= nLinks) return;
     int st = linkState[k];
     if (st < 0) return;
-    int ns = link_unbind_one(k, nSeg, linkFilA[k], linkFilB[k], loc1[k], loc2[k], uVec, end1, strainHist,
-                             strainPlace, lp, step, seed, rng, stream, st);
+    LinkEnd A = link_end_planar(linkFilA[k], nSeg, nullptr, uVec, end1, nullptr);
+    LinkEnd B = link_end_planar(linkFilB[k], nSeg, nullptr, uVec, end1, nullptr);
+    int ns = link_unbind_one(k, A, B, loc1[k], loc2[k], strainHist, strainPlace, lp, step, seed, rng, stream, st);
     if (ns != st) linkState[k] = ns;
 }
 
@@ -74,9 +75,11 @@ __global__ void __launch_bounds__(256) k_link_forces(const float *__restrict__ c
 #pragma unroll
     for (int j = 0; j < 12; j++) pay[j] = 0.0f;
     int a = linkFilA[k], b = linkFilB[k];
-    if (a >= 0 && b >= 0)
-        link_force_one(nSeg, a, b, loc1[k], loc2[k], coord, uVec, end1, bTransGam, filLinkCt[a], filLinkCt[b], lp,
-                       pay);
+    if (a >= 0 && b >= 0) {
+        LinkEnd A = link_end_planar(a, nSeg, coord, uVec, end1, bTransGam);
+        LinkEnd B = link_end_planar(b, nSeg, coord, uVec, end1, bTransGam);
+        link_force_one(A, B, loc1[k], loc2[k], filLinkCt[a], filLinkCt[b], lp, pay);
+    }
     float *d = xlinkData + (size_t)k * XL_STRIDE;
 #pragma unroll
     for (int j = 0; j < 12; j++) d[j] = pay[j];
@@ -98,7 +101,9 @@ __global__ void __launch_bounds__(256) k_link_torsion(const float *__restrict__ 
     float pay[12];
 #pragma unroll
     for (int j = 0; j < 12; j++) pay[j] = d[j];
-    link_torsion_one(k, nSeg, a, b, linkOrientSame[k] != 0, uVec, torqueMagHist, torqueMagPlace, lp, pay);
+    LinkEnd A = link_end_planar(a, nSeg, nullptr, uVec, uVec, nullptr);
+    LinkEnd B = link_end_planar(b, nSeg, nullptr, uVec, uVec, nullptr);
+    link_torsion_one(k, A, B, linkOrientSame[k] != 0, torqueMagHist, torqueMagPlace, lp, pay);
     d[3] = pay[3]; d[4] = pay[4]; d[5] = pay[5];
     d[9] = pay[9]; d[10] = pay[10]; d[11] = pay[11];
 }

@@ -51,7 +51,10 @@ struct sbx_net {
     int *bodyCell = nullptr, *cellOffsets = nullptr, *cellContents = nullptr, *candPartner = nullptr,
         *candCount = nullptr;
     SortScratch sort;
-    int cellsValid = 0;  // bodyCell holds the cells of the current pose (written by the last k_rod)
+    int derivedValid = 0;  // bodyCell and linkPack describe the current pose (written by the last k_rod)
+    float4 *linkPack = nullptr;  // per-rod {c, gpar} {u} {end1} for the crosslink gathers (links only)
+    int *cellNext = nullptr;  // k_rod bins the new pose here; swapped with bodyCell after each step
+    int *lastCell = nullptr;  // the cells the latest step's grid used (what SpatialGrid.bodyCell left)
     // links
     int *offA = nullptr, *idxA = nullptr, *offB = nullptr, *idxB = nullptr;
     LinkParams lp{};
@@ -109,6 +112,7 @@ struct RodArgs {
     int hasMotors;
     int *activeLinkCount;   // written from the gather's active-link tally (may be null)
     int *nextCell;          // bodyCell of the new pose for the next step's grid (null: not fused)
+    float4 *linkPack;       // packed endpoint data of the new pose for the next step's links (or null)
     GridGeom geom;
     const int *dStep;
     int stepOff;
@@ -203,9 +207,28 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         a.end2[k * n + i] = d.e2[k];
     }
     if (a.nextCell) a.nextCell[i] = cell_of(p.c[0], p.c[1], p.c[2], a.geom);
+    if (a.linkPack) {
+        float4 *q = a.linkPack + LINK_PACK * i;
+        q[0] = make_float4(p.c[0], p.c[1], p.c[2], gT[0]);
+        q[1] = make_float4(p.u[0], p.u[1], p.u[2], 0.0f);
+        q[2] = make_float4(d.e1[0], d.e1[1], d.e1[2], 0.0f);
+    }
 }
 
-__global__ void __launch_bounds__(256) k_net_unbind(const float *__restrict__ uVec, const float *__restrict__ end1,
+// linkPack of the caller's pose (before the first step and after sbx_net_publish)
+__global__ void __launch_bounds__(256) k_pack_links(const float *__restrict__ coord, const float *__restrict__ uVec,
+                                                    const float *__restrict__ end1,
+                                                    const float *__restrict__ bTransGam, float4 *__restrict__ pack,
+                                                    int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float4 *q = pack + LINK_PACK * i;
+    q[0] = make_float4(coord[i], coord[n + i], coord[2 * n + i], bTransGam[i]);
+    q[1] = make_float4(uVec[i], uVec[n + i], uVec[2 * n + i], 0.0f);
+    q[2] = make_float4(end1[i], end1[n + i], end1[2 * n + i], 0.0f);
+}
+
+__global__ void __launch_bounds__(256) k_net_unbind(const float4 *__restrict__ pack,
                                                     const int *__restrict__ linkFilA,
                                                     const int *__restrict__ linkFilB, const float *__restrict__ loc1,
                                                     const float *__restrict__ loc2, int *__restrict__ linkState,
@@ -217,14 +240,13 @@ __global__ void __launch_bounds__(256) k_net_unbind(const float *__restrict__ uV
     int st = linkState[k];
     if (st < 0) return;
     int step = *dStep + stepOff;
-    int ns = link_unbind_one(k, nSeg, linkFilA[k], linkFilB[k], loc1[k], loc2[k], uVec, end1, strainHist,
-                             strainPlace, lp, step, seed, rng, stream, st);
+    LinkEnd A = link_end_packed(linkFilA[k], pack), B = link_end_packed(linkFilB[k], pack);
+    int ns = link_unbind_one(k, A, B, loc1[k], loc2[k], strainHist, strainPlace, lp, step, seed, rng, stream, st);
     if (ns != st) linkState[k] = ns;
 }
 
 // linkForces + linkTorsion for one link (CrosslinkerSystem.java:62-123, 96-128)
-__global__ void __launch_bounds__(256) k_net_link(const float *__restrict__ coord, const float *__restrict__ uVec,
-                                                  const float *__restrict__ end1, const float *__restrict__ bTransGam,
+__global__ void __launch_bounds__(256) k_net_link(const float4 *__restrict__ pack,
                                                   const int *__restrict__ linkFilA, const int *__restrict__ linkFilB,
                                                   const float *__restrict__ loc1, const float *__restrict__ loc2,
                                                   const int *__restrict__ offA, const int *__restrict__ idxA,
@@ -242,9 +264,10 @@ __global__ void __launch_bounds__(256) k_net_link(const float *__restrict__ coor
     if (a >= 0 && b >= 0) {
         int ctA = active_links(a, offA, idxA, offB, idxB, linkState);
         int ctB = active_links(b, offA, idxA, offB, idxB, linkState);
-        link_force_one(nSeg, a, b, loc1[k], loc2[k], coord, uVec, end1, bTransGam, ctA, ctB, lp, pay);
+        LinkEnd A = link_end_packed(a, pack), B = link_end_packed(b, pack);
+        link_force_one(A, B, loc1[k], loc2[k], ctA, ctB, lp, pay);
         if (lp.torsionOn && linkState[k] >= 0)
-            link_torsion_one(k, nSeg, a, b, linkOrientSame[k] != 0, uVec, torqueMagHist, torqueMagPlace, lp, pay);
+            link_torsion_one(k, A, B, linkOrientSame[k] != 0, torqueMagHist, torqueMagPlace, lp, pay);
     }
     float *d = xlinkData + (size_t)k * XL_STRIDE;
 #pragma unroll
@@ -651,12 +674,7 @@ int enqueue_step(sbx_net *net, int stepOff) {
     if (net->hasMotors) {
         if ((r = enqueue_myosin(net, coord, uVec, yVec, stepOff))) return r;
     } else if (net->p.gridOn) {
-        if (!net->cellsValid) {  // otherwise the previous k_rod binned the current pose
-            Scope sc(net, T_BIN);
-            r = launch_body_cell(c, coord, n, n, net->geom, net->bodyCell);
-            if (r) return r;
-            net->cellsValid = 1;
-        }
+        if (!net->derivedValid) return fail(c, SBX_ERR_STATE, "net step: derived pose data not prepared");
         {
             Scope sc(net, T_SORT);
             r = counting_sort(c, net->sort, net->bodyCell, n, net->p.gridDims[3], n, net->cellOffsets,
@@ -673,15 +691,15 @@ int enqueue_step(sbx_net *net, int stepOff) {
     if (L > 0) {
         if (net->p.unbindOn) {
             Scope sc(net, T_UNBIND);
-            hipLaunchKernelGGL(k_net_unbind, dim3(blocks_for(L, 256)), dim3(256), 0, s, uVec, net->fil.end1,
+            hipLaunchKernelGGL(k_net_unbind, dim3(blocks_for(L, 256)), dim3(256), 0, s, net->linkPack,
                                net->xl.linkFilA, net->xl.linkFilB, net->xl.loc1, net->xl.loc2, net->xl.linkState,
                                net->xl.strainHist, net->xl.strainPlace, net->lp, net->dStep, stepOff, net->p.seed,
                                net->p.rng, net->p.stream, n, L);
         }
         {
             Scope sc(net, T_LINK);
-            hipLaunchKernelGGL(k_net_link, dim3(blocks_for(L, 256)), dim3(256), 0, s, coord, uVec, net->fil.end1,
-                               net->fil.bTransGam, net->xl.linkFilA, net->xl.linkFilB, net->xl.loc1, net->xl.loc2,
+            hipLaunchKernelGGL(k_net_link, dim3(blocks_for(L, 256)), dim3(256), 0, s, net->linkPack,
+                               net->xl.linkFilA, net->xl.linkFilB, net->xl.loc1, net->xl.loc2,
                                net->offA, net->idxA, net->offB, net->idxB, net->xl.linkState,
                                net->xl.linkOrientSame, net->xl.torqueMagHist,
                                net->xl.torqueMagPlace, net->xl.xlinkData, net->lp, n, L);
@@ -704,7 +722,8 @@ int enqueue_step(sbx_net *net, int stepOff) {
     a.hasMotors = net->hasMotors;
     a.activeLinkCount = net->xl.activeLinkCount;
     a.geom = net->geom;
-    a.nextCell = (net->p.gridOn && !net->hasMotors) ? net->bodyCell : nullptr;
+    a.nextCell = (net->p.gridOn && !net->hasMotors) ? net->cellNext : nullptr;
+    a.linkPack = L > 0 ? net->linkPack : nullptr;
     a.mOff = net->mOff; a.mList = net->mList; a.bondData = net->mo.bondData;
     if (net->hasMotors) {
         Scope sc(net, T_MCSR);
@@ -726,6 +745,12 @@ int enqueue_step(sbx_net *net, int stepOff) {
         hipLaunchKernelGGL(k_rod, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, net->cp);
     }
     net->cur = nxt;
+    net->lastCell = net->bodyCell;
+    if (a.nextCell) {  // the next step's grid reads the cells k_rod just wrote
+        int *t = net->bodyCell;
+        net->bodyCell = net->cellNext;
+        net->cellNext = t;
+    }
     return check_launch(c, "k_rod");
 }
 
@@ -742,7 +767,8 @@ void release(sbx_net *net) {
     if (net->ctx) hipStreamSynchronize(net->ctx->stream);
     if (net->graphExec) hipGraphExecDestroy(net->graphExec);
     if (net->graph) hipGraphDestroy(net->graph);
-    void *bufs[] = {net->ownPose,  net->boundR,    net->bodyCell, net->cellOffsets, net->cellContents,
+    void *bufs[] = {net->ownPose,  net->boundR,    net->bodyCell, net->cellNext,    net->cellContents, net->linkPack,
+                    net->cellOffsets,
                     net->candPartner, net->candCount, net->offA,  net->idxA,        net->offB,
                     net->idxB,     net->dStep,     net->ownerStore, net->ownerSlot,
                     net->mOff,     net->mList,     net->bbOff,    net->bbList};
@@ -824,6 +850,7 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
             net->geom = grid_geom(p->gridParams, p->gridDims, p->periodic, p->periodic ? p->boxLen : nullptr);
             if ((r = alloc(c, &net->boundR, n))) break;
             if ((r = alloc(c, &net->bodyCell, n))) break;
+            if ((r = alloc(c, &net->cellNext, n))) break;
             if ((r = alloc(c, &net->cellOffsets, (size_t)p->gridDims[3] + 1))) break;
             if ((r = alloc(c, &net->cellContents, n))) break;
             if ((r = alloc(c, &net->candPartner, (size_t)n * p->maxCand))) break;
@@ -834,6 +861,7 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
         }
         if (L > 0) {
             if ((r = alloc(c, &net->offA, (size_t)n + 1))) break;
+            if ((r = alloc(c, &net->linkPack, (size_t)LINK_PACK * n))) break;
             if ((r = alloc(c, &net->offB, (size_t)n + 1))) break;
             if ((r = alloc(c, &net->idxA, L))) break;
             if ((r = alloc(c, &net->idxB, L))) break;
@@ -921,6 +949,7 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
     hipFree(net->bodyCell);
     hipFree(net->cellContents);
     net->bodyCell = bodyCell;
+    net->lastCell = nullptr;
     net->cellContents = cellContents;
     net->ownerStore = ownerStore;
     net->ownerSlot = ownerSlot;
@@ -970,9 +999,17 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
     // Eager launches use stepOff = done - advanced, where `advanced` counts the steps already added
     // to the device counter by graph replays (each replay ends with k_add_step(G)).
     int done = 0, advanced = 0;
-    if (net->p.gridOn && !net->hasMotors && !net->cellsValid) {  // bin the caller's pose once, eagerly
-        if ((r = launch_body_cell(c, net->coordB[net->cur], net->n, net->n, net->geom, net->bodyCell))) return r;
-        net->cellsValid = 1;
+    if (!net->derivedValid) {  // bin and pack the caller's pose once, eagerly (k_rod keeps them after)
+        if (net->p.gridOn && !net->hasMotors &&
+            (r = launch_body_cell(c, net->coordB[net->cur], net->n, net->n, net->geom, net->bodyCell)))
+            return r;
+        if (net->L > 0) {
+            hipLaunchKernelGGL(k_pack_links, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream,
+                               net->coordB[net->cur], net->uB[net->cur], net->fil.end1, net->fil.bTransGam,
+                               net->linkPack, net->n);
+            if ((r = check_launch(c, "k_pack_links"))) return r;
+        }
+        net->derivedValid = 1;
     }
     if (net->useGraph && !net->profiling) {
         // one captured graph = 2 steps (the pose ping-pong returns to its start buffers) + step advance
@@ -1034,7 +1071,7 @@ int sbx_net_publish(sbx_net *net) {
     if (!r) r = check_hip(c, hipMemcpyAsync(net->yB[0], net->yB[1], bytes, hipMemcpyDeviceToDevice, c->stream),
                           "publish yVec");
     if (!r) net->cur = 0;
-    net->cellsValid = 0;  // the caller may now edit its pose arrays
+    net->derivedValid = 0;  // the caller may now edit its pose arrays
     return r;
 }
 
@@ -1052,7 +1089,7 @@ int sbx_net_candidates(sbx_net *net, int **candPartner, int **candCount, int **g
     if (candCount) *candCount = net->candCount;
     if (gridCellOffsets) *gridCellOffsets = net->cellOffsets;
     if (gridCellContents) *gridCellContents = net->cellContents;
-    if (bodyCell) *bodyCell = net->bodyCell;
+    if (bodyCell) *bodyCell = net->lastCell ? net->lastCell : net->bodyCell;
     return SBX_OK;
 }
 
