@@ -314,6 +314,59 @@ typedef struct sbx_motor_params {
 
 int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_params *p);
 
+/* ------------------------------------------------------------------ skin broad phase
+ * No counterpart in the reference (SpatialGrid.broadPhase, SpatialGrid.java:340-404, rebuilds the grid
+ * every step).  The candidate slices candPartner/candCount stay identical to the per-step rebuild's:
+ * per-body neighbour lists are built with reach + skin_i + skin_j and filtered exactly every step, and
+ * an exact displacement guard rebuilds whenever any body moved more than its skin.  The skin only
+ * changes the cost.  While a skin mode is on, the fine grid CSR (gridCellOffsets/Contents) is not
+ * rebuilt; bodyCell still is. */
+#define SBX_SKIN_REBUILD 0   /* the reference's per-step grid rebuild + broad phase */
+#define SBX_SKIN_FIXED 1     /* every body gets skinDefault */
+#define SBX_SKIN_LEARNED 2   /* per coarse cell from the skin model (sbx_mlp_*), clamped to [skinMin, skinMax] */
+typedef struct sbx_skin_params {
+    float skinDefault;   /* um; fixed mode, and the first build of learned mode */
+    float skinMin, skinMax;
+    float margin;        /* learned skin = margin * exp(model out0) */
+    int listCap;         /* neighbour slots per body (overflowing bodies fall back to an exact coarse walk) */
+} sbx_skin_params;
+/* modelWeights: device array of sbx_mlp_nparam() floats (learned mode; copied) */
+int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *p, const float *modelWeights);
+/* device counters -> out[0..cap): rebuild pending, steps since rebuild, rebuilds, overflowing bodies,
+ * list entries at the last build, skin-mode steps; returns the number written */
+int sbx_net_skin_stats(sbx_net *net, long long *out, int cap);
+/* device state of the last rebuild (for training): reference centres, skins, coarse cells and CSR;
+ * coarseGrid[8] = xMin, yMin, zMin, cell, 1/cell, nX, nY, nZ */
+int sbx_net_skin_state(sbx_net *net, float **refCoord, float **skin, int **coarseCell, int **coarseOffsets,
+                       int **coarseContents, int *nCoarse, float *coarseGrid);
+
+/* ------------------------------------------------------------------ skin model (MLP 12 -> 64 -> 64 -> 2, GELU)
+ * No counterpart in the reference.  Parameters: one device float array of sbx_mlp_nparam() (layout in
+ * softbox_amd/csrc/sbx_mlp.h); x[b*12 + k], y[b*2 + o], target like y.  Loss = mean over samples of
+ * sum_o 0.5 * outWeight[o] * (y - target)^2; grad has the same layout as the parameters (the trailing
+ * input-standardisation entries are not trained). */
+/* training data of the skin model from a trajectory (no counterpart in the reference): per coarse cell
+ * features[c*12 + k] of its members' motion over H steps (coordPrev -> coordNow); per-body running
+ * max displacement from x0 and first step beyond a per-body threshold; per-cell max (mode 0, float)
+ * or min (mode 1, int >= 0; `fill` when none) over members; per-cell value copied to its members. */
+int sbx_skin_features(sbx_ctx *c, const int *coarseOffsets, const int *coarseContents, int nCells, int n,
+                      const float *coordNow, const float *coordPrev, const float *uNow, const float *uPrev,
+                      const int *activeLinkCount, float H, int periodic, const float *boxLen /* host [3] */,
+                      float *features);
+int sbx_skin_track(sbx_ctx *c, const float *coord, const float *x0, int n, int periodic, const float *boxLen,
+                   float *maxDisp, const float *threshold, int *breachStep, int step);
+int sbx_skin_cell_reduce(sbx_ctx *c, const int *coarseOffsets, const int *coarseContents, int nCells,
+                         const float *bodyFloat, const int *bodyInt, int mode, float fill, float *cellOut);
+int sbx_skin_cell_scatter(sbx_ctx *c, const int *coarseOffsets, const int *coarseContents, int nCells,
+                          const float *cellVal, float *bodyVal);
+int sbx_mlp_nparam(void);
+int sbx_mlp_forward(sbx_ctx *c, const float *weights, const float *x, int nSamples, float *y);
+size_t sbx_mlp_scratch_floats(int nSamples);
+int sbx_mlp_loss_grad(sbx_ctx *c, const float *weights, const float *x, const float *target, int nSamples,
+                      const float *outWeight /* host [2] */, float *scratch, float *grad, float *loss /* device [1] */);
+int sbx_mlp_adam(sbx_ctx *c, float *weights, const float *grad, float *m, float *v, int t, float lr, float beta1,
+                 float beta2, float eps);
+
 int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *xl, const sbx_net_params *p,
                    sbx_net **out);
 void sbx_net_destroy(sbx_net *net);

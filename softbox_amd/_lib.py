@@ -107,6 +107,16 @@ class SbxMotorParams(C.Structure):
     ]
 
 
+SBX_SKIN_REBUILD = 0
+SBX_SKIN_FIXED = 1
+SBX_SKIN_LEARNED = 2
+
+
+class SbxSkinParams(C.Structure):
+    _fields_ = [("skinDefault", C.c_float), ("skinMin", C.c_float), ("skinMax", C.c_float), ("margin", C.c_float),
+                ("listCap", i32)]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPES)
 _SIGS = {
     "sbx_create": [C.POINTER(vp), i32, vp],
@@ -164,9 +174,21 @@ _SIGS = {
     "sbx_net_set_profiling": [vp, i32],
     "sbx_net_kernel_times": [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_longlong), i32],
     "sbx_net_use_graph": [vp, i32],
+    "sbx_net_set_skin": [vp, i32, C.POINTER(SbxSkinParams), vp],
+    "sbx_net_skin_stats": [vp, C.POINTER(C.c_longlong), i32],
+    "sbx_net_skin_state": [vp] + [C.POINTER(vp)] * 5 + [C.POINTER(i32), f32p],
+    "sbx_skin_features": [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, C.c_float, i32, f32p, vp],
+    "sbx_skin_track": [vp, vp, vp, i32, i32, f32p, vp, vp, vp, i32],
+    "sbx_skin_cell_reduce": [vp, vp, vp, i32, vp, vp, i32, C.c_float, vp],
+    "sbx_skin_cell_scatter": [vp, vp, vp, i32, vp, vp],
+    "sbx_mlp_nparam": [],
+    "sbx_mlp_forward": [vp, vp, vp, i32, vp],
+    "sbx_mlp_scratch_floats": [i32],
+    "sbx_mlp_loss_grad": [vp, vp, vp, vp, i32, f32p, vp, vp, vp],
+    "sbx_mlp_adam": [vp, vp, vp, vp, vp, i32, C.c_float, C.c_float, C.c_float, C.c_float],
 }
 _RESTYPES = {"sbx_destroy": None, "sbx_last_error": C.c_char_p, "sbx_clear_error": None,
-             "sbx_version": C.c_char_p, "sbx_net_destroy": None}
+             "sbx_version": C.c_char_p, "sbx_net_destroy": None, "sbx_mlp_scratch_floats": C.c_size_t}
 
 EXPORTED = tuple(_SIGS)
 

@@ -96,7 +96,9 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *total) {
 // when >= 0.
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int *__restrict__ in, int n,
                                                              const int *__restrict__ tileSums, int nTiles,
-                                                             int *__restrict__ out, int clampTotal) {
+                                                             int *__restrict__ out, int clampTotal,
+                                                             const int *__restrict__ gate) {
+    if (gate && *gate == 0) return;
     __shared__ int red[SCAN_THREADS / 64];
     const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -205,7 +207,9 @@ void sort_scratch_free(SortScratch &s) {
 // 5. k_key_place:   rank of m among its key's segment (items with a smaller index) -> list.
 // The result is independent of the atomic order: ascending item index within each key.
 __global__ void __launch_bounds__(256) k_key_count(const int *__restrict__ key, int nItems, int nKeys,
-                                                   int *__restrict__ counts, int *__restrict__ slot) {
+                                                   int *__restrict__ counts, int *__restrict__ slot,
+                                                   const int *__restrict__ gate) {
+    if (gate && *gate == 0) return;
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nItems) return;
     int k = key[m];
@@ -216,7 +220,9 @@ __global__ void __launch_bounds__(256) k_key_count(const int *__restrict__ key, 
 // tileSums[t] = sum of tile t of counts (one block per tile; no atomics: device-scope atomics on a
 // few hot addresses serialise at the memory side)
 __global__ void __launch_bounds__(SCAN_THREADS) k_tile_reduce(const int *__restrict__ in, int n,
-                                                              int *__restrict__ tileSums) {
+                                                              int *__restrict__ tileSums,
+                                                              const int *__restrict__ gate) {
+    if (gate && *gate == 0) return;
     __shared__ int red[SCAN_THREADS / 64];
     long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
     int sum = 0;
@@ -239,7 +245,9 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_tile_reduce(const int *__restr
 
 __global__ void __launch_bounds__(256) k_key_scatter(const int *__restrict__ key, int nItems, int nKeys,
                                                      const int *__restrict__ offsets, const int *__restrict__ slot,
-                                                     int *__restrict__ counts, int *__restrict__ tmp, int contentsCap) {
+                                                     int *__restrict__ counts, int *__restrict__ tmp, int contentsCap,
+                                                     const int *__restrict__ gate) {
+    if (gate && *gate == 0) return;
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nItems) return;
     int k = key[m];
@@ -251,7 +259,9 @@ __global__ void __launch_bounds__(256) k_key_scatter(const int *__restrict__ key
 
 __global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, int nItems, int nKeys,
                                                    const int *__restrict__ offsets, const int *__restrict__ tmp,
-                                                   int *__restrict__ list, int contentsCap) {
+                                                   int *__restrict__ list, int contentsCap,
+                                                   const int *__restrict__ gate) {
+    if (gate && *gate == 0) return;
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nItems) return;
     int k = key[m];
@@ -266,23 +276,23 @@ __global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, 
 }
 
 int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
-                  int *list) {
+                  int *list, const int *gate) {
     int r = sort_scratch_reserve(c, s, nKeys, nItems);
     if (r) return r;
     int nt = (nKeys + SCAN_TILE - 1) / SCAN_TILE;
     if (nt < 1) nt = 1;
     if (nItems > 0) {
         hipLaunchKernelGGL(k_key_count, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems, nKeys,
-                           s.counts, s.slot);
+                           s.counts, s.slot, gate);
     }
-    hipLaunchKernelGGL(k_tile_reduce, dim3(nt), dim3(SCAN_THREADS), 0, c->stream, s.counts, nKeys, s.tileSums);
+    hipLaunchKernelGGL(k_tile_reduce, dim3(nt), dim3(SCAN_THREADS), 0, c->stream, s.counts, nKeys, s.tileSums, gate);
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, c->stream, s.counts, nKeys, s.tileSums, nt,
-                       offsets, contentsCap);
+                       offsets, contentsCap, gate);
     if (nItems > 0) {
         hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems, nKeys,
-                           offsets, s.slot, s.counts, s.tmp, contentsCap);
+                           offsets, s.slot, s.counts, s.tmp, contentsCap, gate);
         hipLaunchKernelGGL(k_key_place, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems, nKeys,
-                           offsets, s.tmp, list, contentsCap);
+                           offsets, s.tmp, list, contentsCap, gate);
     }
     r = check_launch(c, "counting_sort");
     if (r) s.clean = false;

@@ -55,8 +55,10 @@ int sort_scratch_reserve(sbx_ctx *c, SortScratch &s, long long nKeys, long long 
 void sort_scratch_free(SortScratch &s);
 // Stable counting sort of items by integer key (key < 0 skipped) into offsets[nKeys+1] and
 // list[...]; offsets[nKeys] is the total clamped to contentsCap; positions >= contentsCap are dropped.
+// gate (device int, may be null): when it reads 0 every kernel of the sort returns at once (a
+// rebuild the device decides on, inside a captured step).
 int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
-                  int *list);
+                  int *list, const int *gate = nullptr);
 int fill_i32(sbx_ctx *c, int *dst, int value, long long count);
 
 ChainParams chain_params(const float *cp);

@@ -21,7 +21,9 @@
 #include "sbx_grid.h"
 #include "sbx_internal.h"
 #include "sbx_links.h"
+#include "sbx_mlp.h"
 #include "sbx_motor.h"
+#include "sbx_skin.h"
 
 using namespace sbx;
 
@@ -80,6 +82,18 @@ struct sbx_net {
     DimerParams dp{};
     MiniParams mip{};
     ReachGrid rg{};
+    // skin broad phase (sbx_net_set_skin)
+    int skinMode = 0;                       // SBX_SKIN_REBUILD / _FIXED / _LEARNED
+    sbx_skin_params sp{};
+    GridGeom cgeom{};                       // coarse rebuild grid (cell >= largest list reach)
+    int cCells = 0;
+    int *cCell = nullptr, *cOff = nullptr, *cCont = nullptr;  // coarse CSR of the last rebuild
+    float *refC = nullptr, *refU = nullptr, *skin = nullptr;   // pose and skins at the last rebuild
+    float *cellSkin = nullptr, *cellHorizon = nullptr;
+    int *vlist = nullptr, *vcount = nullptr;
+    int *skinFlags = nullptr;
+    float *model = nullptr;                 // MLP_NPARAM device floats (learned mode)
+    float meanOcc = 1.0f;
     // graph
     int useGraph = 0;
     int graphSteps = 0;  // non-zero while a capture is being recorded
@@ -92,12 +106,12 @@ namespace {
 
 enum {
     T_BIN = 0, T_SORT, T_PAIRS, T_UNBIND, T_COUNT, T_LINK, T_ROD, T_MBIND, T_MJOINT, T_DIMER, T_TETHER, T_BACKBONE,
-    T_BOND, T_MINTEG, T_MCSR, T_COUNT_TIMERS
+    T_BOND, T_MINTEG, T_MCSR, T_SKIN_REBUILD, T_COUNT_TIMERS
 };
 const char *kTimerNames[T_COUNT_TIMERS] = {"k_body_cell", "grid_sort",     "k_broad_phase", "k_net_unbind",
                                            "k_net_count", "k_net_link",    "k_rod",         "k_motor_bind",
                                            "k_motor_joints", "k_net_dimer", "k_net_tether", "k_backbone",
-                                           "k_bond_head", "k_motor_integ", "motor_csr"};
+                                           "k_bond_head", "k_motor_integ", "motor_csr", "skin_rebuild"};
 
 struct RodArgs {
     const float *coord, *uVec, *yVec;
@@ -113,6 +127,8 @@ struct RodArgs {
     int *activeLinkCount;   // written from the gather's active-link tally (may be null)
     int *nextCell;          // bodyCell of the new pose for the next step's grid (null: not fused)
     float4 *linkPack;       // packed endpoint data of the new pose for the next step's links (or null)
+    const float *skinRef, *skin;  // skin guard: centre at the last rebuild and per-body skin (or null)
+    int *skinFlags;
     GridGeom geom;
     const int *dStep;
     int stepOff;
@@ -207,6 +223,16 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         a.end2[k * n + i] = d.e2[k];
     }
     if (a.nextCell) a.nextCell[i] = cell_of(p.c[0], p.c[1], p.c[2], a.geom);
+    if (a.skinRef) {  // exact displacement guard: any body beyond its skin forces the next rebuild
+        float dx = p.c[0] - a.skinRef[i], dy = p.c[1] - a.skinRef[n + i], dz = p.c[2] - a.skinRef[2 * n + i];
+        if (a.geom.periodic) {
+            dx = min_image(dx, a.geom.Lx, a.geom.iLx);
+            dy = min_image(dy, a.geom.Ly, a.geom.iLy);
+            dz = min_image(dz, a.geom.Lz, a.geom.iLz);
+        }
+        float sk = a.skin[i];
+        if (dx * dx + dy * dy + dz * dz > sk * sk) a.skinFlags[SKF_REBUILD] = 1;
+    }
     if (a.linkPack) {
         float4 *q = a.linkPack + LINK_PACK * i;
         q[0] = make_float4(p.c[0], p.c[1], p.c[2], gT[0]);
@@ -541,6 +567,265 @@ __global__ void __launch_bounds__(256) k_motor_integ(sbx_motors m, float dt, flo
     }
 }
 
+// ---------------------------------------------------------------- skin broad phase kernels
+// Every kernel but k_skin_pairs is gated on the device flag SKF_REBUILD, so a captured step
+// rebuilds exactly when the guard (in k_rod) fired, with no host round trip.
+struct SkinArgs {
+    int n, mode, listCap, maxCand;
+    const float *coord, *uVec, *boundR;
+    const int *links;        // activeLinkCount (may be null)
+    GridGeom geom, cgeom;    // fine (the broad phase's) and coarse (rebuild) grids
+    int cCells;
+    int *cCell, *cOff, *cCont;
+    float *refC, *refU, *skin, *cellSkin, *cellHorizon;
+    int *vlist, *vcount;
+    int *flags;
+    const float *model;
+    float skinDefault, skinMin, skinMax, margin, meanOcc;
+    const int *bodyCell;     // fine cells of the current pose
+    int *candPartner, *candCount;
+};
+
+// Per coarse cell of the last rebuild: the model's skin for its members (learned mode, after the
+// first rebuild); otherwise the default skin.
+__global__ void __launch_bounds__(256) k_skin_model(SkinArgs a) {
+    if (a.flags[SKF_REBUILD] == 0) return;
+    int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= a.cCells) return;
+    float sk = a.skinDefault, hz = 0.0f;
+    if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && a.model) {
+        float f[SKIN_FEATURES], y[MLP_OUT];
+        float H = (float)(a.flags[SKF_SINCE] > 0 ? a.flags[SKF_SINCE] : 1);
+        skin_cell_features(c, a.cOff, a.cCont, a.n, a.coord, a.refC, a.uVec, a.refU, a.links, H, a.meanOcc, a.geom,
+                           f);
+        mlp_forward(a.model, f, y);
+        sk = fminf(a.skinMax, fmaxf(a.skinMin, a.margin * expf(y[0])));
+        hz = expf(y[1]);
+    }
+    a.cellSkin[c] = sk;
+    a.cellHorizon[c] = hz;
+}
+
+// per body: skin from its coarse cell of the last rebuild, then its coarse cell now
+__global__ void __launch_bounds__(256) k_skin_assign_bin(SkinArgs a) {
+    if (a.flags[SKF_REBUILD] == 0) return;
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    float sk = a.skinDefault;
+    if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && a.model) sk = a.cellSkin[a.cCell[i]];
+    a.skin[i] = sk;
+    const int n = a.n;
+    a.cCell[i] = cell_of(a.coord[i], a.coord[n + i], a.coord[2 * n + i], a.cgeom);
+}
+
+// Neighbour lists: every j > i within ri + rj + cutoff + s_i + s_j (+1 nm for float rounding) of i,
+// from the 27 coarse cells around i (the coarse cell is at least the largest such reach).  Also
+// records the pose the guard measures displacement from.
+constexpr float SKIN_SLACK = 1.0e-3f;  // um
+__global__ void __launch_bounds__(256) k_skin_build(SkinArgs a) {
+    if (a.flags[SKF_REBUILD] == 0) return;
+    int t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const int n = a.n;
+    if (t >= n) return;
+    int i = a.cCont[t];  // coarse-cell order: neighbouring lanes walk neighbouring cells
+    const GridGeom &g = a.cgeom;
+    float icx = a.coord[i], icy = a.coord[n + i], icz = a.coord[2 * n + i];
+    float ri = a.boundR[i], si = a.skin[i];
+    int c = a.cCell[i];
+    const int nXY = g.nX * g.nY;
+    int cz = c / nXY, rem = c - cz * nXY, cy = rem / g.nX, cx = rem - cy * g.nX;
+    int cnt = 0;
+    int *list = a.vlist + (size_t)i * a.listCap;
+    for (int dz = -1; dz <= 1; dz++) {
+        int zz = cz + dz;
+        if (g.periodic) zz = zz < 0 ? zz + g.nZ : (zz >= g.nZ ? zz - g.nZ : zz);
+        else if (zz < 0 || zz >= g.nZ) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            int yy = cy + dy;
+            if (g.periodic) yy = yy < 0 ? yy + g.nY : (yy >= g.nY ? yy - g.nY : yy);
+            else if (yy < 0 || yy >= g.nY) continue;
+            for (int dx = -1; dx <= 1; dx++) {
+                int xx = cx + dx;
+                if (g.periodic) xx = xx < 0 ? xx + g.nX : (xx >= g.nX ? xx - g.nX : xx);
+                else if (xx < 0 || xx >= g.nX) continue;
+                int cc = xx + yy * g.nX + zz * nXY;
+                for (int p = a.cOff[cc]; p < a.cOff[cc + 1]; p++) {
+                    int j = a.cCont[p];
+                    if (j <= i) continue;
+                    float ddx = icx - a.coord[j], ddy = icy - a.coord[n + j], ddz = icz - a.coord[2 * n + j];
+                    if (g.periodic) {
+                        ddx = min_image(ddx, g.Lx, g.iLx);
+                        ddy = min_image(ddy, g.Ly, g.iLy);
+                        ddz = min_image(ddz, g.Lz, g.iLz);
+                    }
+                    float R = ri + a.boundR[j] + a.geom.cutoff + si + a.skin[j] + SKIN_SLACK;
+                    if (ddx * ddx + ddy * ddy + ddz * ddz <= R * R) {
+                        if (cnt < a.listCap) list[cnt] = j;
+                        cnt++;
+                    }
+                }
+            }
+        }
+    }
+    a.vcount[i] = cnt;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        a.refC[k * n + i] = a.coord[k * n + i];
+        a.refU[k * n + i] = a.uVec[k * n + i];
+    }
+    if (cnt > a.listCap) atomicAdd(&a.flags[SKF_OVERFLOW], 1);
+    atomicAdd(&a.flags[SKF_ENTRIES], cnt < a.listCap ? cnt : a.listCap);
+}
+
+// after the model and the binning (which read SINCE and REBUILDS), before the build: the
+// counters of the new rebuild
+__global__ void k_skin_begin(int *flags) {
+    if (flags[SKF_REBUILD] == 0) return;
+    flags[SKF_OVERFLOW] = 0;
+    flags[SKF_ENTRIES] = 0;
+    flags[SKF_SINCE] = 0;
+    flags[SKF_REBUILDS] += 1;
+}
+
+__global__ void __launch_bounds__(256) k_max_radius(const float *__restrict__ r, int n, int *__restrict__ out) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n && r[i] > 0.0f) atomicMax(out, __float_as_int(r[i]));  // positive floats order as ints
+}
+
+// One body's candidate test (the broad phase's predicate, same arithmetic) and its sort key
+// (fine stencil position, index); -1 when j is not a candidate of SpatialGrid.broadPhase.
+__device__ __forceinline__ int skin_key(int i, int j, int ci, float icx, float icy, float icz, float ri,
+                                        const SkinArgs &a) {
+    const int n = a.n;
+    const GridGeom &g = a.geom;
+    float dx = icx - a.coord[j], dy = icy - a.coord[n + j], dz = icz - a.coord[2 * n + j];
+    if (g.periodic) {
+        dx = min_image(dx, g.Lx, g.iLx);
+        dy = min_image(dy, g.Ly, g.iLy);
+        dz = min_image(dz, g.Lz, g.iLz);
+    }
+    float distSq = dx * dx + dy * dy + dz * dz;
+    float reach = ri + a.boundR[j] + g.cutoff;
+    if (!(distSq <= reach * reach)) return -1;
+    int pos = stencil_pos(ci, a.bodyCell[j], g);
+    if (pos < 0) return -1;
+    return (pos << 26) | j;
+}
+
+// Candidate slices of the current pose from the neighbour lists (or, for a body whose list
+// overflowed, from the 27 coarse cells of the last rebuild), in the broad phase's order.
+__global__ void __launch_bounds__(256) k_skin_pairs(SkinArgs a) {
+    int t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // no other thread reads these words in this kernel
+        a.flags[SKF_REBUILD] = 0;
+        a.flags[SKF_SINCE] += 1;
+        a.flags[SKF_STEPS] += 1;
+    }
+    const int n = a.n;
+    if (t >= n) return;
+    int i = a.cCont[t];
+    float icx = a.coord[i], icy = a.coord[n + i], icz = a.coord[2 * n + i];
+    float ri = a.boundR[i];
+    int ci = a.bodyCell[i];
+    int *slice = a.candPartner + (size_t)i * a.maxCand;
+    int m = 0;     // candidates found (may exceed maxCand)
+    int cnt = a.vcount[i];
+    if (cnt <= a.listCap) {
+        const int *list = a.vlist + (size_t)i * a.listCap;
+        for (int k = 0; k < cnt; k++) {
+            int key = skin_key(i, list[k], ci, icx, icy, icz, ri, a);
+            if (key < 0) continue;
+            if (m < a.maxCand) slice[m] = key;
+            m++;
+        }
+    } else {
+        const GridGeom &g = a.cgeom;
+        const int nXY = g.nX * g.nY;
+        int c = a.cCell[i];
+        int cz = c / nXY, rem = c - cz * nXY, cy = rem / g.nX, cx = rem - cy * g.nX;
+        for (int dz = -1; dz <= 1; dz++) {
+            int zz = cz + dz;
+            if (g.periodic) zz = zz < 0 ? zz + g.nZ : (zz >= g.nZ ? zz - g.nZ : zz);
+            else if (zz < 0 || zz >= g.nZ) continue;
+            for (int dy = -1; dy <= 1; dy++) {
+                int yy = cy + dy;
+                if (g.periodic) yy = yy < 0 ? yy + g.nY : (yy >= g.nY ? yy - g.nY : yy);
+                else if (yy < 0 || yy >= g.nY) continue;
+                for (int dx = -1; dx <= 1; dx++) {
+                    int xx = cx + dx;
+                    if (g.periodic) xx = xx < 0 ? xx + g.nX : (xx >= g.nX ? xx - g.nX : xx);
+                    else if (xx < 0 || xx >= g.nX) continue;
+                    int cc = xx + yy * g.nX + zz * nXY;
+                    for (int p = a.cOff[cc]; p < a.cOff[cc + 1]; p++) {
+                        int j = a.cCont[p];
+                        if (j <= i) continue;
+                        int key = skin_key(i, j, ci, icx, icy, icz, ri, a);
+                        if (key < 0) continue;
+                        if (m < a.maxCand) slice[m] = key;
+                        m++;
+                    }
+                }
+            }
+        }
+    }
+    if (m <= a.maxCand) {  // insertion sort of the keys: (stencil position, index) ascending
+        for (int x = 1; x < m; x++) {
+            int v = slice[x];
+            int y = x - 1;
+            while (y >= 0 && slice[y] > v) {
+                slice[y + 1] = slice[y];
+                y--;
+            }
+            slice[y + 1] = v;
+        }
+        for (int x = 0; x < m; x++) slice[x] &= (1 << 26) - 1;
+    } else {
+        // more candidates than slots: the broad phase keeps the first maxCand in its order, so emit
+        // the maxCand smallest keys by repeated selection over the list (rare; exact)
+        const int *list = a.vlist + (size_t)i * a.listCap;
+        int last = -1;
+        for (int x = 0; x < a.maxCand; x++) {
+            int best = 0x7fffffff;
+            if (cnt <= a.listCap) {
+                for (int k = 0; k < cnt; k++) {
+                    int key = skin_key(i, list[k], ci, icx, icy, icz, ri, a);
+                    if (key > last && key < best) best = key;
+                }
+            } else {
+                const GridGeom &g = a.cgeom;
+                const int nXY = g.nX * g.nY;
+                int c = a.cCell[i];
+                int cz = c / nXY, rem = c - cz * nXY, cy = rem / g.nX, cx = rem - cy * g.nX;
+                for (int dz = -1; dz <= 1; dz++) {
+                    int zz = cz + dz;
+                    if (g.periodic) zz = zz < 0 ? zz + g.nZ : (zz >= g.nZ ? zz - g.nZ : zz);
+                    else if (zz < 0 || zz >= g.nZ) continue;
+                    for (int dy = -1; dy <= 1; dy++) {
+                        int yy = cy + dy;
+                        if (g.periodic) yy = yy < 0 ? yy + g.nY : (yy >= g.nY ? yy - g.nY : yy);
+                        else if (yy < 0 || yy >= g.nY) continue;
+                        for (int dx = -1; dx <= 1; dx++) {
+                            int xx = cx + dx;
+                            if (g.periodic) xx = xx < 0 ? xx + g.nX : (xx >= g.nX ? xx - g.nX : xx);
+                            else if (xx < 0 || xx >= g.nX) continue;
+                            int cc = xx + yy * g.nX + zz * nXY;
+                            for (int p = a.cOff[cc]; p < a.cOff[cc + 1]; p++) {
+                                int j = a.cCont[p];
+                                if (j <= i) continue;
+                                int key = skin_key(i, j, ci, icx, icy, icz, ri, a);
+                                if (key > last && key < best) best = key;
+                            }
+                        }
+                    }
+                }
+            }
+            slice[x] = best & ((1 << 26) - 1);
+            last = best;
+        }
+    }
+    a.candCount[i] = m;
+}
+
 hipEvent_t take_event(sbx_net *net) {
     if (!net->eventPool.empty()) {
         hipEvent_t e = net->eventPool.back();
@@ -664,6 +949,49 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
     return check_launch(c, "myosin stage");
 }
 
+SkinArgs skin_args(sbx_net *net, const float *coord, const float *uVec) {
+    SkinArgs a;
+    a.n = net->n; a.mode = net->skinMode; a.listCap = net->sp.listCap; a.maxCand = net->p.maxCand;
+    a.coord = coord; a.uVec = uVec; a.boundR = net->boundR;
+    a.links = net->L > 0 ? net->xl.activeLinkCount : nullptr;
+    a.geom = net->geom; a.cgeom = net->cgeom; a.cCells = net->cCells;
+    a.cCell = net->cCell; a.cOff = net->cOff; a.cCont = net->cCont;
+    a.refC = net->refC; a.refU = net->refU; a.skin = net->skin;
+    a.cellSkin = net->cellSkin; a.cellHorizon = net->cellHorizon;
+    a.vlist = net->vlist; a.vcount = net->vcount; a.flags = net->skinFlags; a.model = net->model;
+    a.skinDefault = net->sp.skinDefault; a.skinMin = net->sp.skinMin; a.skinMax = net->sp.skinMax;
+    a.margin = net->sp.margin; a.meanOcc = net->meanOcc;
+    a.bodyCell = net->bodyCell;
+    a.candPartner = net->candPartner; a.candCount = net->candCount;
+    return a;
+}
+
+// skin broad phase of one step: [gated rebuild: model, assign + bin, coarse sort, build] -> pairs
+int enqueue_skin(sbx_net *net, const float *coord, const float *uVec) {
+    sbx_ctx *c = net->ctx;
+    hipStream_t s = c->stream;
+    const int n = net->n;
+    SkinArgs a = skin_args(net, coord, uVec);
+    int r;
+    {
+        Scope sc(net, T_SKIN_REBUILD);
+        hipLaunchKernelGGL(k_skin_model, dim3(blocks_for(net->cCells, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_skin_assign_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+        if ((r = check_launch(c, "k_skin_model/assign"))) return r;
+        if ((r = counting_sort(c, net->sort, net->cCell, n, net->cCells, n, net->cOff, net->cCont,
+                               net->skinFlags + SKF_REBUILD)))
+            return r;
+        hipLaunchKernelGGL(k_skin_begin, dim3(1), dim3(1), 0, s, net->skinFlags);
+        hipLaunchKernelGGL(k_skin_build, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+        if ((r = check_launch(c, "k_skin_build"))) return r;
+    }
+    {
+        Scope sc(net, T_PAIRS);
+        hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+    }
+    return check_launch(c, "k_skin_pairs");
+}
+
 // one step's launches; stepOff is added to the device step counter
 int enqueue_step(sbx_net *net, int stepOff) {
     sbx_ctx *c = net->ctx;
@@ -675,6 +1003,9 @@ int enqueue_step(sbx_net *net, int stepOff) {
         if ((r = enqueue_myosin(net, coord, uVec, yVec, stepOff))) return r;
     } else if (net->p.gridOn) {
         if (!net->derivedValid) return fail(c, SBX_ERR_STATE, "net step: derived pose data not prepared");
+        if (net->skinMode != SBX_SKIN_REBUILD) {
+            if ((r = enqueue_skin(net, coord, uVec))) return r;
+        } else {
         {
             Scope sc(net, T_SORT);
             r = counting_sort(c, net->sort, net->bodyCell, n, net->p.gridDims[3], n, net->cellOffsets,
@@ -686,6 +1017,7 @@ int enqueue_step(sbx_net *net, int stepOff) {
             r = launch_broad_phase(c, coord, net->boundR, net->bodyCell, net->cellOffsets, net->cellContents,
                                    net->geom, n, n, net->candPartner, net->candCount, net->p.maxCand);
             if (r) return r;
+        }
         }
     }
     if (L > 0) {
@@ -724,6 +1056,10 @@ int enqueue_step(sbx_net *net, int stepOff) {
     a.geom = net->geom;
     a.nextCell = (net->p.gridOn && !net->hasMotors) ? net->cellNext : nullptr;
     a.linkPack = L > 0 ? net->linkPack : nullptr;
+    bool guard = net->skinMode != SBX_SKIN_REBUILD && !net->hasMotors;
+    a.skinRef = guard ? net->refC : nullptr;
+    a.skin = guard ? net->skin : nullptr;
+    a.skinFlags = net->skinFlags;
     a.mOff = net->mOff; a.mList = net->mList; a.bondData = net->mo.bondData;
     if (net->hasMotors) {
         Scope sc(net, T_MCSR);
@@ -771,7 +1107,9 @@ void release(sbx_net *net) {
                     net->cellOffsets,
                     net->candPartner, net->candCount, net->offA,  net->idxA,        net->offB,
                     net->idxB,     net->dStep,     net->ownerStore, net->ownerSlot,
-                    net->mOff,     net->mList,     net->bbOff,    net->bbList};
+                    net->mOff,     net->mList,     net->bbOff,    net->bbList,
+                    net->cCell,    net->cOff,      net->cCont,    net->refC,    net->refU, net->skin,
+                    net->cellSkin, net->cellHorizon, net->vlist,  net->vcount,  net->skinFlags, net->model};
     for (void *b : bufs)
         if (b) hipFree(b);
     sort_scratch_free(net->sort);
@@ -1062,16 +1400,20 @@ int sbx_net_publish(sbx_net *net) {
     if (!net) return SBX_ERR_ARG;
     sbx_ctx *c = net->ctx;
     SBX_CHECK_CTX(c);
-    if (net->cur == 0) return SBX_OK;
-    size_t bytes = sizeof(float) * 3 * (size_t)net->n;
-    int r = check_hip(c, hipMemcpyAsync(net->coordB[0], net->coordB[1], bytes, hipMemcpyDeviceToDevice, c->stream),
+    int r = SBX_OK;
+    if (net->cur != 0) {
+        size_t bytes = sizeof(float) * 3 * (size_t)net->n;
+        r = check_hip(c, hipMemcpyAsync(net->coordB[0], net->coordB[1], bytes, hipMemcpyDeviceToDevice, c->stream),
                       "publish coord");
-    if (!r) r = check_hip(c, hipMemcpyAsync(net->uB[0], net->uB[1], bytes, hipMemcpyDeviceToDevice, c->stream),
-                          "publish uVec");
-    if (!r) r = check_hip(c, hipMemcpyAsync(net->yB[0], net->yB[1], bytes, hipMemcpyDeviceToDevice, c->stream),
-                          "publish yVec");
-    if (!r) net->cur = 0;
-    net->derivedValid = 0;  // the caller may now edit its pose arrays
+        if (!r) r = check_hip(c, hipMemcpyAsync(net->uB[0], net->uB[1], bytes, hipMemcpyDeviceToDevice, c->stream),
+                              "publish uVec");
+        if (!r) r = check_hip(c, hipMemcpyAsync(net->yB[0], net->yB[1], bytes, hipMemcpyDeviceToDevice, c->stream),
+                              "publish yVec");
+        if (!r) net->cur = 0;
+    }
+    // the caller may now edit its pose arrays: re-bin, re-pack and (skin mode) rebuild next step
+    net->derivedValid = 0;
+    if (!r && net->skinFlags) r = fill_i32(c, net->skinFlags + SKF_REBUILD, 1, 1);
     return r;
 }
 
@@ -1090,6 +1432,144 @@ int sbx_net_candidates(sbx_net *net, int **candPartner, int **candCount, int **g
     if (gridCellOffsets) *gridCellOffsets = net->cellOffsets;
     if (gridCellContents) *gridCellContents = net->cellContents;
     if (bodyCell) *bodyCell = net->lastCell ? net->lastCell : net->bodyCell;
+    return SBX_OK;
+}
+
+int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const float *modelWeights) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, mode == SBX_SKIN_REBUILD || mode == SBX_SKIN_FIXED || mode == SBX_SKIN_LEARNED,
+                "net_set_skin: unknown mode %d", mode);
+    SBX_REQUIRE(c, !net->hasMotors, "net_set_skin: the skin broad phase covers the filament-only step");
+    if (mode != SBX_SKIN_REBUILD) {
+        SBX_REQUIRE(c, net->p.gridOn, "net_set_skin: needs the grid (gridOn)");
+        SBX_REQUIRE(c, sp, "net_set_skin: null params");
+        SBX_REQUIRE(c, sp->skinDefault > 0.0f && sp->skinMin > 0.0f && sp->skinMax >= sp->skinMin &&
+                           sp->skinDefault <= sp->skinMax && sp->margin > 0.0f && sp->listCap > 0,
+                    "net_set_skin: need 0 < skinMin <= skinDefault <= skinMax, margin > 0, listCap > 0");
+        SBX_REQUIRE(c, mode != SBX_SKIN_LEARNED || modelWeights, "net_set_skin: learned mode needs model weights");
+        SBX_REQUIRE(c, net->n < (1 << 26), "net_set_skin: more than 2^26 bodies");
+    }
+    int r = set_device(c);
+    if (r) return r;
+    if ((r = check_hip(c, hipStreamSynchronize(c->stream), "net_set_skin sync"))) return r;
+    // a captured step no longer matches the launch sequence
+    if (net->graphExec) { hipGraphExecDestroy(net->graphExec); net->graphExec = nullptr; }
+    if (net->graph) { hipGraphDestroy(net->graph); net->graph = nullptr; }
+    net->skinMode = mode;
+    if (mode == SBX_SKIN_REBUILD) return SBX_OK;
+    net->sp = *sp;
+    const int n = net->n;
+    // largest bounding radius -> coarse cell >= the largest list reach
+    int *dmax = nullptr;
+    if ((r = check_hip(c, hipMalloc(&dmax, sizeof(int)), "hipMalloc"))) return r;
+    int hmax = 0;
+    r = fill_i32(c, dmax, 0, 1);
+    if (!r) {
+        hipLaunchKernelGGL(k_max_radius, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, net->boundR, n, dmax);
+        r = check_launch(c, "k_max_radius");
+    }
+    if (!r) r = check_hip(c, hipMemcpyAsync(&hmax, dmax, sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (!r) r = check_hip(c, hipStreamSynchronize(c->stream), "sync");
+    hipFree(dmax);
+    if (r) return r;
+    float maxR;
+    memcpy(&maxR, &hmax, sizeof(float));
+    const GridGeom &f = net->geom;
+    float fineCell = 1.0f / f.invCell;
+    float need = 2.0f * maxR + f.cutoff + 2.0f * sp->skinMax + 2.0f * SKIN_SLACK;
+    float cell = need > fineCell ? need : fineCell;
+    GridGeom g = f;
+    if (f.periodic) {
+        int nx = (int)floorf(f.Lx / cell), ny = (int)floorf(f.Ly / cell), nz = (int)floorf(f.Lz / cell);
+        SBX_REQUIRE(c, nx >= 3 && ny >= 3 && nz >= 3,
+                    "net_set_skin: box too small for skinMax %g (coarse cell %g um needs >= 3 per axis)",
+                    sp->skinMax, cell);
+        g.nX = nx; g.nY = ny; g.nZ = nz;
+        // cubic cells of the largest size that tiles the shortest... each axis tiles its own length
+        cell = fmaxf(f.Lx / nx, fmaxf(f.Ly / ny, f.Lz / nz));
+        g.invCell = 1.0f / cell;
+        g.xMin = f.xMin; g.yMin = f.yMin; g.zMin = f.zMin;
+    } else {
+        float ex = fineCell * f.nX, ey = fineCell * f.nY, ez = fineCell * f.nZ;
+        g.nX = (int)ceilf(ex / cell); g.nY = (int)ceilf(ey / cell); g.nZ = (int)ceilf(ez / cell);
+        if (g.nX < 1) g.nX = 1;
+        if (g.nY < 1) g.nY = 1;
+        if (g.nZ < 1) g.nZ = 1;
+        g.invCell = 1.0f / cell;
+    }
+    long long cc = (long long)g.nX * g.nY * g.nZ;
+    SBX_REQUIRE(c, cc < (1LL << 30), "net_set_skin: coarse grid too large");
+    net->cgeom = g;
+    net->cCells = (int)cc;
+    net->meanOcc = (float)n / (float)cc;
+    // (re)allocate the skin state
+    void *old[] = {net->cCell, net->cOff, net->cCont, net->refC, net->refU, net->skin, net->cellSkin,
+                   net->cellHorizon, net->vlist, net->vcount, net->skinFlags, net->model};
+    for (void *b : old)
+        if (b) hipFree(b);
+    net->cCell = net->cOff = net->cCont = net->vlist = net->vcount = net->skinFlags = nullptr;
+    net->refC = net->refU = net->skin = net->cellSkin = net->cellHorizon = net->model = nullptr;
+    do {
+        if ((r = alloc(c, &net->cCell, n))) break;
+        if ((r = alloc(c, &net->cOff, (size_t)cc + 1))) break;
+        if ((r = alloc(c, &net->cCont, n))) break;
+        if ((r = alloc(c, &net->refC, (size_t)3 * n))) break;
+        if ((r = alloc(c, &net->refU, (size_t)3 * n))) break;
+        if ((r = alloc(c, &net->skin, n))) break;
+        if ((r = alloc(c, &net->cellSkin, (size_t)cc))) break;
+        if ((r = alloc(c, &net->cellHorizon, (size_t)cc))) break;
+        if ((r = alloc(c, &net->vlist, (size_t)n * sp->listCap))) break;
+        if ((r = alloc(c, &net->vcount, n))) break;
+        if ((r = alloc(c, &net->skinFlags, SKF_WORDS))) break;
+        if (mode == SBX_SKIN_LEARNED) {
+            if ((r = alloc(c, &net->model, MLP_NPARAM))) break;
+            if ((r = check_hip(c, hipMemcpyAsync(net->model, modelWeights, sizeof(float) * MLP_NPARAM,
+                                                 hipMemcpyDeviceToDevice, c->stream), "copy model")))
+                break;
+        }
+        long long keys = net->p.gridDims[3] > cc ? net->p.gridDims[3] : cc;
+        if ((r = sort_scratch_reserve(c, net->sort, keys, n))) break;
+        if ((r = fill_i32(c, net->skinFlags, 0, SKF_WORDS))) break;
+        if ((r = fill_i32(c, net->skinFlags + SKF_REBUILD, 1, 1))) break;  // the first step builds
+        r = sbx_sync(c);
+    } while (0);
+    if (r) net->skinMode = SBX_SKIN_REBUILD;
+    return r;
+}
+
+int sbx_net_skin_stats(sbx_net *net, long long *out, int cap) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, out && cap >= 0, "net_skin_stats: null out");
+    if (!net->skinFlags) return 0;
+    int f[SKF_WORDS];
+    int r = check_hip(c, hipMemcpyAsync(f, net->skinFlags, sizeof(f), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (!r) r = check_hip(c, hipStreamSynchronize(c->stream), "sync");
+    if (r) return r;
+    int k = 0;
+    for (; k < SKF_WORDS && k < cap; k++) out[k] = f[k];
+    return k;
+}
+
+int sbx_net_skin_state(sbx_net *net, float **refCoord, float **skin, int **coarseCell, int **coarseOffsets,
+                       int **coarseContents, int *nCoarse, float *coarseGrid) {
+    if (!net) return SBX_ERR_ARG;
+    SBX_CHECK_CTX(net->ctx);
+    if (refCoord) *refCoord = net->refC;
+    if (skin) *skin = net->skin;
+    if (coarseCell) *coarseCell = net->cCell;
+    if (coarseOffsets) *coarseOffsets = net->cOff;
+    if (coarseContents) *coarseContents = net->cCont;
+    if (nCoarse) *nCoarse = net->cCells;
+    if (coarseGrid) {
+        const GridGeom &g = net->cgeom;
+        float v[8] = {g.xMin, g.yMin, g.zMin, g.invCell > 0 ? 1.0f / g.invCell : 0.0f, g.invCell,
+                      (float)g.nX, (float)g.nY, (float)g.nZ};
+        memcpy(coarseGrid, v, sizeof(v));
+    }
     return SBX_OK;
 }
 

@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .context import Context
+from .context import Context, ptr
 from .stores import DeviceCrosslinks, DeviceFilaments, DeviceMotors
 
 
@@ -36,6 +36,32 @@ class Network:
             self._mp = self.mot.params()
             ctx.check(ctx.lib.sbx_net_attach_motors(self.h, C.byref(self._ms), C.byref(self._mp)))
 
+    def set_skin(self, mode: int, cfg=None, model=None) -> None:
+        """Broad-phase mode: SBX_SKIN_REBUILD (the reference's per-step rebuild), SBX_SKIN_FIXED or
+        SBX_SKIN_LEARNED (with a softbox_amd.skin.SkinModel)."""
+        from .skin import SkinConfig
+
+        cfg = cfg or SkinConfig()
+        self._skin_p = cfg.params()
+        w = ptr(model.w) if model is not None else None
+        self.ctx.check(self.ctx.lib.sbx_net_set_skin(self.h, mode, C.byref(self._skin_p), w))
+
+    def skin_stats(self) -> dict:
+        out = (C.c_longlong * 8)()
+        k = self.ctx.check(self.ctx.lib.sbx_net_skin_stats(self.h, out, 8))
+        names = ("rebuild_pending", "steps_since_rebuild", "rebuilds", "overflow_bodies", "list_entries",
+                 "skin_steps")
+        return {names[i]: int(out[i]) for i in range(min(k, len(names)))}
+
+    def skin_state(self) -> dict:
+        ptrs = [C.c_void_p() for _ in range(5)]
+        nc = C.c_int()
+        grid = (C.c_float * 8)()
+        self.ctx.check(self.ctx.lib.sbx_net_skin_state(self.h, *[C.byref(p) for p in ptrs], C.byref(nc), grid))
+        return {"refCoord": ptrs[0].value, "skin": ptrs[1].value, "coarseCell": ptrs[2].value,
+                "coarseOffsets": ptrs[3].value, "coarseContents": ptrs[4].value, "nCells": nc.value,
+                "grid": list(grid)}
+
     def step(self, step0: int, n: int = 1) -> None:
         self.ctx.check(self.ctx.lib.sbx_net_step(self.h, step0, n))
 
@@ -49,7 +75,7 @@ class Network:
         self.ctx.check(self.ctx.lib.sbx_net_set_profiling(self.h, 1 if on else 0))
 
     def kernel_times(self) -> dict:
-        cap = 16
+        cap = 32
         names = (C.c_char_p * cap)()
         ms = (C.c_double * cap)()
         cnt = (C.c_longlong * cap)()

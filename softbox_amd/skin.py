@@ -1,0 +1,240 @@
+"""Learned-skin broad phase: the skin model, its training data and its training loop.
+
+No counterpart in the reference.  The per-step-rebuild network (SpatialGrid.broadPhase every
+step) generates the trajectory; the library's kernels compute per-coarse-cell features and labels
+(sbx_skin_*), and the MLP is trained with the library's forward/backward/Adam kernels (sbx_mlp_*).
+The trained parameters are handed to sbx_net_set_skin(SBX_SKIN_LEARNED).  Whatever the model
+predicts, the exact displacement guard keeps the candidate slices identical to the per-step
+rebuild's; the model only sizes the neighbour lists.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .context import Context, ptr
+
+NPARAM = 5122 + 24
+NTRAIN = 5122
+N_FEATURES = 12
+
+
+@dataclass
+class SkinConfig:
+    skin_default: float = 0.1     # um
+    skin_min: float = 0.02
+    skin_max: float = 0.3
+    margin: float = 1.15          # learned skin = margin * exp(out0)
+    list_cap: int = 64
+    horizon: int = 8              # steps a skin is trained to last
+    window: int = 32              # steps tracked after a snapshot (censoring of the horizon label)
+
+    def params(self) -> _lib.SbxSkinParams:
+        return _lib.SbxSkinParams(self.skin_default, self.skin_min, self.skin_max, self.margin, self.list_cap)
+
+
+class SkinModel:
+    """Device parameters of the 12 -> 64 -> 64 -> 2 GELU MLP (layout in softbox_amd/csrc/sbx_mlp.h)."""
+
+    def __init__(self, ctx: Context, weights: np.ndarray):
+        assert weights.size == NPARAM and ctx.lib.sbx_mlp_nparam() == NPARAM
+        self.ctx = ctx
+        self.w = ctx.to_device(np.ascontiguousarray(weights, np.float32))
+
+    @staticmethod
+    def initial(ctx: Context, mean: np.ndarray, std: np.ndarray, seed: int = 0) -> "SkinModel":
+        rng = np.random.default_rng(seed)
+        w = np.zeros(NPARAM, np.float32)
+        w[0:768] = rng.standard_normal(768) * math.sqrt(2.0 / 12)
+        w[832:832 + 4096] = rng.standard_normal(4096) * math.sqrt(2.0 / 64)
+        w[4992:4992 + 128] = rng.standard_normal(128) * math.sqrt(1.0 / 64)
+        w[NTRAIN:NTRAIN + 12] = mean
+        w[NTRAIN + 12:] = std
+        return SkinModel(ctx, w)
+
+    def host(self) -> np.ndarray:
+        return self.ctx.to_host(self.w)
+
+    def forward(self, x):
+        """x: device tensor [B, 12] -> device tensor [B, 2]."""
+        import torch
+
+        y = torch.empty((x.shape[0], 2), dtype=torch.float32, device=x.device)
+        torch.cuda.synchronize()
+        self.ctx.check(self.ctx.lib.sbx_mlp_forward(self.ctx.h, ptr(self.w), ptr(x), int(x.shape[0]), ptr(y)))
+        self.ctx.sync()
+        return y
+
+
+def loss_grad(ctx: Context, model: SkinModel, x, t, out_weight=(1.0, 0.25), scratch=None):
+    """(loss, grad) of the batch on the GPU; grad is a device tensor of NPARAM floats."""
+    import torch
+
+    B = int(x.shape[0])
+    need = int(ctx.lib.sbx_mlp_scratch_floats(B))
+    if scratch is None or scratch.numel() < need:
+        scratch = torch.empty(need, dtype=torch.float32, device=x.device)
+    grad = torch.zeros(NPARAM, dtype=torch.float32, device=x.device)
+    loss = torch.zeros(1, dtype=torch.float32, device=x.device)
+    ow = (C.c_float * 2)(*out_weight)
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.sbx_mlp_loss_grad(ctx.h, ptr(model.w), ptr(x), ptr(t), B, ow, ptr(scratch), ptr(grad),
+                                        ptr(loss)))
+    ctx.sync()
+    return float(loss.item()), grad, scratch
+
+
+def train(ctx: Context, model: SkinModel, x, t, iters: int = 400, batch: int = 4096, lr: float = 3e-3,
+          out_weight=(1.0, 0.25), seed: int = 0, log_every: int = 0) -> list:
+    """Adam on the GPU kernels; returns the full-data loss before training and after each epoch."""
+    import torch
+
+    S = int(x.shape[0])
+    batch = min(batch, S)
+    m = torch.zeros(NPARAM, dtype=torch.float32, device=x.device)
+    v = torch.zeros(NPARAM, dtype=torch.float32, device=x.device)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    history = [loss_grad(ctx, model, x, t, out_weight)[0]]
+    scratch = None
+    per_epoch = max(1, S // batch)
+    step = 0
+    while step < iters:
+        perm = torch.randperm(S, generator=g).to(x.device)
+        xs, ts = x[perm].contiguous(), t[perm].contiguous()
+        for e in range(per_epoch):
+            if step >= iters:
+                break
+            xb, tb = xs[e * batch:(e + 1) * batch], ts[e * batch:(e + 1) * batch]
+            _, grad, scratch = loss_grad(ctx, model, xb, tb, out_weight, scratch)
+            step += 1
+            ctx.check(ctx.lib.sbx_mlp_adam(ctx.h, ptr(model.w), ptr(grad), ptr(m), ptr(v), step, lr, 0.9, 0.999,
+                                           1e-8))
+        history.append(loss_grad(ctx, model, x, t, out_weight)[0])
+        if log_every:
+            print(f"[skin] step {step}: loss {history[-1]:.4f}", flush=True)
+    ctx.sync()
+    return history
+
+
+class TrajectorySampler:
+    """Features and labels of the skin model from the per-step-rebuild run of a network.
+
+    One sample = one occupied coarse cell at a snapshot t0, members binned at t0 - H:
+      x = skin_cell_features of the members' motion t0 - H -> t0;
+      y = (log of the largest displacement of a member within H steps after t0,
+           log of the first step a member exceeds that displacement, censored at `window`).
+    """
+
+    def __init__(self, ctx: Context, net, cfg: SkinConfig):
+        import torch
+
+        self.ctx, self.net, self.cfg = ctx, net, cfg
+        # the coarse grid of the skin mode (set up once, then back to the per-step rebuild)
+        net.set_skin(_lib.SBX_SKIN_FIXED, cfg)
+        st = net.skin_state()
+        net.set_skin(_lib.SBX_SKIN_REBUILD, cfg)
+        self.grid = st["grid"]
+        self.nCells = st["nCells"]
+        sc = net.scene
+        self.n = sc.fs.n
+        self.periodic = int(sc.p["periodic"])
+        self.boxLen = (C.c_float * 3)(*(sc.p["boxLen"] if self.periodic else [0.0, 0.0, 0.0]))
+        g = self.grid
+        self.gridParams = (C.c_float * 6)(g[0], g[1], g[2], g[3], g[4], 0.0)
+        self.gridDims = (C.c_int * 4)(int(g[5]), int(g[6]), int(g[7]), int(g[5] * g[6] * g[7]))
+        dev = f"cuda:{ctx.device}"
+        n, nc = self.n, self.nCells
+        self.cell = torch.empty(n, dtype=torch.int32, device=dev)
+        self.off = torch.empty(nc + 1, dtype=torch.int32, device=dev)
+        self.cont = torch.empty(n, dtype=torch.int32, device=dev)
+        self.pPrev = torch.empty(3 * n, dtype=torch.float32, device=dev)
+        self.uPrev = torch.empty(3 * n, dtype=torch.float32, device=dev)
+        self.x0 = torch.empty(3 * n, dtype=torch.float32, device=dev)
+        self.maxDisp = torch.empty(n, dtype=torch.float32, device=dev)
+        self.thr = torch.empty(n, dtype=torch.float32, device=dev)
+        self.breach = torch.empty(n, dtype=torch.int32, device=dev)
+        self.feat = torch.empty(nc * N_FEATURES, dtype=torch.float32, device=dev)
+        self.sStar = torch.empty(nc, dtype=torch.float32, device=dev)
+        self.hStar = torch.empty(nc, dtype=torch.float32, device=dev)
+
+    def _step(self, t):
+        self.net.step(t, 1)
+        self.net.publish()
+
+    def _copy(self, dst, src):
+        self.ctx.check(self.ctx.lib.sbx_copy(self.ctx.h, ptr(dst), ptr(src), dst.numel() * 4))
+
+    def collect(self, t0: int, snapshots: int):
+        """Run the network from step t0; returns (x [S, 12], y [S, 2]) device tensors and the next step."""
+        import torch
+
+        lib, h, fil = self.ctx.lib, self.ctx.h, self.net.fil
+        cfg, n, nc = self.cfg, self.n, self.nCells
+        H, W = cfg.horizon, cfg.window
+        links = self.net.xl.activeLinkCount if self.net.xl is not None else None
+        xs, ys = [], []
+        t = t0
+        counts = (C.c_int * 4)(n, n, 0, 0)
+        for _ in range(snapshots):
+            # t0 - H: positions and the member lists the features are taken over
+            self.net.publish()
+            self._copy(self.pPrev, fil.coord)
+            self._copy(self.uPrev, fil.uVec)
+            self.ctx.check(lib.sbx_body_cell(h, ptr(self.pPrev), self.gridParams, self.gridDims, counts,
+                                             ptr(self.cell), n, self.periodic))
+            self.ctx.check(lib.sbx_grid_build(h, ptr(self.cell), counts, self.gridDims, ptr(self.off),
+                                              ptr(self.cont), n))
+            for _ in range(H):
+                self._step(t)
+                t += 1
+            self.ctx.check(lib.sbx_skin_features(h, ptr(self.off), ptr(self.cont), nc, n, ptr(fil.coord),
+                                                 ptr(self.pPrev), ptr(fil.uVec), ptr(self.uPrev), ptr(links),
+                                                 float(H), self.periodic, self.boxLen, ptr(self.feat)))
+            # labels: the largest displacement within H steps, then the first step beyond it
+            self._copy(self.x0, fil.coord)
+            self.ctx.check(lib.sbx_fill_i32(h, ptr(self.maxDisp), 0, n))
+            self.ctx.check(lib.sbx_fill_i32(h, ptr(self.breach), -1, n))
+            for k in range(1, W + 1):
+                self._step(t)
+                t += 1
+                if k <= H:
+                    self.ctx.check(lib.sbx_skin_track(h, ptr(fil.coord), ptr(self.x0), n, self.periodic,
+                                                      self.boxLen, ptr(self.maxDisp), None, None, k))
+                    if k == H:
+                        self.ctx.check(lib.sbx_skin_cell_reduce(h, ptr(self.off), ptr(self.cont), nc,
+                                                                ptr(self.maxDisp), None, 0, 0.0, ptr(self.sStar)))
+                        self.ctx.check(lib.sbx_skin_cell_scatter(h, ptr(self.off), ptr(self.cont), nc,
+                                                                 ptr(self.sStar), ptr(self.thr)))
+                else:
+                    self.ctx.check(lib.sbx_skin_track(h, ptr(fil.coord), ptr(self.x0), n, self.periodic,
+                                                      self.boxLen, None, ptr(self.thr), ptr(self.breach), k))
+            self.ctx.check(lib.sbx_skin_cell_reduce(h, ptr(self.off), ptr(self.cont), nc, None, ptr(self.breach), 1,
+                                                    float(W + 1), ptr(self.hStar)))
+            self.ctx.sync()
+            f = self.feat.view(nc, N_FEATURES)
+            occupied = (f[:, 11] > 0) & (self.sStar > 0)
+            xs.append(f[occupied].clone())
+            ys.append(torch.stack([torch.log(self.sStar[occupied]), torch.log(self.hStar[occupied])], 1))
+        torch.cuda.synchronize()
+        return torch.cat(xs).contiguous(), torch.cat(ys).contiguous(), t
+
+
+def fit(ctx: Context, net, cfg: SkinConfig, t0: int = 0, snapshots: int = 16, iters: int = 400, seed: int = 0,
+        log_every: int = 0):
+    """Collect a trajectory of `net` (per-step rebuild) and train a skin model on it; returns
+    (model, loss history, samples, next step)."""
+    import torch
+
+    sampler = TrajectorySampler(ctx, net, cfg)
+    x, y, t = sampler.collect(t0, snapshots)
+    xh = ctx.to_host(x)
+    mean = xh.mean(0)
+    std = np.maximum(xh.std(0), 1e-3)
+    model = SkinModel.initial(ctx, mean.astype(np.float32), std.astype(np.float32), seed)
+    torch.cuda.synchronize()
+    hist = train(ctx, model, x, y, iters=iters, seed=seed, log_every=log_every)
+    return model, hist, (x, y), t

@@ -16,7 +16,8 @@ def _declarations():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     decls = {}
-    for m in re.finditer(r"\b(?:int|void|const\s+char\s*\*)\s*\**\s*(sbx_\w+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+    for m in re.finditer(r"\b(?:int|void|size_t|const\s+char\s*\*)\s*\**\s*(sbx_\w+)\s*\(([^;{]*?)\)\s*;", text,
+                         flags=re.S):
         name, args = m.group(1), " ".join(m.group(2).split())
         decls[name] = 0 if args in ("", "void") else len(args.split(","))
     return decls
