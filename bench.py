@@ -2,15 +2,20 @@
 
 Workload (the one-GPU configuration the metric is quoted on): crosslinked actin network,
 20k filaments x 10 rigid rods (0.1 um), 20 um periodic cube, 50k crosslinkers (rest 35 nm, Bell
-k0 = 0.1/s, dx = 0.5 nm), water, dt = 1e-5 s, FP32 state, per-step grid rebuild + 27-cell broad phase
-+ crosslink unbind/count/force/torsion + CSR gathers + chain + Brownian + integrate + derive.
+k0 = 0.1/s, dx = 0.5 nm), water, dt = 1e-5 s, FP32 state, broad phase (per-step grid rebuild +
+27-cell stencil, or the learned skin) + crosslink unbind/count/force/torsion + CSR gathers + chain +
+Brownian + integrate + derive.
 Synthetic, seeded scene (softbox_amd.scenes.network); the whole state is resident in HBM before
 the timed region starts.
 
-Prints ONE JSON line (rank 0).  `roofline` is for the dominant kernel: algorithmic bytes per launch
-divided by its mean duration measured with HIP events on the library's stream during the timed
-region.  `cpu_baseline` times the oracle (single-threaded C restatement of the reference's
-sequential CPU runner) on a bounded sample of the same workload.
+Prints ONE JSON line (rank 0).  By default it runs the workload twice from the same initial state:
+the reference's per-step grid rebuild, and the learned-skin broad phase (its skin model trained
+first, outside the timed region, on a per-step-rebuild trajectory of the same scene); `value` is
+the learned-skin rate, `learned_vs_rebuild` the ratio, and the two runs' candidate slices are
+checked identical.  `roofline` is for the dominant kernel of the headline run: algorithmic bytes
+per launch divided by its mean duration measured with HIP events on the library's stream over
+eager profiled steps.  `cpu_baseline` times the oracle (single-threaded C restatement of the
+reference's sequential CPU runner) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -29,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "rod-steps/sec + HBM GB/s % peak at 1 GPU; learned-skin vs per-step rebuild"
 
 
-def kernel_bytes(sc, name: str) -> float | None:
+def kernel_bytes(sc, name: str, list_entries: int | None = None) -> float | None:
     """Algorithmic (compulsory) HBM bytes of one launch of a network kernel on this workload:
     every input read once and every output written once, at its natural width."""
     n = sc.fs.n
@@ -55,6 +60,26 @@ def kernel_bytes(sc, name: str) -> float | None:
     if name == "k_broad_phase":
         # per body: contents 4 + cell 4 + centre 12 + radius 4 + count 4; per cell: offsets 4
         return 28.0 * n + 4.0 * cells
+    if name == "k_skin_pairs" and list_entries:
+        # per body: order 4 + centre 12 + radius 4 + cell 4 + list length 4 + count 4; per list entry:
+        # index 4 + centre 12 + radius 4 + cell 4
+        return 32.0 * n + 24.0 * list_entries
+    return None
+
+
+def kernel_traffic(name: str):
+    """HBM bytes per launch of `name` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of
+    this workload (profiles/network_step_200k_rods_counters.md), or None when it has no row."""
+    path = os.path.join(ROOT, "profiles", "network_step_200k_rods_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) >= 6 and cols[0] == name:
+            try:
+                return float(cols[5])
+            except ValueError:
+                return None
     return None
 
 
@@ -76,12 +101,51 @@ def run_cpu_baseline(sc, budget_s: float = 15.0) -> dict:
                       f"for {steps} steps = {el:.1f} s"}
 
 
+def timed(net, ctx, step0: int, steps: int) -> float:
+    """Seconds for `steps` whole steps, bracketed by device synchronisation."""
+    import torch
+
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    net.step(step0, steps)
+    ctx.sync()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def profile(net, step0: int, steps: int) -> dict:
+    """Per-kernel mean durations (HIP events on the library's stream) over `steps` eager steps."""
+    net.use_graph(False)
+    net.set_profiling(True)
+    net.step(step0, steps)
+    times = net.kernel_times()
+    net.set_profiling(False)
+    return {k: {"avg_us": 1e3 * ms / cnt, "launches": int(cnt)} for k, (ms, cnt) in times.items() if cnt}
+
+
+def slices_digest(net, sc):
+    """(candCount, order-sensitive digest of the used part of every candidate slice)."""
+    n, mc = sc.fs.n, int(sc.p["maxCand"])
+    cand, cnt, _, _, _ = net.candidates(n, mc)
+    keep = (np.arange(mc)[None, :] < np.minimum(cnt, mc)[:, None])
+    used = np.where(keep, cand.reshape(n, mc), -1).astype(np.int64)
+    w = (np.arange(mc, dtype=np.int64) * 2654435761) % 1000003 + 1
+    return cnt, int(((used + 2) * w[None, :]).sum() % (1 << 61))
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--graph", type=int, default=0, help="1: replay the step as a captured hipGraph")
+    ap.add_argument("--skin", choices=("compare", "rebuild", "fixed", "learned"), default="compare",
+                    help="compare: per-step rebuild and learned skin on identical trajectories (value = learned)")
+    ap.add_argument("--graph", type=int, default=1, help="1: replay the step as a captured hipGraph")
+    ap.add_argument("--profile-steps", type=int, default=40, help="eager steps timed per kernel (0: none)")
+    ap.add_argument("--train-snapshots", type=int, default=12)
+    ap.add_argument("--train-iters", type=int, default=400)
+    ap.add_argument("--horizon", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--n-fil", type=int, default=20000)
     ap.add_argument("--n-links", type=int, default=50000)
@@ -97,15 +161,18 @@ def main() -> int:
 
     import torch
 
-    from softbox_amd import scenes
+    from softbox_amd import _lib, scenes
     from softbox_amd.context import Context
     from softbox_amd.network import Network
+    from softbox_amd.skin import SkinConfig, fit
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     t_build = time.perf_counter()
     sc = scenes.network(n_fil=args.n_fil, segs=10, box=args.box, n_links=args.n_links, seed=0)
     n = sc.fs.n
-    print(f"[bench] scene: {n} rods, {sc.xl.nLinks} links, grid {sc.p['gridDims'][:3]} "
-          f"({time.perf_counter() - t_build:.1f} s)", file=sys.stderr, flush=True)
+    log(f"scene: {n} rods, {sc.xl.nLinks} links, grid {sc.p['gridDims'][:3]} ({time.perf_counter() - t_build:.1f} s)")
     ctx = Context(0)
     if args.calib_copy_mb > 0:
         nb = args.calib_copy_mb << 20
@@ -115,67 +182,89 @@ def main() -> int:
         ctx.check(ctx.lib.sbx_copy(ctx.h, dst.data_ptr(), src.data_ptr(), nb))
         ctx.sync()
         del src, dst
-        print(f"[bench] calibration copy of {args.calib_copy_mb} MiB done", file=sys.stderr, flush=True)
-    net = Network(ctx, sc, keep_forces=False)
-    if args.graph:
-        net.use_graph(True)
-    net.step(0, args.warmup)
-    ctx.sync()
-    torch.cuda.synchronize()
-    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+        log(f"calibration copy of {args.calib_copy_mb} MiB done")
 
-    # timed region: whole steps, per-kernel HIP events on the library stream (eager launches)
-    prof = not args.graph
-    if prof:
-        net.set_profiling(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    net.step(args.warmup, args.steps)
-    ctx.sync()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    times = net.kernel_times() if prof else {}
-    if prof:
-        net.set_profiling(False)
-    ms_per_step = 1e3 * el / args.steps
-    value = n * args.steps / el
+    cfg = SkinConfig(horizon=args.horizon, window=4 * args.horizon)
+    model, train_info = None, None
+    if args.skin == "learned" or args.skin == "compare":
+        t_tr = time.perf_counter()
+        tnet = Network(ctx, sc.copy(), keep_forces=False)
+        model, hist, (x, _), _ = fit(ctx, tnet, cfg, snapshots=args.train_snapshots, iters=args.train_iters)
+        tnet.close()
+        train_info = {"samples": int(x.shape[0]), "loss_start": round(hist[0], 4), "loss_end": round(hist[-1], 4),
+                      "margin": round(cfg.margin, 3), "horizon": cfg.horizon,
+                      "trajectory_steps": args.train_snapshots * (cfg.horizon + cfg.window),
+                      "seconds": round(time.perf_counter() - t_tr, 1)}
+        log(f"skin model trained: {train_info}")
 
-    fs_end = sc.fs.copy()
-    xl_end = sc.xl.copy()
-    net.pull(fs_end, xl_end)
-    assert np.isfinite(fs_end.coord).all(), "non-finite pose after the timed run"
-    alive = int((xl_end.linkState[: xl_end.nLinks] >= 0).sum())
+    modes = {"compare": ["rebuild", "learned"], "rebuild": ["rebuild"], "fixed": ["fixed"], "learned": ["learned"]}
+    runs = {}
+    for mode in modes[args.skin]:
+        net = Network(ctx, sc.copy(), keep_forces=False)
+        if mode != "rebuild":
+            net.set_skin(_lib.SBX_SKIN_LEARNED if mode == "learned" else _lib.SBX_SKIN_FIXED, cfg,
+                         model if mode == "learned" else None)
+        if args.graph:
+            net.use_graph(True)
+        net.step(0, args.warmup)
+        el = timed(net, ctx, args.warmup, args.steps)
+        r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps}
+        r["digest"] = slices_digest(net, sc)
+        if mode != "rebuild":
+            r["skin"] = net.skin_stats()
+        if args.profile_steps:
+            r["kernels"] = profile(net, args.warmup + args.steps, args.profile_steps)
+        xl_end = sc.xl.copy()
+        fs_end = sc.fs.copy()
+        net.pull(fs_end, xl_end)
+        assert np.isfinite(fs_end.coord).all(), "non-finite pose after the timed run"
+        r["links_alive_end"] = int((xl_end.linkState[: xl_end.nLinks] >= 0).sum())
+        net.close()
+        runs[mode] = r
+        log(f"{mode}: {r['value']:.4g} rod-steps/s, {r['ms_per_step']:.4f} ms/step")
 
+    head = runs["learned"] if "learned" in runs else runs[modes[args.skin][0]]
+    if "rebuild" in runs and "learned" in runs:
+        same = (np.array_equal(runs["rebuild"]["digest"][0], runs["learned"]["digest"][0])
+                and runs["rebuild"]["digest"][1] == runs["learned"]["digest"][1])
+        assert same, "learned-skin candidate slices differ from the per-step rebuild"
     roof = None
-    per_kernel = {}
-    if times:
-        for k, (ms, cnt) in times.items():
-            if cnt:
-                per_kernel[k] = {"avg_us": 1e3 * ms / cnt, "launches": int(cnt)}
-        dom = max((k for k in per_kernel if kernel_bytes(sc, k)), key=lambda k: per_kernel[k]["avg_us"])
-        b = kernel_bytes(sc, dom)
+    per_kernel = head.get("kernels", {})
+    entries = head.get("skin", {}).get("list_entries")
+    if per_kernel:
+        dom = max((k for k in per_kernel if kernel_bytes(sc, k, entries)), key=lambda k: per_kernel[k]["avg_us"])
+        b = kernel_bytes(sc, dom, entries)
         achieved = b / (per_kernel[dom]["avg_us"] * 1e-6) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": kernel_traffic(dom),
                 "bytes_per_launch": b}
-        for k, v in per_kernel.items():
-            kb = kernel_bytes(sc, k)
-            if kb:
-                v["GBs"] = round(kb / (v["avg_us"] * 1e-6) / 1e9, 1)
+        for rr in runs.values():
+            for k, v in rr.get("kernels", {}).items():
+                kb = kernel_bytes(sc, k, rr.get("skin", {}).get("list_entries"))
+                if kb:
+                    v["GBs"] = round(kb / (v["avg_us"] * 1e-6) / 1e9, 1)
     cpu = None if args.no_cpu else run_cpu_baseline(sc)
 
+    bp = {"compare": "learned skin (vs per-step rebuild)", "rebuild": "per-step rebuild", "fixed": "fixed skin",
+          "learned": "learned skin"}[args.skin]
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "rod-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 state, f64 force laws", "data": "synthetic (seeded scene)",
+        "metric": METRIC, "value": round(head["value"], 1), "unit": "rod-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(head["ms_per_step"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 state, f64 force laws", "data": "synthetic (seeded scene)",
         "config": {"workload": "crosslinked actin network: 20k filaments x 10 rods (0.1 um), 50k crosslinkers, "
-                               "20 um periodic cube, dt 1e-5 s, per-step grid rebuild",
-                   "rods": n, "links": sc.xl.nLinks, "links_alive_end": alive, "grid": list(sc.p["gridDims"][:3]),
-                   "graph": bool(args.graph), "broad_phase": "per-step rebuild"},
-        "roofline": roof, "cpu_baseline": cpu, "kernels": per_kernel,
+                               "20 um periodic cube, dt 1e-5 s",
+                   "rods": n, "links": sc.xl.nLinks, "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
+                   "broad_phase": bp},
+        "roofline": roof, "cpu_baseline": cpu,
     }
+    if "rebuild" in runs and "learned" in runs:
+        out["learned_vs_rebuild"] = round(runs["learned"]["value"] / runs["rebuild"]["value"], 3)
+        out["pair_sets_identical"] = True
+    out["runs"] = {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items() if k != "digest"}
+                   for m, r in runs.items()}
+    if train_info:
+        out["skin_training"] = train_info
     print(json.dumps(out))
-    net.close()
     ctx.close()
     return 0
 

@@ -106,12 +106,13 @@ namespace {
 
 enum {
     T_BIN = 0, T_SORT, T_PAIRS, T_UNBIND, T_COUNT, T_LINK, T_ROD, T_MBIND, T_MJOINT, T_DIMER, T_TETHER, T_BACKBONE,
-    T_BOND, T_MINTEG, T_MCSR, T_SKIN_REBUILD, T_COUNT_TIMERS
+    T_BOND, T_MINTEG, T_MCSR, T_SKIN_REBUILD, T_SKIN_PAIRS, T_COUNT_TIMERS
 };
 const char *kTimerNames[T_COUNT_TIMERS] = {"k_body_cell", "grid_sort",     "k_broad_phase", "k_net_unbind",
                                            "k_net_count", "k_net_link",    "k_rod",         "k_motor_bind",
                                            "k_motor_joints", "k_net_dimer", "k_net_tether", "k_backbone",
-                                           "k_bond_head", "k_motor_integ", "motor_csr", "skin_rebuild"};
+                                           "k_bond_head", "k_motor_integ", "motor_csr", "skin_rebuild",
+                                           "k_skin_pairs"};
 
 struct RodArgs {
     const float *coord, *uVec, *yVec;
@@ -986,7 +987,7 @@ int enqueue_skin(sbx_net *net, const float *coord, const float *uVec) {
         if ((r = check_launch(c, "k_skin_build"))) return r;
     }
     {
-        Scope sc(net, T_PAIRS);
+        Scope sc(net, T_SKIN_PAIRS);
         hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
     }
     return check_launch(c, "k_skin_pairs");

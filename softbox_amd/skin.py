@@ -28,7 +28,7 @@ class SkinConfig:
     skin_default: float = 0.1     # um
     skin_min: float = 0.02
     skin_max: float = 0.3
-    margin: float = 1.15          # learned skin = margin * exp(out0)
+    margin: float = 1.0           # learned skin = margin * exp(out0) (set by calibrate_margin)
     list_cap: int = 64
     horizon: int = 8              # steps a skin is trained to last
     window: int = 32              # steps tracked after a snapshot (censoring of the horizon label)
@@ -223,10 +223,20 @@ class TrajectorySampler:
         return torch.cat(xs).contiguous(), torch.cat(ys).contiguous(), t
 
 
+def calibrate_margin(ctx: Context, model: SkinModel, x, y, cells_per_snapshot: float) -> float:
+    """Skin margin from the training residuals: the guard rebuilds as soon as ANY cell's members
+    outrun their skin, so the skin must cover the cell's displacement at the (1 - 1/cells) quantile
+    of the residual log(label) - out0, not at its mean (conformal calibration on the samples)."""
+    pred = ctx.to_host(model.forward(x))[:, 0]
+    r = ctx.to_host(y)[:, 0] - pred
+    q = 1.0 - 1.0 / max(2.0, cells_per_snapshot)
+    return float(max(1.0, math.exp(np.quantile(r, q))))
+
+
 def fit(ctx: Context, net, cfg: SkinConfig, t0: int = 0, snapshots: int = 16, iters: int = 400, seed: int = 0,
         log_every: int = 0):
-    """Collect a trajectory of `net` (per-step rebuild) and train a skin model on it; returns
-    (model, loss history, samples, next step)."""
+    """Collect a trajectory of `net` (per-step rebuild) and train a skin model on it; sets
+    cfg.margin by calibration; returns (model, loss history, samples, next step)."""
     import torch
 
     sampler = TrajectorySampler(ctx, net, cfg)
@@ -237,4 +247,5 @@ def fit(ctx: Context, net, cfg: SkinConfig, t0: int = 0, snapshots: int = 16, it
     model = SkinModel.initial(ctx, mean.astype(np.float32), std.astype(np.float32), seed)
     torch.cuda.synchronize()
     hist = train(ctx, model, x, y, iters=iters, seed=seed, log_every=log_every)
+    cfg.margin = calibrate_margin(ctx, model, x, y, x.shape[0] / max(1, snapshots))
     return model, hist, (x, y), t
