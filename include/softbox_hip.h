@@ -346,13 +346,13 @@ int sbx_net_skin_state(sbx_net *net, float **refCoord, float **skin, int **coars
  * sum_o 0.5 * outWeight[o] * (y - target)^2; grad has the same layout as the parameters (the trailing
  * input-standardisation entries are not trained). */
 /* training data of the skin model from a trajectory (no counterpart in the reference): per coarse cell
- * features[c*12 + k] of its members' motion over H steps (coordPrev -> coordNow); per-body running
+ * (edge cellSize) features[c*12 + k] of its members' motion over H steps (coordPrev -> coordNow); per-body running
  * max displacement from x0 and first step beyond a per-body threshold; per-cell max (mode 0, float)
  * or min (mode 1, int >= 0; `fill` when none) over members; per-cell value copied to its members. */
 int sbx_skin_features(sbx_ctx *c, const int *coarseOffsets, const int *coarseContents, int nCells, int n,
                       const float *coordNow, const float *coordPrev, const float *uNow, const float *uPrev,
-                      const int *activeLinkCount, float H, int periodic, const float *boxLen /* host [3] */,
-                      float *features);
+                      const int *activeLinkCount, float H, float cellSize, int periodic,
+                      const float *boxLen /* host [3] */, float *features);
 int sbx_skin_track(sbx_ctx *c, const float *coord, const float *x0, int n, int periodic, const float *boxLen,
                    float *maxDisp, const float *threshold, int *breachStep, int step);
 int sbx_skin_cell_reduce(sbx_ctx *c, const int *coarseOffsets, const int *coarseContents, int nCells,

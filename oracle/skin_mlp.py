@@ -2,7 +2,7 @@
 
 The skin model has no counterpart in the reference; this is the mathematical definition the HIP
 kernels (softbox_amd/csrc/sbx_mlp.hip) are checked against: 12 -> 64 -> 64 -> 2, exact GELU,
-inputs standardised by stored mean/std, loss = mean over samples of sum_o 0.5 * m_o * (y - t)^2.
+inputs standardised by stored mean/std and clamped to +-6, loss = mean over samples of sum_o 0.5 * m_o * (y - t)^2.
 Parameter layout: W1[64][12] | b1[64] | W2[64][64] | b2[64] | W3[2][64] | b3[2] | xMean[12] | xStd[12].
 Its backward pass is pinned by central finite differences (tests/test_oracle_skin.py).
 """
@@ -23,6 +23,7 @@ NTRAIN = B3 + OUT
 XMEAN = NTRAIN
 XSTD = XMEAN + IN
 NPARAM = XSTD + IN
+XCLAMP = 6.0   # standardised inputs are clamped to [-XCLAMP, XCLAMP]
 
 _erf = np.vectorize(math.erf)
 
@@ -43,7 +44,7 @@ def gelu_grad(a):
 
 def forward(w, x):
     w1, b1, w2, b2, w3, b3, mu, sd = unpack(w)
-    xs = (np.asarray(x, np.float64) - mu) / sd
+    xs = np.clip((np.asarray(x, np.float64) - mu) / sd, -XCLAMP, XCLAMP)
     a1 = xs @ w1.T + b1
     h1 = gelu(a1)
     a2 = h1 @ w2.T + b2

@@ -177,7 +177,7 @@ _SIGS = {
     "sbx_net_set_skin": [vp, i32, C.POINTER(SbxSkinParams), vp],
     "sbx_net_skin_stats": [vp, C.POINTER(C.c_longlong), i32],
     "sbx_net_skin_state": [vp] + [C.POINTER(vp)] * 5 + [C.POINTER(i32), f32p],
-    "sbx_skin_features": [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, C.c_float, i32, f32p, vp],
+    "sbx_skin_features": [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, C.c_float, C.c_float, i32, f32p, vp],
     "sbx_skin_track": [vp, vp, vp, i32, i32, f32p, vp, vp, vp, i32],
     "sbx_skin_cell_reduce": [vp, vp, vp, i32, vp, vp, i32, C.c_float, vp],
     "sbx_skin_cell_scatter": [vp, vp, vp, i32, vp, vp],

@@ -53,26 +53,19 @@ __device__ __forceinline__ float min_image(float d, float L, float invL) { retur
 constexpr int BP_THREADS = 256;
 constexpr int BP_BATCH = 4;
 
-// SpatialGrid.broadPhase body loop (SpatialGrid.java:362-404): 27-cell stencil in (z, y, x)
-// order, within-cell ascending, j > i, inclusive sphere predicate; returns the realised count
-// (may exceed maxCand: surplus dropped, the count keeps going, :397-398).
-// The three x-cells of one stencil row are adjacent in the cell-sorted contents, so a row is one
-// contents range [offsets[first], offsets[last + 1]) -- two ranges when a periodic row wraps.  All
-// 18 range bounds are loaded up front (independent loads in flight together) before the ranges are
-// walked in stencil order, which visits exactly the reference's sequence of candidates.
-// segLds: 18 int2 of LDS per thread of the block (BP_THREADS x 18).
-__device__ __forceinline__ int broad_phase_body(int i, int c, const float *__restrict__ center,
-                                                const float *__restrict__ boundingRadius,
-                                                const int *__restrict__ cellOffsets,
-                                                const int *__restrict__ cellContents, const GridGeom &g, int cap,
-                                                int *__restrict__ candPartner, int maxCand, int2 *segLds) {
+// The 27-cell stencil of cell c (SpatialGrid.java:362-380: (z, y, x) order, clamped or, in the
+// periodic extension, wrapped) as contents ranges.  The three x-cells of one stencil row are
+// adjacent in the cell-sorted contents, so a row is one range [offsets[first], offsets[last + 1])
+// -- two when a periodic row wraps.  seg[18] receives the ranges in visiting order (empty ones as
+// (0, 0)); all 36 bounds are loaded before any is used, so they are in flight together.  Returns
+// the number of contents entries in the stencil.
+__device__ __forceinline__ int stencil_ranges(int c, const GridGeom &g, const int *__restrict__ cellOffsets,
+                                              int2 *seg) {
     const int nX = g.nX, nY = g.nY, nZ = g.nZ, nXY = nX * nY;
     const int cz = c / nXY;
     const int rem = c - cz * nXY;
     const int cy = rem / nX;
     const int cx = rem - cy * nX;
-    const float icx = center[i], icy = center[cap + i], icz = center[2 * cap + i];
-    const float ri = boundingRadius[i];
     // x extent of every row: [xa0, xa1] then (periodic wrap only) [xb0, xb1]
     int xa0, xa1, xb0 = 0, xb1 = -1;
     if (!g.periodic) {
@@ -86,8 +79,6 @@ __device__ __forceinline__ int broad_phase_body(int i, int c, const float *__res
         xa0 = cx - 1; xa1 = cx + 1;
     }
     const bool twoSeg = xb1 >= xb0;
-    // Range bounds go to this thread's LDS row so the walk below can index them dynamically.
-    int2 *seg = segLds + threadIdx.x * 18;
     int total = 0;
 #pragma unroll
     for (int r = 0; r < 9; r++) {
@@ -109,28 +100,49 @@ __device__ __forceinline__ int broad_phase_body(int i, int c, const float *__res
         seg[2 * r + 1] = s1;
         total += (s0.y - s0.x) + (s1.y - s1.x);
     }
-    // One flat loop over all candidates (the wave iterates max over lanes of the candidate total,
-    // not the sum over the 18 ranges of each range's maximum), BP_BATCH candidates at a time so
-    // their contents and centre loads are in flight together; tests and appends stay in order.
+    return total;
+}
+
+// Walks the contents indices of stencil_ranges in order; next() may be called exactly `total`
+// times (the ranges live in LDS so they can be indexed dynamically).
+struct RangeWalk {
+    const int2 *seg;
+    int s, idx, end;
+    __device__ explicit RangeWalk(const int2 *sg) : seg(sg), s(0), idx(sg[0].x), end(sg[0].y) {}
+    __device__ int next() {
+        while (idx >= end && s < 17) {
+            s++;
+            int2 e = seg[s];
+            idx = e.x;
+            end = e.y;
+        }
+        return idx++;
+    }
+};
+
+// SpatialGrid.broadPhase body loop (SpatialGrid.java:362-404): 27-cell stencil in (z, y, x)
+// order, within-cell ascending, j > i, inclusive sphere predicate; returns the realised count
+// (may exceed maxCand: surplus dropped, the count keeps going, :397-398).  One flat loop over all
+// candidates (the wave iterates max over lanes of the candidate total, not the sum over the 18
+// ranges of each range's maximum), BP_BATCH candidates at a time so their contents and centre
+// loads are in flight together; tests and appends stay in order.
+// segLds: 18 int2 of LDS per thread of the block (BP_THREADS x 18).
+__device__ __forceinline__ int broad_phase_body(int i, int c, const float *__restrict__ center,
+                                                const float *__restrict__ boundingRadius,
+                                                const int *__restrict__ cellOffsets,
+                                                const int *__restrict__ cellContents, const GridGeom &g, int cap,
+                                                int *__restrict__ candPartner, int maxCand, int2 *segLds) {
+    const float icx = center[i], icy = center[cap + i], icz = center[2 * cap + i];
+    const float ri = boundingRadius[i];
+    int2 *seg = segLds + threadIdx.x * 18;
+    const int total = stencil_ranges(c, g, cellOffsets, seg);
     int cnt = 0;
     int *slice = candPartner + (size_t)i * (size_t)maxCand;
-    int s = 0, idx = seg[0].x, end = seg[0].y;
+    RangeWalk w(seg);
     for (int q = 0; q < total; q += BP_BATCH) {
         int jj[BP_BATCH];
 #pragma unroll
-        for (int u = 0; u < BP_BATCH; u++) {
-            jj[u] = -1;
-            if (q + u < total) {
-                while (idx >= end && s < 17) {
-                    s++;
-                    int2 e = seg[s];
-                    idx = e.x;
-                    end = e.y;
-                }
-                jj[u] = cellContents[idx];
-                idx++;
-            }
-        }
+        for (int u = 0; u < BP_BATCH; u++) jj[u] = (q + u < total) ? cellContents[w.next()] : -1;
         float px[BP_BATCH], py[BP_BATCH], pz[BP_BATCH], pr[BP_BATCH];
 #pragma unroll
         for (int u = 0; u < BP_BATCH; u++) {

@@ -8,8 +8,9 @@
 //
 // Parameter layout (one float array of MLP_NPARAM):
 //   W1[64][12] | b1[64] | W2[64][64] | b2[64] | W3[2][64] | b3[2] | xMean[12] | xStd[12]
-// xMean/xStd standardise the inputs (x - mean) / std before the first layer; they are data
-// statistics, not trained.
+// xMean/xStd standardise the inputs (x - mean) / std before the first layer, clamped to
+// [-MLP_XCLAMP, MLP_XCLAMP] so an input far outside the training data cannot dominate; they are
+// data statistics, not trained.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -26,6 +27,11 @@ constexpr int MLP_NTRAIN = MLP_B3 + MLP_OUT;       // trained parameters
 constexpr int MLP_XMEAN = MLP_NTRAIN;
 constexpr int MLP_XSTD = MLP_XMEAN + MLP_IN;
 constexpr int MLP_NPARAM = MLP_XSTD + MLP_IN;
+constexpr float MLP_XCLAMP = 6.0f;
+
+__device__ __forceinline__ float mlp_standardise(float x, float mean, float sd) {
+    return fminf(MLP_XCLAMP, fmaxf(-MLP_XCLAMP, (x - mean) / sd));
+}
 
 // exact GELU: x * Phi(x) = 0.5 x (1 + erf(x / sqrt 2)), and its derivative Phi(x) + x phi(x)
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
@@ -42,7 +48,7 @@ __device__ __forceinline__ void mlp_forward(const float *__restrict__ w, const f
                                             float y[MLP_OUT]) {
     float x[MLP_IN];
 #pragma unroll
-    for (int k = 0; k < MLP_IN; k++) x[k] = (x_raw[k] - w[MLP_XMEAN + k]) / w[MLP_XSTD + k];
+    for (int k = 0; k < MLP_IN; k++) x[k] = mlp_standardise(x_raw[k], w[MLP_XMEAN + k], w[MLP_XSTD + k]);
     float acc[MLP_H];
 #pragma unroll
     for (int j = 0; j < MLP_H; j++) acc[j] = w[MLP_B2 + j];

@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(T) k_mlp_fwd_bwd(const float *__restrict__ w, 
     float xs[MLP_IN];
 #pragma unroll
     for (int k = 0; k < MLP_IN; k++) {
-        xs[k] = (x[(size_t)b * MLP_IN + k] - ws[MLP_XMEAN + k]) / ws[MLP_XSTD + k];
+        xs[k] = mlp_standardise(x[(size_t)b * MLP_IN + k], ws[MLP_XMEAN + k], ws[MLP_XSTD + k]);
         s.xs[(size_t)b * MLP_IN + k] = xs[k];
     }
     float acc[MLP_H];

@@ -85,15 +85,16 @@ struct sbx_net {
     // skin broad phase (sbx_net_set_skin)
     int skinMode = 0;                       // SBX_SKIN_REBUILD / _FIXED / _LEARNED
     sbx_skin_params sp{};
-    GridGeom cgeom{};                       // coarse rebuild grid (cell >= largest list reach)
-    int cCells = 0;
+    GridGeom *cgDev = nullptr;              // coarse rebuild grid of the last rebuild (device)
+    int *cCellsDev = nullptr;               // its cell count (device)
+    int cCellsMax = 0;                      // cells at the smallest coarse cell (allocation)
+    float maxR = 0.0f, cellFloor = 0.0f;
     int *cCell = nullptr, *cOff = nullptr, *cCont = nullptr;  // coarse CSR of the last rebuild
     float *refC = nullptr, *refU = nullptr, *skin = nullptr;   // pose and skins at the last rebuild
     float *cellSkin = nullptr, *cellHorizon = nullptr;
     int *vlist = nullptr, *vcount = nullptr;
     int *skinFlags = nullptr;
     float *model = nullptr;                 // MLP_NPARAM device floats (learned mode)
-    float meanOcc = 1.0f;
     // graph
     int useGraph = 0;
     int graphSteps = 0;  // non-zero while a capture is being recorded
@@ -570,101 +571,139 @@ __global__ void __launch_bounds__(256) k_motor_integ(sbx_motors m, float dt, flo
 
 // ---------------------------------------------------------------- skin broad phase kernels
 // Every kernel but k_skin_pairs is gated on the device flag SKF_REBUILD, so a captured step
-// rebuilds exactly when the guard (in k_rod) fired, with no host round trip.
+// rebuilds exactly when the guard (in k_rod) fired, with no host round trip.  The coarse (rebuild)
+// grid is sized on the device at each rebuild from the largest skin in use, so small skins get a
+// small neighbourhood to scan.
 struct SkinArgs {
-    int n, mode, listCap, maxCand;
+    int n, mode, listCap, maxCand, cCellsMax;
     const float *coord, *uVec, *boundR;
     const int *links;        // activeLinkCount (may be null)
-    GridGeom geom, cgeom;    // fine (the broad phase's) and coarse (rebuild) grids
-    int cCells;
+    GridGeom geom;           // fine grid (the broad phase's)
+    GridGeom *cg;            // coarse grid of the last rebuild (device)
+    int *cCells;             // its cell count (device)
     int *cCell, *cOff, *cCont;
     float *refC, *refU, *skin, *cellSkin, *cellHorizon;
     int *vlist, *vcount;
     int *flags;
     const float *model;
-    float skinDefault, skinMin, skinMax, margin, meanOcc;
+    float skinDefault, skinMin, skinMax, margin;
+    float maxR, cellFloor;   // largest bounding radius; smallest coarse cell (fine cell or skinMin reach)
     const int *bodyCell;     // fine cells of the current pose
     int *candPartner, *candCount;
 };
+
+constexpr float SKIN_SLACK = 1.0e-3f;  // um, covers float rounding of the list and guard distances
+constexpr int SKF_MAXSKIN = 6;         // flag word: largest skin of the rebuild (float bits)
 
 // Per coarse cell of the last rebuild: the model's skin for its members (learned mode, after the
 // first rebuild); otherwise the default skin.
 __global__ void __launch_bounds__(256) k_skin_model(SkinArgs a) {
     if (a.flags[SKF_REBUILD] == 0) return;
     int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= a.cCells) return;
-    float sk = a.skinDefault, hz = 0.0f;
-    if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && a.model) {
-        float f[SKIN_FEATURES], y[MLP_OUT];
-        float H = (float)(a.flags[SKF_SINCE] > 0 ? a.flags[SKF_SINCE] : 1);
-        skin_cell_features(c, a.cOff, a.cCont, a.n, a.coord, a.refC, a.uVec, a.refU, a.links, H, a.meanOcc, a.geom,
-                           f);
-        mlp_forward(a.model, f, y);
-        sk = fminf(a.skinMax, fmaxf(a.skinMin, a.margin * expf(y[0])));
-        hz = expf(y[1]);
-    }
-    a.cellSkin[c] = sk;
-    a.cellHorizon[c] = hz;
+    if (c == 0) a.flags[SKF_MAXSKIN] = 0;
+    if (!(a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && a.model)) return;
+    if (c >= *a.cCells) return;
+    float f[SKIN_FEATURES], y[MLP_OUT];
+    float H = (float)(a.flags[SKF_SINCE] > 0 ? a.flags[SKF_SINCE] : 1);
+    skin_cell_features(c, a.cOff, a.cCont, a.n, a.coord, a.refC, a.uVec, a.refU, a.links, H, 1.0f / a.cg->invCell,
+                       a.geom, f);
+    mlp_forward(a.model, f, y);
+    a.cellSkin[c] = fminf(a.skinMax, fmaxf(a.skinMin, a.margin * expf(y[0])));
+    a.cellHorizon[c] = expf(y[1]);
 }
 
-// per body: skin from its coarse cell of the last rebuild, then its coarse cell now
-__global__ void __launch_bounds__(256) k_skin_assign_bin(SkinArgs a) {
+// per body: its skin (from its coarse cell of the last rebuild, or the default) and the running max
+__global__ void __launch_bounds__(256) k_skin_assign(SkinArgs a) {
     if (a.flags[SKF_REBUILD] == 0) return;
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n) return;
     float sk = a.skinDefault;
     if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && a.model) sk = a.cellSkin[a.cCell[i]];
     a.skin[i] = sk;
-    const int n = a.n;
-    a.cCell[i] = cell_of(a.coord[i], a.coord[n + i], a.coord[2 * n + i], a.cgeom);
+    atomicMax(&a.flags[SKF_MAXSKIN], __float_as_int(sk));  // positive floats order as ints
 }
 
-// Neighbour lists: every j > i within ri + rj + cutoff + s_i + s_j (+1 nm for float rounding) of i,
-// from the 27 coarse cells around i (the coarse cell is at least the largest such reach).  Also
-// records the pose the guard measures displacement from.
-constexpr float SKIN_SLACK = 1.0e-3f;  // um
-__global__ void __launch_bounds__(256) k_skin_build(SkinArgs a) {
+// the coarse grid of this rebuild: cell >= 2 maxR + cutoff + 2 maxSkin (+ slack), covering the
+// fine grid's extent (clamped) or tiling the cubic box (periodic, >= 3 cells per axis)
+__global__ void k_skin_geom(SkinArgs a) {
     if (a.flags[SKF_REBUILD] == 0) return;
-    int t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    float maxSkin = __int_as_float(a.flags[SKF_MAXSKIN]);
+    float need = 2.0f * a.maxR + a.geom.cutoff + 2.0f * maxSkin + 2.0f * SKIN_SLACK;
+    float cell = fmaxf(need, a.cellFloor);
+    GridGeom g = a.geom;
+    if (g.periodic) {
+        int nb = (int)floorf(g.Lx / cell);
+        if (nb < 3) nb = 3;  // skinMax was checked against the box on the host
+        g.nX = g.nY = g.nZ = nb;
+        g.invCell = (float)nb / g.Lx;
+    } else {
+        float ext = (float)a.geom.nX / a.geom.invCell, eyt = (float)a.geom.nY / a.geom.invCell,
+              ezt = (float)a.geom.nZ / a.geom.invCell;
+        g.nX = max(1, (int)ceilf(ext / cell));
+        g.nY = max(1, (int)ceilf(eyt / cell));
+        g.nZ = max(1, (int)ceilf(ezt / cell));
+        g.invCell = 1.0f / cell;
+    }
+    long long nc = (long long)g.nX * g.nY * g.nZ;
+    if (nc > a.cCellsMax) {  // cannot happen: cell >= cellFloor bounds the count (host-checked)
+        g.nX = g.nY = g.nZ = 1;
+        nc = 1;
+    }
+    *a.cg = g;
+    *a.cCells = (int)nc;
+}
+
+__global__ void __launch_bounds__(256) k_skin_bin(SkinArgs a) {
+    if (a.flags[SKF_REBUILD] == 0) return;
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const int n = a.n;
+    a.cCell[i] = cell_of(a.coord[i], a.coord[n + i], a.coord[2 * n + i], *a.cg);
+}
+
+// Neighbour lists: every j > i within ri + rj + cutoff + s_i + s_j (+ slack) of i, from the 27
+// coarse cells around i.  Also records the pose the guard measures displacement from.
+__global__ void __launch_bounds__(BP_THREADS) k_skin_build(SkinArgs a) {
+    if (a.flags[SKF_REBUILD] == 0) return;
+    __shared__ int2 segLds[BP_THREADS * 18];
+    int t = xcd_block(blockIdx.x, gridDim.x) * BP_THREADS + threadIdx.x;
     const int n = a.n;
     if (t >= n) return;
+    const GridGeom g = *a.cg;
     int i = a.cCont[t];  // coarse-cell order: neighbouring lanes walk neighbouring cells
-    const GridGeom &g = a.cgeom;
     float icx = a.coord[i], icy = a.coord[n + i], icz = a.coord[2 * n + i];
-    float ri = a.boundR[i], si = a.skin[i];
-    int c = a.cCell[i];
-    const int nXY = g.nX * g.nY;
-    int cz = c / nXY, rem = c - cz * nXY, cy = rem / g.nX, cx = rem - cy * g.nX;
+    float base = a.boundR[i] + a.geom.cutoff + a.skin[i] + SKIN_SLACK;
+    int2 *seg = segLds + threadIdx.x * 18;
+    const int total = stencil_ranges(a.cCell[i], g, a.cOff, seg);
+    RangeWalk w(seg);
     int cnt = 0;
     int *list = a.vlist + (size_t)i * a.listCap;
-    for (int dz = -1; dz <= 1; dz++) {
-        int zz = cz + dz;
-        if (g.periodic) zz = zz < 0 ? zz + g.nZ : (zz >= g.nZ ? zz - g.nZ : zz);
-        else if (zz < 0 || zz >= g.nZ) continue;
-        for (int dy = -1; dy <= 1; dy++) {
-            int yy = cy + dy;
-            if (g.periodic) yy = yy < 0 ? yy + g.nY : (yy >= g.nY ? yy - g.nY : yy);
-            else if (yy < 0 || yy >= g.nY) continue;
-            for (int dx = -1; dx <= 1; dx++) {
-                int xx = cx + dx;
-                if (g.periodic) xx = xx < 0 ? xx + g.nX : (xx >= g.nX ? xx - g.nX : xx);
-                else if (xx < 0 || xx >= g.nX) continue;
-                int cc = xx + yy * g.nX + zz * nXY;
-                for (int p = a.cOff[cc]; p < a.cOff[cc + 1]; p++) {
-                    int j = a.cCont[p];
-                    if (j <= i) continue;
-                    float ddx = icx - a.coord[j], ddy = icy - a.coord[n + j], ddz = icz - a.coord[2 * n + j];
-                    if (g.periodic) {
-                        ddx = min_image(ddx, g.Lx, g.iLx);
-                        ddy = min_image(ddy, g.Ly, g.iLy);
-                        ddz = min_image(ddz, g.Lz, g.iLz);
-                    }
-                    float R = ri + a.boundR[j] + a.geom.cutoff + si + a.skin[j] + SKIN_SLACK;
-                    if (ddx * ddx + ddy * ddy + ddz * ddz <= R * R) {
-                        if (cnt < a.listCap) list[cnt] = j;
-                        cnt++;
-                    }
-                }
+    for (int q = 0; q < total; q += BP_BATCH) {
+        int jj[BP_BATCH];
+#pragma unroll
+        for (int u = 0; u < BP_BATCH; u++) jj[u] = (q + u < total) ? a.cCont[w.next()] : -1;
+        float px[BP_BATCH], py[BP_BATCH], pz[BP_BATCH], pr[BP_BATCH];
+#pragma unroll
+        for (int u = 0; u < BP_BATCH; u++) {
+            int j = jj[u] > i ? jj[u] : i;
+            px[u] = a.coord[j];
+            py[u] = a.coord[n + j];
+            pz[u] = a.coord[2 * n + j];
+            pr[u] = a.boundR[j] + a.skin[j];
+        }
+#pragma unroll
+        for (int u = 0; u < BP_BATCH; u++) {
+            if (jj[u] <= i) continue;
+            float dx = icx - px[u], dy = icy - py[u], dz = icz - pz[u];
+            if (g.periodic) {
+                dx = min_image(dx, g.Lx, g.iLx);
+                dy = min_image(dy, g.Ly, g.iLy);
+                dz = min_image(dz, g.Lz, g.iLz);
+            }
+            float R = base + pr[u];
+            if (dx * dx + dy * dy + dz * dz <= R * R) {
+                if (cnt < a.listCap) list[cnt] = jj[u];
+                cnt++;
             }
         }
     }
@@ -678,7 +717,7 @@ __global__ void __launch_bounds__(256) k_skin_build(SkinArgs a) {
     atomicAdd(&a.flags[SKF_ENTRIES], cnt < a.listCap ? cnt : a.listCap);
 }
 
-// after the model and the binning (which read SINCE and REBUILDS), before the build: the
+// after the model and the assignment (which read SINCE and REBUILDS), before the build: the
 // counters of the new rebuild
 __global__ void k_skin_begin(int *flags) {
     if (flags[SKF_REBUILD] == 0) return;
@@ -693,30 +732,33 @@ __global__ void __launch_bounds__(256) k_max_radius(const float *__restrict__ r,
     if (i < n && r[i] > 0.0f) atomicMax(out, __float_as_int(r[i]));  // positive floats order as ints
 }
 
-// One body's candidate test (the broad phase's predicate, same arithmetic) and its sort key
-// (fine stencil position, index); -1 when j is not a candidate of SpatialGrid.broadPhase.
-__device__ __forceinline__ int skin_key(int i, int j, int ci, float icx, float icy, float icz, float ri,
-                                        const SkinArgs &a) {
-    const int n = a.n;
-    const GridGeom &g = a.geom;
-    float dx = icx - a.coord[j], dy = icy - a.coord[n + j], dz = icz - a.coord[2 * n + j];
+// One candidate test (the broad phase's predicate, same arithmetic) and its sort key (fine stencil
+// position, index); -1 when j is not a candidate of SpatialGrid.broadPhase.
+__device__ __forceinline__ int skin_key(int j, int ci, float icx, float icy, float icz, float ri, float pxj,
+                                        float pyj, float pzj, float rj, int cj, const GridGeom &g) {
+    float dx = icx - pxj, dy = icy - pyj, dz = icz - pzj;
     if (g.periodic) {
         dx = min_image(dx, g.Lx, g.iLx);
         dy = min_image(dy, g.Ly, g.iLy);
         dz = min_image(dz, g.Lz, g.iLz);
     }
     float distSq = dx * dx + dy * dy + dz * dz;
-    float reach = ri + a.boundR[j] + g.cutoff;
+    float reach = ri + rj + g.cutoff;
     if (!(distSq <= reach * reach)) return -1;
-    int pos = stencil_pos(ci, a.bodyCell[j], g);
+    int pos = stencil_pos(ci, cj, g);
     if (pos < 0) return -1;
     return (pos << 26) | j;
 }
 
+constexpr int SKIN_KEYS = 24;  // per-thread LDS key buffer of k_skin_pairs
+
 // Candidate slices of the current pose from the neighbour lists (or, for a body whose list
-// overflowed, from the 27 coarse cells of the last rebuild), in the broad phase's order.
-__global__ void __launch_bounds__(256) k_skin_pairs(SkinArgs a) {
-    int t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+// overflowed, from the 27 coarse cells of the last rebuild), in the broad phase's order.  Keys are
+// gathered in LDS and sorted there; a body with more than SKIN_KEYS candidates sorts in its slice.
+__global__ void __launch_bounds__(BP_THREADS) k_skin_pairs(SkinArgs a) {
+    __shared__ int keyLds[BP_THREADS * SKIN_KEYS];
+    __shared__ int2 segLds[BP_THREADS * 18];
+    int t = xcd_block(blockIdx.x, gridDim.x) * BP_THREADS + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // no other thread reads these words in this kernel
         a.flags[SKF_REBUILD] = 0;
         a.flags[SKF_SINCE] += 1;
@@ -724,52 +766,64 @@ __global__ void __launch_bounds__(256) k_skin_pairs(SkinArgs a) {
     }
     const int n = a.n;
     if (t >= n) return;
+    const GridGeom &g = a.geom;
     int i = a.cCont[t];
     float icx = a.coord[i], icy = a.coord[n + i], icz = a.coord[2 * n + i];
     float ri = a.boundR[i];
     int ci = a.bodyCell[i];
+    int *keys = keyLds + threadIdx.x * SKIN_KEYS;
     int *slice = a.candPartner + (size_t)i * a.maxCand;
-    int m = 0;     // candidates found (may exceed maxCand)
+    const int cap = a.maxCand < SKIN_KEYS ? a.maxCand : SKIN_KEYS;
+    int m = 0;  // candidates found (may exceed the buffer and maxCand)
     int cnt = a.vcount[i];
-    if (cnt <= a.listCap) {
-        const int *list = a.vlist + (size_t)i * a.listCap;
-        for (int k = 0; k < cnt; k++) {
-            int key = skin_key(i, list[k], ci, icx, icy, icz, ri, a);
+    const bool overflow = cnt > a.listCap;
+    const int *list = a.vlist + (size_t)i * a.listCap;
+    int total = cnt;
+    int2 *seg = segLds + threadIdx.x * 18;
+    if (overflow) total = stencil_ranges(a.cCell[i], *a.cg, a.cOff, seg);
+    RangeWalk w(seg);
+    for (int q = 0; q < total; q += BP_BATCH) {
+        int jj[BP_BATCH];
+#pragma unroll
+        for (int u = 0; u < BP_BATCH; u++) {
+            int j = -1;
+            if (q + u < total) j = overflow ? a.cCont[w.next()] : list[q + u];
+            jj[u] = j;
+        }
+        float px[BP_BATCH], py[BP_BATCH], pz[BP_BATCH], pr[BP_BATCH];
+        int pc[BP_BATCH];
+#pragma unroll
+        for (int u = 0; u < BP_BATCH; u++) {
+            int j = jj[u] > i ? jj[u] : i;
+            px[u] = a.coord[j];
+            py[u] = a.coord[n + j];
+            pz[u] = a.coord[2 * n + j];
+            pr[u] = a.boundR[j];
+            pc[u] = a.bodyCell[j];
+        }
+#pragma unroll
+        for (int u = 0; u < BP_BATCH; u++) {
+            if (jj[u] <= i) continue;
+            int key = skin_key(jj[u], ci, icx, icy, icz, ri, px[u], py[u], pz[u], pr[u], pc[u], g);
             if (key < 0) continue;
-            if (m < a.maxCand) slice[m] = key;
+            if (m < cap) keys[m] = key;
+            else if (m < a.maxCand) slice[m] = key;
             m++;
         }
-    } else {
-        const GridGeom &g = a.cgeom;
-        const int nXY = g.nX * g.nY;
-        int c = a.cCell[i];
-        int cz = c / nXY, rem = c - cz * nXY, cy = rem / g.nX, cx = rem - cy * g.nX;
-        for (int dz = -1; dz <= 1; dz++) {
-            int zz = cz + dz;
-            if (g.periodic) zz = zz < 0 ? zz + g.nZ : (zz >= g.nZ ? zz - g.nZ : zz);
-            else if (zz < 0 || zz >= g.nZ) continue;
-            for (int dy = -1; dy <= 1; dy++) {
-                int yy = cy + dy;
-                if (g.periodic) yy = yy < 0 ? yy + g.nY : (yy >= g.nY ? yy - g.nY : yy);
-                else if (yy < 0 || yy >= g.nY) continue;
-                for (int dx = -1; dx <= 1; dx++) {
-                    int xx = cx + dx;
-                    if (g.periodic) xx = xx < 0 ? xx + g.nX : (xx >= g.nX ? xx - g.nX : xx);
-                    else if (xx < 0 || xx >= g.nX) continue;
-                    int cc = xx + yy * g.nX + zz * nXY;
-                    for (int p = a.cOff[cc]; p < a.cOff[cc + 1]; p++) {
-                        int j = a.cCont[p];
-                        if (j <= i) continue;
-                        int key = skin_key(i, j, ci, icx, icy, icz, ri, a);
-                        if (key < 0) continue;
-                        if (m < a.maxCand) slice[m] = key;
-                        m++;
-                    }
-                }
-            }
-        }
     }
-    if (m <= a.maxCand) {  // insertion sort of the keys: (stencil position, index) ascending
+    if (m <= cap) {  // the common case: insertion sort in LDS, one pass of writes
+        for (int x = 1; x < m; x++) {
+            int v = keys[x];
+            int y = x - 1;
+            while (y >= 0 && keys[y] > v) {
+                keys[y + 1] = keys[y];
+                y--;
+            }
+            keys[y + 1] = v;
+        }
+        for (int x = 0; x < m; x++) slice[x] = keys[x] & ((1 << 26) - 1);
+    } else if (m <= a.maxCand) {  // more than the LDS buffer: sort in the slice
+        for (int x = 0; x < cap; x++) slice[x] = keys[x];
         for (int x = 1; x < m; x++) {
             int v = slice[x];
             int y = x - 1;
@@ -782,43 +836,17 @@ __global__ void __launch_bounds__(256) k_skin_pairs(SkinArgs a) {
         for (int x = 0; x < m; x++) slice[x] &= (1 << 26) - 1;
     } else {
         // more candidates than slots: the broad phase keeps the first maxCand in its order, so emit
-        // the maxCand smallest keys by repeated selection over the list (rare; exact)
-        const int *list = a.vlist + (size_t)i * a.listCap;
+        // the maxCand smallest keys by repeated selection over the candidates (rare; exact)
         int last = -1;
         for (int x = 0; x < a.maxCand; x++) {
             int best = 0x7fffffff;
-            if (cnt <= a.listCap) {
-                for (int k = 0; k < cnt; k++) {
-                    int key = skin_key(i, list[k], ci, icx, icy, icz, ri, a);
-                    if (key > last && key < best) best = key;
-                }
-            } else {
-                const GridGeom &g = a.cgeom;
-                const int nXY = g.nX * g.nY;
-                int c = a.cCell[i];
-                int cz = c / nXY, rem = c - cz * nXY, cy = rem / g.nX, cx = rem - cy * g.nX;
-                for (int dz = -1; dz <= 1; dz++) {
-                    int zz = cz + dz;
-                    if (g.periodic) zz = zz < 0 ? zz + g.nZ : (zz >= g.nZ ? zz - g.nZ : zz);
-                    else if (zz < 0 || zz >= g.nZ) continue;
-                    for (int dy = -1; dy <= 1; dy++) {
-                        int yy = cy + dy;
-                        if (g.periodic) yy = yy < 0 ? yy + g.nY : (yy >= g.nY ? yy - g.nY : yy);
-                        else if (yy < 0 || yy >= g.nY) continue;
-                        for (int dx = -1; dx <= 1; dx++) {
-                            int xx = cx + dx;
-                            if (g.periodic) xx = xx < 0 ? xx + g.nX : (xx >= g.nX ? xx - g.nX : xx);
-                            else if (xx < 0 || xx >= g.nX) continue;
-                            int cc = xx + yy * g.nX + zz * nXY;
-                            for (int p = a.cOff[cc]; p < a.cOff[cc + 1]; p++) {
-                                int j = a.cCont[p];
-                                if (j <= i) continue;
-                                int key = skin_key(i, j, ci, icx, icy, icz, ri, a);
-                                if (key > last && key < best) best = key;
-                            }
-                        }
-                    }
-                }
+            RangeWalk w2(seg);
+            for (int q = 0; q < total; q++) {
+                int j = overflow ? a.cCont[w2.next()] : list[q];
+                if (j <= i) continue;
+                int key = skin_key(j, ci, icx, icy, icz, ri, a.coord[j], a.coord[n + j], a.coord[2 * n + j],
+                                   a.boundR[j], a.bodyCell[j], g);
+                if (key > last && key < best) best = key;
             }
             slice[x] = best & ((1 << 26) - 1);
             last = best;
@@ -953,21 +981,24 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
 SkinArgs skin_args(sbx_net *net, const float *coord, const float *uVec) {
     SkinArgs a;
     a.n = net->n; a.mode = net->skinMode; a.listCap = net->sp.listCap; a.maxCand = net->p.maxCand;
+    a.cCellsMax = net->cCellsMax;
     a.coord = coord; a.uVec = uVec; a.boundR = net->boundR;
     a.links = net->L > 0 ? net->xl.activeLinkCount : nullptr;
-    a.geom = net->geom; a.cgeom = net->cgeom; a.cCells = net->cCells;
+    a.geom = net->geom; a.cg = net->cgDev; a.cCells = net->cCellsDev;
     a.cCell = net->cCell; a.cOff = net->cOff; a.cCont = net->cCont;
     a.refC = net->refC; a.refU = net->refU; a.skin = net->skin;
     a.cellSkin = net->cellSkin; a.cellHorizon = net->cellHorizon;
     a.vlist = net->vlist; a.vcount = net->vcount; a.flags = net->skinFlags; a.model = net->model;
     a.skinDefault = net->sp.skinDefault; a.skinMin = net->sp.skinMin; a.skinMax = net->sp.skinMax;
-    a.margin = net->sp.margin; a.meanOcc = net->meanOcc;
+    a.margin = net->sp.margin;
+    a.maxR = net->maxR; a.cellFloor = net->cellFloor;
     a.bodyCell = net->bodyCell;
     a.candPartner = net->candPartner; a.candCount = net->candCount;
     return a;
 }
 
-// skin broad phase of one step: [gated rebuild: model, assign + bin, coarse sort, build] -> pairs
+// skin broad phase of one step: [gated rebuild: model, assign, geometry, bin, coarse sort, build]
+// -> pairs
 int enqueue_skin(sbx_net *net, const float *coord, const float *uVec) {
     sbx_ctx *c = net->ctx;
     hipStream_t s = c->stream;
@@ -976,19 +1007,21 @@ int enqueue_skin(sbx_net *net, const float *coord, const float *uVec) {
     int r;
     {
         Scope sc(net, T_SKIN_REBUILD);
-        hipLaunchKernelGGL(k_skin_model, dim3(blocks_for(net->cCells, 256)), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_skin_assign_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
-        if ((r = check_launch(c, "k_skin_model/assign"))) return r;
-        if ((r = counting_sort(c, net->sort, net->cCell, n, net->cCells, n, net->cOff, net->cCont,
+        hipLaunchKernelGGL(k_skin_model, dim3(blocks_for(net->cCellsMax, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_skin_assign, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_skin_geom, dim3(1), dim3(1), 0, s, a);
+        hipLaunchKernelGGL(k_skin_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+        if ((r = check_launch(c, "skin model/assign/geom/bin"))) return r;
+        if ((r = counting_sort(c, net->sort, net->cCell, n, net->cCellsMax, n, net->cOff, net->cCont,
                                net->skinFlags + SKF_REBUILD)))
             return r;
         hipLaunchKernelGGL(k_skin_begin, dim3(1), dim3(1), 0, s, net->skinFlags);
-        hipLaunchKernelGGL(k_skin_build, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_skin_build, dim3(blocks_for(n, BP_THREADS)), dim3(BP_THREADS), 0, s, a);
         if ((r = check_launch(c, "k_skin_build"))) return r;
     }
     {
         Scope sc(net, T_SKIN_PAIRS);
-        hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(n, BP_THREADS)), dim3(BP_THREADS), 0, s, a);
     }
     return check_launch(c, "k_skin_pairs");
 }
@@ -1110,7 +1143,8 @@ void release(sbx_net *net) {
                     net->idxB,     net->dStep,     net->ownerStore, net->ownerSlot,
                     net->mOff,     net->mList,     net->bbOff,    net->bbList,
                     net->cCell,    net->cOff,      net->cCont,    net->refC,    net->refU, net->skin,
-                    net->cellSkin, net->cellHorizon, net->vlist,  net->vcount,  net->skinFlags, net->model};
+                    net->cellSkin, net->cellHorizon, net->vlist,  net->vcount,  net->skinFlags, net->model,
+                    net->cgDev,    net->cCellsDev};
     for (void *b : bufs)
         if (b) hipFree(b);
     sort_scratch_free(net->sort);
@@ -1479,39 +1513,35 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
     memcpy(&maxR, &hmax, sizeof(float));
     const GridGeom &f = net->geom;
     float fineCell = 1.0f / f.invCell;
-    float need = 2.0f * maxR + f.cutoff + 2.0f * sp->skinMax + 2.0f * SKIN_SLACK;
-    float cell = need > fineCell ? need : fineCell;
-    GridGeom g = f;
+    // the coarse cell of a rebuild is 2 maxR + cutoff + 2 maxSkin (+ slack), at least the fine cell;
+    // allocate for the smallest (skinMin), check the largest (skinMax) still fits the box
+    float floorCell = fmaxf(fineCell, 2.0f * maxR + f.cutoff + 2.0f * sp->skinMin + 2.0f * SKIN_SLACK);
+    float topCell = fmaxf(fineCell, 2.0f * maxR + f.cutoff + 2.0f * sp->skinMax + 2.0f * SKIN_SLACK);
+    long long cc;
     if (f.periodic) {
-        int nx = (int)floorf(f.Lx / cell), ny = (int)floorf(f.Ly / cell), nz = (int)floorf(f.Lz / cell);
-        SBX_REQUIRE(c, nx >= 3 && ny >= 3 && nz >= 3,
-                    "net_set_skin: box too small for skinMax %g (coarse cell %g um needs >= 3 per axis)",
-                    sp->skinMax, cell);
-        g.nX = nx; g.nY = ny; g.nZ = nz;
-        // cubic cells of the largest size that tiles the shortest... each axis tiles its own length
-        cell = fmaxf(f.Lx / nx, fmaxf(f.Ly / ny, f.Lz / nz));
-        g.invCell = 1.0f / cell;
-        g.xMin = f.xMin; g.yMin = f.yMin; g.zMin = f.zMin;
+        SBX_REQUIRE(c, f.Lx == f.Ly && f.Ly == f.Lz, "net_set_skin: the periodic box must be cubic");
+        SBX_REQUIRE(c, (int)floorf(f.Lx / topCell) >= 3,
+                    "net_set_skin: box %g um too small for skinMax %g um (needs 3 coarse cells of %g um)", f.Lx,
+                    sp->skinMax, topCell);
+        long long nb = (long long)floorf(f.Lx / floorCell);
+        cc = nb * nb * nb;
     } else {
-        float ex = fineCell * f.nX, ey = fineCell * f.nY, ez = fineCell * f.nZ;
-        g.nX = (int)ceilf(ex / cell); g.nY = (int)ceilf(ey / cell); g.nZ = (int)ceilf(ez / cell);
-        if (g.nX < 1) g.nX = 1;
-        if (g.nY < 1) g.nY = 1;
-        if (g.nZ < 1) g.nZ = 1;
-        g.invCell = 1.0f / cell;
+        cc = (long long)ceilf(fineCell * f.nX / floorCell) * (long long)ceilf(fineCell * f.nY / floorCell) *
+             (long long)ceilf(fineCell * f.nZ / floorCell);
     }
-    long long cc = (long long)g.nX * g.nY * g.nZ;
-    SBX_REQUIRE(c, cc < (1LL << 30), "net_set_skin: coarse grid too large");
-    net->cgeom = g;
-    net->cCells = (int)cc;
-    net->meanOcc = (float)n / (float)cc;
+    SBX_REQUIRE(c, cc >= 1 && cc < (1LL << 30), "net_set_skin: coarse grid too large");
+    net->maxR = maxR;
+    net->cellFloor = floorCell;
+    net->cCellsMax = (int)cc;
     // (re)allocate the skin state
-    void *old[] = {net->cCell, net->cOff, net->cCont, net->refC, net->refU, net->skin, net->cellSkin,
-                   net->cellHorizon, net->vlist, net->vcount, net->skinFlags, net->model};
+    void *old[] = {net->cCell, net->cOff,   net->cCont,     net->refC,  net->refU,      net->skin,
+                   net->cellSkin, net->cellHorizon, net->vlist, net->vcount, net->skinFlags, net->model,
+                   net->cgDev,  net->cCellsDev};
     for (void *b : old)
         if (b) hipFree(b);
-    net->cCell = net->cOff = net->cCont = net->vlist = net->vcount = net->skinFlags = nullptr;
+    net->cCell = net->cOff = net->cCont = net->vlist = net->vcount = net->skinFlags = net->cCellsDev = nullptr;
     net->refC = net->refU = net->skin = net->cellSkin = net->cellHorizon = net->model = nullptr;
+    net->cgDev = nullptr;
     do {
         if ((r = alloc(c, &net->cCell, n))) break;
         if ((r = alloc(c, &net->cOff, (size_t)cc + 1))) break;
@@ -1524,13 +1554,16 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
         if ((r = alloc(c, &net->vlist, (size_t)n * sp->listCap))) break;
         if ((r = alloc(c, &net->vcount, n))) break;
         if ((r = alloc(c, &net->skinFlags, SKF_WORDS))) break;
+        if ((r = alloc(c, &net->cgDev, 1))) break;
+        if ((r = alloc(c, &net->cCellsDev, 1))) break;
+        if ((r = fill_i32(c, net->cCellsDev, 0, 1))) break;
         if (mode == SBX_SKIN_LEARNED) {
             if ((r = alloc(c, &net->model, MLP_NPARAM))) break;
             if ((r = check_hip(c, hipMemcpyAsync(net->model, modelWeights, sizeof(float) * MLP_NPARAM,
                                                  hipMemcpyDeviceToDevice, c->stream), "copy model")))
                 break;
         }
-        long long keys = net->p.gridDims[3] > cc ? net->p.gridDims[3] : cc;
+        long long keys = net->p.gridDims[3] > cc ? (long long)net->p.gridDims[3] : cc;
         if ((r = sort_scratch_reserve(c, net->sort, keys, n))) break;
         if ((r = fill_i32(c, net->skinFlags, 0, SKF_WORDS))) break;
         if ((r = fill_i32(c, net->skinFlags + SKF_REBUILD, 1, 1))) break;  // the first step builds
@@ -1564,9 +1597,18 @@ int sbx_net_skin_state(sbx_net *net, float **refCoord, float **skin, int **coars
     if (coarseCell) *coarseCell = net->cCell;
     if (coarseOffsets) *coarseOffsets = net->cOff;
     if (coarseContents) *coarseContents = net->cCont;
-    if (nCoarse) *nCoarse = net->cCells;
+    GridGeom g{};
+    int nc = 0;
+    if (net->cgDev) {
+        int r = check_hip(net->ctx, hipMemcpyAsync(&g, net->cgDev, sizeof(g), hipMemcpyDeviceToHost, net->ctx->stream),
+                          "copy coarse grid");
+        if (!r) r = check_hip(net->ctx, hipMemcpyAsync(&nc, net->cCellsDev, sizeof(int), hipMemcpyDeviceToHost,
+                                                       net->ctx->stream), "copy coarse count");
+        if (!r) r = check_hip(net->ctx, hipStreamSynchronize(net->ctx->stream), "sync");
+        if (r) return r;
+    }
+    if (nCoarse) *nCoarse = nc;
     if (coarseGrid) {
-        const GridGeom &g = net->cgeom;
         float v[8] = {g.xMin, g.yMin, g.zMin, g.invCell > 0 ? 1.0f / g.invCell : 0.0f, g.invCell,
                       (float)g.nX, (float)g.nY, (float)g.nZ};
         memcpy(coarseGrid, v, sizeof(v));

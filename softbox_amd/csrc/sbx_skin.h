@@ -52,30 +52,31 @@ __device__ __forceinline__ float disp_sq(const float *__restrict__ a, const floa
     return dx * dx + dy * dy + dz * dz;
 }
 
-// The 12 skin-model features of one coarse cell from its members' motion over H steps:
+// The 12 skin-model features of one coarse cell (edge cellSize um) from its members' motion over H
+// steps:
 //  0 log max disp/sqrt(H)   1 log mean disp/sqrt(H)   2 log max disp/H   3 log mean disp/H
-//  4 log max |du|/sqrt(H)   5 log mean |du|/sqrt(H)   6 occupancy / mean occupancy
+//  4 log max |du|/sqrt(H)   5 log mean |du|/sqrt(H)   6 log members per um^3
 //  7 fraction of members with an active crosslink     8 mean active crosslinks per member
-//  9 log H                 10 log std disp/sqrt(H)   11 log(1 + members)
+//  9 log max |du|/H        10 log cellSize           11 log(1 + members)
 // Empty cells get zeros.  Members are read in CSR order, so the result is deterministic.
 __device__ __forceinline__ void skin_cell_features(int c, const int *__restrict__ off, const int *__restrict__ cont,
                                                    int n, const float *__restrict__ cNow,
                                                    const float *__restrict__ cPrev, const float *__restrict__ uNow,
                                                    const float *__restrict__ uPrev, const int *__restrict__ links,
-                                                   float H, float meanOcc, const GridGeom &g, float f[SKIN_FEATURES]) {
+                                                   float H, float cellSize, const GridGeom &g,
+                                                   float f[SKIN_FEATURES]) {
     int a = off[c], b = off[c + 1];
     int cnt = b - a;
 #pragma unroll
     for (int k = 0; k < SKIN_FEATURES; k++) f[k] = 0.0f;
     if (cnt <= 0) return;
-    float sumD = 0.0f, sumD2 = 0.0f, maxD = 0.0f, sumR = 0.0f, maxR = 0.0f, bound = 0.0f, nl = 0.0f;
+    float sumD = 0.0f, maxD = 0.0f, sumR = 0.0f, maxR = 0.0f, bound = 0.0f, nl = 0.0f;
     for (int p = a; p < b; p++) {
         int i = cont[p];
         float d = sqrtf(disp_sq(cNow, cPrev, i, n, g));
         float ux = uNow[i] - uPrev[i], uy = uNow[n + i] - uPrev[n + i], uz = uNow[2 * n + i] - uPrev[2 * n + i];
         float r = sqrtf(ux * ux + uy * uy + uz * uz);
         sumD += d;
-        sumD2 += d * d;
         maxD = fmaxf(maxD, d);
         sumR += r;
         maxR = fmaxf(maxR, r);
@@ -87,7 +88,6 @@ __device__ __forceinline__ void skin_cell_features(int c, const int *__restrict_
     }
     float inv = 1.0f / (float)cnt;
     float sq = sqrtf(H), meanD = sumD * inv;
-    float varD = fmaxf(0.0f, sumD2 * inv - meanD * meanD);
     const float e = 1.0e-6f;
     f[0] = logf(e + maxD / sq);
     f[1] = logf(e + meanD / sq);
@@ -95,11 +95,11 @@ __device__ __forceinline__ void skin_cell_features(int c, const int *__restrict_
     f[3] = logf(e + meanD / H);
     f[4] = logf(e + maxR / sq);
     f[5] = logf(e + sumR * inv / sq);
-    f[6] = (float)cnt / meanOcc;
+    f[6] = logf((float)cnt / (cellSize * cellSize * cellSize));
     f[7] = bound * inv;
     f[8] = nl * inv;
-    f[9] = logf(H);
-    f[10] = logf(e + sqrtf(varD) / sq);
+    f[9] = logf(e + maxR / H);
+    f[10] = logf(cellSize);
     f[11] = logf(1.0f + (float)cnt);
 }
 

@@ -29,12 +29,12 @@ GridGeom min_image_geom(int periodic, const float *boxLen) {
 __global__ void __launch_bounds__(T) k_features(const int *__restrict__ off, const int *__restrict__ cont, int nCells,
                                                 int n, const float *__restrict__ cNow, const float *__restrict__ cPrev,
                                                 const float *__restrict__ uNow, const float *__restrict__ uPrev,
-                                                const int *__restrict__ links, float H, float meanOcc, GridGeom g,
+                                                const int *__restrict__ links, float H, float cellSize, GridGeom g,
                                                 float *__restrict__ out) {
     int c = blockIdx.x * T + threadIdx.x;
     if (c >= nCells) return;
     float f[SKIN_FEATURES];
-    skin_cell_features(c, off, cont, n, cNow, cPrev, uNow, uPrev, links, H, meanOcc, g, f);
+    skin_cell_features(c, off, cont, n, cNow, cPrev, uNow, uPrev, links, H, cellSize, g, f);
 #pragma unroll
     for (int k = 0; k < SKIN_FEATURES; k++) out[(size_t)c * SKIN_FEATURES + k] = f[k];
 }
@@ -87,16 +87,16 @@ extern "C" {
 
 int sbx_skin_features(sbx_ctx *c, const int *coarseOffsets, const int *coarseContents, int nCells, int n,
                       const float *coordNow, const float *coordPrev, const float *uNow, const float *uPrev,
-                      const int *activeLinkCount, float H, int periodic, const float *boxLen, float *features) {
+                      const int *activeLinkCount, float H, float cellSize, int periodic, const float *boxLen,
+                      float *features) {
     SBX_CHECK_CTX(c);
-    SBX_REQUIRE(c, nCells >= 0 && n >= 0 && H > 0.0f, "skin_features: bad sizes or H <= 0");
+    SBX_REQUIRE(c, nCells >= 0 && n >= 0 && H > 0.0f && cellSize > 0.0f, "skin_features: bad sizes, H or cellSize");
     SBX_REQUIRE(c, !periodic || boxLen, "skin_features: periodic needs boxLen");
     if (nCells == 0) return SBX_OK;
     SBX_REQUIRE(c, coarseOffsets && coarseContents && coordNow && coordPrev && uNow && uPrev && features,
                 "skin_features: null array");
-    float meanOcc = nCells > 0 ? (float)n / (float)nCells : 1.0f;
     hipLaunchKernelGGL(k_features, dim3(blocks_for(nCells, T)), dim3(T), 0, c->stream, coarseOffsets, coarseContents,
-                       nCells, n, coordNow, coordPrev, uNow, uPrev, activeLinkCount, H, meanOcc,
+                       nCells, n, coordNow, coordPrev, uNow, uPrev, activeLinkCount, H, cellSize,
                        min_image_geom(periodic, boxLen), features);
     return check_launch(c, "k_features");
 }

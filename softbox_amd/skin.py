@@ -129,27 +129,29 @@ class TrajectorySampler:
            log of the first step a member exceeds that displacement, censored at `window`).
     """
 
+    SLACK = 1.0e-3  # um, as the library's list slack
+
     def __init__(self, ctx: Context, net, cfg: SkinConfig):
         import torch
 
         self.ctx, self.net, self.cfg = ctx, net, cfg
-        # the coarse grid of the skin mode (set up once, then back to the per-step rebuild)
-        net.set_skin(_lib.SBX_SKIN_FIXED, cfg)
-        st = net.skin_state()
-        net.set_skin(_lib.SBX_SKIN_REBUILD, cfg)
-        self.grid = st["grid"]
-        self.nCells = st["nCells"]
         sc = net.scene
-        self.n = sc.fs.n
-        self.periodic = int(sc.p["periodic"])
-        self.boxLen = (C.c_float * 3)(*(sc.p["boxLen"] if self.periodic else [0.0, 0.0, 0.0]))
-        g = self.grid
-        self.gridParams = (C.c_float * 6)(g[0], g[1], g[2], g[3], g[4], 0.0)
-        self.gridDims = (C.c_int * 4)(int(g[5]), int(g[6]), int(g[7]), int(g[5] * g[6] * g[7]))
+        self.n = n = sc.fs.n
+        p = sc.p
+        self.periodic = int(p["periodic"])
+        self.boxLen = (C.c_float * 3)(*(p["boxLen"] if self.periodic else [0.0, 0.0, 0.0]))
+        gp, gd = p["gridParams"], p["gridDims"]
+        self.fine_cell, self.cutoff, self.origin = float(gp[3]), float(gp[5]), [float(v) for v in gp[:3]]
+        self.fine_dims = [int(v) for v in gd[:3]]
+        self.box = float(p["boxLen"][0]) if self.periodic else None
+        self.max_r = float((0.5 * sc.fs.segLength + p["actinRadius"]).max())
+        # coarse cells span what the learned mode can build with: skins in [skin_min, skin_max]
+        self.cell_lo = self._cell(cfg.skin_min)
+        self.cell_hi = self._cell(cfg.skin_max)
+        nmax = int(np.prod(self._dims(self.cell_lo)))
         dev = f"cuda:{ctx.device}"
-        n, nc = self.n, self.nCells
         self.cell = torch.empty(n, dtype=torch.int32, device=dev)
-        self.off = torch.empty(nc + 1, dtype=torch.int32, device=dev)
+        self.off = torch.empty(nmax + 1, dtype=torch.int32, device=dev)
         self.cont = torch.empty(n, dtype=torch.int32, device=dev)
         self.pPrev = torch.empty(3 * n, dtype=torch.float32, device=dev)
         self.uPrev = torch.empty(3 * n, dtype=torch.float32, device=dev)
@@ -157,9 +159,27 @@ class TrajectorySampler:
         self.maxDisp = torch.empty(n, dtype=torch.float32, device=dev)
         self.thr = torch.empty(n, dtype=torch.float32, device=dev)
         self.breach = torch.empty(n, dtype=torch.int32, device=dev)
-        self.feat = torch.empty(nc * N_FEATURES, dtype=torch.float32, device=dev)
-        self.sStar = torch.empty(nc, dtype=torch.float32, device=dev)
-        self.hStar = torch.empty(nc, dtype=torch.float32, device=dev)
+        self.feat = torch.empty(nmax * N_FEATURES, dtype=torch.float32, device=dev)
+        self.sStar = torch.empty(nmax, dtype=torch.float32, device=dev)
+        self.hStar = torch.empty(nmax, dtype=torch.float32, device=dev)
+
+    def _cell(self, skin):
+        """The rebuild grid's cell for a largest skin (sbx_network.hip k_skin_geom)."""
+        return max(self.fine_cell, 2.0 * self.max_r + self.cutoff + 2.0 * skin + 2.0 * self.SLACK)
+
+    def _dims(self, cell):
+        if self.periodic:
+            nb = max(3, int(math.floor(self.box / cell)))
+            return [nb, nb, nb]
+        return [max(1, int(math.ceil(self.fine_cell * d / cell))) for d in self.fine_dims]
+
+    def _grid(self, cell):
+        dims = self._dims(cell)
+        if self.periodic:
+            cell = self.box / dims[0]
+        gp = (C.c_float * 6)(self.origin[0], self.origin[1], self.origin[2], cell, 1.0 / cell, 0.0)
+        gd = (C.c_int * 4)(dims[0], dims[1], dims[2], dims[0] * dims[1] * dims[2])
+        return gp, gd, cell, dims[0] * dims[1] * dims[2]
 
     def _step(self, t):
         self.net.step(t, 1)
@@ -173,27 +193,28 @@ class TrajectorySampler:
         import torch
 
         lib, h, fil = self.ctx.lib, self.ctx.h, self.net.fil
-        cfg, n, nc = self.cfg, self.n, self.nCells
+        cfg, n = self.cfg, self.n
         H, W = cfg.horizon, cfg.window
         links = self.net.xl.activeLinkCount if self.net.xl is not None else None
         xs, ys = [], []
         t = t0
         counts = (C.c_int * 4)(n, n, 0, 0)
-        for _ in range(snapshots):
+        for s_i in range(snapshots):
+            # a rebuild-grid cell size between the smallest and the largest the learned mode uses
+            u = (s_i + 0.5) / snapshots
+            gp, gd, cell, nc = self._grid(self.cell_lo * (self.cell_hi / self.cell_lo) ** u)
             # t0 - H: positions and the member lists the features are taken over
             self.net.publish()
             self._copy(self.pPrev, fil.coord)
             self._copy(self.uPrev, fil.uVec)
-            self.ctx.check(lib.sbx_body_cell(h, ptr(self.pPrev), self.gridParams, self.gridDims, counts,
-                                             ptr(self.cell), n, self.periodic))
-            self.ctx.check(lib.sbx_grid_build(h, ptr(self.cell), counts, self.gridDims, ptr(self.off),
-                                              ptr(self.cont), n))
+            self.ctx.check(lib.sbx_body_cell(h, ptr(self.pPrev), gp, gd, counts, ptr(self.cell), n, self.periodic))
+            self.ctx.check(lib.sbx_grid_build(h, ptr(self.cell), counts, gd, ptr(self.off), ptr(self.cont), n))
             for _ in range(H):
                 self._step(t)
                 t += 1
             self.ctx.check(lib.sbx_skin_features(h, ptr(self.off), ptr(self.cont), nc, n, ptr(fil.coord),
                                                  ptr(self.pPrev), ptr(fil.uVec), ptr(self.uPrev), ptr(links),
-                                                 float(H), self.periodic, self.boxLen, ptr(self.feat)))
+                                                 float(H), float(cell), self.periodic, self.boxLen, ptr(self.feat)))
             # labels: the largest displacement within H steps, then the first step beyond it
             self._copy(self.x0, fil.coord)
             self.ctx.check(lib.sbx_fill_i32(h, ptr(self.maxDisp), 0, n))
@@ -215,10 +236,11 @@ class TrajectorySampler:
             self.ctx.check(lib.sbx_skin_cell_reduce(h, ptr(self.off), ptr(self.cont), nc, None, ptr(self.breach), 1,
                                                     float(W + 1), ptr(self.hStar)))
             self.ctx.sync()
-            f = self.feat.view(nc, N_FEATURES)
-            occupied = (f[:, 11] > 0) & (self.sStar > 0)
+            f = self.feat[: nc * N_FEATURES].view(nc, N_FEATURES)
+            sS, hS = self.sStar[:nc], self.hStar[:nc]
+            occupied = (f[:, 11] > 0) & (sS > 0)
             xs.append(f[occupied].clone())
-            ys.append(torch.stack([torch.log(self.sStar[occupied]), torch.log(self.hStar[occupied])], 1))
+            ys.append(torch.stack([torch.log(sS[occupied]), torch.log(hS[occupied])], 1))
         torch.cuda.synchronize()
         return torch.cat(xs).contiguous(), torch.cat(ys).contiguous(), t
 
