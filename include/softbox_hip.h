@@ -347,8 +347,9 @@ int sbx_net_skin_state(sbx_net *net, float **refCoord, float **skin, int **coars
  * input-standardisation entries are not trained). */
 /* training data of the skin model from a trajectory (no counterpart in the reference): per coarse cell
  * (edge cellSize) features[c*12 + k] of its members' motion over H steps (coordPrev -> coordNow); per-body running
- * max displacement from x0 and first step beyond a per-body threshold; per-cell max (mode 0, float)
- * or min (mode 1, int >= 0; `fill` when none) over members; per-cell value copied to its members. */
+ * max displacement from x0 and first step beyond a per-body threshold; per-cell max (mode 0, float),
+ * min (mode 1, int >= 0) or root mean square (mode 2, float) over members (`fill` when none); per-cell
+ * value copied to its members. */
 int sbx_skin_features(sbx_ctx *c, const int *coarseOffsets, const int *coarseContents, int nCells, int n,
                       const float *coordNow, const float *coordPrev, const float *uNow, const float *uPrev,
                       const int *activeLinkCount, float H, float cellSize, int periodic,

@@ -95,6 +95,7 @@ struct sbx_net {
     int *vlist = nullptr, *vcount = nullptr;
     int *skinFlags = nullptr;
     float *model = nullptr;                 // MLP_NPARAM device floats (learned mode)
+    float4 *skinPack = nullptr;             // {centre, bounding radius} of the current pose (by k_rod)
     // graph
     int useGraph = 0;
     int graphSteps = 0;  // non-zero while a capture is being recorded
@@ -131,6 +132,8 @@ struct RodArgs {
     float4 *linkPack;       // packed endpoint data of the new pose for the next step's links (or null)
     const float *skinRef, *skin;  // skin guard: centre at the last rebuild and per-body skin (or null)
     int *skinFlags;
+    float4 *skinPack;             // {new centre, bounding radius} for the next step's skin kernels
+    const float *boundR;
     GridGeom geom;
     const int *dStep;
     int stepOff;
@@ -234,6 +237,7 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         }
         float sk = a.skin[i];
         if (dx * dx + dy * dy + dz * dz > sk * sk) a.skinFlags[SKF_REBUILD] = 1;
+        a.skinPack[i] = make_float4(p.c[0], p.c[1], p.c[2], a.boundR[i]);
     }
     if (a.linkPack) {
         float4 *q = a.linkPack + LINK_PACK * i;
@@ -241,6 +245,13 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         q[1] = make_float4(p.u[0], p.u[1], p.u[2], 0.0f);
         q[2] = make_float4(d.e1[0], d.e1[1], d.e1[2], 0.0f);
     }
+}
+
+// skinPack of the caller's pose (before the first step and after sbx_net_publish)
+__global__ void __launch_bounds__(256) k_pack_skin(const float *__restrict__ coord, const float *__restrict__ boundR,
+                                                   float4 *__restrict__ pack, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) pack[i] = make_float4(coord[i], coord[n + i], coord[2 * n + i], boundR[i]);
 }
 
 // linkPack of the caller's pose (before the first step and after sbx_net_publish)
@@ -589,45 +600,43 @@ struct SkinArgs {
     float skinDefault, skinMin, skinMax, margin;
     float maxR, cellFloor;   // largest bounding radius; smallest coarse cell (fine cell or skinMin reach)
     const int *bodyCell;     // fine cells of the current pose
+    const float4 *pack;      // {centre, bounding radius} of the current pose
     int *candPartner, *candCount;
 };
 
 constexpr float SKIN_SLACK = 1.0e-3f;  // um, covers float rounding of the list and guard distances
 constexpr int SKF_MAXSKIN = 6;         // flag word: largest skin of the rebuild (float bits)
 
-// Per coarse cell of the last rebuild: the model's skin for its members (learned mode, after the
-// first rebuild); otherwise the default skin.
-__global__ void __launch_bounds__(256) k_skin_model(SkinArgs a) {
-    if (a.flags[SKF_REBUILD] == 0) return;
-    int c = blockIdx.x * 256 + threadIdx.x;
-    if (c == 0) a.flags[SKF_MAXSKIN] = 0;
-    if (!(a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && a.model)) return;
-    if (c >= *a.cCells) return;
-    float f[SKIN_FEATURES], y[MLP_OUT];
-    float H = (float)(a.flags[SKF_SINCE] > 0 ? a.flags[SKF_SINCE] : 1);
-    skin_cell_features(c, a.cOff, a.cCont, a.n, a.coord, a.refC, a.uVec, a.refU, a.links, H, 1.0f / a.cg->invCell,
-                       a.geom, f);
-    mlp_forward(a.model, f, y);
-    a.cellSkin[c] = fminf(a.skinMax, fmaxf(a.skinMin, a.margin * expf(y[0])));
-    a.cellHorizon[c] = expf(y[1]);
-}
-
-// per body: its skin (from its coarse cell of the last rebuild, or the default) and the running max
+// Skins of this rebuild.  Learned mode after the first rebuild: thread c is a coarse cell of the
+// last rebuild -- features over its members, model, clamp, then the skin of every member;
+// otherwise thread i is a body and gets the default skin.  Tracks the largest skin (flag word).
 __global__ void __launch_bounds__(256) k_skin_assign(SkinArgs a) {
     if (a.flags[SKF_REBUILD] == 0) return;
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.n) return;
-    float sk = a.skinDefault;
-    if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && a.model) sk = a.cellSkin[a.cCell[i]];
-    a.skin[i] = sk;
-    atomicMax(&a.flags[SKF_MAXSKIN], __float_as_int(sk));  // positive floats order as ints
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && a.model) {
+        if (k >= *a.cCells) return;
+        float f[SKIN_FEATURES], y[MLP_OUT];
+        float H = (float)(a.flags[SKF_SINCE] > 0 ? a.flags[SKF_SINCE] : 1);
+        skin_cell_features(k, a.cOff, a.cCont, a.n, a.coord, a.refC, a.uVec, a.refU, a.links, H,
+                           1.0f / a.cg->invCell, a.geom, f);
+        mlp_forward(a.model, f, y);
+        float sk = fminf(a.skinMax, fmaxf(a.skinMin, a.margin * expf(y[0])));
+        a.cellSkin[k] = sk;
+        a.cellHorizon[k] = expf(y[1]);
+        int p0 = a.cOff[k], p1 = a.cOff[k + 1];
+        for (int p = p0; p < p1; p++) a.skin[a.cCont[p]] = sk;
+        if (p1 > p0) atomicMax(&a.flags[SKF_MAXSKIN], __float_as_int(sk));  // positive floats order as ints
+    } else {
+        if (k >= a.n) return;
+        a.skin[k] = a.skinDefault;
+        if (k == 0) atomicMax(&a.flags[SKF_MAXSKIN], __float_as_int(a.skinDefault));
+    }
 }
 
-// the coarse grid of this rebuild: cell >= 2 maxR + cutoff + 2 maxSkin (+ slack), covering the
-// fine grid's extent (clamped) or tiling the cubic box (periodic, >= 3 cells per axis)
-__global__ void k_skin_geom(SkinArgs a) {
-    if (a.flags[SKF_REBUILD] == 0) return;
-    float maxSkin = __int_as_float(a.flags[SKF_MAXSKIN]);
+// The coarse grid of this rebuild, cell >= 2 maxR + cutoff + 2 maxSkin (+ slack), covering the fine
+// grid's extent (clamped) or tiling the cubic box (periodic, >= 3 cells per axis); every thread
+// derives it (same arithmetic), thread 0 stores it and opens the rebuild's counters.
+__device__ __forceinline__ GridGeom skin_coarse_geom(const SkinArgs &a, float maxSkin) {
     float need = 2.0f * a.maxR + a.geom.cutoff + 2.0f * maxSkin + 2.0f * SKIN_SLACK;
     float cell = fmaxf(need, a.cellFloor);
     GridGeom g = a.geom;
@@ -637,28 +646,31 @@ __global__ void k_skin_geom(SkinArgs a) {
         g.nX = g.nY = g.nZ = nb;
         g.invCell = (float)nb / g.Lx;
     } else {
-        float ext = (float)a.geom.nX / a.geom.invCell, eyt = (float)a.geom.nY / a.geom.invCell,
-              ezt = (float)a.geom.nZ / a.geom.invCell;
-        g.nX = max(1, (int)ceilf(ext / cell));
-        g.nY = max(1, (int)ceilf(eyt / cell));
-        g.nZ = max(1, (int)ceilf(ezt / cell));
+        float span = 1.0f / a.geom.invCell;
+        g.nX = max(1, (int)ceilf(span * a.geom.nX / cell));
+        g.nY = max(1, (int)ceilf(span * a.geom.nY / cell));
+        g.nZ = max(1, (int)ceilf(span * a.geom.nZ / cell));
         g.invCell = 1.0f / cell;
     }
-    long long nc = (long long)g.nX * g.nY * g.nZ;
-    if (nc > a.cCellsMax) {  // cannot happen: cell >= cellFloor bounds the count (host-checked)
-        g.nX = g.nY = g.nZ = 1;
-        nc = 1;
-    }
-    *a.cg = g;
-    *a.cCells = (int)nc;
+    if ((long long)g.nX * g.nY * g.nZ > a.cCellsMax) g.nX = g.nY = g.nZ = 1;  // bounded by cellFloor (host)
+    return g;
 }
 
 __global__ void __launch_bounds__(256) k_skin_bin(SkinArgs a) {
     if (a.flags[SKF_REBUILD] == 0) return;
     int i = blockIdx.x * 256 + threadIdx.x;
+    GridGeom g = skin_coarse_geom(a, __int_as_float(a.flags[SKF_MAXSKIN]));
+    if (i == 0) {
+        *a.cg = g;
+        *a.cCells = g.nX * g.nY * g.nZ;
+        a.flags[SKF_OVERFLOW] = 0;
+        a.flags[SKF_ENTRIES] = 0;
+        a.flags[SKF_SINCE] = 0;
+        a.flags[SKF_REBUILDS] += 1;
+    }
     if (i >= a.n) return;
-    const int n = a.n;
-    a.cCell[i] = cell_of(a.coord[i], a.coord[n + i], a.coord[2 * n + i], *a.cg);
+    float4 q = a.pack[i];
+    a.cCell[i] = cell_of(q.x, q.y, q.z, g);
 }
 
 // Neighbour lists: every j > i within ri + rj + cutoff + s_i + s_j (+ slack) of i, from the 27
@@ -671,8 +683,9 @@ __global__ void __launch_bounds__(BP_THREADS) k_skin_build(SkinArgs a) {
     if (t >= n) return;
     const GridGeom g = *a.cg;
     int i = a.cCont[t];  // coarse-cell order: neighbouring lanes walk neighbouring cells
-    float icx = a.coord[i], icy = a.coord[n + i], icz = a.coord[2 * n + i];
-    float base = a.boundR[i] + a.geom.cutoff + a.skin[i] + SKIN_SLACK;
+    const float4 qi = a.pack[i];
+    const float icx = qi.x, icy = qi.y, icz = qi.z;
+    float base = qi.w + a.geom.cutoff + a.skin[i] + SKIN_SLACK;
     int2 *seg = segLds + threadIdx.x * 18;
     const int total = stencil_ranges(a.cCell[i], g, a.cOff, seg);
     RangeWalk w(seg);
@@ -686,10 +699,11 @@ __global__ void __launch_bounds__(BP_THREADS) k_skin_build(SkinArgs a) {
 #pragma unroll
         for (int u = 0; u < BP_BATCH; u++) {
             int j = jj[u] > i ? jj[u] : i;
-            px[u] = a.coord[j];
-            py[u] = a.coord[n + j];
-            pz[u] = a.coord[2 * n + j];
-            pr[u] = a.boundR[j] + a.skin[j];
+            float4 q = a.pack[j];
+            px[u] = q.x;
+            py[u] = q.y;
+            pz[u] = q.z;
+            pr[u] = q.w + a.skin[j];
         }
 #pragma unroll
         for (int u = 0; u < BP_BATCH; u++) {
@@ -717,16 +731,6 @@ __global__ void __launch_bounds__(BP_THREADS) k_skin_build(SkinArgs a) {
     atomicAdd(&a.flags[SKF_ENTRIES], cnt < a.listCap ? cnt : a.listCap);
 }
 
-// after the model and the assignment (which read SINCE and REBUILDS), before the build: the
-// counters of the new rebuild
-__global__ void k_skin_begin(int *flags) {
-    if (flags[SKF_REBUILD] == 0) return;
-    flags[SKF_OVERFLOW] = 0;
-    flags[SKF_ENTRIES] = 0;
-    flags[SKF_SINCE] = 0;
-    flags[SKF_REBUILDS] += 1;
-}
-
 __global__ void __launch_bounds__(256) k_max_radius(const float *__restrict__ r, int n, int *__restrict__ out) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n && r[i] > 0.0f) atomicMax(out, __float_as_int(r[i]));  // positive floats order as ints
@@ -735,7 +739,8 @@ __global__ void __launch_bounds__(256) k_max_radius(const float *__restrict__ r,
 // One candidate test (the broad phase's predicate, same arithmetic) and its sort key (fine stencil
 // position, index); -1 when j is not a candidate of SpatialGrid.broadPhase.
 __device__ __forceinline__ int skin_key(int j, int ci, float icx, float icy, float icz, float ri, float pxj,
-                                        float pyj, float pzj, float rj, int cj, const GridGeom &g) {
+                                        float pyj, float pzj, float rj, const int *__restrict__ bodyCell,
+                                        const GridGeom &g) {
     float dx = icx - pxj, dy = icy - pyj, dz = icz - pzj;
     if (g.periodic) {
         dx = min_image(dx, g.Lx, g.iLx);
@@ -745,71 +750,94 @@ __device__ __forceinline__ int skin_key(int j, int ci, float icx, float icy, flo
     float distSq = dx * dx + dy * dy + dz * dz;
     float reach = ri + rj + g.cutoff;
     if (!(distSq <= reach * reach)) return -1;
-    int pos = stencil_pos(ci, cj, g);
+    int pos = stencil_pos(ci, bodyCell[j], g);
     if (pos < 0) return -1;
     return (pos << 26) | j;
 }
 
 constexpr int SKIN_KEYS = 24;  // per-thread LDS key buffer of k_skin_pairs
 
+// Calls visit(j) for every j in the 27 coarse cells around coarse cell c (overflow path).
+template <typename F>
+__device__ __forceinline__ void coarse_stencil(int c, const GridGeom &g, const int *__restrict__ off,
+                                               const int *__restrict__ cont, F visit) {
+    const int nXY = g.nX * g.nY;
+    int cz = c / nXY, rem = c - cz * nXY, cy = rem / g.nX, cx = rem - cy * g.nX;
+    for (int dz = -1; dz <= 1; dz++) {
+        int zz = cz + dz;
+        if (g.periodic) zz = zz < 0 ? zz + g.nZ : (zz >= g.nZ ? zz - g.nZ : zz);
+        else if (zz < 0 || zz >= g.nZ) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            int yy = cy + dy;
+            if (g.periodic) yy = yy < 0 ? yy + g.nY : (yy >= g.nY ? yy - g.nY : yy);
+            else if (yy < 0 || yy >= g.nY) continue;
+            for (int dx = -1; dx <= 1; dx++) {
+                int xx = cx + dx;
+                if (g.periodic) xx = xx < 0 ? xx + g.nX : (xx >= g.nX ? xx - g.nX : xx);
+                else if (xx < 0 || xx >= g.nX) continue;
+                int cc = xx + yy * g.nX + zz * nXY;
+                for (int p = off[cc]; p < off[cc + 1]; p++) visit(cont[p]);
+            }
+        }
+    }
+}
+
 // Candidate slices of the current pose from the neighbour lists (or, for a body whose list
 // overflowed, from the 27 coarse cells of the last rebuild), in the broad phase's order.  Keys are
 // gathered in LDS and sorted there; a body with more than SKIN_KEYS candidates sorts in its slice.
 __global__ void __launch_bounds__(BP_THREADS) k_skin_pairs(SkinArgs a) {
     __shared__ int keyLds[BP_THREADS * SKIN_KEYS];
-    __shared__ int2 segLds[BP_THREADS * 18];
     int t = xcd_block(blockIdx.x, gridDim.x) * BP_THREADS + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // no other thread reads these words in this kernel
         a.flags[SKF_REBUILD] = 0;
         a.flags[SKF_SINCE] += 1;
         a.flags[SKF_STEPS] += 1;
+        a.flags[SKF_MAXSKIN] = 0;  // the next rebuild's running max starts here
     }
     const int n = a.n;
     if (t >= n) return;
     const GridGeom &g = a.geom;
     int i = a.cCont[t];
-    float icx = a.coord[i], icy = a.coord[n + i], icz = a.coord[2 * n + i];
-    float ri = a.boundR[i];
-    int ci = a.bodyCell[i];
+    const float4 qi = a.pack[i];
+    const float icx = qi.x, icy = qi.y, icz = qi.z, ri = qi.w;
+    const int ci = a.bodyCell[i];
     int *keys = keyLds + threadIdx.x * SKIN_KEYS;
     int *slice = a.candPartner + (size_t)i * a.maxCand;
     const int cap = a.maxCand < SKIN_KEYS ? a.maxCand : SKIN_KEYS;
     int m = 0;  // candidates found (may exceed the buffer and maxCand)
-    int cnt = a.vcount[i];
+    auto add = [&](int key) {
+        if (m < cap) keys[m] = key;
+        else if (m < a.maxCand) slice[m] = key;
+        m++;
+    };
+    const int cnt = a.vcount[i];
     const bool overflow = cnt > a.listCap;
     const int *list = a.vlist + (size_t)i * a.listCap;
-    int total = cnt;
-    int2 *seg = segLds + threadIdx.x * 18;
-    if (overflow) total = stencil_ranges(a.cCell[i], *a.cg, a.cOff, seg);
-    RangeWalk w(seg);
-    for (int q = 0; q < total; q += BP_BATCH) {
-        int jj[BP_BATCH];
+    auto test = [&](int j) -> int {
+        if (j <= i) return -1;
+        float4 q = a.pack[j];
+        return skin_key(j, ci, icx, icy, icz, ri, q.x, q.y, q.z, q.w, a.bodyCell, g);
+    };
+    if (!overflow) {
+        for (int q = 0; q < cnt; q += BP_BATCH) {
+            int jj[BP_BATCH];
 #pragma unroll
-        for (int u = 0; u < BP_BATCH; u++) {
-            int j = -1;
-            if (q + u < total) j = overflow ? a.cCont[w.next()] : list[q + u];
-            jj[u] = j;
-        }
-        float px[BP_BATCH], py[BP_BATCH], pz[BP_BATCH], pr[BP_BATCH];
-        int pc[BP_BATCH];
+            for (int u = 0; u < BP_BATCH; u++) jj[u] = (q + u < cnt) ? list[q + u] : -1;
+            float4 pj[BP_BATCH];
 #pragma unroll
-        for (int u = 0; u < BP_BATCH; u++) {
-            int j = jj[u] > i ? jj[u] : i;
-            px[u] = a.coord[j];
-            py[u] = a.coord[n + j];
-            pz[u] = a.coord[2 * n + j];
-            pr[u] = a.boundR[j];
-            pc[u] = a.bodyCell[j];
-        }
+            for (int u = 0; u < BP_BATCH; u++) pj[u] = a.pack[jj[u] > i ? jj[u] : i];
 #pragma unroll
-        for (int u = 0; u < BP_BATCH; u++) {
-            if (jj[u] <= i) continue;
-            int key = skin_key(jj[u], ci, icx, icy, icz, ri, px[u], py[u], pz[u], pr[u], pc[u], g);
-            if (key < 0) continue;
-            if (m < cap) keys[m] = key;
-            else if (m < a.maxCand) slice[m] = key;
-            m++;
+            for (int u = 0; u < BP_BATCH; u++) {
+                if (jj[u] <= i) continue;
+                int key = skin_key(jj[u], ci, icx, icy, icz, ri, pj[u].x, pj[u].y, pj[u].z, pj[u].w, a.bodyCell, g);
+                if (key >= 0) add(key);
+            }
         }
+    } else {
+        coarse_stencil(a.cCell[i], *a.cg, a.cOff, a.cCont, [&](int j) {
+            int key = test(j);
+            if (key >= 0) add(key);
+        });
     }
     if (m <= cap) {  // the common case: insertion sort in LDS, one pass of writes
         for (int x = 1; x < m; x++) {
@@ -840,14 +868,14 @@ __global__ void __launch_bounds__(BP_THREADS) k_skin_pairs(SkinArgs a) {
         int last = -1;
         for (int x = 0; x < a.maxCand; x++) {
             int best = 0x7fffffff;
-            RangeWalk w2(seg);
-            for (int q = 0; q < total; q++) {
-                int j = overflow ? a.cCont[w2.next()] : list[q];
-                if (j <= i) continue;
-                int key = skin_key(j, ci, icx, icy, icz, ri, a.coord[j], a.coord[n + j], a.coord[2 * n + j],
-                                   a.boundR[j], a.bodyCell[j], g);
+            auto pick = [&](int j) {
+                int key = test(j);
                 if (key > last && key < best) best = key;
-            }
+            };
+            if (!overflow)
+                for (int q = 0; q < cnt; q++) pick(list[q]);
+            else
+                coarse_stencil(a.cCell[i], *a.cg, a.cOff, a.cCont, pick);
             slice[x] = best & ((1 << 26) - 1);
             last = best;
         }
@@ -993,11 +1021,12 @@ SkinArgs skin_args(sbx_net *net, const float *coord, const float *uVec) {
     a.margin = net->sp.margin;
     a.maxR = net->maxR; a.cellFloor = net->cellFloor;
     a.bodyCell = net->bodyCell;
+    a.pack = net->skinPack;
     a.candPartner = net->candPartner; a.candCount = net->candCount;
     return a;
 }
 
-// skin broad phase of one step: [gated rebuild: model, assign, geometry, bin, coarse sort, build]
+// skin broad phase of one step: [gated rebuild: skins (model), geometry + bin, coarse sort, build]
 // -> pairs
 int enqueue_skin(sbx_net *net, const float *coord, const float *uVec) {
     sbx_ctx *c = net->ctx;
@@ -1007,15 +1036,13 @@ int enqueue_skin(sbx_net *net, const float *coord, const float *uVec) {
     int r;
     {
         Scope sc(net, T_SKIN_REBUILD);
-        hipLaunchKernelGGL(k_skin_model, dim3(blocks_for(net->cCellsMax, 256)), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_skin_assign, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_skin_geom, dim3(1), dim3(1), 0, s, a);
+        int span = net->cCellsMax > n ? net->cCellsMax : n;
+        hipLaunchKernelGGL(k_skin_assign, dim3(blocks_for(span, 256)), dim3(256), 0, s, a);
         hipLaunchKernelGGL(k_skin_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
-        if ((r = check_launch(c, "skin model/assign/geom/bin"))) return r;
+        if ((r = check_launch(c, "skin assign/bin"))) return r;
         if ((r = counting_sort(c, net->sort, net->cCell, n, net->cCellsMax, n, net->cOff, net->cCont,
                                net->skinFlags + SKF_REBUILD)))
             return r;
-        hipLaunchKernelGGL(k_skin_begin, dim3(1), dim3(1), 0, s, net->skinFlags);
         hipLaunchKernelGGL(k_skin_build, dim3(blocks_for(n, BP_THREADS)), dim3(BP_THREADS), 0, s, a);
         if ((r = check_launch(c, "k_skin_build"))) return r;
     }
@@ -1094,6 +1121,8 @@ int enqueue_step(sbx_net *net, int stepOff) {
     a.skinRef = guard ? net->refC : nullptr;
     a.skin = guard ? net->skin : nullptr;
     a.skinFlags = net->skinFlags;
+    a.skinPack = net->skinPack;
+    a.boundR = net->boundR;
     a.mOff = net->mOff; a.mList = net->mList; a.bondData = net->mo.bondData;
     if (net->hasMotors) {
         Scope sc(net, T_MCSR);
@@ -1144,7 +1173,7 @@ void release(sbx_net *net) {
                     net->mOff,     net->mList,     net->bbOff,    net->bbList,
                     net->cCell,    net->cOff,      net->cCont,    net->refC,    net->refU, net->skin,
                     net->cellSkin, net->cellHorizon, net->vlist,  net->vcount,  net->skinFlags, net->model,
-                    net->cgDev,    net->cCellsDev};
+                    net->cgDev,    net->cCellsDev, net->skinPack};
     for (void *b : bufs)
         if (b) hipFree(b);
     sort_scratch_free(net->sort);
@@ -1376,6 +1405,11 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
         if (net->p.gridOn && !net->hasMotors &&
             (r = launch_body_cell(c, net->coordB[net->cur], net->n, net->n, net->geom, net->bodyCell)))
             return r;
+        if (net->skinMode != SBX_SKIN_REBUILD && !net->hasMotors) {
+            hipLaunchKernelGGL(k_pack_skin, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream,
+                               net->coordB[net->cur], net->boundR, net->skinPack, net->n);
+            if ((r = check_launch(c, "k_pack_skin"))) return r;
+        }
         if (net->L > 0) {
             hipLaunchKernelGGL(k_pack_links, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream,
                                net->coordB[net->cur], net->uB[net->cur], net->fil.end1, net->fil.bTransGam,
@@ -1493,6 +1527,7 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
     if (net->graphExec) { hipGraphExecDestroy(net->graphExec); net->graphExec = nullptr; }
     if (net->graph) { hipGraphDestroy(net->graph); net->graph = nullptr; }
     net->skinMode = mode;
+    net->derivedValid = 0;  // the next step prepares the packed pose the mode reads
     if (mode == SBX_SKIN_REBUILD) return SBX_OK;
     net->sp = *sp;
     const int n = net->n;
@@ -1536,12 +1571,13 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
     // (re)allocate the skin state
     void *old[] = {net->cCell, net->cOff,   net->cCont,     net->refC,  net->refU,      net->skin,
                    net->cellSkin, net->cellHorizon, net->vlist, net->vcount, net->skinFlags, net->model,
-                   net->cgDev,  net->cCellsDev};
+                   net->cgDev,  net->cCellsDev, net->skinPack};
     for (void *b : old)
         if (b) hipFree(b);
     net->cCell = net->cOff = net->cCont = net->vlist = net->vcount = net->skinFlags = net->cCellsDev = nullptr;
     net->refC = net->refU = net->skin = net->cellSkin = net->cellHorizon = net->model = nullptr;
     net->cgDev = nullptr;
+    net->skinPack = nullptr;
     do {
         if ((r = alloc(c, &net->cCell, n))) break;
         if ((r = alloc(c, &net->cOff, (size_t)cc + 1))) break;
@@ -1555,6 +1591,7 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
         if ((r = alloc(c, &net->vcount, n))) break;
         if ((r = alloc(c, &net->skinFlags, SKF_WORDS))) break;
         if ((r = alloc(c, &net->cgDev, 1))) break;
+        if ((r = alloc(c, &net->skinPack, n))) break;
         if ((r = alloc(c, &net->cCellsDev, 1))) break;
         if ((r = fill_i32(c, net->cCellsDev, 0, 1))) break;
         if (mode == SBX_SKIN_LEARNED) {

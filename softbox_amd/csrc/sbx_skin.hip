@@ -4,9 +4,10 @@
 // A sample is one coarse cell at a snapshot time t0, with its members binned at t0 - H:
 //   features  skin_cell_features over the members' motion from t0 - H to t0 (what the learned
 //             mode sees at a rebuild: the motion since the previous rebuild);
-//   labels    out0 = log of the largest displacement any member makes within H steps after t0 (the
-//             skin that would have lasted the horizon), out1 = log of the first step after t0 at
-//             which a member exceeds that skin (censored at the tracking window).
+//   labels    out0 = log of the root mean square over members of each member's largest
+//             displacement within H steps after t0 (the cell's displacement scale over the
+//             horizon), out1 = log of the first step after t0 at which a member is 3 such scales
+//             from where it started (censored at the tracking window).
 #include "sbx_internal.h"
 #include "sbx_skin.h"
 
@@ -49,26 +50,30 @@ __global__ void __launch_bounds__(T) k_track(const float *__restrict__ coord, co
     if (thr && breach[i] < 0 && d > thr[i]) breach[i] = step;
 }
 
-// mode 0: per-cell max of a per-body float; mode 1: per-cell min of a per-body int (>= 0 only),
-// `fill` when no member qualifies; written as float
+// mode 0: per-cell max of a per-body float; mode 1: per-cell min of a per-body int (>= 0 only);
+// mode 2: per-cell root mean square of a per-body float; `fill` when no member qualifies
 __global__ void __launch_bounds__(T) k_cell_reduce(const int *__restrict__ off, const int *__restrict__ cont,
                                                    int nCells, const float *__restrict__ fv,
                                                    const int *__restrict__ iv, int mode, float fill,
                                                    float *__restrict__ out) {
     int c = blockIdx.x * T + threadIdx.x;
     if (c >= nCells) return;
-    float acc = mode == 0 ? 0.0f : fill;
-    bool any = false;
+    float acc = mode == 1 ? fill : 0.0f;
+    int any = 0;
     for (int p = off[c]; p < off[c + 1]; p++) {
         int i = cont[p];
         if (mode == 0) {
             acc = fmaxf(acc, fv[i]);
-            any = true;
+            any++;
+        } else if (mode == 2) {
+            acc += fv[i] * fv[i];
+            any++;
         } else if (iv[i] >= 0) {
             acc = any ? fminf(acc, (float)iv[i]) : (float)iv[i];
-            any = true;
+            any++;
         }
     }
+    if (mode == 2 && any) acc = sqrtf(acc / (float)any);
     out[c] = any ? acc : fill;
 }
 
@@ -116,10 +121,10 @@ int sbx_skin_track(sbx_ctx *c, const float *coord, const float *x0, int n, int p
 int sbx_skin_cell_reduce(sbx_ctx *c, const int *coarseOffsets, const int *coarseContents, int nCells,
                          const float *bodyFloat, const int *bodyInt, int mode, float fill, float *cellOut) {
     SBX_CHECK_CTX(c);
-    SBX_REQUIRE(c, mode == 0 || mode == 1, "skin_cell_reduce: mode 0 (max float) or 1 (min int)");
+    SBX_REQUIRE(c, mode >= 0 && mode <= 2, "skin_cell_reduce: mode 0 (max float), 1 (min int) or 2 (rms float)");
     SBX_REQUIRE(c, nCells >= 0, "skin_cell_reduce: nCells < 0");
     if (nCells == 0) return SBX_OK;
-    SBX_REQUIRE(c, coarseOffsets && coarseContents && cellOut && (mode == 0 ? bodyFloat != nullptr : bodyInt != nullptr),
+    SBX_REQUIRE(c, coarseOffsets && coarseContents && cellOut && (mode != 1 ? bodyFloat != nullptr : bodyInt != nullptr),
                 "skin_cell_reduce: null array");
     hipLaunchKernelGGL(k_cell_reduce, dim3(blocks_for(nCells, T)), dim3(T), 0, c->stream, coarseOffsets,
                        coarseContents, nCells, bodyFloat, bodyInt, mode, fill, cellOut);

@@ -125,8 +125,9 @@ class TrajectorySampler:
 
     One sample = one occupied coarse cell at a snapshot t0, members binned at t0 - H:
       x = skin_cell_features of the members' motion t0 - H -> t0;
-      y = (log of the largest displacement of a member within H steps after t0,
-           log of the first step a member exceeds that displacement, censored at `window`).
+      y = (log rms over members of each one's largest displacement within H steps after t0 (the
+           cell's displacement scale), log of the first step a member is 3 scales out (censored at
+           `window`)).
     """
 
     SLACK = 1.0e-3  # um, as the library's list slack
@@ -162,6 +163,7 @@ class TrajectorySampler:
         self.feat = torch.empty(nmax * N_FEATURES, dtype=torch.float32, device=dev)
         self.sStar = torch.empty(nmax, dtype=torch.float32, device=dev)
         self.hStar = torch.empty(nmax, dtype=torch.float32, device=dev)
+        self.mStar = torch.empty(nmax, dtype=torch.float32, device=dev)
 
     def _cell(self, skin):
         """The rebuild grid's cell for a largest skin (sbx_network.hip k_skin_geom)."""
@@ -189,14 +191,15 @@ class TrajectorySampler:
         self.ctx.check(self.ctx.lib.sbx_copy(self.ctx.h, ptr(dst), ptr(src), dst.numel() * 4))
 
     def collect(self, t0: int, snapshots: int):
-        """Run the network from step t0; returns (x [S, 12], y [S, 2]) device tensors and the next step."""
+        """Run the network from step t0; returns device tensors x [S, 12], y [S, 2], the cells' largest
+        member displacement m [S] (for the margin), and the next step."""
         import torch
 
         lib, h, fil = self.ctx.lib, self.ctx.h, self.net.fil
         cfg, n = self.cfg, self.n
         H, W = cfg.horizon, cfg.window
         links = self.net.xl.activeLinkCount if self.net.xl is not None else None
-        xs, ys = [], []
+        xs, ys, ms = [], [], []
         t = t0
         counts = (C.c_int * 4)(n, n, 0, 0)
         for s_i in range(snapshots):
@@ -215,10 +218,10 @@ class TrajectorySampler:
             self.ctx.check(lib.sbx_skin_features(h, ptr(self.off), ptr(self.cont), nc, n, ptr(fil.coord),
                                                  ptr(self.pPrev), ptr(fil.uVec), ptr(self.uPrev), ptr(links),
                                                  float(H), float(cell), self.periodic, self.boxLen, ptr(self.feat)))
-            # labels: the largest displacement within H steps, then the first step beyond it
+            # labels: each member's largest displacement within H steps -> the cell's rms of it (the
+            # displacement scale) and max (for the margin); then the first step a member is 3 scales out
             self._copy(self.x0, fil.coord)
             self.ctx.check(lib.sbx_fill_i32(h, ptr(self.maxDisp), 0, n))
-            self.ctx.check(lib.sbx_fill_i32(h, ptr(self.breach), -1, n))
             for k in range(1, W + 1):
                 self._step(t)
                 t += 1
@@ -227,9 +230,15 @@ class TrajectorySampler:
                                                       self.boxLen, ptr(self.maxDisp), None, None, k))
                     if k == H:
                         self.ctx.check(lib.sbx_skin_cell_reduce(h, ptr(self.off), ptr(self.cont), nc,
-                                                                ptr(self.maxDisp), None, 0, 0.0, ptr(self.sStar)))
+                                                                ptr(self.maxDisp), None, 2, 0.0, ptr(self.sStar)))
+                        self.ctx.check(lib.sbx_skin_cell_reduce(h, ptr(self.off), ptr(self.cont), nc,
+                                                                ptr(self.maxDisp), None, 0, 0.0, ptr(self.mStar)))
                         self.ctx.check(lib.sbx_skin_cell_scatter(h, ptr(self.off), ptr(self.cont), nc,
                                                                  ptr(self.sStar), ptr(self.thr)))
+                        self.ctx.sync()
+                        self.thr *= 3.0
+                        self.breach.copy_(torch.where(self.maxDisp > self.thr, H, -1).to(torch.int32))
+                        torch.cuda.synchronize()
                 else:
                     self.ctx.check(lib.sbx_skin_track(h, ptr(fil.coord), ptr(self.x0), n, self.periodic,
                                                       self.boxLen, None, ptr(self.thr), ptr(self.breach), k))
@@ -237,22 +246,24 @@ class TrajectorySampler:
                                                     float(W + 1), ptr(self.hStar)))
             self.ctx.sync()
             f = self.feat[: nc * N_FEATURES].view(nc, N_FEATURES)
-            sS, hS = self.sStar[:nc], self.hStar[:nc]
+            sS, hS, mS = self.sStar[:nc], self.hStar[:nc], self.mStar[:nc]
             occupied = (f[:, 11] > 0) & (sS > 0)
             xs.append(f[occupied].clone())
             ys.append(torch.stack([torch.log(sS[occupied]), torch.log(hS[occupied])], 1))
+            ms.append(mS[occupied].clone())
         torch.cuda.synchronize()
-        return torch.cat(xs).contiguous(), torch.cat(ys).contiguous(), t
+        self.cells_per_snapshot = sum(int(m.shape[0]) for m in ms) / max(1, len(ms))
+        return torch.cat(xs).contiguous(), torch.cat(ys).contiguous(), torch.cat(ms).contiguous(), t
 
 
-def calibrate_margin(ctx: Context, model: SkinModel, x, y, cells_per_snapshot: float) -> float:
-    """Skin margin from the training residuals: the guard rebuilds as soon as ANY cell's members
-    outrun their skin, so the skin must cover the cell's displacement at the (1 - 1/cells) quantile
-    of the residual log(label) - out0, not at its mean (conformal calibration on the samples)."""
+def calibrate_margin(ctx: Context, model: SkinModel, x, m, cells_per_snapshot: float) -> float:
+    """Skin margin from the training samples: the guard rebuilds as soon as ANY body outruns its
+    skin, so margin * exp(out0) must cover each cell's LARGEST member displacement m, over all cells
+    of a rebuild: the (1 - 1/cells) quantile of m / exp(out0) (conformal calibration)."""
     pred = ctx.to_host(model.forward(x))[:, 0]
-    r = ctx.to_host(y)[:, 0] - pred
+    ratio = ctx.to_host(m) / np.exp(pred)
     q = 1.0 - 1.0 / max(2.0, cells_per_snapshot)
-    return float(max(1.0, math.exp(np.quantile(r, q))))
+    return float(max(1.0, np.quantile(ratio, q)))
 
 
 def fit(ctx: Context, net, cfg: SkinConfig, t0: int = 0, snapshots: int = 16, iters: int = 400, seed: int = 0,
@@ -262,12 +273,12 @@ def fit(ctx: Context, net, cfg: SkinConfig, t0: int = 0, snapshots: int = 16, it
     import torch
 
     sampler = TrajectorySampler(ctx, net, cfg)
-    x, y, t = sampler.collect(t0, snapshots)
+    x, y, m, t = sampler.collect(t0, snapshots)
     xh = ctx.to_host(x)
     mean = xh.mean(0)
     std = np.maximum(xh.std(0), 1e-3)
     model = SkinModel.initial(ctx, mean.astype(np.float32), std.astype(np.float32), seed)
     torch.cuda.synchronize()
     hist = train(ctx, model, x, y, iters=iters, seed=seed, log_every=log_every)
-    cfg.margin = calibrate_margin(ctx, model, x, y, x.shape[0] / max(1, snapshots))
+    cfg.margin = calibrate_margin(ctx, model, x, m, sampler.cells_per_snapshot)
     return model, hist, (x, y), t

@@ -167,7 +167,7 @@ def test_learned_skin_beats_fixed_default_on_heterogeneous_mobility(ctx):
     from softbox_amd.network import Network
 
     sc = _hetero_scene(31)
-    cfg = SkinConfig(horizon=8, window=24)
+    cfg = SkinConfig(horizon=4, window=16)
     net = Network(ctx, sc.copy())
     model, hist, _, t = fit(ctx, net, cfg, snapshots=8, iters=300)
     net.close()
