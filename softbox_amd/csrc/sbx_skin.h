@@ -57,7 +57,7 @@ __device__ __forceinline__ float disp_sq(const float *__restrict__ a, const floa
 //  0 log max disp/sqrt(H)   1 log mean disp/sqrt(H)   2 log max disp/H   3 log mean disp/H
 //  4 log max |du|/sqrt(H)   5 log mean |du|/sqrt(H)   6 log members per um^3
 //  7 fraction of members with an active crosslink     8 mean active crosslinks per member
-//  9 log max |du|/H        10 log cellSize           11 log(1 + members)
+//  9 log H                 10 log cellSize           11 log(1 + members)
 // Empty cells get zeros.  Members are read in CSR order, so the result is deterministic.
 __device__ __forceinline__ void skin_cell_features(int c, const int *__restrict__ off, const int *__restrict__ cont,
                                                    int n, const float *__restrict__ cNow,
@@ -98,7 +98,7 @@ __device__ __forceinline__ void skin_cell_features(int c, const int *__restrict_
     f[6] = logf((float)cnt / (cellSize * cellSize * cellSize));
     f[7] = bound * inv;
     f[8] = nl * inv;
-    f[9] = logf(e + maxR / H);
+    f[9] = logf(H);
     f[10] = logf(cellSize);
     f[11] = logf(1.0f + (float)cnt);
 }

@@ -123,8 +123,9 @@ def train(ctx: Context, model: SkinModel, x, t, iters: int = 400, batch: int = 4
 class TrajectorySampler:
     """Features and labels of the skin model from the per-step-rebuild run of a network.
 
-    One sample = one occupied coarse cell at a snapshot t0, members binned at t0 - H:
-      x = skin_cell_features of the members' motion t0 - H -> t0;
+    One sample = one occupied coarse cell at a snapshot t0, members binned at t0 - h (h cycles over
+    1..H):
+      x = skin_cell_features of the members' motion t0 - h -> t0;
       y = (log rms over members of each one's largest displacement within H steps after t0 (the
            cell's displacement scale), log of the first step a member is 3 scales out (censored at
            `window`)).
@@ -203,21 +204,24 @@ class TrajectorySampler:
         t = t0
         counts = (C.c_int * 4)(n, n, 0, 0)
         for s_i in range(snapshots):
-            # a rebuild-grid cell size between the smallest and the largest the learned mode uses
+            # a rebuild-grid cell size between the smallest and the largest the learned mode uses, and a
+            # feature look-back between 1 step and the horizon (at run time the features cover the
+            # steps since the last rebuild, which the guard may cut short)
             u = (s_i + 0.5) / snapshots
             gp, gd, cell, nc = self._grid(self.cell_lo * (self.cell_hi / self.cell_lo) ** u)
+            Hf = 1 + (s_i * 7) % H
             # t0 - H: positions and the member lists the features are taken over
             self.net.publish()
             self._copy(self.pPrev, fil.coord)
             self._copy(self.uPrev, fil.uVec)
             self.ctx.check(lib.sbx_body_cell(h, ptr(self.pPrev), gp, gd, counts, ptr(self.cell), n, self.periodic))
             self.ctx.check(lib.sbx_grid_build(h, ptr(self.cell), counts, gd, ptr(self.off), ptr(self.cont), n))
-            for _ in range(H):
+            for _ in range(Hf):
                 self._step(t)
                 t += 1
             self.ctx.check(lib.sbx_skin_features(h, ptr(self.off), ptr(self.cont), nc, n, ptr(fil.coord),
                                                  ptr(self.pPrev), ptr(fil.uVec), ptr(self.uPrev), ptr(links),
-                                                 float(H), float(cell), self.periodic, self.boxLen, ptr(self.feat)))
+                                                 float(Hf), float(cell), self.periodic, self.boxLen, ptr(self.feat)))
             # labels: each member's largest displacement within H steps -> the cell's rms of it (the
             # displacement scale) and max (for the margin); then the first step a member is 3 scales out
             self._copy(self.x0, fil.coord)
