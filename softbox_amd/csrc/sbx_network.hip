@@ -738,7 +738,7 @@ __global__ void __launch_bounds__(256) k_max_radius(const float *__restrict__ r,
 
 // One candidate test (the broad phase's predicate, same arithmetic) and its sort key (fine stencil
 // position, index); -1 when j is not a candidate of SpatialGrid.broadPhase.
-__device__ __forceinline__ int skin_key(int j, int ci, float icx, float icy, float icz, float ri, float pxj,
+__device__ __forceinline__ int skin_key(int j, int3 ci, float icx, float icy, float icz, float ri, float pxj,
                                         float pyj, float pzj, float rj, const int *__restrict__ bodyCell,
                                         const GridGeom &g) {
     float dx = icx - pxj, dy = icy - pyj, dz = icz - pzj;
@@ -800,7 +800,7 @@ __global__ void __launch_bounds__(BP_THREADS) k_skin_pairs(SkinArgs a) {
     int i = a.cCont[t];
     const float4 qi = a.pack[i];
     const float icx = qi.x, icy = qi.y, icz = qi.z, ri = qi.w;
-    const int ci = a.bodyCell[i];
+    const int3 ci = cell_xyz(a.bodyCell[i], g);
     int *keys = keyLds + threadIdx.x * SKIN_KEYS;
     int *slice = a.candPartner + (size_t)i * a.maxCand;
     const int cap = a.maxCand < SKIN_KEYS ? a.maxCand : SKIN_KEYS;

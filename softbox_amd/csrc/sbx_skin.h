@@ -24,13 +24,28 @@ constexpr int SKF_ENTRIES = 4;     // list entries at the last rebuild (truncate
 constexpr int SKF_STEPS = 5;       // steps in skin mode so far
 constexpr int SKF_WORDS = 8;
 
-// stencil position 0..26 of fine cell cj relative to ci in (z, y, x) order, or -1 when cj is not
-// one of the (up to) 27 cells SpatialGrid.broadPhase visits for ci
-__device__ __forceinline__ int stencil_pos(int ci, int cj, const GridGeom &g) {
+// q = c / d (c >= 0, d > 0, q < 2^20) by a float reciprocal and exact integer correction
+__device__ __forceinline__ int div_small(int c, int d, float invD) {
+    int q = (int)((float)c * invD);
+    while (q > 0 && q * d > c) q--;
+    while ((q + 1) * d <= c) q++;
+    return q;
+}
+
+// (x, y, z) of linear cell c
+__device__ __forceinline__ int3 cell_xyz(int c, const GridGeom &g) {
     const int nXY = g.nX * g.nY;
-    int iz = ci / nXY, ir = ci - iz * nXY, iy = ir / g.nX, ix = ir - iy * g.nX;
-    int jz = cj / nXY, jr = cj - jz * nXY, jy = jr / g.nX, jx = jr - jy * g.nX;
-    int dx = jx - ix, dy = jy - iy, dz = jz - iz;
+    int z = div_small(c, nXY, 1.0f / (float)nXY);
+    int r = c - z * nXY;
+    int y = div_small(r, g.nX, 1.0f / (float)g.nX);
+    return make_int3(r - y * g.nX, y, z);
+}
+
+// stencil position 0..26 of fine cell cj relative to cell (ix, iy, iz) in (z, y, x) order, or -1
+// when cj is not one of the (up to) 27 cells SpatialGrid.broadPhase visits for it
+__device__ __forceinline__ int stencil_pos(int3 ci, int cj, const GridGeom &g) {
+    int3 j = cell_xyz(cj, g);
+    int dx = j.x - ci.x, dy = j.y - ci.y, dz = j.z - ci.z;
     if (g.periodic) {  // >= 3 bins per axis, so each offset maps to one of -1, 0, +1 at most once
         if (dx == g.nX - 1) dx = -1; else if (dx == 1 - g.nX) dx = 1;
         if (dy == g.nY - 1) dy = -1; else if (dy == 1 - g.nY) dy = 1;

@@ -29,7 +29,7 @@ class SkinConfig:
     skin_min: float = 0.02
     skin_max: float = 0.3
     margin: float = 1.0           # learned skin = margin * exp(out0) (set by calibrate_margin)
-    list_cap: int = 64
+    list_cap: int = 256           # neighbour slots per body (bundles are ~10x denser than average)
     horizon: int = 8              # steps a skin is trained to last
     window: int = 32              # steps tracked after a snapshot (censoring of the horizon label)
 
