@@ -755,7 +755,8 @@ __device__ __forceinline__ int skin_key(int j, int3 ci, float icx, float icy, fl
     return (pos << 26) | j;
 }
 
-constexpr int SKIN_KEYS = 24;  // per-thread LDS key buffer of k_skin_pairs
+constexpr int SKIN_KEYS = 64;     // per-thread LDS key buffer of k_skin_pairs (dense bundles reach ~50)
+constexpr int SKIN_THREADS = 128;  // k_skin_pairs block: 32 KB of keys
 
 // Calls visit(j) for every j in the 27 coarse cells around coarse cell c (overflow path).
 template <typename F>
@@ -785,9 +786,9 @@ __device__ __forceinline__ void coarse_stencil(int c, const GridGeom &g, const i
 // Candidate slices of the current pose from the neighbour lists (or, for a body whose list
 // overflowed, from the 27 coarse cells of the last rebuild), in the broad phase's order.  Keys are
 // gathered in LDS and sorted there; a body with more than SKIN_KEYS candidates sorts in its slice.
-__global__ void __launch_bounds__(BP_THREADS) k_skin_pairs(SkinArgs a) {
-    __shared__ int keyLds[BP_THREADS * SKIN_KEYS];
-    int t = xcd_block(blockIdx.x, gridDim.x) * BP_THREADS + threadIdx.x;
+__global__ void __launch_bounds__(SKIN_THREADS) k_skin_pairs(SkinArgs a) {
+    __shared__ int keyLds[SKIN_THREADS * SKIN_KEYS];
+    int t = xcd_block(blockIdx.x, gridDim.x) * SKIN_THREADS + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // no other thread reads these words in this kernel
         a.flags[SKF_REBUILD] = 0;
         a.flags[SKF_SINCE] += 1;
@@ -801,12 +802,12 @@ __global__ void __launch_bounds__(BP_THREADS) k_skin_pairs(SkinArgs a) {
     const float4 qi = a.pack[i];
     const float icx = qi.x, icy = qi.y, icz = qi.z, ri = qi.w;
     const int3 ci = cell_xyz(a.bodyCell[i], g);
-    int *keys = keyLds + threadIdx.x * SKIN_KEYS;
+    int *keys = keyLds + threadIdx.x;  // key x at keys[x * SKIN_THREADS]: lanes hit distinct banks
     int *slice = a.candPartner + (size_t)i * a.maxCand;
     const int cap = a.maxCand < SKIN_KEYS ? a.maxCand : SKIN_KEYS;
     int m = 0;  // candidates found (may exceed the buffer and maxCand)
     auto add = [&](int key) {
-        if (m < cap) keys[m] = key;
+        if (m < cap) keys[m * SKIN_THREADS] = key;
         else if (m < a.maxCand) slice[m] = key;
         m++;
     };
@@ -841,17 +842,17 @@ __global__ void __launch_bounds__(BP_THREADS) k_skin_pairs(SkinArgs a) {
     }
     if (m <= cap) {  // the common case: insertion sort in LDS, one pass of writes
         for (int x = 1; x < m; x++) {
-            int v = keys[x];
+            int v = keys[x * SKIN_THREADS];
             int y = x - 1;
-            while (y >= 0 && keys[y] > v) {
-                keys[y + 1] = keys[y];
+            while (y >= 0 && keys[y * SKIN_THREADS] > v) {
+                keys[(y + 1) * SKIN_THREADS] = keys[y * SKIN_THREADS];
                 y--;
             }
-            keys[y + 1] = v;
+            keys[(y + 1) * SKIN_THREADS] = v;
         }
-        for (int x = 0; x < m; x++) slice[x] = keys[x] & ((1 << 26) - 1);
+        for (int x = 0; x < m; x++) slice[x] = keys[x * SKIN_THREADS] & ((1 << 26) - 1);
     } else if (m <= a.maxCand) {  // more than the LDS buffer: sort in the slice
-        for (int x = 0; x < cap; x++) slice[x] = keys[x];
+        for (int x = 0; x < cap; x++) slice[x] = keys[x * SKIN_THREADS];
         for (int x = 1; x < m; x++) {
             int v = slice[x];
             int y = x - 1;
@@ -1048,7 +1049,7 @@ int enqueue_skin(sbx_net *net, const float *coord, const float *uVec) {
     }
     {
         Scope sc(net, T_SKIN_PAIRS);
-        hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(n, BP_THREADS)), dim3(BP_THREADS), 0, s, a);
+        hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(n, SKIN_THREADS)), dim3(SKIN_THREADS), 0, s, a);
     }
     return check_launch(c, "k_skin_pairs");
 }
