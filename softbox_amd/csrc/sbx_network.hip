@@ -612,14 +612,18 @@ constexpr int SKF_MAXSKIN = 6;         // flag word: largest skin of the rebuild
 // otherwise thread i is a body and gets the default skin.  Tracks the largest skin (flag word).
 __global__ void __launch_bounds__(256) k_skin_assign(SkinArgs a) {
     if (a.flags[SKF_REBUILD] == 0) return;
+    __shared__ float ws[MLP_NPARAM];
     int k = blockIdx.x * 256 + threadIdx.x;
     if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && a.model) {
+        if ((int)(blockIdx.x * 256) >= *a.cCells) return;  // block-uniform: no barrier below is skipped
+        for (int w = threadIdx.x; w < MLP_NPARAM; w += 256) ws[w] = a.model[w];
+        __syncthreads();
         if (k >= *a.cCells) return;
         float f[SKIN_FEATURES], y[MLP_OUT];
         float H = (float)(a.flags[SKF_SINCE] > 0 ? a.flags[SKF_SINCE] : 1);
         skin_cell_features(k, a.cOff, a.cCont, a.n, a.coord, a.refC, a.uVec, a.refU, a.links, H,
                            1.0f / a.cg->invCell, a.geom, f);
-        mlp_forward(a.model, f, y);
+        mlp_forward(ws, f, y);
         float sk = fminf(a.skinMax, fmaxf(a.skinMin, a.margin * expf(y[0])));
         a.cellSkin[k] = sk;
         a.cellHorizon[k] = expf(y[1]);
@@ -840,15 +844,20 @@ __global__ void __launch_bounds__(SKIN_THREADS) k_skin_pairs(SkinArgs a) {
             if (key >= 0) add(key);
         });
     }
-    if (m <= cap) {  // the common case: insertion sort in LDS, one pass of writes
-        for (int x = 1; x < m; x++) {
-            int v = keys[x * SKIN_THREADS];
-            int y = x - 1;
-            while (y >= 0 && keys[y * SKIN_THREADS] > v) {
-                keys[(y + 1) * SKIN_THREADS] = keys[y * SKIN_THREADS];
-                y--;
+    if (m <= cap) {  // the common case: shell sort in LDS (gaps 23, 10, 4, 1), one pass of writes
+        const int gaps[4] = {23, 10, 4, 1};
+#pragma unroll
+        for (int gi = 0; gi < 4; gi++) {
+            const int gap = gaps[gi];
+            for (int x = gap; x < m; x++) {
+                int v = keys[x * SKIN_THREADS];
+                int y = x - gap;
+                while (y >= 0 && keys[y * SKIN_THREADS] > v) {
+                    keys[(y + gap) * SKIN_THREADS] = keys[y * SKIN_THREADS];
+                    y -= gap;
+                }
+                keys[(y + gap) * SKIN_THREADS] = v;
             }
-            keys[(y + 1) * SKIN_THREADS] = v;
         }
         for (int x = 0; x < m; x++) slice[x] = keys[x * SKIN_THREADS] & ((1 << 26) - 1);
     } else if (m <= a.maxCand) {  // more than the LDS buffer: sort in the slice
