@@ -329,11 +329,15 @@ typedef struct sbx_skin_params {
     float skinMin, skinMax;
     float margin;        /* learned skin = margin * exp(model out0) */
     int listCap;         /* neighbour slots per body (overflowing bodies fall back to an exact coarse walk) */
+    int blockKeys;       /* keys a block buffers for one dense body, 0 = the maximum (4096); a body with
+                          * more is reduced to its first maxCand in chunks (smaller values exercise that) */
 } sbx_skin_params;
-/* modelWeights: device array of sbx_mlp_nparam() floats (learned mode; copied) */
+/* modelWeights: device array of sbx_mlp_nparam() floats (learned mode; copied); the skin modes need
+ * the net's maxCand <= SBX_MAX_CAND */
 int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *p, const float *modelWeights);
 /* device counters -> out[0..cap): rebuild pending, steps since rebuild, rebuilds, overflowing bodies,
- * list entries at the last build, skin-mode steps; returns the number written */
+ * list entries at the last build, skin-mode steps, body-steps done block-wide so far (more than 64
+ * candidates, or an overflowed list); returns the number written (<= 7) */
 int sbx_net_skin_stats(sbx_net *net, long long *out, int cap);
 /* device state of the last rebuild (for training): reference centres, skins, coarse cells and CSR;
  * coarseGrid[8] = xMin, yMin, zMin, cell, 1/cell, nX, nY, nZ */
@@ -386,6 +390,11 @@ int sbx_net_set_profiling(sbx_net *net, int on);
 int sbx_net_kernel_times(sbx_net *net, const char **names, double *total_ms, long long *launches, int cap);
 /* 1: capture one step as a hipGraph and replay it (re-captured when nSteps patterns change) */
 int sbx_net_use_graph(sbx_net *net, int on);
+/* 1 (default): the filament-only step runs its broad phase on a second stream of the library,
+ * concurrently with the crosslink and rod kernels (the candidate slices feed nothing later in the
+ * step); 0: one stream.  Results are identical; sbx_net_step's work is complete on the context's
+ * stream either way.  Off while per-kernel timers are on. */
+int sbx_net_set_overlap(sbx_net *net, int on);
 
 #ifdef __cplusplus
 }

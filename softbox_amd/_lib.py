@@ -114,7 +114,7 @@ SBX_SKIN_LEARNED = 2
 
 class SbxSkinParams(C.Structure):
     _fields_ = [("skinDefault", C.c_float), ("skinMin", C.c_float), ("skinMax", C.c_float), ("margin", C.c_float),
-                ("listCap", i32)]
+                ("listCap", i32), ("blockKeys", i32)]
 
 
 # name -> argtypes (restype int unless listed in _RESTYPES)
@@ -174,6 +174,7 @@ _SIGS = {
     "sbx_net_set_profiling": [vp, i32],
     "sbx_net_kernel_times": [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_longlong), i32],
     "sbx_net_use_graph": [vp, i32],
+    "sbx_net_set_overlap": [vp, i32],
     "sbx_net_set_skin": [vp, i32, C.POINTER(SbxSkinParams), vp],
     "sbx_net_skin_stats": [vp, C.POINTER(C.c_longlong), i32],
     "sbx_net_skin_state": [vp] + [C.POINTER(vp)] * 5 + [C.POINTER(i32), f32p],

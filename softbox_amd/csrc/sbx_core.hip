@@ -276,22 +276,23 @@ __global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, 
 }
 
 int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
-                  int *list, const int *gate) {
+                  int *list, const int *gate, hipStream_t st) {
     int r = sort_scratch_reserve(c, s, nKeys, nItems);
     if (r) return r;
+    hipStream_t q = st ? st : c->stream;
     int nt = (nKeys + SCAN_TILE - 1) / SCAN_TILE;
     if (nt < 1) nt = 1;
     if (nItems > 0) {
-        hipLaunchKernelGGL(k_key_count, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems, nKeys,
+        hipLaunchKernelGGL(k_key_count, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
                            s.counts, s.slot, gate);
     }
-    hipLaunchKernelGGL(k_tile_reduce, dim3(nt), dim3(SCAN_THREADS), 0, c->stream, s.counts, nKeys, s.tileSums, gate);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, c->stream, s.counts, nKeys, s.tileSums, nt,
+    hipLaunchKernelGGL(k_tile_reduce, dim3(nt), dim3(SCAN_THREADS), 0, q, s.counts, nKeys, s.tileSums, gate);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, q, s.counts, nKeys, s.tileSums, nt,
                        offsets, contentsCap, gate);
     if (nItems > 0) {
-        hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems, nKeys,
+        hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
                            offsets, s.slot, s.counts, s.tmp, contentsCap, gate);
-        hipLaunchKernelGGL(k_key_place, dim3(blocks_for(nItems, 256)), dim3(256), 0, c->stream, key, nItems, nKeys,
+        hipLaunchKernelGGL(k_key_place, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
                            offsets, s.tmp, list, contentsCap, gate);
     }
     r = check_launch(c, "counting_sort");

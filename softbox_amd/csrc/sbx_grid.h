@@ -34,17 +34,17 @@ __device__ __forceinline__ int wrap_cell(float v, int nBins) {
     return c >= nBins ? 0 : c;
 }
 
-__device__ __forceinline__ int cell_of(float x, float y, float z, const GridGeom &g) {
+__device__ __forceinline__ int3 cell3_of(float x, float y, float z, const GridGeom &g) {
     float fx = (x - g.xMin) * g.invCell;
     float fy = (y - g.yMin) * g.invCell;
     float fz = (z - g.zMin) * g.invCell;
-    int ix, iy, iz;
-    if (g.periodic) {
-        ix = wrap_cell(fx, g.nX); iy = wrap_cell(fy, g.nY); iz = wrap_cell(fz, g.nZ);
-    } else {
-        ix = clamp_cell(fx, g.nX); iy = clamp_cell(fy, g.nY); iz = clamp_cell(fz, g.nZ);
-    }
-    return ix + iy * g.nX + iz * g.nX * g.nY;
+    if (g.periodic) return make_int3(wrap_cell(fx, g.nX), wrap_cell(fy, g.nY), wrap_cell(fz, g.nZ));
+    return make_int3(clamp_cell(fx, g.nX), clamp_cell(fy, g.nY), clamp_cell(fz, g.nZ));
+}
+
+__device__ __forceinline__ int cell_of(float x, float y, float z, const GridGeom &g) {
+    int3 c = cell3_of(x, y, z, g);
+    return c.x + c.y * g.nX + c.z * g.nX * g.nY;
 }
 
 // minimum image of a coordinate difference: d - L * rint(d * (1/L)), rint = round half to even
@@ -177,6 +177,6 @@ int check_grid(sbx_ctx *c, const float *gridParams, const int *gridDims, int per
 int launch_body_cell(sbx_ctx *c, const float *center, int cap, int S, const GridGeom &g, int *bodyCell);
 int launch_broad_phase(sbx_ctx *c, const float *center, const float *boundingRadius, const int *bodyCell,
                        const int *cellOffsets, const int *cellContents, const GridGeom &g, int cap, int S,
-                       int *candPartner, int *candCount, int maxCand);
+                       int *candPartner, int *candCount, int maxCand, hipStream_t st = nullptr);
 
 }  // namespace sbx

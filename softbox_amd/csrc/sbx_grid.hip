@@ -95,9 +95,10 @@ int launch_body_cell(sbx_ctx *c, const float *center, int cap, int S, const Grid
 
 int launch_broad_phase(sbx_ctx *c, const float *center, const float *boundingRadius, const int *bodyCell,
                        const int *cellOffsets, const int *cellContents, const GridGeom &g, int cap, int S,
-                       int *candPartner, int *candCount, int maxCand) {
+                       int *candPartner, int *candCount, int maxCand, hipStream_t st) {
     // bodies with a cell are exactly the first S (counting sort places each live body once)
-    hipLaunchKernelGGL(k_broad_phase, dim3(blocks_for(cap, BP_THREADS)), dim3(BP_THREADS), 0, c->stream, center,
+    hipLaunchKernelGGL(k_broad_phase, dim3(blocks_for(cap, BP_THREADS)), dim3(BP_THREADS), 0, st ? st : c->stream,
+                       center,
                        boundingRadius,
                        bodyCell, cellOffsets, cellContents, g, cap, S, S, candPartner, candCount, maxCand);
     return check_launch(c, "k_broad_phase");

@@ -56,9 +56,9 @@ void sort_scratch_free(SortScratch &s);
 // Stable counting sort of items by integer key (key < 0 skipped) into offsets[nKeys+1] and
 // list[...]; offsets[nKeys] is the total clamped to contentsCap; positions >= contentsCap are dropped.
 // gate (device int, may be null): when it reads 0 every kernel of the sort returns at once (a
-// rebuild the device decides on, inside a captured step).
+// rebuild the device decides on, inside a captured step).  st: stream (null: the context's).
 int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
-                  int *list, const int *gate = nullptr);
+                  int *list, const int *gate = nullptr, hipStream_t st = nullptr);
 int fill_i32(sbx_ctx *c, int *dst, int value, long long count);
 
 ChainParams chain_params(const float *cp);

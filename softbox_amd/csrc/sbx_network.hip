@@ -95,7 +95,11 @@ struct sbx_net {
     int *vlist = nullptr, *vcount = nullptr;
     int *skinFlags = nullptr;
     float *model = nullptr;                 // MLP_NPARAM device floats (learned mode)
-    float4 *skinPack = nullptr;             // {centre, bounding radius} of the current pose (by k_rod)
+    float4 *skinPackB[2] = {nullptr, nullptr};  // {centre, bounding radius} per pose buffer (by k_rod)
+    // broad phase on a second stream, overlapping the mechanics kernels (it feeds nothing in the step)
+    hipStream_t side = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr, evBuilt = nullptr;
+    int overlap = 1;
     // graph
     int useGraph = 0;
     int graphSteps = 0;  // non-zero while a capture is being recorded
@@ -131,7 +135,7 @@ struct RodArgs {
     int *nextCell;          // bodyCell of the new pose for the next step's grid (null: not fused)
     float4 *linkPack;       // packed endpoint data of the new pose for the next step's links (or null)
     const float *skinRef, *skin;  // skin guard: centre at the last rebuild and per-body skin (or null)
-    int *skinFlags;
+    int *rebuildFlag;             // skin rebuild request word of the new pose's buffer
     float4 *skinPack;             // {new centre, bounding radius} for the next step's skin kernels
     const float *boundR;
     GridGeom geom;
@@ -236,7 +240,7 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
             dz = min_image(dz, a.geom.Lz, a.geom.iLz);
         }
         float sk = a.skin[i];
-        if (dx * dx + dy * dy + dz * dz > sk * sk) a.skinFlags[SKF_REBUILD] = 1;
+        if (dx * dx + dy * dy + dz * dz > sk * sk) *a.rebuildFlag = 1;
         a.skinPack[i] = make_float4(p.c[0], p.c[1], p.c[2], a.boundR[i]);
     }
     if (a.linkPack) {
@@ -587,6 +591,8 @@ __global__ void __launch_bounds__(256) k_motor_integ(sbx_motors m, float dt, flo
 // small neighbourhood to scan.
 struct SkinArgs {
     int n, mode, listCap, maxCand, cCellsMax;
+    int gate;                // flag word of this pose's rebuild request (skf_rebuild(pose buffer))
+    int blockKeys;           // keys k_skin_pairs buffers for one heavy body (<= PT_BODIES * PT_KEYS)
     const float *coord, *uVec, *boundR;
     const int *links;        // activeLinkCount (may be null)
     GridGeom geom;           // fine grid (the broad phase's)
@@ -606,12 +612,13 @@ struct SkinArgs {
 
 constexpr float SKIN_SLACK = 1.0e-3f;  // um, covers float rounding of the list and guard distances
 constexpr int SKF_MAXSKIN = 6;         // flag word: largest skin of the rebuild (float bits)
+constexpr int SKF_HEAVY = 7;           // flag word: bodies k_skin_pairs did block-wide so far
 
 // Skins of this rebuild.  Learned mode after the first rebuild: thread c is a coarse cell of the
 // last rebuild -- features over its members, model, clamp, then the skin of every member;
 // otherwise thread i is a body and gets the default skin.  Tracks the largest skin (flag word).
 __global__ void __launch_bounds__(256) k_skin_assign(SkinArgs a) {
-    if (a.flags[SKF_REBUILD] == 0) return;
+    if (a.flags[a.gate] == 0) return;
     __shared__ float ws[MLP_NPARAM];
     int k = blockIdx.x * 256 + threadIdx.x;
     if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && a.model) {
@@ -661,7 +668,7 @@ __device__ __forceinline__ GridGeom skin_coarse_geom(const SkinArgs &a, float ma
 }
 
 __global__ void __launch_bounds__(256) k_skin_bin(SkinArgs a) {
-    if (a.flags[SKF_REBUILD] == 0) return;
+    if (a.flags[a.gate] == 0) return;
     int i = blockIdx.x * 256 + threadIdx.x;
     GridGeom g = skin_coarse_geom(a, __int_as_float(a.flags[SKF_MAXSKIN]));
     if (i == 0) {
@@ -680,7 +687,7 @@ __global__ void __launch_bounds__(256) k_skin_bin(SkinArgs a) {
 // Neighbour lists: every j > i within ri + rj + cutoff + s_i + s_j (+ slack) of i, from the 27
 // coarse cells around i.  Also records the pose the guard measures displacement from.
 __global__ void __launch_bounds__(BP_THREADS) k_skin_build(SkinArgs a) {
-    if (a.flags[SKF_REBUILD] == 0) return;
+    if (a.flags[a.gate] == 0) return;
     __shared__ int2 segLds[BP_THREADS * 18];
     int t = xcd_block(blockIdx.x, gridDim.x) * BP_THREADS + threadIdx.x;
     const int n = a.n;
@@ -694,7 +701,7 @@ __global__ void __launch_bounds__(BP_THREADS) k_skin_build(SkinArgs a) {
     const int total = stencil_ranges(a.cCell[i], g, a.cOff, seg);
     RangeWalk w(seg);
     int cnt = 0;
-    int *list = a.vlist + (size_t)i * a.listCap;
+    int *list = a.vlist + (size_t)t * a.listCap;  // per coarse slot: k_skin_pairs reads a tile's lists in one sweep
     for (int q = 0; q < total; q += BP_BATCH) {
         int jj[BP_BATCH];
 #pragma unroll
@@ -725,7 +732,7 @@ __global__ void __launch_bounds__(BP_THREADS) k_skin_build(SkinArgs a) {
             }
         }
     }
-    a.vcount[i] = cnt;
+    a.vcount[t] = cnt;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         a.refC[k * n + i] = a.coord[k * n + i];
@@ -741,156 +748,255 @@ __global__ void __launch_bounds__(256) k_max_radius(const float *__restrict__ r,
 }
 
 // One candidate test (the broad phase's predicate, same arithmetic) and its sort key (fine stencil
-// position, index); -1 when j is not a candidate of SpatialGrid.broadPhase.
-__device__ __forceinline__ int skin_key(int j, int3 ci, float icx, float icy, float icz, float ri, float pxj,
-                                        float pyj, float pzj, float rj, const int *__restrict__ bodyCell,
+// position << 26 | index); -1 when j is not a candidate of SpatialGrid.broadPhase.  j's fine cell
+// is binned from the same centre k_rod binned it from, so it equals the broad phase's cell of j.
+__device__ __forceinline__ int skin_key(int j, int3 ci, float icx, float icy, float icz, float ri, float4 qj,
                                         const GridGeom &g) {
-    float dx = icx - pxj, dy = icy - pyj, dz = icz - pzj;
+    float dx = icx - qj.x, dy = icy - qj.y, dz = icz - qj.z;
     if (g.periodic) {
         dx = min_image(dx, g.Lx, g.iLx);
         dy = min_image(dy, g.Ly, g.iLy);
         dz = min_image(dz, g.Lz, g.iLz);
     }
     float distSq = dx * dx + dy * dy + dz * dz;
-    float reach = ri + rj + g.cutoff;
+    float reach = ri + qj.w + g.cutoff;
     if (!(distSq <= reach * reach)) return -1;
-    int pos = stencil_pos(ci, bodyCell[j], g);
+    int pos = stencil_pos(ci, cell3_of(qj.x, qj.y, qj.z, g), g);
     if (pos < 0) return -1;
     return (pos << 26) | j;
 }
 
-constexpr int SKIN_KEYS = 64;     // per-thread LDS key buffer of k_skin_pairs (dense bundles reach ~50)
-constexpr int SKIN_THREADS = 128;  // k_skin_pairs block: 32 KB of keys
+constexpr int PT_BODIES = 64;    // coarse slots per k_skin_pairs block
+constexpr int PT_THREADS = 256;  // k_skin_pairs block
+constexpr int PT_KEYS = 64;      // candidates a block buffers per body (dense bundles: p99.9 ~40)
+constexpr int PT_UNROLL = 4;     // list entries in flight per thread
+constexpr int BIG_KEEP = SBX_MAX_CAND;  // keys kept for a body with more than PT_BODIES * PT_KEYS (maxCand <= this)
+static_assert(BIG_KEEP >= PT_THREADS && (BIG_KEEP & (BIG_KEEP - 1)) == 0 && 2 * BIG_KEEP <= PT_BODIES * PT_KEYS,
+              "bitonic halves in the key buffer");
+static_assert(PT_BODIES == 64, "the slot prefix sum is one wave's scan");
 
-// Calls visit(j) for every j in the 27 coarse cells around coarse cell c (overflow path).
-template <typename F>
-__device__ __forceinline__ void coarse_stencil(int c, const GridGeom &g, const int *__restrict__ off,
-                                               const int *__restrict__ cont, F visit) {
-    const int nXY = g.nX * g.nY;
-    int cz = c / nXY, rem = c - cz * nXY, cy = rem / g.nX, cx = rem - cy * g.nX;
-    for (int dz = -1; dz <= 1; dz++) {
-        int zz = cz + dz;
-        if (g.periodic) zz = zz < 0 ? zz + g.nZ : (zz >= g.nZ ? zz - g.nZ : zz);
-        else if (zz < 0 || zz >= g.nZ) continue;
-        for (int dy = -1; dy <= 1; dy++) {
-            int yy = cy + dy;
-            if (g.periodic) yy = yy < 0 ? yy + g.nY : (yy >= g.nY ? yy - g.nY : yy);
-            else if (yy < 0 || yy >= g.nY) continue;
-            for (int dx = -1; dx <= 1; dx++) {
-                int xx = cx + dx;
-                if (g.periodic) xx = xx < 0 ? xx + g.nX : (xx >= g.nX ? xx - g.nX : xx);
-                else if (xx < 0 || xx >= g.nX) continue;
-                int cc = xx + yy * g.nX + zz * nXY;
-                for (int p = off[cc]; p < off[cc + 1]; p++) visit(cont[p]);
+// The 27 coarse cells (of the last rebuild) around body i's coarse cell as contents ranges: thread
+// r < 27 stores range r's start in segLo[r] and its length in segStart[r + 1]; the caller sets
+// segStart[0] = 0 and prefix-sums.  Out-of-grid cells (box) are empty ranges; a periodic coarse
+// grid has >= 3 cells per axis, so the 27 cells are distinct.
+__device__ __forceinline__ void coarse_ranges(const SkinArgs &a, int i, int r, int *segLo, int *segStart) {
+    if (r == 0) segStart[0] = 0;
+    if (r >= 27) return;
+    const GridGeom cgm = *a.cg;
+    const int3 cc = cell_xyz(a.cCell[i], cgm);
+    int xx = cc.x + r % 3 - 1, yy = cc.y + (r / 3) % 3 - 1, zz = cc.z + r / 9 - 1;
+    bool ok = true;
+    if (cgm.periodic) {
+        xx = xx < 0 ? xx + cgm.nX : (xx >= cgm.nX ? xx - cgm.nX : xx);
+        yy = yy < 0 ? yy + cgm.nY : (yy >= cgm.nY ? yy - cgm.nY : yy);
+        zz = zz < 0 ? zz + cgm.nZ : (zz >= cgm.nZ ? zz - cgm.nZ : zz);
+    } else {
+        ok = xx >= 0 && xx < cgm.nX && yy >= 0 && yy < cgm.nY && zz >= 0 && zz < cgm.nZ;
+    }
+    int lo = 0, hi = 0;
+    if (ok) {
+        int c = xx + yy * cgm.nX + zz * cgm.nX * cgm.nY;
+        lo = a.cOff[c];
+        hi = a.cOff[c + 1];
+    }
+    segLo[r] = lo;
+    segStart[r + 1] = hi - lo;
+}
+
+// entry k of the concatenated coarse ranges
+__device__ __forceinline__ int coarse_entry(const SkinArgs &a, int k, const int *segLo, const int *segStart) {
+    int r = 0;
+    while (segStart[r + 1] <= k) r++;
+    return a.cCont[segLo[r] + (k - segStart[r])];
+}
+
+// Keeps the BIG_KEEP smallest keys of body i's candidates (list entries, or the concatenated coarse
+// ranges when over) in buf[0, BIG_KEEP), ascending, padded with INT_MAX: the candidates are tested
+// in chunks of PT_THREADS and (kept | chunk) is bitonic-sorted in LDS after each chunk.  For bodies
+// with more candidates than the block buffers; called by the whole block.
+__device__ void block_keep_smallest(const SkinArgs &a, int t, int i, bool over, int total, int3 ci, float4 qi,
+                                    const int *segLo, const int *segStart, int *buf, int *chunkN) {
+    const int tid = threadIdx.x;
+    for (int x = tid; x < BIG_KEEP; x += PT_THREADS) buf[x] = 0x7fffffff;
+    for (int base = 0; base < total; base += PT_THREADS) {
+        __syncthreads();  // the previous chunk's sort and every read of *chunkN are done
+        if (tid == 0) *chunkN = 0;
+        for (int x = tid; x < BIG_KEEP; x += PT_THREADS) buf[BIG_KEEP + x] = 0x7fffffff;
+        __syncthreads();
+        const int k = base + tid;
+        if (k < total) {
+            int j = over ? coarse_entry(a, k, segLo, segStart) : a.vlist[(size_t)t * a.listCap + k];
+            if (j > i) {
+                int key = skin_key(j, ci, qi.x, qi.y, qi.z, qi.w, a.pack[j], a.geom);
+                if (key >= 0) buf[BIG_KEEP + atomicAdd(chunkN, 1)] = key;  // <= PT_THREADS per chunk
+            }
+        }
+        __syncthreads();
+        if (*chunkN == 0) continue;
+        for (int size = 2; size <= 2 * BIG_KEEP; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int p = tid; p < BIG_KEEP; p += PT_THREADS) {
+                    int lo = ((p & ~(stride - 1)) << 1) | (p & (stride - 1));
+                    int hi = lo + stride;
+                    int x = buf[lo], y = buf[hi];
+                    if ((x > y) == ((lo & size) == 0)) {
+                        buf[lo] = y;
+                        buf[hi] = x;
+                    }
+                }
+                __syncthreads();
             }
         }
     }
+    __syncthreads();
 }
 
-// Candidate slices of the current pose from the neighbour lists (or, for a body whose list
-// overflowed, from the 27 coarse cells of the last rebuild), in the broad phase's order.  Keys are
-// gathered in LDS and sorted there; a body with more than SKIN_KEYS candidates sorts in its slice.
-__global__ void __launch_bounds__(SKIN_THREADS) k_skin_pairs(SkinArgs a) {
-    __shared__ int keyLds[SKIN_THREADS * SKIN_KEYS];
-    int t = xcd_block(blockIdx.x, gridDim.x) * SKIN_THREADS + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {  // no other thread reads these words in this kernel
-        a.flags[SKF_REBUILD] = 0;
+// Candidate slices of the current pose from the neighbour lists, in the broad phase's order.  A
+// block takes PT_BODIES consecutive coarse slots and walks the concatenation of their lists with
+// all its threads (so a long list does not hold back a wave), collects each body's keys in LDS and
+// places each key at its rank among its body's keys (keys are distinct).  A body with more than
+// PT_KEYS candidates, or whose list overflowed at the rebuild, is then done by the whole block.
+__global__ void __launch_bounds__(PT_THREADS) k_skin_pairs(SkinArgs a) {
+    __shared__ int keyS[PT_BODIES * PT_KEYS];
+    __shared__ float4 qS[PT_BODIES];
+    __shared__ int4 cS[PT_BODIES];     // fine cell (x, y, z) and body index
+    __shared__ int startS[PT_BODIES + 1], mS[PT_BODIES];
+    __shared__ int heavyS[PT_BODIES], heavyN, foundS;
+    __shared__ int segLo[27], segStart[28];
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0 && tid == 0) {  // no other thread reads these words in this kernel
+        a.flags[a.gate] = 0;  // the concurrent k_rod sets only the other buffer's word
         a.flags[SKF_SINCE] += 1;
         a.flags[SKF_STEPS] += 1;
         a.flags[SKF_MAXSKIN] = 0;  // the next rebuild's running max starts here
     }
     const int n = a.n;
-    if (t >= n) return;
     const GridGeom &g = a.geom;
-    int i = a.cCont[t];
-    const float4 qi = a.pack[i];
-    const float icx = qi.x, icy = qi.y, icz = qi.z, ri = qi.w;
-    const int3 ci = cell_xyz(a.bodyCell[i], g);
-    int *keys = keyLds + threadIdx.x;  // key x at keys[x * SKIN_THREADS]: lanes hit distinct banks
-    int *slice = a.candPartner + (size_t)i * a.maxCand;
-    const int cap = a.maxCand < SKIN_KEYS ? a.maxCand : SKIN_KEYS;
-    int m = 0;  // candidates found (may exceed the buffer and maxCand)
-    auto add = [&](int key) {
-        if (m < cap) keys[m * SKIN_THREADS] = key;
-        else if (m < a.maxCand) slice[m] = key;
-        m++;
-    };
-    const int cnt = a.vcount[i];
-    const bool overflow = cnt > a.listCap;
-    const int *list = a.vlist + (size_t)i * a.listCap;
-    auto test = [&](int j) -> int {
-        if (j <= i) return -1;
-        float4 q = a.pack[j];
-        return skin_key(j, ci, icx, icy, icz, ri, q.x, q.y, q.z, q.w, a.bodyCell, g);
-    };
-    if (!overflow) {
-        for (int q = 0; q < cnt; q += BP_BATCH) {
-            int jj[BP_BATCH];
-#pragma unroll
-            for (int u = 0; u < BP_BATCH; u++) jj[u] = (q + u < cnt) ? list[q + u] : -1;
-            float4 pj[BP_BATCH];
-#pragma unroll
-            for (int u = 0; u < BP_BATCH; u++) pj[u] = a.pack[jj[u] > i ? jj[u] : i];
-#pragma unroll
-            for (int u = 0; u < BP_BATCH; u++) {
-                if (jj[u] <= i) continue;
-                int key = skin_key(jj[u], ci, icx, icy, icz, ri, pj[u].x, pj[u].y, pj[u].z, pj[u].w, a.bodyCell, g);
-                if (key >= 0) add(key);
+    const int t0 = xcd_block(blockIdx.x, gridDim.x) * PT_BODIES;
+    if (tid < PT_BODIES) {  // wave 0: the tile's bodies and the prefix sum of their list lengths
+        const int t = t0 + tid;
+        int cnt = 0;
+        bool over = false;
+        if (t < n) {
+            cnt = a.vcount[t];
+            const int i = a.cCont[t];
+            const float4 q = a.pack[i];
+            const int3 c3 = cell3_of(q.x, q.y, q.z, g);
+            qS[tid] = q;
+            cS[tid] = make_int4(c3.x, c3.y, c3.z, i);
+            if (cnt > a.listCap) {  // overflowed list: walked below from the coarse cells
+                cnt = 0;
+                over = true;
             }
         }
-    } else {
-        coarse_stencil(a.cCell[i], *a.cg, a.cOff, a.cCont, [&](int j) {
-            int key = test(j);
-            if (key >= 0) add(key);
-        });
+        mS[tid] = over ? PT_KEYS + 1 : 0;  // an overflowed body counts as heavy
+        if (tid == 0) heavyN = 0;
+        int incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            int v = __shfl_up(incl, d, 64);
+            if (tid >= d) incl += v;
+        }
+        startS[tid + 1] = incl;
+        if (tid == 0) startS[0] = 0;
     }
-    if (m <= cap) {  // the common case: shell sort in LDS (gaps 23, 10, 4, 1), one pass of writes
-        const int gaps[4] = {23, 10, 4, 1};
+    __syncthreads();
+    const int total = startS[PT_BODIES];
+    for (int e0 = 0; e0 < total; e0 += PT_THREADS * PT_UNROLL) {
+        int bb[PT_UNROLL], jj[PT_UNROLL];
 #pragma unroll
-        for (int gi = 0; gi < 4; gi++) {
-            const int gap = gaps[gi];
-            for (int x = gap; x < m; x++) {
-                int v = keys[x * SKIN_THREADS];
-                int y = x - gap;
-                while (y >= 0 && keys[y * SKIN_THREADS] > v) {
-                    keys[(y + gap) * SKIN_THREADS] = keys[y * SKIN_THREADS];
-                    y -= gap;
-                }
-                keys[(y + gap) * SKIN_THREADS] = v;
+        for (int u = 0; u < PT_UNROLL; u++) {
+            const int e = e0 + u * PT_THREADS + tid;
+            bb[u] = -1;
+            jj[u] = 0;
+            if (e < total) {
+                int lo = 0;  // last body whose list starts at or before e
+#pragma unroll
+                for (int step = PT_BODIES / 2; step > 0; step >>= 1)
+                    if (startS[lo + step] <= e) lo += step;
+                bb[u] = lo;
+                jj[u] = a.vlist[(size_t)(t0 + lo) * a.listCap + (e - startS[lo])];
             }
         }
-        for (int x = 0; x < m; x++) slice[x] = keys[x * SKIN_THREADS] & ((1 << 26) - 1);
-    } else if (m <= a.maxCand) {  // more than the LDS buffer: sort in the slice
-        for (int x = 0; x < cap; x++) slice[x] = keys[x * SKIN_THREADS];
-        for (int x = 1; x < m; x++) {
-            int v = slice[x];
-            int y = x - 1;
-            while (y >= 0 && slice[y] > v) {
-                slice[y + 1] = slice[y];
-                y--;
+        float4 qj[PT_UNROLL];
+#pragma unroll
+        for (int u = 0; u < PT_UNROLL; u++) qj[u] = a.pack[jj[u]];
+#pragma unroll
+        for (int u = 0; u < PT_UNROLL; u++) {
+            if (bb[u] < 0) continue;
+            const float4 qi = qS[bb[u]];
+            const int4 ci = cS[bb[u]];
+            int key = skin_key(jj[u], make_int3(ci.x, ci.y, ci.z), qi.x, qi.y, qi.z, qi.w, qj[u], g);
+            if (key >= 0) {
+                int k = atomicAdd(&mS[bb[u]], 1);
+                if (k < PT_KEYS) keyS[bb[u] * PT_KEYS + k] = key;
             }
-            slice[y + 1] = v;
-        }
-        for (int x = 0; x < m; x++) slice[x] &= (1 << 26) - 1;
-    } else {
-        // more candidates than slots: the broad phase keeps the first maxCand in its order, so emit
-        // the maxCand smallest keys by repeated selection over the candidates (rare; exact)
-        int last = -1;
-        for (int x = 0; x < a.maxCand; x++) {
-            int best = 0x7fffffff;
-            auto pick = [&](int j) {
-                int key = test(j);
-                if (key > last && key < best) best = key;
-            };
-            if (!overflow)
-                for (int q = 0; q < cnt; q++) pick(list[q]);
-            else
-                coarse_stencil(a.cCell[i], *a.cg, a.cOff, a.cCont, pick);
-            slice[x] = best & ((1 << 26) - 1);
-            last = best;
         }
     }
-    a.candCount[i] = m;
+    __syncthreads();
+    // rank placement: key (b, k) goes to slot #{keys of b smaller than it}
+    for (int p = tid; p < PT_BODIES * PT_KEYS; p += PT_THREADS) {
+        const int b = p / PT_KEYS, k = p - b * PT_KEYS;
+        const int m = mS[b];
+        if (k >= m || m > PT_KEYS) continue;
+        const int key = keyS[p];
+        const int *kb = keyS + b * PT_KEYS;
+        int rank = 0;
+        for (int x = 0; x < m; x++) rank += kb[x] < key ? 1 : 0;
+        if (rank < a.maxCand) a.candPartner[(size_t)cS[b].w * a.maxCand + rank] = key & ((1 << 26) - 1);
+    }
+    if (tid < PT_BODIES && t0 + tid < n) {
+        const int m = mS[tid];
+        if (m <= PT_KEYS) a.candCount[cS[tid].w] = m;
+        else heavyS[atomicAdd(&heavyN, 1)] = tid;
+    }
+    __syncthreads();
+    // heavy bodies (more than PT_KEYS candidates, or an overflowed list), one at a time with the
+    // whole block: keys gathered into the whole key buffer and rank-placed; more than that many
+    // keys goes to k_skin_pairs_big
+    const int nHeavy = heavyN;
+    for (int h = 0; h < nHeavy; h++) {
+        const int b = heavyS[h], t = t0 + b;
+        const int i = cS[b].w;
+        const int cnt = a.vcount[t];
+        const bool over = cnt > a.listCap;
+        if (over) coarse_ranges(a, i, tid, segLo, segStart);
+        if (tid == 0) foundS = 0;
+        __syncthreads();
+        if (over && tid == 0)
+            for (int r = 0; r < 27; r++) segStart[r + 1] += segStart[r];
+        __syncthreads();
+        const int total = over ? segStart[27] : cnt;
+        const float4 qi = qS[b];
+        const int3 ci = make_int3(cS[b].x, cS[b].y, cS[b].z);
+        for (int k = tid; k < total; k += PT_THREADS) {
+            int j = over ? coarse_entry(a, k, segLo, segStart) : a.vlist[(size_t)t * a.listCap + k];
+            if (j <= i) continue;
+            int key = skin_key(j, ci, qi.x, qi.y, qi.z, qi.w, a.pack[j], g);
+            if (key >= 0) {
+                int p = atomicAdd(&foundS, 1);
+                if (p < a.blockKeys) keyS[p] = key;
+            }
+        }
+        __syncthreads();
+        const int m = foundS;
+        if (m <= a.blockKeys) {
+            for (int p = tid; p < m; p += PT_THREADS) {
+                const int key = keyS[p];
+                int rank = 0;
+                for (int x = 0; x < m; x++) rank += keyS[x] < key ? 1 : 0;
+                if (rank < a.maxCand) a.candPartner[(size_t)i * a.maxCand + rank] = key & ((1 << 26) - 1);
+            }
+            if (tid == 0) a.candCount[i] = m;
+        } else {  // more keys than the buffer: keep the maxCand smallest over chunks (exact for any number)
+            block_keep_smallest(a, t, i, over, total, ci, qi, segLo, segStart, keyS, &foundS);
+            const int w = m < a.maxCand ? m : a.maxCand;
+            for (int x = tid; x < w; x += PT_THREADS) a.candPartner[(size_t)i * a.maxCand + x] = keyS[x] & ((1 << 26) - 1);
+            if (tid == 0) a.candCount[i] = m;
+        }
+        if (tid == 0) atomicAdd(&a.flags[SKF_HEAVY], 1);
+        __syncthreads();  // before the next heavy body reuses the buffers
+    }
 }
 
 hipEvent_t take_event(sbx_net *net) {
@@ -1019,6 +1125,9 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
 SkinArgs skin_args(sbx_net *net, const float *coord, const float *uVec) {
     SkinArgs a;
     a.n = net->n; a.mode = net->skinMode; a.listCap = net->sp.listCap; a.maxCand = net->p.maxCand;
+    a.gate = skf_rebuild(net->cur);
+    a.blockKeys = net->sp.blockKeys > 0 && net->sp.blockKeys < PT_BODIES * PT_KEYS ? net->sp.blockKeys
+                                                                                  : PT_BODIES * PT_KEYS;
     a.cCellsMax = net->cCellsMax;
     a.coord = coord; a.uVec = uVec; a.boundR = net->boundR;
     a.links = net->L > 0 ? net->xl.activeLinkCount : nullptr;
@@ -1026,21 +1135,21 @@ SkinArgs skin_args(sbx_net *net, const float *coord, const float *uVec) {
     a.cCell = net->cCell; a.cOff = net->cOff; a.cCont = net->cCont;
     a.refC = net->refC; a.refU = net->refU; a.skin = net->skin;
     a.cellSkin = net->cellSkin; a.cellHorizon = net->cellHorizon;
-    a.vlist = net->vlist; a.vcount = net->vcount; a.flags = net->skinFlags; a.model = net->model;
+    a.vlist = net->vlist; a.vcount = net->vcount;
+    a.flags = net->skinFlags; a.model = net->model;
     a.skinDefault = net->sp.skinDefault; a.skinMin = net->sp.skinMin; a.skinMax = net->sp.skinMax;
     a.margin = net->sp.margin;
     a.maxR = net->maxR; a.cellFloor = net->cellFloor;
     a.bodyCell = net->bodyCell;
-    a.pack = net->skinPack;
+    a.pack = net->skinPackB[net->cur];
     a.candPartner = net->candPartner; a.candCount = net->candCount;
     return a;
 }
 
-// skin broad phase of one step: [gated rebuild: skins (model), geometry + bin, coarse sort, build]
-// -> pairs
-int enqueue_skin(sbx_net *net, const float *coord, const float *uVec) {
+// skin broad phase of one step on stream s: [gated rebuild: skins (model), geometry + bin, coarse
+// sort, build] -> (evBuilt when overlapping) -> pairs
+int enqueue_skin(sbx_net *net, const float *coord, const float *uVec, hipStream_t s, bool ovl) {
     sbx_ctx *c = net->ctx;
-    hipStream_t s = c->stream;
     const int n = net->n;
     SkinArgs a = skin_args(net, coord, uVec);
     int r;
@@ -1051,44 +1160,67 @@ int enqueue_skin(sbx_net *net, const float *coord, const float *uVec) {
         hipLaunchKernelGGL(k_skin_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
         if ((r = check_launch(c, "skin assign/bin"))) return r;
         if ((r = counting_sort(c, net->sort, net->cCell, n, net->cCellsMax, n, net->cOff, net->cCont,
-                               net->skinFlags + SKF_REBUILD)))
+                               net->skinFlags + a.gate, s)))
             return r;
         hipLaunchKernelGGL(k_skin_build, dim3(blocks_for(n, BP_THREADS)), dim3(BP_THREADS), 0, s, a);
         if ((r = check_launch(c, "k_skin_build"))) return r;
+        // k_rod's guard reads the skins and reference pose of this rebuild
+        if (ovl && (r = check_hip(c, hipEventRecord(net->evBuilt, s), "record evBuilt"))) return r;
     }
     {
         Scope sc(net, T_SKIN_PAIRS);
-        hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(n, SKIN_THREADS)), dim3(SKIN_THREADS), 0, s, a);
+        hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(n, PT_BODIES)), dim3(PT_THREADS), 0, s, a);
     }
     return check_launch(c, "k_skin_pairs");
 }
 
-// one step's launches; stepOff is added to the device step counter
-int enqueue_step(sbx_net *net, int stepOff) {
+// Does the step run its broad phase on the side stream?  Only for the filament-only step with the
+// grid on, and not while per-kernel timers (events on the main stream) are recorded.
+bool overlapped(const sbx_net *net) {
+    return net->overlap && net->side && !net->profiling && !net->hasMotors && net->p.gridOn;
+}
+
+// One step's launches; stepOff is added to the device step counter.  With overlap the broad phase
+// (its output is the step's candidate slices, read by no later kernel of the step) runs on the side
+// stream, forked after the previous step's k_rod:
+//   side: [sort + broad phase | gated skin rebuild -> evBuilt -> pairs] -> evJoin
+//   main: unbind -> link -> (wait: evJoin of the previous step | evBuilt) -> k_rod
+// k_rod overwrites the pose buffers, cells and skin pack the previous step's broad phase read, and
+// its skin guard reads this step's rebuild; the next fork orders the next broad phase after k_rod.
+// firstInCapture: no wait on an evJoin recorded outside the graph being captured (the caller joined
+// the side stream before the capture began).
+int enqueue_step(sbx_net *net, int stepOff, bool firstInCapture = false) {
     sbx_ctx *c = net->ctx;
     hipStream_t s = c->stream;
     const int n = net->n, L = net->L;
     const float *coord = net->coordB[net->cur], *uVec = net->uB[net->cur], *yVec = net->yB[net->cur];
+    const bool ovl = overlapped(net);
+    const bool skinOn = net->skinMode != SBX_SKIN_REBUILD && !net->hasMotors;
+    hipStream_t s2 = ovl ? net->side : s;
     int r;
     if (net->hasMotors) {
         if ((r = enqueue_myosin(net, coord, uVec, yVec, stepOff))) return r;
     } else if (net->p.gridOn) {
         if (!net->derivedValid) return fail(c, SBX_ERR_STATE, "net step: derived pose data not prepared");
-        if (net->skinMode != SBX_SKIN_REBUILD) {
-            if ((r = enqueue_skin(net, coord, uVec))) return r;
+        if (ovl) {
+            if ((r = check_hip(c, hipEventRecord(net->evFork, s), "record evFork"))) return r;
+            if ((r = check_hip(c, hipStreamWaitEvent(s2, net->evFork, 0), "wait evFork"))) return r;
+        }
+        if (skinOn) {
+            if ((r = enqueue_skin(net, coord, uVec, s2, ovl))) return r;
         } else {
-        {
-            Scope sc(net, T_SORT);
-            r = counting_sort(c, net->sort, net->bodyCell, n, net->p.gridDims[3], n, net->cellOffsets,
-                              net->cellContents);
-            if (r) return r;
-        }
-        {
-            Scope sc(net, T_PAIRS);
-            r = launch_broad_phase(c, coord, net->boundR, net->bodyCell, net->cellOffsets, net->cellContents,
-                                   net->geom, n, n, net->candPartner, net->candCount, net->p.maxCand);
-            if (r) return r;
-        }
+            {
+                Scope sc(net, T_SORT);
+                r = counting_sort(c, net->sort, net->bodyCell, n, net->p.gridDims[3], n, net->cellOffsets,
+                                  net->cellContents, nullptr, s2);
+                if (r) return r;
+            }
+            {
+                Scope sc(net, T_PAIRS);
+                r = launch_broad_phase(c, coord, net->boundR, net->bodyCell, net->cellOffsets, net->cellContents,
+                                       net->geom, n, n, net->candPartner, net->candCount, net->p.maxCand, s2);
+                if (r) return r;
+            }
         }
     }
     if (L > 0) {
@@ -1127,11 +1259,10 @@ int enqueue_step(sbx_net *net, int stepOff) {
     a.geom = net->geom;
     a.nextCell = (net->p.gridOn && !net->hasMotors) ? net->cellNext : nullptr;
     a.linkPack = L > 0 ? net->linkPack : nullptr;
-    bool guard = net->skinMode != SBX_SKIN_REBUILD && !net->hasMotors;
-    a.skinRef = guard ? net->refC : nullptr;
-    a.skin = guard ? net->skin : nullptr;
-    a.skinFlags = net->skinFlags;
-    a.skinPack = net->skinPack;
+    a.skinRef = skinOn ? net->refC : nullptr;
+    a.skin = skinOn ? net->skin : nullptr;
+    a.rebuildFlag = skinOn ? net->skinFlags + skf_rebuild(nxt) : nullptr;
+    a.skinPack = net->skinPackB[nxt];
     a.boundR = net->boundR;
     a.mOff = net->mOff; a.mList = net->mList; a.bondData = net->mo.bondData;
     if (net->hasMotors) {
@@ -1149,10 +1280,18 @@ int enqueue_step(sbx_net *net, int stepOff) {
     a.halfY = 0.5 * net->p.boxParams[2] - net->p.boxParams[4];
     a.halfZ = 0.5 * net->p.boxParams[3] - net->p.boxParams[4];
     a.coeff = net->p.boxParams[5];
+    if (ovl) {
+        if (skinOn)
+            r = check_hip(c, hipStreamWaitEvent(s, net->evBuilt, 0), "wait evBuilt");
+        else if (!firstInCapture)
+            r = check_hip(c, hipStreamWaitEvent(s, net->evJoin, 0), "wait evJoin");
+        if (r) return r;
+    }
     {
         Scope sc(net, T_ROD);
         hipLaunchKernelGGL(k_rod, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, net->cp);
     }
+    if (ovl && (r = check_hip(c, hipEventRecord(net->evJoin, s2), "record evJoin"))) return r;
     net->cur = nxt;
     net->lastCell = net->bodyCell;
     if (a.nextCell) {  // the next step's grid reads the cells k_rod just wrote
@@ -1174,6 +1313,7 @@ void release(sbx_net *net) {
     if (!net) return;
     if (net->ctx) hipSetDevice(net->ctx->device);
     if (net->ctx) hipStreamSynchronize(net->ctx->stream);
+    if (net->side) hipStreamSynchronize(net->side);
     if (net->graphExec) hipGraphExecDestroy(net->graphExec);
     if (net->graph) hipGraphDestroy(net->graph);
     void *bufs[] = {net->ownPose,  net->boundR,    net->bodyCell, net->cellNext,    net->cellContents, net->linkPack,
@@ -1183,7 +1323,7 @@ void release(sbx_net *net) {
                     net->mOff,     net->mList,     net->bbOff,    net->bbList,
                     net->cCell,    net->cOff,      net->cCont,    net->refC,    net->refU, net->skin,
                     net->cellSkin, net->cellHorizon, net->vlist,  net->vcount,  net->skinFlags, net->model,
-                    net->cgDev,    net->cCellsDev, net->skinPack};
+                    net->cgDev,    net->cCellsDev, net->skinPackB[0], net->skinPackB[1]};
     for (void *b : bufs)
         if (b) hipFree(b);
     sort_scratch_free(net->sort);
@@ -1192,6 +1332,9 @@ void release(sbx_net *net) {
         hipEventDestroy(pe.second.second);
     }
     for (hipEvent_t e : net->eventPool) hipEventDestroy(e);
+    for (hipEvent_t e : {net->evFork, net->evJoin, net->evBuilt})
+        if (e) hipEventDestroy(e);
+    if (net->side) hipStreamDestroy(net->side);
     delete net;
 }
 
@@ -1267,6 +1410,11 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
             if ((r = alloc(c, &net->cellContents, n))) break;
             if ((r = alloc(c, &net->candPartner, (size_t)n * p->maxCand))) break;
             if ((r = alloc(c, &net->candCount, n))) break;
+            // default priority, like the context's stream
+            if ((r = check_hip(c, hipStreamCreateWithFlags(&net->side, hipStreamNonBlocking), "side stream"))) break;
+            if ((r = check_hip(c, hipEventCreateWithFlags(&net->evFork, hipEventDisableTiming), "event"))) break;
+            if ((r = check_hip(c, hipEventCreateWithFlags(&net->evJoin, hipEventDisableTiming), "event"))) break;
+            if ((r = check_hip(c, hipEventCreateWithFlags(&net->evBuilt, hipEventDisableTiming), "event"))) break;
             hipLaunchKernelGGL(k_bounding_radius, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, fil->segLength,
                                p->actinRadius, net->boundR, n);
             if ((r = check_launch(c, "k_bounding_radius"))) break;
@@ -1417,7 +1565,7 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
             return r;
         if (net->skinMode != SBX_SKIN_REBUILD && !net->hasMotors) {
             hipLaunchKernelGGL(k_pack_skin, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream,
-                               net->coordB[net->cur], net->boundR, net->skinPack, net->n);
+                               net->coordB[net->cur], net->boundR, net->skinPackB[net->cur], net->n);
             if ((r = check_launch(c, "k_pack_skin"))) return r;
         }
         if (net->L > 0) {
@@ -1433,11 +1581,16 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
         const int G = 2;
         if (!net->graphExec) {
             int savedCur = net->cur;
+            // side-stream work of earlier eager steps is joined before the graph (which joins its own)
+            if (overlapped(net) && (r = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin, 0), "join")))
+                return r;
             if ((r = check_hip(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
                 return r;
             net->graphSteps = G;
             int er = SBX_OK;
-            for (int k = 0; k < G && !er; k++) er = enqueue_step(net, k);
+            for (int k = 0; k < G && !er; k++) er = enqueue_step(net, k, k == 0);
+            if (!er && overlapped(net))
+                er = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin, 0), "join in capture");
             if (!er) hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, c->stream, net->dStep, G);
             hipGraph_t g = nullptr;
             hipError_t ee = hipStreamEndCapture(c->stream, &g);
@@ -1463,6 +1616,8 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
                 if ((r = check_launch(c, "k_add_step"))) return r;
                 advanced = done;
             }
+            if (overlapped(net) && (r = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin, 0), "join")))
+                return r;
             if ((r = check_hip(c, hipGraphLaunch(net->graphExec, c->stream), "graph launch"))) return r;
             done += G;
             advanced += G;  // G is even: net->cur is unchanged by a replay
@@ -1471,6 +1626,8 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
     for (; done < nSteps; done++) {
         if ((r = enqueue_step(net, done - advanced))) return r;
     }
+    // the caller's stream covers the whole call: join the side stream
+    if (overlapped(net) && (r = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin, 0), "join"))) return r;
     if (net->profiling) return drain_timers(net);
     return SBX_OK;
 }
@@ -1492,7 +1649,10 @@ int sbx_net_publish(sbx_net *net) {
     }
     // the caller may now edit its pose arrays: re-bin, re-pack and (skin mode) rebuild next step
     net->derivedValid = 0;
-    if (!r && net->skinFlags) r = fill_i32(c, net->skinFlags + SKF_REBUILD, 1, 1);
+    if (!r && net->skinFlags) {
+        r = fill_i32(c, net->skinFlags + skf_rebuild(net->cur), 1, 1);
+        if (!r) r = fill_i32(c, net->skinFlags + skf_rebuild(1 - net->cur), 0, 1);
+    }
     return r;
 }
 
@@ -1525,10 +1685,12 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
         SBX_REQUIRE(c, net->p.gridOn, "net_set_skin: needs the grid (gridOn)");
         SBX_REQUIRE(c, sp, "net_set_skin: null params");
         SBX_REQUIRE(c, sp->skinDefault > 0.0f && sp->skinMin > 0.0f && sp->skinMax >= sp->skinMin &&
-                           sp->skinDefault <= sp->skinMax && sp->margin > 0.0f && sp->listCap > 0,
-                    "net_set_skin: need 0 < skinMin <= skinDefault <= skinMax, margin > 0, listCap > 0");
+                           sp->skinDefault <= sp->skinMax && sp->margin > 0.0f && sp->listCap > 0 &&
+                           sp->blockKeys >= 0,
+                    "net_set_skin: need 0 < skinMin <= skinDefault <= skinMax, margin > 0, listCap > 0, blockKeys >= 0");
         SBX_REQUIRE(c, mode != SBX_SKIN_LEARNED || modelWeights, "net_set_skin: learned mode needs model weights");
         SBX_REQUIRE(c, net->n < (1 << 26), "net_set_skin: more than 2^26 bodies");
+        SBX_REQUIRE(c, net->p.maxCand <= BIG_KEEP, "net_set_skin: maxCand %d above %d", net->p.maxCand, BIG_KEEP);
     }
     int r = set_device(c);
     if (r) return r;
@@ -1581,13 +1743,13 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
     // (re)allocate the skin state
     void *old[] = {net->cCell, net->cOff,   net->cCont,     net->refC,  net->refU,      net->skin,
                    net->cellSkin, net->cellHorizon, net->vlist, net->vcount, net->skinFlags, net->model,
-                   net->cgDev,  net->cCellsDev, net->skinPack};
+                   net->cgDev,  net->cCellsDev, net->skinPackB[0], net->skinPackB[1]};
     for (void *b : old)
         if (b) hipFree(b);
     net->cCell = net->cOff = net->cCont = net->vlist = net->vcount = net->skinFlags = net->cCellsDev = nullptr;
     net->refC = net->refU = net->skin = net->cellSkin = net->cellHorizon = net->model = nullptr;
     net->cgDev = nullptr;
-    net->skinPack = nullptr;
+    net->skinPackB[0] = net->skinPackB[1] = nullptr;
     do {
         if ((r = alloc(c, &net->cCell, n))) break;
         if ((r = alloc(c, &net->cOff, (size_t)cc + 1))) break;
@@ -1601,7 +1763,8 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
         if ((r = alloc(c, &net->vcount, n))) break;
         if ((r = alloc(c, &net->skinFlags, SKF_WORDS))) break;
         if ((r = alloc(c, &net->cgDev, 1))) break;
-        if ((r = alloc(c, &net->skinPack, n))) break;
+        if ((r = alloc(c, &net->skinPackB[0], n))) break;
+        if ((r = alloc(c, &net->skinPackB[1], n))) break;
         if ((r = alloc(c, &net->cCellsDev, 1))) break;
         if ((r = fill_i32(c, net->cCellsDev, 0, 1))) break;
         if (mode == SBX_SKIN_LEARNED) {
@@ -1613,7 +1776,7 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
         long long keys = net->p.gridDims[3] > cc ? (long long)net->p.gridDims[3] : cc;
         if ((r = sort_scratch_reserve(c, net->sort, keys, n))) break;
         if ((r = fill_i32(c, net->skinFlags, 0, SKF_WORDS))) break;
-        if ((r = fill_i32(c, net->skinFlags + SKF_REBUILD, 1, 1))) break;  // the first step builds
+        if ((r = fill_i32(c, net->skinFlags + skf_rebuild(net->cur), 1, 1))) break;  // the first step builds
         r = sbx_sync(c);
     } while (0);
     if (r) net->skinMode = SBX_SKIN_REBUILD;
@@ -1630,8 +1793,9 @@ int sbx_net_skin_stats(sbx_net *net, long long *out, int cap) {
     int r = check_hip(c, hipMemcpyAsync(f, net->skinFlags, sizeof(f), hipMemcpyDeviceToHost, c->stream), "copy");
     if (!r) r = check_hip(c, hipStreamSynchronize(c->stream), "sync");
     if (r) return r;
+    const int words[] = {skf_rebuild(net->cur), SKF_SINCE, SKF_REBUILDS, SKF_OVERFLOW, SKF_ENTRIES, SKF_STEPS, SKF_HEAVY};
     int k = 0;
-    for (; k < SKF_WORDS && k < cap; k++) out[k] = f[k];
+    for (; k < (int)(sizeof(words) / sizeof(words[0])) && k < cap; k++) out[k] = f[words[k]];
     return k;
 }
 
@@ -1692,6 +1856,21 @@ int sbx_net_use_graph(sbx_net *net, int on) {
     SBX_CHECK_CTX(net->ctx);
     if (on) SBX_REQUIRE(net->ctx, net->ctx->stream != nullptr, "net_use_graph: needs a non-default stream");
     net->useGraph = on ? 1 : 0;
+    return SBX_OK;
+}
+
+int sbx_net_set_overlap(sbx_net *net, int on) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    int r = set_device(c);
+    if (r) return r;
+    if ((r = check_hip(c, hipStreamSynchronize(c->stream), "net_set_overlap sync"))) return r;
+    if (net->side && (r = check_hip(c, hipStreamSynchronize(net->side), "net_set_overlap sync"))) return r;
+    // a captured step no longer matches the launch sequence
+    if (net->graphExec) { hipGraphExecDestroy(net->graphExec); net->graphExec = nullptr; }
+    if (net->graph) { hipGraphDestroy(net->graph); net->graph = nullptr; }
+    net->overlap = on ? 1 : 0;
     return SBX_OK;
 }
 

@@ -16,13 +16,15 @@ namespace sbx {
 
 constexpr int SKIN_FEATURES = 12;
 // device flag words of the skin state
-constexpr int SKF_REBUILD = 0;     // 1: rebuild before this step's pairs (guard breach, first step)
+constexpr int SKF_REBUILD = 0;     // 1: rebuild before the pairs of the pose in buffer 0 (guard breach, first step)
+constexpr int SKF_REBUILD1 = 8;    // the same for a pose in buffer 1 (the guard sets the other buffer's word)
 constexpr int SKF_SINCE = 1;       // steps since the last rebuild
 constexpr int SKF_REBUILDS = 2;    // rebuilds so far
 constexpr int SKF_OVERFLOW = 3;    // bodies whose list overflowed at the last rebuild
 constexpr int SKF_ENTRIES = 4;     // list entries at the last rebuild (truncated to 32 bits)
 constexpr int SKF_STEPS = 5;       // steps in skin mode so far
-constexpr int SKF_WORDS = 8;
+constexpr int SKF_WORDS = 16;
+__host__ __device__ __forceinline__ int skf_rebuild(int buf) { return buf ? SKF_REBUILD1 : SKF_REBUILD; }
 
 // q = c / d (c >= 0, d > 0, q < 2^20) by a float reciprocal and exact integer correction
 __device__ __forceinline__ int div_small(int c, int d, float invD) {
@@ -41,10 +43,9 @@ __device__ __forceinline__ int3 cell_xyz(int c, const GridGeom &g) {
     return make_int3(r - y * g.nX, y, z);
 }
 
-// stencil position 0..26 of fine cell cj relative to cell (ix, iy, iz) in (z, y, x) order, or -1
-// when cj is not one of the (up to) 27 cells SpatialGrid.broadPhase visits for it
-__device__ __forceinline__ int stencil_pos(int3 ci, int cj, const GridGeom &g) {
-    int3 j = cell_xyz(cj, g);
+// stencil position 0..26 of fine cell j relative to cell i in (z, y, x) order, or -1 when j is not
+// one of the (up to) 27 cells SpatialGrid.broadPhase visits for i
+__device__ __forceinline__ int stencil_pos(int3 ci, int3 j, const GridGeom &g) {
     int dx = j.x - ci.x, dy = j.y - ci.y, dz = j.z - ci.z;
     if (g.periodic) {  // >= 3 bins per axis, so each offset maps to one of -1, 0, +1 at most once
         if (dx == g.nX - 1) dx = -1; else if (dx == 1 - g.nX) dx = 1;

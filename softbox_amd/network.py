@@ -50,7 +50,7 @@ class Network:
         out = (C.c_longlong * 8)()
         k = self.ctx.check(self.ctx.lib.sbx_net_skin_stats(self.h, out, 8))
         names = ("rebuild_pending", "steps_since_rebuild", "rebuilds", "overflow_bodies", "list_entries",
-                 "skin_steps")
+                 "skin_steps", "block_sorted")
         return {names[i]: int(out[i]) for i in range(min(k, len(names)))}
 
     def skin_state(self) -> dict:
@@ -70,6 +70,10 @@ class Network:
 
     def use_graph(self, on: bool = True) -> None:
         self.ctx.check(self.ctx.lib.sbx_net_use_graph(self.h, 1 if on else 0))
+
+    def set_overlap(self, on: bool = True) -> None:
+        """Broad phase on a second stream, concurrent with the mechanics kernels (default on)."""
+        self.ctx.check(self.ctx.lib.sbx_net_set_overlap(self.h, 1 if on else 0))
 
     def set_profiling(self, on: bool) -> None:
         self.ctx.check(self.ctx.lib.sbx_net_set_profiling(self.h, 1 if on else 0))

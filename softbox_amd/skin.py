@@ -32,9 +32,11 @@ class SkinConfig:
     list_cap: int = 256           # neighbour slots per body (bundles are ~10x denser than average)
     horizon: int = 8              # steps a skin is trained to last
     window: int = 32              # steps tracked after a snapshot (censoring of the horizon label)
+    block_keys: int = 0           # keys a block buffers for one dense body (0: the library's maximum)
 
     def params(self) -> _lib.SbxSkinParams:
-        return _lib.SbxSkinParams(self.skin_default, self.skin_min, self.skin_max, self.margin, self.list_cap)
+        return _lib.SbxSkinParams(self.skin_default, self.skin_min, self.skin_max, self.margin, self.list_cap,
+                                  self.block_keys)
 
 
 class SkinModel:

@@ -149,3 +149,42 @@ def test_free_rods_fdt(ctx):
     Dr = -np.log(C) / (2 * T)
     assert abs(Dr / Drot - 1) < 0.2, (Dr, Drot, C)  # ~3 standard errors at 1024 rods
     net.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("skin", [False, True])
+def test_overlap_equals_single_stream(ctx, graph, skin):
+    """The broad phase on the side stream (default) gives the same candidate slices, cells and pose
+    as one stream, step after step, eager and captured, per-step rebuild and skin broad phase."""
+    from softbox_amd import _lib
+    from softbox_amd.network import Network
+    from softbox_amd.skin import SkinConfig
+
+    sc = small_network(seed=8, n_fil=200, links=500)
+    a = Network(ctx, sc.copy())
+    b = Network(ctx, sc.copy())
+    a.set_overlap(False)
+    for net in (a, b):
+        if skin:
+            net.set_skin(_lib.SBX_SKIN_FIXED, SkinConfig(skin_default=0.15))
+        if graph:
+            net.use_graph(True)
+    n, mc = sc.fs.n, int(sc.p["maxCand"])
+    t = 0
+    for k in (3, 4, 2, 5):
+        a.step(t, k)
+        b.step(t, k)
+        t += k
+        ca, na, _, _, la = a.candidates(n, mc)
+        cb, nb, _, _, lb = b.candidates(n, mc)
+        assert_same(nb, na, f"candCount after {t} steps")
+        keep = (np.arange(mc)[None, :] < np.minimum(na, mc)[:, None]).ravel()
+        assert_same(cb[keep], ca[keep], f"candPartner after {t} steps")
+        assert_same(lb, la, f"bodyCell after {t} steps")
+    fa, xa = a.pull(sc.fs.copy(), sc.xl.copy())
+    fb, xb = b.pull(sc.fs.copy(), sc.xl.copy())
+    for k in ("coord", "uVec", "yVec", "end1", "forceSum"):
+        assert_same(getattr(fb, k), getattr(fa, k), f"overlap vs one stream {k}")
+    assert_same(xb.linkState, xa.linkState, "overlap vs one stream linkState")
+    a.close()
+    b.close()

@@ -119,6 +119,19 @@ def test_list_and_slice_overflow_stay_exact(ctx):
     assert stats["overflow_bodies"] > 0, stats
 
 
+@pytest.mark.parametrize("max_cand,block_keys", [(256, 0), (36, 0), (256, 16), (36, 16)])
+def test_dense_bodies_block_sorted_stay_exact(ctx, max_cand, block_keys):
+    """Bodies with more than 64 candidates are done by the whole block; maxCand 36 keeps only the
+    first 36 of more; block_keys 16 sends them through the chunked keep-smallest reduction."""
+    from softbox_amd import scenes
+
+    sc = scenes.network(n_fil=400, segs=10, box=1.0, n_links=600, seed=25, periodic=False)
+    sc.p["maxCand"] = max_cand
+    cfg = SkinConfig(skin_default=0.15, block_keys=block_keys)
+    stats = _run_pair(ctx, sc, _lib.SBX_SKIN_FIXED, cfg, steps=12)
+    assert stats["block_sorted"] > 0, stats
+
+
 def test_skin_graph_replay_identical(ctx):
     sc = small_network(seed=23, n_fil=300, links=600)
     _run_pair(ctx, sc, _lib.SBX_SKIN_FIXED, SkinConfig(skin_default=0.15), steps=24, graph=True, check_every=4)
@@ -138,9 +151,9 @@ def test_learned_skin_trains_and_stays_exact(ctx):
     assert stats["rebuilds"] >= 1
 
 
-def _hetero_scene(seed):
+def _hetero_scene(seed, eta):
     """Crosslinked network whose x > 0 half is 'hot': Brownian amplitude x2 there."""
-    sc = small_network(seed=seed, n_fil=600, links=1200)
+    sc = small_network(seed=seed, n_fil=600, links=1200, eta=eta)
     fs = sc.fs
     hot = fs.coord[: fs.n] > 0.0
     fs.brownTransScale[hot] *= 2.0
@@ -163,11 +176,12 @@ def _cost(ctx, sc, mode, cfg, steps, model=None, rebuild_weight=30.0):
     return entries + rebuild_weight * sc.fs.n * st["rebuilds"], st
 
 
-def test_learned_skin_beats_fixed_default_on_heterogeneous_mobility(ctx):
+@pytest.mark.parametrize("eta,horizon", [(0.1, 4), (0.1, 8)])
+def test_learned_skin_beats_fixed_default_on_heterogeneous_mobility(ctx, eta, horizon):
     from softbox_amd.network import Network
 
-    sc = _hetero_scene(31)
-    cfg = SkinConfig(horizon=4, window=16)
+    sc = _hetero_scene(31, eta)
+    cfg = SkinConfig(horizon=horizon, window=4 * horizon)
     net = Network(ctx, sc.copy())
     model, hist, _, t = fit(ctx, net, cfg, snapshots=8, iters=300)
     net.close()
