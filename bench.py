@@ -142,6 +142,8 @@ def main() -> int:
     ap.add_argument("--skin", choices=("compare", "rebuild", "fixed", "learned"), default="compare",
                     help="compare: per-step rebuild and learned skin on identical trajectories (value = learned)")
     ap.add_argument("--graph", type=int, default=1, help="1: replay the step as a captured hipGraph")
+    ap.add_argument("--overlap", type=int, default=1,
+                    help="1: broad phase on a second stream, concurrent with the mechanics kernels")
     ap.add_argument("--profile-steps", type=int, default=40, help="eager steps timed per kernel (0: none)")
     ap.add_argument("--train-snapshots", type=int, default=12)
     ap.add_argument("--train-iters", type=int, default=400)
@@ -205,6 +207,7 @@ def main() -> int:
         if mode != "rebuild":
             net.set_skin(_lib.SBX_SKIN_LEARNED if mode == "learned" else _lib.SBX_SKIN_FIXED, cfg,
                          model if mode == "learned" else None)
+        net.set_overlap(bool(args.overlap))
         if args.graph:
             net.use_graph(True)
         net.step(0, args.warmup)

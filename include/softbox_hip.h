@@ -265,7 +265,7 @@ typedef struct sbx_net_params {
     int confineOn;
     int gridOn;             /* 1: grid + broad phase every step (per-step rebuild) */
     int periodic;
-    int maxCand;            /* per-body candidate slice (SpatialGrid.MAX_CAND = 256) */
+    int maxCand;            /* per-body candidate slice, 1..SBX_MAX_CAND (SpatialGrid.MAX_CAND = 256) */
     float gridParams[6];
     int gridDims[4];
     float boxLen[3];
@@ -332,8 +332,7 @@ typedef struct sbx_skin_params {
     int blockKeys;       /* keys a block buffers for one dense body, 0 = the maximum (4096); a body with
                           * more is reduced to its first maxCand in chunks (smaller values exercise that) */
 } sbx_skin_params;
-/* modelWeights: device array of sbx_mlp_nparam() floats (learned mode; copied); the skin modes need
- * the net's maxCand <= SBX_MAX_CAND */
+/* modelWeights: device array of sbx_mlp_nparam() floats (learned mode; copied) */
 int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *p, const float *modelWeights);
 /* device counters -> out[0..cap): rebuild pending, steps since rebuild, rebuilds, overflowing bodies,
  * list entries at the last build, skin-mode steps, body-steps done block-wide so far (more than 64

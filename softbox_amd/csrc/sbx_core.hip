@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(256) k_key_scatter(const int *__restrict__ key
 __global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, int nItems, int nKeys,
                                                    const int *__restrict__ offsets, const int *__restrict__ tmp,
                                                    int *__restrict__ list, int contentsCap,
-                                                   const int *__restrict__ gate) {
+                                                   const int *__restrict__ gate, SortPayload pay) {
     if (gate && *gate == 0) return;
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nItems) return;
@@ -272,11 +272,15 @@ __global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, 
     int rank = 0;
     for (int p = start; p < end; p++) rank += (tmp[p] < m) ? 1 : 0;
     int pos = start + rank;
-    if (pos < contentsCap) list[pos] = m;
+    if (pos < contentsCap) {
+        list[pos] = m;
+        if (pay.in) pay.out[pos] = pay.in[m];
+        if (pay.keyOut) pay.keyOut[pos] = k;
+    }
 }
 
 int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
-                  int *list, const int *gate, hipStream_t st) {
+                  int *list, const int *gate, hipStream_t st, SortPayload pay) {
     int r = sort_scratch_reserve(c, s, nKeys, nItems);
     if (r) return r;
     hipStream_t q = st ? st : c->stream;
@@ -293,7 +297,7 @@ int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nK
         hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
                            offsets, s.slot, s.counts, s.tmp, contentsCap, gate);
         hipLaunchKernelGGL(k_key_place, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
-                           offsets, s.tmp, list, contentsCap, gate);
+                           offsets, s.tmp, list, contentsCap, gate, pay);
     }
     r = check_launch(c, "counting_sort");
     if (r) s.clean = false;

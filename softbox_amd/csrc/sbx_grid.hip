@@ -32,27 +32,106 @@ __global__ void __launch_bounds__(256) k_body_cell(const float *__restrict__ cen
     bodyCell[i] = cell_of(center[i], center[cap + i], center[2 * cap + i], g);
 }
 
-__global__ void __launch_bounds__(BP_THREADS) k_broad_phase(const float *__restrict__ center,
-                                                     const float *__restrict__ boundingRadius,
-                                                     const int *__restrict__ bodyCell,
-                                                     const int *__restrict__ cellOffsets,
-                                                     const int *__restrict__ cellContents, GridGeom g, int cap, int S,
-                                                     int nSorted, int *__restrict__ candPartner,
-                                                     int *__restrict__ candCount, int maxCand) {
+// Pose of a candidate: at(q, j) = {centre, bounding radius} of body j = contents[q]; own(t, i) and
+// cell(t, i) for the querying body i = contents[t].
+// PlanarPose: planar centre[3][cap] + radius[cap] and bodyCell, by body index.
+struct PlanarPose {
+    const float *c, *r;
+    const int *bodyCell;
+    int cap;
+    __device__ float4 at(int, int j) const { return make_float4(c[j], c[cap + j], c[2 * cap + j], r[j]); }
+    __device__ float4 own(int, int i) const { return at(0, i); }
+    __device__ int cell(int, int i) const { return bodyCell[i]; }
+};
+
+// SortedPose: pose and cell already laid out in contents order (by the sort's place pass), so a
+// candidate's pose sits at its contents position -- no gather through the body index.
+struct SortedPose {
+    const float4 *p;
+    const int *cellOf;
+    __device__ float4 at(int q, int) const { return p[q]; }
+    __device__ float4 own(int t, int) const { return p[t]; }
+    __device__ int cell(int t, int) const { return cellOf[t]; }
+};
+
+// Candidate test of the broad phase (SpatialGrid.java:383-396: minimum image in the periodic
+// extension, inclusive sphere predicate on ri + rj + cutoff).
+__device__ __forceinline__ bool bp_hit(float4 qi, float4 qj, const GridGeom &g) {
+    float dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
+    if (g.periodic) {
+        dx = min_image(dx, g.Lx, g.iLx);
+        dy = min_image(dy, g.Ly, g.iLy);
+        dz = min_image(dz, g.Lz, g.iLz);
+    }
+    float distSq = dx * dx + dy * dy + dz * dz;
+    float reach = qi.w + qj.w + g.cutoff;
+    return distSq <= reach * reach;
+}
+
+// SpatialGrid.broadPhase (SpatialGrid.java:340-406).  A group of G consecutive lanes takes one
+// body of the cell-sorted contents (neighbouring groups query neighbouring cells, so offsets,
+// contents and poses stay in cache) and walks its stencil -- 18 contents ranges in the reference's
+// visiting order ((z, y, x) cells, ascending index within a cell), mapped to positions in registers
+// -- G * U consecutive positions per round: lane l of the group takes positions base + k G + l
+// (k < U), so a group's loads are contiguous.  Hits (j > i and the test) are ordered by position
+// with a ballot: a hit's slot is the group's count so far plus the hits at earlier positions of the
+// round.  Counts keep going past maxCand (SpatialGrid.java:397-398).
+constexpr int BP_GROUP = 4;  // lanes per body
+constexpr int BP_ROUND = 4;  // positions per lane and round
+
+template <typename Pose, int G, int U>
+__global__ void __launch_bounds__(BP_THREADS) k_broad_phase(Pose pose, const int *__restrict__ cellOffsets,
+                                                            const int *__restrict__ cellContents, GridGeom g,
+                                                            int cap, int nSorted, int *__restrict__ candPartner,
+                                                            int *__restrict__ candCount, int maxCand) {
+    static_assert(64 % G == 0, "groups tile a wave");
     // XCD-aware: each XCD walks one contiguous run of the cell-sorted bodies (a spatial slab), so
     // its L2 holds that slab's cell offsets and centres instead of the whole grid
-    int t = xcd_block(blockIdx.x, gridDim.x) * BP_THREADS + threadIdx.x;
-    if (t >= cap) return;
-    if (t >= nSorted) {  // padding slots [S, cap): count cleared, nothing emitted
-        candCount[t] = 0;
+    const int lane = threadIdx.x & 63, gl = lane % G, grp = lane / G;
+    const int wave = (xcd_block(blockIdx.x, gridDim.x) * BP_THREADS + threadIdx.x) >> 6;
+    const int t = wave * (64 / G) + grp;
+    if (t >= cap) return;  // whole groups leave together (t is uniform in a group)
+    if (t >= nSorted) {    // padding slots [S, cap): count cleared, nothing emitted
+        if (gl == 0) candCount[t] = 0;
         return;
     }
-    __shared__ int2 segLds[BP_THREADS * 18];
-    int i = cellContents[t];  // cell-sorted order: neighbouring lanes query neighbouring cells
-    int c = bodyCell[i];
-    if (c < 0) { candCount[i] = 0; return; }
-    candCount[i] = broad_phase_body(i, c, center, boundingRadius, cellOffsets, cellContents, g, cap, candPartner,
-                                    maxCand, segLds);
+    const int i = cellContents[t];
+    const int c = pose.cell(t, i);
+    if (c < 0) {
+        if (gl == 0) candCount[i] = 0;
+        return;
+    }
+    const float4 qi = pose.own(t, i);
+    int pre[18], shift[18];
+    const int total = stencil_map(c, g, cellOffsets, pre, shift);
+    int *slice = candPartner + (size_t)i * (size_t)maxCand;
+    const unsigned long long groupBits = G == 64 ? ~0ull : ((1ull << G) - 1) << (grp * G);
+    const unsigned long long below = (1ull << lane) - 1;
+    int cnt = 0;
+    for (int base = 0; base < total; base += G * U) {
+        int at[U], jj[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int q = base + k * G + gl;
+            at[k] = q < total ? stencil_at(pre, shift, q) : t;
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) jj[k] = (base + k * G + gl < total) ? cellContents[at[k]] : -1;
+        float4 qj[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) qj[k] = pose.at(jj[k] > i ? at[k] : t, jj[k] > i ? jj[k] : i);
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const bool hit = jj[k] > i && bp_hit(qi, qj[k], g);
+            const unsigned long long m = __ballot(hit) & groupBits;
+            if (hit) {
+                const int slot = cnt + __popcll(m & below);
+                if (slot < maxCand) slice[slot] = jj[k];
+            }
+            cnt += __popcll(m);
+        }
+    }
+    if (gl == 0) candCount[i] = cnt;
 }
 
 GridGeom grid_geom(const float *gridParams, const int *gridDims, int periodic, const float *boxLen) {
@@ -97,10 +176,22 @@ int launch_broad_phase(sbx_ctx *c, const float *center, const float *boundingRad
                        const int *cellOffsets, const int *cellContents, const GridGeom &g, int cap, int S,
                        int *candPartner, int *candCount, int maxCand, hipStream_t st) {
     // bodies with a cell are exactly the first S (counting sort places each live body once)
-    hipLaunchKernelGGL(k_broad_phase, dim3(blocks_for(cap, BP_THREADS)), dim3(BP_THREADS), 0, st ? st : c->stream,
-                       center,
-                       boundingRadius,
-                       bodyCell, cellOffsets, cellContents, g, cap, S, S, candPartner, candCount, maxCand);
+    PlanarPose pose{center, boundingRadius, bodyCell, cap};
+    auto kern = k_broad_phase<PlanarPose, BP_GROUP, BP_ROUND>;
+    hipLaunchKernelGGL(kern, dim3(blocks_for((long long)cap * BP_GROUP, BP_THREADS)), dim3(BP_THREADS), 0,
+                       st ? st : c->stream, pose, cellOffsets, cellContents, g, cap, S, candPartner, candCount,
+                       maxCand);
+    return check_launch(c, "k_broad_phase");
+}
+
+int launch_broad_phase_sorted(sbx_ctx *c, const float4 *sortedPose, const int *sortedCell, const int *cellOffsets,
+                              const int *cellContents, const GridGeom &g, int cap, int S, int *candPartner,
+                              int *candCount, int maxCand, hipStream_t st) {
+    SortedPose pose{sortedPose, sortedCell};
+    auto kern = k_broad_phase<SortedPose, BP_GROUP, BP_ROUND>;
+    hipLaunchKernelGGL(kern, dim3(blocks_for((long long)cap * BP_GROUP, BP_THREADS)), dim3(BP_THREADS), 0,
+                       st ? st : c->stream, pose, cellOffsets, cellContents, g, cap, S, candPartner, candCount,
+                       maxCand);
     return check_launch(c, "k_broad_phase");
 }
 

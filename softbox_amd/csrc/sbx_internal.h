@@ -57,8 +57,14 @@ void sort_scratch_free(SortScratch &s);
 // list[...]; offsets[nKeys] is the total clamped to contentsCap; positions >= contentsCap are dropped.
 // gate (device int, may be null): when it reads 0 every kernel of the sort returns at once (a
 // rebuild the device decides on, inside a captured step).  st: stream (null: the context's).
+// pay: optional per-item data written in list order (out[pos] = in[list[pos]], keyOut[pos] = key).
+struct SortPayload {
+    const float4 *in = nullptr;
+    float4 *out = nullptr;
+    int *keyOut = nullptr;
+};
 int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
-                  int *list, const int *gate = nullptr, hipStream_t st = nullptr);
+                  int *list, const int *gate = nullptr, hipStream_t st = nullptr, SortPayload pay = SortPayload());
 int fill_i32(sbx_ctx *c, int *dst, int value, long long count);
 
 ChainParams chain_params(const float *cp);

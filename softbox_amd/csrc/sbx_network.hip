@@ -24,6 +24,7 @@
 #include "sbx_mlp.h"
 #include "sbx_motor.h"
 #include "sbx_skin.h"
+#include "sbx_tiles.h"
 
 using namespace sbx;
 
@@ -95,10 +96,12 @@ struct sbx_net {
     int *vlist = nullptr, *vcount = nullptr;
     int *skinFlags = nullptr;
     float *model = nullptr;                 // MLP_NPARAM device floats (learned mode)
-    float4 *skinPackB[2] = {nullptr, nullptr};  // {centre, bounding radius} per pose buffer (by k_rod)
+    float4 *bodyPack[2] = {nullptr, nullptr};  // {centre, bounding radius} per pose buffer (by k_rod; grid on)
     // broad phase on a second stream, overlapping the mechanics kernels (it feeds nothing in the step)
     hipStream_t side = nullptr;
-    hipEvent_t evFork = nullptr, evJoin = nullptr, evBuilt = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr, evBuilt = nullptr, evPlaced = nullptr;
+    float4 *sortedPose = nullptr;  // bodyPack in contents order (written by the grid sort's place pass)
+    int *sortedCell = nullptr;     // bodyCell in contents order
     int overlap = 1;
     // graph
     int useGraph = 0;
@@ -136,7 +139,7 @@ struct RodArgs {
     float4 *linkPack;       // packed endpoint data of the new pose for the next step's links (or null)
     const float *skinRef, *skin;  // skin guard: centre at the last rebuild and per-body skin (or null)
     int *rebuildFlag;             // skin rebuild request word of the new pose's buffer
-    float4 *skinPack;             // {new centre, bounding radius} for the next step's skin kernels
+    float4 *bodyPack;             // {new centre, bounding radius} for the next step's broad phase (or null)
     const float *boundR;
     GridGeom geom;
     const int *dStep;
@@ -241,8 +244,8 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         }
         float sk = a.skin[i];
         if (dx * dx + dy * dy + dz * dz > sk * sk) *a.rebuildFlag = 1;
-        a.skinPack[i] = make_float4(p.c[0], p.c[1], p.c[2], a.boundR[i]);
     }
+    if (a.bodyPack) a.bodyPack[i] = make_float4(p.c[0], p.c[1], p.c[2], a.boundR[i]);
     if (a.linkPack) {
         float4 *q = a.linkPack + LINK_PACK * i;
         q[0] = make_float4(p.c[0], p.c[1], p.c[2], gT[0]);
@@ -251,8 +254,8 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
     }
 }
 
-// skinPack of the caller's pose (before the first step and after sbx_net_publish)
-__global__ void __launch_bounds__(256) k_pack_skin(const float *__restrict__ coord, const float *__restrict__ boundR,
+// bodyPack of the caller's pose (before the first step and after sbx_net_publish)
+__global__ void __launch_bounds__(256) k_pack_bodies(const float *__restrict__ coord, const float *__restrict__ boundR,
                                                    float4 *__restrict__ pack, int n) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) pack[i] = make_float4(coord[i], coord[n + i], coord[2 * n + i], boundR[i]);
@@ -592,7 +595,7 @@ __global__ void __launch_bounds__(256) k_motor_integ(sbx_motors m, float dt, flo
 struct SkinArgs {
     int n, mode, listCap, maxCand, cCellsMax;
     int gate;                // flag word of this pose's rebuild request (skf_rebuild(pose buffer))
-    int blockKeys;           // keys k_skin_pairs buffers for one heavy body (<= PT_BODIES * PT_KEYS)
+    int blockKeys;           // keys k_skin_pairs buffers for one heavy body (<= TILE_BODIES * TILE_KEYS)
     const float *coord, *uVec, *boundR;
     const int *links;        // activeLinkCount (may be null)
     GridGeom geom;           // fine grid (the broad phase's)
@@ -688,7 +691,6 @@ __global__ void __launch_bounds__(256) k_skin_bin(SkinArgs a) {
 // coarse cells around i.  Also records the pose the guard measures displacement from.
 __global__ void __launch_bounds__(BP_THREADS) k_skin_build(SkinArgs a) {
     if (a.flags[a.gate] == 0) return;
-    __shared__ int2 segLds[BP_THREADS * 18];
     int t = xcd_block(blockIdx.x, gridDim.x) * BP_THREADS + threadIdx.x;
     const int n = a.n;
     if (t >= n) return;
@@ -697,15 +699,14 @@ __global__ void __launch_bounds__(BP_THREADS) k_skin_build(SkinArgs a) {
     const float4 qi = a.pack[i];
     const float icx = qi.x, icy = qi.y, icz = qi.z;
     float base = qi.w + a.geom.cutoff + a.skin[i] + SKIN_SLACK;
-    int2 *seg = segLds + threadIdx.x * 18;
-    const int total = stencil_ranges(a.cCell[i], g, a.cOff, seg);
-    RangeWalk w(seg);
+    int pre[18], shift[18];
+    const int total = stencil_map(a.cCell[i], g, a.cOff, pre, shift);
     int cnt = 0;
     int *list = a.vlist + (size_t)t * a.listCap;  // per coarse slot: k_skin_pairs reads a tile's lists in one sweep
     for (int q = 0; q < total; q += BP_BATCH) {
         int jj[BP_BATCH];
 #pragma unroll
-        for (int u = 0; u < BP_BATCH; u++) jj[u] = (q + u < total) ? a.cCont[w.next()] : -1;
+        for (int u = 0; u < BP_BATCH; u++) jj[u] = (q + u < total) ? a.cCont[stencil_at(pre, shift, q + u)] : -1;
         float px[BP_BATCH], py[BP_BATCH], pz[BP_BATCH], pr[BP_BATCH];
 #pragma unroll
         for (int u = 0; u < BP_BATCH; u++) {
@@ -766,14 +767,6 @@ __device__ __forceinline__ int skin_key(int j, int3 ci, float icx, float icy, fl
     return (pos << 26) | j;
 }
 
-constexpr int PT_BODIES = 64;    // coarse slots per k_skin_pairs block
-constexpr int PT_THREADS = 256;  // k_skin_pairs block
-constexpr int PT_KEYS = 64;      // candidates a block buffers per body (dense bundles: p99.9 ~40)
-constexpr int PT_UNROLL = 4;     // list entries in flight per thread
-constexpr int BIG_KEEP = SBX_MAX_CAND;  // keys kept for a body with more than PT_BODIES * PT_KEYS (maxCand <= this)
-static_assert(BIG_KEEP >= PT_THREADS && (BIG_KEEP & (BIG_KEEP - 1)) == 0 && 2 * BIG_KEEP <= PT_BODIES * PT_KEYS,
-              "bitonic halves in the key buffer");
-static_assert(PT_BODIES == 64, "the slot prefix sum is one wave's scan");
 
 // The 27 coarse cells (of the last rebuild) around body i's coarse cell as contents ranges: thread
 // r < 27 stores range r's start in segLo[r] and its length in segStart[r + 1]; the caller sets
@@ -810,58 +803,17 @@ __device__ __forceinline__ int coarse_entry(const SkinArgs &a, int k, const int 
     return a.cCont[segLo[r] + (k - segStart[r])];
 }
 
-// Keeps the BIG_KEEP smallest keys of body i's candidates (list entries, or the concatenated coarse
-// ranges when over) in buf[0, BIG_KEEP), ascending, padded with INT_MAX: the candidates are tested
-// in chunks of PT_THREADS and (kept | chunk) is bitonic-sorted in LDS after each chunk.  For bodies
-// with more candidates than the block buffers; called by the whole block.
-__device__ void block_keep_smallest(const SkinArgs &a, int t, int i, bool over, int total, int3 ci, float4 qi,
-                                    const int *segLo, const int *segStart, int *buf, int *chunkN) {
-    const int tid = threadIdx.x;
-    for (int x = tid; x < BIG_KEEP; x += PT_THREADS) buf[x] = 0x7fffffff;
-    for (int base = 0; base < total; base += PT_THREADS) {
-        __syncthreads();  // the previous chunk's sort and every read of *chunkN are done
-        if (tid == 0) *chunkN = 0;
-        for (int x = tid; x < BIG_KEEP; x += PT_THREADS) buf[BIG_KEEP + x] = 0x7fffffff;
-        __syncthreads();
-        const int k = base + tid;
-        if (k < total) {
-            int j = over ? coarse_entry(a, k, segLo, segStart) : a.vlist[(size_t)t * a.listCap + k];
-            if (j > i) {
-                int key = skin_key(j, ci, qi.x, qi.y, qi.z, qi.w, a.pack[j], a.geom);
-                if (key >= 0) buf[BIG_KEEP + atomicAdd(chunkN, 1)] = key;  // <= PT_THREADS per chunk
-            }
-        }
-        __syncthreads();
-        if (*chunkN == 0) continue;
-        for (int size = 2; size <= 2 * BIG_KEEP; size <<= 1) {
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int p = tid; p < BIG_KEEP; p += PT_THREADS) {
-                    int lo = ((p & ~(stride - 1)) << 1) | (p & (stride - 1));
-                    int hi = lo + stride;
-                    int x = buf[lo], y = buf[hi];
-                    if ((x > y) == ((lo & size) == 0)) {
-                        buf[lo] = y;
-                        buf[hi] = x;
-                    }
-                }
-                __syncthreads();
-            }
-        }
-    }
-    __syncthreads();
-}
-
 // Candidate slices of the current pose from the neighbour lists, in the broad phase's order.  A
-// block takes PT_BODIES consecutive coarse slots and walks the concatenation of their lists with
+// block takes TILE_BODIES consecutive coarse slots and walks the concatenation of their lists with
 // all its threads (so a long list does not hold back a wave), collects each body's keys in LDS and
 // places each key at its rank among its body's keys (keys are distinct).  A body with more than
-// PT_KEYS candidates, or whose list overflowed at the rebuild, is then done by the whole block.
-__global__ void __launch_bounds__(PT_THREADS) k_skin_pairs(SkinArgs a) {
-    __shared__ int keyS[PT_BODIES * PT_KEYS];
-    __shared__ float4 qS[PT_BODIES];
-    __shared__ int4 cS[PT_BODIES];     // fine cell (x, y, z) and body index
-    __shared__ int startS[PT_BODIES + 1], mS[PT_BODIES];
-    __shared__ int heavyS[PT_BODIES], heavyN, foundS;
+// TILE_KEYS candidates, or whose list overflowed at the rebuild, is then done by the whole block.
+__global__ void __launch_bounds__(TILE_THREADS) k_skin_pairs(SkinArgs a) {
+    __shared__ int keyS[TILE_BODIES * TILE_KEYS];
+    __shared__ float4 qS[TILE_BODIES];
+    __shared__ int4 cS[TILE_BODIES];     // fine cell (x, y, z) and body index
+    __shared__ int startS[TILE_BODIES + 1], mS[TILE_BODIES];
+    __shared__ int heavyS[TILE_BODIES], heavyN, foundS;
     __shared__ int segLo[27], segStart[28];
     const int tid = threadIdx.x;
     if (blockIdx.x == 0 && tid == 0) {  // no other thread reads these words in this kernel
@@ -872,8 +824,8 @@ __global__ void __launch_bounds__(PT_THREADS) k_skin_pairs(SkinArgs a) {
     }
     const int n = a.n;
     const GridGeom &g = a.geom;
-    const int t0 = xcd_block(blockIdx.x, gridDim.x) * PT_BODIES;
-    if (tid < PT_BODIES) {  // wave 0: the tile's bodies and the prefix sum of their list lengths
+    const int t0 = xcd_block(blockIdx.x, gridDim.x) * TILE_BODIES;
+    if (tid < TILE_BODIES) {  // wave 0: the tile's bodies and the prefix sum of their list lengths
         const int t = t0 + tid;
         int cnt = 0;
         bool over = false;
@@ -889,69 +841,63 @@ __global__ void __launch_bounds__(PT_THREADS) k_skin_pairs(SkinArgs a) {
                 over = true;
             }
         }
-        mS[tid] = over ? PT_KEYS + 1 : 0;  // an overflowed body counts as heavy
+        mS[tid] = over ? TILE_KEYS + 1 : 0;  // an overflowed body counts as heavy
         if (tid == 0) heavyN = 0;
-        int incl = cnt;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            int v = __shfl_up(incl, d, 64);
-            if (tid >= d) incl += v;
-        }
-        startS[tid + 1] = incl;
+        startS[tid + 1] = wave_inclusive_scan(cnt, tid);
         if (tid == 0) startS[0] = 0;
     }
     __syncthreads();
-    const int total = startS[PT_BODIES];
-    for (int e0 = 0; e0 < total; e0 += PT_THREADS * PT_UNROLL) {
-        int bb[PT_UNROLL], jj[PT_UNROLL];
+    const int total = startS[TILE_BODIES];
+    // runs of TILE_CHUNK consecutive entries per thread: one search for the run's first body, a
+    // forward walk for the rest
+    for (int c0 = tid * TILE_CHUNK; c0 < total; c0 += TILE_THREADS * TILE_CHUNK) {
+        int bb[TILE_CHUNK], jj[TILE_CHUNK];
+        int b = tile_body_of(startS, c0);
 #pragma unroll
-        for (int u = 0; u < PT_UNROLL; u++) {
-            const int e = e0 + u * PT_THREADS + tid;
+        for (int u = 0; u < TILE_CHUNK; u++) {
+            const int e = c0 + u;
             bb[u] = -1;
             jj[u] = 0;
             if (e < total) {
-                int lo = 0;  // last body whose list starts at or before e
-#pragma unroll
-                for (int step = PT_BODIES / 2; step > 0; step >>= 1)
-                    if (startS[lo + step] <= e) lo += step;
-                bb[u] = lo;
-                jj[u] = a.vlist[(size_t)(t0 + lo) * a.listCap + (e - startS[lo])];
+                while (startS[b + 1] <= e) b++;  // next body with entries left (startS[64] = total)
+                bb[u] = b;
+                jj[u] = a.vlist[(size_t)(t0 + b) * a.listCap + (e - startS[b])];
             }
         }
-        float4 qj[PT_UNROLL];
+        float4 qj[TILE_CHUNK];
 #pragma unroll
-        for (int u = 0; u < PT_UNROLL; u++) qj[u] = a.pack[jj[u]];
+        for (int u = 0; u < TILE_CHUNK; u++) qj[u] = a.pack[jj[u]];
 #pragma unroll
-        for (int u = 0; u < PT_UNROLL; u++) {
+        for (int u = 0; u < TILE_CHUNK; u++) {
             if (bb[u] < 0) continue;
             const float4 qi = qS[bb[u]];
             const int4 ci = cS[bb[u]];
             int key = skin_key(jj[u], make_int3(ci.x, ci.y, ci.z), qi.x, qi.y, qi.z, qi.w, qj[u], g);
             if (key >= 0) {
                 int k = atomicAdd(&mS[bb[u]], 1);
-                if (k < PT_KEYS) keyS[bb[u] * PT_KEYS + k] = key;
+                if (k < TILE_KEYS) keyS[bb[u] * TILE_KEYS + k] = key;
             }
         }
     }
     __syncthreads();
     // rank placement: key (b, k) goes to slot #{keys of b smaller than it}
-    for (int p = tid; p < PT_BODIES * PT_KEYS; p += PT_THREADS) {
-        const int b = p / PT_KEYS, k = p - b * PT_KEYS;
+    for (int p = tid; p < TILE_BODIES * TILE_KEYS; p += TILE_THREADS) {
+        const int b = p / TILE_KEYS, k = p - b * TILE_KEYS;
         const int m = mS[b];
-        if (k >= m || m > PT_KEYS) continue;
+        if (k >= m || m > TILE_KEYS) continue;
         const int key = keyS[p];
-        const int *kb = keyS + b * PT_KEYS;
+        const int *kb = keyS + b * TILE_KEYS;
         int rank = 0;
         for (int x = 0; x < m; x++) rank += kb[x] < key ? 1 : 0;
-        if (rank < a.maxCand) a.candPartner[(size_t)cS[b].w * a.maxCand + rank] = key & ((1 << 26) - 1);
+        if (rank < a.maxCand) a.candPartner[(size_t)cS[b].w * a.maxCand + rank] = key & KEY_INDEX_MASK;
     }
-    if (tid < PT_BODIES && t0 + tid < n) {
+    if (tid < TILE_BODIES && t0 + tid < n) {
         const int m = mS[tid];
-        if (m <= PT_KEYS) a.candCount[cS[tid].w] = m;
+        if (m <= TILE_KEYS) a.candCount[cS[tid].w] = m;
         else heavyS[atomicAdd(&heavyN, 1)] = tid;
     }
     __syncthreads();
-    // heavy bodies (more than PT_KEYS candidates, or an overflowed list), one at a time with the
+    // heavy bodies (more than TILE_KEYS candidates, or an overflowed list), one at a time with the
     // whole block: keys gathered into the whole key buffer and rank-placed; more than that many
     // keys goes to k_skin_pairs_big
     const int nHeavy = heavyN;
@@ -969,10 +915,13 @@ __global__ void __launch_bounds__(PT_THREADS) k_skin_pairs(SkinArgs a) {
         const int total = over ? segStart[27] : cnt;
         const float4 qi = qS[b];
         const int3 ci = make_int3(cS[b].x, cS[b].y, cS[b].z);
-        for (int k = tid; k < total; k += PT_THREADS) {
+        auto keyAt = [&](int k) -> int {
             int j = over ? coarse_entry(a, k, segLo, segStart) : a.vlist[(size_t)t * a.listCap + k];
-            if (j <= i) continue;
-            int key = skin_key(j, ci, qi.x, qi.y, qi.z, qi.w, a.pack[j], g);
+            if (j <= i) return -1;
+            return skin_key(j, ci, qi.x, qi.y, qi.z, qi.w, a.pack[j], g);
+        };
+        for (int k = tid; k < total; k += TILE_THREADS) {
+            const int key = keyAt(k);
             if (key >= 0) {
                 int p = atomicAdd(&foundS, 1);
                 if (p < a.blockKeys) keyS[p] = key;
@@ -980,20 +929,15 @@ __global__ void __launch_bounds__(PT_THREADS) k_skin_pairs(SkinArgs a) {
         }
         __syncthreads();
         const int m = foundS;
+        int *slice = a.candPartner + (size_t)i * a.maxCand;
         if (m <= a.blockKeys) {
-            for (int p = tid; p < m; p += PT_THREADS) {
-                const int key = keyS[p];
-                int rank = 0;
-                for (int x = 0; x < m; x++) rank += keyS[x] < key ? 1 : 0;
-                if (rank < a.maxCand) a.candPartner[(size_t)i * a.maxCand + rank] = key & ((1 << 26) - 1);
-            }
-            if (tid == 0) a.candCount[i] = m;
+            block_rank_place(keyS, m, a.maxCand, [&](int rank, int key) { slice[rank] = key & KEY_INDEX_MASK; });
         } else {  // more keys than the buffer: keep the maxCand smallest over chunks (exact for any number)
-            block_keep_smallest(a, t, i, over, total, ci, qi, segLo, segStart, keyS, &foundS);
+            block_keep_smallest(total, keyAt, keyS, &foundS);
             const int w = m < a.maxCand ? m : a.maxCand;
-            for (int x = tid; x < w; x += PT_THREADS) a.candPartner[(size_t)i * a.maxCand + x] = keyS[x] & ((1 << 26) - 1);
-            if (tid == 0) a.candCount[i] = m;
+            for (int x = tid; x < w; x += TILE_THREADS) slice[x] = keyS[x] & KEY_INDEX_MASK;
         }
+        if (tid == 0) a.candCount[i] = m;
         if (tid == 0) atomicAdd(&a.flags[SKF_HEAVY], 1);
         __syncthreads();  // before the next heavy body reuses the buffers
     }
@@ -1126,8 +1070,8 @@ SkinArgs skin_args(sbx_net *net, const float *coord, const float *uVec) {
     SkinArgs a;
     a.n = net->n; a.mode = net->skinMode; a.listCap = net->sp.listCap; a.maxCand = net->p.maxCand;
     a.gate = skf_rebuild(net->cur);
-    a.blockKeys = net->sp.blockKeys > 0 && net->sp.blockKeys < PT_BODIES * PT_KEYS ? net->sp.blockKeys
-                                                                                  : PT_BODIES * PT_KEYS;
+    a.blockKeys = net->sp.blockKeys > 0 && net->sp.blockKeys < TILE_BODIES * TILE_KEYS ? net->sp.blockKeys
+                                                                                  : TILE_BODIES * TILE_KEYS;
     a.cCellsMax = net->cCellsMax;
     a.coord = coord; a.uVec = uVec; a.boundR = net->boundR;
     a.links = net->L > 0 ? net->xl.activeLinkCount : nullptr;
@@ -1141,7 +1085,7 @@ SkinArgs skin_args(sbx_net *net, const float *coord, const float *uVec) {
     a.margin = net->sp.margin;
     a.maxR = net->maxR; a.cellFloor = net->cellFloor;
     a.bodyCell = net->bodyCell;
-    a.pack = net->skinPackB[net->cur];
+    a.pack = net->bodyPack[net->cur];
     a.candPartner = net->candPartner; a.candCount = net->candCount;
     return a;
 }
@@ -1169,7 +1113,7 @@ int enqueue_skin(sbx_net *net, const float *coord, const float *uVec, hipStream_
     }
     {
         Scope sc(net, T_SKIN_PAIRS);
-        hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(n, PT_BODIES)), dim3(PT_THREADS), 0, s, a);
+        hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(n, TILE_BODIES)), dim3(TILE_THREADS), 0, s, a);
     }
     return check_launch(c, "k_skin_pairs");
 }
@@ -1183,12 +1127,14 @@ bool overlapped(const sbx_net *net) {
 // One step's launches; stepOff is added to the device step counter.  With overlap the broad phase
 // (its output is the step's candidate slices, read by no later kernel of the step) runs on the side
 // stream, forked after the previous step's k_rod:
-//   side: [sort + broad phase | gated skin rebuild -> evBuilt -> pairs] -> evJoin
-//   main: unbind -> link -> (wait: evJoin of the previous step | evBuilt) -> k_rod
-// k_rod overwrites the pose buffers, cells and skin pack the previous step's broad phase read, and
-// its skin guard reads this step's rebuild; the next fork orders the next broad phase after k_rod.
-// firstInCapture: no wait on an evJoin recorded outside the graph being captured (the caller joined
-// the side stream before the capture began).
+//   side: [sort (place also lays out pose + cell in contents order) -> evPlaced -> broad phase |
+//          gated skin rebuild -> evBuilt -> pairs] -> evJoin
+//   main: (wait evPlaced of the previous step) unbind -> link -> (wait evBuilt) -> k_rod
+// k_rod overwrites the packed pose and cells the previous step's place pass read (the broad phase
+// itself reads only the sorted copies), and its skin guard reads this step's rebuild (skin mode:
+// whose pairs read the packed pose k_rod overwrites a step later); the next fork orders the next
+// broad phase after k_rod.  firstInCapture: no wait on an event recorded outside the graph being
+// captured (the caller joined the side stream before the capture began).
 int enqueue_step(sbx_net *net, int stepOff, bool firstInCapture = false) {
     sbx_ctx *c = net->ctx;
     hipStream_t s = c->stream;
@@ -1205,20 +1151,29 @@ int enqueue_step(sbx_net *net, int stepOff, bool firstInCapture = false) {
         if (ovl) {
             if ((r = check_hip(c, hipEventRecord(net->evFork, s), "record evFork"))) return r;
             if ((r = check_hip(c, hipStreamWaitEvent(s2, net->evFork, 0), "wait evFork"))) return r;
+            if (!skinOn && !firstInCapture &&
+                (r = check_hip(c, hipStreamWaitEvent(s, net->evPlaced, 0), "wait evPlaced")))
+                return r;
         }
         if (skinOn) {
             if ((r = enqueue_skin(net, coord, uVec, s2, ovl))) return r;
         } else {
             {
                 Scope sc(net, T_SORT);
+                SortPayload pay;
+                pay.in = net->bodyPack[net->cur];
+                pay.out = net->sortedPose;
+                pay.keyOut = net->sortedCell;
                 r = counting_sort(c, net->sort, net->bodyCell, n, net->p.gridDims[3], n, net->cellOffsets,
-                                  net->cellContents, nullptr, s2);
+                                  net->cellContents, nullptr, s2, pay);
                 if (r) return r;
+                if (ovl && (r = check_hip(c, hipEventRecord(net->evPlaced, s2), "record evPlaced"))) return r;
             }
             {
                 Scope sc(net, T_PAIRS);
-                r = launch_broad_phase(c, coord, net->boundR, net->bodyCell, net->cellOffsets, net->cellContents,
-                                       net->geom, n, n, net->candPartner, net->candCount, net->p.maxCand, s2);
+                r = launch_broad_phase_sorted(c, net->sortedPose, net->sortedCell, net->cellOffsets,
+                                              net->cellContents, net->geom, n, n, net->candPartner, net->candCount,
+                                              net->p.maxCand, s2);
                 if (r) return r;
             }
         }
@@ -1262,7 +1217,7 @@ int enqueue_step(sbx_net *net, int stepOff, bool firstInCapture = false) {
     a.skinRef = skinOn ? net->refC : nullptr;
     a.skin = skinOn ? net->skin : nullptr;
     a.rebuildFlag = skinOn ? net->skinFlags + skf_rebuild(nxt) : nullptr;
-    a.skinPack = net->skinPackB[nxt];
+    a.bodyPack = (net->p.gridOn && !net->hasMotors) ? net->bodyPack[nxt] : nullptr;
     a.boundR = net->boundR;
     a.mOff = net->mOff; a.mList = net->mList; a.bondData = net->mo.bondData;
     if (net->hasMotors) {
@@ -1280,13 +1235,7 @@ int enqueue_step(sbx_net *net, int stepOff, bool firstInCapture = false) {
     a.halfY = 0.5 * net->p.boxParams[2] - net->p.boxParams[4];
     a.halfZ = 0.5 * net->p.boxParams[3] - net->p.boxParams[4];
     a.coeff = net->p.boxParams[5];
-    if (ovl) {
-        if (skinOn)
-            r = check_hip(c, hipStreamWaitEvent(s, net->evBuilt, 0), "wait evBuilt");
-        else if (!firstInCapture)
-            r = check_hip(c, hipStreamWaitEvent(s, net->evJoin, 0), "wait evJoin");
-        if (r) return r;
-    }
+    if (ovl && skinOn && (r = check_hip(c, hipStreamWaitEvent(s, net->evBuilt, 0), "wait evBuilt"))) return r;
     {
         Scope sc(net, T_ROD);
         hipLaunchKernelGGL(k_rod, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, net->cp);
@@ -1323,7 +1272,8 @@ void release(sbx_net *net) {
                     net->mOff,     net->mList,     net->bbOff,    net->bbList,
                     net->cCell,    net->cOff,      net->cCont,    net->refC,    net->refU, net->skin,
                     net->cellSkin, net->cellHorizon, net->vlist,  net->vcount,  net->skinFlags, net->model,
-                    net->cgDev,    net->cCellsDev, net->skinPackB[0], net->skinPackB[1]};
+                    net->cgDev,    net->cCellsDev, net->bodyPack[0], net->bodyPack[1], net->sortedPose,
+                    net->sortedCell};
     for (void *b : bufs)
         if (b) hipFree(b);
     sort_scratch_free(net->sort);
@@ -1332,7 +1282,7 @@ void release(sbx_net *net) {
         hipEventDestroy(pe.second.second);
     }
     for (hipEvent_t e : net->eventPool) hipEventDestroy(e);
-    for (hipEvent_t e : {net->evFork, net->evJoin, net->evBuilt})
+    for (hipEvent_t e : {net->evFork, net->evJoin, net->evBuilt, net->evPlaced})
         if (e) hipEventDestroy(e);
     if (net->side) hipStreamDestroy(net->side);
     delete net;
@@ -1377,7 +1327,8 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
     if (p->gridOn) {
         int r = check_grid(c, p->gridParams, p->gridDims, p->periodic);
         if (r) return r;
-        SBX_REQUIRE(c, p->maxCand > 0, "net_create: maxCand must be > 0");
+        SBX_REQUIRE(c, p->maxCand > 0 && p->maxCand <= SBX_MAX_CAND, "net_create: maxCand must be in [1, %d]",
+                    SBX_MAX_CAND);
         SBX_REQUIRE(c, (long long)n * p->maxCand < (1LL << 33), "net_create: candidate slices too large");
     }
     int r = set_device(c);
@@ -1410,11 +1361,16 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
             if ((r = alloc(c, &net->cellContents, n))) break;
             if ((r = alloc(c, &net->candPartner, (size_t)n * p->maxCand))) break;
             if ((r = alloc(c, &net->candCount, n))) break;
+            if ((r = alloc(c, &net->bodyPack[0], n))) break;
+            if ((r = alloc(c, &net->bodyPack[1], n))) break;
             // default priority, like the context's stream
             if ((r = check_hip(c, hipStreamCreateWithFlags(&net->side, hipStreamNonBlocking), "side stream"))) break;
             if ((r = check_hip(c, hipEventCreateWithFlags(&net->evFork, hipEventDisableTiming), "event"))) break;
             if ((r = check_hip(c, hipEventCreateWithFlags(&net->evJoin, hipEventDisableTiming), "event"))) break;
             if ((r = check_hip(c, hipEventCreateWithFlags(&net->evBuilt, hipEventDisableTiming), "event"))) break;
+            if ((r = check_hip(c, hipEventCreateWithFlags(&net->evPlaced, hipEventDisableTiming), "event"))) break;
+            if ((r = alloc(c, &net->sortedPose, n))) break;
+            if ((r = alloc(c, &net->sortedCell, n))) break;
             hipLaunchKernelGGL(k_bounding_radius, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, fil->segLength,
                                p->actinRadius, net->boundR, n);
             if ((r = check_launch(c, "k_bounding_radius"))) break;
@@ -1563,10 +1519,10 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
         if (net->p.gridOn && !net->hasMotors &&
             (r = launch_body_cell(c, net->coordB[net->cur], net->n, net->n, net->geom, net->bodyCell)))
             return r;
-        if (net->skinMode != SBX_SKIN_REBUILD && !net->hasMotors) {
-            hipLaunchKernelGGL(k_pack_skin, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream,
-                               net->coordB[net->cur], net->boundR, net->skinPackB[net->cur], net->n);
-            if ((r = check_launch(c, "k_pack_skin"))) return r;
+        if (net->p.gridOn && !net->hasMotors) {
+            hipLaunchKernelGGL(k_pack_bodies, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream,
+                               net->coordB[net->cur], net->boundR, net->bodyPack[net->cur], net->n);
+            if ((r = check_launch(c, "k_pack_bodies"))) return r;
         }
         if (net->L > 0) {
             hipLaunchKernelGGL(k_pack_links, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream,
@@ -1690,7 +1646,6 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
                     "net_set_skin: need 0 < skinMin <= skinDefault <= skinMax, margin > 0, listCap > 0, blockKeys >= 0");
         SBX_REQUIRE(c, mode != SBX_SKIN_LEARNED || modelWeights, "net_set_skin: learned mode needs model weights");
         SBX_REQUIRE(c, net->n < (1 << 26), "net_set_skin: more than 2^26 bodies");
-        SBX_REQUIRE(c, net->p.maxCand <= BIG_KEEP, "net_set_skin: maxCand %d above %d", net->p.maxCand, BIG_KEEP);
     }
     int r = set_device(c);
     if (r) return r;
@@ -1743,13 +1698,12 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
     // (re)allocate the skin state
     void *old[] = {net->cCell, net->cOff,   net->cCont,     net->refC,  net->refU,      net->skin,
                    net->cellSkin, net->cellHorizon, net->vlist, net->vcount, net->skinFlags, net->model,
-                   net->cgDev,  net->cCellsDev, net->skinPackB[0], net->skinPackB[1]};
+                   net->cgDev,  net->cCellsDev};
     for (void *b : old)
         if (b) hipFree(b);
     net->cCell = net->cOff = net->cCont = net->vlist = net->vcount = net->skinFlags = net->cCellsDev = nullptr;
     net->refC = net->refU = net->skin = net->cellSkin = net->cellHorizon = net->model = nullptr;
     net->cgDev = nullptr;
-    net->skinPackB[0] = net->skinPackB[1] = nullptr;
     do {
         if ((r = alloc(c, &net->cCell, n))) break;
         if ((r = alloc(c, &net->cOff, (size_t)cc + 1))) break;
@@ -1763,8 +1717,6 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
         if ((r = alloc(c, &net->vcount, n))) break;
         if ((r = alloc(c, &net->skinFlags, SKF_WORDS))) break;
         if ((r = alloc(c, &net->cgDev, 1))) break;
-        if ((r = alloc(c, &net->skinPackB[0], n))) break;
-        if ((r = alloc(c, &net->skinPackB[1], n))) break;
         if ((r = alloc(c, &net->cCellsDev, 1))) break;
         if ((r = fill_i32(c, net->cCellsDev, 0, 1))) break;
         if (mode == SBX_SKIN_LEARNED) {
