@@ -279,6 +279,32 @@ __global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, 
     }
 }
 
+int count_keys(sbx_ctx *c, const int *key, int nItems, int nKeys, int *counts, int *slot, hipStream_t st) {
+    if (nItems <= 0) return SBX_OK;
+    hipLaunchKernelGGL(k_key_count, dim3(blocks_for(nItems, 256)), dim3(256), 0, st ? st : c->stream, key, nItems,
+                       nKeys, counts, slot, nullptr);
+    return check_launch(c, "k_key_count");
+}
+
+int counting_sort_counted(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap,
+                          int *offsets, int *list, int *counts, const int *slot, hipStream_t st, SortPayload pay) {
+    int r = sort_scratch_reserve(c, s, nKeys, nItems);
+    if (r) return r;
+    hipStream_t q = st ? st : c->stream;
+    int nt = (nKeys + SCAN_TILE - 1) / SCAN_TILE;
+    if (nt < 1) nt = 1;
+    hipLaunchKernelGGL(k_tile_reduce, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, s.tileSums, nullptr);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, s.tileSums, nt, offsets,
+                       contentsCap, nullptr);
+    if (nItems > 0) {
+        hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
+                           offsets, slot, counts, s.tmp, contentsCap, nullptr);
+        hipLaunchKernelGGL(k_key_place, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
+                           offsets, s.tmp, list, contentsCap, nullptr, pay);
+    }
+    return check_launch(c, "counting_sort_counted");
+}
+
 int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
                   int *list, const int *gate, hipStream_t st, SortPayload pay) {
     int r = sort_scratch_reserve(c, s, nKeys, nItems);

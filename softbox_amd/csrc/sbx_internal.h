@@ -65,6 +65,14 @@ struct SortPayload {
 };
 int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
                   int *list, const int *gate = nullptr, hipStream_t st = nullptr, SortPayload pay = SortPayload());
+// The same sort with the histogram already taken: counts[key] (all-zero outside the keys present)
+// and slot[m] = the atomicAdd return of item m (k_rod takes them when it bins the new pose).  The
+// sort leaves counts all-zero again.
+int counting_sort_counted(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap,
+                          int *offsets, int *list, int *counts, const int *slot, hipStream_t st = nullptr,
+                          SortPayload pay = SortPayload());
+// the histogram pass alone: slot[m] = atomicAdd(counts[key[m]], 1)
+int count_keys(sbx_ctx *c, const int *key, int nItems, int nKeys, int *counts, int *slot, hipStream_t st = nullptr);
 int fill_i32(sbx_ctx *c, int *dst, int value, long long count);
 
 ChainParams chain_params(const float *cp);

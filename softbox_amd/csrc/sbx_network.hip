@@ -101,6 +101,9 @@ struct sbx_net {
     hipStream_t side = nullptr;
     hipEvent_t evFork = nullptr, evJoin = nullptr, evBuilt = nullptr, evPlaced = nullptr;
     float4 *sortedPose = nullptr;  // bodyPack in contents order (written by the grid sort's place pass)
+    // grid histogram per pose buffer: k_rod takes counts[cell]++ and the slot as it bins the new
+    // pose (per-step rebuild only); the next step's sort consumes and re-zeroes them
+    int *gridCounts[2] = {nullptr, nullptr}, *gridSlot[2] = {nullptr, nullptr};
     int *sortedCell = nullptr;     // bodyCell in contents order
     int overlap = 1;
     // graph
@@ -136,6 +139,7 @@ struct RodArgs {
     int hasMotors;
     int *activeLinkCount;   // written from the gather's active-link tally (may be null)
     int *nextCell;          // bodyCell of the new pose for the next step's grid (null: not fused)
+    int *nextCounts, *nextSlot;  // the new pose's grid histogram (null: not taken here)
     float4 *linkPack;       // packed endpoint data of the new pose for the next step's links (or null)
     const float *skinRef, *skin;  // skin guard: centre at the last rebuild and per-body skin (or null)
     int *rebuildFlag;             // skin rebuild request word of the new pose's buffer
@@ -234,7 +238,11 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         a.end1[k * n + i] = d.e1[k];
         a.end2[k * n + i] = d.e2[k];
     }
-    if (a.nextCell) a.nextCell[i] = cell_of(p.c[0], p.c[1], p.c[2], a.geom);
+    if (a.nextCell) {
+        const int cell = cell_of(p.c[0], p.c[1], p.c[2], a.geom);
+        a.nextCell[i] = cell;
+        if (a.nextCounts) a.nextSlot[i] = atomicAdd(&a.nextCounts[cell], 1);
+    }
     if (a.skinRef) {  // exact displacement guard: any body beyond its skin forces the next rebuild
         float dx = p.c[0] - a.skinRef[i], dy = p.c[1] - a.skinRef[n + i], dz = p.c[2] - a.skinRef[2 * n + i];
         if (a.geom.periodic) {
@@ -1164,8 +1172,9 @@ int enqueue_step(sbx_net *net, int stepOff, bool firstInCapture = false) {
                 pay.in = net->bodyPack[net->cur];
                 pay.out = net->sortedPose;
                 pay.keyOut = net->sortedCell;
-                r = counting_sort(c, net->sort, net->bodyCell, n, net->p.gridDims[3], n, net->cellOffsets,
-                                  net->cellContents, nullptr, s2, pay);
+                r = counting_sort_counted(c, net->sort, net->bodyCell, n, net->p.gridDims[3], n, net->cellOffsets,
+                                          net->cellContents, net->gridCounts[net->cur], net->gridSlot[net->cur], s2,
+                                          pay);
                 if (r) return r;
                 if (ovl && (r = check_hip(c, hipEventRecord(net->evPlaced, s2), "record evPlaced"))) return r;
             }
@@ -1213,6 +1222,8 @@ int enqueue_step(sbx_net *net, int stepOff, bool firstInCapture = false) {
     a.activeLinkCount = net->xl.activeLinkCount;
     a.geom = net->geom;
     a.nextCell = (net->p.gridOn && !net->hasMotors) ? net->cellNext : nullptr;
+    a.nextCounts = (a.nextCell && !skinOn) ? net->gridCounts[nxt] : nullptr;
+    a.nextSlot = a.nextCounts ? net->gridSlot[nxt] : nullptr;
     a.linkPack = L > 0 ? net->linkPack : nullptr;
     a.skinRef = skinOn ? net->refC : nullptr;
     a.skin = skinOn ? net->skin : nullptr;
@@ -1273,7 +1284,7 @@ void release(sbx_net *net) {
                     net->cCell,    net->cOff,      net->cCont,    net->refC,    net->refU, net->skin,
                     net->cellSkin, net->cellHorizon, net->vlist,  net->vcount,  net->skinFlags, net->model,
                     net->cgDev,    net->cCellsDev, net->bodyPack[0], net->bodyPack[1], net->sortedPose,
-                    net->sortedCell};
+                    net->sortedCell, net->gridCounts[0], net->gridCounts[1], net->gridSlot[0], net->gridSlot[1]};
     for (void *b : bufs)
         if (b) hipFree(b);
     sort_scratch_free(net->sort);
@@ -1357,6 +1368,7 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
             if ((r = alloc(c, &net->boundR, n))) break;
             if ((r = alloc(c, &net->bodyCell, n))) break;
             if ((r = alloc(c, &net->cellNext, n))) break;
+            if ((r = fill_i32(c, net->bodyCell, -1, n))) break;  // no cells yet (the first step bins)
             if ((r = alloc(c, &net->cellOffsets, (size_t)p->gridDims[3] + 1))) break;
             if ((r = alloc(c, &net->cellContents, n))) break;
             if ((r = alloc(c, &net->candPartner, (size_t)n * p->maxCand))) break;
@@ -1371,6 +1383,12 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
             if ((r = check_hip(c, hipEventCreateWithFlags(&net->evPlaced, hipEventDisableTiming), "event"))) break;
             if ((r = alloc(c, &net->sortedPose, n))) break;
             if ((r = alloc(c, &net->sortedCell, n))) break;
+            for (int b = 0; b < 2 && !r; b++) {
+                if ((r = alloc(c, &net->gridCounts[b], (size_t)p->gridDims[3] + 1))) break;
+                if ((r = alloc(c, &net->gridSlot[b], n))) break;
+                r = fill_i32(c, net->gridCounts[b], 0, (long long)p->gridDims[3] + 1);
+            }
+            if (r) break;
             hipLaunchKernelGGL(k_bounding_radius, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, fil->segLength,
                                p->actinRadius, net->boundR, n);
             if ((r = check_launch(c, "k_bounding_radius"))) break;
@@ -1516,9 +1534,17 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
     // to the device counter by graph replays (each replay ends with k_add_step(G)).
     int done = 0, advanced = 0;
     if (!net->derivedValid) {  // bin and pack the caller's pose once, eagerly (k_rod keeps them after)
-        if (net->p.gridOn && !net->hasMotors &&
-            (r = launch_body_cell(c, net->coordB[net->cur], net->n, net->n, net->geom, net->bodyCell)))
-            return r;
+        if (net->p.gridOn && !net->hasMotors) {
+            const int nk = net->p.gridDims[3];
+            // both histograms start over: the one k_rod took may sit under the other buffer index
+            // (sbx_net_publish moves the pose to buffer 0), and the caller may have moved bodies
+            for (int b = 0; b < 2; b++)
+                if ((r = fill_i32(c, net->gridCounts[b], 0, (long long)nk + 1))) return r;
+            if ((r = launch_body_cell(c, net->coordB[net->cur], net->n, net->n, net->geom, net->bodyCell))) return r;
+            if (net->skinMode == SBX_SKIN_REBUILD &&
+                (r = count_keys(c, net->bodyCell, net->n, nk, net->gridCounts[net->cur], net->gridSlot[net->cur])))
+                return r;
+        }
         if (net->p.gridOn && !net->hasMotors) {
             hipLaunchKernelGGL(k_pack_bodies, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream,
                                net->coordB[net->cur], net->boundR, net->bodyPack[net->cur], net->n);

@@ -79,6 +79,28 @@ def test_network_box_mode_with_walls(ctx):
     net.close()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_publish_between_steps_matches_oracle(ctx, graph):
+    """Steps interleaved with pulls (each publishes the pose back to the caller's buffer and
+    re-bins it): the grid, its histogram and every output stay exactly the oracle's."""
+    from softbox_amd.network import Network
+
+    sc = small_network(seed=9, n_fil=150, links=400)
+    orc = ffi.OracleNet(sc)
+    net = Network(ctx, sc)
+    if graph:
+        net.use_graph(True)
+    t = 0
+    for k in (1, 3, 2, 5, 1, 4):
+        net.step(t, k)
+        net.pull(sc.fs.copy(), sc.xl.copy())
+        t += k
+    for s in range(t):
+        orc.step(s, 1)
+    _compare(net, orc, sc, f"publish between steps, graph={graph}, after {t} steps")
+    net.close()
+
+
 def test_graph_replay_equals_eager(ctx):
     """A hipGraph-captured run must equal eager launches, replay after replay, whatever mix of odd
     and even step counts (eager alignment steps before replays, remainders after)."""
