@@ -41,10 +41,11 @@ def kernel_bytes(sc, name: str, list_entries: int | None = None) -> float | None
     L = sc.xl.nLinks if sc.xl is not None else 0
     cells = int(sc.p["gridDims"][3])
     if name == "k_rod":
-        # read pose 36 + segLength 4 + gamma 24 + Brownian scales 8 + topology 16 + CSR offsets 16,
-        # write pose 36 + zVec/end1/end2 36 + active-link count 4 + next-step cell 4 -> 184 B/rod;
+        # read pose 36 + segLength 4 + gamma 24 + Brownian scales 8 + topology 16 + CSR offsets 16 +
+        # bounding radius 4, write pose 36 + zVec/end1/end2 36 + active-link count 4 + next-step cell 4
+        # + histogram count/slot 8 + packed pose 16 + link pack 48 -> 260 B/rod;
         # per link end: index 4 + state 4 + payload 24
-        return 184.0 * n + 32.0 * 2 * L
+        return 260.0 * n + 32.0 * 2 * L
     if name == "k_net_link":
         # ids 8 + locs 8 + state 4 + orient 4 + torque ring 20+4 + endpoints A/B (end1, u, coord, gamma) 2x40
         # + both segments' CSR bounds 2x16; write payload 48 + ring slot 4 + place 4
@@ -54,11 +55,12 @@ def kernel_bytes(sc, name: str, list_entries: int | None = None) -> float | None
     if name == "k_body_cell":
         return 16.0 * n
     if name == "grid_sort":
-        # per body: count (key 4, slot 4, counts/tile atomics 8), scatter (key, offset, slot 12, tmp 4,
-        # counts 4), place (key 4, offsets 8, segment 4, list 4); per cell: counts 4 + offsets 4
-        return 56.0 * n + 8.0 * cells
+        # (the histogram is taken by k_rod) per body: scatter (key 4, offset 4, slot 4, count reset 4,
+        # position 4), place (key 4, offsets 8, cell segment 4, contents 4, packed pose 16 + 16,
+        # sorted cell 4); per cell: counts read twice (tile sums, scan) 8 + offsets 4
+        return 76.0 * n + 12.0 * cells
     if name == "k_broad_phase":
-        # per body: contents 4 + cell 4 + centre 12 + radius 4 + count 4; per cell: offsets 4
+        # per body: contents 4 + sorted pose 16 + sorted cell 4 + count 4; per cell: offsets 4
         return 28.0 * n + 4.0 * cells
     if name == "k_skin_pairs" and list_entries:
         # per body: order 4 + centre 12 + radius 4 + cell 4 + list length 4 + count 4; per list entry:
