@@ -387,7 +387,8 @@ int sbx_net_candidates(sbx_net *net, int **candPartner, int **candCount, int **g
 int sbx_net_set_profiling(sbx_net *net, int on);
 /* accumulated [kernel count] names / total ms / launches; returns number of kernels written (<= cap) */
 int sbx_net_kernel_times(sbx_net *net, const char **names, double *total_ms, long long *launches, int cap);
-/* 1: capture one step as a hipGraph and replay it (re-captured when nSteps patterns change) */
+/* on = 1 (or 2): capture 2 steps as one hipGraph and replay it; 0: eager launches.  Step counts
+ * that are not a multiple of 2 run their remainder eagerly. */
 int sbx_net_use_graph(sbx_net *net, int on);
 /* 1 (default): the filament-only step runs its broad phase on a second stream of the library,
  * concurrently with the crosslink and rod kernels (the candidate slices feed nothing later in the

@@ -97,19 +97,28 @@ struct sbx_net {
     int *skinFlags = nullptr;
     float *model = nullptr;                 // MLP_NPARAM device floats (learned mode)
     float4 *bodyPack[2] = {nullptr, nullptr};  // {centre, bounding radius} per pose buffer (by k_rod; grid on)
-    // broad phase on a second stream, overlapping the mechanics kernels (it feeds nothing in the step)
-    hipStream_t side = nullptr;
-    hipEvent_t evFork = nullptr, evJoin = nullptr, evBuilt = nullptr, evPlaced = nullptr;
-    float4 *sortedPose = nullptr;  // bodyPack in contents order (written by the grid sort's place pass)
+    // broad phase off the main stream, overlapping the mechanics kernels (it feeds nothing in the
+    // step); per-step rebuild: the grid sort on a third stream and the grid CSR double-buffered by
+    // pose buffer, so the next pose's sort runs while this pose's broad phase still reads its CSR
+    hipStream_t side = nullptr, sortS = nullptr;
+    hipEvent_t evFork = nullptr, evBuilt = nullptr;
+    hipEvent_t evRod[2] = {nullptr, nullptr}, evPlaced[2] = {nullptr, nullptr}, evJoin[2] = {nullptr, nullptr};
+    int needFork = 1;  // the side streams must be ordered after the main stream before their next work
+    int *csrOff[2] = {nullptr, nullptr}, *csrCont[2] = {nullptr, nullptr};  // [0] = cellOffsets/cellContents
+    float4 *sortedPose[2] = {nullptr, nullptr};  // bodyPack in contents order (the sort's place pass)
+    int *sortedCell[2] = {nullptr, nullptr};     // bodyCell in contents order
+    int lastCsr = 0;                             // CSR buffer of the latest step's grid
     // grid histogram per pose buffer: k_rod takes counts[cell]++ and the slot as it bins the new
     // pose (per-step rebuild only); the next step's sort consumes and re-zeroes them
     int *gridCounts[2] = {nullptr, nullptr}, *gridSlot[2] = {nullptr, nullptr};
-    int *sortedCell = nullptr;     // bodyCell in contents order
     int overlap = 1;
     // graph
     int useGraph = 0;
     int graphSteps = 0;  // non-zero while a capture is being recorded
     int graphCur = 0;    // pose buffer index the captured graph starts from
+    int graphLen = 2;    // steps per captured graph (even: the pose ping-pong returns to its start)
+    int graphLastCsr = 0;          // CSR buffer of the captured graph's last step
+    int *graphLastCell = nullptr;  // cell buffer of the captured graph's last step
     hipGraphExec_t graphExec = nullptr;
     hipGraph_t graph = nullptr;
 };
@@ -1132,59 +1141,89 @@ bool overlapped(const sbx_net *net) {
     return net->overlap && net->side && !net->profiling && !net->hasMotors && net->p.gridOn;
 }
 
+// The main stream waits for all side-stream work enqueued so far (every chain ends in an evJoin
+// record: the broad phase waits for its sort); the next side work forks anew.
+int join_side(sbx_net *net) {
+    if (!overlapped(net)) return SBX_OK;
+    sbx_ctx *c = net->ctx;
+    int r = SBX_OK;
+    for (int b = 0; b < 2 && !r; b++) r = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin[b], 0), "join");
+    net->needFork = 1;
+    return r;
+}
+
 // One step's launches; stepOff is added to the device step counter.  With overlap the broad phase
-// (its output is the step's candidate slices, read by no later kernel of the step) runs on the side
-// stream, forked after the previous step's k_rod:
-//   side: [sort (place also lays out pose + cell in contents order) -> evPlaced -> broad phase |
-//          gated skin rebuild -> evBuilt -> pairs] -> evJoin
-//   main: (wait evPlaced of the previous step) unbind -> link -> (wait evBuilt) -> k_rod
-// k_rod overwrites the packed pose and cells the previous step's place pass read (the broad phase
-// itself reads only the sorted copies), and its skin guard reads this step's rebuild (skin mode:
-// whose pairs read the packed pose k_rod overwrites a step later); the next fork orders the next
-// broad phase after k_rod.  firstInCapture: no wait on an event recorded outside the graph being
-// captured (the caller joined the side stream before the capture began).
-int enqueue_step(sbx_net *net, int stepOff, bool firstInCapture = false) {
+// (its output is the step's candidate slices, read by no later kernel of the step) leaves the
+// main stream.  Per-step rebuild, pose buffer c = cur, x = 1 - c, three streams:
+//   sort:  (wait evRod[x]: the previous k_rod binned, counted and packed this pose)
+//          (wait evJoin[c]: the broad phase two steps back last read CSR[c])
+//          sort -> CSR[c] (+ pose and cell in contents order) -> evPlaced[c]
+//   side:  (wait evPlaced[c]) broad phase on CSR[c] -> evJoin[c]
+//   main:  (wait evPlaced[x]: k_rod overwrites the pack, cells and histogram the previous sort read)
+//          unbind -> link -> k_rod -> evRod[c]
+// so the next step's sort runs while this step's broad phase does.  Skin modes, two streams:
+//   side:  (wait evRod[x]) gated skin rebuild -> evBuilt -> pairs -> evJoin[c]
+//   main:  unbind -> link -> (wait evBuilt: the guard reads this step's rebuild) -> k_rod -> evRod[c]
+// (pairs read the packed pose k_rod overwrites a step later; the next rebuild follows them on side).
+// The side streams are forked from the main stream (needFork) at the start of a call, of a capture
+// and after a graph replay, so a wait on an event recorded before the fork is only redundant.
+// capK: index of the step in the graph being captured (-1: eager); a capture waits on no event
+// recorded outside it.
+int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
     sbx_ctx *c = net->ctx;
     hipStream_t s = c->stream;
     const int n = net->n, L = net->L;
     const float *coord = net->coordB[net->cur], *uVec = net->uB[net->cur], *yVec = net->yB[net->cur];
     const bool ovl = overlapped(net);
     const bool skinOn = net->skinMode != SBX_SKIN_REBUILD && !net->hasMotors;
-    hipStream_t s2 = ovl ? net->side : s;
+    hipStream_t s2 = ovl ? net->side : s, sS = ovl ? net->sortS : s;
+    const int cb = net->cur, xb = 1 - cb;
+    const bool prev1 = capK < 0 || capK >= 1, prev2 = capK < 0 || capK >= 2;  // step s-1 / s-2 recorded here
     int r;
     if (net->hasMotors) {
         if ((r = enqueue_myosin(net, coord, uVec, yVec, stepOff))) return r;
     } else if (net->p.gridOn) {
         if (!net->derivedValid) return fail(c, SBX_ERR_STATE, "net step: derived pose data not prepared");
-        if (ovl) {
+        if (ovl && (net->needFork || capK == 0)) {
             if ((r = check_hip(c, hipEventRecord(net->evFork, s), "record evFork"))) return r;
             if ((r = check_hip(c, hipStreamWaitEvent(s2, net->evFork, 0), "wait evFork"))) return r;
-            if (!skinOn && !firstInCapture &&
-                (r = check_hip(c, hipStreamWaitEvent(s, net->evPlaced, 0), "wait evPlaced")))
-                return r;
+            if ((r = check_hip(c, hipStreamWaitEvent(sS, net->evFork, 0), "wait evFork"))) return r;
+            net->needFork = 0;
         }
         if (skinOn) {
+            if (ovl && prev1 && (r = check_hip(c, hipStreamWaitEvent(s2, net->evRod[xb], 0), "wait evRod")))
+                return r;
             if ((r = enqueue_skin(net, coord, uVec, s2, ovl))) return r;
+            if (ovl && (r = check_hip(c, hipEventRecord(net->evJoin[cb], s2), "record evJoin"))) return r;
         } else {
             {
                 Scope sc(net, T_SORT);
+                if (ovl && prev1 && (r = check_hip(c, hipStreamWaitEvent(sS, net->evRod[xb], 0), "wait evRod")))
+                    return r;
+                if (ovl && prev2 && (r = check_hip(c, hipStreamWaitEvent(sS, net->evJoin[cb], 0), "wait evJoin")))
+                    return r;
                 SortPayload pay;
-                pay.in = net->bodyPack[net->cur];
-                pay.out = net->sortedPose;
-                pay.keyOut = net->sortedCell;
-                r = counting_sort_counted(c, net->sort, net->bodyCell, n, net->p.gridDims[3], n, net->cellOffsets,
-                                          net->cellContents, net->gridCounts[net->cur], net->gridSlot[net->cur], s2,
-                                          pay);
+                pay.in = net->bodyPack[cb];
+                pay.out = net->sortedPose[cb];
+                pay.keyOut = net->sortedCell[cb];
+                r = counting_sort_counted(c, net->sort, net->bodyCell, n, net->p.gridDims[3], n, net->csrOff[cb],
+                                          net->csrCont[cb], net->gridCounts[cb], net->gridSlot[cb], sS, pay);
                 if (r) return r;
-                if (ovl && (r = check_hip(c, hipEventRecord(net->evPlaced, s2), "record evPlaced"))) return r;
+                if (ovl && (r = check_hip(c, hipEventRecord(net->evPlaced[cb], sS), "record evPlaced"))) return r;
             }
             {
                 Scope sc(net, T_PAIRS);
-                r = launch_broad_phase_sorted(c, net->sortedPose, net->sortedCell, net->cellOffsets,
-                                              net->cellContents, net->geom, n, n, net->candPartner, net->candCount,
+                if (ovl && (r = check_hip(c, hipStreamWaitEvent(s2, net->evPlaced[cb], 0), "wait evPlaced")))
+                    return r;
+                r = launch_broad_phase_sorted(c, net->sortedPose[cb], net->sortedCell[cb], net->csrOff[cb],
+                                              net->csrCont[cb], net->geom, n, n, net->candPartner, net->candCount,
                                               net->p.maxCand, s2);
                 if (r) return r;
+                if (ovl && (r = check_hip(c, hipEventRecord(net->evJoin[cb], s2), "record evJoin"))) return r;
             }
+            net->lastCsr = cb;
+            if (ovl && prev1 && (r = check_hip(c, hipStreamWaitEvent(s, net->evPlaced[xb], 0), "wait evPlaced")))
+                return r;
         }
     }
     if (L > 0) {
@@ -1251,7 +1290,7 @@ int enqueue_step(sbx_net *net, int stepOff, bool firstInCapture = false) {
         Scope sc(net, T_ROD);
         hipLaunchKernelGGL(k_rod, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, net->cp);
     }
-    if (ovl && (r = check_hip(c, hipEventRecord(net->evJoin, s2), "record evJoin"))) return r;
+    if (ovl && (r = check_hip(c, hipEventRecord(net->evRod[cb], s), "record evRod"))) return r;
     net->cur = nxt;
     net->lastCell = net->bodyCell;
     if (a.nextCell) {  // the next step's grid reads the cells k_rod just wrote
@@ -1274,6 +1313,7 @@ void release(sbx_net *net) {
     if (net->ctx) hipSetDevice(net->ctx->device);
     if (net->ctx) hipStreamSynchronize(net->ctx->stream);
     if (net->side) hipStreamSynchronize(net->side);
+    if (net->sortS) hipStreamSynchronize(net->sortS);
     if (net->graphExec) hipGraphExecDestroy(net->graphExec);
     if (net->graph) hipGraphDestroy(net->graph);
     void *bufs[] = {net->ownPose,  net->boundR,    net->bodyCell, net->cellNext,    net->cellContents, net->linkPack,
@@ -1283,8 +1323,9 @@ void release(sbx_net *net) {
                     net->mOff,     net->mList,     net->bbOff,    net->bbList,
                     net->cCell,    net->cOff,      net->cCont,    net->refC,    net->refU, net->skin,
                     net->cellSkin, net->cellHorizon, net->vlist,  net->vcount,  net->skinFlags, net->model,
-                    net->cgDev,    net->cCellsDev, net->bodyPack[0], net->bodyPack[1], net->sortedPose,
-                    net->sortedCell, net->gridCounts[0], net->gridCounts[1], net->gridSlot[0], net->gridSlot[1]};
+                    net->cgDev,    net->cCellsDev, net->bodyPack[0], net->bodyPack[1], net->sortedPose[0],
+                    net->sortedPose[1], net->sortedCell[0], net->sortedCell[1], net->csrOff[1], net->csrCont[1],
+                    net->gridCounts[0], net->gridCounts[1], net->gridSlot[0], net->gridSlot[1]};
     for (void *b : bufs)
         if (b) hipFree(b);
     sort_scratch_free(net->sort);
@@ -1293,9 +1334,11 @@ void release(sbx_net *net) {
         hipEventDestroy(pe.second.second);
     }
     for (hipEvent_t e : net->eventPool) hipEventDestroy(e);
-    for (hipEvent_t e : {net->evFork, net->evJoin, net->evBuilt, net->evPlaced})
+    for (hipEvent_t e : {net->evFork, net->evBuilt, net->evRod[0], net->evRod[1], net->evPlaced[0], net->evPlaced[1],
+                         net->evJoin[0], net->evJoin[1]})
         if (e) hipEventDestroy(e);
     if (net->side) hipStreamDestroy(net->side);
+    if (net->sortS) hipStreamDestroy(net->sortS);
     delete net;
 }
 
@@ -1377,12 +1420,20 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
             if ((r = alloc(c, &net->bodyPack[1], n))) break;
             // default priority, like the context's stream
             if ((r = check_hip(c, hipStreamCreateWithFlags(&net->side, hipStreamNonBlocking), "side stream"))) break;
-            if ((r = check_hip(c, hipEventCreateWithFlags(&net->evFork, hipEventDisableTiming), "event"))) break;
-            if ((r = check_hip(c, hipEventCreateWithFlags(&net->evJoin, hipEventDisableTiming), "event"))) break;
-            if ((r = check_hip(c, hipEventCreateWithFlags(&net->evBuilt, hipEventDisableTiming), "event"))) break;
-            if ((r = check_hip(c, hipEventCreateWithFlags(&net->evPlaced, hipEventDisableTiming), "event"))) break;
-            if ((r = alloc(c, &net->sortedPose, n))) break;
-            if ((r = alloc(c, &net->sortedCell, n))) break;
+            if ((r = check_hip(c, hipStreamCreateWithFlags(&net->sortS, hipStreamNonBlocking), "sort stream"))) break;
+            for (hipEvent_t *e : {&net->evFork, &net->evBuilt, &net->evRod[0], &net->evRod[1], &net->evPlaced[0],
+                                  &net->evPlaced[1], &net->evJoin[0], &net->evJoin[1]})
+                if (!r) r = check_hip(c, hipEventCreateWithFlags(e, hipEventDisableTiming), "event");
+            if (r) break;
+            net->csrOff[0] = net->cellOffsets;
+            net->csrCont[0] = net->cellContents;
+            if ((r = alloc(c, &net->csrOff[1], (size_t)p->gridDims[3] + 1))) break;
+            if ((r = alloc(c, &net->csrCont[1], n))) break;
+            for (int b = 0; b < 2 && !r; b++) {
+                if ((r = alloc(c, &net->sortedPose[b], n))) break;
+                r = alloc(c, &net->sortedCell[b], n);
+            }
+            if (r) break;
             for (int b = 0; b < 2 && !r; b++) {
                 if ((r = alloc(c, &net->gridCounts[b], (size_t)p->gridDims[3] + 1))) break;
                 if ((r = alloc(c, &net->gridSlot[b], n))) break;
@@ -1485,6 +1536,7 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
     net->bodyCell = bodyCell;
     net->lastCell = nullptr;
     net->cellContents = cellContents;
+    net->csrCont[0] = cellContents;
     net->ownerStore = ownerStore;
     net->ownerSlot = ownerSlot;
     net->mOff = mOff;
@@ -1559,20 +1611,21 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
         net->derivedValid = 1;
     }
     if (net->useGraph && !net->profiling) {
-        // one captured graph = 2 steps (the pose ping-pong returns to its start buffers) + step advance
-        const int G = 2;
+        // one captured graph = graphLen steps (even: the pose ping-pong returns to its start buffers) +
+        // step advance; a longer graph drains the stream pipeline less often
+        const int G = net->graphLen;
         if (!net->graphExec) {
             int savedCur = net->cur;
             // side-stream work of earlier eager steps is joined before the graph (which joins its own)
-            if (overlapped(net) && (r = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin, 0), "join")))
-                return r;
+            if ((r = join_side(net))) return r;
             if ((r = check_hip(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
                 return r;
             net->graphSteps = G;
             int er = SBX_OK;
-            for (int k = 0; k < G && !er; k++) er = enqueue_step(net, k, k == 0);
-            if (!er && overlapped(net))
-                er = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin, 0), "join in capture");
+            for (int k = 0; k < G && !er; k++) er = enqueue_step(net, k, k);
+            if (!er) er = join_side(net);
+            net->graphLastCsr = net->lastCsr;
+            net->graphLastCell = net->lastCell;
             if (!er) hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, c->stream, net->dStep, G);
             hipGraph_t g = nullptr;
             hipError_t ee = hipStreamEndCapture(c->stream, &g);
@@ -1598,9 +1651,11 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
                 if ((r = check_launch(c, "k_add_step"))) return r;
                 advanced = done;
             }
-            if (overlapped(net) && (r = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin, 0), "join")))
-                return r;
+            if ((r = join_side(net))) return r;
             if ((r = check_hip(c, hipGraphLaunch(net->graphExec, c->stream), "graph launch"))) return r;
+            net->needFork = 1;  // the replay's side work is joined into it; later side work forks anew
+            net->lastCsr = net->graphLastCsr;
+            net->lastCell = net->graphLastCell;
             done += G;
             advanced += G;  // G is even: net->cur is unchanged by a replay
         }
@@ -1608,8 +1663,8 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
     for (; done < nSteps; done++) {
         if ((r = enqueue_step(net, done - advanced))) return r;
     }
-    // the caller's stream covers the whole call: join the side stream
-    if (overlapped(net) && (r = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin, 0), "join"))) return r;
+    // the caller's stream covers the whole call: join the side streams
+    if ((r = join_side(net))) return r;
     if (net->profiling) return drain_timers(net);
     return SBX_OK;
 }
@@ -1650,8 +1705,9 @@ int sbx_net_candidates(sbx_net *net, int **candPartner, int **candCount, int **g
     SBX_CHECK_CTX(net->ctx);
     if (candPartner) *candPartner = net->candPartner;
     if (candCount) *candCount = net->candCount;
-    if (gridCellOffsets) *gridCellOffsets = net->cellOffsets;
-    if (gridCellContents) *gridCellContents = net->cellContents;
+    const int b = (net->p.gridOn && !net->hasMotors) ? net->lastCsr : 0;
+    if (gridCellOffsets) *gridCellOffsets = b ? net->csrOff[1] : net->cellOffsets;
+    if (gridCellContents) *gridCellContents = b ? net->csrCont[1] : net->cellContents;
     if (bodyCell) *bodyCell = net->lastCell ? net->lastCell : net->bodyCell;
     return SBX_OK;
 }
@@ -1832,7 +1888,18 @@ int sbx_net_kernel_times(sbx_net *net, const char **names, double *total_ms, lon
 int sbx_net_use_graph(sbx_net *net, int on) {
     if (!net) return SBX_ERR_ARG;
     SBX_CHECK_CTX(net->ctx);
+    // captures longer than 2 steps are switched off: an 8-step capture ended the host process
+    // (segmentation fault inside sbx_net_step) with no cause found in this library
+    SBX_REQUIRE(net->ctx, on == 0 || on == 1 || on == 2, "net_use_graph: on = 0, 1 or 2 (steps per graph)");
     if (on) SBX_REQUIRE(net->ctx, net->ctx->stream != nullptr, "net_use_graph: needs a non-default stream");
+    const int len = 2;
+    if (on && len != net->graphLen) {  // a captured graph of another length no longer applies
+        int r = check_hip(net->ctx, hipStreamSynchronize(net->ctx->stream), "net_use_graph sync");
+        if (r) return r;
+        if (net->graphExec) { hipGraphExecDestroy(net->graphExec); net->graphExec = nullptr; }
+        if (net->graph) { hipGraphDestroy(net->graph); net->graph = nullptr; }
+        net->graphLen = len;
+    }
     net->useGraph = on ? 1 : 0;
     return SBX_OK;
 }

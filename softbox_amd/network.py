@@ -68,8 +68,9 @@ class Network:
     def publish(self) -> None:
         self.ctx.check(self.ctx.lib.sbx_net_publish(self.h))
 
-    def use_graph(self, on: bool = True) -> None:
-        self.ctx.check(self.ctx.lib.sbx_net_use_graph(self.h, 1 if on else 0))
+    def use_graph(self, on: bool | int = True) -> None:
+        """True (or 2): replay 2 captured steps at a time; False: eager."""
+        self.ctx.check(self.ctx.lib.sbx_net_use_graph(self.h, int(on) if not isinstance(on, bool) else int(on)))
 
     def set_overlap(self, on: bool = True) -> None:
         """Broad phase on a second stream, concurrent with the mechanics kernels (default on)."""

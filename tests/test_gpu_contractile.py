@@ -85,7 +85,7 @@ def test_contractile_graph_replay_matches_eager(ctx):
     sc = _scene(seed=4, rng=RNG_PHILOX)
     orc = ffi.OracleContractile(sc)
     net = Network(ctx, sc)
-    net.use_graph(True)
+    net.use_graph(2)
     t = 0
     for chunk in (5, 6, 3, 4, 1, 7):
         net.step(t, chunk)

@@ -89,7 +89,7 @@ def test_publish_between_steps_matches_oracle(ctx, graph):
     orc = ffi.OracleNet(sc)
     net = Network(ctx, sc)
     if graph:
-        net.use_graph(True)
+        net.use_graph(2)
     t = 0
     for k in (1, 3, 2, 5, 1, 4):
         net.step(t, k)
@@ -101,7 +101,10 @@ def test_publish_between_steps_matches_oracle(ctx, graph):
     net.close()
 
 
-def test_graph_replay_equals_eager(ctx):
+@pytest.mark.parametrize("glen", [2, pytest.param(8, marks=pytest.mark.skip(
+    reason="8-step graph capture ended the host process (segfault inside sbx_net_step), cause not found; "
+           "the library accepts only 2-step graphs"))])
+def test_graph_replay_equals_eager(ctx, glen):
     """A hipGraph-captured run must equal eager launches, replay after replay, whatever mix of odd
     and even step counts (eager alignment steps before replays, remainders after)."""
     from softbox_amd.network import Network
@@ -109,9 +112,9 @@ def test_graph_replay_equals_eager(ctx):
     sc = small_network(seed=7, n_fil=150, links=400)
     a = Network(ctx, sc.copy())
     b = Network(ctx, sc.copy())
-    b.use_graph(True)
+    b.use_graph(glen)
     t = 0
-    for n in (5, 6, 3, 4, 1, 7):
+    for n in (5, 6, 3, 4, 1, 7, 17, 9):
         a.step(t, n)
         b.step(t, n)
         t += n
@@ -190,10 +193,10 @@ def test_overlap_equals_single_stream(ctx, graph, skin):
         if skin:
             net.set_skin(_lib.SBX_SKIN_FIXED, SkinConfig(skin_default=0.15))
         if graph:
-            net.use_graph(True)
+            net.use_graph(2)
     n, mc = sc.fs.n, int(sc.p["maxCand"])
     t = 0
-    for k in (3, 4, 2, 5):
+    for k in (3, 4, 2, 5, 11):
         a.step(t, k)
         b.step(t, k)
         t += k

@@ -81,7 +81,7 @@ def _run_pair(ctx, sc, mode, cfg, steps, model=None, graph=False, check_every=1)
     skin = Network(ctx, sc.copy())
     skin.set_skin(mode, cfg, model)
     if graph:
-        skin.use_graph(True)
+        skin.use_graph(2)
     t = 0
     while t < steps:
         k = check_every
