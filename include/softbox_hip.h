@@ -378,7 +378,8 @@ void sbx_net_destroy(sbx_net *net);
 int sbx_net_step(sbx_net *net, int step0, int nSteps);
 /* make the caller's coord/uVec/yVec hold the current pose (the step ping-pongs pose buffers) */
 int sbx_net_publish(sbx_net *net);
-/* rebuild the crosslink CSR after the caller changed linkFilA/linkFilB (formation) */
+/* rebuild the crosslink CSR and recount the active links after the caller changed linkFilA/linkFilB or
+ * linkState between steps (formation); sbx_net_publish also recounts on the next step */
 int sbx_net_topology_changed(sbx_net *net);
 /* device pointers of the latest broad-phase output and grid CSR */
 int sbx_net_candidates(sbx_net *net, int **candPartner, int **candCount, int **gridCellOffsets,

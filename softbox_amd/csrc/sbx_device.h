@@ -92,7 +92,7 @@ __device__ __forceinline__ void brownian_noise(int rng, uint32_t body, int32_t s
     for (int a = 0; a < 3; a++) {
         float ua = fmax_java(1.0e-7f, uniform01(h[2 * a]));
         float ub = uniform01(h[2 * a + 1]);
-        float r = (float)sqrt((double)(-2.0f * (float)log((double)ua)));
+        float r = sqrtf(-2.0f * (float)log((double)ua));
         float th = 2.0f * 3.14159265f * ub;
         double s, c;
         sincos((double)th, &s, &c);
@@ -101,9 +101,12 @@ __device__ __forceinline__ void brownian_noise(int rng, uint32_t body, int32_t s
     }
 }
 
-// randForce/randTorque amplitude (BrownianForceSystem.java:234-243): ((tS*bfm)*sqrt(g))*gauss
+// randForce/randTorque amplitude (BrownianForceSystem.java:234-243): ((tS*bfm)*sqrt(g))*gauss.
+// The reference's (float)Math.sqrt((double)x) is sqrtf(x) for every float x: a square root rounded to
+// double and then to float is correctly rounded (53 >= 2 * 24 + 2), and sqrtf is correctly rounded
+// here (-fhip-fp32-correctly-rounded-divide-sqrt).
 __device__ __forceinline__ float brown_amp(float scale, float bfm, float gam, float g) {
-    return scale * bfm * (float)sqrt((double)gam) * g;
+    return scale * bfm * sqrtf(gam) * g;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -126,7 +129,7 @@ __device__ __forceinline__ void integrate_derive(Pose &p, const float f[3], cons
     float zx = uyc * yzc - uzc * yyc;
     float zy = uzc * yx - ux * yzc;
     float zz = ux * yyc - uyc * yx;
-    float zlen = 1.0f / (float)sqrt((double)(zx * zx + zy * zy + zz * zz));
+    float zlen = 1.0f / sqrtf(zx * zx + zy * zy + zz * zz);
     zx *= zlen; zy *= zlen; zz *= zlen;
     float fx = f[0], fy = f[1], fz = f[2], tx = t[0], ty = t[1], tz = t[2];
     float bfx = ux * fx + uyc * fy + uzc * fz + rf[0];
@@ -152,14 +155,14 @@ __device__ __forceinline__ void integrate_derive(Pose &p, const float f[3], cons
     float nuX = ux + yx * uTransInY + zx * uTransInZ;
     float nuY = uyc + yyc * uTransInY + zy * uTransInZ;
     float nuZ = uzc + yzc * uTransInY + zz * uTransInZ;
-    float nuInv = 1.0f / (float)sqrt((double)(nuX * nuX + nuY * nuY + nuZ * nuZ));
+    float nuInv = 1.0f / sqrtf(nuX * nuX + nuY * nuY + nuZ * nuZ);
     float nux = nuX * nuInv, nuy = nuY * nuInv, nuz = nuZ * nuInv;
     float yTransInX = -uTransInY;
     float yTransInZ = bwx * dt;
     float nyX = ux * yTransInX + yx + zx * yTransInZ;
     float nyY = uyc * yTransInX + yyc + zy * yTransInZ;
     float nyZ = uzc * yTransInX + yzc + zz * yTransInZ;
-    float nyInv = 1.0f / (float)sqrt((double)(nyX * nyX + nyY * nyY + nyZ * nyZ));
+    float nyInv = 1.0f / sqrtf(nyX * nyX + nyY * nyY + nyZ * nyZ);
     float ny0 = nyX * nyInv, ny1 = nyY * nyInv, ny2 = nyZ * nyInv;
     // derive
     float dzx = nuy * ny2 - nuz * ny1;
@@ -167,7 +170,7 @@ __device__ __forceinline__ void integrate_derive(Pose &p, const float f[3], cons
     float dzz = nux * ny1 - nuy * ny0;
     float zmag2 = dzx * dzx + dzy * dzy + dzz * dzz;
     if (zmag2 > 0.0f) {
-        float inv = 1.0f / (float)sqrt((double)zmag2);
+        float inv = 1.0f / sqrtf(zmag2);
         dzx *= inv; dzy *= inv; dzz *= inv;
     }
     d.z[0] = dzx; d.z[1] = dzy; d.z[2] = dzz;
@@ -294,15 +297,27 @@ struct ChainParams {
     double dt, fracMove, fracR, fracMoveTorq, springActive, spring, monoR;
 };
 
-// One joint of chainForces: endSign +1 = end2 block (:152-235), -1 = end1 block (:237-321).
-// Neighbour data (nc, nu, nlen, nBTGx, nBTGy, nBRGy) is passed in so callers can stage it.
-__device__ __forceinline__ void chain_joint(int i, int nbr, int side, double endSign, double cx, double cy, double cz,
-                                            double ux, double uy, double uz, double len, double rBTGx, double rBTGy,
-                                            double rBRGy, double ncx, double ncy, double ncz, double nux, double nuy,
-                                            double nuz, double nlen, double nBTGx, double nBTGy, double nBRGy,
-                                            const ChainParams &cp, double f[3], double t[3]) {
+// The terms of one joint of chainForces that the two rods of the joint share: the bond force F on
+// this rod (the damping-limited link, :182-209) and the torsion torque Tq (:215-234; `torsion` is 0
+// when |mu x nu|^2 <= 1e-30 and nothing is added).  endSign +1 = end2 block (:152-235), -1 = end1
+// block (:237-321).  Evaluated from the neighbour's side, the same joint gives exactly -F and -Tq:
+// the link and tip points, the strain, moveC1 + moveC2 (the two terms swap), forceMag, |tv|^2, the
+// dot product and the torsion magnitude are the same floating-point operations on the same
+// values, fSign flips, and tv = mu x nuE flips with every product of it unchanged.
+struct JointTerms {
+    double F[3], Tq[3];
+    int torsion;
+};
+
+__device__ __forceinline__ JointTerms chain_joint_terms(int i, int nbr, int side, double endSign, double cx, double cy,
+                                                        double cz, double ux, double uy, double uz, double len,
+                                                        double rBTGx, double rBTGy, double rBRGy, double ncx,
+                                                        double ncy, double ncz, double nux, double nuy, double nuz,
+                                                        double nlen, double nBTGx, double nBTGy, double nBRGy,
+                                                        const ChainParams &cp) {
     const double DEG2RAD = SBX_PI_D / 180.0;
     const double RAD2DEG = 180.0 / SBX_PI_D;
+    JointTerms J;
     double halfLen_um = 0.5 * len;
     double lSqSelf = 1.0e-12 * len * len;
     double nHalfLen_um = 0.5 * nlen;
@@ -350,18 +365,16 @@ __device__ __forceinline__ void chain_joint(int i, int nbr, int side, double end
     double denom = cp.dt * (moveC1 + moveC2);
     double forceMag = (denom > 0.0) ? (cp.fracMove * 1.0e-6 * strainDist / denom) : 0.0;
     double fSign = (i < nbr) ? 1.0 : -1.0;
-    double Fx = fSign * forceMag * luX, Fy = fSign * forceMag * luY, Fz = fSign * forceMag * luZ;
-    f[0] += Fx; f[1] += Fy; f[2] += Fz;
-    double Rscale = (endSign > 0) ? (0.5e-6 * len * cp.fracR) : (-0.5e-6 * len * cp.fracR);
-    double Rx = Rscale * ux, Ry = Rscale * uy, Rz = Rscale * uz;
-    t[0] += Ry * Fz - Rz * Fy; t[1] += Rz * Fx - Rx * Fz; t[2] += Rx * Fy - Ry * Fx;
+    J.F[0] = fSign * forceMag * luX; J.F[1] = fSign * forceMag * luY; J.F[2] = fSign * forceMag * luZ;
     double nuxE, nuyE, nuzE;
     if (side == 0) { nuxE = nux; nuyE = nuy; nuzE = nuz; } else { nuxE = -nux; nuyE = -nuy; nuzE = -nuz; }
     double mux, muy, muz;
     if (endSign > 0) { mux = ux; muy = uy; muz = uz; } else { mux = -ux; muy = -uy; muz = -uz; }
     double tvx = muy * nuzE - muz * nuyE, tvy = muz * nuxE - mux * nuzE, tvz = mux * nuyE - muy * nuxE;
     double tvMag2 = tvx * tvx + tvy * tvy + tvz * tvz;
-    if (tvMag2 > 1.0e-30) {
+    J.torsion = tvMag2 > 1.0e-30;
+    J.Tq[0] = J.Tq[1] = J.Tq[2] = 0.0;
+    if (J.torsion) {
         double invMag = 1.0 / sqrt(tvMag2);
         tvx *= invMag; tvy *= invMag; tvz *= invMag;
         double dotV = mux * nuxE + muy * nuyE + muz * nuzE;
@@ -375,8 +388,33 @@ __device__ __forceinline__ void chain_joint(int i, int nbr, int side, double end
             double invBRG = 1.0 / rBRGy + 1.0 / nBRGy;
             torsionMag = cp.fracMoveTorq * DEG2RAD * angTween / (invBRG * cp.dt);
         }
-        t[0] += tvx * torsionMag; t[1] += tvy * torsionMag; t[2] += tvz * torsionMag;
+        J.Tq[0] = tvx * torsionMag; J.Tq[1] = tvy * torsionMag; J.Tq[2] = tvz * torsionMag;
     }
+    return J;
+}
+
+// The joint's contribution to this rod, in the reference's order: F, the lever-arm torque R x F
+// (R = +-1/2 L fracR u), then the torsion torque.
+__device__ __forceinline__ void chain_joint_apply(double endSign, double len, double ux, double uy, double uz,
+                                                  const ChainParams &cp, const JointTerms &J, double f[3],
+                                                  double t[3]) {
+    const double Fx = J.F[0], Fy = J.F[1], Fz = J.F[2];
+    f[0] += Fx; f[1] += Fy; f[2] += Fz;
+    double Rscale = (endSign > 0) ? (0.5e-6 * len * cp.fracR) : (-0.5e-6 * len * cp.fracR);
+    double Rx = Rscale * ux, Ry = Rscale * uy, Rz = Rscale * uz;
+    t[0] += Ry * Fz - Rz * Fy; t[1] += Rz * Fx - Rx * Fz; t[2] += Rx * Fy - Ry * Fx;
+    if (J.torsion) { t[0] += J.Tq[0]; t[1] += J.Tq[1]; t[2] += J.Tq[2]; }
+}
+
+// One joint of chainForces evaluated and applied by this rod.
+__device__ __forceinline__ void chain_joint(int i, int nbr, int side, double endSign, double cx, double cy, double cz,
+                                            double ux, double uy, double uz, double len, double rBTGx, double rBTGy,
+                                            double rBRGy, double ncx, double ncy, double ncz, double nux, double nuy,
+                                            double nuz, double nlen, double nBTGx, double nBTGy, double nBRGy,
+                                            const ChainParams &cp, double f[3], double t[3]) {
+    const JointTerms J = chain_joint_terms(i, nbr, side, endSign, cx, cy, cz, ux, uy, uz, len, rBTGx, rBTGy, rBRGy,
+                                           ncx, ncy, ncz, nux, nuy, nuz, nlen, nBTGx, nBTGy, nBRGy, cp);
+    chain_joint_apply(endSign, len, ux, uy, uz, cp, J, f, t);
 }
 
 // Whole chain contribution of body i reading planar arrays (stride n).
@@ -400,6 +438,68 @@ __device__ __forceinline__ void chain_body(int n, int i, const float *__restrict
         chain_joint(i, j, e1Side, -1.0, cx, cy, cz, ux, uy, uz, len, rBTGx, rBTGy, rBRGy, coord[j], coord[n + j],
                     coord[2 * n + j], uVec[j], uVec[n + j], uVec[2 * n + j], segLength[j], bTransGam[j],
                     bTransGam[n + j], bRotGam[n + j], cp, f, t);
+    }
+}
+
+// Rods finished per wave by chain_body_wave: lane 0 of a wave holds the previous wave's last rod
+// and only evaluates its end2 joint for lane 1.
+constexpr int CHAIN_SPAN = 63;
+
+// chain_body for the rods of one wave laid out as rod = 63 * wave + lane - 1.  Every lane evaluates
+// the joint at its rod's end2; a rod whose end1 is joined to the end2 of the rod in the lane below
+// takes that joint's terms negated (exactly its own evaluation, see JointTerms), any other end1
+// joint is evaluated by the rod itself.  One joint evaluation per rod instead of two.  Called by
+// every lane of the wave (the shuffles read all lanes); `live` = the lane holds a rod, `own` = the
+// lane finishes it (f, t are only meaningful there).  c, u, len, gT, gR: the rod's own data.
+__device__ __forceinline__ void chain_body_wave(int n, int i, bool live, bool own, int lane, const float c[3],
+                                                const float u[3], float len, const float gT[3], const float gR[3],
+                                                const float *__restrict__ coord, const float *__restrict__ uVec,
+                                                const float *__restrict__ segLength,
+                                                const float *__restrict__ bTransGam,
+                                                const float *__restrict__ bRotGam, int e2Slot, int e2Side,
+                                                int e1Slot, int e1Side, const ChainParams &cp, double f[3],
+                                                double t[3]) {
+    const int j2 = e2Slot >= 0 ? e2Slot : i, j1 = e1Slot >= 0 ? e1Slot : i;
+    // both neighbours' data, issued before either joint is evaluated
+    float n2[9], n1[9];
+    n2[0] = coord[j2]; n2[1] = coord[n + j2]; n2[2] = coord[2 * n + j2];
+    n2[3] = uVec[j2]; n2[4] = uVec[n + j2]; n2[5] = uVec[2 * n + j2];
+    n2[6] = segLength[j2]; n2[7] = bTransGam[j2]; n2[8] = bTransGam[n + j2];
+    n1[0] = coord[j1]; n1[1] = coord[n + j1]; n1[2] = coord[2 * n + j1];
+    n1[3] = uVec[j1]; n1[4] = uVec[n + j1]; n1[5] = uVec[2 * n + j1];
+    n1[6] = segLength[j1]; n1[7] = bTransGam[j1]; n1[8] = bTransGam[n + j1];
+    const float nR2 = bRotGam[n + j2], nR1 = bRotGam[n + j1];
+    const double cx = c[0], cy = c[1], cz = c[2], ux = u[0], uy = u[1], uz = u[2], L = len;
+    const double rBTGx = gT[0], rBTGy = gT[1], rBRGy = gR[1];
+    const bool has2 = live && e2Slot >= 0;
+    JointTerms J2;
+    J2.F[0] = J2.F[1] = J2.F[2] = 0.0;
+    J2.Tq[0] = J2.Tq[1] = J2.Tq[2] = 0.0;
+    J2.torsion = 0;
+    if (has2)
+        J2 = chain_joint_terms(i, j2, e2Side, +1.0, cx, cy, cz, ux, uy, uz, L, rBTGx, rBTGy, rBRGy, n2[0], n2[1],
+                               n2[2], n2[3], n2[4], n2[5], n2[6], n2[7], n2[8], nR2, cp);
+    const int src = lane > 0 ? lane - 1 : 0;
+    const int pSlot = __shfl(has2 ? e2Slot : -1, src, 64);
+    const int pSide = __shfl(e2Side, src, 64);
+    JointTerms P;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        P.F[k] = -__shfl(J2.F[k], src, 64);
+        P.Tq[k] = -__shfl(J2.Tq[k], src, 64);
+    }
+    P.torsion = __shfl(J2.torsion, src, 64);
+    if (!own) return;
+    if (has2) chain_joint_apply(+1.0, L, ux, uy, uz, cp, J2, f, t);
+    if (e1Slot >= 0) {
+        if (lane > 0 && e1Slot == i - 1 && e1Side == 1 && pSlot == i && pSide == 0) {
+            chain_joint_apply(-1.0, L, ux, uy, uz, cp, P, f, t);
+        } else {
+            const JointTerms J1 = chain_joint_terms(i, j1, e1Side, -1.0, cx, cy, cz, ux, uy, uz, L, rBTGx, rBTGy,
+                                                    rBRGy, n1[0], n1[1], n1[2], n1[3], n1[4], n1[5], n1[6], n1[7],
+                                                    n1[8], nR1, cp);
+            chain_joint_apply(-1.0, L, ux, uy, uz, cp, J1, f, t);
+        }
     }
 }
 

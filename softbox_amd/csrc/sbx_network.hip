@@ -3,7 +3,8 @@
 // Replaces the filament block of DenseContractileHarness.buildPlan/cpuStep
 // (DenseContractileHarness.java:350-429, 460-606), which the reference runs as ~30 TaskGraph tasks:
 //   [grid]  counting sort (count, one-pass scan, scatter, place) -> k_broad_phase
-//   [links] k_net_unbind -> k_net_link (active counts from the CSR + force + torsion payload)
+//   [links] k_net_unbind (deaths also decrement both ends' active-link counts) -> k_net_link
+//           (force + torsion payload; the counts are k_rod's tally of the last step less this step's deaths)
 //   [rods]  k_rod: zero + Brownian + chain + gather A + gather B + confine + integrate + derive,
 //           then the active-link count and the cell of the new pose (the next step's binning)
 // k_rod keeps forceSum/torqueSum and the Brownian draw in registers and ping-pongs the pose buffers
@@ -55,6 +56,7 @@ struct sbx_net {
         *candCount = nullptr;
     SortScratch sort;
     int derivedValid = 0;  // bodyCell and linkPack describe the current pose (written by the last k_rod)
+    int countsValid = 0;   // xl.activeLinkCount matches linkState (kept by the step after a recount)
     float4 *linkPack = nullptr;  // per-rod {c, gpar} {u} {end1} for the crosslink gathers (links only)
     int *cellNext = nullptr;  // k_rod bins the new pose here; swapped with bodyCell after each step
     int *lastCell = nullptr;  // the cells the latest step's grid used (what SpatialGrid.bodyCell left)
@@ -165,22 +167,41 @@ struct RodArgs {
 
 // Per-rod fused step.  The force accumulation keeps the reference's exact float rounding sequence:
 // forceSum = 0 -> (float)(f + chain) -> (float)(f + sumA) -> (float)(f + sumB) -> confine -> integrate.
+// Rods are laid out CHAIN_SPAN per wave (rod = 63 * wave + lane - 1) so each chain joint is evaluated
+// once per rod; lane 0 of a wave only supplies the joint at the end2 of the previous wave's last rod.
 __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    int n = a.n;
-    if (i >= n) return;
-    int step = *a.dStep + a.stepOff;
+    const int n = a.n;
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+    if (wave * CHAIN_SPAN - 1 >= n) return;  // the whole wave is past the last rod
+    const int i = wave * CHAIN_SPAN + lane - 1;
+    const bool live = i >= 0 && i < n;
+    const bool own = live && lane > 0;
+    const int ic = live ? i : (i < 0 ? 0 : n - 1);  // in-range index for the loads of idle lanes
+    Pose p;
+    float gT[3], gR[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        p.c[k] = a.coord[k * n + ic];
+        p.u[k] = a.uVec[k * n + ic];
+        p.y[k] = a.yVec[k * n + ic];
+        gT[k] = a.bTransGam[k * n + ic];
+        gR[k] = a.bRotGam[k * n + ic];
+    }
+    const float segLen = a.segLength[ic], tS = a.bts[ic], rS = a.brs[ic];
     float f[3] = {0.0f, 0.0f, 0.0f}, t[3] = {0.0f, 0.0f, 0.0f};
     if (a.chainOn) {
         double fd[3] = {0.0, 0.0, 0.0}, td[3] = {0.0, 0.0, 0.0};
-        chain_body(n, i, a.coord, a.uVec, a.segLength, a.e2Slot[i], a.e2Side[i], a.e1Slot[i], a.e1Side[i],
-                   a.bTransGam, a.bRotGam, cp, fd, td);
+        chain_body_wave(n, ic, live, own, lane, p.c, p.u, segLen, gT, gR, a.coord, a.uVec, a.segLength, a.bTransGam,
+                        a.bRotGam, a.e2Slot[ic], a.e2Side[ic], a.e1Slot[ic], a.e1Side[ic], cp, fd, td);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             f[k] = (float)(f[k] + fd[k]);
             t[k] = (float)(t[k] + td[k]);
         }
     }
+    if (!own) return;
+    const int step = *a.dStep + a.stepOff;
     if (a.hasLinks) {
         double fa[3] = {0, 0, 0}, ta[3] = {0, 0, 0};
         int active = 0;
@@ -208,17 +229,6 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
             t[k] = (float)(t[k] + tm[k]);
         }
     }
-    Pose p;
-    float gT[3], gR[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        p.c[k] = a.coord[k * n + i];
-        p.u[k] = a.uVec[k * n + i];
-        p.y[k] = a.yVec[k * n + i];
-        gT[k] = a.bTransGam[k * n + i];
-        gR[k] = a.bRotGam[k * n + i];
-    }
-    float segLen = a.segLength[i];
     if (a.confineOn && !(a.checkInt > 1 && (step % a.checkInt) != 0))
         confine_body(p.c, p.u, segLen, gT[0], a.tau, a.halfX, a.halfY, a.halfZ, a.coeff, f, t);
     if (a.forceSum) {
@@ -230,7 +240,6 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
     }
     float gf[3], gt[3], rf[3], rt[3];
     brownian_noise(a.rng, (uint32_t)i, step, a.seed, a.stream, gf, gt);
-    float tS = a.bts[i], rS = a.brs[i];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         rf[k] = brown_amp(tS, a.bfm, gT[k], gf[k]);
@@ -297,23 +306,40 @@ __global__ void __launch_bounds__(256) k_net_unbind(const float4 *__restrict__ p
                                                     const float *__restrict__ loc2, int *__restrict__ linkState,
                                                     float *__restrict__ strainHist, int *__restrict__ strainPlace,
                                                     LinkParams lp, const int *__restrict__ dStep, int stepOff,
-                                                    int seed, int rng, uint32_t stream, int nSeg, int nLinks) {
+                                                    int seed, int rng, uint32_t stream, int *__restrict__ activeLinks,
+                                                    int nSeg, int nLinks) {
     int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= nLinks) return;
     int st = linkState[k];
     if (st < 0) return;
     int step = *dStep + stepOff;
-    LinkEnd A = link_end_packed(linkFilA[k], pack), B = link_end_packed(linkFilB[k], pack);
+    const int a = linkFilA[k], b = linkFilB[k];
+    LinkEnd A = link_end_packed(a, pack), B = link_end_packed(b, pack);
     int ns = link_unbind_one(k, A, B, loc1[k], loc2[k], strainHist, strainPlace, lp, step, seed, rng, stream, st);
-    if (ns != st) linkState[k] = ns;
+    if (ns != st) {
+        linkState[k] = ns;
+        if (ns < 0) {  // death: the link no longer counts at either end (CSR A and CSR B each held it)
+            atomicSub(&activeLinks[a], 1);
+            atomicSub(&activeLinks[b], 1);
+        }
+    }
+}
+
+// CrosslinkerSystem.countActiveLinks (CrosslinkerSystem.java:694-706) for every segment: the
+// active-link counts the step keeps from then on (k_net_unbind decrements, k_rod re-tallies)
+__global__ void __launch_bounds__(256) k_link_counts(const int *__restrict__ offA, const int *__restrict__ idxA,
+                                                     const int *__restrict__ offB, const int *__restrict__ idxB,
+                                                     const int *__restrict__ linkState, int *__restrict__ activeLinks,
+                                                     int n) {
+    int s = blockIdx.x * 256 + threadIdx.x;
+    if (s < n) activeLinks[s] = active_links(s, offA, idxA, offB, idxB, linkState);
 }
 
 // linkForces + linkTorsion for one link (CrosslinkerSystem.java:62-123, 96-128)
 __global__ void __launch_bounds__(256) k_net_link(const float4 *__restrict__ pack,
                                                   const int *__restrict__ linkFilA, const int *__restrict__ linkFilB,
                                                   const float *__restrict__ loc1, const float *__restrict__ loc2,
-                                                  const int *__restrict__ offA, const int *__restrict__ idxA,
-                                                  const int *__restrict__ offB, const int *__restrict__ idxB,
+                                                  const int *__restrict__ activeLinks,
                                                   const int *__restrict__ linkState,
                                                   const int *__restrict__ linkOrientSame,
                                                   float *__restrict__ torqueMagHist, int *__restrict__ torqueMagPlace,
@@ -325,8 +351,7 @@ __global__ void __launch_bounds__(256) k_net_link(const float4 *__restrict__ pac
     for (int j = 0; j < 12; j++) pay[j] = 0.0f;
     int a = linkFilA[k], b = linkFilB[k];
     if (a >= 0 && b >= 0) {
-        int ctA = active_links(a, offA, idxA, offB, idxB, linkState);
-        int ctB = active_links(b, offA, idxA, offB, idxB, linkState);
+        const int ctA = activeLinks[a], ctB = activeLinks[b];
         LinkEnd A = link_end_packed(a, pack), B = link_end_packed(b, pack);
         link_force_one(A, B, loc1[k], loc2[k], ctA, ctB, lp, pay);
         if (lp.torsionOn && linkState[k] >= 0)
@@ -1232,14 +1257,14 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
             hipLaunchKernelGGL(k_net_unbind, dim3(blocks_for(L, 256)), dim3(256), 0, s, net->linkPack,
                                net->xl.linkFilA, net->xl.linkFilB, net->xl.loc1, net->xl.loc2, net->xl.linkState,
                                net->xl.strainHist, net->xl.strainPlace, net->lp, net->dStep, stepOff, net->p.seed,
-                               net->p.rng, net->p.stream, n, L);
+                               net->p.rng, net->p.stream, net->xl.activeLinkCount, n, L);
         }
         {
             Scope sc(net, T_LINK);
             hipLaunchKernelGGL(k_net_link, dim3(blocks_for(L, 256)), dim3(256), 0, s, net->linkPack,
                                net->xl.linkFilA, net->xl.linkFilB, net->xl.loc1, net->xl.loc2,
-                               net->offA, net->idxA, net->offB, net->idxB, net->xl.linkState,
-                               net->xl.linkOrientSame, net->xl.torqueMagHist,
+                               net->xl.activeLinkCount, net->xl.linkState, net->xl.linkOrientSame,
+                               net->xl.torqueMagHist,
                                net->xl.torqueMagPlace, net->xl.xlinkData, net->lp, n, L);
         }
         r = check_launch(c, "link kernels");
@@ -1288,7 +1313,8 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
     if (ovl && skinOn && (r = check_hip(c, hipStreamWaitEvent(s, net->evBuilt, 0), "wait evBuilt"))) return r;
     {
         Scope sc(net, T_ROD);
-        hipLaunchKernelGGL(k_rod, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, net->cp);
+        const long long waves = ((long long)n + CHAIN_SPAN - 1) / CHAIN_SPAN;
+        hipLaunchKernelGGL(k_rod, dim3(blocks_for(waves * 64, 256)), dim3(256), 0, s, a, net->cp);
     }
     if (ovl && (r = check_hip(c, hipEventRecord(net->evRod[cb], s), "record evRod"))) return r;
     net->cur = nxt;
@@ -1610,6 +1636,12 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
         }
         net->derivedValid = 1;
     }
+    if (net->L > 0 && !net->countsValid) {  // creation, publish or a topology change: count afresh
+        hipLaunchKernelGGL(k_link_counts, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream, net->offA,
+                           net->idxA, net->offB, net->idxB, net->xl.linkState, net->xl.activeLinkCount, net->n);
+        if ((r = check_launch(c, "k_link_counts"))) return r;
+        net->countsValid = 1;
+    }
     if (net->useGraph && !net->profiling) {
         // one captured graph = graphLen steps (even: the pose ping-pong returns to its start buffers) +
         // step advance; a longer graph drains the stream pipeline less often
@@ -1684,8 +1716,10 @@ int sbx_net_publish(sbx_net *net) {
                               "publish yVec");
         if (!r) net->cur = 0;
     }
-    // the caller may now edit its pose arrays: re-bin, re-pack and (skin mode) rebuild next step
+    // the caller may now edit its pose and link arrays: re-bin, re-pack, recount the active links and
+    // (skin mode) rebuild next step
     net->derivedValid = 0;
+    net->countsValid = 0;
     if (!r && net->skinFlags) {
         r = fill_i32(c, net->skinFlags + skf_rebuild(net->cur), 1, 1);
         if (!r) r = fill_i32(c, net->skinFlags + skf_rebuild(1 - net->cur), 0, 1);
@@ -1696,6 +1730,7 @@ int sbx_net_publish(sbx_net *net) {
 int sbx_net_topology_changed(sbx_net *net) {
     if (!net) return SBX_ERR_ARG;
     SBX_CHECK_CTX(net->ctx);
+    net->countsValid = 0;
     return build_link_csr(net);
 }
 

@@ -68,6 +68,11 @@ class Network:
     def publish(self) -> None:
         self.ctx.check(self.ctx.lib.sbx_net_publish(self.h))
 
+    def topology_changed(self) -> None:
+        """After the caller edited linkFilA/linkFilB or linkState of the device store between steps
+        (a formation pass): the crosslink CSR is rebuilt and the active links are recounted."""
+        self.ctx.check(self.ctx.lib.sbx_net_topology_changed(self.h))
+
     def use_graph(self, on: bool | int = True) -> None:
         """True (or 2): replay 2 captured steps at a time; False: eager."""
         self.ctx.check(self.ctx.lib.sbx_net_use_graph(self.h, int(on) if not isinstance(on, bool) else int(on)))

@@ -64,6 +64,41 @@ def test_network_steps_match_oracle(ctx, periodic, rng):
     net.close()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_link_edits_between_steps_are_recounted(ctx, graph):
+    """The step keeps each segment's active-link count itself (k_rod's tally less this step's deaths);
+    links killed and revived by the caller between steps (a formation pass, say) reach the counts
+    through sbx_net_topology_changed.  Half the active links die and dead ones come back, so many
+    segments go between 1, 2 and 3 links and the link forces' 0.4 / max(ctA, ctB, 1) shows it."""
+    import torch
+
+    from softbox_amd.network import Network
+
+    sc = small_network(seed=8, n_fil=150, links=600)
+    orc = ffi.OracleNet(sc)
+    net = Network(ctx, sc)
+    if graph:
+        net.use_graph(2)
+    for t in range(6):
+        orc.step(t, 1)
+    net.step(0, 6)
+    ctx.sync()
+    st = orc.xl.linkState.copy()
+    assert_same(net.xl.linkState.cpu().numpy(), st, "linkState before the edit")
+    alive, dead = np.flatnonzero(st >= 0), np.flatnonzero(st < 0)
+    st[alive[::2]] = -1
+    st[dead[: len(dead) // 2]] = 0
+    assert (st != orc.xl.linkState).sum() > 50
+    orc.xl.linkState[:] = st
+    net.xl.linkState.copy_(torch.from_numpy(st))
+    net.topology_changed()
+    for t in range(6, 12):
+        orc.step(t, 1)
+    net.step(6, 6)
+    _compare(net, orc, sc, f"after a caller edit of linkState (graph={graph})")
+    net.close()
+
+
 def test_network_box_mode_with_walls(ctx):
     """Reference parity mode: clamped grid + containment box; rods start partly outside the walls."""
     from softbox_amd.network import Network
