@@ -396,6 +396,38 @@ int sbx_net_use_graph(sbx_net *net, int on);
  * step); 0: one stream.  Results are identical; sbx_net_step's work is complete on the context's
  * stream either way.  Off while per-kernel timers are on. */
 int sbx_net_set_overlap(sbx_net *net, int on);
+/* Launch shape of the step: broad-phase lanes per body (2 or 4), scan keys per thread (4, 8 or 16: scan
+ * tiles of 256 * scanItems cells) and k_rod block size (64, 128 or 256).  No counterpart in the reference;
+ * results do not depend on it. */
+int sbx_net_set_shape(sbx_net *net, int bpGroup, int scanItems, int rodThreads);
+/* Autotuner (no counterpart in the reference): times trialSteps steps (after 2 warm-up steps) for each
+ * launch shape and keeps the fastest.  The steps are real steps step0, step0+1, ... of the run (results
+ * do not depend on the shape): *advanced = steps taken; shape[3] = {bpGroup, scanItems, rodThreads} and
+ * *msPerStep (may be NULL) report the choice. */
+int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *shape, float *msPerStep);
+
+/* ------------------------------------------------------------------ batched step
+ * No counterpart in the reference (its harnesses step one network): nRep independent networks of uneven
+ * size, concatenated in HBM and advanced together, one launch per stage for all of them (and one
+ * captured graph with sbx_net_use_graph on the handle of sbx_batch_net).  Rods, links and grid cells of
+ * replica r start at its rodBase / linkBase / cellBase (sbx_batch_layout): rod and link ids in the
+ * batch's topology, CSR and candidate slices are offset by rodBase.  Each replica is binned in its own
+ * grid and draws its noise and unbinding events under its own seed and own rod / link indices, so
+ * every replica's state is bit-identical to stepping it alone with sbx_net_step.  The replicas must share
+ * the rod physics (params, chainParams, chainOn, maxCand, actinRadius, rng, stream) and, among those with
+ * crosslinks, the link physics (xlParams, offParams, torsionParams, unbindOn); grid, box and seed may
+ * differ; containment and motors are not batched.
+ * The replicas' stores are copied in at creation and written back by sbx_batch_publish. */
+typedef struct sbx_batch sbx_batch;
+int sbx_batch_create(sbx_ctx *c, int nRep, const sbx_filaments *fil, const sbx_crosslinks *xl /* [nRep] or NULL */,
+                     const sbx_net_params *p, sbx_batch **out);
+void sbx_batch_destroy(sbx_batch *b);
+int sbx_batch_step(sbx_batch *b, int step0, int nSteps);
+/* copy every replica's pose, derived geometry, accumulated F/T and crosslink state into its own stores */
+int sbx_batch_publish(sbx_batch *b);
+/* the network handle of the concatenated replicas (graphs, overlap, shape, tuning, timers, candidates) */
+int sbx_batch_net(sbx_batch *b, sbx_net **net);
+int sbx_batch_layout(sbx_batch *b, int rep, int *rodBase, int *linkBase, int *cellBase);
 
 #ifdef __cplusplus
 }

@@ -175,6 +175,15 @@ _SIGS = {
     "sbx_net_kernel_times": [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_longlong), i32],
     "sbx_net_use_graph": [vp, i32],
     "sbx_net_set_overlap": [vp, i32],
+    "sbx_net_set_shape": [vp, i32, i32, i32],
+    "sbx_net_tune": [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), f32p],
+    "sbx_batch_create": [vp, i32, C.POINTER(SbxFilaments), C.POINTER(SbxCrosslinks), C.POINTER(SbxNetParams),
+                         C.POINTER(vp)],
+    "sbx_batch_destroy": [vp],
+    "sbx_batch_step": [vp, i32, i32],
+    "sbx_batch_publish": [vp],
+    "sbx_batch_net": [vp, C.POINTER(vp)],
+    "sbx_batch_layout": [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)],
     "sbx_net_set_skin": [vp, i32, C.POINTER(SbxSkinParams), vp],
     "sbx_net_skin_stats": [vp, C.POINTER(C.c_longlong), i32],
     "sbx_net_skin_state": [vp] + [C.POINTER(vp)] * 5 + [C.POINTER(i32), f32p],
@@ -189,7 +198,7 @@ _SIGS = {
     "sbx_mlp_adam": [vp, vp, vp, vp, vp, i32, C.c_float, C.c_float, C.c_float, C.c_float],
 }
 _RESTYPES = {"sbx_destroy": None, "sbx_last_error": C.c_char_p, "sbx_clear_error": None,
-             "sbx_version": C.c_char_p, "sbx_net_destroy": None, "sbx_mlp_scratch_floats": C.c_size_t}
+             "sbx_version": C.c_char_p, "sbx_net_destroy": None, "sbx_batch_destroy": None, "sbx_mlp_scratch_floats": C.c_size_t}
 
 EXPORTED = tuple(_SIGS)
 

@@ -52,8 +52,7 @@ int fill_i32(sbx_ctx *c, int *dst, int value, long long count) {
 // ------------------------------------------------------------------ scan
 // Tile = 256 threads x 8 items.  Wave64 inclusive scan by __shfl_up, then wave totals in LDS.
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 8;
-constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+constexpr int SCAN_MIN_TILE = SCAN_THREADS * 4;  // smallest tile (scanItems 4): sizes the tile sums
 
 __device__ __forceinline__ int wave_inclusive_scan(int v) {
     const int lane = threadIdx.x & 63;
@@ -94,11 +93,13 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *total) {
 // prefix is the sum of the tile sums before it -- a block-wide reduction over at most n/2048
 // L2-resident ints -- so no tile waits on another.  out[n] = grand total, clamped to clampTotal
 // when >= 0.
+template <int SCAN_ITEMS>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int *__restrict__ in, int n,
                                                              const int *__restrict__ tileSums, int nTiles,
                                                              int *__restrict__ out, int clampTotal,
                                                              const int *__restrict__ gate) {
     if (gate && *gate == 0) return;
+    constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
     __shared__ int red[SCAN_THREADS / 64];
     const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -108,9 +109,11 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int *__restri
     int v[SCAN_ITEMS];
     if ((long long)(tile + 1) * SCAN_TILE <= n) {
         const int4 *p = reinterpret_cast<const int4 *>(in + base);
-        int4 a = p[0], b = p[1];
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS / 4; q++) {
+            const int4 a = p[q];
+            v[4 * q] = a.x; v[4 * q + 1] = a.y; v[4 * q + 2] = a.z; v[4 * q + 3] = a.w;
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < SCAN_ITEMS; k++) v[k] = (base + k < n) ? in[base + k] : 0;
@@ -153,7 +156,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int *__restri
 }
 
 int sort_scratch_reserve(sbx_ctx *c, SortScratch &s, long long nKeys, long long nItems) {
-    long long nt = (nKeys + SCAN_TILE - 1) / SCAN_TILE + 1;
+    long long nt = (nKeys + SCAN_MIN_TILE - 1) / SCAN_MIN_TILE + 1;
     if (nKeys > s.keyCap || nt > s.tileCap) {
         if (s.counts) hipFree(s.counts);
         if (s.tileSums) hipFree(s.tileSums);
@@ -219,17 +222,22 @@ __global__ void __launch_bounds__(256) k_key_count(const int *__restrict__ key, 
 
 // tileSums[t] = sum of tile t of counts (one block per tile; no atomics: device-scope atomics on a
 // few hot addresses serialise at the memory side)
+template <int SCAN_ITEMS>
 __global__ void __launch_bounds__(SCAN_THREADS) k_tile_reduce(const int *__restrict__ in, int n,
                                                               int *__restrict__ tileSums,
                                                               const int *__restrict__ gate) {
     if (gate && *gate == 0) return;
+    constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
     __shared__ int red[SCAN_THREADS / 64];
     long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
     int sum = 0;
     if ((long long)(blockIdx.x + 1) * SCAN_TILE <= n) {
         const int4 *p = reinterpret_cast<const int4 *>(in + base);
-        int4 a = p[0], b = p[1];
-        sum = a.x + a.y + a.z + a.w + b.x + b.y + b.z + b.w;
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS / 4; q++) {
+            const int4 a = p[q];
+            sum += a.x + a.y + a.z + a.w;
+        }
     } else {
         for (int k = 0; k < SCAN_ITEMS; k++) sum += (base + k < n) ? in[base + k] : 0;
     }
@@ -279,6 +287,24 @@ __global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, 
     }
 }
 
+// offsets = exclusive scan of counts[nKeys] (tile sums, then the tiles): scanItems keys per thread
+template <int ITEMS>
+void launch_scan_items(const int *counts, int nKeys, int *tileSums, int *offsets, int clampTotal, const int *gate,
+                       hipStream_t q) {
+    int nt = (nKeys + SCAN_THREADS * ITEMS - 1) / (SCAN_THREADS * ITEMS);
+    if (nt < 1) nt = 1;
+    hipLaunchKernelGGL(k_tile_reduce<ITEMS>, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, tileSums, gate);
+    hipLaunchKernelGGL(k_scan_tiles<ITEMS>, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, tileSums, nt, offsets,
+                       clampTotal, gate);
+}
+
+void launch_scan(const SortScratch &s, const int *counts, int nKeys, int *offsets, int clampTotal, const int *gate,
+                 hipStream_t q) {
+    if (s.scanItems == 4) launch_scan_items<4>(counts, nKeys, s.tileSums, offsets, clampTotal, gate, q);
+    else if (s.scanItems == 16) launch_scan_items<16>(counts, nKeys, s.tileSums, offsets, clampTotal, gate, q);
+    else launch_scan_items<8>(counts, nKeys, s.tileSums, offsets, clampTotal, gate, q);
+}
+
 int count_keys(sbx_ctx *c, const int *key, int nItems, int nKeys, int *counts, int *slot, hipStream_t st) {
     if (nItems <= 0) return SBX_OK;
     hipLaunchKernelGGL(k_key_count, dim3(blocks_for(nItems, 256)), dim3(256), 0, st ? st : c->stream, key, nItems,
@@ -291,11 +317,7 @@ int counting_sort_counted(sbx_ctx *c, SortScratch &s, const int *key, int nItems
     int r = sort_scratch_reserve(c, s, nKeys, nItems);
     if (r) return r;
     hipStream_t q = st ? st : c->stream;
-    int nt = (nKeys + SCAN_TILE - 1) / SCAN_TILE;
-    if (nt < 1) nt = 1;
-    hipLaunchKernelGGL(k_tile_reduce, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, s.tileSums, nullptr);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, s.tileSums, nt, offsets,
-                       contentsCap, nullptr);
+    launch_scan(s, counts, nKeys, offsets, contentsCap, nullptr, q);
     if (nItems > 0) {
         hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
                            offsets, slot, counts, s.tmp, contentsCap, nullptr);
@@ -310,15 +332,11 @@ int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nK
     int r = sort_scratch_reserve(c, s, nKeys, nItems);
     if (r) return r;
     hipStream_t q = st ? st : c->stream;
-    int nt = (nKeys + SCAN_TILE - 1) / SCAN_TILE;
-    if (nt < 1) nt = 1;
     if (nItems > 0) {
         hipLaunchKernelGGL(k_key_count, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
                            s.counts, s.slot, gate);
     }
-    hipLaunchKernelGGL(k_tile_reduce, dim3(nt), dim3(SCAN_THREADS), 0, q, s.counts, nKeys, s.tileSums, gate);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, q, s.counts, nKeys, s.tileSums, nt,
-                       offsets, contentsCap, gate);
+    launch_scan(s, s.counts, nKeys, offsets, contentsCap, gate, q);
     if (nItems > 0) {
         hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
                            offsets, s.slot, s.counts, s.tmp, contentsCap, gate);

@@ -15,6 +15,32 @@ struct GridGeom {
     float iLx, iLy, iLz;  // 1.0f / L (host IEEE division)
 };
 
+// One replica of a batched network step: its grid, and where its cells, rods and links start in the
+// batch's concatenated arrays (cells are numbered replica after replica); seed keys its draws.
+struct RepInfo {
+    GridGeom g;
+    int cellBase, rodBase, linkBase, seed;
+};
+
+// The grid a body is binned in.  SingleGeo: one grid for every body; BatchGeo: the grid of the
+// body's replica, its cells offset by the replica's cell base.
+struct BodyGeo {
+    GridGeom g;
+    int cellBase;
+};
+struct SingleGeo {
+    GridGeom g;
+    __device__ BodyGeo of(int) const { return BodyGeo{g, 0}; }
+};
+struct BatchGeo {
+    const RepInfo *reps;
+    const int *rodRep;
+    __device__ BodyGeo of(int i) const {
+        const RepInfo &r = reps[rodRep[i]];
+        return BodyGeo{r.g, r.cellBase};
+    }
+};
+
 // Java (int) of a float followed by the reference clamp to [0, nBins-1] (SpatialGrid.java:67-72);
 // values below 0 and NaN (Java converts NaN to 0) land on 0.
 __device__ __forceinline__ int clamp_cell(float v, int nBins) {
@@ -135,8 +161,11 @@ int launch_broad_phase(sbx_ctx *c, const float *center, const float *boundingRad
                        int *candPartner, int *candCount, int maxCand, hipStream_t st = nullptr);
 // the same from {centre, bounding radius} and cell laid out in contents order (sortedPose[q] and
 // sortedCell[q] belong to body cellContents[q]; counting_sort's payload)
+// reps == nullptr: every body in grid g; else body i in the grid of replica rodRep[i] (g unused).
+// group: lanes per body, 2 or 4.
 int launch_broad_phase_sorted(sbx_ctx *c, const float4 *sortedPose, const int *sortedCell, const int *cellOffsets,
-                              const int *cellContents, const GridGeom &g, int cap, int S, int *candPartner,
-                              int *candCount, int maxCand, hipStream_t st = nullptr);
+                              const int *cellContents, const GridGeom &g, const RepInfo *reps, const int *rodRep,
+                              int group, int cap, int S, int *candPartner, int *candCount, int maxCand,
+                              hipStream_t st = nullptr);
 
 }  // namespace sbx

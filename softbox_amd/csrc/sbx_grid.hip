@@ -76,13 +76,13 @@ __device__ __forceinline__ bool bp_hit(float4 qi, float4 qj, const GridGeom &g) 
 // (k < U), so a group's loads are contiguous.  Hits (j > i and the test) are ordered by position
 // with a ballot: a hit's slot is the group's count so far plus the hits at earlier positions of the
 // round.  Counts keep going past maxCand (SpatialGrid.java:397-398).
-constexpr int BP_GROUP = 4;  // lanes per body
+constexpr int BP_GROUP = 4;  // lanes per body (stand-alone entry point; the network step chooses 2 or 4)
 constexpr int BP_ROUND = 4;  // positions per lane and round
 
-template <typename Pose, int G, int U>
-__global__ void __launch_bounds__(BP_THREADS) k_broad_phase(Pose pose, const int *__restrict__ cellOffsets,
-                                                            const int *__restrict__ cellContents, GridGeom g,
-                                                            int cap, int nSorted, int *__restrict__ candPartner,
+template <typename Pose, typename Geo, int G, int U>
+__global__ void __launch_bounds__(BP_THREADS) k_broad_phase(Pose pose, Geo geo, const int *__restrict__ cellOffsets,
+                                                            const int *__restrict__ cellContents, int cap,
+                                                            int nSorted, int *__restrict__ candPartner,
                                                             int *__restrict__ candCount, int maxCand) {
     static_assert(64 % G == 0, "groups tile a wave");
     // XCD-aware: each XCD walks one contiguous run of the cell-sorted bodies (a spatial slab), so
@@ -102,8 +102,10 @@ __global__ void __launch_bounds__(BP_THREADS) k_broad_phase(Pose pose, const int
         return;
     }
     const float4 qi = pose.own(t, i);
+    const BodyGeo bg = geo.of(i);  // the body's grid; its cells start at bg.cellBase
+    const GridGeom &g = bg.g;
     int pre[18], shift[18];
-    const int total = stencil_map(c, g, cellOffsets, pre, shift);
+    const int total = stencil_map(c - bg.cellBase, g, cellOffsets + bg.cellBase, pre, shift);
     int *slice = candPartner + (size_t)i * (size_t)maxCand;
     const unsigned long long groupBits = G == 64 ? ~0ull : ((1ull << G) - 1) << (grp * G);
     const unsigned long long below = (1ull << lane) - 1;
@@ -177,21 +179,41 @@ int launch_broad_phase(sbx_ctx *c, const float *center, const float *boundingRad
                        int *candPartner, int *candCount, int maxCand, hipStream_t st) {
     // bodies with a cell are exactly the first S (counting sort places each live body once)
     PlanarPose pose{center, boundingRadius, bodyCell, cap};
-    auto kern = k_broad_phase<PlanarPose, BP_GROUP, BP_ROUND>;
+    auto kern = k_broad_phase<PlanarPose, SingleGeo, BP_GROUP, BP_ROUND>;
     hipLaunchKernelGGL(kern, dim3(blocks_for((long long)cap * BP_GROUP, BP_THREADS)), dim3(BP_THREADS), 0,
-                       st ? st : c->stream, pose, cellOffsets, cellContents, g, cap, S, candPartner, candCount,
-                       maxCand);
+                       st ? st : c->stream, pose, SingleGeo{g}, cellOffsets, cellContents, cap, S, candPartner,
+                       candCount, maxCand);
     return check_launch(c, "k_broad_phase");
 }
 
+template <typename Geo, int G>
+void launch_sorted(sbx_ctx *c, SortedPose pose, Geo geo, const int *cellOffsets, const int *cellContents, int cap,
+                   int S, int *candPartner, int *candCount, int maxCand, hipStream_t st) {
+    auto kern = k_broad_phase<SortedPose, Geo, G, BP_ROUND>;
+    hipLaunchKernelGGL(kern, dim3(blocks_for((long long)cap * G, BP_THREADS)), dim3(BP_THREADS), 0, st, pose, geo,
+                       cellOffsets, cellContents, cap, S, candPartner, candCount, maxCand);
+}
+
 int launch_broad_phase_sorted(sbx_ctx *c, const float4 *sortedPose, const int *sortedCell, const int *cellOffsets,
-                              const int *cellContents, const GridGeom &g, int cap, int S, int *candPartner,
-                              int *candCount, int maxCand, hipStream_t st) {
+                              const int *cellContents, const GridGeom &g, const RepInfo *reps, const int *rodRep,
+                              int group, int cap, int S, int *candPartner, int *candCount, int maxCand,
+                              hipStream_t st) {
+    SBX_REQUIRE(c, group == 2 || group == 4, "broad phase: %d lanes per body (2 or 4)", group);
     SortedPose pose{sortedPose, sortedCell};
-    auto kern = k_broad_phase<SortedPose, BP_GROUP, BP_ROUND>;
-    hipLaunchKernelGGL(kern, dim3(blocks_for((long long)cap * BP_GROUP, BP_THREADS)), dim3(BP_THREADS), 0,
-                       st ? st : c->stream, pose, cellOffsets, cellContents, g, cap, S, candPartner, candCount,
-                       maxCand);
+    hipStream_t q = st ? st : c->stream;
+    if (reps) {
+        const BatchGeo geo{reps, rodRep};
+        if (group == 4) launch_sorted<BatchGeo, 4>(c, pose, geo, cellOffsets, cellContents, cap, S, candPartner,
+                                                   candCount, maxCand, q);
+        else launch_sorted<BatchGeo, 2>(c, pose, geo, cellOffsets, cellContents, cap, S, candPartner, candCount,
+                                        maxCand, q);
+    } else {
+        const SingleGeo geo{g};
+        if (group == 4) launch_sorted<SingleGeo, 4>(c, pose, geo, cellOffsets, cellContents, cap, S, candPartner,
+                                                    candCount, maxCand, q);
+        else launch_sorted<SingleGeo, 2>(c, pose, geo, cellOffsets, cellContents, cap, S, candPartner, candCount,
+                                         maxCand, q);
+    }
     return check_launch(c, "k_broad_phase");
 }
 

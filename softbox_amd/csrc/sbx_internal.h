@@ -15,6 +15,7 @@ struct SortScratch {
     int *counts = nullptr, *tileSums = nullptr, *tmp = nullptr, *slot = nullptr;
     long long keyCap = 0, itemCap = 0, tileCap = 0;
     bool clean = false;
+    int scanItems = 8;  // keys per scan thread: tiles of 256 * scanItems keys (4, 8 or 16; same result)
 };
 struct ChainParams;
 }  // namespace sbx

@@ -60,8 +60,10 @@ __device__ __forceinline__ void link_points(const LinkEnd &A, const LinkEnd &B, 
 }
 
 // CrosslinkerSystem.unbind body (CrosslinkerSystem.java:245-279) for one ACTIVE link; returns the
-// new linkState (unchanged or -1) and updates the strain ring in place.
-__device__ __forceinline__ int link_unbind_one(int k, const LinkEnd &A, const LinkEnd &B, float l1, float l2,
+// new linkState (unchanged or -1) and updates the strain ring in place.  k indexes the rings, key is
+// the link index the draw is keyed by (the same, except in a batch: the replica's own link index).
+__device__ __forceinline__ int link_unbind_one(int k, uint32_t key, const LinkEnd &A, const LinkEnd &B, float l1,
+                                               float l2,
                                                float *__restrict__ strainHist, int *__restrict__ strainPlace,
                                                const LinkParams &lp, int step, int seed, int rng, uint32_t stream,
                                                int state) {
@@ -82,7 +84,7 @@ __device__ __forceinline__ int link_unbind_one(int k, const LinkEnd &A, const Li
     ave /= STRAIN_WIN;
     double kOff = lp.offConst + lp.offCoeff * exp(ave * lp.offExp);
     double pBreak = kOff * lp.offDt;
-    uint32_t h = event_bits(rng, (uint32_t)k, step, seed, stream, 0x584C4B42u);
+    uint32_t h = event_bits(rng, key, step, seed, stream, 0x584C4B42u);
     float u = uniform01(h);
     return ((double)u < pBreak) ? -1 : state;
 }

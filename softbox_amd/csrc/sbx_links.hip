@@ -57,7 +57,8 @@ __global__ void __launch_bounds__(256) k_link_unbind(const float *__restrict__ u
     if (st < 0) return;
     LinkEnd A = link_end_planar(linkFilA[k], nSeg, nullptr, uVec, end1, nullptr);
     LinkEnd B = link_end_planar(linkFilB[k], nSeg, nullptr, uVec, end1, nullptr);
-    int ns = link_unbind_one(k, A, B, loc1[k], loc2[k], strainHist, strainPlace, lp, step, seed, rng, stream, st);
+    int ns = link_unbind_one(k, (uint32_t)k, A, B, loc1[k], loc2[k], strainHist, strainPlace, lp, step, seed, rng,
+                             stream, st);
     if (ns != st) linkState[k] = ns;
 }
 

@@ -14,6 +14,7 @@
 // The crosslink CSR (keyed by linkFilA / linkFilB) depends only on the link topology, so it is built
 // at creation and on sbx_net_topology_changed instead of every step; linkState (which unbinding
 // changes every step) is applied by the gather guard, as in CrosslinkerSystem.segGatherA/B.
+#include <cstring>
 #include <new>
 #include <string>
 #include <vector>
@@ -114,6 +115,13 @@ struct sbx_net {
     // pose (per-step rebuild only); the next step's sort consumes and re-zeroes them
     int *gridCounts[2] = {nullptr, nullptr}, *gridSlot[2] = {nullptr, nullptr};
     int overlap = 1;
+    // batched step (sbx_batch_create): replicas concatenated in this network's arrays
+    int R = 1;
+    RepInfo *reps = nullptr;                          // device table [R]
+    int *rodRep = nullptr, *linkRep = nullptr;        // replica of each rod / link
+    // launch shape (results do not depend on it; sbx_net_tune picks it by timing steps)
+    int bpGroup = 4;      // broad-phase lanes per body (2 or 4)
+    int rodThreads = 256; // k_rod block size (64, 128 or 256)
     // graph
     int useGraph = 0;
     int graphSteps = 0;  // non-zero while a capture is being recorded
@@ -157,6 +165,8 @@ struct RodArgs {
     float4 *bodyPack;             // {new centre, bounding radius} for the next step's broad phase (or null)
     const float *boundR;
     GridGeom geom;
+    const RepInfo *reps;  // batched step: replica table and each rod's replica (null otherwise)
+    const int *rodRep;
     const int *dStep;
     int stepOff;
     int n, chainOn, hasLinks, confineOn, checkInt, rng, seed;
@@ -169,10 +179,13 @@ struct RodArgs {
 // forceSum = 0 -> (float)(f + chain) -> (float)(f + sumA) -> (float)(f + sumB) -> confine -> integrate.
 // Rods are laid out CHAIN_SPAN per wave (rod = 63 * wave + lane - 1) so each chain joint is evaluated
 // once per rod; lane 0 of a wave only supplies the joint at the end2 of the previous wave's last rod.
+// BATCH: the rods of many replicas, each binned in its replica's grid and drawing its noise under the
+// replica's seed and own rod index (so every replica steps exactly as it would alone).
+template <bool BATCH>
 __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
     const int n = a.n;
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (wave * CHAIN_SPAN - 1 >= n) return;  // the whole wave is past the last rod
     const int i = wave * CHAIN_SPAN + lane - 1;
     const bool live = i >= 0 && i < n;
@@ -202,6 +215,16 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
     }
     if (!own) return;
     const int step = *a.dStep + a.stepOff;
+    uint32_t body = (uint32_t)i;
+    int seed = a.seed, cellBase = 0;
+    GridGeom geom = a.geom;
+    if constexpr (BATCH) {
+        const RepInfo &R = a.reps[a.rodRep[i]];
+        body = (uint32_t)(i - R.rodBase);
+        seed = R.seed;
+        cellBase = R.cellBase;
+        geom = R.g;
+    }
     if (a.hasLinks) {
         double fa[3] = {0, 0, 0}, ta[3] = {0, 0, 0};
         int active = 0;
@@ -239,7 +262,7 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         }
     }
     float gf[3], gt[3], rf[3], rt[3];
-    brownian_noise(a.rng, (uint32_t)i, step, a.seed, a.stream, gf, gt);
+    brownian_noise(a.rng, body, step, seed, a.stream, gf, gt);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         rf[k] = brown_amp(tS, a.bfm, gT[k], gf[k]);
@@ -257,7 +280,7 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         a.end2[k * n + i] = d.e2[k];
     }
     if (a.nextCell) {
-        const int cell = cell_of(p.c[0], p.c[1], p.c[2], a.geom);
+        const int cell = cellBase + cell_of(p.c[0], p.c[1], p.c[2], geom);
         a.nextCell[i] = cell;
         if (a.nextCounts) a.nextSlot[i] = atomicAdd(&a.nextCounts[cell], 1);
     }
@@ -280,6 +303,16 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
     }
 }
 
+// cells of the caller's pose in a batch: rod i in the grid of its replica
+__global__ void __launch_bounds__(256) k_body_cell_batch(const float *__restrict__ coord, const RepInfo *__restrict__ reps,
+                                                       const int *__restrict__ rodRep, int *__restrict__ bodyCell,
+                                                       int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const RepInfo &R = reps[rodRep[i]];
+    bodyCell[i] = R.cellBase + cell_of(coord[i], coord[n + i], coord[2 * n + i], R.g);
+}
+
 // bodyPack of the caller's pose (before the first step and after sbx_net_publish)
 __global__ void __launch_bounds__(256) k_pack_bodies(const float *__restrict__ coord, const float *__restrict__ boundR,
                                                    float4 *__restrict__ pack, int n) {
@@ -300,6 +333,8 @@ __global__ void __launch_bounds__(256) k_pack_links(const float *__restrict__ co
     q[2] = make_float4(end1[i], end1[n + i], end1[2 * n + i], 0.0f);
 }
 
+// BATCH: each link draws under its replica's seed and own link index
+template <bool BATCH>
 __global__ void __launch_bounds__(256) k_net_unbind(const float4 *__restrict__ pack,
                                                     const int *__restrict__ linkFilA,
                                                     const int *__restrict__ linkFilB, const float *__restrict__ loc1,
@@ -307,7 +342,8 @@ __global__ void __launch_bounds__(256) k_net_unbind(const float4 *__restrict__ p
                                                     float *__restrict__ strainHist, int *__restrict__ strainPlace,
                                                     LinkParams lp, const int *__restrict__ dStep, int stepOff,
                                                     int seed, int rng, uint32_t stream, int *__restrict__ activeLinks,
-                                                    int nSeg, int nLinks) {
+                                                    const RepInfo *__restrict__ reps,
+                                                    const int *__restrict__ linkRep, int nSeg, int nLinks) {
     int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= nLinks) return;
     int st = linkState[k];
@@ -315,7 +351,14 @@ __global__ void __launch_bounds__(256) k_net_unbind(const float4 *__restrict__ p
     int step = *dStep + stepOff;
     const int a = linkFilA[k], b = linkFilB[k];
     LinkEnd A = link_end_packed(a, pack), B = link_end_packed(b, pack);
-    int ns = link_unbind_one(k, A, B, loc1[k], loc2[k], strainHist, strainPlace, lp, step, seed, rng, stream, st);
+    uint32_t key = (uint32_t)k;
+    if constexpr (BATCH) {
+        const RepInfo &R = reps[linkRep[k]];
+        key = (uint32_t)(k - R.linkBase);
+        seed = R.seed;
+    }
+    int ns = link_unbind_one(k, key, A, B, loc1[k], loc2[k], strainHist, strainPlace, lp, step, seed, rng, stream,
+                             st);
     if (ns != st) {
         linkState[k] = ns;
         if (ns < 0) {  // death: the link no longer counts at either end (CSR A and CSR B each held it)
@@ -1241,8 +1284,8 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
                 if (ovl && (r = check_hip(c, hipStreamWaitEvent(s2, net->evPlaced[cb], 0), "wait evPlaced")))
                     return r;
                 r = launch_broad_phase_sorted(c, net->sortedPose[cb], net->sortedCell[cb], net->csrOff[cb],
-                                              net->csrCont[cb], net->geom, n, n, net->candPartner, net->candCount,
-                                              net->p.maxCand, s2);
+                                              net->csrCont[cb], net->geom, net->reps, net->rodRep, net->bpGroup, n,
+                                              n, net->candPartner, net->candCount, net->p.maxCand, s2);
                 if (r) return r;
                 if (ovl && (r = check_hip(c, hipEventRecord(net->evJoin[cb], s2), "record evJoin"))) return r;
             }
@@ -1254,10 +1297,11 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
     if (L > 0) {
         if (net->p.unbindOn) {
             Scope sc(net, T_UNBIND);
-            hipLaunchKernelGGL(k_net_unbind, dim3(blocks_for(L, 256)), dim3(256), 0, s, net->linkPack,
-                               net->xl.linkFilA, net->xl.linkFilB, net->xl.loc1, net->xl.loc2, net->xl.linkState,
-                               net->xl.strainHist, net->xl.strainPlace, net->lp, net->dStep, stepOff, net->p.seed,
-                               net->p.rng, net->p.stream, net->xl.activeLinkCount, n, L);
+            auto kern = net->reps ? k_net_unbind<true> : k_net_unbind<false>;
+            hipLaunchKernelGGL(kern, dim3(blocks_for(L, 256)), dim3(256), 0, s, net->linkPack, net->xl.linkFilA,
+                               net->xl.linkFilB, net->xl.loc1, net->xl.loc2, net->xl.linkState, net->xl.strainHist,
+                               net->xl.strainPlace, net->lp, net->dStep, stepOff, net->p.seed, net->p.rng,
+                               net->p.stream, net->xl.activeLinkCount, net->reps, net->linkRep, n, L);
         }
         {
             Scope sc(net, T_LINK);
@@ -1285,6 +1329,8 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
     a.hasMotors = net->hasMotors;
     a.activeLinkCount = net->xl.activeLinkCount;
     a.geom = net->geom;
+    a.reps = net->reps;
+    a.rodRep = net->rodRep;
     a.nextCell = (net->p.gridOn && !net->hasMotors) ? net->cellNext : nullptr;
     a.nextCounts = (a.nextCell && !skinOn) ? net->gridCounts[nxt] : nullptr;
     a.nextSlot = a.nextCounts ? net->gridSlot[nxt] : nullptr;
@@ -1314,7 +1360,9 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
     {
         Scope sc(net, T_ROD);
         const long long waves = ((long long)n + CHAIN_SPAN - 1) / CHAIN_SPAN;
-        hipLaunchKernelGGL(k_rod, dim3(blocks_for(waves * 64, 256)), dim3(256), 0, s, a, net->cp);
+        auto kern = net->reps ? k_rod<true> : k_rod<false>;
+        hipLaunchKernelGGL(kern, dim3(blocks_for(waves * 64, net->rodThreads)), dim3(net->rodThreads), 0, s, a,
+                           net->cp);
     }
     if (ovl && (r = check_hip(c, hipEventRecord(net->evRod[cb], s), "record evRod"))) return r;
     net->cur = nxt;
@@ -1351,7 +1399,8 @@ void release(sbx_net *net) {
                     net->cellSkin, net->cellHorizon, net->vlist,  net->vcount,  net->skinFlags, net->model,
                     net->cgDev,    net->cCellsDev, net->bodyPack[0], net->bodyPack[1], net->sortedPose[0],
                     net->sortedPose[1], net->sortedCell[0], net->sortedCell[1], net->csrOff[1], net->csrCont[1],
-                    net->gridCounts[0], net->gridCounts[1], net->gridSlot[0], net->gridSlot[1]};
+                    net->gridCounts[0], net->gridCounts[1], net->gridSlot[0], net->gridSlot[1],
+                    net->reps,     net->rodRep,    net->linkRep};
     for (void *b : bufs)
         if (b) hipFree(b);
     sort_scratch_free(net->sort);
@@ -1373,12 +1422,17 @@ int alloc(sbx_ctx *c, P **p, size_t count) {
     return check_hip(c, hipMalloc((void **)p, sizeof(P) * (count ? count : 1)), "sbx_net alloc");
 }
 
-}  // namespace
+// replicas of a batch in the concatenated stores: one RepInfo per replica, the replica of every rod
+// and every link
+struct BatchLayout {
+    std::vector<RepInfo> reps;
+    std::vector<int> rodRep, linkRep;
+};
 
-extern "C" {
-
-int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *xl, const sbx_net_params *p,
-                   sbx_net **out) {
+// A network over caller-owned stores; batch != nullptr: the stores are the concatenated replicas of
+// a batch (each replica's grid already checked), laid out as batch describes.
+int net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *xl, const sbx_net_params *p,
+               const BatchLayout *batch, sbx_net **out) {
     SBX_CHECK_CTX(c);
     SBX_REQUIRE(c, fil && p && out, "net_create: null argument");
     *out = nullptr;
@@ -1405,7 +1459,7 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
     }
     if (p->confineOn) SBX_REQUIRE(c, p->boxParams[0] != 0.0f, "net_create: boxParams tau must be non-zero");
     if (p->gridOn) {
-        int r = check_grid(c, p->gridParams, p->gridDims, p->periodic);
+        int r = batch ? SBX_OK : check_grid(c, p->gridParams, p->gridDims, p->periodic);
         if (r) return r;
         SBX_REQUIRE(c, p->maxCand > 0 && p->maxCand <= SBX_MAX_CAND, "net_create: maxCand must be in [1, %d]",
                     SBX_MAX_CAND);
@@ -1433,7 +1487,8 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
         net->yB[1] = net->ownPose + 6 * (size_t)n;
         if ((r = alloc(c, &net->dStep, 1))) break;
         if (p->gridOn) {
-            net->geom = grid_geom(p->gridParams, p->gridDims, p->periodic, p->periodic ? p->boxLen : nullptr);
+            net->geom = batch ? batch->reps[0].g
+                              : grid_geom(p->gridParams, p->gridDims, p->periodic, p->periodic ? p->boxLen : nullptr);
             if ((r = alloc(c, &net->boundR, n))) break;
             if ((r = alloc(c, &net->bodyCell, n))) break;
             if ((r = alloc(c, &net->cellNext, n))) break;
@@ -1478,6 +1533,20 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
             if ((r = alloc(c, &net->idxB, L))) break;
             if ((r = build_link_csr(net))) break;
         }
+        if (batch) {
+            net->R = (int)batch->reps.size();
+            if ((r = alloc(c, &net->reps, batch->reps.size()))) break;
+            if ((r = alloc(c, &net->rodRep, (size_t)n))) break;
+            if ((r = alloc(c, &net->linkRep, (size_t)(L > 0 ? L : 1)))) break;
+            r = check_hip(c, hipMemcpy(net->reps, batch->reps.data(), sizeof(RepInfo) * batch->reps.size(),
+                                       hipMemcpyHostToDevice), "upload replicas");
+            if (!r) r = check_hip(c, hipMemcpy(net->rodRep, batch->rodRep.data(), sizeof(int) * (size_t)n,
+                                               hipMemcpyHostToDevice), "upload rod replicas");
+            if (!r && L > 0)
+                r = check_hip(c, hipMemcpy(net->linkRep, batch->linkRep.data(), sizeof(int) * (size_t)L,
+                                           hipMemcpyHostToDevice), "upload link replicas");
+            if (r) break;
+        }
         // reserve the sort scratch now: nothing may allocate inside a captured step
         long long keys = p->gridOn ? (long long)p->gridDims[3] : 1;
         if (keys < n) keys = n;
@@ -1492,6 +1561,15 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
     return SBX_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *xl, const sbx_net_params *p,
+                   sbx_net **out) {
+    return net_create(c, fil, xl, p, nullptr, out);
+}
+
 void sbx_net_destroy(sbx_net *net) { release(net); }
 
 int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_params *p) {
@@ -1500,6 +1578,7 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
     SBX_CHECK_CTX(c);
     SBX_REQUIRE(c, m && p, "net_attach_motors: null argument");
     SBX_REQUIRE(c, !net->hasMotors, "net_attach_motors: motors already attached");
+    SBX_REQUIRE(c, !net->reps, "net_attach_motors: a batch steps filament networks only");
     SBX_REQUIRE(c, net->p.gridOn && !net->p.periodic,
                 "net_attach_motors: the binding grid needs gridOn with the clamped (non-periodic) grid");
     const int n = net->n, nM = m->nM;
@@ -1618,7 +1697,13 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
             // (sbx_net_publish moves the pose to buffer 0), and the caller may have moved bodies
             for (int b = 0; b < 2; b++)
                 if ((r = fill_i32(c, net->gridCounts[b], 0, (long long)nk + 1))) return r;
-            if ((r = launch_body_cell(c, net->coordB[net->cur], net->n, net->n, net->geom, net->bodyCell))) return r;
+            if (net->reps) {
+                hipLaunchKernelGGL(k_body_cell_batch, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream,
+                                   net->coordB[net->cur], net->reps, net->rodRep, net->bodyCell, net->n);
+                if ((r = check_launch(c, "k_body_cell_batch"))) return r;
+            } else if ((r = launch_body_cell(c, net->coordB[net->cur], net->n, net->n, net->geom, net->bodyCell))) {
+                return r;
+            }
             if (net->skinMode == SBX_SKIN_REBUILD &&
                 (r = count_keys(c, net->bodyCell, net->n, nk, net->gridCounts[net->cur], net->gridSlot[net->cur])))
                 return r;
@@ -1754,6 +1839,7 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
     SBX_REQUIRE(c, mode == SBX_SKIN_REBUILD || mode == SBX_SKIN_FIXED || mode == SBX_SKIN_LEARNED,
                 "net_set_skin: unknown mode %d", mode);
     SBX_REQUIRE(c, !net->hasMotors, "net_set_skin: the skin broad phase covers the filament-only step");
+    SBX_REQUIRE(c, mode == SBX_SKIN_REBUILD || !net->reps, "net_set_skin: a batch runs the per-step rebuild");
     if (mode != SBX_SKIN_REBUILD) {
         SBX_REQUIRE(c, net->p.gridOn, "net_set_skin: needs the grid (gridOn)");
         SBX_REQUIRE(c, sp, "net_set_skin: null params");
@@ -1955,3 +2041,369 @@ int sbx_net_set_overlap(sbx_net *net, int on) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ launch shape and autotuner
+namespace {
+
+// a captured step no longer matches the launch sequence
+int drop_graph(sbx_net *net) {
+    sbx_ctx *c = net->ctx;
+    int r = check_hip(c, hipStreamSynchronize(c->stream), "net shape sync");
+    if (!r && net->side) r = check_hip(c, hipStreamSynchronize(net->side), "net shape sync");
+    if (!r && net->sortS) r = check_hip(c, hipStreamSynchronize(net->sortS), "net shape sync");
+    if (net->graphExec) { hipGraphExecDestroy(net->graphExec); net->graphExec = nullptr; }
+    if (net->graph) { hipGraphDestroy(net->graph); net->graph = nullptr; }
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbx_net_set_shape(sbx_net *net, int bpGroup, int scanItems, int rodThreads) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, bpGroup == 2 || bpGroup == 4, "net_set_shape: bpGroup 2 or 4");
+    SBX_REQUIRE(c, scanItems == 4 || scanItems == 8 || scanItems == 16, "net_set_shape: scanItems 4, 8 or 16");
+    SBX_REQUIRE(c, rodThreads == 64 || rodThreads == 128 || rodThreads == 256, "net_set_shape: rodThreads 64, 128 or 256");
+    int r = set_device(c);
+    if (!r) r = drop_graph(net);
+    if (r) return r;
+    net->bpGroup = bpGroup;
+    net->sort.scanItems = scanItems;
+    net->rodThreads = rodThreads;
+    return SBX_OK;
+}
+
+int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *shape, float *msPerStep) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, trialSteps >= 1 && advanced, "net_tune: trialSteps >= 1 and advanced required");
+    int r = set_device(c);
+    if (r) return r;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if ((r = check_hip(c, hipEventCreate(&e0), "tune event"))) return r;
+    if ((r = check_hip(c, hipEventCreate(&e1), "tune event"))) { hipEventDestroy(e0); return r; }
+    const int groups[2] = {4, 2}, items[3] = {8, 4, 16}, threads[2] = {256, 128};
+    int t = step0, best[3] = {net->bpGroup, net->sort.scanItems, net->rodThreads};
+    float bestMs = -1.0f;
+    for (int gi = 0; gi < 2 && !r; gi++)
+        for (int ii = 0; ii < 3 && !r; ii++)
+            for (int ti = 0; ti < 2 && !r; ti++) {
+                if ((r = sbx_net_set_shape(net, groups[gi], items[ii], threads[ti]))) break;
+                if ((r = sbx_net_step(net, t, 2))) break;  // warm (and capture, with graphs on)
+                t += 2;
+                if ((r = join_side(net))) break;
+                if ((r = check_hip(c, hipEventRecord(e0, c->stream), "tune record"))) break;
+                if ((r = sbx_net_step(net, t, trialSteps))) break;
+                t += trialSteps;
+                if ((r = join_side(net))) break;
+                if ((r = check_hip(c, hipEventRecord(e1, c->stream), "tune record"))) break;
+                if ((r = check_hip(c, hipEventSynchronize(e1), "tune sync"))) break;
+                float ms = 0.0f;
+                if ((r = check_hip(c, hipEventElapsedTime(&ms, e0, e1), "tune elapsed"))) break;
+                ms /= (float)trialSteps;
+                if (bestMs < 0.0f || ms < bestMs) {
+                    bestMs = ms;
+                    best[0] = groups[gi]; best[1] = items[ii]; best[2] = threads[ti];
+                }
+            }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *advanced = t - step0;
+    if (r) return r;
+    if ((r = sbx_net_set_shape(net, best[0], best[1], best[2]))) return r;
+    if (shape) { shape[0] = best[0]; shape[1] = best[1]; shape[2] = best[2]; }
+    if (msPerStep) *msPerStep = bestMs;
+    return SBX_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batched step
+struct sbx_batch {
+    sbx_ctx *ctx = nullptr;
+    std::vector<sbx_filaments> fil;  // the caller's replicas
+    std::vector<sbx_crosslinks> xl;
+    BatchLayout lay;
+    sbx_filaments cf{};               // the concatenated stores (library-owned)
+    sbx_crosslinks cx{};
+    std::vector<void *> owned;
+    sbx_net *net = nullptr;
+    int n = 0, L = 0, cells = 0;
+    bool forces = false;
+};
+
+namespace {
+
+// rod and link ids of a replica's topology become ids in the concatenated stores (-1 stays -1)
+__global__ void __launch_bounds__(256) k_offset_ids(int *__restrict__ v, int count, int base) {
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < count && v[k] >= 0) v[k] += base;
+}
+
+template <typename T>
+int batch_alloc(sbx_batch *b, T **p, size_t count) {
+    int r = check_hip(b->ctx, hipMalloc((void **)p, sizeof(T) * (count ? count : 1)), "sbx_batch alloc");
+    if (!r) b->owned.push_back((void *)*p);
+    return r;
+}
+
+// rows [base, base + count) of a stride-`total` planar array with `comps` planes <-> a replica's
+// planar array (stride count); toBatch picks the direction
+template <typename T>
+int copy_planes(sbx_ctx *c, T *batchArr, long long total, long long base, T *repArr, long long count, int comps,
+                bool toBatch) {
+    for (int k = 0; k < comps; k++) {
+        T *bp = batchArr + k * total + base;
+        T *rp = repArr + k * count;
+        int r = check_hip(c, hipMemcpyAsync(toBatch ? (void *)bp : (void *)rp, toBatch ? (void *)rp : (void *)bp,
+                                            sizeof(T) * (size_t)count, hipMemcpyDeviceToDevice, c->stream),
+                          "batch copy");
+        if (r) return r;
+    }
+    return SBX_OK;
+}
+
+bool same_bytes(const void *x, const void *y, size_t bytes) { return std::memcmp(x, y, bytes) == 0; }
+
+// the rod physics every replica of a batch shares
+bool same_physics(const sbx_net_params &a, const sbx_net_params &b) {
+    return same_bytes(a.params, b.params, sizeof(a.params)) &&
+           same_bytes(a.chainParams, b.chainParams, sizeof(a.chainParams)) && a.chainOn == b.chainOn &&
+           a.maxCand == b.maxCand && a.actinRadius == b.actinRadius && a.rng == b.rng && a.stream == b.stream;
+}
+
+// the crosslink physics every replica with crosslinks shares
+bool same_link_physics(const sbx_net_params &a, const sbx_net_params &b) {
+    return same_bytes(a.xlParams, b.xlParams, sizeof(a.xlParams)) &&
+           same_bytes(a.offParams, b.offParams, sizeof(a.offParams)) &&
+           same_bytes(a.torsionParams, b.torsionParams, sizeof(a.torsionParams)) && a.unbindOn == b.unbindOn;
+}
+
+void batch_release(sbx_batch *b) {
+    if (!b) return;
+    if (b->net) release(b->net);
+    if (b->ctx) hipSetDevice(b->ctx->device);
+    for (void *q : b->owned) hipFree(q);
+    delete b;
+}
+
+// the state a step writes, between the concatenated stores and the replicas' own
+int batch_copy_state(sbx_batch *b, bool toBatch, bool everything) {
+    sbx_ctx *c = b->ctx;
+    const int R = (int)b->fil.size();
+    int r = SBX_OK;
+    for (int q = 0; q < R && !r; q++) {
+        const sbx_filaments &f = b->fil[q];
+        const RepInfo &ri = b->lay.reps[q];
+        const long long N = b->n, base = ri.rodBase, nr = f.n;
+        float *const pairs3[] = {b->cf.coord, f.coord, b->cf.uVec, f.uVec, b->cf.yVec, f.yVec,
+                                 b->cf.zVec, f.zVec, b->cf.end1, f.end1, b->cf.end2, f.end2};
+        for (int k = 0; k < 6 && !r; k++) r = copy_planes(c, pairs3[2 * k], N, base, pairs3[2 * k + 1], nr, 3, toBatch);
+        if (!r && b->forces) {
+            r = copy_planes(c, b->cf.forceSum, N, base, f.forceSum, nr, 3, toBatch);
+            if (!r) r = copy_planes(c, b->cf.torqueSum, N, base, f.torqueSum, nr, 3, toBatch);
+        }
+        if (!r && everything) {
+            float *const pf[] = {b->cf.segLength, f.segLength, b->cf.brownTransScale, f.brownTransScale,
+                                 b->cf.brownRotScale, f.brownRotScale};
+            for (int k = 0; k < 3 && !r; k++) r = copy_planes(c, pf[2 * k], N, base, pf[2 * k + 1], nr, 1, toBatch);
+            if (!r) r = copy_planes(c, b->cf.bTransGam, N, base, f.bTransGam, nr, 3, toBatch);
+            if (!r) r = copy_planes(c, b->cf.bRotGam, N, base, f.bRotGam, nr, 3, toBatch);
+            int *const pi[] = {b->cf.end1NbrSlot, f.end1NbrSlot, b->cf.end1NbrSide, f.end1NbrSide,
+                               b->cf.end2NbrSlot, f.end2NbrSlot, b->cf.end2NbrSide, f.end2NbrSide};
+            for (int k = 0; k < 4 && !r; k++)
+                if (pi[2 * k + 1]) r = copy_planes(c, pi[2 * k], N, base, pi[2 * k + 1], nr, 1, toBatch);
+        }
+        if (r || b->L == 0) continue;
+        const sbx_crosslinks &x = b->xl[q];
+        const long long Lt = b->L, lb = ri.linkBase, lr = x.nLinks;
+        if (!r && x.activeLinkCount) r = copy_planes(c, b->cx.activeLinkCount, N, base, x.activeLinkCount, nr, 1, toBatch);
+        if (lr == 0) continue;
+        if (!r) r = copy_planes(c, b->cx.linkState, Lt, lb, x.linkState, lr, 1, toBatch);
+        if (!r) r = copy_planes(c, b->cx.strainPlace, Lt, lb, x.strainPlace, lr, 1, toBatch);
+        if (!r) r = copy_planes(c, b->cx.torqueMagPlace, Lt, lb, x.torqueMagPlace, lr, 1, toBatch);
+        if (!r) r = copy_planes(c, b->cx.strainHist, Lt * SBX_STRAIN_WIN, lb * SBX_STRAIN_WIN, x.strainHist,
+                                lr * SBX_STRAIN_WIN, 1, toBatch);
+        if (!r) r = copy_planes(c, b->cx.torqueMagHist, Lt * SBX_TORQUE_WIN, lb * SBX_TORQUE_WIN, x.torqueMagHist,
+                                lr * SBX_TORQUE_WIN, 1, toBatch);
+        if (!r) r = copy_planes(c, b->cx.xlinkData, Lt * SBX_XL_STRIDE, lb * SBX_XL_STRIDE, x.xlinkData,
+                                lr * SBX_XL_STRIDE, 1, toBatch);
+        if (!r && everything) {
+            float *const pf[] = {b->cx.loc1, x.loc1, b->cx.loc2, x.loc2};
+            for (int k = 0; k < 2 && !r; k++) r = copy_planes(c, pf[2 * k], Lt, lb, pf[2 * k + 1], lr, 1, toBatch);
+            int *const pi[] = {b->cx.linkFilA, x.linkFilA, b->cx.linkFilB, x.linkFilB, b->cx.linkOrientSame,
+                               x.linkOrientSame};
+            for (int k = 0; k < 3 && !r; k++) r = copy_planes(c, pi[2 * k], Lt, lb, pi[2 * k + 1], lr, 1, toBatch);
+        }
+    }
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbx_batch_create(sbx_ctx *c, int nRep, const sbx_filaments *fil, const sbx_crosslinks *xl,
+                     const sbx_net_params *p, sbx_batch **out) {
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, out && fil && p && nRep >= 1, "batch_create: null argument or nRep < 1");
+    *out = nullptr;
+    long long n = 0, L = 0, cells = 0;
+    const bool forces = fil[0].forceSum != nullptr;
+    int linkRef = -1;  // first replica with crosslinks: the link physics of the batch
+    for (int q = 0; q < nRep; q++) {
+        const sbx_net_params &pq = p[q];
+        SBX_REQUIRE(c, fil[q].n > 0, "batch_create: replica %d has no rods", q);
+        SBX_REQUIRE(c, pq.gridOn && !pq.confineOn, "batch_create: replica %d: grid on, containment off", q);
+        SBX_REQUIRE(c, same_physics(pq, p[0]), "batch_create: replica %d: physics differs from replica 0 (only "
+                       "grid, box and seed may differ)", q);
+        if (xl && xl[q].nLinks > 0) {
+            if (linkRef < 0) linkRef = q;
+            SBX_REQUIRE(c, same_link_physics(pq, p[linkRef]), "batch_create: replica %d: crosslink physics "
+                           "differs from replica %d", q, linkRef);
+        }
+        SBX_REQUIRE(c, (fil[q].forceSum != nullptr) == forces, "batch_create: forceSum set for some replicas only");
+        if (pq.chainOn)
+            SBX_REQUIRE(c, fil[q].end1NbrSlot && fil[q].end1NbrSide && fil[q].end2NbrSlot && fil[q].end2NbrSide,
+                        "batch_create: replica %d: chain on but topology arrays null", q);
+        int r = check_grid(c, pq.gridParams, pq.gridDims, pq.periodic);
+        if (r) return r;
+        n += fil[q].n;
+        L += xl ? xl[q].nLinks : 0;
+        cells += pq.gridDims[3];
+    }
+    SBX_REQUIRE(c, n < (1LL << 30) && L < (1LL << 30) && cells < (1LL << 30), "batch_create: batch too large");
+    int r = set_device(c);
+    if (r) return r;
+    sbx_batch *b = new (std::nothrow) sbx_batch();
+    if (!b) return fail(c, SBX_ERR_NOMEM, "batch_create: host allocation");
+    b->ctx = c;
+    b->fil.assign(fil, fil + nRep);
+    if (xl) b->xl.assign(xl, xl + nRep);
+    else b->xl.assign(nRep, sbx_crosslinks{});
+    b->n = (int)n;
+    b->L = (int)L;
+    b->cells = (int)cells;
+    b->forces = forces;
+    b->lay.rodRep.resize((size_t)n);
+    b->lay.linkRep.resize((size_t)L);
+    long long rb = 0, lb = 0, cb = 0;
+    for (int q = 0; q < nRep; q++) {
+        const sbx_net_params &pq = p[q];
+        RepInfo ri;
+        ri.g = grid_geom(pq.gridParams, pq.gridDims, pq.periodic, pq.periodic ? pq.boxLen : nullptr);
+        ri.cellBase = (int)cb;
+        ri.rodBase = (int)rb;
+        ri.linkBase = (int)lb;
+        ri.seed = pq.seed;
+        b->lay.reps.push_back(ri);
+        for (int i = 0; i < fil[q].n; i++) b->lay.rodRep[(size_t)(rb + i)] = q;
+        for (int k = 0; k < b->xl[q].nLinks; k++) b->lay.linkRep[(size_t)(lb + k)] = q;
+        rb += fil[q].n;
+        lb += b->xl[q].nLinks;
+        cb += pq.gridDims[3];
+    }
+    do {
+        sbx_filaments &f = b->cf;
+        f.n = (int)n;
+        for (float **a : {&f.coord, &f.uVec, &f.yVec, &f.zVec, &f.end1, &f.end2, &f.bTransGam, &f.bRotGam})
+            if (!r) r = batch_alloc(b, a, 3 * (size_t)n);
+        for (float **a : {&f.segLength, &f.brownTransScale, &f.brownRotScale})
+            if (!r) r = batch_alloc(b, a, (size_t)n);
+        if (!r && forces) r = batch_alloc(b, &f.forceSum, 3 * (size_t)n);
+        if (!r && forces) r = batch_alloc(b, &f.torqueSum, 3 * (size_t)n);
+        for (int **a : {&f.end1NbrSlot, &f.end1NbrSide, &f.end2NbrSlot, &f.end2NbrSide})
+            if (!r) r = batch_alloc(b, a, (size_t)n);
+        if (r) break;
+        for (int *a : {f.end1NbrSlot, f.end1NbrSide, f.end2NbrSlot, f.end2NbrSide})  // replicas without topology
+            if ((r = fill_i32(c, a, -1, n))) break;
+        if (r) break;
+        sbx_crosslinks &x = b->cx;
+        x.nLinks = (int)L;
+        if (L > 0) {
+            for (int **a : {&x.linkFilA, &x.linkFilB, &x.linkState, &x.strainPlace, &x.linkOrientSame,
+                            &x.torqueMagPlace})
+                if (!r) r = batch_alloc(b, a, (size_t)L);
+            for (float **a : {&x.loc1, &x.loc2})
+                if (!r) r = batch_alloc(b, a, (size_t)L);
+            if (!r) r = batch_alloc(b, &x.strainHist, (size_t)L * SBX_STRAIN_WIN);
+            if (!r) r = batch_alloc(b, &x.torqueMagHist, (size_t)L * SBX_TORQUE_WIN);
+            if (!r) r = batch_alloc(b, &x.xlinkData, (size_t)L * SBX_XL_STRIDE);
+            if (!r) r = batch_alloc(b, &x.activeLinkCount, (size_t)n);
+            if (r) break;
+            if ((r = fill_i32(c, x.activeLinkCount, 0, n))) break;
+        }
+        if ((r = batch_copy_state(b, true, true))) break;
+        for (int q = 0; q < nRep && !r; q++) {  // ids into the concatenated stores
+            const RepInfo &ri = b->lay.reps[q];
+            const int nr = fil[q].n, lr = b->xl[q].nLinks;
+            for (int *a : {f.end1NbrSlot, f.end2NbrSlot}) {
+                hipLaunchKernelGGL(k_offset_ids, dim3(blocks_for(nr, 256)), dim3(256), 0, c->stream, a + ri.rodBase,
+                                   nr, ri.rodBase);
+            }
+            if (lr > 0) {
+                for (int *a : {x.linkFilA, x.linkFilB}) {
+                    hipLaunchKernelGGL(k_offset_ids, dim3(blocks_for(lr, 256)), dim3(256), 0, c->stream,
+                                       a + ri.linkBase, lr, ri.rodBase);
+                }
+            }
+            r = check_launch(c, "k_offset_ids");
+        }
+        if (r) break;
+        sbx_net_params pc = p[0];
+        pc.gridDims[3] = (int)cells;
+        if (linkRef >= 0) {
+            std::memcpy(pc.xlParams, p[linkRef].xlParams, sizeof(pc.xlParams));
+            std::memcpy(pc.offParams, p[linkRef].offParams, sizeof(pc.offParams));
+            std::memcpy(pc.torsionParams, p[linkRef].torsionParams, sizeof(pc.torsionParams));
+            pc.unbindOn = p[linkRef].unbindOn;
+        }
+        r = net_create(c, &b->cf, L > 0 ? &b->cx : nullptr, &pc, &b->lay, &b->net);
+    } while (0);
+    if (r) {
+        batch_release(b);
+        return r;
+    }
+    *out = b;
+    return SBX_OK;
+}
+
+int sbx_batch_net(sbx_batch *b, sbx_net **net) {
+    if (!b || !net) return SBX_ERR_ARG;
+    *net = b->net;
+    return SBX_OK;
+}
+
+int sbx_batch_step(sbx_batch *b, int step0, int nSteps) {
+    if (!b) return SBX_ERR_ARG;
+    return sbx_net_step(b->net, step0, nSteps);
+}
+
+int sbx_batch_publish(sbx_batch *b) {
+    if (!b) return SBX_ERR_ARG;
+    SBX_CHECK_CTX(b->ctx);
+    int r = sbx_net_publish(b->net);
+    if (!r) r = set_device(b->ctx);
+    if (!r) r = batch_copy_state(b, false, false);
+    return r;
+}
+
+int sbx_batch_layout(sbx_batch *b, int rep, int *rodBase, int *linkBase, int *cellBase) {
+    if (!b) return SBX_ERR_ARG;
+    SBX_CHECK_CTX(b->ctx);
+    SBX_REQUIRE(b->ctx, rep >= 0 && rep < (int)b->lay.reps.size(), "batch_layout: replica %d out of range", rep);
+    const RepInfo &ri = b->lay.reps[rep];
+    if (rodBase) *rodBase = ri.rodBase;
+    if (linkBase) *linkBase = ri.linkBase;
+    if (cellBase) *cellBase = ri.cellBase;
+    return SBX_OK;
+}
+
+void sbx_batch_destroy(sbx_batch *b) { batch_release(b); }
+
+}  // extern "C"
+

@@ -229,6 +229,27 @@ def network(n_fil: int = 20000, segs: int = 10, seg_len: float = 0.1, box: float
     return Scene("network", fs, xl, p, box, seed, n_fil=n_fil, segs_per_fil=segs)
 
 
+MONOMERS_PER_UM = 370.0   # actin subunits per micrometre of filament (2.7 nm rise)
+AVOGADRO = 6.02214076e23
+
+
+def ensemble(n_rep: int = 64, seed: int = 0, fil_range=(2000, 40000), conc_range=(5.0, 30.0), segs: int = 10,
+             seg_len: float = 0.1, links_per_fil: float = 2.5, **kw) -> list[Scene]:
+    """Replicas of the crosslinked-network recipe for a batched step: filament count log-uniform in
+    fil_range, actin concentration uniform in conc_range (uM) setting the periodic box, links per
+    filament as the 20k-filament network (50k links), replica r seeded with seed * 1000 + r."""
+    g = np.random.default_rng(seed)
+    out = []
+    for r in range(n_rep):
+        n_fil = int(round(math.exp(g.uniform(math.log(fil_range[0]), math.log(fil_range[1])))))
+        conc = g.uniform(*conc_range) * 1.0e-6
+        monomers = n_fil * segs * seg_len * MONOMERS_PER_UM
+        box = (monomers / (conc * AVOGADRO) * 1.0e15) ** (1.0 / 3.0)  # litres -> um^3
+        out.append(network(n_fil=n_fil, segs=segs, seg_len=seg_len, box=box,
+                           n_links=int(round(links_per_fil * n_fil)), seed=seed * 1000 + r, **kw))
+    return out
+
+
 def derive_host(fs: FilamentStore) -> None:
     """Initial derived geometry in float32 with the reference's operation order
     (DerivedGeometrySystem.java:156-185), so the first step reads consistent end1/end2."""

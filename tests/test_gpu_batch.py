@@ -1,0 +1,139 @@
+"""Batched step on the GPU: replicas of uneven size concatenated and advanced together.
+
+Every replica's state after the batched steps -- pose, derived geometry, accumulated forces, crosslink
+lifecycle and rings, grid CSR and candidate slices -- is bit-identical to stepping that replica alone
+(eager and captured graphs, single stream and overlapped), and one replica is also checked against
+the oracle directly.  Launch shapes and the autotuner change nothing but the time.
+"""
+import numpy as np
+import pytest
+
+from _common import assert_same
+from oracle import ffi
+from softbox_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("coord", "uVec", "yVec", "zVec", "end1", "end2", "forceSum", "torqueSum")
+LINKS = ("linkState", "strainHist", "strainPlace", "torqueMagHist", "torqueMagPlace", "xlinkData", "activeLinkCount")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from softbox_amd.context import Context
+
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _replicas(seed=3, n_rep=4):
+    # uneven sizes and densities; the last replica is box-clamped, one has no crosslinks
+    reps = scenes.ensemble(n_rep - 1, seed=seed, fil_range=(60, 600), conc_range=(5.0, 30.0))
+    reps[1] = scenes.network(n_fil=90, segs=10, box=reps[1].box, n_links=0, seed=seed * 1000 + 1)
+    reps.append(scenes.network(n_fil=140, segs=10, box=2.0, n_links=300, seed=seed * 1000 + 9, periodic=False,
+                               confine=False))
+    return reps
+
+
+def _alone(ctx, sc, steps, graph=False, overlap=True):
+    from softbox_amd.network import Network
+
+    net = Network(ctx, sc.copy())
+    net.set_overlap(overlap)
+    if graph:
+        net.use_graph(2)
+    net.step(0, steps)
+    fs = sc.fs.copy()
+    xl = sc.xl.copy() if sc.xl is not None else None
+    net.pull(fs, xl)
+    cand = net.candidates(sc.fs.n, int(sc.p["maxCand"]))
+    net.close()
+    return fs, xl, cand
+
+
+def _check_replica(bt, r, sc, want, what):
+    fs_w, xl_w, cand_w = want
+    fs = sc.fs.copy()
+    xl = sc.xl.copy() if sc.xl is not None else None
+    bt.pull(r, fs, xl)
+    for k in STATE:
+        assert_same(getattr(fs, k), getattr(fs_w, k), f"{what}: replica {r} {k}")
+    if xl is not None:
+        for k in LINKS:
+            assert_same(getattr(xl, k), getattr(xl_w, k), f"{what}: replica {r} {k}")
+    cand, cnt, off, cont, cell = bt.candidates(r)
+    wc, wn, woff, wcont, wcell = cand_w
+    mc = int(sc.p["maxCand"])
+    assert_same(cnt, wn, f"{what}: replica {r} candCount")
+    used = (np.arange(mc)[None, :] < np.minimum(wn, mc)[:, None]).ravel()
+    assert_same(cand[used], wc[used], f"{what}: replica {r} candPartner")
+    assert_same(off, woff, f"{what}: replica {r} gridCellOffsets")
+    assert_same(cont, wcont, f"{what}: replica {r} gridCellContents")
+    assert_same(cell, wcell, f"{what}: replica {r} bodyCell")
+
+
+@pytest.mark.parametrize("graph,overlap", [(False, False), (False, True), (True, True)])
+def test_batch_replicas_equal_stepping_alone(ctx, graph, overlap):
+    from softbox_amd.batch import Batch
+
+    reps = _replicas()
+    steps = 12
+    want = [_alone(ctx, sc, steps, graph=graph, overlap=overlap) for sc in reps]
+    bt = Batch(ctx, [sc.copy() for sc in reps])
+    bt.net.set_overlap(overlap)
+    if graph:
+        bt.net.use_graph(2)
+    bt.step(0, 5)
+    bt.step(5, steps - 5)
+    for r, sc in enumerate(reps):
+        _check_replica(bt, r, sc, want[r], f"graph={graph} overlap={overlap}")
+    bt.close()
+
+
+def test_batch_replica_matches_oracle(ctx):
+    from softbox_amd.batch import Batch
+
+    reps = _replicas(seed=4)
+    bt = Batch(ctx, [sc.copy() for sc in reps])
+    steps = 8
+    bt.step(0, steps)
+    r = 2
+    orc = ffi.OracleNet(reps[r].copy())
+    for t in range(steps):
+        orc.step(t, 1)
+    fs, xl = reps[r].fs.copy(), reps[r].xl.copy()
+    bt.pull(r, fs, xl)
+    for k in STATE:
+        assert_same(getattr(fs, k), getattr(orc.fs, k), f"oracle: {k}")
+    for k in LINKS:
+        assert_same(getattr(xl, k), getattr(orc.xl, k), f"oracle: {k}")
+    cand, cnt, off, cont, cell = bt.candidates(r)
+    assert_same(off, orc.grid["gridCellOffsets"], "oracle: gridCellOffsets")
+    assert_same(cont, orc.grid["gridCellContents"], "oracle: gridCellContents")
+    assert_same(cnt, orc.grid["candCount"], "oracle: candCount")
+    mc = int(reps[r].p["maxCand"])
+    used = (np.arange(mc)[None, :] < np.minimum(cnt, mc)[:, None]).ravel()
+    assert_same(cand[used], orc.grid["candPartner"][used], "oracle: candPartner")
+    bt.close()
+
+
+def test_launch_shapes_and_tuner_change_only_time(ctx):
+    """Every launch shape gives the same steps; the tuner's trial steps are steps of the run."""
+    from softbox_amd.batch import Batch
+
+    reps = _replicas(seed=5)
+    bt = Batch(ctx, [sc.copy() for sc in reps])
+    t = 0
+    for shape in ((2, 4, 64), (4, 16, 128), (2, 8, 256)):
+        bt.net.set_shape(*shape)
+        bt.step(t, 3)
+        t += 3
+    res = bt.net.tune(t, trial_steps=1)
+    assert res["steps"] == 12 * 3 and res["shape"][0] in (2, 4) and res["ms_per_step"] > 0, res
+    t += res["steps"]
+    bt.step(t, 2)
+    t += 2
+    for r, sc in enumerate(reps):
+        _check_replica(bt, r, sc, _alone(ctx, sc, t), f"shapes + tuner ({t} steps)")
+    bt.close()
