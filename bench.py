@@ -155,6 +155,10 @@ def main() -> int:
     ap.add_argument("--n-links", type=int, default=50000)
     ap.add_argument("--box", type=float, default=20.0)
     ap.add_argument("--eta", type=float, default=1.0e-3, help="viscosity, Pa s (the workload: water)")
+    ap.add_argument("--tune-steps", type=int, default=8,
+                    help="autotuner: steps timed per launch shape after the warm-up (0: default shape)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="N > 0: the batched ensemble instead (N replicas, 2k-40k filaments log-uniform, 5-30 uM)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="before the warm-up, one device-to-device copy of this many MiB (a plain copy kernel "
                          "of known size that calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -174,6 +178,8 @@ def main() -> int:
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+    if args.batch:
+        return run_batch(args, log)
     t_build = time.perf_counter()
     sc = scenes.network(n_fil=args.n_fil, segs=10, box=args.box, n_links=args.n_links, seed=0, eta=args.eta)
     n = sc.fs.n
@@ -204,6 +210,7 @@ def main() -> int:
 
     modes = {"compare": ["rebuild", "learned"], "rebuild": ["rebuild"], "fixed": ["fixed"], "learned": ["learned"]}
     runs = {}
+    extra = 0  # steps the tuner took in the rebuild run: the other runs warm up as long (same end step)
     for mode in modes[args.skin]:
         net = Network(ctx, sc.copy(), keep_forces=False)
         if mode != "rebuild":
@@ -213,13 +220,24 @@ def main() -> int:
         if args.graph:
             net.use_graph(True)
         net.step(0, args.warmup)
-        el = timed(net, ctx, args.warmup, args.steps)
+        t0 = args.warmup
+        if args.tune_steps and mode == "rebuild":
+            tuned = net.tune(t0, args.tune_steps)
+            t0 += tuned["steps"]
+            extra = tuned["steps"]
+            log(f"tuned launch shape {tuned}")
+        elif extra:
+            net.step(t0, extra)
+            t0 += extra
+        el = timed(net, ctx, t0, args.steps)
         r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps}
+        if args.tune_steps and mode == "rebuild":
+            r["shape"] = dict(zip(("bp_group", "scan_items", "rod_threads"), tuned["shape"]))
         r["digest"] = slices_digest(net, sc)
         if mode != "rebuild":
             r["skin"] = net.skin_stats()
         if args.profile_steps:
-            r["kernels"] = profile(net, args.warmup + args.steps, args.profile_steps)
+            r["kernels"] = profile(net, t0 + args.steps, args.profile_steps)
         xl_end = sc.xl.copy()
         fs_end = sc.fs.copy()
         net.pull(fs_end, xl_end)
@@ -278,6 +296,68 @@ def main() -> int:
     if train_info:
         out["skin_training"] = train_info
     print(json.dumps(out))
+    ctx.close()
+    return 0
+
+
+def run_batch(args, log) -> int:
+    """The batched ensemble: args.batch replicas of the network recipe (filament count log-uniform in
+    [2k, 40k], actin 5-30 uM setting each periodic box, 2.5 links per filament), concatenated and
+    advanced together; rod-steps/s over all replicas and the p50 / p99 of the batch step time."""
+    import torch
+
+    from softbox_amd import scenes
+    from softbox_amd.batch import Batch
+    from softbox_amd.context import Context
+
+    t_build = time.perf_counter()
+    reps = scenes.ensemble(args.batch, seed=0, eta=args.eta)
+    rods = [sc.fs.n for sc in reps]
+    n = sum(rods)
+    log(f"ensemble: {len(reps)} replicas, {n} rods (per replica {min(rods)}..{max(rods)}), "
+        f"{sum(sc.xl.nLinks for sc in reps)} links ({time.perf_counter() - t_build:.1f} s)")
+    ctx = Context(0)
+    bt = Batch(ctx, reps, keep_forces=False)
+    bt.net.set_overlap(bool(args.overlap))
+    if args.graph:
+        bt.net.use_graph(True)
+    bt.step(0, args.warmup)
+    t0 = args.warmup
+    shape = None
+    if args.tune_steps:
+        tuned = bt.net.tune(t0, args.tune_steps)
+        t0 += tuned["steps"]
+        shape = dict(zip(("bp_group", "scan_items", "rod_threads"), tuned["shape"]))
+        log(f"tuned launch shape {tuned}")
+    el = timed(bt, ctx, t0, args.steps)
+    t0 += args.steps
+    # batch step times: pairs of steps (one captured graph) between device synchronisations
+    per = []
+    for _ in range(max(8, min(64, args.steps // 2))):
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        bt.step(t0, 2)
+        t0 += 2
+        ctx.sync()
+        torch.cuda.synchronize()
+        per.append(1e3 * (time.perf_counter() - ts) / 2.0)
+    per = np.array(per)
+    fs = reps[0].fs.copy()
+    bt.pull(0, fs)
+    assert np.isfinite(fs.coord).all(), "non-finite pose after the timed run"
+    out = {
+        "metric": "rod-steps/sec, batched ensemble at 1 GPU", "value": round(n * args.steps / el, 1),
+        "unit": "rod-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * el / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 state, f64 force laws", "data": "synthetic (seeded ensemble)",
+        "config": {"workload": f"{len(reps)} replicas of the crosslinked network recipe, 2k-40k filaments x 10 rods "
+                               "(log-uniform), actin 5-30 uM (periodic box), 2.5 links per filament, dt 1e-5 s",
+                   "rods": n, "replicas": len(reps), "rods_per_replica": [min(rods), int(np.median(rods)), max(rods)],
+                   "graph": bool(args.graph), "shape": shape},
+        "step_ms": {"p50": round(float(np.percentile(per, 50)), 4), "p99": round(float(np.percentile(per, 99)), 4)},
+    }
+    print(json.dumps(out))
+    bt.close()
     ctx.close()
     return 0
 
