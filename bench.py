@@ -157,6 +157,8 @@ def main() -> int:
     ap.add_argument("--eta", type=float, default=1.0e-3, help="viscosity, Pa s (the workload: water)")
     ap.add_argument("--tune-steps", type=int, default=8,
                     help="autotuner: steps timed per launch shape after the warm-up (0: default shape)")
+    ap.add_argument("--shape", type=str, default="",
+                    help="bp_group,scan_items,rod_threads: fixed launch shape instead of the autotuner")
     ap.add_argument("--batch", type=int, default=0,
                     help="N > 0: the batched ensemble instead (N replicas, 2k-40k filaments log-uniform, 5-30 uM)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -221,7 +223,9 @@ def main() -> int:
             net.use_graph(True)
         net.step(0, args.warmup)
         t0 = args.warmup
-        if args.tune_steps and mode == "rebuild":
+        if args.shape:
+            net.set_shape(*[int(v) for v in args.shape.split(",")])
+        elif args.tune_steps and mode == "rebuild":
             tuned = net.tune(t0, args.tune_steps)
             t0 += tuned["steps"]
             extra = tuned["steps"]
@@ -231,8 +235,9 @@ def main() -> int:
             t0 += extra
         el = timed(net, ctx, t0, args.steps)
         r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps}
-        if args.tune_steps and mode == "rebuild":
-            r["shape"] = dict(zip(("bp_group", "scan_items", "rod_threads"), tuned["shape"]))
+        if args.shape or (args.tune_steps and mode == "rebuild"):
+            sh = [int(v) for v in args.shape.split(",")] if args.shape else tuned["shape"]
+            r["shape"] = dict(zip(("bp_group", "scan_items", "rod_threads"), sh))
         r["digest"] = slices_digest(net, sc)
         if mode != "rebuild":
             r["skin"] = net.skin_stats()
@@ -324,7 +329,11 @@ def run_batch(args, log) -> int:
     bt.step(0, args.warmup)
     t0 = args.warmup
     shape = None
-    if args.tune_steps:
+    if args.shape:
+        sh = [int(v) for v in args.shape.split(",")]
+        bt.net.set_shape(*sh)
+        shape = dict(zip(("bp_group", "scan_items", "rod_threads"), sh))
+    elif args.tune_steps:
         tuned = bt.net.tune(t0, args.tune_steps)
         t0 += tuned["steps"]
         shape = dict(zip(("bp_group", "scan_items", "rod_threads"), tuned["shape"]))
