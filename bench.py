@@ -236,8 +236,8 @@ def main() -> int:
         el = timed(net, ctx, t0, args.steps)
         r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps}
         if args.shape or (args.tune_steps and mode == "rebuild"):
-            sh = [int(v) for v in args.shape.split(",")] if args.shape else tuned["shape"]
-            r["shape"] = dict(zip(("bp_group", "scan_items", "rod_threads"), sh))
+            sh = [int(v) for v in args.shape.split(",")] + [int(args.overlap)] if args.shape else tuned["shape"]
+            r["shape"] = dict(zip(("bp_group", "scan_items", "rod_threads", "overlap"), sh))
         r["digest"] = slices_digest(net, sc)
         if mode != "rebuild":
             r["skin"] = net.skin_stats()
@@ -332,11 +332,11 @@ def run_batch(args, log) -> int:
     if args.shape:
         sh = [int(v) for v in args.shape.split(",")]
         bt.net.set_shape(*sh)
-        shape = dict(zip(("bp_group", "scan_items", "rod_threads"), sh))
+        shape = dict(zip(("bp_group", "scan_items", "rod_threads", "overlap"), sh + [int(args.overlap)]))
     elif args.tune_steps:
         tuned = bt.net.tune(t0, args.tune_steps)
         t0 += tuned["steps"]
-        shape = dict(zip(("bp_group", "scan_items", "rod_threads"), tuned["shape"]))
+        shape = dict(zip(("bp_group", "scan_items", "rod_threads", "overlap"), tuned["shape"]))
         log(f"tuned launch shape {tuned}")
     el = timed(bt, ctx, t0, args.steps)
     t0 += args.steps
@@ -351,6 +351,7 @@ def run_batch(args, log) -> int:
         torch.cuda.synchronize()
         per.append(1e3 * (time.perf_counter() - ts) / 2.0)
     per = np.array(per)
+    kernels = profile(bt.net, t0, args.profile_steps) if args.profile_steps else None
     fs = reps[0].fs.copy()
     bt.pull(0, fs)
     assert np.isfinite(fs.coord).all(), "non-finite pose after the timed run"
@@ -365,6 +366,8 @@ def run_batch(args, log) -> int:
                    "graph": bool(args.graph), "shape": shape},
         "step_ms": {"p50": round(float(np.percentile(per, 50)), 4), "p99": round(float(np.percentile(per, 99)), 4)},
     }
+    if kernels:
+        out["kernels"] = {k: {"avg_us": round(v["avg_us"], 1), "launches": v["launches"]} for k, v in kernels.items()}
     print(json.dumps(out))
     bt.close()
     ctx.close()

@@ -198,19 +198,23 @@ int launch_broad_phase_sorted(sbx_ctx *c, const float4 *sortedPose, const int *s
                               const int *cellContents, const GridGeom &g, const RepInfo *reps, const int *rodRep,
                               int group, int cap, int S, int *candPartner, int *candCount, int maxCand,
                               hipStream_t st) {
-    SBX_REQUIRE(c, group == 2 || group == 4, "broad phase: %d lanes per body (2 or 4)", group);
+    SBX_REQUIRE(c, group == 2 || group == 4 || group == 8, "broad phase: %d lanes per body (2, 4 or 8)", group);
     SortedPose pose{sortedPose, sortedCell};
     hipStream_t q = st ? st : c->stream;
     if (reps) {
         const BatchGeo geo{reps, rodRep};
-        if (group == 4) launch_sorted<BatchGeo, 4>(c, pose, geo, cellOffsets, cellContents, cap, S, candPartner,
+        if (group == 8) launch_sorted<BatchGeo, 8>(c, pose, geo, cellOffsets, cellContents, cap, S, candPartner,
                                                    candCount, maxCand, q);
+        else if (group == 4) launch_sorted<BatchGeo, 4>(c, pose, geo, cellOffsets, cellContents, cap, S,
+                                                        candPartner, candCount, maxCand, q);
         else launch_sorted<BatchGeo, 2>(c, pose, geo, cellOffsets, cellContents, cap, S, candPartner, candCount,
                                         maxCand, q);
     } else {
         const SingleGeo geo{g};
-        if (group == 4) launch_sorted<SingleGeo, 4>(c, pose, geo, cellOffsets, cellContents, cap, S, candPartner,
+        if (group == 8) launch_sorted<SingleGeo, 8>(c, pose, geo, cellOffsets, cellContents, cap, S, candPartner,
                                                     candCount, maxCand, q);
+        else if (group == 4) launch_sorted<SingleGeo, 4>(c, pose, geo, cellOffsets, cellContents, cap, S,
+                                                         candPartner, candCount, maxCand, q);
         else launch_sorted<SingleGeo, 2>(c, pose, geo, cellOffsets, cellContents, cap, S, candPartner, candCount,
                                          maxCand, q);
     }

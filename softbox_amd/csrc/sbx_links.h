@@ -160,10 +160,9 @@ __device__ __forceinline__ void link_torsion_one(int k, const LinkEnd &A, const 
 }
 
 // One side of segGatherA/B (CrosslinkerSystem.java:132-147): FP64 sum in CSR order, skip dead links.
-__device__ __forceinline__ void gather_side(int s, const int *__restrict__ offsets, const int *__restrict__ idx,
-                                            const float *__restrict__ xlinkData, const int *__restrict__ linkState,
-                                            int off, double f[3], double t[3], int &active) {
-    int start = offsets[s], end = offsets[s + 1];
+__device__ __forceinline__ void gather_range(int start, int end, const int *__restrict__ idx,
+                                             const float *__restrict__ xlinkData, const int *__restrict__ linkState,
+                                             int off, double f[3], double t[3], int &active) {
     for (int k = start; k < end; k++) {
         int li = idx[k];
         if (linkState[li] < 0) continue;
@@ -172,6 +171,13 @@ __device__ __forceinline__ void gather_side(int s, const int *__restrict__ offse
         f[0] += d[0]; f[1] += d[1]; f[2] += d[2];
         t[0] += d[3]; t[1] += d[4]; t[2] += d[5];
     }
+}
+
+// segGatherA/B of segment s over its CSR range
+__device__ __forceinline__ void gather_side(int s, const int *__restrict__ offsets, const int *__restrict__ idx,
+                                            const float *__restrict__ xlinkData, const int *__restrict__ linkState,
+                                            int off, double f[3], double t[3], int &active) {
+    gather_range(offsets[s], offsets[s + 1], idx, xlinkData, linkState, off, f, t, active);
 }
 
 // CrosslinkerSystem.countActiveLinks for one segment (CrosslinkerSystem.java:694-706): ACTIVE links

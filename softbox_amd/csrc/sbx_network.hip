@@ -120,7 +120,7 @@ struct sbx_net {
     RepInfo *reps = nullptr;                          // device table [R]
     int *rodRep = nullptr, *linkRep = nullptr;        // replica of each rod / link
     // launch shape (results do not depend on it; sbx_net_tune picks it by timing steps)
-    int bpGroup = 4;      // broad-phase lanes per body (2 or 4)
+    int bpGroup = 4;      // broad-phase lanes per body (2, 4 or 8)
     int rodThreads = 256; // k_rod block size (64, 128 or 256)
     // graph
     int useGraph = 0;
@@ -2064,7 +2064,7 @@ int sbx_net_set_shape(sbx_net *net, int bpGroup, int scanItems, int rodThreads) 
     if (!net) return SBX_ERR_ARG;
     sbx_ctx *c = net->ctx;
     SBX_CHECK_CTX(c);
-    SBX_REQUIRE(c, bpGroup == 2 || bpGroup == 4, "net_set_shape: bpGroup 2 or 4");
+    SBX_REQUIRE(c, bpGroup == 2 || bpGroup == 4 || bpGroup == 8, "net_set_shape: bpGroup 2, 4 or 8");
     SBX_REQUIRE(c, scanItems == 4 || scanItems == 8 || scanItems == 16, "net_set_shape: scanItems 4, 8 or 16");
     SBX_REQUIRE(c, rodThreads == 64 || rodThreads == 128 || rodThreads == 256, "net_set_shape: rodThreads 64, 128 or 256");
     int r = set_device(c);
@@ -2086,13 +2086,16 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if ((r = check_hip(c, hipEventCreate(&e0), "tune event"))) return r;
     if ((r = check_hip(c, hipEventCreate(&e1), "tune event"))) { hipEventDestroy(e0); return r; }
-    const int groups[2] = {4, 2}, items[3] = {8, 4, 16}, threads[2] = {256, 128};
-    int t = step0, best[3] = {net->bpGroup, net->sort.scanItems, net->rodThreads};
+    const int groups[3] = {4, 2, 8}, items[3] = {8, 4, 16}, threads[2] = {256, 128};
+    const int overlaps = net->side ? 2 : 1;  // with streams: the overlapped and the one-stream step
+    int t = step0, best[4] = {net->bpGroup, net->sort.scanItems, net->rodThreads, net->overlap};
     float bestMs = -1.0f;
-    for (int gi = 0; gi < 2 && !r; gi++)
+    for (int oi = 0; oi < overlaps && !r; oi++)
+    for (int gi = 0; gi < 3 && !r; gi++)
         for (int ii = 0; ii < 3 && !r; ii++)
             for (int ti = 0; ti < 2 && !r; ti++) {
                 if ((r = sbx_net_set_shape(net, groups[gi], items[ii], threads[ti]))) break;
+                net->overlap = oi == 0 ? 1 : 0;
                 if ((r = sbx_net_step(net, t, 2))) break;  // warm (and capture, with graphs on)
                 t += 2;
                 if ((r = join_side(net))) break;
@@ -2107,7 +2110,7 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
                 ms /= (float)trialSteps;
                 if (bestMs < 0.0f || ms < bestMs) {
                     bestMs = ms;
-                    best[0] = groups[gi]; best[1] = items[ii]; best[2] = threads[ti];
+                    best[0] = groups[gi]; best[1] = items[ii]; best[2] = threads[ti]; best[3] = net->overlap;
                 }
             }
     hipEventDestroy(e0);
@@ -2115,7 +2118,8 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
     *advanced = t - step0;
     if (r) return r;
     if ((r = sbx_net_set_shape(net, best[0], best[1], best[2]))) return r;
-    if (shape) { shape[0] = best[0]; shape[1] = best[1]; shape[2] = best[2]; }
+    net->overlap = best[3];
+    if (shape) { shape[0] = best[0]; shape[1] = best[1]; shape[2] = best[2]; shape[3] = best[3]; }
     if (msPerStep) *msPerStep = bestMs;
     return SBX_OK;
 }

@@ -86,10 +86,11 @@ class Network:
         self.ctx.check(self.ctx.lib.sbx_net_set_shape(self.h, bp_group, scan_items, rod_threads))
 
     def tune(self, step0: int, trial_steps: int = 20) -> dict:
-        """Time every launch shape over real steps step0, step0+1, ... and keep the fastest; returns
-        {"steps": steps taken, "shape": (bp_group, scan_items, rod_threads), "ms_per_step": best}."""
+        """Time every launch shape (with and without the concurrent streams) over real steps step0,
+        step0+1, ... and keep the fastest; returns {"steps": steps taken, "shape": (bp_group,
+        scan_items, rod_threads, overlap), "ms_per_step": best}."""
         adv = C.c_int()
-        shape = (C.c_int * 3)()
+        shape = (C.c_int * 4)()
         ms = C.c_float()
         self.ctx.check(self.ctx.lib.sbx_net_tune(self.h, step0, trial_steps, C.byref(adv), shape, C.byref(ms)))
         return {"steps": adv.value, "shape": tuple(shape), "ms_per_step": ms.value}

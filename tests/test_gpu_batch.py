@@ -125,12 +125,12 @@ def test_launch_shapes_and_tuner_change_only_time(ctx):
     reps = _replicas(seed=5)
     bt = Batch(ctx, [sc.copy() for sc in reps])
     t = 0
-    for shape in ((2, 4, 64), (4, 16, 128), (2, 8, 256)):
+    for shape in ((2, 4, 64), (4, 16, 128), (8, 8, 256)):
         bt.net.set_shape(*shape)
         bt.step(t, 3)
         t += 3
-    res = bt.net.tune(t, trial_steps=1)
-    assert res["steps"] == 12 * 3 and res["shape"][0] in (2, 4) and res["ms_per_step"] > 0, res
+    res = bt.net.tune(t, trial_steps=1)  # 3 x 3 x 2 shapes, overlapped and not, 2 + 1 steps each
+    assert res["steps"] == 36 * 3 and res["shape"][0] in (2, 4, 8) and res["ms_per_step"] > 0, res
     t += res["steps"]
     bt.step(t, 2)
     t += 2
