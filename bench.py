@@ -103,6 +103,16 @@ def run_cpu_baseline(sc, budget_s: float = 15.0) -> dict:
                       f"for {steps} steps = {el:.1f} s"}
 
 
+def shape_of(args, tuned=None) -> dict:
+    """The launch shape a run used: the tuner's (group, staged, items, threads, overlap) or --shape."""
+    if tuned is not None:
+        g, st, it, th, ov = tuned["shape"]
+    else:
+        v = [int(x) for x in args.shape.split(",")] + [0]
+        g, it, th, st, ov = v[0], v[1], v[2], v[3], int(args.overlap)
+    return {"bp_group": g, "bp_staged": st, "scan_items": it, "rod_threads": th, "overlap": ov}
+
+
 def timed(net, ctx, step0: int, steps: int) -> float:
     """Seconds for `steps` whole steps, bracketed by device synchronisation."""
     import torch
@@ -158,7 +168,7 @@ def main() -> int:
     ap.add_argument("--tune-steps", type=int, default=8,
                     help="autotuner: steps timed per launch shape after the warm-up (0: default shape)")
     ap.add_argument("--shape", type=str, default="",
-                    help="bp_group,scan_items,rod_threads: fixed launch shape instead of the autotuner")
+                    help="bp_group,scan_items,rod_threads[,bp_staged]: fixed launch shape instead of the autotuner")
     ap.add_argument("--batch", type=int, default=0,
                     help="N > 0: the batched ensemble instead (N replicas, 2k-40k filaments log-uniform, 5-30 uM)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -236,8 +246,7 @@ def main() -> int:
         el = timed(net, ctx, t0, args.steps)
         r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps}
         if args.shape or (args.tune_steps and mode == "rebuild"):
-            sh = [int(v) for v in args.shape.split(",")] + [int(args.overlap)] if args.shape else tuned["shape"]
-            r["shape"] = dict(zip(("bp_group", "scan_items", "rod_threads", "overlap"), sh))
+            r["shape"] = shape_of(args, None if args.shape else tuned)
         r["digest"] = slices_digest(net, sc)
         if mode != "rebuild":
             r["skin"] = net.skin_stats()
@@ -330,13 +339,12 @@ def run_batch(args, log) -> int:
     t0 = args.warmup
     shape = None
     if args.shape:
-        sh = [int(v) for v in args.shape.split(",")]
-        bt.net.set_shape(*sh)
-        shape = dict(zip(("bp_group", "scan_items", "rod_threads", "overlap"), sh + [int(args.overlap)]))
+        bt.net.set_shape(*[int(v) for v in args.shape.split(",")])
+        shape = shape_of(args)
     elif args.tune_steps:
         tuned = bt.net.tune(t0, args.tune_steps)
         t0 += tuned["steps"]
-        shape = dict(zip(("bp_group", "scan_items", "rod_threads", "overlap"), tuned["shape"]))
+        shape = shape_of(args, tuned)
         log(f"tuned launch shape {tuned}")
     el = timed(bt, ctx, t0, args.steps)
     t0 += args.steps

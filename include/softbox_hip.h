@@ -396,14 +396,15 @@ int sbx_net_use_graph(sbx_net *net, int on);
  * step); 0: one stream.  Results are identical; sbx_net_step's work is complete on the context's
  * stream either way.  Off while per-kernel timers are on. */
 int sbx_net_set_overlap(sbx_net *net, int on);
-/* Launch shape of the step: broad-phase lanes per body (2, 4 or 8), scan keys per thread (4, 8 or 16: scan
- * tiles of 256 * scanItems cells) and k_rod block size (64, 128 or 256).  No counterpart in the reference;
- * results do not depend on it. */
-int sbx_net_set_shape(sbx_net *net, int bpGroup, int scanItems, int rodThreads);
+/* Launch shape of the step: broad-phase lanes per body (2, 4 or 8), the broad phase's neighbourhoods
+ * staged in LDS per block (bpStaged 1, with 4 or 8 lanes), scan keys per thread (4, 8 or 16: scan tiles of
+ * 256 * scanItems cells) and k_rod block size (64, 128 or 256).  No counterpart in the reference; results
+ * do not depend on it. */
+int sbx_net_set_shape(sbx_net *net, int bpGroup, int bpStaged, int scanItems, int rodThreads);
 /* Autotuner (no counterpart in the reference): times trialSteps steps (after 2 warm-up steps) for each
  * launch shape, with and without the concurrent streams (sbx_net_set_overlap), and keeps the fastest.
  * The steps are real steps step0, step0+1, ... of the run (results depend on neither): *advanced = steps
- * taken; shape[4] = {bpGroup, scanItems, rodThreads, overlap} and *msPerStep (may be NULL) report the
+ * taken; shape[5] = {bpGroup, bpStaged, scanItems, rodThreads, overlap} and *msPerStep (may be NULL) report the
  * choice. */
 int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *shape, float *msPerStep);
 

@@ -175,7 +175,7 @@ _SIGS = {
     "sbx_net_kernel_times": [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_longlong), i32],
     "sbx_net_use_graph": [vp, i32],
     "sbx_net_set_overlap": [vp, i32],
-    "sbx_net_set_shape": [vp, i32, i32, i32],
+    "sbx_net_set_shape": [vp, i32, i32, i32, i32],
     "sbx_net_tune": [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), f32p],
     "sbx_batch_create": [vp, i32, C.POINTER(SbxFilaments), C.POINTER(SbxCrosslinks), C.POINTER(SbxNetParams),
                          C.POINTER(vp)],

@@ -162,10 +162,10 @@ int launch_broad_phase(sbx_ctx *c, const float *center, const float *boundingRad
 // the same from {centre, bounding radius} and cell laid out in contents order (sortedPose[q] and
 // sortedCell[q] belong to body cellContents[q]; counting_sort's payload)
 // reps == nullptr: every body in grid g; else body i in the grid of replica rodRep[i] (g unused).
-// group: lanes per body, 2, 4 or 8.
+// group: lanes per body, 2, 4 or 8; staged: stage each block's neighbourhood in LDS (4 or 8 lanes).
 int launch_broad_phase_sorted(sbx_ctx *c, const float4 *sortedPose, const int *sortedCell, const int *cellOffsets,
                               const int *cellContents, const GridGeom &g, const RepInfo *reps, const int *rodRep,
-                              int group, int cap, int S, int *candPartner, int *candCount, int maxCand,
+                              int group, int staged, int cap, int S, int *candPartner, int *candCount, int maxCand,
                               hipStream_t st = nullptr);
 
 }  // namespace sbx

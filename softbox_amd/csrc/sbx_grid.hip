@@ -186,6 +186,158 @@ int launch_broad_phase(sbx_ctx *c, const float *center, const float *boundingRad
     return check_launch(c, "k_broad_phase");
 }
 
+// LDS-staged form of the same query (SpatialGrid.java:340-406), for dense grids.  A block takes
+// BP_THREADS / G consecutive bodies of the sorted contents, whose cells run from cFirst to cLast.
+// For each of the 9 (dz, dy) rows of the stencil, every body's row lies in the cells cFirst + dz nXY
+// + dy nX - 1 .. cLast + dz nXY + dy nX + 1, one contiguous range of the sorted contents: the block
+// copies those 9 ranges (candidate index and packed pose) into LDS once, and each body walks its own
+// stencil -- the same 18 ranges in the same order as k_broad_phase -- reading LDS for every range that
+// lies inside its row's staged range and global memory for the rest (a periodic wrap or a grid edge
+// of another replica).  Hits are ordered exactly as in k_broad_phase.  A block whose ranges exceed
+// BPS_CAP entries, or whose bodies span two replicas, walks global memory throughout.
+constexpr int BPS_CAP = 1536;  // staged entries per block (20 B each)
+
+template <typename Geo, int G>
+__global__ void __launch_bounds__(BP_THREADS) k_broad_phase_staged(SortedPose pose, Geo geo,
+                                                                   const int *__restrict__ cellOffsets,
+                                                                   const int *__restrict__ cellContents, int cap,
+                                                                   int nSorted, int *__restrict__ candPartner,
+                                                                   int *__restrict__ candCount, int maxCand) {
+    static_assert(64 % G == 0, "groups tile a wave");
+    constexpr int U = BP_ROUND;
+    constexpr int TB = BP_THREADS / G;  // bodies per block
+    __shared__ int sJ[BPS_CAP];
+    __shared__ float4 sP[BPS_CAP];
+    __shared__ int slabG[9], slabN[9], slabL[9], staged;
+    const int tid = threadIdx.x, lane = tid & 63, gl = lane % G;
+    const int t0 = xcd_block(blockIdx.x, gridDim.x) * TB;
+    const int t = t0 + tid / G;
+    const int tEnd = min(nSorted, min(cap, t0 + TB));  // bodies [t0, tEnd) have cells
+    // the 9 staged ranges, from the first and the last body's cells (thread k < 9: row (dz, dy))
+    if (tid < 9) {
+        int g0 = 0, n = 0;
+        if (t0 < tEnd) {
+            const int iF = cellContents[t0], iL = cellContents[tEnd - 1];
+            const BodyGeo bF = geo.of(iF), bL = geo.of(iL);
+            if (bF.cellBase == bL.cellBase) {  // one replica
+                const int nX = bF.g.nX, nXY = bF.g.nX * bF.g.nY, cells = nXY * bF.g.nZ;
+                const int off = (tid / 3 - 1) * nXY + (tid % 3 - 1) * nX;
+                int a = pose.cell(t0, iF) + off - 1, b = pose.cell(tEnd - 1, iL) + off + 1;
+                a = max(a, bF.cellBase);
+                b = min(b, bF.cellBase + cells - 1);
+                if (a <= b) {
+                    g0 = cellOffsets[a];
+                    n = cellOffsets[b + 1] - g0;
+                }
+            } else {
+                n = -1;  // bodies of two replicas: no staging
+            }
+        }
+        slabG[tid] = g0;
+        slabN[tid] = n;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int tot = 0, ok = 1;
+        for (int k = 0; k < 9; k++) {
+            ok = ok && slabN[k] >= 0;
+            slabL[k] = tot;
+            tot += max(slabN[k], 0);
+        }
+        staged = ok && tot <= BPS_CAP;
+    }
+    __syncthreads();
+    if (staged) {
+        for (int k = 0; k < 9; k++) {
+            const int g0 = slabG[k], n = slabN[k], l0 = slabL[k];
+            for (int e = tid; e < n; e += BP_THREADS) {
+                sJ[l0 + e] = cellContents[g0 + e];
+                sP[l0 + e] = pose.p[g0 + e];
+            }
+        }
+    }
+    __syncthreads();
+    if (t >= cap) return;  // whole groups leave together; no barrier follows
+    if (t >= nSorted) {
+        if (gl == 0) candCount[t] = 0;
+        return;
+    }
+    const int i = cellContents[t];
+    const int c = pose.cell(t, i);
+    if (c < 0) {
+        if (gl == 0) candCount[i] = 0;
+        return;
+    }
+    const float4 qi = pose.own(t, i);
+    const BodyGeo bg = geo.of(i);
+    const GridGeom &g = bg.g;
+    int pre[18], shift[18];
+    const int total = stencil_map(c - bg.cellBase, g, cellOffsets + bg.cellBase, pre, shift);
+    // ranges inside their row's staged range read LDS: their shift then maps a walk position to an
+    // LDS index (bit r of inLds)
+    unsigned inLds = 0;
+    if (staged) {
+#pragma unroll
+        for (int r = 0; r < 18; r++) {
+            const int k = r >> 1;
+            const int lo = pre[r] + shift[r];
+            const int hi = lo + ((r < 17 ? pre[r + 1] : total) - pre[r]);
+            if (lo >= slabG[k] && hi <= slabG[k] + slabN[k]) {
+                shift[r] += slabL[k] - slabG[k];
+                inLds |= 1u << r;
+            }
+        }
+    }
+    int *slice = candPartner + (size_t)i * (size_t)maxCand;
+    const unsigned long long groupBits = G == 64 ? ~0ull : ((1ull << G) - 1) << ((lane / G) * G);
+    const unsigned long long below = (1ull << lane) - 1;
+    int cnt = 0;
+    for (int base = 0; base < total; base += G * U) {
+        int at[U], jj[U];
+        bool fromLds[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int q = base + k * G + gl;
+            int sh = shift[0], rr = 0;
+#pragma unroll
+            for (int r = 1; r < 18; r++)
+                if (q >= pre[r]) { sh = shift[r]; rr = r; }
+            fromLds[k] = q < total && ((inLds >> rr) & 1u);
+            at[k] = q < total ? q + sh : t;
+        }
+        float4 qj[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const bool valid = base + k * G + gl < total;
+            jj[k] = !valid ? -1 : (fromLds[k] ? sJ[at[k]] : cellContents[at[k]]);
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const bool use = jj[k] > i;
+            qj[k] = !use ? qi : (fromLds[k] ? sP[at[k]] : pose.at(at[k], jj[k]));
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const bool hit = jj[k] > i && bp_hit(qi, qj[k], g);
+            const unsigned long long m = __ballot(hit) & groupBits;
+            if (hit) {
+                const int slot = cnt + __popcll(m & below);
+                if (slot < maxCand) slice[slot] = jj[k];
+            }
+            cnt += __popcll(m);
+        }
+    }
+    if (gl == 0) candCount[i] = cnt;
+}
+
+template <typename Geo, int G>
+void launch_staged(SortedPose pose, Geo geo, const int *cellOffsets, const int *cellContents, int cap, int S,
+                   int *candPartner, int *candCount, int maxCand, hipStream_t st) {
+    auto kern = k_broad_phase_staged<Geo, G>;
+    hipLaunchKernelGGL(kern, dim3(blocks_for((long long)cap * G, BP_THREADS)), dim3(BP_THREADS), 0, st, pose, geo,
+                       cellOffsets, cellContents, cap, S, candPartner, candCount, maxCand);
+}
+
 template <typename Geo, int G>
 void launch_sorted(sbx_ctx *c, SortedPose pose, Geo geo, const int *cellOffsets, const int *cellContents, int cap,
                    int S, int *candPartner, int *candCount, int maxCand, hipStream_t st) {
@@ -196,11 +348,28 @@ void launch_sorted(sbx_ctx *c, SortedPose pose, Geo geo, const int *cellOffsets,
 
 int launch_broad_phase_sorted(sbx_ctx *c, const float4 *sortedPose, const int *sortedCell, const int *cellOffsets,
                               const int *cellContents, const GridGeom &g, const RepInfo *reps, const int *rodRep,
-                              int group, int cap, int S, int *candPartner, int *candCount, int maxCand,
+                              int group, int staged, int cap, int S, int *candPartner, int *candCount, int maxCand,
                               hipStream_t st) {
     SBX_REQUIRE(c, group == 2 || group == 4 || group == 8, "broad phase: %d lanes per body (2, 4 or 8)", group);
     SortedPose pose{sortedPose, sortedCell};
     hipStream_t q = st ? st : c->stream;
+    if (staged) {
+        SBX_REQUIRE(c, group == 4 || group == 8, "staged broad phase: 4 or 8 lanes per body");
+        if (reps) {
+            const BatchGeo geo{reps, rodRep};
+            if (group == 8) launch_staged<BatchGeo, 8>(pose, geo, cellOffsets, cellContents, cap, S, candPartner,
+                                                       candCount, maxCand, q);
+            else launch_staged<BatchGeo, 4>(pose, geo, cellOffsets, cellContents, cap, S, candPartner, candCount,
+                                            maxCand, q);
+        } else {
+            const SingleGeo geo{g};
+            if (group == 8) launch_staged<SingleGeo, 8>(pose, geo, cellOffsets, cellContents, cap, S, candPartner,
+                                                        candCount, maxCand, q);
+            else launch_staged<SingleGeo, 4>(pose, geo, cellOffsets, cellContents, cap, S, candPartner, candCount,
+                                             maxCand, q);
+        }
+        return check_launch(c, "k_broad_phase_staged");
+    }
     if (reps) {
         const BatchGeo geo{reps, rodRep};
         if (group == 8) launch_sorted<BatchGeo, 8>(c, pose, geo, cellOffsets, cellContents, cap, S, candPartner,

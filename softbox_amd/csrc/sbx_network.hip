@@ -121,6 +121,7 @@ struct sbx_net {
     int *rodRep = nullptr, *linkRep = nullptr;        // replica of each rod / link
     // launch shape (results do not depend on it; sbx_net_tune picks it by timing steps)
     int bpGroup = 4;      // broad-phase lanes per body (2, 4 or 8)
+    int bpStaged = 0;     // 1: each block's neighbourhood staged in LDS (4 or 8 lanes)
     int rodThreads = 256; // k_rod block size (64, 128 or 256)
     // graph
     int useGraph = 0;
@@ -1284,8 +1285,9 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
                 if (ovl && (r = check_hip(c, hipStreamWaitEvent(s2, net->evPlaced[cb], 0), "wait evPlaced")))
                     return r;
                 r = launch_broad_phase_sorted(c, net->sortedPose[cb], net->sortedCell[cb], net->csrOff[cb],
-                                              net->csrCont[cb], net->geom, net->reps, net->rodRep, net->bpGroup, n,
-                                              n, net->candPartner, net->candCount, net->p.maxCand, s2);
+                                              net->csrCont[cb], net->geom, net->reps, net->rodRep, net->bpGroup,
+                                              net->bpStaged, n, n, net->candPartner, net->candCount, net->p.maxCand,
+                                              s2);
                 if (r) return r;
                 if (ovl && (r = check_hip(c, hipEventRecord(net->evJoin[cb], s2), "record evJoin"))) return r;
             }
@@ -2060,17 +2062,20 @@ int drop_graph(sbx_net *net) {
 
 extern "C" {
 
-int sbx_net_set_shape(sbx_net *net, int bpGroup, int scanItems, int rodThreads) {
+int sbx_net_set_shape(sbx_net *net, int bpGroup, int bpStaged, int scanItems, int rodThreads) {
     if (!net) return SBX_ERR_ARG;
     sbx_ctx *c = net->ctx;
     SBX_CHECK_CTX(c);
     SBX_REQUIRE(c, bpGroup == 2 || bpGroup == 4 || bpGroup == 8, "net_set_shape: bpGroup 2, 4 or 8");
+    SBX_REQUIRE(c, bpStaged == 0 || (bpStaged == 1 && bpGroup != 2), "net_set_shape: bpStaged 0, or 1 with 4 or 8 "
+                   "lanes per body");
     SBX_REQUIRE(c, scanItems == 4 || scanItems == 8 || scanItems == 16, "net_set_shape: scanItems 4, 8 or 16");
     SBX_REQUIRE(c, rodThreads == 64 || rodThreads == 128 || rodThreads == 256, "net_set_shape: rodThreads 64, 128 or 256");
     int r = set_device(c);
     if (!r) r = drop_graph(net);
     if (r) return r;
     net->bpGroup = bpGroup;
+    net->bpStaged = bpStaged;
     net->sort.scanItems = scanItems;
     net->rodThreads = rodThreads;
     return SBX_OK;
@@ -2088,13 +2093,15 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
     if ((r = check_hip(c, hipEventCreate(&e1), "tune event"))) { hipEventDestroy(e0); return r; }
     const int groups[3] = {4, 2, 8}, items[3] = {8, 4, 16}, threads[2] = {256, 128};
     const int overlaps = net->side ? 2 : 1;  // with streams: the overlapped and the one-stream step
-    int t = step0, best[4] = {net->bpGroup, net->sort.scanItems, net->rodThreads, net->overlap};
+    int t = step0, best[5] = {net->bpGroup, net->bpStaged, net->sort.scanItems, net->rodThreads, net->overlap};
     float bestMs = -1.0f;
     for (int oi = 0; oi < overlaps && !r; oi++)
     for (int gi = 0; gi < 3 && !r; gi++)
+    for (int si = 0; si < 2 && !r; si++)
         for (int ii = 0; ii < 3 && !r; ii++)
             for (int ti = 0; ti < 2 && !r; ti++) {
-                if ((r = sbx_net_set_shape(net, groups[gi], items[ii], threads[ti]))) break;
+                if (si == 1 && groups[gi] == 2) continue;  // staging takes 4 or 8 lanes per body
+                if ((r = sbx_net_set_shape(net, groups[gi], si, items[ii], threads[ti]))) break;
                 net->overlap = oi == 0 ? 1 : 0;
                 if ((r = sbx_net_step(net, t, 2))) break;  // warm (and capture, with graphs on)
                 t += 2;
@@ -2110,16 +2117,18 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
                 ms /= (float)trialSteps;
                 if (bestMs < 0.0f || ms < bestMs) {
                     bestMs = ms;
-                    best[0] = groups[gi]; best[1] = items[ii]; best[2] = threads[ti]; best[3] = net->overlap;
+                    best[0] = groups[gi]; best[1] = si; best[2] = items[ii]; best[3] = threads[ti];
+                    best[4] = net->overlap;
                 }
             }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     *advanced = t - step0;
     if (r) return r;
-    if ((r = sbx_net_set_shape(net, best[0], best[1], best[2]))) return r;
-    net->overlap = best[3];
-    if (shape) { shape[0] = best[0]; shape[1] = best[1]; shape[2] = best[2]; shape[3] = best[3]; }
+    if ((r = sbx_net_set_shape(net, best[0], best[1], best[2], best[3]))) return r;
+    net->overlap = best[4];
+    if (shape)
+        for (int k = 0; k < 5; k++) shape[k] = best[k];
     if (msPerStep) *msPerStep = bestMs;
     return SBX_OK;
 }

@@ -81,16 +81,16 @@ class Network:
         """Broad phase on a second stream, concurrent with the mechanics kernels (default on)."""
         self.ctx.check(self.ctx.lib.sbx_net_set_overlap(self.h, 1 if on else 0))
 
-    def set_shape(self, bp_group: int = 4, scan_items: int = 8, rod_threads: int = 256) -> None:
+    def set_shape(self, bp_group: int = 4, scan_items: int = 8, rod_threads: int = 256, bp_staged: int = 0) -> None:
         """Launch shape of the step (results do not depend on it)."""
-        self.ctx.check(self.ctx.lib.sbx_net_set_shape(self.h, bp_group, scan_items, rod_threads))
+        self.ctx.check(self.ctx.lib.sbx_net_set_shape(self.h, bp_group, bp_staged, scan_items, rod_threads))
 
     def tune(self, step0: int, trial_steps: int = 20) -> dict:
         """Time every launch shape (with and without the concurrent streams) over real steps step0,
         step0+1, ... and keep the fastest; returns {"steps": steps taken, "shape": (bp_group,
-        scan_items, rod_threads, overlap), "ms_per_step": best}."""
+        bp_staged, scan_items, rod_threads, overlap), "ms_per_step": best}."""
         adv = C.c_int()
-        shape = (C.c_int * 4)()
+        shape = (C.c_int * 5)()
         ms = C.c_float()
         self.ctx.check(self.ctx.lib.sbx_net_tune(self.h, step0, trial_steps, C.byref(adv), shape, C.byref(ms)))
         return {"steps": adv.value, "shape": tuple(shape), "ms_per_step": ms.value}

@@ -73,8 +73,8 @@ def _check_replica(bt, r, sc, want, what):
     assert_same(cell, wcell, f"{what}: replica {r} bodyCell")
 
 
-@pytest.mark.parametrize("graph,overlap", [(False, False), (False, True), (True, True)])
-def test_batch_replicas_equal_stepping_alone(ctx, graph, overlap):
+@pytest.mark.parametrize("graph,overlap,staged", [(False, False, 0), (False, True, 0), (True, True, 0), (True, True, 1)])
+def test_batch_replicas_equal_stepping_alone(ctx, graph, overlap, staged):
     from softbox_amd.batch import Batch
 
     reps = _replicas()
@@ -82,12 +82,14 @@ def test_batch_replicas_equal_stepping_alone(ctx, graph, overlap):
     want = [_alone(ctx, sc, steps, graph=graph, overlap=overlap) for sc in reps]
     bt = Batch(ctx, [sc.copy() for sc in reps])
     bt.net.set_overlap(overlap)
+    if staged:
+        bt.net.set_shape(4, 8, 256, 1)
     if graph:
         bt.net.use_graph(2)
     bt.step(0, 5)
     bt.step(5, steps - 5)
     for r, sc in enumerate(reps):
-        _check_replica(bt, r, sc, want[r], f"graph={graph} overlap={overlap}")
+        _check_replica(bt, r, sc, want[r], f"graph={graph} overlap={overlap} staged={staged}")
     bt.close()
 
 
@@ -125,12 +127,12 @@ def test_launch_shapes_and_tuner_change_only_time(ctx):
     reps = _replicas(seed=5)
     bt = Batch(ctx, [sc.copy() for sc in reps])
     t = 0
-    for shape in ((2, 4, 64), (4, 16, 128), (8, 8, 256)):
+    for shape in ((2, 4, 64, 0), (4, 16, 128, 1), (8, 8, 256, 0), (8, 4, 256, 1), (4, 8, 256, 0)):
         bt.net.set_shape(*shape)
         bt.step(t, 3)
         t += 3
-    res = bt.net.tune(t, trial_steps=1)  # 3 x 3 x 2 shapes, overlapped and not, 2 + 1 steps each
-    assert res["steps"] == 36 * 3 and res["shape"][0] in (2, 4, 8) and res["ms_per_step"] > 0, res
+    res = bt.net.tune(t, trial_steps=1)  # (1 + 2 + 2) x 3 x 2 shapes, overlapped and not, 2 + 1 steps each
+    assert res["steps"] == 60 * 3 and res["shape"][0] in (2, 4, 8) and res["ms_per_step"] > 0, res
     t += res["steps"]
     bt.step(t, 2)
     t += 2

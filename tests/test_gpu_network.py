@@ -99,6 +99,28 @@ def test_link_edits_between_steps_are_recounted(ctx, graph):
     net.close()
 
 
+@pytest.mark.parametrize("lanes", [4, 8])
+@pytest.mark.parametrize("periodic,dense", [(True, False), (True, True), (False, True)])
+def test_staged_broad_phase_matches_oracle(ctx, lanes, periodic, dense):
+    """The LDS-staged broad phase (each block's 9 neighbourhood ranges in LDS; ranges outside them,
+    at periodic wraps and grid edges, read global memory) gives the reference's slices and grid."""
+    from softbox_amd.network import Network
+
+    if dense:  # ~2 bodies per fine cell: staged ranges of hundreds of entries, some blocks over capacity
+        sc = scenes.network(n_fil=500, segs=10, box=1.6 if periodic else 1.2, n_links=800, seed=12,
+                            periodic=periodic, confine=False)
+    else:
+        sc = small_network(seed=13, n_fil=300, links=600, periodic=periodic)
+    orc = ffi.OracleNet(sc)
+    net = Network(ctx, sc)
+    net.set_shape(lanes, 8, 256, 1)
+    for t in range(6):
+        orc.step(t, 1)
+    net.step(0, 6)
+    _compare(net, orc, sc, f"staged lanes={lanes} periodic={periodic} dense={dense}")
+    net.close()
+
+
 def test_network_box_mode_with_walls(ctx):
     """Reference parity mode: clamped grid + containment box; rods start partly outside the walls."""
     from softbox_amd.network import Network
