@@ -296,8 +296,9 @@ def main() -> int:
         "metric": METRIC, "value": round(head["value"], 1), "unit": "rod-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(head["ms_per_step"], 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 state, f64 force laws", "data": "synthetic (seeded scene)",
-        "config": {"workload": "crosslinked actin network: 20k filaments x 10 rods (0.1 um), 50k crosslinkers, "
-                               f"20 um periodic cube, dt 1e-5 s, viscosity {args.eta:g} Pa s",
+        "config": {"workload": f"crosslinked actin network: {args.n_fil // 1000}k filaments x 10 rods (0.1 um), "
+                               f"{args.n_links // 1000}k crosslinkers, {args.box:g} um periodic cube, dt 1e-5 s, "
+                               f"viscosity {args.eta:g} Pa s",
                    "rods": n, "links": sc.xl.nLinks, "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
                    "broad_phase": bp},
         "roofline": roof, "cpu_baseline": cpu,

@@ -89,22 +89,46 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *total) {
     return waveOff + inc - v;
 }
 
-// Exclusive scan of counts[n] into out[n+1] given tileSums[t] (the sum of tile t): each tile's
-// prefix is the sum of the tile sums before it -- a block-wide reduction over at most n/2048
-// L2-resident ints -- so no tile waits on another.  out[n] = grand total, clamped to clampTotal
-// when >= 0.
+// Exclusive scan of tileSums[nTiles] into tilePrefix[nTiles + 1] (tilePrefix[nTiles] = the grand
+// total): one block, each thread a contiguous run of tiles.
+__global__ void __launch_bounds__(1024) k_tile_prefix(const int *__restrict__ tileSums, int nTiles,
+                                                      int *__restrict__ tilePrefix, const int *__restrict__ gate) {
+    if (gate && *gate == 0) return;
+    __shared__ int waveSums[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int per = (nTiles + 1023) / 1024;
+    const int b = threadIdx.x * per;
+    int sum = 0;
+    for (int k = 0; k < per; k++) sum += b + k < nTiles ? tileSums[b + k] : 0;
+    const int inc = wave_inclusive_scan(sum);
+    if (lane == 63) waveSums[wave] = inc;
+    __syncthreads();
+    int run = inc - sum;
+    for (int w = 0; w < wave; w++) run += waveSums[w];
+    for (int k = 0; k < per; k++) {
+        if (b + k >= nTiles) break;
+        tilePrefix[b + k] = run;
+        run += tileSums[b + k];
+    }
+    if (threadIdx.x == 1023) {
+        int grand = 0;
+        for (int w = 0; w < 16; w++) grand += waveSums[w];
+        tilePrefix[nTiles] = grand;
+    }
+}
+
+// Exclusive scan of counts[n] into out[n+1] given tilePrefix[t] (the sum of the tiles before tile t,
+// tilePrefix[nTiles] = the grand total), so no tile waits on another.  out[n] = grand total, clamped
+// to clampTotal when >= 0.
 template <int SCAN_ITEMS>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int *__restrict__ in, int n,
-                                                             const int *__restrict__ tileSums, int nTiles,
+                                                             const int *__restrict__ tilePrefix, int nTiles,
                                                              int *__restrict__ out, int clampTotal,
                                                              const int *__restrict__ gate) {
     if (gate && *gate == 0) return;
     constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
-    __shared__ int red[SCAN_THREADS / 64];
     const int tile = blockIdx.x;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // this thread's 8 consecutive items (two 16-byte loads when the tile is full), issued before the
-    // tile-prefix reduction so both sets of loads are in flight together
+    // this thread's SCAN_ITEMS consecutive items (16-byte loads when the tile is full)
     long long base = (long long)tile * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS];
     if ((long long)(tile + 1) * SCAN_TILE <= n) {
@@ -118,40 +142,21 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int *__restri
 #pragma unroll
         for (int k = 0; k < SCAN_ITEMS; k++) v[k] = (base + k < n) ? in[base + k] : 0;
     }
-    // prefix of this tile (and, for the last tile, the grand total)
-    int part = 0, lastPart = 0;
-    for (int t = threadIdx.x; t < nTiles; t += SCAN_THREADS) {
-        int ts = tileSums[t];
-        part += t < tile ? ts : 0;
-        lastPart += ts;
-    }
-    part = wave_sum(part);
-    if (lane == 0) red[wave] = part;
-    __syncthreads();
-    int tilePrefix = 0;
-#pragma unroll
-    for (int w = 0; w < SCAN_THREADS / 64; w++) tilePrefix += red[w];
-    __syncthreads();
+    const int prefix = tilePrefix[tile];
     int sum = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) sum += v[k];
     int total;
-    int pre = block_exclusive_scan(sum, &total) + tilePrefix;
+    int pre = block_exclusive_scan(sum, &total) + prefix;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
         long long idx = base + k;
         if (idx < n) out[idx] = pre;
         pre += v[k];
     }
-    if (tile == nTiles - 1) {
-        lastPart = wave_sum(lastPart);
-        if (lane == 0) red[wave] = lastPart;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int grand = 0;
-            for (int w = 0; w < SCAN_THREADS / 64; w++) grand += red[w];
-            out[n] = (clampTotal >= 0 && grand > clampTotal) ? clampTotal : grand;
-        }
+    if (tile == nTiles - 1 && threadIdx.x == 0) {
+        const int grand = tilePrefix[nTiles];
+        out[n] = (clampTotal >= 0 && grand > clampTotal) ? clampTotal : grand;
     }
 }
 
@@ -160,11 +165,14 @@ int sort_scratch_reserve(sbx_ctx *c, SortScratch &s, long long nKeys, long long 
     if (nKeys > s.keyCap || nt > s.tileCap) {
         if (s.counts) hipFree(s.counts);
         if (s.tileSums) hipFree(s.tileSums);
-        s.counts = s.tileSums = nullptr;
+        if (s.tilePrefix) hipFree(s.tilePrefix);
+        s.counts = s.tileSums = s.tilePrefix = nullptr;
         s.keyCap = s.tileCap = 0;
         int r = check_hip(c, hipMalloc(&s.counts, sizeof(int) * (size_t)(nKeys + 1)), "hipMalloc counts");
         if (r) return r;
         r = check_hip(c, hipMalloc(&s.tileSums, sizeof(int) * (size_t)(nt + 1)), "hipMalloc tileSums");
+        if (r) return r;
+        r = check_hip(c, hipMalloc(&s.tilePrefix, sizeof(int) * (size_t)(nt + 1)), "hipMalloc tilePrefix");
         if (r) return r;
         s.keyCap = nKeys;
         s.tileCap = nt;
@@ -193,6 +201,7 @@ int sort_scratch_reserve(sbx_ctx *c, SortScratch &s, long long nKeys, long long 
 void sort_scratch_free(SortScratch &s) {
     if (s.counts) hipFree(s.counts);
     if (s.tileSums) hipFree(s.tileSums);
+    if (s.tilePrefix) hipFree(s.tilePrefix);
     if (s.tmp) hipFree(s.tmp);
     if (s.slot) hipFree(s.slot);
     s = SortScratch();
@@ -204,8 +213,8 @@ void sort_scratch_free(SortScratch &s) {
 // for a sparse grid is ~10x larger than the item count.
 //
 // 1. k_key_count:   slot[m] = atomicAdd(counts[key]) (a provisional, unordered position);
-// 2. k_tile_reduce: tileSums = per-tile sums of counts;
-// 3. k_scan_tiles:  offsets = exclusive scan of counts (tile prefixes from the tile sums);
+// 2. k_tile_reduce: tileSums = per-tile sums of counts; k_tile_prefix: their exclusive scan (one block);
+// 3. k_scan_tiles:  offsets = exclusive scan of counts (each tile from its prefix);
 // 4. k_key_scatter: tmp[offsets[key] + slot[m]] = m; counts[key] = 0;
 // 5. k_key_place:   rank of m among its key's segment (items with a smaller index) -> list.
 // The result is independent of the atomic order: ascending item index within each key.
@@ -287,23 +296,27 @@ __global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, 
     }
 }
 
-// offsets = exclusive scan of counts[nKeys] (tile sums, then the tiles): scanItems keys per thread
+// offsets = exclusive scan of counts[nKeys]: tile sums, their prefix (one block), then the tiles
+// (scanItems keys per thread)
 template <int ITEMS>
-void launch_scan_items(const int *counts, int nKeys, int *tileSums, int *offsets, int clampTotal, const int *gate,
-                       hipStream_t q) {
+void launch_scan_items(const SortScratch &s, const int *counts, int nKeys, int *offsets, int clampTotal,
+                       const int *gate, hipStream_t q) {
     int nt = (nKeys + SCAN_THREADS * ITEMS - 1) / (SCAN_THREADS * ITEMS);
     if (nt < 1) nt = 1;
-    hipLaunchKernelGGL(k_tile_reduce<ITEMS>, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, tileSums, gate);
-    hipLaunchKernelGGL(k_scan_tiles<ITEMS>, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, tileSums, nt, offsets,
-                       clampTotal, gate);
+    hipLaunchKernelGGL(k_tile_reduce<ITEMS>, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, s.tileSums, gate);
+    hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(1024), 0, q, s.tileSums, nt, s.tilePrefix, gate);
+    hipLaunchKernelGGL(k_scan_tiles<ITEMS>, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, s.tilePrefix, nt,
+                       offsets, clampTotal, gate);
 }
 
 void launch_scan(const SortScratch &s, const int *counts, int nKeys, int *offsets, int clampTotal, const int *gate,
                  hipStream_t q) {
-    if (s.scanItems == 4) launch_scan_items<4>(counts, nKeys, s.tileSums, offsets, clampTotal, gate, q);
-    else if (s.scanItems == 16) launch_scan_items<16>(counts, nKeys, s.tileSums, offsets, clampTotal, gate, q);
-    else launch_scan_items<8>(counts, nKeys, s.tileSums, offsets, clampTotal, gate, q);
+    if (s.scanItems == 4) launch_scan_items<4>(s, counts, nKeys, offsets, clampTotal, gate, q);
+    else if (s.scanItems == 16) launch_scan_items<16>(s, counts, nKeys, offsets, clampTotal, gate, q);
+    else launch_scan_items<8>(s, counts, nKeys, offsets, clampTotal, gate, q);
 }
+
+
 
 int count_keys(sbx_ctx *c, const int *key, int nItems, int nKeys, int *counts, int *slot, hipStream_t st) {
     if (nItems <= 0) return SBX_OK;

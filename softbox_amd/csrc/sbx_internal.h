@@ -12,7 +12,7 @@ namespace sbx {
 // Scratch of the stable counting sort (keys -> CSR offsets + ascending item list).
 // counts[] is kept all-zero between sorts (`clean`), so no per-sort fill is needed.
 struct SortScratch {
-    int *counts = nullptr, *tileSums = nullptr, *tmp = nullptr, *slot = nullptr;
+    int *counts = nullptr, *tileSums = nullptr, *tilePrefix = nullptr, *tmp = nullptr, *slot = nullptr;
     long long keyCap = 0, itemCap = 0, tileCap = 0;
     bool clean = false;
     int scanItems = 8;  // keys per scan thread: tiles of 256 * scanItems keys (4, 8 or 16; same result)
