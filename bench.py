@@ -48,8 +48,8 @@ def kernel_bytes(sc, name: str, list_entries: int | None = None) -> float | None
         return 260.0 * n + 32.0 * 2 * L
     if name == "k_net_link":
         # ids 8 + locs 8 + state 4 + orient 4 + torque ring 20+4 + endpoints A/B (end1, u, coord, gamma) 2x40
-        # + both segments' CSR bounds 2x16; write payload 48 + ring slot 4 + place 4
-        return 224.0 * L
+        # + both segments' active-link counts 2x4; write payload 48 + ring slot 4 + place 4
+        return 192.0 * L
     if name == "k_net_unbind":
         return (8 + 8 + 4 + 40 + 4 + 2 * 24 + 40 + 4) * float(L)
     if name == "k_body_cell":
