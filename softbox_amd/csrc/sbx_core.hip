@@ -99,16 +99,18 @@ __global__ void __launch_bounds__(1024) k_tile_prefix(const int *__restrict__ ti
     const int per = (nTiles + 1023) / 1024;
     const int b = threadIdx.x * per;
     int sum = 0;
+#pragma unroll 8
     for (int k = 0; k < per; k++) sum += b + k < nTiles ? tileSums[b + k] : 0;
     const int inc = wave_inclusive_scan(sum);
     if (lane == 63) waveSums[wave] = inc;
     __syncthreads();
     int run = inc - sum;
     for (int w = 0; w < wave; w++) run += waveSums[w];
+#pragma unroll 8
     for (int k = 0; k < per; k++) {
-        if (b + k >= nTiles) break;
-        tilePrefix[b + k] = run;
-        run += tileSums[b + k];
+        const int v = b + k < nTiles ? tileSums[b + k] : 0;
+        if (b + k < nTiles) tilePrefix[b + k] = run;
+        run += v;
     }
     if (threadIdx.x == 1023) {
         int grand = 0;
