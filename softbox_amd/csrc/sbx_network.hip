@@ -107,6 +107,10 @@ struct sbx_net {
     hipEvent_t evFork = nullptr, evBuilt = nullptr;
     hipEvent_t evRod[2] = {nullptr, nullptr}, evPlaced[2] = {nullptr, nullptr}, evJoin[2] = {nullptr, nullptr};
     int needFork = 1;  // the side streams must be ordered after the main stream before their next work
+    // 1: the latest record of some event was made inside a stream capture (a captured record stands
+    // for no work scheduled outside its graph) and the main stream has seen only captures and graph
+    // replays since; settle_events re-records them eagerly before any wait outside a capture
+    int evCaptured = 0;
     int *csrOff[2] = {nullptr, nullptr}, *csrCont[2] = {nullptr, nullptr};  // [0] = cellOffsets/cellContents
     float4 *sortedPose[2] = {nullptr, nullptr};  // bodyPack in contents order (the sort's place pass)
     int *sortedCell[2] = {nullptr, nullptr};     // bodyCell in contents order
@@ -132,6 +136,15 @@ struct sbx_net {
     int *graphLastCell = nullptr;  // cell buffer of the captured graph's last step
     hipGraphExec_t graphExec = nullptr;
     hipGraph_t graph = nullptr;
+    // overlapped per-step rebuild: one 1-step graph per pose buffer, each = the previous step's broad
+    // phase + this step's sort + this step's mechanics, three chains with no wait between them
+    hipGraphExec_t unitExec[2] = {nullptr, nullptr};
+    hipGraph_t unitGraph[2] = {nullptr, nullptr};
+    int unitLastCsr[2] = {0, 0};
+    int *unitLastCell[2] = {nullptr, nullptr};
+    int bpPending = 0, bpPendingBuf = 0;  // a broad phase of CSR[buf] still to enqueue
+    int csrValid[2] = {0, 0};             // CSR[b] has been built
+    int joinLive[2] = {0, 0}, placedLive[2] = {0, 0};  // evJoin / evPlaced recorded in the current capture
 };
 
 namespace {
@@ -1210,15 +1223,61 @@ bool overlapped(const sbx_net *net) {
     return net->overlap && net->side && !net->profiling && !net->hasMotors && net->p.gridOn;
 }
 
+// Outside a capture: every event whose latest record was captured is recorded again on the main
+// stream, which follows all the work of the captures and replays launched so far (a graph joins its
+// side chains before it ends), so a later wait on it is a valid, conservative one.  Called before
+// any eager launch that waits on an event.
+int settle_events(sbx_net *net) {
+    if (!net->evCaptured || net->graphSteps) return SBX_OK;
+    sbx_ctx *c = net->ctx;
+    for (hipEvent_t e : {net->evFork, net->evBuilt, net->evRod[0], net->evRod[1], net->evPlaced[0], net->evPlaced[1],
+                         net->evJoin[0], net->evJoin[1]}) {
+        if (!e) continue;
+        int r = check_hip(c, hipEventRecord(e, c->stream), "settle events");
+        if (r) return r;
+    }
+    net->evCaptured = 0;
+    return SBX_OK;
+}
+
 // The main stream waits for all side-stream work enqueued so far (every chain ends in an evJoin
 // record: the broad phase waits for its sort); the next side work forks anew.
 int join_side(sbx_net *net) {
     if (!overlapped(net)) return SBX_OK;
+    if (net->evCaptured && !net->graphSteps) {  // only captures and replays since: already joined
+        net->needFork = 1;
+        return SBX_OK;
+    }
     sbx_ctx *c = net->ctx;
     int r = SBX_OK;
-    for (int b = 0; b < 2 && !r; b++) r = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin[b], 0), "join");
+    // inside a capture only on the capture's own records; a 1-step graph's last sort ends no evJoin
+    // chain, so the sorts are waited for too
+    const bool rebuild = net->skinMode == SBX_SKIN_REBUILD && net->p.gridOn && !net->hasMotors;
+    for (int b = 0; b < 2 && !r; b++) {
+        if (!net->graphSteps || net->joinLive[b])
+            r = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin[b], 0), "join");
+        if (!r && rebuild && (!net->graphSteps || net->placedLive[b]))
+            r = check_hip(c, hipStreamWaitEvent(c->stream, net->evPlaced[b], 0), "join");
+    }
     net->needFork = 1;
     return r;
+}
+
+// The broad phase on CSR[b] (side stream s2): after that CSR's sort, recorded as evJoin[b].
+int enqueue_bp(sbx_net *net, int b, hipStream_t s2, bool ovl, bool waitPlaced) {
+    sbx_ctx *c = net->ctx;
+    int r;
+    Scope sc(net, T_PAIRS);
+    if (ovl && waitPlaced && (r = check_hip(c, hipStreamWaitEvent(s2, net->evPlaced[b], 0), "wait evPlaced")))
+        return r;
+    r = launch_broad_phase_sorted(c, net->sortedPose[b], net->sortedCell[b], net->csrOff[b], net->csrCont[b],
+                                  net->geom, net->reps, net->rodRep, net->bpGroup, net->bpStaged, net->n, net->n,
+                                  net->candPartner, net->candCount, net->p.maxCand, s2);
+    if (r) return r;
+    if (ovl && (r = check_hip(c, hipEventRecord(net->evJoin[b], s2), "record evJoin"))) return r;
+    if (net->graphSteps) net->joinLive[b] = 1;
+    net->lastCsr = b;
+    return SBX_OK;
 }
 
 // One step's launches; stepOff is added to the device step counter.  With overlap the broad phase
@@ -1238,7 +1297,7 @@ int join_side(sbx_net *net) {
 // and after a graph replay, so a wait on an event recorded before the fork is only redundant.
 // capK: index of the step in the graph being captured (-1: eager); a capture waits on no event
 // recorded outside it.
-int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
+int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false) {
     sbx_ctx *c = net->ctx;
     hipStream_t s = c->stream;
     const int n = net->n, L = net->L;
@@ -1248,7 +1307,8 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
     hipStream_t s2 = ovl ? net->side : s, sS = ovl ? net->sortS : s;
     const int cb = net->cur, xb = 1 - cb;
     const bool prev1 = capK < 0 || capK >= 1, prev2 = capK < 0 || capK >= 2;  // step s-1 / s-2 recorded here
-    int r;
+    int r = settle_events(net);
+    if (r) return r;
     if (net->hasMotors) {
         if ((r = enqueue_myosin(net, coord, uVec, yVec, stepOff))) return r;
     } else if (net->p.gridOn) {
@@ -1264,7 +1324,12 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
                 return r;
             if ((r = enqueue_skin(net, coord, uVec, s2, ovl))) return r;
             if (ovl && (r = check_hip(c, hipEventRecord(net->evJoin[cb], s2), "record evJoin"))) return r;
+            if (net->graphSteps) net->joinLive[cb] = 1;
         } else {
+            if (net->bpPending) {  // the broad phase the previous step left; its sort is ordered before
+                if ((r = enqueue_bp(net, net->bpPendingBuf, s2, ovl, !net->graphSteps))) return r;
+                net->bpPending = 0;
+            }
             {
                 Scope sc(net, T_SORT);
                 if (ovl && prev1 && (r = check_hip(c, hipStreamWaitEvent(sS, net->evRod[xb], 0), "wait evRod")))
@@ -1279,19 +1344,15 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1) {
                                           net->csrCont[cb], net->gridCounts[cb], net->gridSlot[cb], sS, pay);
                 if (r) return r;
                 if (ovl && (r = check_hip(c, hipEventRecord(net->evPlaced[cb], sS), "record evPlaced"))) return r;
+                if (net->graphSteps) net->placedLive[cb] = 1;
+                else net->csrValid[cb] = 1;
             }
-            {
-                Scope sc(net, T_PAIRS);
-                if (ovl && (r = check_hip(c, hipStreamWaitEvent(s2, net->evPlaced[cb], 0), "wait evPlaced")))
-                    return r;
-                r = launch_broad_phase_sorted(c, net->sortedPose[cb], net->sortedCell[cb], net->csrOff[cb],
-                                              net->csrCont[cb], net->geom, net->reps, net->rodRep, net->bpGroup,
-                                              net->bpStaged, n, n, net->candPartner, net->candCount, net->p.maxCand,
-                                              s2);
-                if (r) return r;
-                if (ovl && (r = check_hip(c, hipEventRecord(net->evJoin[cb], s2), "record evJoin"))) return r;
+            if (deferBp && ovl) {  // left to the next step (the next 1-step graph's first node)
+                net->bpPending = 1;
+                net->bpPendingBuf = cb;
+            } else if ((r = enqueue_bp(net, cb, s2, ovl, true))) {
+                return r;
             }
-            net->lastCsr = cb;
             if (ovl && prev1 && (r = check_hip(c, hipStreamWaitEvent(s, net->evPlaced[xb], 0), "wait evPlaced")))
                 return r;
         }
@@ -1384,14 +1445,25 @@ int build_link_csr(sbx_net *net) {
     return counting_sort(net->ctx, net->sort, net->xl.linkFilB, net->L, net->n, net->L, net->offB, net->idxB);
 }
 
+// a captured step no longer matches the launch sequence; no event keeps a record made in a capture
+// of the graphs destroyed
+void destroy_graphs(sbx_net *net) {
+    settle_events(net);
+    if (net->graphExec) { hipGraphExecDestroy(net->graphExec); net->graphExec = nullptr; }
+    if (net->graph) { hipGraphDestroy(net->graph); net->graph = nullptr; }
+    for (int b = 0; b < 2; b++) {
+        if (net->unitExec[b]) { hipGraphExecDestroy(net->unitExec[b]); net->unitExec[b] = nullptr; }
+        if (net->unitGraph[b]) { hipGraphDestroy(net->unitGraph[b]); net->unitGraph[b] = nullptr; }
+    }
+}
+
 void release(sbx_net *net) {
     if (!net) return;
     if (net->ctx) hipSetDevice(net->ctx->device);
     if (net->ctx) hipStreamSynchronize(net->ctx->stream);
     if (net->side) hipStreamSynchronize(net->side);
     if (net->sortS) hipStreamSynchronize(net->sortS);
-    if (net->graphExec) hipGraphExecDestroy(net->graphExec);
-    if (net->graph) hipGraphDestroy(net->graph);
+    destroy_graphs(net);
     void *bufs[] = {net->ownPose,  net->boundR,    net->bodyCell, net->cellNext,    net->cellContents, net->linkPack,
                     net->cellOffsets,
                     net->candPartner, net->candCount, net->offA,  net->idxA,        net->offB,
@@ -1667,15 +1739,7 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
     // tethers are static (MiniFilamentSystem.csrBuild keys headBackboneSlot): build the backbone CSR once
     if (m->nBb > 0 && (r = counting_sort(c, net->sort, m->headBackboneSlot, m->nMD, m->nBb, m->nMD, bbOff, bbList)))
         return r;
-    // a captured step no longer matches the launch sequence
-    if (net->graphExec) {
-        hipGraphExecDestroy(net->graphExec);
-        net->graphExec = nullptr;
-    }
-    if (net->graph) {
-        hipGraphDestroy(net->graph);
-        net->graph = nullptr;
-    }
+    destroy_graphs(net);  // a captured step no longer matches the launch sequence
     return sbx_sync(c);
 }
 
@@ -1729,7 +1793,78 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
         if ((r = check_launch(c, "k_link_counts"))) return r;
         net->countsValid = 1;
     }
-    if (net->useGraph && !net->profiling) {
+    const bool units = net->useGraph && !net->profiling && overlapped(net) && net->skinMode == SBX_SKIN_REBUILD &&
+                       !net->hasMotors;
+    if (units) {
+        // one 1-step graph per pose buffer: [broad phase of the previous step | sort | mechanics] on
+        // three streams with no wait between the chains (each hazard's first half is in the previous
+        // graph, complete before this one starts), then the step advance
+        // replays read the other buffer's CSR: it must have been built
+        while (nSteps - done > 0 && !net->csrValid[1 - net->cur]) {
+            if ((r = enqueue_step(net, done - advanced))) return r;
+            done++;
+        }
+        while (nSteps - done > 0) {
+            const int p = net->cur;
+            if (!net->unitExec[p]) {
+                int *savedCell = net->bodyCell, *savedNext = net->cellNext, *savedLast = net->lastCell;
+                const int savedLastCsr = net->lastCsr, savedPending = net->bpPending;
+                if ((r = join_side(net))) return r;
+                if ((r = check_hip(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
+                                   "begin capture")))
+                    return r;
+                net->graphSteps = 1;
+                for (int b = 0; b < 2; b++) net->joinLive[b] = net->placedLive[b] = 0;
+                net->bpPending = 1;
+                net->bpPendingBuf = 1 - p;
+                int er = enqueue_step(net, 0, 0, true);
+                if (!er) er = join_side(net);
+                if (!er) hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, c->stream, net->dStep, 1);
+                hipGraph_t g = nullptr;
+                hipError_t ee = hipStreamEndCapture(c->stream, &g);
+                net->graphSteps = 0;
+                net->evCaptured = 1;
+                for (int b = 0; b < 2; b++) net->joinLive[b] = net->placedLive[b] = 0;
+                net->unitLastCell[p] = net->lastCell;
+                net->cur = p;
+                net->bodyCell = savedCell;
+                net->cellNext = savedNext;
+                net->lastCell = savedLast;
+                net->lastCsr = savedLastCsr;
+                net->bpPending = savedPending;
+                if (er) {
+                    if (g) hipGraphDestroy(g);
+                    return er;
+                }
+                if ((r = check_hip(c, ee, "end capture"))) return r;
+                net->unitGraph[p] = g;
+                if ((r = check_hip(c, hipGraphInstantiate(&net->unitExec[p], g, nullptr, nullptr, 0),
+                                   "graph instantiate")))
+                    return r;
+            }
+            if (done != advanced) {  // a replay reads its step from the device counter: bring it up to date
+                hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, c->stream, net->dStep, done - advanced);
+                if ((r = check_launch(c, "k_add_step"))) return r;
+                advanced = done;
+            }
+            if ((r = join_side(net))) return r;
+            if ((r = check_hip(c, hipGraphLaunch(net->unitExec[p], c->stream), "graph launch"))) return r;
+            net->needFork = 1;
+            net->evCaptured = 1;
+            // the replay ran the pending broad phase (or re-ran the last one), sorted and stepped pose
+            // buffer p and left its broad phase; the host mirrors its pointer swaps
+            net->csrValid[0] = net->csrValid[1] = 1;
+            net->bpPending = 1;
+            net->bpPendingBuf = p;
+            net->lastCell = net->unitLastCell[p];
+            int *t = net->bodyCell;
+            net->bodyCell = net->cellNext;
+            net->cellNext = t;
+            net->cur = 1 - p;
+            done++;
+            advanced++;
+        }
+    } else if (net->useGraph && !net->profiling) {
         // one captured graph = graphLen steps (even: the pose ping-pong returns to its start buffers) +
         // step advance; a longer graph drains the stream pipeline less often
         const int G = net->graphLen;
@@ -1749,6 +1884,7 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
             hipGraph_t g = nullptr;
             hipError_t ee = hipStreamEndCapture(c->stream, &g);
             net->graphSteps = 0;
+            net->evCaptured = 1;
             net->cur = savedCur;
             net->graphCur = savedCur;
             if (er) {
@@ -1773,6 +1909,7 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
             if ((r = join_side(net))) return r;
             if ((r = check_hip(c, hipGraphLaunch(net->graphExec, c->stream), "graph launch"))) return r;
             net->needFork = 1;  // the replay's side work is joined into it; later side work forks anew
+            net->evCaptured = 1;
             net->lastCsr = net->graphLastCsr;
             net->lastCell = net->graphLastCell;
             done += G;
@@ -1781,6 +1918,20 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
     }
     for (; done < nSteps; done++) {
         if ((r = enqueue_step(net, done - advanced))) return r;
+    }
+    if (net->bpPending) {  // the last replay's broad phase: the slices of the call's last pose
+        const bool ovl = overlapped(net);
+        // a replay records none of its events outside the graph: the side stream follows the replay
+        // (which sorted this CSR) through a fork from the main stream, not through evPlaced
+        if ((r = settle_events(net))) return r;
+        if (ovl && net->needFork) {
+            if ((r = check_hip(c, hipEventRecord(net->evFork, c->stream), "record evFork"))) return r;
+            if ((r = check_hip(c, hipStreamWaitEvent(net->side, net->evFork, 0), "wait evFork"))) return r;
+            if ((r = check_hip(c, hipStreamWaitEvent(net->sortS, net->evFork, 0), "wait evFork"))) return r;
+            net->needFork = 0;
+        }
+        if ((r = enqueue_bp(net, net->bpPendingBuf, ovl ? net->side : c->stream, ovl, true))) return r;
+        net->bpPending = 0;
     }
     // the caller's stream covers the whole call: join the side streams
     if ((r = join_side(net))) return r;
@@ -1856,8 +2007,7 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
     if (r) return r;
     if ((r = check_hip(c, hipStreamSynchronize(c->stream), "net_set_skin sync"))) return r;
     // a captured step no longer matches the launch sequence
-    if (net->graphExec) { hipGraphExecDestroy(net->graphExec); net->graphExec = nullptr; }
-    if (net->graph) { hipGraphDestroy(net->graph); net->graph = nullptr; }
+    destroy_graphs(net);
     net->skinMode = mode;
     net->derivedValid = 0;  // the next step prepares the packed pose the mode reads
     if (mode == SBX_SKIN_REBUILD) return SBX_OK;
@@ -2019,8 +2169,7 @@ int sbx_net_use_graph(sbx_net *net, int on) {
     if (on && len != net->graphLen) {  // a captured graph of another length no longer applies
         int r = check_hip(net->ctx, hipStreamSynchronize(net->ctx->stream), "net_use_graph sync");
         if (r) return r;
-        if (net->graphExec) { hipGraphExecDestroy(net->graphExec); net->graphExec = nullptr; }
-        if (net->graph) { hipGraphDestroy(net->graph); net->graph = nullptr; }
+        destroy_graphs(net);
         net->graphLen = len;
     }
     net->useGraph = on ? 1 : 0;
@@ -2036,8 +2185,7 @@ int sbx_net_set_overlap(sbx_net *net, int on) {
     if ((r = check_hip(c, hipStreamSynchronize(c->stream), "net_set_overlap sync"))) return r;
     if (net->side && (r = check_hip(c, hipStreamSynchronize(net->side), "net_set_overlap sync"))) return r;
     // a captured step no longer matches the launch sequence
-    if (net->graphExec) { hipGraphExecDestroy(net->graphExec); net->graphExec = nullptr; }
-    if (net->graph) { hipGraphDestroy(net->graph); net->graph = nullptr; }
+    destroy_graphs(net);
     net->overlap = on ? 1 : 0;
     return SBX_OK;
 }
@@ -2053,8 +2201,7 @@ int drop_graph(sbx_net *net) {
     int r = check_hip(c, hipStreamSynchronize(c->stream), "net shape sync");
     if (!r && net->side) r = check_hip(c, hipStreamSynchronize(net->side), "net shape sync");
     if (!r && net->sortS) r = check_hip(c, hipStreamSynchronize(net->sortS), "net shape sync");
-    if (net->graphExec) { hipGraphExecDestroy(net->graphExec); net->graphExec = nullptr; }
-    if (net->graph) { hipGraphDestroy(net->graph); net->graph = nullptr; }
+    destroy_graphs(net);
     return r;
 }
 
