@@ -1952,7 +1952,16 @@ int sbx_net_publish(sbx_net *net) {
                               "publish uVec");
         if (!r) r = check_hip(c, hipMemcpyAsync(net->yB[0], net->yB[1], bytes, hipMemcpyDeviceToDevice, c->stream),
                               "publish yVec");
-        if (!r) net->cur = 0;
+        if (!r) {
+            net->cur = 0;
+            // each pose buffer keeps its cell array (a captured step sorts the cells of pose buffer b
+            // from the array it was captured with): the pose prep bins the moved pose into bodyCell
+            if (net->p.gridOn && !net->hasMotors) {
+                int *t = net->bodyCell;
+                net->bodyCell = net->cellNext;
+                net->cellNext = t;
+            }
+        }
     }
     // the caller may now edit its pose and link arrays: re-bin, re-pack, recount the active links and
     // (skin mode) rebuild next step

@@ -158,6 +158,40 @@ def test_publish_between_steps_matches_oracle(ctx, graph):
     net.close()
 
 
+@pytest.mark.parametrize("overlap", [False, True])
+def test_publish_between_captured_steps_keeps_slices(ctx, overlap):
+    """Captured steps (the 1-step graph pair with the side streams, 2-step graphs on one stream)
+    interleaved with pulls, which move the pose back to the caller's buffer, with odd and single-step
+    calls among them: the candidate slices and cells after every call equal the eager run's, so each
+    captured step sorts the cells of the pose it steps."""
+    from softbox_amd.network import Network
+
+    sc = small_network(seed=10, n_fil=200, links=500)
+    a = Network(ctx, sc.copy())
+    b = Network(ctx, sc.copy())
+    for net in (a, b):
+        net.set_overlap(overlap)
+    b.use_graph(2)
+    n, mc = sc.fs.n, int(sc.p["maxCand"])
+    t = 0
+    for k in (1, 3, 1, 2, 1, 5, 1, 1, 4):
+        a.step(t, k)
+        b.step(t, k)
+        t += k
+        fa, xa = a.pull(sc.fs.copy(), sc.xl.copy())
+        fb, xb = b.pull(sc.fs.copy(), sc.xl.copy())
+        ca, na, _, _, la = a.candidates(n, mc)
+        cb, nb, _, _, lb = b.candidates(n, mc)
+        assert_same(nb, na, f"candCount after {t} steps")
+        keep = (np.arange(mc)[None, :] < np.minimum(na, mc)[:, None]).ravel()
+        assert_same(cb[keep], ca[keep], f"candPartner after {t} steps")
+        assert_same(lb, la, f"bodyCell after {t} steps")
+        assert_same(fb.coord, fa.coord, f"coord after {t} steps")
+        assert_same(xb.linkState, xa.linkState, f"linkState after {t} steps")
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("glen", [2, pytest.param(8, marks=pytest.mark.skip(
     reason="8-step graph capture ended the host process (segfault inside sbx_net_step), cause not found; "
            "the library accepts only 2-step graphs"))])
