@@ -388,8 +388,10 @@ int sbx_net_candidates(sbx_net *net, int **candPartner, int **candCount, int **g
 int sbx_net_set_profiling(sbx_net *net, int on);
 /* accumulated [kernel count] names / total ms / launches; returns number of kernels written (<= cap) */
 int sbx_net_kernel_times(sbx_net *net, const char **names, double *total_ms, long long *launches, int cap);
-/* on = 1 (or 2): capture 2 steps as one hipGraph and replay it; 0: eager launches.  Step counts
- * that are not a multiple of 2 run their remainder eagerly. */
+/* on = 1 (or 2): replay captured steps; 0: eager launches.  The per-step rebuild on the side streams
+ * replays one captured step per pose buffer (each step's broad phase runs at the start of the next
+ * replay, the call's last one after its last replay); the other forms capture 2 steps as one
+ * hipGraph and run a remainder that is not a multiple of 2 eagerly.  Results equal eager launches. */
 int sbx_net_use_graph(sbx_net *net, int on);
 /* 1 (default): the filament-only step runs its broad phase on a second stream of the library,
  * concurrently with the crosslink and rod kernels (the candidate slices feed nothing later in the

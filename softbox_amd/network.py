@@ -74,8 +74,9 @@ class Network:
         self.ctx.check(self.ctx.lib.sbx_net_topology_changed(self.h))
 
     def use_graph(self, on: bool | int = True) -> None:
-        """True (or 2): replay 2 captured steps at a time; False: eager."""
-        self.ctx.check(self.ctx.lib.sbx_net_use_graph(self.h, int(on) if not isinstance(on, bool) else int(on)))
+        """True (or 2): replay captured steps (one per pose buffer for the per-step rebuild on the side
+        streams, two at a time otherwise); False: eager."""
+        self.ctx.check(self.ctx.lib.sbx_net_use_graph(self.h, int(on)))
 
     def set_overlap(self, on: bool = True) -> None:
         """Broad phase on a second stream, concurrent with the mechanics kernels (default on)."""
