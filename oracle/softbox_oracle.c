@@ -436,7 +436,11 @@ double sbo_angle_from_sin_cos(double sin2, double c) {
 
 /* One joint of ChainBendingForceSystem.chainForces (end2 side :152-235 with endSign=+1,
  * end1 side :237-321 with endSign=-1).  The two reference blocks differ only in the sign of
- * the endpoint offset, the self link-point offset, the lever arm and the torsion self axis. */
+ * the endpoint offset, the self link-point offset, the lever arm and the torsion self axis.
+ * The reference squares Math.sin(accurateAcos(cosB)) (:196-197, :202-203, :281-282, :287-288);
+ * that is sin^2(acos(cosB)) = (1 - cosB)(1 + cosB) for the clamped cosB, taken here in that form
+ * (a few double ulps from the reference's evaluation; the mobility it enters is dominated by the
+ * cosB^2 term). */
 static void chain_joint(int n, int i, int nbr, int side, double endSign,
                         const float *coord, const float *uVec, const float *segLength,
                         const float *bTransGam, const float *bRotGam,
@@ -485,15 +489,13 @@ static void chain_joint(int n, int i, int nbr, int side, double endSign,
     double cosB1 = ux * luX + uy * luY + uz * luZ;
     if (cosB1 > 1.0) cosB1 = 1.0;
     if (cosB1 < -1.0) cosB1 = -1.0;
-    double cosA1 = sin(sbo_accurate_acos(cosB1));
-    double cosA1_2 = cosA1 * cosA1;
+    double cosA1_2 = (1.0 - cosB1) * (1.0 + cosB1);  // sin^2(acos(cosB1)), an identity on [-1, 1]
     double moveC1 = cosB1 * cosB1 / rBTGx + cosA1_2 / rBTGy + lSqSelf * cosA1_2 / (4.0 * rBRGy);
 
     double cosB2 = nux * luX + nuy * luY + nuz * luZ;
     if (cosB2 > 1.0) cosB2 = 1.0;
     if (cosB2 < -1.0) cosB2 = -1.0;
-    double cosA2 = sin(sbo_accurate_acos(cosB2));
-    double cosA2_2 = cosA2 * cosA2;
+    double cosA2_2 = (1.0 - cosB2) * (1.0 + cosB2);  // sin^2(acos(cosB2)), an identity on [-1, 1]
     double moveC2 = cosB2 * cosB2 / nBTGx + cosA2_2 / nBTGy + lSqN * cosA2_2 / (4.0 * nBRGy);
 
     double denom = dt * (moveC1 + moveC2);

@@ -304,6 +304,9 @@ struct ChainParams {
 // the link and tip points, the strain, moveC1 + moveC2 (the two terms swap), forceMag, |tv|^2, the
 // dot product and the torsion magnitude are the same floating-point operations on the same
 // values, fSign flips, and tv = mu x nuE flips with every product of it unchanged.
+// The reference squares Math.sin(accurateAcos(cosB)) (ChainBendingForceSystem.java:196-197,
+// :202-203); sin^2(acos(cosB)) = (1 - cosB)(1 + cosB) on the clamped range, evaluated in that form:
+// two roundings instead of six transcendental evaluations, a few double ulps from the reference's.
 struct JointTerms {
     double F[3], Tq[3];
     int torsion;
@@ -353,14 +356,12 @@ __device__ __forceinline__ JointTerms chain_joint_terms(int i, int nbr, int side
     double cosB1 = ux * luX + uy * luY + uz * luZ;
     if (cosB1 > 1.0) cosB1 = 1.0;
     if (cosB1 < -1.0) cosB1 = -1.0;
-    double cosA1 = sin(accurate_acos(cosB1));
-    double cosA1_2 = cosA1 * cosA1;
+    double cosA1_2 = (1.0 - cosB1) * (1.0 + cosB1);  // sin^2(acos(cosB1)), an identity on [-1, 1]
     double moveC1 = cosB1 * cosB1 / rBTGx + cosA1_2 / rBTGy + lSqSelf * cosA1_2 / (4.0 * rBRGy);
     double cosB2 = nux * luX + nuy * luY + nuz * luZ;
     if (cosB2 > 1.0) cosB2 = 1.0;
     if (cosB2 < -1.0) cosB2 = -1.0;
-    double cosA2 = sin(accurate_acos(cosB2));
-    double cosA2_2 = cosA2 * cosA2;
+    double cosA2_2 = (1.0 - cosB2) * (1.0 + cosB2);  // sin^2(acos(cosB2)), an identity on [-1, 1]
     double moveC2 = cosB2 * cosB2 / nBTGx + cosA2_2 / nBTGy + lSqN * cosA2_2 / (4.0 * nBRGy);
     double denom = cp.dt * (moveC1 + moveC2);
     double forceMag = (denom > 0.0) ? (cp.fracMove * 1.0e-6 * strainDist / denom) : 0.0;
