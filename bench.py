@@ -69,20 +69,33 @@ def kernel_bytes(sc, name: str, list_entries: int | None = None) -> float | None
     return None
 
 
-def kernel_traffic(name: str):
-    """HBM bytes per launch of `name` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of
-    this workload (profiles/network_step_200k_rods_counters.md), or None when it has no row."""
-    path = os.path.join(ROOT, "profiles", "network_step_200k_rods_counters.md")
+def profiled_column(fname: str, name: str, col: int):
+    """Column `col` of kernel `name`'s row in a committed rocprofv3 summary table under profiles/, as a
+    float (a trailing ' %' dropped), or None when the file or the row is missing."""
+    path = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(path):
         return None
     for line in open(path):
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cols) >= 6 and cols[0] == name:
+        if len(cols) > col and cols[0] == name:
             try:
-                return float(cols[5])
+                return float(cols[col].rstrip("%").strip())
             except ValueError:
                 return None
     return None
+
+
+def kernel_traffic(name: str):
+    """HBM bytes per launch of `name` from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the 200k-rod
+    workload (profiles/network_step_200k_rods_counters.md)."""
+    return profiled_column("network_step_200k_rods_counters.md", name, 5)
+
+
+def kernel_valu_issue(name: str):
+    """Lower bound on the fraction of `name`'s solo duration spent issuing VALU instructions, from the
+    rocprofv3 SQ summary of the 200k-rod workload (profiles/network_step_200k_rods_instructions.md)."""
+    v = profiled_column("network_step_200k_rods_instructions.md", name, 7)
+    return None if v is None else v / 100.0
 
 
 def run_cpu_baseline(sc, budget_s: float = 15.0) -> dict:
@@ -280,9 +293,12 @@ def main() -> int:
         dom = max((k for k in per_kernel if kernel_bytes(sc, k, entries)), key=lambda k: per_kernel[k]["avg_us"])
         b = kernel_bytes(sc, dom, entries)
         achieved = b / (per_kernel[dom]["avg_us"] * 1e-6) / 1e9
+        # the committed counter summaries are of the default workload (200k rods, 50k links, 20 um, water)
+        profiled = (args.n_fil, args.n_links, args.box, args.eta) == (20000, 50000, 20.0, 1.0e-3)
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": kernel_traffic(dom),
-                "bytes_per_launch": b}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": kernel_traffic(dom) if profiled else None, "bytes_per_launch": b,
+                "valu_issue_frac": kernel_valu_issue(dom) if profiled else None}
         for rr in runs.values():
             for k, v in rr.get("kernels", {}).items():
                 kb = kernel_bytes(sc, k, rr.get("skin", {}).get("list_entries"))
