@@ -105,10 +105,19 @@ __device__ __forceinline__ int stencil_at(const int *pre, const int *shift, int 
     return k + sh;
 }
 
+// stencil_at for a walk whose second segments are all empty (no periodic row wraps: the cell is not
+// in the first or last x column): an empty range owns no position, so the 9 first segments decide.
+__device__ __forceinline__ int stencil_at9(const int *pre, const int *shift, int k) {
+    int sh = shift[0];
+#pragma unroll
+    for (int r = 1; r < 9; r++) sh = (k >= pre[2 * r]) ? shift[2 * r] : sh;
+    return k + sh;
+}
+
 // The 18 ranges of cell c into pre/shift (all 36 bounds are loaded before any is used, so they are
-// in flight together); returns the walk's length.
+// in flight together); returns the walk's length.  *wraps: some row has a second segment.
 __device__ __forceinline__ int stencil_map(int c, const GridGeom &g, const int *__restrict__ cellOffsets, int *pre,
-                                           int *shift) {
+                                           int *shift, bool *wraps = nullptr) {
     const int nX = g.nX, nY = g.nY, nZ = g.nZ, nXY = nX * nY;
     const int cz = c / nXY;
     const int rem = c - cz * nXY;
@@ -126,6 +135,7 @@ __device__ __forceinline__ int stencil_map(int c, const GridGeom &g, const int *
         xa0 = cx - 1; xa1 = cx + 1;
     }
     const bool twoSeg = xb1 >= xb0;
+    if (wraps) *wraps = twoSeg;
     int lo[18], hi[18];
 #pragma unroll
     for (int r = 0; r < 9; r++) {

@@ -105,7 +105,11 @@ __global__ void __launch_bounds__(BP_THREADS) k_broad_phase(Pose pose, Geo geo, 
     const BodyGeo bg = geo.of(i);  // the body's grid; its cells start at bg.cellBase
     const GridGeom &g = bg.g;
     int pre[18], shift[18];
-    const int total = stencil_map(c - bg.cellBase, g, cellOffsets + bg.cellBase, pre, shift);
+    bool wraps;
+    const int total = stencil_map(c - bg.cellBase, g, cellOffsets + bg.cellBase, pre, shift, &wraps);
+    // wave-uniform: without a wrapped row among the wave's bodies, the 9-segment map gives the same
+    // positions for fewer compares
+    const bool anyWrap = __ballot(wraps) != 0ull;
     int *slice = candPartner + (size_t)i * (size_t)maxCand;
     const unsigned long long groupBits = G == 64 ? ~0ull : ((1ull << G) - 1) << (grp * G);
     const unsigned long long below = (1ull << lane) - 1;
@@ -115,7 +119,7 @@ __global__ void __launch_bounds__(BP_THREADS) k_broad_phase(Pose pose, Geo geo, 
 #pragma unroll
         for (int k = 0; k < U; k++) {
             const int q = base + k * G + gl;
-            at[k] = q < total ? stencil_at(pre, shift, q) : t;
+            at[k] = q < total ? (anyWrap ? stencil_at(pre, shift, q) : stencil_at9(pre, shift, q)) : t;
         }
 #pragma unroll
         for (int k = 0; k < U; k++) jj[k] = (base + k * G + gl < total) ? cellContents[at[k]] : -1;
