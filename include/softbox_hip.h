@@ -388,6 +388,11 @@ int sbx_net_candidates(sbx_net *net, int **candPartner, int **candCount, int **g
 int sbx_net_set_profiling(sbx_net *net, int on);
 /* accumulated [kernel count] names / total ms / launches; returns number of kernels written (<= cap) */
 int sbx_net_kernel_times(sbx_net *net, const char **names, double *total_ms, long long *launches, int cap);
+/* No counterpart in the reference (a measurement helper): re-runs the latest step's per-step-rebuild
+ * broad phase reps times back to back on its grid, between two HIP events on the context's stream,
+ * and returns the mean microseconds per launch.  The candidate slices are rewritten with the same
+ * values.  Needs a step taken first; waits for all of the handle's streams. */
+int sbx_net_time_broad_phase(sbx_net *net, int reps, float *usPerLaunch);
 /* on = 1 (or 2): replay captured steps; 0: eager launches.  The per-step rebuild on the side streams
  * replays one captured step per pose buffer (each step's broad phase runs at the start of the next
  * replay, the call's last one after its last replay); the other forms capture 2 steps as one

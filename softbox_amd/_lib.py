@@ -173,6 +173,7 @@ _SIGS = {
     "sbx_net_candidates": [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)],
     "sbx_net_set_profiling": [vp, i32],
     "sbx_net_kernel_times": [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_longlong), i32],
+    "sbx_net_time_broad_phase": [vp, i32, C.POINTER(C.c_float)],
     "sbx_net_use_graph": [vp, i32],
     "sbx_net_set_overlap": [vp, i32],
     "sbx_net_set_shape": [vp, i32, i32, i32, i32],

@@ -1048,8 +1048,10 @@ hipEvent_t take_event(sbx_net *net) {
         net->eventPool.pop_back();
         return e;
     }
+    // timing only: no system-scope fence at the record (no cache writeback and invalidation
+    // between the kernels timed); nullptr when the event cannot be created (that launch is untimed)
     hipEvent_t e = nullptr;
-    hipEventCreate(&e);
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) e = nullptr;
     return e;
 }
 
@@ -1061,7 +1063,15 @@ struct Scope {
         if (net->profiling && !net->graphSteps) {
             a = take_event(net);
             b = take_event(net);
-            hipEventRecord(a, net->ctx->stream);
+            if (!a || !b) {
+                if (a) net->eventPool.push_back(a);
+                if (b) net->eventPool.push_back(b);
+                a = b = nullptr;
+            } else if (hipEventRecord(a, net->ctx->stream) != hipSuccess) {
+                net->eventPool.push_back(a);
+                net->eventPool.push_back(b);
+                a = b = nullptr;
+            }
         }
     }
     ~Scope() {
@@ -1912,6 +1922,8 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
             net->evCaptured = 1;
             net->lastCsr = net->graphLastCsr;
             net->lastCell = net->graphLastCell;
+            if (net->p.gridOn && !net->hasMotors && net->skinMode == SBX_SKIN_REBUILD)
+                net->csrValid[0] = net->csrValid[1] = 1;  // its two steps sorted both CSR buffers
             done += G;
             advanced += G;  // G is even: net->cur is unchanged by a replay
         }
@@ -2150,6 +2162,44 @@ int sbx_net_set_profiling(sbx_net *net, int on) {
     net->profiling = on ? 1 : 0;
     if (on)
         for (auto &t : net->timers) { t.ms = 0.0; t.launches = 0; }
+    return r;
+}
+
+int sbx_net_time_broad_phase(sbx_net *net, int reps, float *usPerLaunch) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, usPerLaunch && reps > 0 && reps <= 10000, "net_time_broad_phase: need 0 < reps <= 10000 and an out");
+    SBX_REQUIRE(c, net->p.gridOn && !net->hasMotors && net->skinMode == SBX_SKIN_REBUILD,
+                "net_time_broad_phase: the per-step-rebuild broad phase only");
+    const int b = net->lastCsr;
+    SBX_REQUIRE(c, net->csrValid[b], "net_time_broad_phase: no grid built yet (step first)");
+    int r = set_device(c);
+    if (r) return r;
+    if (net->bpPending) return fail(c, SBX_ERR_STATE, "net_time_broad_phase: a step is still in flight");
+    // every stream idle: the launches below re-run the last step's broad phase on its grid, so the
+    // candidate slices are rewritten with the same values
+    if ((r = check_hip(c, hipStreamSynchronize(c->stream), "time bp sync"))) return r;
+    if (net->side && (r = check_hip(c, hipStreamSynchronize(net->side), "time bp sync"))) return r;
+    if (net->sortS && (r = check_hip(c, hipStreamSynchronize(net->sortS), "time bp sync"))) return r;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if ((r = check_hip(c, hipEventCreate(&e0), "time bp event"))) return r;
+    if ((r = check_hip(c, hipEventCreate(&e1), "time bp event"))) {
+        hipEventDestroy(e0);
+        return r;
+    }
+    r = check_hip(c, hipEventRecord(e0, c->stream), "time bp record");
+    for (int k = 0; k < reps && !r; k++)
+        r = launch_broad_phase_sorted(c, net->sortedPose[b], net->sortedCell[b], net->csrOff[b], net->csrCont[b],
+                                      net->geom, net->reps, net->rodRep, net->bpGroup, net->bpStaged, net->n, net->n,
+                                      net->candPartner, net->candCount, net->p.maxCand, c->stream);
+    if (!r) r = check_hip(c, hipEventRecord(e1, c->stream), "time bp record");
+    if (!r) r = check_hip(c, hipEventSynchronize(e1), "time bp sync");
+    float ms = 0.0f;
+    if (!r) r = check_hip(c, hipEventElapsedTime(&ms, e0, e1), "time bp elapsed");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (!r) *usPerLaunch = 1.0e3f * ms / (float)reps;
     return r;
 }
 

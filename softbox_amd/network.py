@@ -107,6 +107,13 @@ class Network:
         k = self.ctx.check(self.ctx.lib.sbx_net_kernel_times(self.h, names, ms, cnt, cap))
         return {names[i].decode(): (ms[i], cnt[i]) for i in range(k)}
 
+    def time_broad_phase(self, reps: int = 20) -> float:
+        """Mean microseconds per launch of the latest step's broad phase, re-run `reps` times back to
+        back on its grid (the slices are rewritten with the same values)."""
+        us = C.c_float()
+        self.ctx.check(self.ctx.lib.sbx_net_time_broad_phase(self.h, reps, C.byref(us)))
+        return us.value
+
     def candidates(self, n: int, max_cand: int):
         """Host copies of (candPartner[n*maxCand], candCount[n], offsets, contents, bodyCell)."""
         import torch

@@ -304,3 +304,25 @@ def test_overlap_equals_single_stream(ctx, graph, skin):
     assert_same(xb.linkState, xa.linkState, "overlap vs one stream linkState")
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_time_broad_phase_rewrites_the_same_slices(ctx, graph):
+    """sbx_net_time_broad_phase re-runs the latest step's broad phase: a positive time per launch, and
+    the candidate slices and counts unchanged."""
+    from softbox_amd.network import Network
+
+    sc = small_network(seed=12, n_fil=200, links=500)
+    net = Network(ctx, sc)
+    if graph:
+        net.use_graph(2)
+    net.step(0, 5)
+    n, mc = sc.fs.n, int(sc.p["maxCand"])
+    ca, na, _, _, _ = net.candidates(n, mc)
+    us = net.time_broad_phase(5)
+    cb, nb, _, _, _ = net.candidates(n, mc)
+    assert us > 0.0
+    assert_same(nb, na, "candCount after timing")
+    keep = (np.arange(mc)[None, :] < np.minimum(na, mc)[:, None]).ravel()
+    assert_same(cb[keep], ca[keep], "candPartner after timing")
+    net.close()
