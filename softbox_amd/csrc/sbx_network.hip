@@ -216,6 +216,14 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         gR[k] = a.bRotGam[k * n + ic];
     }
     const float segLen = a.segLength[ic], tS = a.bts[ic], rS = a.brs[ic];
+    // the crosslink CSR bounds and the step number are loaded with the pose, so their latency runs
+    // under the chain's arithmetic rather than after it
+    int ra0 = 0, ra1 = 0, rb0 = 0, rb1 = 0;
+    if (a.hasLinks) {
+        ra0 = a.offA[ic]; ra1 = a.offA[ic + 1];
+        rb0 = a.offB[ic]; rb1 = a.offB[ic + 1];
+    }
+    const int step = *a.dStep + a.stepOff;
     float f[3] = {0.0f, 0.0f, 0.0f}, t[3] = {0.0f, 0.0f, 0.0f};
     if (a.chainOn) {
         double fd[3] = {0.0, 0.0, 0.0}, td[3] = {0.0, 0.0, 0.0};
@@ -228,7 +236,6 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         }
     }
     if (!own) return;
-    const int step = *a.dStep + a.stepOff;
     uint32_t body = (uint32_t)i;
     int seed = a.seed, cellBase = 0;
     GridGeom geom = a.geom;
@@ -242,14 +249,14 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
     if (a.hasLinks) {
         double fa[3] = {0, 0, 0}, ta[3] = {0, 0, 0};
         int active = 0;
-        gather_side(i, a.offA, a.idxA, a.xlinkData, a.linkState, 0, fa, ta, active);
+        gather_range(ra0, ra1, a.idxA, a.xlinkData, a.linkState, 0, fa, ta, active);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             f[k] = (float)(f[k] + fa[k]);
             t[k] = (float)(t[k] + ta[k]);
         }
         double fb[3] = {0, 0, 0}, tb[3] = {0, 0, 0};
-        gather_side(i, a.offB, a.idxB, a.xlinkData, a.linkState, 6, fb, tb, active);
+        gather_range(rb0, rb1, a.idxB, a.xlinkData, a.linkState, 6, fb, tb, active);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             f[k] = (float)(f[k] + fb[k]);
