@@ -189,7 +189,7 @@ def main() -> int:
                          "of known size that calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: the step runs and is measured on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU design (spatial slabs + halo exchange) is documentation only", file=sys.stderr)
         return 2
 
