@@ -326,3 +326,23 @@ def test_time_broad_phase_rewrites_the_same_slices(ctx, graph):
     keep = (np.arange(mc)[None, :] < np.minimum(na, mc)[:, None]).ravel()
     assert_same(cb[keep], ca[keep], "candPartner after timing")
     net.close()
+
+
+@pytest.mark.parametrize("shape", [(2, 8, 256, 0), (4, 8, 128, 1)])
+def test_bench_workload_matches_oracle(ctx, shape):
+    """The bench's own scene at full size (200k rods, 50k links, 136^3 grid, water), stepped by the
+    production path (captured steps, side streams, a launch shape the tuner picks): pose, forces,
+    crosslink state, grid CSR and every candidate slice are the oracle's, bit for bit."""
+    from softbox_amd.network import Network
+
+    sc = scenes.network(n_fil=20000, segs=10, box=20.0, n_links=50000, seed=0, eta=1.0e-3)
+    orc = ffi.OracleNet(sc.copy())
+    net = Network(ctx, sc)
+    net.use_graph(2)
+    net.set_shape(*shape)
+    steps = 4
+    for t in range(steps):
+        orc.step(t, 1)
+    net.step(0, steps)
+    _compare(net, orc, sc, f"bench workload, shape {shape}, after {steps} steps")
+    net.close()
