@@ -110,3 +110,23 @@ def test_contractile_walls(ctx):
     net.step(0, 10)
     _compare(net, orc, sc, "walls")
     net.close()
+
+
+def test_contractile_workload_full_size_matches_oracle(ctx):
+    """The actomyosin workload at full size (50k filaments x 10 rods, 2k minifilaments of 16 heads, 100k
+    crosslinkers, 30 um box with walls), stepped through captured 2-step graphs: filaments, motors,
+    backbones, crosslinks and the reach lists equal the oracle's bit for bit after 4 steps (motors
+    bind within them)."""
+    from softbox_amd.network import Network
+
+    sc = scenes.contractile(seed=0)
+    orc = ffi.OracleContractile(sc.copy())
+    net = Network(ctx, sc)
+    net.use_graph(2)
+    steps = 4
+    for t in range(steps):
+        orc.step(t, 1)
+    net.step(0, steps)
+    assert int((orc.mot.boundSeg >= 0).sum()) > 0
+    _compare(net, orc, sc, f"full-size contractile workload after {steps} steps")
+    net.close()
