@@ -130,3 +130,21 @@ def test_contractile_workload_full_size_matches_oracle(ctx):
     assert int((orc.mot.boundSeg >= 0).sum()) > 0
     _compare(net, orc, sc, f"full-size contractile workload after {steps} steps")
     net.close()
+
+
+def test_contractile_workload_2m_rods_matches_oracle(ctx):
+    """The same recipe scaled to 2M rods (200k filaments, 8k minifilaments, 400k crosslinkers, 60 um
+    box, 47M grid cells): 3 steps (a 2-step graph replay and an eager step) equal the oracle's."""
+    from softbox_amd.network import Network
+
+    sc = scenes.contractile(n_fil=200000, box=60.0, n_mini=8000, n_links=400000, seed=0)
+    orc = ffi.OracleContractile(sc.copy())
+    net = Network(ctx, sc)
+    net.use_graph(2)
+    steps = 3
+    for t in range(steps):
+        orc.step(t, 1)
+    net.step(0, steps)
+    assert int((orc.mot.boundSeg >= 0).sum()) > 0
+    _compare(net, orc, sc, f"2M-rod contractile workload after {steps} steps")
+    net.close()
