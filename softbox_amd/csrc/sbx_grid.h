@@ -136,18 +136,28 @@ __device__ __forceinline__ int stencil_map(int c, const GridGeom &g, const int *
     }
     const bool twoSeg = xb1 >= xb0;
     if (wraps) *wraps = twoSeg;
-    int lo[18], hi[18];
+    // the three z planes and three y rows of the stencil (wrapped or range-checked once per axis)
+    int zBase[3], yBase[3];
+    bool zOk[3], yOk[3];
 #pragma unroll
-    for (int r = 0; r < 9; r++) {
-        int zz = cz + r / 3 - 1, yy = cy + r % 3 - 1;
-        bool ok = true;
+    for (int d = 0; d < 3; d++) {
+        int zz = cz + d - 1, yy = cy + d - 1;
         if (g.periodic) {
             zz = zz < 0 ? zz + nZ : (zz >= nZ ? zz - nZ : zz);
             yy = yy < 0 ? yy + nY : (yy >= nY ? yy - nY : yy);
+            zOk[d] = yOk[d] = true;
         } else {
-            ok = zz >= 0 && zz < nZ && yy >= 0 && yy < nY;
+            zOk[d] = zz >= 0 && zz < nZ;
+            yOk[d] = yy >= 0 && yy < nY;
         }
-        const int base = yy * nX + zz * nXY;
+        zBase[d] = zz * nXY;
+        yBase[d] = yy * nX;
+    }
+    int lo[18], hi[18];
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        const bool ok = zOk[r / 3] && yOk[r % 3];
+        const int base = yBase[r % 3] + zBase[r / 3];
         lo[2 * r] = ok ? cellOffsets[base + xa0] : 0;
         hi[2 * r] = ok ? cellOffsets[base + xa1 + 1] : 0;
         lo[2 * r + 1] = ok && twoSeg ? cellOffsets[base + xb0] : 0;
