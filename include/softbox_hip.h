@@ -66,7 +66,7 @@ const char *sbx_version(void);
 /* ChainBendingForceSystem.zeroAccumulators (ChainBendingForceSystem.java:38-45) */
 int sbx_zero_accumulators(sbx_ctx *c, float *forceSum, float *torqueSum, int n);
 
-/* BrownianForceSystem.brownianForce (BrownianForceSystem.java:183-245); params[1] = sqrt(2kT/dt),
+/* BrownianForceSystem.brownianForce (BrownianForceSystem.java:44-106); params[1] = sqrt(2kT/dt),
  * counts[1] = step, counts[2] = seed.  rng/stream select the random stream (stream: replica key). */
 int sbx_brownian_force(sbx_ctx *c, float *randForce, float *randTorque, const float *bTransGam, const float *bRotGam,
                        const float *brownTransScale, const float *brownRotScale, const float *params,
@@ -77,7 +77,7 @@ int sbx_integrate(sbx_ctx *c, float *coord, float *uVec, float *yVec, const floa
                   const float *randForce, const float *randTorque, const float *bTransGam, const float *bRotGam,
                   const float *params, const int *counts, int n);
 
-/* DerivedGeometrySystem.derive (DerivedGeometrySystem.java:140-186) */
+/* DerivedGeometrySystem.derive (DerivedGeometrySystem.java:27-73) */
 int sbx_derive(sbx_ctx *c, const float *coord, const float *uVec, float *yVec, float *zVec, float *end1, float *end2,
                const float *segLength, const int *counts, int n);
 

@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_softbox.so")
+LIB_OMP_PATH = os.path.join(HERE, "liboracle_softbox_omp.so")   # same restatement, OpenMP per-item loops
 
 vp = C.c_void_p
 i32 = C.c_int
@@ -20,8 +21,8 @@ u32 = C.c_uint32
 
 
 def build() -> str:
-    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
-    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle_softbox.so"], check=True)
+    """Compile the oracle (and its OpenMP build) with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "all"], check=True)
     return LIB_PATH
 
 
@@ -47,6 +48,7 @@ class SboNet(C.Structure):
 
 
 _lib = None
+_lib_omp = None
 
 
 def load() -> C.CDLL:
@@ -54,26 +56,39 @@ def load() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        lib = C.CDLL(LIB_PATH)
-        lib.sbo_const.restype = C.c_double
-        lib.sbo_wang_hash.restype = u32
-        lib.sbo_wang_hash.argtypes = [u32]
-        lib.sbo_event_bits.restype = u32
-        lib.sbo_event_bits.argtypes = [i32, u32, i32, i32, u32, u32]
-        lib.sbo_uniform.restype = C.c_float
-        lib.sbo_uniform.argtypes = [u32]
-        for name in ("sbo_accurate_acos", "sbo_accurate_asin"):
-            getattr(lib, name).restype = C.c_double
-            getattr(lib, name).argtypes = [C.c_double]
-        lib.sbo_angle_from_sin_cos.restype = C.c_double
-        lib.sbo_angle_from_sin_cos.argtypes = [C.c_double, C.c_double]
-        lib.sbo_rod_drag_si.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
-        lib.sbo_sphere_drag_si.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
-        lib.sbo_network_step.argtypes = [C.POINTER(SboNet), i32]
-        lib.sbo_brownian_bits.argtypes = [i32, u32, i32, i32, u32, C.POINTER(u32)]
-        lib.sbo_philox4x32_10.argtypes = [C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
-        _lib = lib
+        _lib = _configure(C.CDLL(LIB_PATH))
     return _lib
+
+
+def load_omp() -> C.CDLL:
+    """The OpenMP build (all host threads; OMP_NUM_THREADS sets how many)."""
+    global _lib_omp
+    if _lib_omp is None:
+        if not os.path.exists(LIB_OMP_PATH):
+            build()
+        _lib_omp = _configure(C.CDLL(LIB_OMP_PATH))
+    return _lib_omp
+
+
+def _configure(lib: C.CDLL) -> C.CDLL:
+    lib.sbo_const.restype = C.c_double
+    lib.sbo_wang_hash.restype = u32
+    lib.sbo_wang_hash.argtypes = [u32]
+    lib.sbo_event_bits.restype = u32
+    lib.sbo_event_bits.argtypes = [i32, u32, i32, i32, u32, u32]
+    lib.sbo_uniform.restype = C.c_float
+    lib.sbo_uniform.argtypes = [u32]
+    for name in ("sbo_accurate_acos", "sbo_accurate_asin"):
+        getattr(lib, name).restype = C.c_double
+        getattr(lib, name).argtypes = [C.c_double]
+    lib.sbo_angle_from_sin_cos.restype = C.c_double
+    lib.sbo_angle_from_sin_cos.argtypes = [C.c_double, C.c_double]
+    lib.sbo_rod_drag_si.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+    lib.sbo_sphere_drag_si.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
+    lib.sbo_network_step.argtypes = [C.POINTER(SboNet), i32]
+    lib.sbo_brownian_bits.argtypes = [i32, u32, i32, i32, u32, C.POINTER(u32)]
+    lib.sbo_philox4x32_10.argtypes = [C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+    return lib
 
 
 def P(a: np.ndarray | None) -> C.c_void_p:
@@ -106,7 +121,7 @@ def call(name: str, *args):
 class OracleNet:
     """sbo_network_step over host copies of a Scene (DenseContractileHarness.cpuStep filament block)."""
 
-    def __init__(self, scene):
+    def __init__(self, scene, lib: C.CDLL | None = None):
         fs, xl, p = scene.fs.copy(), (scene.xl.copy() if scene.xl is not None else None), scene.p
         self.fs, self.xl, self.p = fs, xl, p
         n = fs.n
@@ -145,7 +160,7 @@ class OracleNet:
             for k, a in self.grid.items():
                 setattr(s, k, P(a).value)
         self.s = s
-        self.lib = load()
+        self.lib = lib if lib is not None else load()
 
     def step(self, step0: int, n: int = 1) -> None:
         for t in range(step0, step0 + n):
@@ -177,8 +192,8 @@ class SboMotors(C.Structure):
 class OracleContractile(OracleNet):
     """sbo_contractile_step over host copies of a ContractileScene (DenseContractileHarness.cpuStep)."""
 
-    def __init__(self, scene):
-        super().__init__(scene)
+    def __init__(self, scene, lib: C.CDLL | None = None):
+        super().__init__(scene, lib)
         mot, dim, mini = scene.mot.copy(), scene.dim.copy(), scene.mini.copy()
         self.mot, self.dim, self.mini = mot, dim, mini
         n, nM = self.fs.n, mot.nM

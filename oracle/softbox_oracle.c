@@ -17,6 +17,18 @@
 
 /* ------------------------------------------------------------------ constants */
 /* Constants.java:25-51 */
+/* Built with -fopenmp (liboracle_softbox_omp.so, the all-cores CPU baseline) the per-item loops of
+ * the reference's @Parallel Systems run on all host threads; every such loop writes only its own
+ * item's slots (the reference's one-writer-per-slot rule), so the results do not change.  Without
+ * OpenMP (liboracle_softbox.so, the checker) they are plain loops. */
+#ifdef _OPENMP
+#define SBO_PAR_FOR _Pragma("omp parallel for schedule(static)")
+#define SBO_PAR_FOR_DYNAMIC _Pragma("omp parallel for schedule(dynamic, 256)")
+#else
+#define SBO_PAR_FOR
+#define SBO_PAR_FOR_DYNAMIC
+#endif
+
 static const double K_BOLTZ = 1.380662e-23;
 static const double K_TEMPK = 298.15;
 static const double K_AETA = 0.1;
@@ -108,7 +120,7 @@ void sbo_drag_fixed_length(int n, double lengthUm, double radiusUm, double aeta,
 }
 
 /* ------------------------------------------------------------------ random streams */
-/* BrownianForceSystem.wangHash, BrownianForceSystem.java:174-181 (Java int arithmetic = uint32 wrap) */
+/* BrownianForceSystem.wangHash, BrownianForceSystem.java:35-42 (Java int arithmetic = uint32 wrap) */
 uint32_t sbo_wang_hash(uint32_t seed) {
     seed = (seed ^ 61u) ^ (seed >> 16);
     seed *= 9u;
@@ -137,12 +149,12 @@ void sbo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* Reference keying (BrownianForceSystem.java:203): (i*1000003) ^ (step*999983) ^ (seed*7919), 32-bit wrap */
+/* Reference keying (BrownianForceSystem.java:64): (i*1000003) ^ (step*999983) ^ (seed*7919), 32-bit wrap */
 static uint32_t wang_base(uint32_t i, int32_t step, int32_t seed) {
     return (i * 1000003u) ^ ((uint32_t)step * 999983u) ^ ((uint32_t)seed * 7919u);
 }
 
-/* The six per-body-step draws.  WANG: salts of BrownianForceSystem.java:204-209.
+/* The six per-body-step draws.  WANG: salts of BrownianForceSystem.java:65-70.
  * PHILOX: counter (body, step, block, 0), key (seed, stream); block 0 gives h1..h4, block 1 gives h5,h6. */
 void sbo_brownian_bits(int rng, uint32_t body, int32_t step, int32_t seed, uint32_t stream, uint32_t h[6]) {
     if (rng == SBO_RNG_WANG) {
@@ -175,7 +187,7 @@ uint32_t sbo_event_bits(int rng, uint32_t item, int32_t step, int32_t seed, uint
     return o[0];
 }
 
-/* BrownianForceSystem.java:211: (h >>> 1) / 2147483647.0f (int -> float rounds to nearest) */
+/* BrownianForceSystem.java:72-77: (h >>> 1) / 2147483647.0f (int -> float rounds to nearest) */
 float sbo_uniform(uint32_t h) {
     return (float)(int32_t)(h >> 1) / 2147483647.0f;
 }
@@ -189,11 +201,12 @@ void sbo_zero_accumulators(int n, float *forceSum, float *torqueSum) {
     memset(torqueSum, 0, sizeof(float) * 3 * (size_t)n);
 }
 
-/* BrownianForceSystem.brownianForce, BrownianForceSystem.java:183-245.
+/* BrownianForceSystem.brownianForce, BrownianForceSystem.java:44-106.
  * bodyBase offsets the RNG body index (replicas share one counter space). */
 void sbo_brownian_force(int n, float *randForce, float *randTorque, const float *bTransGam, const float *bRotGam,
                         const float *brownTransScale, const float *brownRotScale, float brownianForceMag,
                         int32_t step, int32_t seed, int rng, uint32_t stream, uint32_t bodyBase) {
+    SBO_PAR_FOR
     for (int i = 0; i < n; i++) {
         int iy = n + i, iz = 2 * n + i;
         uint32_t h[6];
@@ -235,6 +248,7 @@ void sbo_brownian_force(int n, float *randForce, float *randTorque, const float 
 void sbo_integrate(int n, float *coord, float *uVec, float *yVec, const float *forceSum, const float *torqueSum,
                    const float *randForce, const float *randTorque, const float *bTransGam, const float *bRotGam,
                    float dt) {
+    SBO_PAR_FOR
     for (int i = 0; i < n; i++) {
         int iy = n + i, iz = 2 * n + i;
         float ux = uVec[i], uyc = uVec[iy], uzc = uVec[iz];
@@ -290,9 +304,10 @@ void sbo_integrate(int n, float *coord, float *uVec, float *yVec, const float *f
     }
 }
 
-/* DerivedGeometrySystem.derive, DerivedGeometrySystem.java:140-186 */
+/* DerivedGeometrySystem.derive, DerivedGeometrySystem.java:27-73 */
 void sbo_derive(int n, const float *coord, const float *uVec, float *yVec, float *zVec, float *end1, float *end2,
                 const float *segLength) {
+    SBO_PAR_FOR
     for (int i = 0; i < n; i++) {
         int iy = n + i, iz = 2 * n + i;
         float ux = uVec[i], uyc = uVec[iy], uzc = uVec[iz];
@@ -340,6 +355,7 @@ void sbo_confine(int n, const float *coord, const float *uVec, const float *segL
     double halfY = 0.5 * boxParams[2] - boxParams[4];
     double halfZ = 0.5 * boxParams[3] - boxParams[4];
     double coeff = boxParams[5];
+    SBO_PAR_FOR
     for (int i = 0; i < n; i++) {
         int iy = n + i, iz = 2 * n + i;
         double cx = coord[i], cy = coord[iy], cz = coord[iz];
@@ -434,6 +450,20 @@ double sbo_angle_from_sin_cos(double sin2, double c) {
     return sbo_accurate_acos(c);
 }
 
+/* The chain mobility's sin^2(acos(c)).  The reference squares Math.sin(accurateAcos(c))
+ * (ChainBendingForceSystem.java:196-197, :202-203, :281-282, :287-288); by default it is taken in the
+ * equal form (1 - c)(1 + c), which the device uses too.  sbo_set_chain_reference_mobility(1) switches
+ * the oracle to the reference's own evaluation, so a test can bound what the identity changes. */
+static int g_chain_reference_mobility = 0;
+void sbo_set_chain_reference_mobility(int on) { g_chain_reference_mobility = on; }
+static double sin2_of_acos(double c) {
+    if (g_chain_reference_mobility) {
+        double s = sin(sbo_accurate_acos(c));
+        return s * s;
+    }
+    return (1.0 - c) * (1.0 + c);
+}
+
 /* One joint of ChainBendingForceSystem.chainForces (end2 side :152-235 with endSign=+1,
  * end1 side :237-321 with endSign=-1).  The two reference blocks differ only in the sign of
  * the endpoint offset, the self link-point offset, the lever arm and the torsion self axis.
@@ -489,13 +519,13 @@ static void chain_joint(int n, int i, int nbr, int side, double endSign,
     double cosB1 = ux * luX + uy * luY + uz * luZ;
     if (cosB1 > 1.0) cosB1 = 1.0;
     if (cosB1 < -1.0) cosB1 = -1.0;
-    double cosA1_2 = (1.0 - cosB1) * (1.0 + cosB1);  // sin^2(acos(cosB1)), an identity on [-1, 1]
+    double cosA1_2 = sin2_of_acos(cosB1);
     double moveC1 = cosB1 * cosB1 / rBTGx + cosA1_2 / rBTGy + lSqSelf * cosA1_2 / (4.0 * rBRGy);
 
     double cosB2 = nux * luX + nuy * luY + nuz * luZ;
     if (cosB2 > 1.0) cosB2 = 1.0;
     if (cosB2 < -1.0) cosB2 = -1.0;
-    double cosA2_2 = (1.0 - cosB2) * (1.0 + cosB2);  // sin^2(acos(cosB2)), an identity on [-1, 1]
+    double cosA2_2 = sin2_of_acos(cosB2);
     double moveC2 = cosB2 * cosB2 / nBTGx + cosA2_2 / nBTGy + lSqN * cosA2_2 / (4.0 * nBRGy);
 
     double denom = dt * (moveC1 + moveC2);
@@ -539,6 +569,7 @@ void sbo_chain_forces(int n, const float *coord, const float *uVec, const float 
     double dt = chainParams[0], fracMove = chainParams[1], fracR = chainParams[2];
     double fracMoveTorq = chainParams[3], filTorqSpringActive = chainParams[4];
     double filTorqSpring = chainParams[5], actinMonoRadius = chainParams[6];
+    SBO_PAR_FOR
     for (int i = 0; i < n; i++) {
         int iy = n + i, iz = 2 * n + i;
         double cx = coord[i], cy = coord[iy], cz = coord[iz];
@@ -571,6 +602,7 @@ void sbo_chain_forces(int n, const float *coord, const float *uVec, const float 
 /* FilamentStore.publishToBodyView, FilamentStore.java:275-295 */
 void sbo_publish_filaments(int n, int cap, const float *coord, const float *segLength, float actinRadius,
                            float *center, float *boundingRadius) {
+    SBO_PAR_FOR
     for (int i = 0; i < n; i++) {
         center[i] = coord[i];
         center[cap + i] = coord[n + i];
@@ -605,6 +637,7 @@ void sbo_body_cell(int cap, int S, const float *center, const float *gridParams,
     float invCell = gridParams[4];
     int nX = gridDims[0], nY = gridDims[1], nZ = gridDims[2];
     int nXY = nX * nY;
+    SBO_PAR_FOR
     for (int i = 0; i < cap; i++) {
         if (i >= S) { bodyCell[i] = -1; continue; }
         float fx = (center[i] - xMin) * invCell;
@@ -658,6 +691,7 @@ void sbo_broad_phase(int cap, int S, const float *center, const float *boundingR
     int nX = gridDims[0], nY = gridDims[1], nZ = gridDims[2];
     int nXY = nX * nY;
     float cutoff = gridParams[5];
+    SBO_PAR_FOR_DYNAMIC
     for (int i = 0; i < cap; i++) {
         candCount[i] = 0;
         if (i >= S) continue;
@@ -791,6 +825,7 @@ void sbo_link_unbind(int nSeg, int nLinks, const float *filUVec, const float *fi
     double offConst = offParams[0], offCoeff = offParams[1], offExp = offParams[2];
     double dt = offParams[3], restLength = offParams[4];
     const int W = STRAIN_WIN;
+    SBO_PAR_FOR
     for (int k = 0; k < nLinks; k++) {
         if (linkState[k] < 0) continue;
         int a = linkFilA[k], b = linkFilB[k];
@@ -831,6 +866,7 @@ void sbo_link_forces(int nSeg, int nLinks, const float *filCoord, const float *f
     double restLength = xlParams[0];
     double fracMoveBase = xlParams[1];
     double dt = xlParams[2];
+    SBO_PAR_FOR
     for (int k = 0; k < nLinks; k++) {
         int d = k * XL_STRIDE;
         for (int j = 0; j < XL_STRIDE; j++) xlinkData[d + j] = 0.0f;
@@ -893,6 +929,7 @@ void sbo_link_torsion(int nSeg, int nLinks, const float *filUVec, const int *lin
     if (torsionParams[1] < 0.5f) return;
     double spring = torsionParams[0];
     const int W = TORQUE_WIN;
+    SBO_PAR_FOR
     for (int k = 0; k < nLinks; k++) {
         if (linkState[k] < 0) continue;
         int a = linkFilA[k], b = linkFilB[k];
@@ -931,6 +968,7 @@ void sbo_link_torsion(int nSeg, int nLinks, const float *filUVec, const int *lin
 
 static void gather_side(int nSeg, const int *segOffsets, const int *segLinkIdx, const float *xlinkData,
                         const int *linkState, float *filForceSum, float *filTorqueSum, int off) {
+    SBO_PAR_FOR
     for (int s = 0; s < nSeg; s++) {
         double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0;
         int start = segOffsets[s], end = segOffsets[s + 1];

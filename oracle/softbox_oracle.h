@@ -67,17 +67,18 @@ float sbo_uniform(uint32_t h);  /* (h>>>1)/2147483647f */
 void sbo_zero_accumulators(int n, float *forceSum, float *torqueSum);                   /* ChainBendingForceSystem.java:38-45 */
 void sbo_brownian_force(int n, float *randForce, float *randTorque, const float *bTransGam, const float *bRotGam,
                         const float *brownTransScale, const float *brownRotScale, float brownianForceMag,
-                        int32_t step, int32_t seed, int rng, uint32_t stream, uint32_t bodyBase); /* BrownianForceSystem.java:183-245 */
+                        int32_t step, int32_t seed, int rng, uint32_t stream, uint32_t bodyBase); /* BrownianForceSystem.java:44-106 */
 void sbo_integrate(int n, float *coord, float *uVec, float *yVec, const float *forceSum, const float *torqueSum,
                    const float *randForce, const float *randTorque, const float *bTransGam, const float *bRotGam,
                    float dt);                                                             /* RigidRodLangevinIntegrationSystem.java:33-112 */
 void sbo_derive(int n, const float *coord, const float *uVec, float *yVec, float *zVec, float *end1, float *end2,
-                const float *segLength);                                                  /* DerivedGeometrySystem.java:140-186 */
+                const float *segLength);                                                  /* DerivedGeometrySystem.java:27-73 */
 void sbo_confine(int n, const float *coord, const float *uVec, const float *segLength, const float *bTransGam,
                  float *forceSum, float *torqueSum, const float *boxParams, int32_t step);  /* ContainmentSystem.java:58-126 */
 
 /* ---- chain (ChainBendingForceSystem.java:47-331) ---- */
 double sbo_accurate_acos(double x);
+void sbo_set_chain_reference_mobility(int on);  /* 1: sin(accurateAcos(c))^2 as the reference (:196-203) */
 double sbo_accurate_asin(double s);
 double sbo_angle_from_sin_cos(double sin2, double cos);
 void sbo_chain_forces(int n, const float *coord, const float *uVec, const float *segLength,

@@ -15,7 +15,7 @@ namespace sbx {
 enum { RNG_WANG = 0, RNG_PHILOX = 1 };
 
 // Thomas Wang's 32-bit integer mixer as used by BrownianForceSystem.wangHash
-// (BrownianForceSystem.java:174-181); Java int wrap-around == uint32 arithmetic.
+// (BrownianForceSystem.java:35-42); Java int wrap-around == uint32 arithmetic.
 __device__ __forceinline__ uint32_t wang_hash(uint32_t s) {
     s = (s ^ 61u) ^ (s >> 16);
     s *= 9u;
@@ -41,12 +41,12 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-// Reference key (BrownianForceSystem.java:203): (i*1000003) ^ (step*999983) ^ (seed*7919)
+// Reference key (BrownianForceSystem.java:64): (i*1000003) ^ (step*999983) ^ (seed*7919)
 __device__ __forceinline__ uint32_t wang_base(uint32_t i, int32_t step, int32_t seed) {
     return (i * 1000003u) ^ ((uint32_t)step * 999983u) ^ ((uint32_t)seed * 7919u);
 }
 
-// The six raw draws of one body-step.  WANG = BrownianForceSystem.java:204-209 salts;
+// The six raw draws of one body-step.  WANG = BrownianForceSystem.java:65-70 salts;
 // PHILOX = counter (body, step, block, 0), key (seed, stream).
 __device__ __forceinline__ void brownian_bits(int rng, uint32_t body, int32_t step, int32_t seed, uint32_t stream,
                                               uint32_t h[6]) {
@@ -76,13 +76,13 @@ __device__ __forceinline__ uint32_t event_bits(int rng, uint32_t item, int32_t s
     return o[0];
 }
 
-// (h >>> 1) / 2147483647f  (BrownianForceSystem.java:211)
+// (h >>> 1) / 2147483647f  (BrownianForceSystem.java:72-77)
 __device__ __forceinline__ float uniform01(uint32_t h) { return (float)(int32_t)(h >> 1) / 2147483647.0f; }
 
 __device__ __forceinline__ float fmax_java(float a, float b) { return a >= b ? a : b; }
 
 // Box-Muller noise of one body-step: gf = cos branch (force), gt = sin branch (torque),
-// BrownianForceSystem.java:211-232.  log/cos/sin are evaluated in double and cast, as Java's
+// BrownianForceSystem.java:72-93.  log/cos/sin are evaluated in double and cast, as Java's
 // (float) Math.log((double) u) does.
 __device__ __forceinline__ void brownian_noise(int rng, uint32_t body, int32_t step, int32_t seed, uint32_t stream,
                                                float gf[3], float gt[3]) {
@@ -101,7 +101,7 @@ __device__ __forceinline__ void brownian_noise(int rng, uint32_t body, int32_t s
     }
 }
 
-// randForce/randTorque amplitude (BrownianForceSystem.java:234-243): ((tS*bfm)*sqrt(g))*gauss.
+// randForce/randTorque amplitude (BrownianForceSystem.java:95-104): ((tS*bfm)*sqrt(g))*gauss.
 // The reference's (float)Math.sqrt((double)x) is sqrtf(x) for every float x: a square root rounded to
 // double and then to float is correctly rounded (53 >= 2 * 24 + 2), and sqrtf is correctly rounded
 // here (-fhip-fp32-correctly-rounded-divide-sqrt).
@@ -111,7 +111,7 @@ __device__ __forceinline__ float brown_amp(float scale, float bfm, float gam, fl
 
 // ---------------------------------------------------------------------------------------
 // Integrate + derive of one body (RigidRodLangevinIntegrationSystem.java:49-111 followed by
-// DerivedGeometrySystem.java:152-185; the fused order of MechanicsFusion.integDerive,
+// DerivedGeometrySystem.java:43-71; the fused order of MechanicsFusion.integDerive,
 // MechanicsFusion.java:129-200).  In: pose, lab-frame F/T, body-frame random F/T, drag.
 // Out: new coord, u, y (re-orthogonalised), z, end1, end2.
 struct Pose {

@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(T) k_integrate(float *__restrict__ coord, floa
     yVec[iz] = nyZ * nyInv;
 }
 
-// DerivedGeometrySystem.java:152-185
+// DerivedGeometrySystem.java:43-71
 __global__ void __launch_bounds__(T) k_derive(const float *__restrict__ coord, const float *__restrict__ uVec,
                                               float *__restrict__ yVec, float *__restrict__ zVec,
                                               float *__restrict__ end1, float *__restrict__ end2,

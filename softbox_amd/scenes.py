@@ -252,7 +252,7 @@ def ensemble(n_rep: int = 64, seed: int = 0, fil_range=(2000, 40000), conc_range
 
 def derive_host(fs: FilamentStore) -> None:
     """Initial derived geometry in float32 with the reference's operation order
-    (DerivedGeometrySystem.java:156-185), so the first step reads consistent end1/end2."""
+    (DerivedGeometrySystem.java:63-71), so the first step reads consistent end1/end2."""
     f = np.float32
     u = fs.plane("uVec")
     y = fs.plane("yVec")

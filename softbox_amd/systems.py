@@ -51,7 +51,7 @@ class ChainBendingForceSystem:
 
 
 class BrownianForceSystem:
-    """BrownianForceSystem.java:170-245 (rng selects the reference Wang stream or Philox)."""
+    """BrownianForceSystem.java:44-106 (rng selects the reference Wang stream or Philox)."""
 
     @staticmethod
     def brownianForce(ctx: Context, randForce, randTorque, bTransGam, bRotGam, brownTransScale, brownRotScale,
@@ -77,7 +77,7 @@ class RigidRodLangevinIntegrationSystem:
 
 
 class DerivedGeometrySystem:
-    """DerivedGeometrySystem.java:137-186"""
+    """DerivedGeometrySystem.java:27-73"""
 
     @staticmethod
     def derive(ctx: Context, coord, uVec, yVec, zVec, end1, end2, segLength, counts):

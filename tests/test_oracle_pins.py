@@ -101,7 +101,7 @@ def test_uniform_mapping_edges():
 @pytest.mark.parametrize("rng", [0, 1])
 def test_brownian_draws_are_standard_normal_and_fdt_scaled(rng):
     """Box-Muller output: mean 0, variance 1, independent force/torque branches; amplitude
-    tS*sqrt(2kT/dt)*sqrt(gamma) (BrownianForceSystem.java:159-163)."""
+    tS*sqrt(2kT/dt)*sqrt(gamma) (BrownianForceSystem.java:95-104)."""
     n = 200000
     g = np.ones(3 * n, np.float32)
     scale = np.ones(n, np.float32)
