@@ -49,18 +49,39 @@ def flags() -> list[str]:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every csrc/*.hip into one shared library; returns its path."""
+    """Compile every csrc/*.hip to an object (in parallel) and link them into one shared library;
+    returns its path.  Objects go to a build folder under csrc/ that git ignores."""
+    from concurrent.futures import ThreadPoolExecutor
+
     srcs = sources()
     if not srcs:
         raise RuntimeError(f"no HIP sources under {CSRC}")
+    obj_dir = os.path.join(CSRC, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    comp = [f for f in flags() if f not in ("-shared",)]
+
+    def compile_one(src: str) -> str:
+        obj = os.path.join(obj_dir, os.path.basename(src) + ".o")
+        cmd = [hipcc(), *comp, "-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        proc = subprocess.run(cmd, capture_output=True, text=True)
+        if proc.returncode != 0:
+            sys.stderr.write(proc.stdout + proc.stderr)
+            raise RuntimeError(f"hipcc failed on {os.path.basename(src)} with exit code {proc.returncode}")
+        return obj
+
+    workers = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        objs = list(pool.map(compile_one, srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *flags(), "-o", tmp, *srcs]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         sys.stderr.write(proc.stdout + proc.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {proc.returncode}")
+        raise RuntimeError(f"hipcc link failed with exit code {proc.returncode}")
     os.replace(tmp, LIB)
     return LIB
 

@@ -4,9 +4,11 @@
 // and CrossBridgeSystem's CSR inverse (CrossBridgeSystem.java:190-298).  The reference builds it
 // with per-chunk private histogram rows so that the within-key order is ascending item index.  On
 // gfx950 the same unique result is produced by: atomic key histogram (which also hands each item a
-// provisional slot) -> one-pass tiled scan -> scatter to the provisional slots -> per-item rank
-// placement (rank = number of same-key items with a smaller index, read back from the unordered
-// segment).  The totals and the final order do not depend on atomic ordering, so the CSR is
+// provisional slot) -> one-pass tiled scan -> scatter to the provisional slots -> stable placement
+// (ascending item index within each key's segment): a segment of up to LONG_SEG items is placed by
+// per-item rank (the number of same-key items with a smaller index, read back from the unordered
+// segment), a longer one by one workgroup sorting it in LDS, so no segment costs more than
+// O(L log^2 L) work.  The totals and the final order do not depend on atomic ordering, so the CSR is
 // bit-identical to the serial reference build.
 #include "sbx_internal.h"
 
@@ -52,7 +54,8 @@ int fill_i32(sbx_ctx *c, int *dst, int value, long long count) {
 // ------------------------------------------------------------------ scan
 // Tile = 256 threads x 8 items.  Wave64 inclusive scan by __shfl_up, then wave totals in LDS.
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_MIN_TILE = SCAN_THREADS * 4;  // smallest tile (scanItems 4): sizes the tile sums
+constexpr int SCAN_MIN_TILE = SCAN_MIN_TILE_KEYS;  // smallest tile (scanItems 4): sizes the tile sums
+static_assert(SCAN_MIN_TILE == SCAN_THREADS * 4, "scan tile sizes");
 
 __device__ __forceinline__ int wave_inclusive_scan(int v) {
     const int lane = threadIdx.x & 63;
@@ -92,8 +95,10 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *total) {
 // Exclusive scan of tileSums[nTiles] into tilePrefix[nTiles + 1] (tilePrefix[nTiles] = the grand
 // total): one block, each thread a contiguous run of tiles.
 __global__ void __launch_bounds__(1024) k_tile_prefix(const int *__restrict__ tileSums, int nTiles,
-                                                      int *__restrict__ tilePrefix, const int *__restrict__ gate) {
+                                                      int *__restrict__ tilePrefix, const int *__restrict__ gate,
+                                                      int *__restrict__ longCount) {
     if (gate && *gate == 0) return;
+    if (longCount && threadIdx.x == 0) *longCount = 0;  // the sort's long-segment list starts empty
     __shared__ int waveSums[16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int per = (nTiles + 1023) / 1024;
@@ -180,15 +185,21 @@ int sort_scratch_reserve(sbx_ctx *c, SortScratch &s, long long nKeys, long long 
         s.tileCap = nt;
         s.clean = false;  // zeroed by the next sort
     }
-    if (nItems > s.itemCap) {
+    if (nItems > s.itemCap || !s.longCount) {
         if (s.tmp) hipFree(s.tmp);
         if (s.slot) hipFree(s.slot);
-        s.tmp = s.slot = nullptr;
+        if (s.longCount) hipFree(s.longCount);
+        s.tmp = s.slot = s.longList = s.longCount = nullptr;
         s.itemCap = 0;
         int r = check_hip(c, hipMalloc(&s.tmp, sizeof(int) * (size_t)(nItems + 1)), "hipMalloc tmp");
         if (r) return r;
         r = check_hip(c, hipMalloc(&s.slot, sizeof(int) * (size_t)(nItems + 1)), "hipMalloc slot");
         if (r) return r;
+        // [0] = count of long segments, then their keys (at most nItems / (LONG_SEG + 1))
+        r = check_hip(c, hipMalloc(&s.longList, sizeof(int) * (size_t)(long_list_len(nItems) + 1)), "hipMalloc long");
+        if (r) return r;
+        s.longCount = s.longList;
+        s.longList = s.longCount + 1;
         s.itemCap = nItems;
     }
     if (!s.clean) {
@@ -201,11 +212,16 @@ int sort_scratch_reserve(sbx_ctx *c, SortScratch &s, long long nKeys, long long 
 }
 
 void sort_scratch_free(SortScratch &s) {
+    if (!s.owned) {
+        s = SortScratch();
+        return;
+    }
     if (s.counts) hipFree(s.counts);
     if (s.tileSums) hipFree(s.tileSums);
     if (s.tilePrefix) hipFree(s.tilePrefix);
     if (s.tmp) hipFree(s.tmp);
     if (s.slot) hipFree(s.slot);
+    if (s.longCount) hipFree(s.longCount);
     s = SortScratch();
 }
 
@@ -279,7 +295,8 @@ __global__ void __launch_bounds__(256) k_key_scatter(const int *__restrict__ key
 __global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, int nItems, int nKeys,
                                                    const int *__restrict__ offsets, const int *__restrict__ tmp,
                                                    int *__restrict__ list, int contentsCap,
-                                                   const int *__restrict__ gate, SortPayload pay) {
+                                                   const int *__restrict__ gate, SortPayload pay,
+                                                   int *__restrict__ longCount, int *__restrict__ longList) {
     if (gate && *gate == 0) return;
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nItems) return;
@@ -288,6 +305,11 @@ __global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, 
     int start = offsets[k];
     int end = offsets[k + 1];
     if (end > contentsCap) end = contentsCap;
+    if (end - start > LONG_SEG) {
+        // a long segment is placed by k_sort_long; the item at its first position lists it once
+        if (tmp[start] == m) longList[atomicAdd(longCount, 1)] = k;
+        return;
+    }
     int rank = 0;
     for (int p = start; p < end; p++) rank += (tmp[p] < m) ? 1 : 0;
     int pos = start + rank;
@@ -295,6 +317,63 @@ __global__ void __launch_bounds__(256) k_key_place(const int *__restrict__ key, 
         list[pos] = m;
         if (pay.in) pay.out[pos] = pay.in[m];
         if (pay.keyOut) pay.keyOut[pos] = k;
+    }
+}
+
+// Segments longer than LONG_SEG: one workgroup per segment (grid-stride over the listed keys).  Up to
+// LONG_LDS items are sorted ascending in LDS by a bitonic network (padded with INT_MAX); a longer
+// segment is placed by per-item rank over the whole segment, strided over the workgroup.
+__global__ void __launch_bounds__(LONG_THREADS) k_sort_long(const int *__restrict__ offsets,
+                                                            const int *__restrict__ tmp, int *__restrict__ list,
+                                                            int contentsCap, const int *__restrict__ gate,
+                                                            SortPayload pay, const int *__restrict__ longCount,
+                                                            const int *__restrict__ longList) {
+    if (gate && *gate == 0) return;
+    __shared__ int v[LONG_LDS];
+    const int nLong = *longCount;
+    for (int q = blockIdx.x; q < nLong; q += gridDim.x) {
+        const int k = longList[q];
+        const int start = offsets[k];
+        int end = offsets[k + 1];
+        if (end > contentsCap) end = contentsCap;
+        const int len = end - start;
+        if (len <= LONG_LDS) {
+            int pw = 1;
+            while (pw < len) pw <<= 1;
+            for (int i = threadIdx.x; i < pw; i += LONG_THREADS) v[i] = i < len ? tmp[start + i] : 0x7fffffff;
+            __syncthreads();
+            for (int size = 2; size <= pw; size <<= 1) {
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (int i = threadIdx.x; i < (pw >> 1); i += LONG_THREADS) {
+                        const int lo = 2 * stride * (i / stride) + (i % stride);
+                        const int hi = lo + stride;
+                        const bool up = (lo & size) == 0;
+                        const int a = v[lo], b = v[hi];
+                        if ((a > b) == up) {
+                            v[lo] = b;
+                            v[hi] = a;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int i = threadIdx.x; i < len; i += LONG_THREADS) {
+                const int m = v[i];
+                list[start + i] = m;
+                if (pay.in) pay.out[start + i] = pay.in[m];
+                if (pay.keyOut) pay.keyOut[start + i] = k;
+            }
+            __syncthreads();
+        } else {
+            for (int p = start + threadIdx.x; p < end; p += LONG_THREADS) {
+                const int m = tmp[p];
+                int rank = 0;
+                for (int r = start; r < end; r++) rank += (tmp[r] < m) ? 1 : 0;
+                list[start + rank] = m;
+                if (pay.in) pay.out[start + rank] = pay.in[m];
+                if (pay.keyOut) pay.keyOut[start + rank] = k;
+            }
+        }
     }
 }
 
@@ -306,7 +385,7 @@ void launch_scan_items(const SortScratch &s, const int *counts, int nKeys, int *
     int nt = (nKeys + SCAN_THREADS * ITEMS - 1) / (SCAN_THREADS * ITEMS);
     if (nt < 1) nt = 1;
     hipLaunchKernelGGL(k_tile_reduce<ITEMS>, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, s.tileSums, gate);
-    hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(1024), 0, q, s.tileSums, nt, s.tilePrefix, gate);
+    hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(1024), 0, q, s.tileSums, nt, s.tilePrefix, gate, s.longCount);
     hipLaunchKernelGGL(k_scan_tiles<ITEMS>, dim3(nt), dim3(SCAN_THREADS), 0, q, counts, nKeys, s.tilePrefix, nt,
                        offsets, clampTotal, gate);
 }
@@ -319,6 +398,18 @@ void launch_scan(const SortScratch &s, const int *counts, int nKeys, int *offset
 }
 
 
+
+// stable placement: short segments by per-item rank, long ones by a workgroup each
+void launch_place(const SortScratch &s, const int *key, int nItems, int nKeys, const int *offsets, int *list,
+                  int contentsCap, const int *gate, SortPayload pay, hipStream_t q) {
+    hipLaunchKernelGGL(k_key_place, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys, offsets,
+                       s.tmp, list, contentsCap, gate, pay, s.longCount, s.longList);
+    long long maxLong = nItems / (LONG_SEG + 1);
+    if (maxLong < 1) return;  // no segment of these items can be long
+    unsigned nb = (unsigned)(maxLong < LONG_BLOCKS ? maxLong : LONG_BLOCKS);
+    hipLaunchKernelGGL(k_sort_long, dim3(nb), dim3(LONG_THREADS), 0, q, offsets, s.tmp, list, contentsCap, gate, pay,
+                       s.longCount, s.longList);
+}
 
 int count_keys(sbx_ctx *c, const int *key, int nItems, int nKeys, int *counts, int *slot, hipStream_t st) {
     if (nItems <= 0) return SBX_OK;
@@ -336,16 +427,15 @@ int counting_sort_counted(sbx_ctx *c, SortScratch &s, const int *key, int nItems
     if (nItems > 0) {
         hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
                            offsets, slot, counts, s.tmp, contentsCap, nullptr);
-        hipLaunchKernelGGL(k_key_place, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
-                           offsets, s.tmp, list, contentsCap, nullptr, pay);
+        launch_place(s, key, nItems, nKeys, offsets, list, contentsCap, nullptr, pay, q);
     }
     return check_launch(c, "counting_sort_counted");
 }
 
-int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
-                  int *list, const int *gate, hipStream_t st, SortPayload pay) {
-    int r = sort_scratch_reserve(c, s, nKeys, nItems);
-    if (r) return r;
+// the sort on scratch that is already large enough, with counts[0, nKeys] and tileSums all zero
+int counting_sort_core(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap,
+                       int *offsets, int *list, const int *gate, hipStream_t st, SortPayload pay) {
+    int r;
     hipStream_t q = st ? st : c->stream;
     if (nItems > 0) {
         hipLaunchKernelGGL(k_key_count, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
@@ -355,12 +445,18 @@ int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nK
     if (nItems > 0) {
         hipLaunchKernelGGL(k_key_scatter, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
                            offsets, s.slot, s.counts, s.tmp, contentsCap, gate);
-        hipLaunchKernelGGL(k_key_place, dim3(blocks_for(nItems, 256)), dim3(256), 0, q, key, nItems, nKeys,
-                           offsets, s.tmp, list, contentsCap, gate, pay);
+        launch_place(s, key, nItems, nKeys, offsets, list, contentsCap, gate, pay, q);
     }
     r = check_launch(c, "counting_sort");
     if (r) s.clean = false;
     return r;
+}
+
+int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap, int *offsets,
+                  int *list, const int *gate, hipStream_t st, SortPayload pay) {
+    int r = sort_scratch_reserve(c, s, nKeys, nItems);
+    if (r) return r;
+    return counting_sort_core(c, s, key, nItems, nKeys, contentsCap, offsets, list, gate, st, pay);
 }
 
 }  // namespace sbx

@@ -11,10 +11,19 @@
 namespace sbx {
 // Scratch of the stable counting sort (keys -> CSR offsets + ascending item list).
 // counts[] is kept all-zero between sorts (`clean`), so no per-sort fill is needed.
+// Segments (items of one key) longer than LONG_SEG are placed by a workgroup each (LONG_THREADS
+// threads, LDS sort of up to LONG_LDS items), at most LONG_BLOCKS workgroups per sort.
+constexpr int LONG_SEG = 32;
+constexpr int LONG_LDS = 4096;
+constexpr int LONG_THREADS = 512;
+constexpr int LONG_BLOCKS = 256;
+inline long long long_list_len(long long nItems) { return nItems / (LONG_SEG + 1) + 1; }
 struct SortScratch {
     int *counts = nullptr, *tileSums = nullptr, *tilePrefix = nullptr, *tmp = nullptr, *slot = nullptr;
+    int *longCount = nullptr, *longList = nullptr;  // long segments of the current sort (count, keys)
     long long keyCap = 0, itemCap = 0, tileCap = 0;
     bool clean = false;
+    bool owned = true;  // false: the pointers are views of caller-provided memory (never freed here)
     int scanItems = 8;  // keys per scan thread: tiles of 256 * scanItems keys (4, 8 or 16; same result)
 };
 struct ChainParams;
@@ -72,6 +81,12 @@ int counting_sort(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nK
 int counting_sort_counted(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap,
                           int *offsets, int *list, int *counts, const int *slot, hipStream_t st = nullptr,
                           SortPayload pay = SortPayload());
+// the sort on scratch that is already large enough, with counts[0, nKeys] and tileSums all zero
+int counting_sort_core(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap,
+                       int *offsets, int *list, const int *gate = nullptr, hipStream_t st = nullptr,
+                       SortPayload pay = SortPayload());
+// smallest scan tile (cells): sizes the tile-sum arrays of a sort
+constexpr int SCAN_MIN_TILE_KEYS = 1024;
 // the histogram pass alone: slot[m] = atomicAdd(counts[key[m]], 1)
 int count_keys(sbx_ctx *c, const int *key, int nItems, int nKeys, int *counts, int *slot, hipStream_t st = nullptr);
 int fill_i32(sbx_ctx *c, int *dst, int value, long long count);

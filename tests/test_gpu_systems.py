@@ -229,6 +229,42 @@ def test_csr_build(ctx):
     assert_same(host(ctx, d_lst)[:m], lst[:m], "csr list")
 
 
+@pytest.mark.parametrize("sizes", [(5000, 33, 32, 1), (4096, 4097, 64, 31, 2), (20000,), (40, 33, 34, 35, 36)])
+def test_csr_build_dense_clusters(ctx, sizes):
+    """Keys shared by many items (a dense cluster in one grid cell, many motors on one segment): short
+    segments are ranked per item, segments over 32 items sorted by a workgroup in LDS (up to 4096) or
+    ranked over the whole segment beyond that.  Bit-exact against the serial counting sort, with the
+    items of each key scattered through the index range and mixed with unkeyed items."""
+    g = np.random.default_rng(len(sizes))
+    n_keys = 300
+    keys = np.concatenate([np.full(sz, 7 + 13 * i, np.int32) for i, sz in enumerate(sizes)]
+                          + [g.integers(-2, n_keys, 3000).astype(np.int32)])
+    key = keys[g.permutation(keys.size)]
+    n_items = key.size
+    cnt = np.zeros(n_keys, np.int32)
+    off = np.zeros(n_keys + 1, np.int32)
+    lst = np.zeros(n_items, np.int32)
+    ffi.call("sbo_csr_build", n_keys, n_items, key, cnt, off, lst)
+    for rep in range(2):  # a second sort on the same scratch starts from its cleaned state
+        d_off, d_lst = dev(ctx, np.zeros(n_keys + 1, np.int32)), dev(ctx, np.full(n_items, -7, np.int32))
+        CrossBridgeSystem.csrBuild(ctx, dev(ctx, key), n_keys, d_off, d_lst)
+        assert_same(host(ctx, d_off), off, f"csr offsets (sort {rep})")
+        m = int(off[-1])
+        assert_same(host(ctx, d_lst)[:m], lst[:m], f"csr list (sort {rep})")
+    # the grid form: bodies of a dense cluster binned into few cells
+    gd = np.array([4, 4, 4, 64], np.int32)
+    cell = np.where(key >= 0, key % 64, -1).astype(np.int32)
+    counts = np.array([0, n_items, 0, 0], np.int32)
+    gcnt = np.zeros(64, np.int32)
+    goff = np.zeros(65, np.int32)
+    gcont = np.zeros(n_items, np.int32)
+    ffi.call("sbo_grid_build_serial", n_items, n_items, cell, gd, gcnt, goff, gcont)
+    d_off, d_cont = dev(ctx, np.zeros(65, np.int32)), dev(ctx, np.zeros(n_items, np.int32))
+    SpatialGrid.gridBuild(ctx, dev(ctx, cell), counts, gd, d_off, d_cont)
+    assert_same(host(ctx, d_off), goff, "dense gridCellOffsets")
+    assert_same(host(ctx, d_cont)[: int(goff[-1])], gcont[: int(goff[-1])], "dense gridCellContents")
+
+
 @pytest.mark.parametrize("rng", [RNG_WANG, RNG_PHILOX])
 def test_link_systems(ctx, rng):
     sc = small_network(seed=4, links=600)
