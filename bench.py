@@ -2,20 +2,23 @@
 
 Workload (the one-GPU configuration the metric is quoted on): crosslinked actin network,
 20k filaments x 10 rigid rods (0.1 um), 20 um periodic cube, 50k crosslinkers (rest 35 nm, Bell
-k0 = 0.1/s, dx = 0.5 nm), water, dt = 1e-5 s, FP32 state, broad phase (per-step grid rebuild +
-27-cell stencil, or the learned skin) + crosslink unbind/count/force/torsion + CSR gathers + chain +
-Brownian + integrate + derive.
+k0 = 0.1/s, dx = 0.5 nm), viscosity 0.1 Pa s (the reference's Constants.aeta; the workload states
+none), dt = 1e-5 s, FP32 state, broad phase (per-step grid rebuild + 27-cell stencil, or the learned
+skin) + crosslink unbind/count/force/torsion + CSR gathers + chain + Brownian + integrate + derive.
 Synthetic, seeded scene (softbox_amd.scenes.network); the whole state is resident in HBM before
-the timed region starts.
+the timed region starts.  The crosslinks stay alive through the timed steps (links_alive_start /
+links_alive_end report how many).
 
 Prints ONE JSON line (rank 0).  By default it runs the workload twice from the same initial state:
 the reference's per-step grid rebuild, and the learned-skin broad phase (its skin model trained
-first, outside the timed region, on a per-step-rebuild trajectory of the same scene); `value` is
-the learned-skin rate, `learned_vs_rebuild` the ratio, and the two runs' candidate slices are
+first, outside the timed region, on a per-step-rebuild trajectory of the same scene).  The learned
+skin keeps the per-step rebuild as its fallback and judge: `value` is the faster of the two (named
+in config.broad_phase), `learned_vs_rebuild` is their ratio, and the two runs' candidate slices are
 checked identical.  `roofline` is for the dominant kernel of the headline run: algorithmic bytes
-per launch divided by its mean duration measured with HIP events on the library's stream over
-eager profiled steps.  `cpu_baseline` times the oracle (single-threaded C restatement of the
-reference's sequential CPU runner) on a bounded sample of the same workload.
+per launch divided by its mean duration measured with HIP events on the library's stream.
+`cpu_baseline` times the oracle (the C restatement of the reference's sequential CPU runner) on the
+same steps from the same state the GPU's timed region starts from: the OpenMP build on all host
+threads (`cpu_baseline`) and the single-thread build (`cpu_baseline_single_thread`).
 """
 from __future__ import annotations
 
@@ -98,22 +101,27 @@ def kernel_valu_issue(name: str):
     return None if v is None else v / 100.0
 
 
-def run_cpu_baseline(sc, budget_s: float = 15.0) -> dict:
+def run_cpu_baseline(sc, step0: int, budget_s: float = 12.0, openmp: bool = False) -> dict:
+    """The oracle stepping the workload from state `sc` (the GPU's state at the start of its timed
+    region) over the same step numbers step0, step0 + 1, ... for about budget_s seconds."""
     from oracle import ffi
 
-    orc = ffi.OracleNet(sc.copy())
-    orc.step(0, 1)  # warm (page-in)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)) if openmp else 1
+    orc = ffi.OracleNet(sc.copy(), ffi.load_omp() if openmp else None)
     steps, t0 = 0, time.perf_counter()
     while True:
-        orc.step(1 + steps, 1)
+        orc.step(step0 + steps, 1)
         steps += 1
         el = time.perf_counter() - t0
-        if el > budget_s or steps >= 200:
+        if el > budget_s or steps >= 400:
             break
-    return {"value": sc.fs.n * steps / el, "unit": "rod-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle C restatement (single thread, same seeded scene, {sc.fs.n} rods + "
-                      f"{sc.xl.nLinks if sc.xl is not None else 0} links, full step incl. grid) "
-                      f"for {steps} steps = {el:.1f} s"}
+    alive = int((orc.xl.linkState[: orc.xl.nLinks] >= 0).sum()) if orc.xl is not None else 0
+    form = f"OpenMP build, {threads} threads" if openmp else "single thread"
+    return {"value": sc.fs.n * steps / el, "unit": "rod-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle C restatement ({form}), the GPU's state at the start of its timed region "
+                      f"({sc.fs.n} rods, {sc.xl.nLinks if sc.xl is not None else 0} link slots), steps "
+                      f"{step0}..{step0 + steps - 1}, full step incl. grid: {steps} steps in {el:.1f} s; "
+                      f"{alive} links alive at the end"}
 
 
 def shape_of(args, tuned=None) -> dict:
@@ -177,7 +185,8 @@ def main() -> int:
     ap.add_argument("--n-fil", type=int, default=20000)
     ap.add_argument("--n-links", type=int, default=50000)
     ap.add_argument("--box", type=float, default=20.0)
-    ap.add_argument("--eta", type=float, default=1.0e-3, help="viscosity, Pa s (the workload: water)")
+    ap.add_argument("--eta", type=float, default=0.1,
+                    help="viscosity, Pa s (default: the reference's Constants.aeta = 0.1)")
     ap.add_argument("--tune-steps", type=int, default=8,
                     help="autotuner: steps timed per launch shape after the warm-up (0: default shape)")
     ap.add_argument("--shape", type=str, default="",
@@ -256,8 +265,14 @@ def main() -> int:
         elif extra:
             net.step(t0, extra)
             t0 += extra
+        fs0, xl0 = net.pull(sc.fs.copy(), sc.xl.copy())
+        links_start = int((xl0.linkState[: xl0.nLinks] >= 0).sum())
+        if mode == modes[args.skin][0]:
+            start_state = (sc.copy(), t0)
+            start_state[0].fs, start_state[0].xl = fs0, xl0
         el = timed(net, ctx, t0, args.steps)
-        r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps}
+        r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "timed_steps": [t0, t0 + args.steps - 1],
+             "links_alive_start": links_start}
         if args.shape or (args.tune_steps and mode == "rebuild"):
             r["shape"] = shape_of(args, None if args.shape else tuned)
         r["digest"] = slices_digest(net, sc)
@@ -301,7 +316,7 @@ def main() -> int:
             dur_us, timing = head["broad_phase_back_to_back_us"], "HIP events around 20 back-to-back launches"
         achieved = b / (dur_us * 1e-6) / 1e9
         # the committed counter summaries are of the default workload (200k rods, 50k links, 20 um, water)
-        profiled = (args.n_fil, args.n_links, args.box, args.eta) == (20000, 50000, 20.0, 1.0e-3)
+        profiled = (args.n_fil, args.n_links, args.box, args.eta) == (20000, 50000, 20.0, 0.1)
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": kernel_traffic(dom) if profiled else None, "bytes_per_launch": b,
@@ -312,7 +327,10 @@ def main() -> int:
                 kb = kernel_bytes(sc, k, rr.get("skin", {}).get("list_entries"))
                 if kb:
                     v["GBs"] = round(kb / (v["avg_us"] * 1e-6) / 1e9, 1)
-    cpu = None if args.no_cpu else run_cpu_baseline(sc)
+    cpu = cpu1 = None
+    if not args.no_cpu:
+        cpu = run_cpu_baseline(start_state[0], start_state[1], openmp=True)
+        cpu1 = run_cpu_baseline(start_state[0], start_state[1], openmp=False)
 
     bp = {"compare": f"learned-skin policy (faster of learned skin and per-step rebuild: {choice})",
           "rebuild": "per-step rebuild", "fixed": "fixed skin", "learned": "learned skin"}[args.skin]
@@ -325,7 +343,8 @@ def main() -> int:
                                f"viscosity {args.eta:g} Pa s",
                    "rods": n, "links": sc.xl.nLinks, "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
                    "broad_phase": bp},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu1,
+        "links_alive_start": head["links_alive_start"], "links_alive_end": head["links_alive_end"],
     }
     if "rebuild" in runs and "learned" in runs:
         out["learned_vs_rebuild"] = round(runs["learned"]["value"] / runs["rebuild"]["value"], 3)
