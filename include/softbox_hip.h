@@ -141,6 +141,16 @@ int sbx_broad_phase(sbx_ctx *c, const float *center, const float *boundingRadius
  * counting sort of items by key (key < 0 skipped): offsets[nKeys+1], list[nItems] ascending item index. */
 int sbx_csr_build(sbx_ctx *c, const int *key, int nItems, int nKeys, int *offsets, int *list);
 
+/* No counterpart in the reference: sbx_csr_build with no device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as sbx_csr_build.  Its scratch is a fixed device array of the library (about
+ * 34 MB per GPU, held until the process ends) sized for at most SBX_NOALLOC_MAX_KEYS keys and
+ * SBX_NOALLOC_MAX_ITEMS items; larger inputs are rejected with SBX_ERR_ARG.  Every call on one GPU shares
+ * that array, so calls that overlap on one GPU (other host threads, other streams) race. */
+#define SBX_NOALLOC_MAX_KEYS 4194304
+#define SBX_NOALLOC_MAX_ITEMS 2097152
+int sbx_csr_build_noalloc(sbx_ctx *c, const int *key, int nItems, int nKeys, int *offsets, int *list);
+
 /* CrosslinkerSystem.countActiveLinks (CrosslinkerSystem.java:694-706) */
 int sbx_count_active_links(sbx_ctx *c, const int *linkState, const int *linkFilA, const int *linkFilB,
                            int *activeLinkCount, int C, int nSeg);

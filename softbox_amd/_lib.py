@@ -11,6 +11,8 @@ import os
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsoftbox_hip.so")
 
 SBX_OK = 0
+SBX_NOALLOC_MAX_KEYS = 4194304     # sbx_csr_build_noalloc limits (include/softbox_hip.h)
+SBX_NOALLOC_MAX_ITEMS = 2097152
 SBX_ERR_ARG = -1
 SBX_ERR_HIP = -2
 SBX_ERR_NOMEM = -3
@@ -145,6 +147,7 @@ _SIGS = {
     "sbx_grid_build": [vp, vp, i32p, i32p, vp, vp, i32],
     "sbx_broad_phase": [vp, vp, vp, vp, vp, vp, i32p, f32p, i32p, vp, vp, i32, i32, i32],
     "sbx_csr_build": [vp, vp, i32, i32, vp, vp],
+    "sbx_csr_build_noalloc": [vp, vp, i32, i32, vp, vp],
     "sbx_count_active_links": [vp, vp, vp, vp, vp, i32, i32],
     "sbx_link_unbind": [vp] + [vp] * 10 + [f32p, i32p, i32, i32, u32],
     "sbx_link_forces": [vp] + [vp] * 10 + [f32p, i32, i32],

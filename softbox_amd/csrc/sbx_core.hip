@@ -487,6 +487,11 @@ int sbx_create(sbx_ctx **out, int device, void *hip_stream) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    if (noalloc_setup(c) != SBX_OK) {  // this device's address of sbx_csr_build_noalloc's scratch
+        if (c->owns_stream) hipStreamDestroy(c->stream);
+        delete c;
+        return SBX_ERR_HIP;
+    }
     *out = c;
     return SBX_OK;
 }

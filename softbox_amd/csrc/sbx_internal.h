@@ -17,7 +17,7 @@ constexpr int LONG_SEG = 32;
 constexpr int LONG_LDS = 4096;
 constexpr int LONG_THREADS = 512;
 constexpr int LONG_BLOCKS = 256;
-inline long long long_list_len(long long nItems) { return nItems / (LONG_SEG + 1) + 1; }
+constexpr long long long_list_len(long long nItems) { return nItems / (LONG_SEG + 1) + 1; }
 struct SortScratch {
     int *counts = nullptr, *tileSums = nullptr, *tilePrefix = nullptr, *tmp = nullptr, *slot = nullptr;
     int *longCount = nullptr, *longList = nullptr;  // long segments of the current sort (count, keys)
@@ -85,6 +85,8 @@ int counting_sort_counted(sbx_ctx *c, SortScratch &s, const int *key, int nItems
 int counting_sort_core(sbx_ctx *c, SortScratch &s, const int *key, int nItems, int nKeys, int contentsCap,
                        int *offsets, int *list, const int *gate = nullptr, hipStream_t st = nullptr,
                        SortPayload pay = SortPayload());
+// the allocation-free entry point's set-up on c's device (sbx_noalloc.hip), called by sbx_create
+int noalloc_setup(sbx_ctx *c);
 // smallest scan tile (cells): sizes the tile-sum arrays of a sort
 constexpr int SCAN_MIN_TILE_KEYS = 1024;
 // the histogram pass alone: slot[m] = atomicAdd(counts[key[m]], 1)
