@@ -26,6 +26,12 @@ def build() -> str:
     return LIB_PATH
 
 
+FORM_SCRATCH_I = ("filID", "filIDScratch", "bodyCell", "cellCount", "gridCellOffsets", "gridCellContents",
+                  "candCountSeg", "candBaseSeg", "reqFilA", "reqFilB", "reqOrient", "gatePass", "minCand",
+                  "acceptFlag", "freeCount", "freeOffsets", "freeList", "rankOffsets")
+FORM_SCRATCH_F = ("center", "reqLoc1", "reqLoc2")
+
+
 class SboNet(C.Structure):
     _fields_ = (
         [("nRod", i32)]
@@ -43,7 +49,20 @@ class SboNet(C.Structure):
            ("gridDims", C.c_int * 4)]
         + [(k, vp) for k in ("center", "boundingRadius", "bodyCell", "cellCount", "gridCellOffsets",
                              "gridCellContents", "candPartner", "candCount")]
-        + [("seed", i32), ("rng", i32), ("stream", u32)]
+        + [("seed", i32), ("rng", i32), ("stream", u32), ("form", vp), ("formInt", i32)]
+    )
+
+
+class SboForm(C.Structure):
+    _fields_ = (
+        [("nSeg", i32), ("C", i32), ("reqCap", i32)]
+        + [(k, vp) for k in ("coord", "uVec", "end1", "end2", "segLength", "filState", "end2NbrSlot", "linkFilA",
+                             "linkFilB", "linkState", "strainPlace", "linkOrientSame", "torqueMagPlace",
+                             "activeLinkCount", "loc1", "loc2", "strainHist", "torqueMagHist")]
+        + [("gridParams", C.c_float * 6), ("gridDims", C.c_int * 4), ("periodic", i32), ("boxLen", C.c_float * 3),
+           ("formParams", C.c_float * 7), ("mode", i32), ("formCounts", C.c_int * 4)]
+        + [(k, vp) for k in FORM_SCRATCH_I + FORM_SCRATCH_F]
+        + [("lastFormed", i32)]
     )
 
 
@@ -159,6 +178,11 @@ class OracleNet:
                              candPartner=np.full(n * mc, -1, np.int32), candCount=np.zeros(n, np.int32))
             for k, a in self.grid.items():
                 setattr(s, k, P(a).value)
+        self.form = None
+        if p.get("formOn"):
+            self.form = OracleFormation(fs, xl, p)
+            s.form = C.cast(C.pointer(self.form.s), C.c_void_p).value
+            s.formInt = int(p["formInt"])
         self.s = s
         self.lib = lib if lib is not None else load()
 
@@ -259,3 +283,49 @@ class OracleContractile(OracleNet):
     def step(self, step0: int, n: int = 1) -> None:
         for t in range(step0, step0 + n):
             self.lib.sbo_contractile_step(C.byref(self.s), C.byref(self.m), t)
+
+
+class OracleFormation:
+    """An sbo_form over a filament store and crosslinker store (host arrays, updated in place by
+    sbo_formation): the formation grid, request arrays and allocator scratch it needs."""
+
+    def __init__(self, fs, xl, p):
+        n, L = fs.n, xl.nLinks
+        f = SboForm()
+        f.nSeg, f.C = n, L
+        f.reqCap = int(p["formReqCap"])
+        self.keep = []
+        for k in ("coord", "uVec", "end1", "end2", "segLength", "end2NbrSlot"):
+            setattr(f, k, P(getattr(fs, k)).value)
+        self.filState = getattr(fs, "filState", None)
+        f.filState = P(self.filState).value if self.filState is not None else None
+        for k in ("linkFilA", "linkFilB", "linkState", "strainPlace", "linkOrientSame", "torqueMagPlace",
+                  "activeLinkCount", "loc1", "loc2", "strainHist", "torqueMagHist"):
+            setattr(f, k, P(getattr(xl, k)).value)
+        for i, v in enumerate(p["formGridParams"]):
+            f.gridParams[i] = v
+        for i, v in enumerate(p["formGridDims"]):
+            f.gridDims[i] = v
+        f.periodic = int(p["periodic"])
+        for i, v in enumerate(p["boxLen"]):
+            f.boxLen[i] = v
+        for i, v in enumerate(p["formParams"]):
+            f.formParams[i] = v
+        f.mode = int(p["formMode"])
+        cells = int(p["formGridDims"][3])
+        rc = f.reqCap
+        sizes = dict(filID=n, filIDScratch=n, bodyCell=n, cellCount=cells, gridCellOffsets=cells + 1,
+                     gridCellContents=n, candCountSeg=n, candBaseSeg=n + 1, reqFilA=rc, reqFilB=rc, reqOrient=rc,
+                     gatePass=rc, minCand=n, acceptFlag=rc, freeCount=max(1, L), freeOffsets=L + 1,
+                     freeList=max(1, L), rankOffsets=rc + 1)
+        self.a = {k: np.zeros(v, np.int32) for k, v in sizes.items()}
+        self.a.update(center=np.zeros(3 * n, np.float32), reqLoc1=np.zeros(rc, np.float32),
+                      reqLoc2=np.zeros(rc, np.float32))
+        for k, arr in self.a.items():
+            setattr(f, k, P(arr).value)
+        self.s = f
+
+    @property
+    def n_candidates(self) -> int:
+        return int(self.s.formCounts[0])
+

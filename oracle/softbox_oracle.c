@@ -1080,6 +1080,8 @@ void sbo_network_finish(sbo_net *net, int32_t step) {
  * linkForces -> linkTorsion -> CSR A -> gatherA -> CSR B -> gatherB] -> confine -> integrate -> derive. */
 void sbo_network_step(sbo_net *net, int32_t step) {
     if (net->gridOn) sbo_network_grid(net);
+    if (net->form && net->formInt > 0 && step % net->formInt == 0)   /* V2OneXHarness.java:379 */
+        net->form->lastFormed = sbo_formation(net->form, step, net->seed, net->rng, net->stream);
     sbo_network_forces(net, step);
     sbo_network_finish(net, step);
 }

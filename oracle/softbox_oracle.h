@@ -201,7 +201,67 @@ typedef struct sbo_net {
     int32_t seed;
     int rng;
     uint32_t stream;
+    /* crosslinker formation at a cadence (V2OneXHarness.java:379): NULL = off */
+    struct sbo_form *form;
+    int formInt;
 } sbo_net;
+
+/* ---- crosslinker formation (softbox_oracle_form.c; CrosslinkerSystem.java:298-760, FormationGrid.java,
+ * FilIDSystem.java) ---- */
+#define SBO_FORM_MAXC 256      /* CrosslinkerSystem.FORM_MAXC (CrosslinkerSystem.java:457) */
+#define SBO_STRAIN_WIN 10
+#define SBO_TORQUE_WIN 5
+int sbo_filid_rounds(int n);                                                              /* FilIDSystem.java:31-34 */
+void sbo_filid_init(int n, const int *filState, const int *end2NbrSlot, int *ptr);       /* :38-47 */
+void sbo_filid_jump(int n, const int *ptrIn, int *ptrOut, const int *filState);          /* :52-59 */
+void sbo_filid(int n, const int *filState, const int *end2NbrSlot, int *filID, int *scratch);
+void sbo_fil_fil_candidates(int nSeg, const float *coord, const float *segLength, const int *filID, int reqCap,
+                            int *reqFilA, int *reqFilB, const float *formParams, int *formCounts, int periodic,
+                            const float *boxLen);                                         /* CrosslinkerSystem.java:405-431 */
+void sbo_grid_form_count(int nSeg, const float *coord, const float *segLength, const int *filID,
+                         const float *gridParams, const int *gridDims, const int *offsets, const int *contents,
+                         int *candCountSeg, const float *formParams, int periodic, const float *boxLen); /* :461-513 */
+void sbo_grid_form_scan(int nSeg, const int *candCountSeg, int *candBaseSeg, int reqCap, int *reqFilA, int *reqFilB,
+                        int *formCounts);                                                 /* :516-529 */
+void sbo_grid_form_emit(int nSeg, const float *coord, const float *segLength, const int *filID,
+                        const float *gridParams, const int *gridDims, const int *offsets, const int *contents,
+                        const int *candBaseSeg, const int *candCountSeg, int reqCap, int *reqFilA, int *reqFilB,
+                        const float *formParams, int periodic, const float *boxLen);     /* :535-604 */
+void sbo_form_gates(int nSeg, int nCand, const float *coord, const float *uVec, const float *end1, const float *end2,
+                    const float *segLength, const int *reqFilA, const int *reqFilB, float *reqLoc1, float *reqLoc2,
+                    int *reqOrient, int *gatePass, const float *formParams, int32_t step, int32_t seed, int mode,
+                    int rng, uint32_t stream, int periodic, const float *boxLen);         /* :607-676 */
+void sbo_form_admit_reduce(int nCand, int reqCap, int nSeg, const int *reqFilA, const int *reqFilB,
+                           const int *gatePass, int *minCand);                            /* :680-691 */
+void sbo_form_admit(int nCand, int C, const int *reqFilA, const int *reqFilB, const float *reqLoc1,
+                    const float *reqLoc2, const int *gatePass, const int *minCand, const int *activeLinkCount,
+                    const int *linkState, const int *linkFilA, const int *linkFilB, const float *loc1,
+                    const float *loc2, int *acceptFlag, const float *formParams);         /* :713-738 */
+int sbo_form_allocate(int K, int C, const int *acceptFlag, const int *reqFilA, const int *reqFilB,
+                      const float *reqLoc1, const float *reqLoc2, const int *reqOrient, int *freeCount,
+                      int *freeOffsets, int *freeList, int *rankOffsets, int *linkFilA, int *linkFilB, float *loc1,
+                      float *loc2, int *linkState, float *strainHist, int *strainPlace, int *linkOrientSame,
+                      float *torqueMagHist, int *torqueMagPlace);                         /* :298-357, 742-760 */
+typedef struct sbo_form {
+    int nSeg, C, reqCap;
+    const float *coord, *uVec, *end1, *end2, *segLength;    /* filament store (pose at the pass) */
+    const int *filState, *end2NbrSlot;                      /* filState NULL: every slot ACTIVE */
+    int *linkFilA, *linkFilB, *linkState, *strainPlace, *linkOrientSame, *torqueMagPlace, *activeLinkCount;
+    float *loc1, *loc2, *strainHist, *torqueMagHist;
+    float gridParams[6];                                    /* the formation grid (FormationGrid.java:63-76) */
+    int gridDims[4];
+    int periodic;
+    float boxLen[3];
+    float formParams[7];                                    /* CrosslinkerStore.setFormParams (:196-206) */
+    int mode;
+    int formCounts[4];                                      /* [0] = candidates of the last pass */
+    int *filID, *filIDScratch, *bodyCell, *cellCount, *gridCellOffsets, *gridCellContents, *candCountSeg,
+        *candBaseSeg, *reqFilA, *reqFilB, *reqOrient, *gatePass, *minCand, *acceptFlag, *freeCount, *freeOffsets,
+        *freeList, *rankOffsets;
+    float *center, *reqLoc1, *reqLoc2;
+    int lastFormed;
+} sbo_form;
+int sbo_formation(sbo_form *f, int32_t step, int32_t seed, int rng, uint32_t stream); /* V2OneXHarness.java:444-464 */
 
 void sbo_network_step(sbo_net *net, int32_t step);
 void sbo_network_grid(sbo_net *net);                   /* publish + body cell + grid + broad phase */

@@ -724,6 +724,8 @@ void sbo_contractile_step(sbo_net *net, sbo_motors *mo, int32_t step) {
     sbo_register_force_dot(nM, mo->bondData, mo->boundSeg, mo->forceDotFil, mo->forceMag, mo->forceDotHist,
                            mo->forceDotPlace);
     /* --- filament block: forces, motor -> segment CSR + gather, confine, integrate, derive --- */
+    if (net->form && net->formInt > 0 && step % net->formInt == 0)   /* V2OneXHarness.java:379 */
+        net->form->lastFormed = sbo_formation(net->form, step, net->seed, net->rng, net->stream);
     sbo_network_forces(net, step);
     sbo_csr_build(n, nM, mo->boundSeg, mo->segMotorCount, mo->segMotorOffsets, mo->segMotorMyo);
     sbo_motor_seg_gather(n, mo->segMotorOffsets, mo->segMotorMyo, mo->bondData, net->forceSum, net->torqueSum);

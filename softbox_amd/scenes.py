@@ -97,6 +97,29 @@ def grid_for(box: float, bound_r: float, cutoff: float, periodic: bool, min_cell
                 boxLen=box_len, periodic=1 if periodic else 0)
 
 
+ACTIN_MONO_DIAM = 2.0 * ACTIN_MONO_RADIUS   # Constants.actinMonoDiam = 0.0054 um
+
+
+def formation_params(box: float, seg_len: float, n_seg: int, periodic: bool, grab: float = 0.04,
+                     max_angle: float = 0.6, min_sep: float = 5.0 * ACTIN_MONO_DIAM, max_links: int = 10,
+                     on_rate: float = 40.0, conc: float = 1.0, dt: float = 1.0e-5, interval: int = 100,
+                     jitter: float = 2.0 * ACTIN_MONO_DIAM, mode: int = 0, req_cap: int | None = None) -> dict:
+    """Crosslinker formation at a cadence, with the dense preset of V2OneXHarness (V2OneXHarness.java:74-96:
+    aperture 0.6 rad, spacing 5 monomer diameters, 10 links per segment, kon 40 /uM/s at 1 uM, jitter 2
+    monomer diameters, every 100 steps) and the workloads' binding radius 40 nm.  P_form =
+    1 - exp(-kon conc dt interval) (V2OneXHarness.pForm).  The formation grid is FormationGrid's
+    (FormationGrid.java:63-76): cell 2 (L/2 + r) + grab, a cell of margin around the box (periodic: the
+    box tiled exactly).  req_cap bounds the candidate list (the reference: n_seg * FORM_MAXC)."""
+    cell = 2.0 * (0.5 * seg_len + ACTIN_RADIUS) + grab
+    g = grid_for(box, 0.5 * seg_len + ACTIN_RADIUS, grab, periodic, margin=cell)
+    p_form = 1.0 - math.exp(-on_rate * conc * dt * interval)
+    if req_cap is None:
+        req_cap = min(n_seg * 256, max(4096, 16 * n_seg))
+    fp = np.array([max_angle, grab * grab, min_sep, max_links, p_form, jitter, grab], np.float32)
+    return dict(formOn=1, formInt=int(interval), formParams=[float(v) for v in fp], formMode=int(mode),
+                formGridParams=g["gridParams"], formGridDims=g["gridDims"], formReqCap=int(req_cap))
+
+
 def _base_params(dt: float, kT: float) -> dict:
     return dict(params=[dt, brownian_force_mag(dt, kT)], chainParams=[dt, 0.5, 0.1, 0.265, 0.0, 1.0e-20,
                                                                        ACTIN_MONO_RADIUS],
