@@ -151,6 +151,41 @@ int sbx_csr_build(sbx_ctx *c, const int *key, int nItems, int nKeys, int *offset
 #define SBX_NOALLOC_MAX_ITEMS 2097152
 int sbx_csr_build_noalloc(sbx_ctx *c, const int *key, int nItems, int nKeys, int *offsets, int *list);
 
+/* ------------------------------------------------------------------ crosslinker formation
+ * V2OneXHarness.formationCpu (V2OneXHarness.java:444-464).  formParams (host, 7): CrosslinkerStore.formParams;
+ * formCounts (host, 4): [1] step, [2] seed, [3] mode.  periodic / boxLen (host): the periodic extension. */
+/* FilIDSystem.init + rounds(n) jumps (FilIDSystem.java:31-60; V2OneXHarness.computeFilID :466-470);
+ * filState may be NULL (every slot ACTIVE) */
+int sbx_filid(sbx_ctx *c, const int *filState, const int *end2NbrSlot, int *filID, int *filIDScratch, int n);
+/* FormationGrid.formCandidatesCpu = CrosslinkerSystem.gridFormCount + gridFormScan + gridFormEmit
+ * (CrosslinkerSystem.java:461-604) over the formation grid built with sbx_body_cell + sbx_grid_build;
+ * formCountsDev (device): [0] = candidates (the request count, clamped to reqCap) */
+int sbx_grid_form_candidates(sbx_ctx *c, const float *filCoord, const float *filSegLength, const int *filID,
+                             const float *gridParams, const int *gridDims, const int *gridCellOffsets,
+                             const int *gridCellContents, int *candCountSeg, int *candBaseSeg, int *reqFilA,
+                             int *reqFilB, const float *formParams, int *formCountsDev, int nSeg, int reqCap,
+                             int periodic, const float *boxLen);
+/* CrosslinkerSystem.formGates (CrosslinkerSystem.java:607-676) */
+int sbx_form_gates(sbx_ctx *c, const float *filCoord, const float *filUVec, const float *filEnd1, const float *filEnd2,
+                   const float *filSegLength, const int *reqFilA, const int *reqFilB, float *reqLoc1, float *reqLoc2,
+                   int *reqOrient, int *gatePass, const float *formParams, const int *formCounts, int nSeg, int reqCap,
+                   int rng, uint32_t stream, int periodic, const float *boxLen);
+/* CrosslinkerSystem.formAdmitReduce (CrosslinkerSystem.java:680-691) */
+int sbx_form_admit_reduce(sbx_ctx *c, const int *reqFilA, const int *reqFilB, const int *gatePass, int *minCand,
+                          int reqCap, int nSeg);
+/* CrosslinkerSystem.formAdmit (CrosslinkerSystem.java:713-738) */
+int sbx_form_admit(sbx_ctx *c, const int *reqFilA, const int *reqFilB, const float *reqLoc1, const float *reqLoc2,
+                   const int *gatePass, const int *minCand, const int *activeLinkCount, const int *linkState,
+                   const int *linkFilA, const int *linkFilB, const float *loc1, const float *loc2, int *acceptFlag,
+                   const float *formParams, int reqCap, int nSeg, int C);
+/* CrosslinkerSystem.freeFlags + CrossBridgeSystem.csrScan + freeScatter + csrScan + allocate + placeOrient
+ * (CrosslinkerSystem.java:298-357, 742-760); K requests, C link slots */
+int sbx_form_allocate(sbx_ctx *c, const int *reqFilA, const int *reqFilB, const float *reqLoc1, const float *reqLoc2,
+                      const int *reqOrient, const int *acceptFlag, int *freeCount, int *freeOffsets, int *freeList,
+                      int *rankOffsets, int *linkFilA, int *linkFilB, float *loc1, float *loc2, int *linkState,
+                      float *strainHist, int *strainPlace, int *linkOrientSame, float *torqueMagHist,
+                      int *torqueMagPlace, int K, int C);
+
 /* CrosslinkerSystem.countActiveLinks (CrosslinkerSystem.java:694-706) */
 int sbx_count_active_links(sbx_ctx *c, const int *linkState, const int *linkFilA, const int *linkFilB,
                            int *activeLinkCount, int C, int nSeg);
@@ -283,6 +318,16 @@ typedef struct sbx_net_params {
     int seed;
     int rng;
     uint32_t stream;
+    /* crosslinker formation before the step's unbinding on every step that is a multiple of formInt
+     * (V2OneXHarness.java:379, formationCpu :444-464); the formation grid is FormationGrid's
+     * (FormationGrid.java:63-76), periodic with the network; formReqCap candidate slots */
+    int formOn;
+    int formInt;
+    float formParams[7];    /* CrosslinkerStore.setFormParams (CrosslinkerStore.java:196-206) */
+    int formMode;           /* 0 both orientations, 1 parallel, -1 antiparallel */
+    float formGridParams[6];
+    int formGridDims[4];
+    int formReqCap;
 } sbx_net_params;
 
 /* Myosin stage of the whole step (DenseContractileHarness.cpuStep, DenseContractileHarness.java:350-395):
@@ -386,6 +431,9 @@ int sbx_net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *x
 void sbx_net_destroy(sbx_net *net);
 /* advance nSteps steps; step counters step0 .. step0+nSteps-1 (counts[1] of the reference) */
 int sbx_net_step(sbx_net *net, int step0, int nSteps);
+/* formation counters: out[0] passes, [1] candidates of the last pass, [2] links formed by the last pass,
+ * [3] links formed in all; returns the number written (<= 4) */
+int sbx_net_form_stats(sbx_net *net, long long *out, int cap);
 /* make the caller's coord/uVec/yVec hold the current pose (the step ping-pongs pose buffers) */
 int sbx_net_publish(sbx_net *net);
 /* rebuild the crosslink CSR and recount the active links after the caller changed linkFilA/linkFilB or

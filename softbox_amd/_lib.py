@@ -73,6 +73,13 @@ class SbxNetParams(C.Structure):
         ("seed", i32),
         ("rng", i32),
         ("stream", u32),
+        ("formOn", i32),
+        ("formInt", i32),
+        ("formParams", C.c_float * 7),
+        ("formMode", i32),
+        ("formGridParams", C.c_float * 6),
+        ("formGridDims", C.c_int * 4),
+        ("formReqCap", i32),
     ]
 
 
@@ -149,6 +156,13 @@ _SIGS = {
     "sbx_csr_build": [vp, vp, i32, i32, vp, vp],
     "sbx_csr_build_noalloc": [vp, vp, i32, i32, vp, vp],
     "sbx_count_active_links": [vp, vp, vp, vp, vp, i32, i32],
+    "sbx_filid": [vp, vp, vp, vp, vp, i32],
+    "sbx_grid_form_candidates": [vp] + [vp] * 3 + [f32p, i32p] + [vp] * 6 + [f32p, vp, i32, i32, i32, f32p],
+    "sbx_form_gates": [vp] + [vp] * 11 + [f32p, i32p, i32, i32, i32, u32, i32, f32p],
+    "sbx_form_admit_reduce": [vp, vp, vp, vp, vp, i32, i32],
+    "sbx_form_admit": [vp] + [vp] * 13 + [f32p, i32, i32, i32],
+    "sbx_form_allocate": [vp] + [vp] * 20 + [i32, i32],
+    "sbx_net_form_stats": [vp, C.POINTER(C.c_longlong), i32],
     "sbx_link_unbind": [vp] + [vp] * 10 + [f32p, i32p, i32, i32, u32],
     "sbx_link_forces": [vp] + [vp] * 10 + [f32p, i32, i32],
     "sbx_link_torsion": [vp] + [vp] * 8 + [f32p, i32, i32],

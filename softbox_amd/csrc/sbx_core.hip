@@ -411,6 +411,13 @@ void launch_place(const SortScratch &s, const int *key, int nItems, int nKeys, c
                        s.longCount, s.longList);
 }
 
+int exclusive_scan(sbx_ctx *c, SortScratch &s, const int *in, int n, int *out, hipStream_t st) {
+    int r = sort_scratch_reserve(c, s, n, 0);
+    if (r) return r;
+    launch_scan(s, in, n, out, -1, nullptr, st ? st : c->stream);
+    return check_launch(c, "exclusive_scan");
+}
+
 int count_keys(sbx_ctx *c, const int *key, int nItems, int nKeys, int *counts, int *slot, hipStream_t st) {
     if (nItems <= 0) return SBX_OK;
     hipLaunchKernelGGL(k_key_count, dim3(blocks_for(nItems, 256)), dim3(256), 0, st ? st : c->stream, key, nItems,
@@ -501,6 +508,7 @@ void sbx_destroy(sbx_ctx *c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     sort_scratch_free(c->api_sort);
+    if (c->form_csr) hipFree(c->form_csr);
     if (c->owns_stream) {
         hipStreamSynchronize(c->stream);
         hipStreamDestroy(c->stream);

@@ -15,8 +15,8 @@ namespace sbx {
 // threads, LDS sort of up to LONG_LDS items), at most LONG_BLOCKS workgroups per sort.
 constexpr int LONG_SEG = 32;
 constexpr int LONG_LDS = 4096;
-constexpr int LONG_THREADS = 512;
-constexpr int LONG_BLOCKS = 256;
+constexpr int LONG_THREADS = 256;
+constexpr int LONG_BLOCKS = 64;
 constexpr long long long_list_len(long long nItems) { return nItems / (LONG_SEG + 1) + 1; }
 struct SortScratch {
     int *counts = nullptr, *tileSums = nullptr, *tilePrefix = nullptr, *tmp = nullptr, *slot = nullptr;
@@ -37,6 +37,8 @@ struct sbx_ctx {
     char err[512];
     int cu_count;
     sbx::SortScratch api_sort;  // scratch of the stand-alone sbx_grid_build / sbx_csr_build entry points
+    int *form_csr;              // sbx_form_admit's link CSRs (by linkFilA, by linkFilB)
+    long long form_csr_ints;
 };
 
 namespace sbx {
@@ -89,6 +91,8 @@ int counting_sort_core(sbx_ctx *c, SortScratch &s, const int *key, int nItems, i
 int noalloc_setup(sbx_ctx *c);
 // smallest scan tile (cells): sizes the tile-sum arrays of a sort
 constexpr int SCAN_MIN_TILE_KEYS = 1024;
+// out[0..n] = exclusive prefix sum of in[0..n) (out[n] = the total), with s's tile scratch
+int exclusive_scan(sbx_ctx *c, SortScratch &s, const int *in, int n, int *out, hipStream_t st = nullptr);
 // the histogram pass alone: slot[m] = atomicAdd(counts[key[m]], 1)
 int count_keys(sbx_ctx *c, const int *key, int nItems, int nKeys, int *counts, int *slot, hipStream_t st = nullptr);
 int fill_i32(sbx_ctx *c, int *dst, int value, long long count);

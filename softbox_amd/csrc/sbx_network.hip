@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "sbx_device.h"
+#include "sbx_form.h"
 #include "sbx_grid.h"
 #include "sbx_internal.h"
 #include "sbx_links.h"
@@ -145,6 +146,14 @@ struct sbx_net {
     int bpPending = 0, bpPendingBuf = 0;  // a broad phase of CSR[buf] still to enqueue
     int csrValid[2] = {0, 0};             // CSR[b] has been built
     int joinLive[2] = {0, 0}, placedLive[2] = {0, 0};  // evJoin / evPlaced recorded in the current capture
+    // crosslinker formation every formInt steps (formOn): its own sort scratch, buffers and counters
+    int formOn = 0;
+    FormPass form{};
+    SortScratch formSort;
+    int formFilIDValid = 0;     // filID follows the chain topology, which only the caller changes
+    int *formInts = nullptr;    // one allocation behind the pass's int buffers
+    float *formFloats = nullptr;
+    long long formPasses = 0;
 };
 
 namespace {
@@ -735,6 +744,12 @@ __global__ void __launch_bounds__(256) k_skin_assign(SkinArgs a) {
         for (int w = threadIdx.x; w < MLP_NPARAM; w += 256) ws[w] = a.model[w];
         __syncthreads();
         if (k >= *a.cCells) return;
+        const int p0 = a.cOff[k], p1 = a.cOff[k + 1];
+        if (p1 == p0) {  // an empty cell has no member to give a skin: no model evaluation
+            a.cellSkin[k] = 0.0f;
+            a.cellHorizon[k] = 0.0f;
+            return;
+        }
         float f[SKIN_FEATURES], y[MLP_OUT];
         float H = (float)(a.flags[SKF_SINCE] > 0 ? a.flags[SKF_SINCE] : 1);
         skin_cell_features(k, a.cOff, a.cCont, a.n, a.coord, a.refC, a.uVec, a.refU, a.links, H,
@@ -743,9 +758,8 @@ __global__ void __launch_bounds__(256) k_skin_assign(SkinArgs a) {
         float sk = fminf(a.skinMax, fmaxf(a.skinMin, a.margin * expf(y[0])));
         a.cellSkin[k] = sk;
         a.cellHorizon[k] = expf(y[1]);
-        int p0 = a.cOff[k], p1 = a.cOff[k + 1];
         for (int p = p0; p < p1; p++) a.skin[a.cCont[p]] = sk;
-        if (p1 > p0) atomicMax(&a.flags[SKF_MAXSKIN], __float_as_int(sk));  // positive floats order as ints
+        atomicMax(&a.flags[SKF_MAXSKIN], __float_as_int(sk));  // positive floats order as ints
     } else {
         if (k >= a.n) return;
         a.skin[k] = a.skinDefault;
@@ -1491,10 +1505,11 @@ void release(sbx_net *net) {
                     net->cgDev,    net->cCellsDev, net->bodyPack[0], net->bodyPack[1], net->sortedPose[0],
                     net->sortedPose[1], net->sortedCell[0], net->sortedCell[1], net->csrOff[1], net->csrCont[1],
                     net->gridCounts[0], net->gridCounts[1], net->gridSlot[0], net->gridSlot[1],
-                    net->reps,     net->rodRep,    net->linkRep};
+                    net->reps,     net->rodRep,    net->linkRep,  net->formInts, net->formFloats};
     for (void *b : bufs)
         if (b) hipFree(b);
     sort_scratch_free(net->sort);
+    sort_scratch_free(net->formSort);
     for (auto &pe : net->pending) {
         hipEventDestroy(pe.second.first);
         hipEventDestroy(pe.second.second);
@@ -1522,6 +1537,76 @@ struct BatchLayout {
 
 // A network over caller-owned stores; batch != nullptr: the stores are the concatenated replicas of
 // a batch (each replica's grid already checked), laid out as batch describes.
+// buffers of the formation pass (FormPass): one int and one float allocation carved into arrays
+int form_setup(sbx_net *net) {
+    sbx_ctx *c = net->ctx;
+    const sbx_net_params *p = &net->p;
+    const long long n = net->n, L = net->L, K = p->formReqCap, cells = p->formGridDims[3];
+    const long long ints = 2 * n + n + (cells + 1) + n + n + (n + 1) + 5 * K + (K + 1) + n + L + (L + 1) + L + 4;
+    net->formOn = 1;
+    int r = alloc(c, &net->formInts, (size_t)ints);
+    if (!r) r = alloc(c, &net->formFloats, (size_t)(2 * K));
+    if (r) return r;
+    FormPass &f = net->form;
+    int *q = net->formInts;
+    auto take = [&](long long count) {
+        int *b = q;
+        q += count;
+        return b;
+    };
+    f.filID = take(n); f.filIDScratch = take(n); f.bodyCell = take(n); f.gridOff = take(cells + 1);
+    f.gridCont = take(n); f.candCount = take(n); f.candBase = take(n + 1);
+    f.reqFilA = take(K); f.reqFilB = take(K); f.reqOrient = take(K); f.gatePass = take(K); f.acceptFlag = take(K);
+    f.rankOffsets = take(K + 1); f.minCand = take(n);
+    f.freeCount = take(L); f.freeOffsets = take(L + 1); f.freeList = take(L); f.stats = take(4);
+    f.reqLoc1 = net->formFloats;
+    f.reqLoc2 = net->formFloats + K;
+    if ((r = fill_i32(c, f.stats, 0, 4))) return r;
+    f.nSeg = net->n; f.C = net->L; f.reqCap = (int)K;
+    f.end1 = net->fil.end1; f.end2 = net->fil.end2; f.segLength = net->fil.segLength;
+    f.filState = nullptr;
+    f.end2NbrSlot = net->fil.end2NbrSlot;
+    f.linkFilA = net->xl.linkFilA; f.linkFilB = net->xl.linkFilB; f.linkState = net->xl.linkState;
+    f.strainPlace = net->xl.strainPlace; f.linkOrientSame = net->xl.linkOrientSame;
+    f.torqueMagPlace = net->xl.torqueMagPlace; f.activeLinkCount = net->xl.activeLinkCount;
+    f.loc1 = net->xl.loc1; f.loc2 = net->xl.loc2; f.strainHist = net->xl.strainHist;
+    f.torqueMagHist = net->xl.torqueMagHist;
+    f.offA = net->offA; f.idxA = net->idxA; f.offB = net->offB; f.idxB = net->idxB;
+    f.geom = grid_geom(p->formGridParams, p->formGridDims, p->periodic, p->periodic ? p->boxLen : nullptr);
+    f.fp = form_params(p->formParams, p->formMode);
+    f.rng = p->rng; f.seed = p->seed; f.stream = p->stream;
+    long long keys = cells;
+    for (long long v : {n + 1, K + 1, L + 1}) keys = keys > v ? keys : v;
+    return sort_scratch_reserve(c, net->formSort, keys, n > L ? n : L);
+}
+
+__global__ void k_form_tally(int *stats) { stats[2] += stats[1]; }
+
+// One formation pass before step `step` (V2OneXHarness.java:379), on the main stream after the previous
+// step's mechanics: it reads the pose, its derived ends and the start-of-pass link state, writes new
+// links into free slots, then rebuilds the link CSR; the next step recounts the active links.
+int run_formation(sbx_net *net, int step) {
+    sbx_ctx *c = net->ctx;
+    int r;
+    if (!net->countsValid) {  // start-of-pass active counts (countActiveLinks, V2OneXHarness.java:449)
+        hipLaunchKernelGGL(k_link_counts, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream, net->offA,
+                           net->idxA, net->offB, net->idxB, net->xl.linkState, net->xl.activeLinkCount, net->n);
+        if ((r = check_launch(c, "k_link_counts"))) return r;
+        net->countsValid = 1;
+    }
+    FormPass &f = net->form;
+    f.coord = net->coordB[net->cur];
+    f.uVec = net->uB[net->cur];
+    if ((r = form_pass(c, net->formSort, f, step, net->formFilIDValid != 0, c->stream))) return r;
+    net->formFilIDValid = 1;
+    hipLaunchKernelGGL(k_form_tally, dim3(1), dim3(1), 0, c->stream, f.stats);
+    if ((r = counting_sort(c, net->formSort, net->xl.linkFilA, net->L, net->n, net->L, net->offA, net->idxA))) return r;
+    if ((r = counting_sort(c, net->formSort, net->xl.linkFilB, net->L, net->n, net->L, net->offB, net->idxB))) return r;
+    net->countsValid = 0;
+    net->formPasses++;
+    return check_launch(c, "formation");
+}
+
 int net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *xl, const sbx_net_params *p,
                const BatchLayout *batch, sbx_net **out) {
     SBX_CHECK_CTX(c);
@@ -1549,6 +1634,18 @@ int net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *xl, c
         if (p->unbindOn) SBX_REQUIRE(c, p->offParams[4] != 0.0f, "net_create: offParams restLength must be non-zero");
     }
     if (p->confineOn) SBX_REQUIRE(c, p->boxParams[0] != 0.0f, "net_create: boxParams tau must be non-zero");
+    if (p->formOn) {
+        SBX_REQUIRE(c, !batch, "net_create: formation is not batched");
+        SBX_REQUIRE(c, L > 0, "net_create: formation needs a crosslink pool (nLinks > 0)");
+        SBX_REQUIRE(c, p->formInt > 0 && p->formReqCap > 0, "net_create: formInt and formReqCap must be > 0");
+        SBX_REQUIRE(c, fil->end2NbrSlot, "net_create: formation needs end2NbrSlot (filament identity)");
+        SBX_REQUIRE(c, xl->activeLinkCount, "net_create: formation needs activeLinkCount");
+        int r = check_grid(c, p->formGridParams, p->formGridDims, p->periodic);
+        if (r) return r;
+        if (p->periodic)
+            SBX_REQUIRE(c, p->formGridDims[0] >= 3 && p->formGridDims[1] >= 3 && p->formGridDims[2] >= 3,
+                        "net_create: a periodic formation grid needs >= 3 cells per axis");
+    }
     if (p->gridOn) {
         int r = batch ? SBX_OK : check_grid(c, p->gridParams, p->gridDims, p->periodic);
         if (r) return r;
@@ -1624,6 +1721,7 @@ int net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *xl, c
             if ((r = alloc(c, &net->idxB, L))) break;
             if ((r = build_link_csr(net))) break;
         }
+        if (p->formOn && (r = form_setup(net))) break;
         if (batch) {
             net->R = (int)batch->reps.size();
             if ((r = alloc(c, &net->reps, batch->reps.size()))) break;
@@ -1760,11 +1858,13 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
     return sbx_sync(c);
 }
 
-int sbx_net_step(sbx_net *net, int step0, int nSteps) {
-    if (!net) return SBX_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// steps step0 .. step0 + nSteps - 1 with no formation pass among them
+int step_range(sbx_net *net, int step0, int nSteps) {
     sbx_ctx *c = net->ctx;
-    SBX_CHECK_CTX(c);
-    SBX_REQUIRE(c, nSteps >= 0, "net_step: nSteps < 0");
     if (nSteps == 0) return SBX_OK;
     int r = set_device(c);
     if (r) return r;
@@ -1956,6 +2056,47 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
     if ((r = join_side(net))) return r;
     if (net->profiling) return drain_timers(net);
     return SBX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbx_net_step(sbx_net *net, int step0, int nSteps) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, nSteps >= 0, "net_step: nSteps < 0");
+    if (!net->formOn) return step_range(net, step0, nSteps);
+    SBX_REQUIRE(c, step0 >= 0, "net_step: formation steps count from 0 (step0 < 0)");
+    const int every = net->p.formInt, end = step0 + nSteps;
+    int r = set_device(c);
+    if (r) return r;
+    for (int t = step0; t < end;) {
+        if (t % every == 0 && (r = run_formation(net, t))) return r;
+        int next = (t / every + 1) * every;
+        if (next > end) next = end;
+        if ((r = step_range(net, t, next - t))) return r;
+        t = next;
+    }
+    return SBX_OK;
+}
+
+int sbx_net_form_stats(sbx_net *net, long long *out, int cap) {
+    if (!net || !out || cap < 0) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    int st[4] = {0, 0, 0, 0};
+    if (net->formOn) {
+        int r = check_hip(c, hipMemcpyAsync(st, net->form.stats, sizeof(st), hipMemcpyDeviceToHost, c->stream),
+                          "form stats");
+        if (!r) r = check_hip(c, hipStreamSynchronize(c->stream), "form stats sync");
+        if (r) return r;
+    }
+    long long v[4] = {net->formPasses, st[0], st[1], st[2]};
+    int k = cap < 4 ? cap : 4;
+    for (int i = 0; i < k; i++) out[i] = v[i];
+    return k;
 }
 
 int sbx_net_publish(sbx_net *net) {

@@ -53,6 +53,13 @@ class Network:
                  "skin_steps", "block_sorted")
         return {names[i]: int(out[i]) for i in range(min(k, len(names)))}
 
+    def form_stats(self) -> dict:
+        """Formation counters: passes, candidates and links formed by the last pass, links formed in all."""
+        out = (C.c_longlong * 4)()
+        k = self.ctx.check(self.ctx.lib.sbx_net_form_stats(self.h, out, 4))
+        names = ("passes", "candidates", "formed_last", "formed_total")
+        return {names[i]: int(out[i]) for i in range(k)}
+
     def skin_state(self) -> dict:
         ptrs = [C.c_void_p() for _ in range(5)]
         nc = C.c_int()
