@@ -187,6 +187,8 @@ def main() -> int:
     ap.add_argument("--box", type=float, default=20.0)
     ap.add_argument("--eta", type=float, default=0.1,
                     help="viscosity, Pa s (default: the reference's Constants.aeta = 0.1)")
+    ap.add_argument("--formation", type=int, default=100,
+                    help="crosslinker formation every this many steps (binding radius 40 nm; 0: off)")
     ap.add_argument("--tune-steps", type=int, default=8,
                     help="autotuner: steps timed per launch shape after the warm-up (0: default shape)")
     ap.add_argument("--shape", type=str, default="",
@@ -216,6 +218,8 @@ def main() -> int:
         return run_batch(args, log)
     t_build = time.perf_counter()
     sc = scenes.network(n_fil=args.n_fil, segs=10, box=args.box, n_links=args.n_links, seed=0, eta=args.eta)
+    if args.formation > 0:  # crosslinker formation at its cadence: the workload's 40 nm binding radius
+        sc.p.update(scenes.formation_params(args.box, 0.1, sc.fs.n, True, grab=0.04, interval=args.formation))
     n = sc.fs.n
     log(f"scene: {n} rods, {sc.xl.nLinks} links, grid {sc.p['gridDims'][:3]} ({time.perf_counter() - t_build:.1f} s)")
     ctx = Context(0)
@@ -270,9 +274,14 @@ def main() -> int:
         if mode == modes[args.skin][0]:
             start_state = (sc.copy(), t0)
             start_state[0].fs, start_state[0].xl = fs0, xl0
+        formed0 = net.form_stats().get("formed_total", 0) if args.formation > 0 else 0
         el = timed(net, ctx, t0, args.steps)
         r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "timed_steps": [t0, t0 + args.steps - 1],
              "links_alive_start": links_start}
+        if args.formation > 0:
+            fst = net.form_stats()
+            r["links_formed_timed"] = fst["formed_total"] - formed0
+            r["formation"] = fst
         if args.shape or (args.tune_steps and mode == "rebuild"):
             r["shape"] = shape_of(args, None if args.shape else tuned)
         r["digest"] = slices_digest(net, sc)
@@ -316,7 +325,7 @@ def main() -> int:
             dur_us, timing = head["broad_phase_back_to_back_us"], "HIP events around 20 back-to-back launches"
         achieved = b / (dur_us * 1e-6) / 1e9
         # the committed counter summaries are of the default workload (200k rods, 50k links, 20 um, water)
-        profiled = (args.n_fil, args.n_links, args.box, args.eta) == (20000, 50000, 20.0, 0.1)
+        profiled = (args.n_fil, args.n_links, args.box, args.eta, args.formation) == (20000, 50000, 20.0, 0.1, 100)
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": kernel_traffic(dom) if profiled else None, "bytes_per_launch": b,
@@ -339,8 +348,10 @@ def main() -> int:
         "warmup": args.warmup, "ms_per_step": round(head["ms_per_step"], 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 state, f64 force laws", "data": "synthetic (seeded scene)",
         "config": {"workload": f"crosslinked actin network: {args.n_fil // 1000}k filaments x 10 rods (0.1 um), "
-                               f"{args.n_links // 1000}k crosslinkers, {args.box:g} um periodic cube, dt 1e-5 s, "
-                               f"viscosity {args.eta:g} Pa s",
+                               f"{args.n_links // 1000}k crosslinker slots, {args.box:g} um periodic cube, dt 1e-5 s, "
+                               f"viscosity {args.eta:g} Pa s"
+                               + (f", formation every {args.formation} steps (binding radius 40 nm)"
+                                  if args.formation > 0 else ""),
                    "rods": n, "links": sc.xl.nLinks, "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
                    "broad_phase": bp},
         "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu1,

@@ -42,7 +42,7 @@ struct FormPass {
 int form_filid(sbx_ctx *c, const int *filState, const int *end2NbrSlot, int *filID, int *scratch, int n,
                hipStream_t q);
 int form_candidates(sbx_ctx *c, SortScratch &s, const float *coord, const float *segLength, const int *filID,
-                    const int *bodyCell, const int *gridOff, const int *gridCont, const GridGeom &g, double grab,
+                    const int *gridOff, const int *gridCont, const GridGeom &g, double grab,
                     int *candCount, int *candBase, int *reqFilA, int *reqFilB, int *total, int nSeg, int reqCap,
                     hipStream_t q);
 int form_gates(sbx_ctx *c, const float *coord, const float *uVec, const float *end1, const float *end2,

@@ -79,37 +79,32 @@ __device__ __forceinline__ bool form_pair(int n, double cix, double ciy, double 
     return !(cd > bound);
 }
 
-// The 27 cells of segment i's cell, (z, y, x) order, clamped (box) or wrapped (periodic); calls
-// visit(j) for every segment j > i of another filament within the capsule bound.
+// The 27 cells of segment i's cell (clamped in a box, wrapped when periodic) as the broad phase's walk
+// over the cell-sorted contents: all 36 range bounds are loaded at once and the partners are read in
+// batches, so a lane does not wait on one cell before the next.  Calls visit(j) for every segment j > i
+// of another filament within the capsule bound; the visiting order does not matter to either caller.
+constexpr int FORM_BATCH = 4;
 template <typename F>
 __device__ __forceinline__ void form_walk(int i, int n, const float *__restrict__ coord,
                                           const float *__restrict__ segLength, const int *__restrict__ filID,
                                           const int *__restrict__ off, const int *__restrict__ cont,
                                           const GridGeom &g, double grab, F visit) {
     const float fx = coord[i], fy = coord[n + i], fz = coord[2 * n + i];
-    const int3 cc = cell3_of(fx, fy, fz, g);
+    int pre[18], shift[18];
+    const int total = stencil_map(cell_of(fx, fy, fz, g), g, off, pre, shift);
     const double cix = fx, ciy = fy, ciz = fz, hi = 0.5 * (double)segLength[i];
     const int fi = filID[i];
-    int x0 = cc.x - 1, x1 = cc.x + 1, y0 = cc.y - 1, y1 = cc.y + 1, z0 = cc.z - 1, z1 = cc.z + 1;
-    if (!g.periodic) {
-        x0 = max(x0, 0); x1 = min(x1, g.nX - 1);
-        y0 = max(y0, 0); y1 = min(y1, g.nY - 1);
-        z0 = max(z0, 0); z1 = min(z1, g.nZ - 1);
-    }
-    for (int zz = z0; zz <= z1; zz++) {
-        const int wz = g.periodic ? (zz + g.nZ) % g.nZ : zz;
-        for (int yy = y0; yy <= y1; yy++) {
-            const int wy = g.periodic ? (yy + g.nY) % g.nY : yy;
-            for (int xx = x0; xx <= x1; xx++) {
-                const int wx = g.periodic ? (xx + g.nX) % g.nX : xx;
-                const int c = wx + wy * g.nX + wz * g.nX * g.nY;
-                const int e = off[c + 1];
-                for (int idx = off[c]; idx < e; idx++) {
-                    const int j = cont[idx];
-                    if (j <= i || filID[j] == fi) continue;
-                    if (form_pair(n, cix, ciy, ciz, hi, j, coord, segLength, grab, g)) visit(j);
-                }
-            }
+    for (int q = 0; q < total; q += FORM_BATCH) {
+        int jj[FORM_BATCH];
+#pragma unroll
+        for (int u = 0; u < FORM_BATCH; u++) jj[u] = (q + u < total) ? cont[stencil_at(pre, shift, q + u)] : -1;
+        int fj[FORM_BATCH];
+#pragma unroll
+        for (int u = 0; u < FORM_BATCH; u++) fj[u] = jj[u] > i ? filID[jj[u]] : fi;
+#pragma unroll
+        for (int u = 0; u < FORM_BATCH; u++) {
+            if (jj[u] <= i || fj[u] == fi) continue;
+            if (form_pair(n, cix, ciy, ciz, hi, jj[u], coord, segLength, grab, g)) visit(jj[u]);
         }
     }
 }
@@ -126,36 +121,73 @@ __global__ void __launch_bounds__(256) k_form_count(const float *__restrict__ co
     candCount[i] = cnt < FORM_MAXC ? cnt : FORM_MAXC;
 }
 
-// gridFormScan's total (:524-526) and gridFormEmit (:535-604): segment i inserts its partners ascending
-// into its own region, keeping the smallest `cap`
+// gridFormScan's total (:524-526) and gridFormEmit (:535-604): segment i writes its partners ascending into
+// its own region.  The 256 segments of a block own one contiguous span of the list: when the span fits in
+// LDS and no region is capped (FORM_MAXC) or clamped (reqCap), each segment appends its partners to its
+// part of the span in LDS, then the block places every entry at its rank within its segment's part (the
+// partners of a segment are distinct), spreading long regions over all threads.  Otherwise each segment
+// inserts into its region in place, keeping the smallest `cap`, as the reference does.
+constexpr int FORM_EMIT_LDS = 8192;
 __global__ void __launch_bounds__(256) k_form_emit(const float *__restrict__ coord, const float *__restrict__ segLength,
                                                    const int *__restrict__ filID, const int *__restrict__ off,
                                                    const int *__restrict__ cont, GridGeom g, double grab, int n,
                                                    const int *__restrict__ candBase, const int *__restrict__ candCount,
                                                    int reqCap, int *__restrict__ reqFilA, int *__restrict__ reqFilB,
                                                    int *__restrict__ total) {
-    int i = blockIdx.x * 256 + threadIdx.x;
+    __shared__ int buf[FORM_EMIT_LDS];
+    __shared__ unsigned char own[FORM_EMIT_LDS];
+    __shared__ int rLo[256], rLen[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
     if (i == 0) {
         int acc = candBase[n];
         *total = acc < reqCap ? acc : reqCap;
     }
-    if (i >= n) return;
-    const int base = candBase[i];
-    int cap = candCount[i];
-    if (base >= reqCap) cap = 0;
-    else if (base + cap > reqCap) cap = reqCap - base;
+    const int i0 = blockIdx.x * 256, i1 = min(i0 + 256, n);
+    const int spanLo = candBase[i0], spanHi = candBase[i1];
+    int base = 0, cnt = 0, cap = 0;
+    if (i < n) {
+        base = candBase[i];
+        cnt = candCount[i];
+        cap = cnt;
+        if (base >= reqCap) cap = 0;
+        else if (base + cap > reqCap) cap = reqCap - base;
+    }
+    const bool whole = cap == cnt && cnt < FORM_MAXC;
+    const bool fast = __syncthreads_and(whole) && spanHi - spanLo <= FORM_EMIT_LDS;
+    if (fast) {
+        const int lo = base - spanLo;
+        rLo[threadIdx.x] = lo;
+        rLen[threadIdx.x] = cap;
+        if (cap > 0) {
+            int k = lo;
+            form_walk(i, n, coord, segLength, filID, off, cont, g, grab, [&](int j) {
+                buf[k] = j;
+                own[k] = (unsigned char)threadIdx.x;
+                k++;
+            });
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < spanHi - spanLo; p += 256) {
+            const int o = own[p], rl = rLo[o], len = rLen[o], x = buf[p];
+            int rank = 0;
+            for (int q = rl; q < rl + len; q++) rank += buf[q] < x ? 1 : 0;
+            reqFilB[spanLo + rl + rank] = x;
+            reqFilA[spanLo + rl + rank] = i0 + o;
+        }
+        return;
+    }
     if (cap <= 0) return;
-    int cnt = 0;
+    int cntIns = 0;
     int *r = reqFilB;
     form_walk(i, n, coord, segLength, filID, off, cont, g, grab, [&](int j) {
-        if (cnt < cap) {
-            int p = base + cnt;
+        if (cntIns < cap) {
+            int p = base + cntIns;
             while (p > base && r[p - 1] > j) {
                 r[p] = r[p - 1];
                 p--;
             }
             r[p] = j;
-            cnt++;
+            cntIns++;
         } else if (j < r[base + cap - 1]) {
             int p = base + cap - 1;
             while (p > base && r[p - 1] > j) {
@@ -165,7 +197,7 @@ __global__ void __launch_bounds__(256) k_form_emit(const float *__restrict__ coo
             r[p] = j;
         }
     });
-    for (int k = 0; k < cap; k++) reqFilA[base + k] = i;
+    for (int q = 0; q < cap; q++) reqFilA[base + q] = i;
 }
 
 // the unused request tail keyed -1 (gridFormScan :527)
@@ -389,7 +421,7 @@ int form_filid(sbx_ctx *c, const int *filState, const int *end2NbrSlot, int *fil
 }
 
 int form_candidates(sbx_ctx *c, SortScratch &s, const float *coord, const float *segLength, const int *filID,
-                    const int *, const int *gridOff, const int *gridCont, const GridGeom &g, double grab,
+                    const int *gridOff, const int *gridCont, const GridGeom &g, double grab,
                     int *candCount, int *candBase, int *reqFilA, int *reqFilB, int *total, int nSeg, int reqCap,
                     hipStream_t q) {
     if (nSeg > 0) {
@@ -460,7 +492,7 @@ int form_pass(sbx_ctx *c, SortScratch &s, const FormPass &f, int step, bool filI
         if ((r = check_launch(c, "formation grid cells"))) return r;
     }
     if ((r = counting_sort(c, s, f.bodyCell, n, cells, n, f.gridOff, f.gridCont, nullptr, q))) return r;
-    if ((r = form_candidates(c, s, f.coord, f.segLength, f.filID, f.bodyCell, f.gridOff, f.gridCont, f.geom,
+    if ((r = form_candidates(c, s, f.coord, f.segLength, f.filID, f.gridOff, f.gridCont, f.geom,
                              (double)f.fp.grab, f.candCount, f.candBase, f.reqFilA, f.reqFilB, f.stats, n, f.reqCap,
                              q)))
         return r;
@@ -507,7 +539,7 @@ int sbx_grid_form_candidates(sbx_ctx *c, const float *filCoord, const float *fil
                                  candCountSeg && candBaseSeg && reqFilA && reqFilB),
                 "grid_form_candidates: null array");
     GridGeom g = grid_geom(gridParams, gridDims, periodic, boxLen);
-    return form_candidates(c, c->api_sort, filCoord, filSegLength, filID, nullptr, gridCellOffsets, gridCellContents,
+    return form_candidates(c, c->api_sort, filCoord, filSegLength, filID, gridCellOffsets, gridCellContents,
                            g, (double)formParams[6], candCountSeg, candBaseSeg, reqFilA, reqFilB, formCounts, nSeg,
                            reqCap, c->stream);
 }

@@ -49,22 +49,24 @@ __device__ __forceinline__ void mlp_forward(const float *__restrict__ w, const f
     float x[MLP_IN];
 #pragma unroll
     for (int k = 0; k < MLP_IN; k++) x[k] = mlp_standardise(x_raw[k], w[MLP_XMEAN + k], w[MLP_XSTD + k]);
-    float acc[MLP_H];
+    // layer 1 in registers, then one layer-2 unit at a time (each unit sums over k in ascending order,
+    // so the result does not depend on this loop order; 64 live accumulators would cap occupancy at 1)
+    float h1[MLP_H];
 #pragma unroll
-    for (int j = 0; j < MLP_H; j++) acc[j] = w[MLP_B2 + j];
     for (int k = 0; k < MLP_H; k++) {
         float a = w[MLP_B1 + k];
 #pragma unroll
         for (int i = 0; i < MLP_IN; i++) a += w[MLP_W1 + k * MLP_IN + i] * x[i];
-        float h = gelu(a);
-#pragma unroll
-        for (int j = 0; j < MLP_H; j++) acc[j] += w[MLP_W2 + j * MLP_H + k] * h;
+        h1[k] = gelu(a);
     }
 #pragma unroll
     for (int o = 0; o < MLP_OUT; o++) y[o] = w[MLP_B3 + o];
-#pragma unroll
+#pragma unroll 1
     for (int j = 0; j < MLP_H; j++) {
-        float h = gelu(acc[j]);
+        float a = w[MLP_B2 + j];
+#pragma unroll
+        for (int k = 0; k < MLP_H; k++) a += w[MLP_W2 + j * MLP_H + k] * h1[k];
+        float h = gelu(a);
 #pragma unroll
         for (int o = 0; o < MLP_OUT; o++) y[o] += w[MLP_W3 + o * MLP_H + j] * h;
     }

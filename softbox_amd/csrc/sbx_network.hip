@@ -720,7 +720,7 @@ struct SkinArgs {
     float *refC, *refU, *skin, *cellSkin, *cellHorizon;
     int *vlist, *vcount;
     int *flags;
-    const float *model;
+    const float *__restrict__ model;
     float skinDefault, skinMin, skinMax, margin;
     float maxR, cellFloor;   // largest bounding radius; smallest coarse cell (fine cell or skinMin reach)
     const int *bodyCell;     // fine cells of the current pose
@@ -735,14 +735,10 @@ constexpr int SKF_HEAVY = 7;           // flag word: bodies k_skin_pairs did blo
 // Skins of this rebuild.  Learned mode after the first rebuild: thread c is a coarse cell of the
 // last rebuild -- features over its members, model, clamp, then the skin of every member;
 // otherwise thread i is a body and gets the default skin.  Tracks the largest skin (flag word).
-__global__ void __launch_bounds__(256) k_skin_assign(SkinArgs a) {
+__global__ void __launch_bounds__(256) k_skin_assign(SkinArgs a, const float *__restrict__ model) {
     if (a.flags[a.gate] == 0) return;
-    __shared__ float ws[MLP_NPARAM];
     int k = blockIdx.x * 256 + threadIdx.x;
-    if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && a.model) {
-        if ((int)(blockIdx.x * 256) >= *a.cCells) return;  // block-uniform: no barrier below is skipped
-        for (int w = threadIdx.x; w < MLP_NPARAM; w += 256) ws[w] = a.model[w];
-        __syncthreads();
+    if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && model) {
         if (k >= *a.cCells) return;
         const int p0 = a.cOff[k], p1 = a.cOff[k + 1];
         if (p1 == p0) {  // an empty cell has no member to give a skin: no model evaluation
@@ -754,7 +750,7 @@ __global__ void __launch_bounds__(256) k_skin_assign(SkinArgs a) {
         float H = (float)(a.flags[SKF_SINCE] > 0 ? a.flags[SKF_SINCE] : 1);
         skin_cell_features(k, a.cOff, a.cCont, a.n, a.coord, a.refC, a.uVec, a.refU, a.links, H,
                            1.0f / a.cg->invCell, a.geom, f);
-        mlp_forward(ws, f, y);
+        mlp_forward(model, f, y);  // every lane reads the same weight: scalar loads, no LDS round trips
         float sk = fminf(a.skinMax, fmaxf(a.skinMin, a.margin * expf(y[0])));
         a.cellSkin[k] = sk;
         a.cellHorizon[k] = expf(y[1]);
@@ -1230,7 +1226,7 @@ int enqueue_skin(sbx_net *net, const float *coord, const float *uVec, hipStream_
     {
         Scope sc(net, T_SKIN_REBUILD);
         int span = net->cCellsMax > n ? net->cCellsMax : n;
-        hipLaunchKernelGGL(k_skin_assign, dim3(blocks_for(span, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_skin_assign, dim3(blocks_for(span, 256)), dim3(256), 0, s, a, a.model);
         hipLaunchKernelGGL(k_skin_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
         if ((r = check_launch(c, "skin assign/bin"))) return r;
         if ((r = counting_sort(c, net->sort, net->cCell, n, net->cCellsMax, n, net->cOff, net->cCont,
