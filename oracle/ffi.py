@@ -14,6 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_softbox.so")
 LIB_OMP_PATH = os.path.join(HERE, "liboracle_softbox_omp.so")   # same restatement, OpenMP per-item loops
+LIB_F64_PATH = os.path.join(HERE, "liboracle_softbox_f64.so")   # every float of the state widened to double
 
 vp = C.c_void_p
 i32 = C.c_int
@@ -21,7 +22,7 @@ u32 = C.c_uint32
 
 
 def build() -> str:
-    """Compile the oracle (and its OpenMP build) with its Makefile (gcc, -ffp-contract=off)."""
+    """Compile the oracle, its OpenMP build and its FP64 build with its Makefile (gcc, -ffp-contract=off)."""
     subprocess.run(["make", "-s", "-C", HERE, "-B", "all"], check=True)
     return LIB_PATH
 
@@ -66,8 +67,46 @@ class SboForm(C.Structure):
     )
 
 
+def _widen(fields):
+    """The same struct with every float field and float array widened to double (the FP64 build)."""
+    out = []
+    for name, t in fields:
+        if t is C.c_float:
+            t = C.c_double
+        elif getattr(t, "_type_", None) is C.c_float:
+            t = C.c_double * t._length_
+        out.append((name, t))
+    return out
+
+
+class SboNet64(C.Structure):
+    _fields_ = _widen(SboNet._fields_)
+
+
+class SboForm64(C.Structure):
+    _fields_ = _widen(SboForm._fields_)
+
+
+def _fill(arr, values) -> None:
+    """Copy parameters into a struct array; floats are rounded to float32 first, so that the FP32 and
+    FP64 runs start from the same parameter values."""
+    floating = arr._type_ in (C.c_float, C.c_double)
+    for i, v in enumerate(values):
+        arr[i] = float(np.float32(v)) if floating else v
+
+
+def _widen_store(obj) -> None:
+    """Every float32 array of a host store (and of the stores it holds) replaced by a float64 copy."""
+    for k, v in list(vars(obj).items()):
+        if isinstance(v, np.ndarray) and v.dtype == np.float32:
+            setattr(obj, k, v.astype(np.float64))
+        elif hasattr(v, "__dict__") and any(isinstance(x, np.ndarray) for x in vars(v).values()):
+            _widen_store(v)
+
+
 _lib = None
 _lib_omp = None
+_lib_f64 = None
 
 
 def load() -> C.CDLL:
@@ -87,6 +126,18 @@ def load_omp() -> C.CDLL:
             build()
         _lib_omp = _configure(C.CDLL(LIB_OMP_PATH))
     return _lib_omp
+
+
+def load_f64() -> C.CDLL:
+    """The FP64 build: the same restatement with float state widened to double (the `wide` oracles)."""
+    global _lib_f64
+    if _lib_f64 is None:
+        if not os.path.exists(LIB_F64_PATH):
+            build()
+        _lib_f64 = C.CDLL(LIB_F64_PATH)
+        _lib_f64.sbo_network_step.argtypes = [C.POINTER(SboNet64), i32]
+        _lib_f64.sbo_contractile_step.argtypes = [C.POINTER(SboNet64), C.POINTER(SboMotors64), i32]
+    return _lib_f64
 
 
 def _configure(lib: C.CDLL) -> C.CDLL:
@@ -140,11 +191,16 @@ def call(name: str, *args):
 class OracleNet:
     """sbo_network_step over host copies of a Scene (DenseContractileHarness.cpuStep filament block)."""
 
-    def __init__(self, scene, lib: C.CDLL | None = None):
+    def __init__(self, scene, lib: C.CDLL | None = None, wide: bool = False):
         fs, xl, p = scene.fs.copy(), (scene.xl.copy() if scene.xl is not None else None), scene.p
-        self.fs, self.xl, self.p = fs, xl, p
+        if wide:  # the FP64 run: float64 copies of every float array, the FP64 build of the oracle
+            _widen_store(fs)
+            if xl is not None:
+                _widen_store(xl)
+        self.fs, self.xl, self.p, self.wide = fs, xl, p, wide
+        self.fdt = np.float64 if wide else np.float32
         n = fs.n
-        s = SboNet()
+        s = SboNet64() if wide else SboNet()
         s.nRod = n
         for k in ("coord", "uVec", "yVec", "zVec", "end1", "end2", "segLength", "bTransGam", "bRotGam",
                   "forceSum", "torqueSum", "randForce", "randTorque", "brownTransScale", "brownRotScale",
@@ -152,13 +208,11 @@ class OracleNet:
             setattr(s, k, P(getattr(fs, k)).value)
         for k in ("params", "chainParams", "xlParams", "offParams", "torsionParams", "boxParams", "gridParams",
                   "boxLen", "gridDims"):
-            arr = getattr(s, k)
-            for i, v in enumerate(p[k]):
-                arr[i] = v
+            _fill(getattr(s, k), p[k])
         for k in ("chainOn", "unbindOn", "confineOn", "gridOn", "periodic", "maxCand", "seed", "rng"):
             setattr(s, k, int(p[k]))
         s.stream = int(p["stream"])
-        s.actinRadius = p["actinRadius"]
+        s.actinRadius = float(np.float32(p["actinRadius"]))
         self.keep = []
         if xl is not None:
             s.nLinks = xl.nLinks
@@ -172,7 +226,7 @@ class OracleNet:
         if p["gridOn"]:
             cells = int(p["gridDims"][3])
             mc = int(p["maxCand"])
-            self.grid = dict(center=np.zeros(3 * n, np.float32), boundingRadius=np.zeros(n, np.float32),
+            self.grid = dict(center=np.zeros(3 * n, self.fdt), boundingRadius=np.zeros(n, self.fdt),
                              bodyCell=np.zeros(n, np.int32), cellCount=np.zeros(cells, np.int32),
                              gridCellOffsets=np.zeros(cells + 1, np.int32), gridCellContents=np.zeros(n, np.int32),
                              candPartner=np.full(n * mc, -1, np.int32), candCount=np.zeros(n, np.int32))
@@ -180,11 +234,12 @@ class OracleNet:
                 setattr(s, k, P(a).value)
         self.form = None
         if p.get("formOn"):
-            self.form = OracleFormation(fs, xl, p)
+            self.form = OracleFormation(fs, xl, p, wide)
             s.form = C.cast(C.pointer(self.form.s), C.c_void_p).value
             s.formInt = int(p["formInt"])
         self.s = s
-        self.lib = lib if lib is not None else load()
+        assert not (wide and lib is not None), "the FP64 run uses the FP64 build"
+        self.lib = load_f64() if wide else (lib if lib is not None else load())
 
     def step(self, step0: int, n: int = 1) -> None:
         for t in range(step0, step0 + n):
@@ -213,25 +268,32 @@ class SboMotors(C.Structure):
     )
 
 
+class SboMotors64(C.Structure):
+    _fields_ = _widen(SboMotors._fields_)
+
+
 class OracleContractile(OracleNet):
     """sbo_contractile_step over host copies of a ContractileScene (DenseContractileHarness.cpuStep)."""
 
-    def __init__(self, scene, lib: C.CDLL | None = None):
-        super().__init__(scene, lib)
+    def __init__(self, scene, lib: C.CDLL | None = None, wide: bool = False):
+        super().__init__(scene, lib, wide)
         mot, dim, mini = scene.mot.copy(), scene.dim.copy(), scene.mini.copy()
+        if wide:
+            for st in (mot, dim, mini):
+                _widen_store(st)
         self.mot, self.dim, self.mini = mot, dim, mini
         n, nM = self.fs.n, mot.nM
         cap = n + nM
         cells = int(self.p["gridDims"][3])
         mc = int(self.p["maxCand"])
         # grid arrays cover filaments + motor heads
-        self.grid = dict(center=np.zeros(3 * cap, np.float32), boundingRadius=np.zeros(cap, np.float32),
+        self.grid = dict(center=np.zeros(3 * cap, self.fdt), boundingRadius=np.zeros(cap, self.fdt),
                          bodyCell=np.zeros(cap, np.int32), cellCount=np.zeros(cells, np.int32),
                          gridCellOffsets=np.zeros(cells + 1, np.int32), gridCellContents=np.zeros(cap, np.int32),
                          candPartner=np.full(1, -1, np.int32), candCount=np.zeros(1, np.int32))
         for k, a in self.grid.items():
             setattr(self.s, k, P(a).value)
-        m = SboMotors()
+        m = SboMotors64() if wide else SboMotors()
         m.nM = nM
         body = mot.body
         for k in ("coord", "uVec", "yVec", "zVec", "end1", "end2", "segLength", "bTransGam", "bRotGam", "forceSum",
@@ -242,9 +304,7 @@ class OracleContractile(OracleNet):
                   "forceDotPlace", "bindArc", "forceDotFil", "forceMag", "forceDotHist", "bondData"):
             setattr(m, k, P(getattr(mot, k)).value)
         for k in ("bodyParams", "jointParams", "kinParams", "nucParams", "xbParams"):
-            arr = getattr(m, k)
-            for i, v in enumerate(getattr(mot, k)):
-                arr[i] = v
+            _fill(getattr(m, k), getattr(mot, k))
         self.mreach = dict(reachSeg=np.full(nM * mc, -1, np.int32), reachCount=np.zeros(nM, np.int32),
                            segMotorCount=np.zeros(n, np.int32), segMotorOffsets=np.zeros(n + 1, np.int32),
                            segMotorMyo=np.zeros(max(1, nM), np.int32), ownerStore=np.zeros(cap, np.int32),
@@ -256,8 +316,7 @@ class OracleContractile(OracleNet):
         m.nD = dim.nD
         for k in ("motorA", "motorB", "parallel"):
             setattr(m, k, P(getattr(dim, k)).value)
-        for i, v in enumerate(dim.dimerParams):
-            m.dimerParams[i] = v
+        _fill(m.dimerParams, dim.dimerParams)
         m.nBb, m.nMD = mini.nBb, mini.nMD
         bb = mini.backbone
         for src, dst in (("coord", "bbCoord"), ("uVec", "bbUVec"), ("yVec", "bbYVec"), ("zVec", "bbZVec"),
@@ -271,14 +330,13 @@ class OracleContractile(OracleNet):
         m.tetherMotor = P(mini.motorA).value
         m.attachAxial = P(mini.attachAxial).value
         m.miniData = P(mini.miniData).value
-        for i, v in enumerate(mini.miniParams):
-            m.miniParams[i] = v
-        for i, v in enumerate(mini.bbBodyParams):
-            m.bbBodyParams[i] = v
+        _fill(m.miniParams, mini.miniParams)
+        _fill(m.bbBodyParams, mini.bbBodyParams)
         m.seedMotor = int(self.p["seedMotor"])
         m.seedBackbone = int(self.p["seedBackbone"])
         self.m = m
-        self.lib.sbo_contractile_step.argtypes = [C.POINTER(SboNet), C.POINTER(SboMotors), i32]
+        if not wide:
+            self.lib.sbo_contractile_step.argtypes = [C.POINTER(SboNet), C.POINTER(SboMotors), i32]
 
     def step(self, step0: int, n: int = 1) -> None:
         for t in range(step0, step0 + n):
@@ -289,9 +347,9 @@ class OracleFormation:
     """An sbo_form over a filament store and crosslinker store (host arrays, updated in place by
     sbo_formation): the formation grid, request arrays and allocator scratch it needs."""
 
-    def __init__(self, fs, xl, p):
+    def __init__(self, fs, xl, p, wide: bool = False):
         n, L = fs.n, xl.nLinks
-        f = SboForm()
+        f = SboForm64() if wide else SboForm()
         f.nSeg, f.C = n, L
         f.reqCap = int(p["formReqCap"])
         self.keep = []
@@ -302,15 +360,11 @@ class OracleFormation:
         for k in ("linkFilA", "linkFilB", "linkState", "strainPlace", "linkOrientSame", "torqueMagPlace",
                   "activeLinkCount", "loc1", "loc2", "strainHist", "torqueMagHist"):
             setattr(f, k, P(getattr(xl, k)).value)
-        for i, v in enumerate(p["formGridParams"]):
-            f.gridParams[i] = v
-        for i, v in enumerate(p["formGridDims"]):
-            f.gridDims[i] = v
+        _fill(f.gridParams, p["formGridParams"])
+        _fill(f.gridDims, p["formGridDims"])
         f.periodic = int(p["periodic"])
-        for i, v in enumerate(p["boxLen"]):
-            f.boxLen[i] = v
-        for i, v in enumerate(p["formParams"]):
-            f.formParams[i] = v
+        _fill(f.boxLen, p["boxLen"])
+        _fill(f.formParams, p["formParams"])
         f.mode = int(p["formMode"])
         cells = int(p["formGridDims"][3])
         rc = f.reqCap
@@ -319,8 +373,8 @@ class OracleFormation:
                      gatePass=rc, minCand=n, acceptFlag=rc, freeCount=max(1, L), freeOffsets=L + 1,
                      freeList=max(1, L), rankOffsets=rc + 1)
         self.a = {k: np.zeros(v, np.int32) for k, v in sizes.items()}
-        self.a.update(center=np.zeros(3 * n, np.float32), reqLoc1=np.zeros(rc, np.float32),
-                      reqLoc2=np.zeros(rc, np.float32))
+        fdt = np.float64 if wide else np.float32
+        self.a.update(center=np.zeros(3 * n, fdt), reqLoc1=np.zeros(rc, fdt), reqLoc2=np.zeros(rc, fdt))
         for k, arr in self.a.items():
             setattr(f, k, P(arr).value)
         self.s = f
@@ -328,4 +382,3 @@ class OracleFormation:
     @property
     def n_candidates(self) -> int:
         return int(self.s.formCounts[0])
-
