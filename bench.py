@@ -185,8 +185,12 @@ def main() -> int:
     ap.add_argument("--n-fil", type=int, default=20000)
     ap.add_argument("--n-links", type=int, default=50000)
     ap.add_argument("--box", type=float, default=20.0)
-    ap.add_argument("--eta", type=float, default=0.1,
-                    help="viscosity, Pa s (default: the reference's Constants.aeta = 0.1)")
+    ap.add_argument("--workload", choices=("network", "actomyosin"), default="network",
+                    help="network: the crosslinked network (the headline); actomyosin: the full actomyosin "
+                         "network (50k filaments, 2k minifilaments of 16 heads, 100k crosslinkers, 30 um box)")
+    ap.add_argument("--eta", type=float, default=None,
+                    help="viscosity, Pa s (default: the reference's Constants.aeta = 0.1 for the network, the "
+                         "dense actomyosin fixture's 1.0 for the actomyosin network)")
     ap.add_argument("--formation", type=int, default=100,
                     help="crosslinker formation every this many steps (binding radius 40 nm; 0: off)")
     ap.add_argument("--tune-steps", type=int, default=8,
@@ -214,8 +218,12 @@ def main() -> int:
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+    if args.eta is None:
+        args.eta = 1.0 if args.workload == "actomyosin" else 0.1
     if args.batch:
         return run_batch(args, log)
+    if args.workload == "actomyosin":
+        return run_actomyosin(args, log)
     t_build = time.perf_counter()
     sc = scenes.network(n_fil=args.n_fil, segs=10, box=args.box, n_links=args.n_links, seed=0, eta=args.eta)
     if args.formation > 0:  # crosslinker formation at its cadence: the workload's 40 nm binding radius
@@ -364,6 +372,127 @@ def main() -> int:
                    for m, r in runs.items()}
     if train_info:
         out["skin_training"] = train_info
+    print(json.dumps(out))
+    ctx.close()
+    return 0
+
+
+def run_cpu_baseline_actomyosin(sc, step0: int, budget_s: float = 12.0, openmp: bool = False) -> dict:
+    """The oracle's actomyosin step (DenseContractileHarness.cpuStep order) from the GPU's state at the
+    start of its timed region, over the same step numbers, for about budget_s seconds."""
+    from oracle import ffi
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)) if openmp else 1
+    orc = ffi.OracleContractile(sc, ffi.load_omp() if openmp else None)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        orc.step(step0 + steps, 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 200:
+            break
+    form = f"OpenMP build, {threads} threads" if openmp else "single thread"
+    return {"value": sc.fs.n * steps / el, "unit": "rod-steps/s (filament rods)", "cores": threads, "kind": "port",
+            "sample": f"oracle C restatement ({form}), the GPU's state at the start of its timed region "
+                      f"({sc.fs.n} filament rods, {sc.mot.nM} motor heads, {sc.xl.nLinks} link slots), steps "
+                      f"{step0}..{step0 + steps - 1}, full actomyosin step incl. grid: {steps} steps in {el:.1f} s"}
+
+
+def run_actomyosin(args, log) -> int:
+    """The full actomyosin network: 50k filaments x 10 rods (0.1 um) in the reference's clamped 30 um box
+    with containment walls, 2k minifilaments (4 dimers each end -> 16 heads, stall 4 pN), 100k
+    crosslinkers; motor reach grid + binding + catch-slip + cycle + bond forces + joints + dimer
+    coupling + tethers + backbones on top of the network step (DenseContractileHarness.cpuStep order).
+    Viscosity 1 Pa s, the dense fixture's crowded cytoplasm (DenseContractileHarness.java:58): at water
+    viscosity the placed heads and links fling segments micrometres per step."""
+    import torch  # noqa: F401  (device runtime first, as in the network workload)
+
+    from softbox_amd import scenes
+    from softbox_amd.context import Context
+    from softbox_amd.network import Network
+
+    n_fil = args.n_fil if args.n_fil != 20000 else 50000
+    n_links = args.n_links if args.n_links != 50000 else 100000
+    box = args.box if args.box != 20.0 else 30.0
+    t_build = time.perf_counter()
+    sc = scenes.contractile(n_fil=n_fil, segs=10, box=box, n_mini=max(1, n_fil // 25), n_links=n_links, seed=0,
+                            eta=args.eta)
+    if args.formation > 0:
+        sc.p.update(scenes.formation_params(box, 0.1, sc.fs.n, False, grab=0.04, interval=args.formation))
+    n = sc.fs.n
+    log(f"actomyosin scene: {n} filament rods, {sc.mot.nM} heads, {sc.mini.nBb} backbones, {sc.xl.nLinks} links "
+        f"({time.perf_counter() - t_build:.1f} s)")
+    ctx = Context(0)
+    net = Network(ctx, sc.copy(), keep_forces=False)
+    net.set_overlap(bool(args.overlap))
+    if args.graph:
+        net.use_graph(True)
+    net.step(0, args.warmup)
+    t0 = args.warmup
+    tuned = None
+    if args.shape:
+        net.set_shape(*[int(v) for v in args.shape.split(",")])
+    elif args.tune_steps:
+        tuned = net.tune(t0, args.tune_steps)
+        t0 += tuned["steps"]
+        log(f"tuned launch shape {tuned}")
+    start = sc.copy()
+    start.mot, start.dim, start.mini = sc.mot.copy(), sc.dim.copy(), sc.mini.copy()
+    net.pull(start.fs, start.xl)
+    net.pull_motors(start.mot, start.mini)
+    links_start = int((start.xl.linkState[: start.xl.nLinks] >= 0).sum())
+    bound_start = int((start.mot.boundSeg >= 0).sum())
+    formed0 = net.form_stats().get("formed_total", 0) if args.formation > 0 else 0
+    el = timed(net, ctx, t0, args.steps)
+    r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "timed_steps": [t0, t0 + args.steps - 1]}
+    if args.formation > 0:
+        r["links_formed_timed"] = net.form_stats()["formed_total"] - formed0
+    kernels = profile(net, t0 + args.steps, args.profile_steps) if args.profile_steps else {}
+    end = sc.copy()
+    end.mot, end.mini = sc.mot.copy(), sc.mini.copy()
+    net.pull(end.fs, end.xl)
+    net.pull_motors(end.mot, end.mini)
+    assert np.isfinite(end.fs.coord).all() and np.isfinite(end.mot.body.coord).all(), "non-finite pose"
+    net.close()
+    roof = None
+    if kernels:
+        known = {k: kernel_bytes(sc, k) for k in kernels}
+        dom = max((k for k in kernels if known[k]), key=lambda k: kernels[k]["avg_us"])
+        b = known[dom]
+        achieved = b / (kernels[dom]["avg_us"] * 1e-6) / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": b,
+                "duration_us": round(kernels[dom]["avg_us"], 2),
+                "timing": f"HIP events around each launch of {args.profile_steps} eager steps"}
+        for k, v in kernels.items():
+            if known[k]:
+                v["GBs"] = round(known[k] / (v["avg_us"] * 1e-6) / 1e9, 1)
+    cpu = cpu1 = None
+    if not args.no_cpu:
+        cpu = run_cpu_baseline_actomyosin(start, t0, openmp=True)
+        cpu1 = run_cpu_baseline_actomyosin(start, t0, openmp=False)
+    out = {
+        "metric": "rod-steps/sec (filament rods) of the actomyosin step at 1 GPU", "value": round(r["value"], 1),
+        "unit": "rod-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 state, f64 force laws", "data": "synthetic (seeded scene)",
+        "config": {"workload": f"full actomyosin network: {n_fil // 1000}k filaments x 10 rods (0.1 um), "
+                               f"{sc.mini.nBb} minifilaments x 16 heads (stall 4 pN), {n_links // 1000}k crosslinker "
+                               f"slots, {box:g} um clamped box with walls, dt 1e-5 s, viscosity {args.eta:g} Pa s"
+                               + (f", formation every {args.formation} steps" if args.formation > 0 else ""),
+                   "rods": n, "motor_heads": sc.mot.nM, "backbones": sc.mini.nBb, "links": sc.xl.nLinks,
+                   "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
+                   "shape": shape_of(args, tuned) if (args.shape or tuned) else None},
+        "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu1,
+        "links_alive_start": links_start, "links_alive_end": int((end.xl.linkState[: end.xl.nLinks] >= 0).sum()),
+        "heads_bound_start": bound_start, "heads_bound_end": int((end.mot.boundSeg >= 0).sum()),
+        "timed_steps": r["timed_steps"],
+    }
+    if "links_formed_timed" in r:
+        out["links_formed_timed"] = r["links_formed_timed"]
+    if kernels:
+        out["kernels"] = {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                          for k, v in kernels.items()}
     print(json.dumps(out))
     ctx.close()
     return 0

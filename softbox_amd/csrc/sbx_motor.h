@@ -129,6 +129,15 @@ __device__ __forceinline__ void catch_slip_one(int m, int &bs, int &cool, int *_
     }
 }
 
+// The arc position of the head's foot along the segment (BindingDetectionSystem.java:380-384)
+__device__ __forceinline__ float bind_arc(float mx, float my, float mz, float e1x, float e1y, float e1z, float e2x,
+                                          float e2y, float e2z) {
+    float r1x = e2x - e1x, r1y = e2y - e1y, r1z = e2z - e1z;
+    float denom = r1x * r1x + r1y * r1y + r1z * r1z;
+    float numer = (mx - e1x) * r1x + (my - e1y) * r1y + (mz - e1z) * r1z;
+    return numer / (float)sqrt((double)denom);
+}
+
 // BindingDetectionSystem.bindNearest body (BindingDetectionSystem.java:367-387): strict < keeps the
 // first candidate in list order on ties.
 __device__ __forceinline__ void bind_nearest_one(int m, int nM, int nSeg, const float *__restrict__ head,
@@ -151,10 +160,7 @@ __device__ __forceinline__ void bind_nearest_one(int m, int nM, int nSeg, const 
         if (d >= 0.0f && d < bestD) {
             bestD = d;
             bestSeg = s;
-            float r1x = e2x - e1x, r1y = e2y - e1y, r1z = e2z - e1z;
-            float denom = r1x * r1x + r1y * r1y + r1z * r1z;
-            float numer = (mx - e1x) * r1x + (my - e1y) * r1y + (mz - e1z) * r1z;
-            bestArc = numer / (float)sqrt((double)denom);
+            bestArc = bind_arc(mx, my, mz, e1x, e1y, e1z, e2x, e2y, e2z);
         }
     }
     if (bestSeg >= 0) {

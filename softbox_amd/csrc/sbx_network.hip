@@ -498,24 +498,89 @@ __global__ void __launch_bounds__(256) k_pub_heads(sbx_motors m) {
 }
 
 // gridReachable + catchSlipRelease + bindNearest + cycle for one motor (each reads only its own kinetic
-// state and the step-start filament geometry, so the per-motor fusion equals the reference's four passes)
+// state and the step-start filament geometry, so the per-motor fusion equals the reference's four passes).
+// A group of MB_GROUP lanes takes one motor: its reach stencil (BindingDetectionSystem.gridReachable,
+// BindingDetectionSystem.java:157-227: the 3x3x3 cells around the head's cell, clamped at the grid's
+// faces, (z, y, x) order, ascending index within a cell) is mapped to contiguous positions of the
+// cell-sorted contents, and the lanes take consecutive positions.  A hit's place in reachSeg is the
+// group's count so far plus the hits at earlier positions of the round (ballot), so the list is the
+// reference's in its order; bindNearest (:367-387) becomes an argmin over (distance, list place) of the
+// first maxCand hits, which is the serial scan's strict-< winner.
+constexpr int MB_GROUP = 16;
+constexpr int MB_ROUND = 2;
+
 __global__ void __launch_bounds__(256) k_motor_bind(MotorArgs a) {
-    int k = blockIdx.x * 256 + threadIdx.x;
+    constexpr int G = MB_GROUP, U = MB_ROUND;
+    const int lane = threadIdx.x & 63, gl = lane % G, grp = lane / G;
+    const int k = (blockIdx.x * 256 + threadIdx.x) / G;
     const sbx_motors &m = a.m;
-    if (k >= m.nM) return;
-    int step = *a.dStep + a.stepOff;
-    int cnt = grid_reachable_one(k, m.nM, a.nSeg, m.head, m.hUVec, m.rodUVec, a.e1, a.e2, a.rg, a.off, a.cont,
-                                 a.ownerStore, a.ownerSlot, m.reachSeg, a.maxCand, a.kp.myoColTol, a.kp.alignTol);
+    if (k >= m.nM) return;  // whole groups leave together (k is uniform in a group)
+    const int nM = m.nM, nSeg = a.nSeg;
+    const float mx = m.head[k], my = m.head[nM + k], mz = m.head[2 * nM + k];
+    const float mux = m.hUVec[k], muy = m.hUVec[nM + k], muz = m.hUVec[2 * nM + k];
+    const float rux = m.rodUVec[k], ruy = m.rodUVec[nM + k], ruz = m.rodUVec[2 * nM + k];
+    const ReachGrid &rg = a.rg;
+    const int cx = clamp_bin((mx - rg.xMin) * rg.invCell, rg.nX);
+    const int cy = clamp_bin((my - rg.yMin) * rg.invCell, rg.nY);
+    const int cz = clamp_bin((mz - rg.zMin) * rg.invCell, rg.nZ);
+    GridGeom g{};
+    g.nX = rg.nX; g.nY = rg.nY; g.nZ = rg.nZ;
+    g.periodic = 0;  // the reach stencil is clamped at the faces, periodic or not
+    int pre[18], shift[18];
+    const int total = stencil_map(cx + cy * rg.nX + cz * rg.nX * rg.nY, g, a.off, pre, shift);
+    const unsigned long long groupBits = ((1ull << G) - 1) << (grp * G);
+    const unsigned long long below = (1ull << lane) - 1;
+    int *slice = m.reachSeg + (size_t)k * a.maxCand;
+    const float tol = a.kp.myoColTol, align = a.kp.alignTol;
+    int cnt = 0, bestPos = 0x7fffffff, bestSeg = -1;
+    float bestD = 1.0e30f;
+    for (int base = 0; base < total; base += G * U) {
+        int seg[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int q = base + u * G + gl;
+            const int j = q < total ? a.cont[stencil_at9(pre, shift, q)] : -1;
+            seg[u] = (j >= 0 && a.ownerStore[j] == 0) ? a.ownerSlot[j] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            float d = -1.0f;
+            const int sg = seg[u];
+            if (sg >= 0)
+                d = reach_test(mx, my, mz, mux, muy, muz, rux, ruy, ruz, a.e1[sg], a.e1[nSeg + sg],
+                               a.e1[2 * nSeg + sg], a.e2[sg], a.e2[nSeg + sg], a.e2[2 * nSeg + sg], tol, align);
+            const bool hit = d >= 0.0f;
+            const unsigned long long bits = __ballot(hit) & groupBits;
+            if (hit) {
+                const int slot = cnt + __popcll(bits & below);
+                if (slot < a.maxCand) {
+                    slice[slot] = sg;
+                    if (d < bestD) { bestD = d; bestPos = slot; bestSeg = sg; }
+                }
+            }
+            cnt += __popcll(bits);
+        }
+    }
+#pragma unroll
+    for (int o = G / 2; o >= 1; o >>= 1) {
+        const float od = __shfl_xor(bestD, o, 64);
+        const int op = __shfl_xor(bestPos, o, 64), os = __shfl_xor(bestSeg, o, 64);
+        if (od < bestD || (od == bestD && op < bestPos)) { bestD = od; bestPos = op; bestSeg = os; }
+    }
+    if (gl != 0) return;
+    const int step = *a.dStep + a.stepOff;
     m.reachCount[k] = cnt;
     int bs = m.boundSeg[k], cool = m.cooldown[k];
     catch_slip_one(k, bs, cool, m.stats, m.capStats, m.forceDotFil[k], m.forceMag[k], a.kp, step, a.seed, a.rng,
                    a.stream);
-    float arc = m.bindArc[k];
-    bind_nearest_one(k, m.nM, a.nSeg, m.head, m.hUVec, m.rodUVec, a.e1, a.e2, m.reachSeg, cnt, a.maxCand, bs, arc,
-                     a.kp.myoColTol, a.kp.alignTol);
+    if (bs == FREE_BINDABLE && bestSeg >= 0) {
+        const int sg = bestSeg;
+        m.bindArc[k] = bind_arc(mx, my, mz, a.e1[sg], a.e1[nSeg + sg], a.e1[2 * nSeg + sg], a.e2[sg],
+                                a.e2[nSeg + sg], a.e2[2 * nSeg + sg]);
+        bs = sg;
+    }
     m.boundSeg[k] = bs;
     m.cooldown[k] = cool;
-    m.bindArc[k] = arc;
     m.nucleotideState[k] = nucleotide_one(k, m.nucleotideState[k], bs >= 0, m.forceDotHist, a.np, step, a.seed,
                                           a.rng, a.stream);
 }
@@ -1149,7 +1214,7 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
         a.rg = net->rg; a.kp = net->kp; a.np = net->np;
         a.maxCand = net->p.maxCand; a.nSeg = n; a.seed = net->mp.seedMotor; a.rng = net->p.rng;
         a.stream = net->p.stream; a.dStep = net->dStep; a.stepOff = stepOff;
-        hipLaunchKernelGGL(k_motor_bind, dim3(blocks_for(nM, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_motor_bind, dim3(blocks_for(nM, 256 / MB_GROUP)), dim3(256), 0, s, a);
     }
     {
         Scope sc(net, T_MJOINT);
