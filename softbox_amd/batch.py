@@ -76,16 +76,18 @@ class Batch:
             self.xl[r].pull(xl)
         return fs, xl
 
-    def candidates(self, r: int):
+    def candidates(self, r: int, rows: tuple[int, int] | None = None):
         """Replica r's (candPartner, candCount, gridCellOffsets, gridCellContents, bodyCell) in its own
-        rod and cell numbering (the batch's arrays, rebased)."""
+        rod and cell numbering (the batch's arrays, rebased); rows=(lo, hi) limits candPartner to the
+        slices of the replica's bodies lo..hi-1."""
         sc = self.scenes[r]
         rb, _, cb = self.layout[r]
         n, mc, cells = sc.fs.n, int(sc.p["maxCand"]), int(sc.p["gridDims"][3])
-        cand, cnt, off, cont, cell = self.net.candidates(self.n, mc)
-        cand = cand.reshape(self.n, mc)[rb:rb + n].copy()
+        lo, hi = rows if rows is not None else (0, n)
+        cand, cnt, off, cont, cell = self.net.candidates(self.n, mc, rows=(rb + lo, rb + hi))
+        cand = cand.reshape(hi - lo, mc)
         cnt = cnt[rb:rb + n].copy()
-        used = np.arange(mc)[None, :] < np.minimum(cnt, mc)[:, None]
+        used = np.arange(mc)[None, :] < np.minimum(cnt[lo:hi], mc)[:, None]
         cand[used] -= rb
         return (cand.ravel(), cnt, off[cb:cb + cells + 1] - rb, cont[rb:rb + n] - rb, cell[rb:rb + n] - cb)
 

@@ -121,22 +121,24 @@ class Network:
         self.ctx.check(self.ctx.lib.sbx_net_time_broad_phase(self.h, reps, C.byref(us)))
         return us.value
 
-    def candidates(self, n: int, max_cand: int):
-        """Host copies of (candPartner[n*maxCand], candCount[n], offsets, contents, bodyCell)."""
-        import torch
-
+    def candidates(self, n: int, max_cand: int, rows: tuple[int, int] | None = None):
+        """Host copies of (candPartner[n*maxCand], candCount[n], offsets, contents, bodyCell); with
+        rows=(lo, hi) only the candidate slices of bodies lo..hi-1 are copied (candPartner[(hi-lo)*maxCand])."""
         ptrs = [C.c_void_p() for _ in range(5)]
         self.ctx.check(self.ctx.lib.sbx_net_candidates(self.h, *[C.byref(p) for p in ptrs]))
         self.ctx.sync()
         cells = int(self._p.gridDims[3])
-        sizes = [n * max_cand, n, cells + 1, n, n]
+        lo, hi = rows if rows is not None else (0, n)
+        sizes = [(hi - lo) * max_cand, n, cells + 1, n, n]
+        skip = [lo * max_cand * 4, 0, 0, 0, 0]
         out = []
-        for p, sz in zip(ptrs, sizes):
+        for p, sz, sk in zip(ptrs, sizes, skip):
             if not p.value:
                 out.append(None)
                 continue
             a = np.empty(sz, np.int32)
-            self.ctx.check(self.ctx.lib.sbx_pull(self.ctx.h, a.ctypes.data_as(C.c_void_p), p, a.nbytes))
+            self.ctx.check(self.ctx.lib.sbx_pull(self.ctx.h, a.ctypes.data_as(C.c_void_p), C.c_void_p(p.value + sk),
+                                                 a.nbytes))
             out.append(a)
         return out
 

@@ -36,7 +36,7 @@ def _replicas(seed=3, n_rep=4):
     return reps
 
 
-def _alone(ctx, sc, steps, graph=False, overlap=True):
+def _alone(ctx, sc, steps, graph=False, overlap=True, rows=None):
     from softbox_amd.network import Network
 
     net = Network(ctx, sc.copy())
@@ -47,7 +47,7 @@ def _alone(ctx, sc, steps, graph=False, overlap=True):
     fs = sc.fs.copy()
     xl = sc.xl.copy() if sc.xl is not None else None
     net.pull(fs, xl)
-    cand = net.candidates(sc.fs.n, int(sc.p["maxCand"]))
+    cand = net.candidates(sc.fs.n, int(sc.p["maxCand"]), rows=rows)
     net.close()
     return fs, xl, cand
 
@@ -138,4 +138,51 @@ def test_launch_shapes_and_tuner_change_only_time(ctx):
     t += 2
     for r, sc in enumerate(reps):
         _check_replica(bt, r, sc, _alone(ctx, sc, t), f"shapes + tuner ({t} steps)")
+    bt.close()
+
+
+def test_ensemble_64_replicas_at_stated_size(ctx):
+    """The batched ensemble as stated: 64 replicas of the crosslinked-network recipe, filament counts
+    log-uniform in [2k, 40k], actin 5-30 uM (9.02M rods, 2.26M links), advanced together for 10 steps
+    (captured two-step graphs, overlapped streams, the autotuned launch shape).  Every replica's pose,
+    forces, link lifecycle, grid CSR and candidate counts equal stepping it alone, as do the candidate
+    slices of its first 2048 bodies; the smallest replica also equals the oracle."""
+    from softbox_amd.batch import Batch
+
+    reps = scenes.ensemble(64, seed=0, eta=0.1)
+    assert len(reps) == 64 and sum(sc.fs.n for sc in reps) > 8_000_000
+    bt = Batch(ctx, [sc.copy() for sc in reps])
+    bt.net.use_graph(2)
+    tuned = bt.net.tune(0, trial_steps=1)
+    t = tuned["steps"]
+    bt.step(t, 10)
+    t += 10
+    rows = 2048
+    for r, sc in enumerate(reps):
+        fs_w, xl_w, cand_w = _alone(ctx, sc, t, graph=True, overlap=True, rows=(0, rows))
+        fs, xl = sc.fs.copy(), sc.xl.copy()
+        bt.pull(r, fs, xl)
+        what = f"replica {r} ({sc.fs.n} rods)"
+        for k in STATE:
+            assert_same(getattr(fs, k), getattr(fs_w, k), f"{what}: {k}")
+        for k in LINKS:
+            assert_same(getattr(xl, k), getattr(xl_w, k), f"{what}: {k}")
+        cand, cnt, off, cont, cell = bt.candidates(r, rows=(0, rows))
+        wc, wn, woff, wcont, wcell = cand_w
+        mc = int(sc.p["maxCand"])
+        assert_same(cnt, wn, f"{what}: candCount")
+        used = (np.arange(mc)[None, :] < np.minimum(wn[:rows], mc)[:, None]).ravel()
+        assert_same(cand[used], wc[used], f"{what}: candPartner of the first {rows} bodies")
+        assert_same(off, woff, f"{what}: gridCellOffsets")
+        assert_same(cont, wcont, f"{what}: gridCellContents")
+        assert_same(cell, wcell, f"{what}: bodyCell")
+    small = int(np.argmin([sc.fs.n for sc in reps]))
+    orc = ffi.OracleNet(reps[small].copy())
+    orc.step(0, t)
+    fs, xl = reps[small].fs.copy(), reps[small].xl.copy()
+    bt.pull(small, fs, xl)
+    for k in STATE:
+        assert_same(getattr(fs, k), getattr(orc.fs, k), f"oracle, replica {small}: {k}")
+    for k in LINKS:
+        assert_same(getattr(xl, k), getattr(orc.xl, k), f"oracle, replica {small}: {k}")
     bt.close()
