@@ -199,6 +199,9 @@ def main() -> int:
                     help="bp_group,scan_items,rod_threads[,bp_staged]: fixed launch shape instead of the autotuner")
     ap.add_argument("--batch", type=int, default=0,
                     help="N > 0: the batched ensemble instead (N replicas, 2k-40k filaments log-uniform, 5-30 uM)")
+    ap.add_argument("--cell-factors", type=str, default="1,1.5,2,3",
+                    help="batched ensemble: grid cell multiples the cell-size autotuner times ('' or '1': the "
+                         "reference grid)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="before the warm-up, one device-to-device copy of this many MiB (a plain copy kernel "
                          "of known size that calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -505,7 +508,7 @@ def run_batch(args, log) -> int:
     import torch
 
     from softbox_amd import scenes
-    from softbox_amd.batch import Batch
+    from softbox_amd.batch import Batch, tune_cell_size
     from softbox_amd.context import Context
 
     t_build = time.perf_counter()
@@ -515,6 +518,13 @@ def run_batch(args, log) -> int:
     log(f"ensemble: {len(reps)} replicas, {n} rods (per replica {min(rods)}..{max(rods)}), "
         f"{sum(sc.xl.nLinks for sc in reps)} links ({time.perf_counter() - t_build:.1f} s)")
     ctx = Context(0)
+    factors = tuple(float(v) for v in args.cell_factors.split(",") if v.strip()) if args.cell_factors else (1.0,)
+    cell = {"factor": 1.0, "ms_per_step": None}
+    if len(factors) > 1:
+        cell = tune_cell_size(ctx, reps, factors=factors, trial_steps=10)
+        log(f"cell-size tuner: {cell}")
+        if cell["factor"] != 1.0:
+            reps = [scenes.retile(sc, cell["factor"]) for sc in reps]
     bt = Batch(ctx, reps, keep_forces=False)
     bt.net.set_overlap(bool(args.overlap))
     if args.graph:
@@ -555,7 +565,8 @@ def run_batch(args, log) -> int:
         "config": {"workload": f"{len(reps)} replicas of the crosslinked network recipe, 2k-40k filaments x 10 rods "
                                "(log-uniform), actin 5-30 uM (periodic box), 2.5 links per filament, dt 1e-5 s",
                    "rods": n, "replicas": len(reps), "rods_per_replica": [min(rods), int(np.median(rods)), max(rods)],
-                   "graph": bool(args.graph), "shape": shape},
+                   "graph": bool(args.graph), "shape": shape, "cell_factor": cell["factor"],
+                   "cell_tuning_ms_per_step": cell["ms_per_step"]},
         "step_ms": {"p50": round(float(np.percentile(per, 50)), 4), "p99": round(float(np.percentile(per, 99)), 4)},
     }
     if kernels:

@@ -97,6 +97,21 @@ def grid_for(box: float, bound_r: float, cutoff: float, periodic: bool, min_cell
                 boxLen=box_len, periodic=1 if periodic else 0)
 
 
+def retile(sc: "Scene", factor: float) -> "Scene":
+    """A copy of the scene whose broad-phase grid has cells `factor` times the scene's (factor >= 1):
+    the 27-cell stencil stays complete, so every candidate pair set is the same; the cell ids, CSR
+    offsets and the order of each candidate slice are the coarser grid's.  Used by the cell-size
+    autotuner; factor 1 is the reference's grid."""
+    if factor < 1.0:
+        raise ValueError("retile: cells can only grow (the stencil needs cell >= 2 r + cutoff)")
+    out = sc.copy()
+    if not sc.p.get("gridOn") or factor == 1.0:
+        return out
+    gp = sc.p["gridParams"]
+    out.p.update(grid_for(sc.box, 0.0, float(gp[5]), bool(sc.p["periodic"]), min_cell=factor * float(gp[3])))
+    return out
+
+
 ACTIN_MONO_DIAM = 2.0 * ACTIN_MONO_RADIUS   # Constants.actinMonoDiam = 0.0054 um
 
 

@@ -186,3 +186,45 @@ def test_ensemble_64_replicas_at_stated_size(ctx):
     for k in LINKS:
         assert_same(getattr(xl, k), getattr(orc.xl, k), f"oracle, replica {small}: {k}")
     bt.close()
+
+
+def test_cell_size_changes_only_slice_order_and_time(ctx):
+    """Replicas on grids with 1x, 1.6x and 2.5x their cells: pose, forces and link lifecycle are
+    bit-identical, and every body's candidate set (count and members) is the same; only the order
+    within a slice follows the grid.  The autotuner picks one of the factors it timed."""
+    from softbox_amd.batch import Batch, tune_cell_size
+
+    reps = _replicas(seed=6)
+    steps = 9
+    runs = {}
+    for f in (1.0, 1.6, 2.5):
+        tiles = [scenes.retile(sc, f) for sc in reps]
+        if f > 1.0:
+            assert all(t.p["gridDims"][3] < sc.p["gridDims"][3] for t, sc in zip(tiles, reps) if sc.p["gridOn"])
+        bt = Batch(ctx, tiles)
+        bt.net.use_graph(2)
+        bt.step(0, steps)
+        out = []
+        for r, sc in enumerate(tiles):
+            fs, xl = sc.fs.copy(), sc.xl.copy() if sc.xl is not None else None
+            bt.pull(r, fs, xl)
+            cand, cnt, _, _, _ = bt.candidates(r)
+            mc = int(sc.p["maxCand"])
+            sets = [np.sort(cand.reshape(-1, mc)[i, :min(c, mc)]) for i, c in enumerate(cnt)]
+            out.append((fs, xl, cnt, sets))
+        bt.close()
+        runs[f] = out
+    for f in (1.6, 2.5):
+        for r in range(len(reps)):
+            fs_a, xl_a, cnt_a, sets_a = runs[1.0][r]
+            fs_b, xl_b, cnt_b, sets_b = runs[f][r]
+            for k in STATE:
+                assert_same(getattr(fs_b, k), getattr(fs_a, k), f"cell x{f} replica {r}: {k}")
+            if xl_a is not None:
+                for k in LINKS:
+                    assert_same(getattr(xl_b, k), getattr(xl_a, k), f"cell x{f} replica {r}: {k}")
+            assert_same(cnt_b, cnt_a, f"cell x{f} replica {r}: candCount")
+            for i in range(len(sets_a)):
+                assert np.array_equal(sets_a[i], sets_b[i]), f"cell x{f} replica {r} body {i}: candidate set"
+    res = tune_cell_size(ctx, reps, factors=(1.0, 2.0), trial_steps=2)
+    assert res["factor"] in (1.0, 2.0) and all(v > 0 for v in res["ms_per_step"].values())
