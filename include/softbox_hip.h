@@ -186,6 +186,67 @@ int sbx_form_allocate(sbx_ctx *c, const int *reqFilA, const int *reqFilB, const 
                       float *strainHist, int *strainPlace, int *linkOrientSame, float *torqueMagHist,
                       int *torqueMagPlace, int K, int C);
 
+/* ------------------------------------------------------------------ actin turnover
+ * Per-segment state over the C slots of a filament store (device arrays): filState (FilamentStore.filState,
+ * >= 0 ACTIVE, < 0 FREE), monomerCount, seedNode (NodeNucleationStore.seedNode: the node holding a growing
+ * barbed tip, < 0 none), nucFrac (3C planar ATP | ADP-Pi | ADP, AgingStore.nucFrac), cofFrac (C,
+ * SeverStore.cofFrac).  The per-System entry points follow the Java methods' arguments; `fires` is the
+ * cadence flag of their counts arrays, step / seed the draw keys (Wang: the reference's slot-step-seed base
+ * with the salts GROW / DEPO; or Philox). */
+/* AgingSystem.age (AgingSystem.java:44-64); agingParams (host, 2): pH, pD */
+int sbx_aging_age(sbx_ctx *c, const int *filState, float *nucFrac, const float *agingParams, int fires, int C);
+/* AgingSystem.growthAtp (AgingSystem.java:72-84) */
+int sbx_aging_growth_atp(sbx_ctx *c, const int *grewFlag, const int *monomerCount, float *nucFrac, int C);
+/* AgingSystem.splitInheritNuc (AgingSystem.java:94-108); K requests */
+int sbx_aging_split_inherit_nuc(sbx_ctx *c, const int *rankOffsets, const int *freeList, const int *freeOffsets,
+                                float *nucFrac, int C, int K);
+/* DepolySystem.depoly (DepolySystem.java:70-101), or depolyProxy (:115-150) when nucFrac is non-NULL;
+ * depolyParams (host, 3): P, halfmono, floor; depolyRateParams (host, 2): pATP, pADP */
+int sbx_depoly(sbx_ctx *c, const int *filState, int *monomerCount, float *coord, const float *uVec,
+               const int *end1NbrSlot, const float *nucFrac, int *returnedMon, int *deathFlag,
+               const float *depolyParams, const float *depolyRateParams, int step, int seed, int fires, int C,
+               int rng, uint32_t stream);
+/* DepolySystem.applyDeath (DepolySystem.java:159-187): every slot clears its own links to dying slots, so the
+ * link break needs no scatter (the same result as the reference's on reciprocal chains, which every
+ * turnover operation keeps) */
+int sbx_apply_death(sbx_ctx *c, int *filState, int *monomerCount, int *seedNode, int *end1NbrSlot, int *end1NbrSide,
+                    int *end2NbrSlot, int *end2NbrSide, float *brownTransScale, float *brownRotScale,
+                    const int *deathFlag, int C);
+/* GrowthSystem.grow (GrowthSystem.java:73-106); growParams (host, 3): P, halfmono, maxMon */
+int sbx_growth_grow(sbx_ctx *c, const int *seedNode, int *monomerCount, float *coord, const float *uVec,
+                    int *grewFlag, const float *growParams, int step, int seed, int fires, int C, int rng,
+                    uint32_t stream);
+/* GrowthSystem.markSplits (GrowthSystem.java:115-145); requests at stride C; splitParams (host, 3) */
+int sbx_growth_mark_splits(sbx_ctx *c, const int *seedNode, const int *monomerCount, const float *coord,
+                           const float *uVec, const float *yVec, int *acceptFlag, float *reqCoord, float *reqUVec,
+                           float *reqYVec, const float *splitParams, int fires, int C);
+/* GrowthSystem.splitWire (GrowthSystem.java:155-196); K requests */
+int sbx_growth_split_wire(sbx_ctx *c, const int *rankOffsets, const int *freeList, const int *freeOffsets,
+                          int *monomerCount, float *coord, const float *uVec, int *end1NbrSlot, int *end1NbrSide,
+                          int *end2NbrSlot, int *end2NbrSide, int *seedNode, const float *splitParams, int C, int K);
+/* GrowthSystem.recomputeDrag (GrowthSystem.java:206-227); dragParams (host, 9); bTransDiff / bRotDiff may be
+ * NULL */
+int sbx_growth_recompute_drag(sbx_ctx *c, const int *monomerCount, float *segLength, const int *end1NbrSlot,
+                              const int *end2NbrSlot, float *bTransGam, float *bRotGam, float *bTransDiff,
+                              float *bRotDiff, const float *dragParams, int fires, int C);
+/* SeveringSystem.cofilinAccumulate (SeveringSystem.java:37-50); cofilinParams (host, 2): pCof, ratio */
+int sbx_cofilin_accumulate(sbx_ctx *c, const int *filState, const float *nucFrac, float *cofFrac,
+                           const float *cofilinParams, int fires, int C);
+/* SeveringSystem.cofilinDissolve (SeveringSystem.java:59-73) */
+int sbx_cofilin_dissolve(sbx_ctx *c, const int *filState, const int *monomerCount, const float *cofFrac,
+                         int *severDeathFlag, int *severReturnedMon, const float *cofilinParams, int fires, int C);
+/* SeveringSystem.nucleateFreshCofilin (SeveringSystem.java:87-97); K requests */
+int sbx_cofilin_fresh(sbx_ctx *c, const int *rankOffsets, const int *freeList, const int *freeOffsets, float *cofFrac,
+                      int C, int K);
+/* FullSystemDemoHarness.allocCpu (FullSystemDemoHarness.java:950-957): FilamentBirthSystem.freeFlags +
+ * CrossBridgeSystem.csrScan + freeScatter + csrScan of the K requests + allocate (FilamentBirthSystem.java:41-86).
+ * Requests at stride RC; freeCount and acceptFlag come back cleared (csrScan's cursor side effect);
+ * birthParams (host, 2): the born slot's Brownian scales */
+int sbx_birth_alloc(sbx_ctx *c, int *filState, int *freeCount, int *freeOffsets, int *freeList, int *acceptFlag,
+                    int *rankOffsets, const float *reqCoord, const float *reqUVec, const float *reqYVec, float *coord,
+                    float *uVec, float *yVec, float *brownTransScale, float *brownRotScale, const float *birthParams,
+                    int C, int K, int RC);
+
 /* CrosslinkerSystem.countActiveLinks (CrosslinkerSystem.java:694-706) */
 int sbx_count_active_links(sbx_ctx *c, const int *linkState, const int *linkFilA, const int *linkFilB,
                            int *activeLinkCount, int C, int nSeg);
@@ -434,6 +495,41 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps);
 /* formation counters: out[0] passes, [1] candidates of the last pass, [2] links formed by the last pass,
  * [3] links formed in all; returns the number written (<= 4) */
 int sbx_net_form_stats(sbx_net *net, long long *out, int cap);
+typedef struct sbx_turnover {
+    int C;                                     /* slots: the filament store's n */
+    int *filState, *monomerCount, *seedNode;
+    float *nucFrac;                            /* 3C */
+    float *cofFrac;                            /* C */
+    float *bTransDiff, *bRotDiff;              /* 3C each, or NULL (written by recomputeDrag) */
+    double *pool;                              /* device [1]: free actin, uM (ActinPool.conc) */
+    long long *ledger;                         /* device [2]: monomers taken, returned (ActinPool ledger) */
+    int *stats;                                /* device [8]: last cadence's monomers grown, returned by depoly,
+                                                  returned by dissolves, deaths, dissolves, splits; [6] cadences */
+} sbx_turnover;
+
+typedef struct sbx_turnover_params {
+    int interval;                              /* steps per biochem cadence (GrowthStore.biochemCheckInt) */
+    int seed;                                  /* growCounts[2] / depolyCounts[2] */
+    int growthOn;                              /* growth P = onRate * conc * biochemDeltaT (GrowthStore.refreshRate) */
+    int proxy;                                 /* 1: depolyProxy (nucleotide-dependent), 0: fixed-rate depoly */
+    float growParams[3], splitParams[3], dragParams[9], depolyParams[3], depolyRateParams[2], agingParams[2],
+        cofilinParams[2], birthParams[2];
+    double onRate, biochemDeltaT, uMPerMonomer;
+} sbx_turnover_params;
+
+/* One biochem cadence of FullSystemDemoHarness.cpuStep (FullSystemDemoHarness.java:650-684; one growth pass,
+ * no node nucleation) over a filament store: refreshRate, age, depoly(Proxy), csrScan + put, applyDeath,
+ * grow, csrScan, growthAtp (a no-op: the scan cleared grewFlag), take, cofilinAccumulate, cofilinDissolve,
+ * csrScan, applyDeath, put, markSplits, allocCpu, splitWire, splitInheritNuc, nucleateFreshCofilin,
+ * recomputeDrag; the pool and the stats are updated on the device. */
+int sbx_turnover_cadence(sbx_ctx *c, const sbx_filaments *fil, const sbx_turnover *t, const sbx_turnover_params *p,
+                         int step, int rng, uint32_t stream);
+
+/* Actin turnover at its biochem cadence inside sbx_net_step: on every step t with t % p->interval == 0 the
+ * cadence of sbx_turnover_cadence runs on the current pose before the step (before its grid and formation:
+ * FullSystemDemoHarness.cpuStep order); the formation pass then reads t->filState.  The arrays of *t stay the
+ * caller's (pass t = NULL to switch turnover off).  Not with the skin broad phase or a batch. */
+int sbx_net_set_turnover(sbx_net *net, const sbx_turnover *t, const sbx_turnover_params *p);
 /* make the caller's coord/uVec/yVec hold the current pose (the step ping-pongs pose buffers) */
 int sbx_net_publish(sbx_net *net);
 /* rebuild the crosslink CSR and recount the active links after the caller changed linkFilA/linkFilB or

@@ -50,7 +50,7 @@ class SboNet(C.Structure):
            ("gridDims", C.c_int * 4)]
         + [(k, vp) for k in ("center", "boundingRadius", "bodyCell", "cellCount", "gridCellOffsets",
                              "gridCellContents", "candPartner", "candCount")]
-        + [("seed", i32), ("rng", i32), ("stream", u32), ("form", vp), ("formInt", i32)]
+        + [("seed", i32), ("rng", i32), ("stream", u32), ("form", vp), ("formInt", i32), ("turn", vp)]
     )
 
 
@@ -64,6 +64,26 @@ class SboForm(C.Structure):
            ("formParams", C.c_float * 7), ("mode", i32), ("formCounts", C.c_int * 4)]
         + [(k, vp) for k in FORM_SCRATCH_I + FORM_SCRATCH_F]
         + [("lastFormed", i32)]
+    )
+
+
+TURN_I = ("filState", "monomerCount", "seedNode")
+TURN_F = ("nucFrac", "cofFrac", "bTransDiff", "bRotDiff")
+TURN_SCRATCH_I = ("grewFlag", "returnedMon", "deathFlag", "severDeathFlag", "severReturnedMon", "acceptFlag")
+TURN_SCRATCH_F = ("reqCoord", "reqUVec", "reqYVec")
+TURN_OFFS = ("freeCount", "freeOffsets", "freeList", "rankOffsets", "scanOffsets")
+
+
+class SboTurnover(C.Structure):
+    _fields_ = (
+        [("C", i32)]
+        + [(k, vp) for k in TURN_I + TURN_F + TURN_SCRATCH_I + TURN_SCRATCH_F + TURN_OFFS]
+        + [("interval", i32), ("seed", i32), ("growthOn", i32), ("proxy", i32)]
+        + [("growParams", C.c_float * 3), ("splitParams", C.c_float * 3), ("dragParams", C.c_float * 9),
+           ("depolyParams", C.c_float * 3), ("depolyRateParams", C.c_float * 2), ("agingParams", C.c_float * 2),
+           ("cofilinParams", C.c_float * 2), ("birthParams", C.c_float * 2)]
+        + [("onRate", C.c_double), ("biochemDeltaT", C.c_double), ("uMPerMonomer", C.c_double), ("conc", C.c_double),
+           ("taken", C.c_longlong), ("returned", C.c_longlong), ("last", C.c_int * 6), ("cadences", i32)]
     )
 
 
@@ -85,6 +105,10 @@ class SboNet64(C.Structure):
 
 class SboForm64(C.Structure):
     _fields_ = _widen(SboForm._fields_)
+
+
+class SboTurnover64(C.Structure):
+    _fields_ = _widen(SboTurnover._fields_)
 
 
 def _fill(arr, values) -> None:
@@ -232,6 +256,10 @@ class OracleNet:
                              candPartner=np.full(n * mc, -1, np.int32), candCount=np.zeros(n, np.int32))
             for k, a in self.grid.items():
                 setattr(s, k, P(a).value)
+        self.turn = None
+        if getattr(scene, "turn", None) is not None:
+            self.turn = OracleTurnover(fs, scene.turn, wide)
+            s.turn = C.cast(C.pointer(self.turn.s), C.c_void_p).value
         self.form = None
         if p.get("formOn"):
             self.form = OracleFormation(fs, xl, p, wide)
@@ -244,6 +272,10 @@ class OracleNet:
     def step(self, step0: int, n: int = 1) -> None:
         for t in range(step0, step0 + n):
             self.lib.sbo_network_step(C.byref(self.s), t)
+
+    def cadence(self, step: int) -> None:
+        """One turnover cadence alone (sbo_turnover_cadence) on the store, no mechanics step."""
+        self.lib.sbo_turnover_cadence(C.byref(self.turn.s), C.byref(self.s), int(step))
 
 
 class SboMotors(C.Structure):
@@ -382,3 +414,49 @@ class OracleFormation:
     @property
     def n_candidates(self) -> int:
         return int(self.s.formCounts[0])
+
+
+class OracleTurnover:
+    """An sbo_turnover over a filament store (host arrays, updated in place by sbo_turnover_cadence): a copy
+    of the scene's TurnoverStore, the cadence scratch, the pool and the last cadence's counters."""
+
+    def __init__(self, fs, ts, wide: bool = False):
+        t = ts.copy()
+        if wide:
+            _widen_store(t)
+        self.ts = t
+        n = t.C
+        f = SboTurnover64() if wide else SboTurnover()
+        f.C = n
+        fdt = np.float64 if wide else np.float32
+        self.a = {k: getattr(t, k) for k in TURN_I + ("nucFrac", "cofFrac")}
+        self.a["bTransDiff"] = fs.bTransDiff
+        self.a["bRotDiff"] = fs.bRotDiff
+        for k in TURN_SCRATCH_I:
+            self.a[k] = np.zeros(n, np.int32)
+        for k in TURN_SCRATCH_F:
+            self.a[k] = np.zeros(3 * n, fdt)
+        for k in TURN_OFFS:
+            self.a[k] = np.zeros(n + 1, np.int32)
+        for k, arr in self.a.items():
+            setattr(f, k, P(arr).value)
+        f.interval, f.seed, f.growthOn, f.proxy = int(t.interval), int(t.seed), int(t.growthOn), int(t.proxy)
+        for k in ("growParams", "splitParams", "dragParams", "depolyParams", "depolyRateParams", "agingParams",
+                  "cofilinParams", "birthParams"):
+            _fill(getattr(f, k), getattr(t, k))
+        f.onRate, f.biochemDeltaT, f.uMPerMonomer = float(t.onRate), float(t.biochemDeltaT), float(t.uMPerMonomer)
+        f.conc = float(t.pool[0])
+        f.taken, f.returned = int(t.ledger[0]), int(t.ledger[1])
+        self.s = f
+
+    @property
+    def conc(self) -> float:
+        return float(self.s.conc)
+
+    @property
+    def ledger(self):
+        return int(self.s.taken), int(self.s.returned)
+
+    @property
+    def last(self):
+        return list(self.s.last)

@@ -1079,6 +1079,8 @@ void sbo_network_finish(sbo_net *net, int32_t step) {
  * motors: [publish + grid + broad phase] -> zero + Brownian -> chain -> [unbind -> countActiveLinks ->
  * linkForces -> linkTorsion -> CSR A -> gatherA -> CSR B -> gatherB] -> confine -> integrate -> derive. */
 void sbo_network_step(sbo_net *net, int32_t step) {
+    if (net->turn && net->turn->interval > 0 && step % net->turn->interval == 0)   /* GrowthStore.firesAt */
+        sbo_turnover_cadence(net->turn, net, step);
     if (net->gridOn) sbo_network_grid(net);
     if (net->form && net->formInt > 0 && step % net->formInt == 0)   /* V2OneXHarness.java:379 */
         net->form->lastFormed = sbo_formation(net->form, step, net->seed, net->rng, net->stream);

@@ -204,6 +204,8 @@ typedef struct sbo_net {
     /* crosslinker formation at a cadence (V2OneXHarness.java:379): NULL = off */
     struct sbo_form *form;
     int formInt;
+    /* actin turnover at the biochem cadence (FullSystemDemoHarness.java:650-684): NULL = off */
+    struct sbo_turnover *turn;
 } sbo_net;
 
 /* ---- crosslinker formation (softbox_oracle_form.c; CrosslinkerSystem.java:298-760, FormationGrid.java,
@@ -262,6 +264,64 @@ typedef struct sbo_form {
     int lastFormed;
 } sbo_form;
 int sbo_formation(sbo_form *f, int32_t step, int32_t seed, int rng, uint32_t stream); /* V2OneXHarness.java:444-464 */
+
+/* ---- actin turnover (softbox_oracle_turnover.c; AgingSystem, DepolySystem, GrowthSystem, SeveringSystem,
+ * FilamentBirthSystem, ActinPool) ---- */
+#define SBO_FIL_ACTIVE 0      /* FilamentStore.FIL_ACTIVE (FilamentStore.java:112) */
+#define SBO_FIL_FREE (-1)     /* FilamentStore.FIL_FREE (:113) */
+void sbo_aging_age(int C, const int *filState, float *nucFrac, const float agingParams[2], int fires);
+void sbo_aging_growth_atp(int C, const int *grewFlag, const int *monomerCount, float *nucFrac);
+void sbo_aging_split_inherit_nuc(int C, int K, const int *rankOffsets, const int *freeList, const int *freeOffsets,
+                                 float *nucFrac);
+void sbo_aging_nucleate_fresh_atp(int C, int K, const int *rankOffsets, const int *freeList, const int *freeOffsets,
+                                  float *nucFrac);
+void sbo_depoly(int C, const int *filState, int *monomerCount, float *coord, const float *uVec,
+                const int *end1NbrSlot, const float *nucFrac, int *returnedMon, int *deathFlag,
+                const float depolyParams[3], const float depolyRateParams[2], int32_t step, int32_t seed, int rng,
+                uint32_t stream, int fires);
+void sbo_apply_death(int C, int *filState, int *monomerCount, int *seedNode, int *e1s, int *e1d, int *e2s, int *e2d,
+                     float *bts, float *brs, const int *deathFlag);
+void sbo_grow(int C, const int *seedNode, int *monomerCount, float *coord, const float *uVec, int *grewFlag,
+              const float growParams[3], int32_t step, int32_t seed, int rng, uint32_t stream, int fires);
+void sbo_mark_splits(int C, const int *seedNode, const int *monomerCount, const float *coord, const float *uVec,
+                     const float *yVec, int *acceptFlag, float *reqCoord, float *reqUVec, float *reqYVec, int RC,
+                     const float splitParams[3], int fires);
+void sbo_split_wire(int C, int K, const int *rankOffsets, const int *freeList, const int *freeOffsets,
+                    int *monomerCount, float *coord, const float *uVec, int *e1s, int *e1d, int *e2s, int *e2d,
+                    int *seedNode, const float splitParams[3]);
+void sbo_recompute_drag(int C, const int *monomerCount, float *segLength, const int *end1NbrSlot,
+                        const int *end2NbrSlot, float *bTransGam, float *bRotGam, float *bTransDiff, float *bRotDiff,
+                        const float dragParams[9], int fires);
+void sbo_cofilin_accumulate(int C, const int *filState, const float *nucFrac, float *cofFrac,
+                            const float cofilinParams[2], int fires);
+void sbo_cofilin_dissolve(int C, const int *filState, const int *monomerCount, const float *cofFrac,
+                          int *severDeathFlag, int *severReturnedMon, const float cofilinParams[2], int fires);
+void sbo_nucleate_fresh_cofilin(int C, int K, const int *rankOffsets, const int *freeList, const int *freeOffsets,
+                                float *cofFrac);
+int sbo_csr_scan(int n, int *counts, int *offsets);
+void sbo_birth_alloc(int C, int K, int *filState, int *freeCount, int *freeOffsets, int *freeList, int *acceptFlag,
+                     int *rankOffsets, const float *reqCoord, const float *reqUVec, const float *reqYVec, int RC,
+                     float *coord, float *uVec, float *yVec, float *bts, float *brs, const float birthParams[2]);
+typedef struct sbo_turnover {
+    int C;                                    /* slots: the filament store's capacity */
+    int *filState, *monomerCount, *seedNode;  /* FilamentStore.filState/monomerCount, NodeNucleationStore.seedNode */
+    float *nucFrac;                           /* 3C: ATP | ADP-Pi | ADP (AgingStore.nucFrac) */
+    float *cofFrac;                           /* C (SeverStore.cofFrac) */
+    float *bTransDiff, *bRotDiff;             /* 3C each, or NULL */
+    int *grewFlag, *returnedMon, *deathFlag, *severDeathFlag, *severReturnedMon, *acceptFlag;
+    float *reqCoord, *reqUVec, *reqYVec;      /* 3C planar, request r = slot r */
+    int *freeCount, *freeOffsets, *freeList, *rankOffsets, *scanOffsets;   /* offsets: C + 1 */
+    int interval, seed, growthOn, proxy;
+    float growParams[3], splitParams[3], dragParams[9], depolyParams[3], depolyRateParams[2], agingParams[2],
+        cofilinParams[2], birthParams[2];
+    double onRate, biochemDeltaT, uMPerMonomer;
+    double conc;                              /* ActinPool: free actin (uM) and the integer ledger */
+    long long taken, returned;
+    int last[6];                              /* last cadence: grew, returned, severed monomers, deaths,
+                                                 dissolves, splits */
+    int cadences;
+} sbo_turnover;
+void sbo_turnover_cadence(sbo_turnover *t, sbo_net *net, int32_t step);   /* FullSystemDemoHarness.java:650-684 */
 
 void sbo_network_step(sbo_net *net, int32_t step);
 void sbo_network_grid(sbo_net *net);                   /* publish + body cell + grid + broad phase */

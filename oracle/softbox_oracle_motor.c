@@ -676,6 +676,9 @@ void sbo_contractile_step(sbo_net *net, sbo_motors *mo, int32_t step) {
     int n = net->nRod, nM = mo->nM, nB = 3 * nM, nBb = mo->nBb;
     int cap = n + nM;
     int mc = net->maxCand;
+    /* --- turnover at the biochem cadence (FullSystemDemoHarness.java:650-684), before binding --- */
+    if (net->turn && net->turn->interval > 0 && step % net->turn->interval == 0)   /* GrowthStore.firesAt */
+        sbo_turnover_cadence(net->turn, net, step);
     /* --- binding --- */
     sbo_publish_head_from_body(nM, mo->coord, mo->uVec, mo->segLength, mo->head, mo->hUVec, mo->rodUVec);
     sbo_publish_filaments(n, cap, net->coord, net->segLength, net->actinRadius, net->center, net->boundingRadius);

@@ -126,6 +126,20 @@ class SbxSkinParams(C.Structure):
                 ("listCap", i32), ("blockKeys", i32)]
 
 
+class SbxTurnover(C.Structure):
+    _fields_ = [("C", i32)] + [(name, vp) for name in (
+        "filState", "monomerCount", "seedNode", "nucFrac", "cofFrac", "bTransDiff", "bRotDiff", "pool", "ledger",
+        "stats")]
+
+
+class SbxTurnoverParams(C.Structure):
+    _fields_ = [("interval", i32), ("seed", i32), ("growthOn", i32), ("proxy", i32),
+                ("growParams", C.c_float * 3), ("splitParams", C.c_float * 3), ("dragParams", C.c_float * 9),
+                ("depolyParams", C.c_float * 3), ("depolyRateParams", C.c_float * 2), ("agingParams", C.c_float * 2),
+                ("cofilinParams", C.c_float * 2), ("birthParams", C.c_float * 2),
+                ("onRate", C.c_double), ("biochemDeltaT", C.c_double), ("uMPerMonomer", C.c_double)]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPES)
 _SIGS = {
     "sbx_create": [C.POINTER(vp), i32, vp],
@@ -163,6 +177,22 @@ _SIGS = {
     "sbx_form_admit": [vp] + [vp] * 13 + [f32p, i32, i32, i32],
     "sbx_form_allocate": [vp] + [vp] * 20 + [i32, i32],
     "sbx_net_form_stats": [vp, C.POINTER(C.c_longlong), i32],
+    "sbx_aging_age": [vp, vp, vp, f32p, i32, i32],
+    "sbx_aging_growth_atp": [vp, vp, vp, vp, i32],
+    "sbx_aging_split_inherit_nuc": [vp, vp, vp, vp, vp, i32, i32],
+    "sbx_depoly": [vp] + [vp] * 8 + [f32p, f32p, i32, i32, i32, i32, i32, u32],
+    "sbx_apply_death": [vp] + [vp] * 10 + [i32],
+    "sbx_growth_grow": [vp] + [vp] * 5 + [f32p, i32, i32, i32, i32, i32, u32],
+    "sbx_growth_mark_splits": [vp] + [vp] * 9 + [f32p, i32, i32],
+    "sbx_growth_split_wire": [vp] + [vp] * 11 + [f32p, i32, i32],
+    "sbx_growth_recompute_drag": [vp] + [vp] * 8 + [f32p, i32, i32],
+    "sbx_cofilin_accumulate": [vp, vp, vp, vp, f32p, i32, i32],
+    "sbx_cofilin_dissolve": [vp] + [vp] * 5 + [f32p, i32, i32],
+    "sbx_cofilin_fresh": [vp, vp, vp, vp, vp, i32, i32],
+    "sbx_birth_alloc": [vp] + [vp] * 14 + [f32p, i32, i32, i32],
+    "sbx_turnover_cadence": [vp, C.POINTER(SbxFilaments), C.POINTER(SbxTurnover), C.POINTER(SbxTurnoverParams), i32,
+                             i32, u32],
+    "sbx_net_set_turnover": [vp, C.POINTER(SbxTurnover), C.POINTER(SbxTurnoverParams)],
     "sbx_link_unbind": [vp] + [vp] * 10 + [f32p, i32p, i32, i32, u32],
     "sbx_link_forces": [vp] + [vp] * 10 + [f32p, i32, i32],
     "sbx_link_torsion": [vp] + [vp] * 8 + [f32p, i32, i32],

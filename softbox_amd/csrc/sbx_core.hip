@@ -509,6 +509,7 @@ void sbx_destroy(sbx_ctx *c) {
     hipStreamSynchronize(c->stream);
     sort_scratch_free(c->api_sort);
     if (c->form_csr) hipFree(c->form_csr);
+    turn_scratch_destroy(c->turn);
     if (c->owns_stream) {
         hipStreamSynchronize(c->stream);
         hipStreamDestroy(c->stream);

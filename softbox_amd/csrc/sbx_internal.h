@@ -27,6 +27,7 @@ struct SortScratch {
     int scanItems = 8;  // keys per scan thread: tiles of 256 * scanItems keys (4, 8 or 16; same result)
 };
 struct ChainParams;
+struct TurnScratch;
 }  // namespace sbx
 
 struct sbx_ctx {
@@ -39,6 +40,7 @@ struct sbx_ctx {
     sbx::SortScratch api_sort;  // scratch of the stand-alone sbx_grid_build / sbx_csr_build entry points
     int *form_csr;              // sbx_form_admit's link CSRs (by linkFilA, by linkFilB)
     long long form_csr_ints;
+    sbx::TurnScratch *turn;     // scratch of the stand-alone sbx_turnover_cadence
 };
 
 namespace sbx {
@@ -96,6 +98,8 @@ int exclusive_scan(sbx_ctx *c, SortScratch &s, const int *in, int n, int *out, h
 // the histogram pass alone: slot[m] = atomicAdd(counts[key[m]], 1)
 int count_keys(sbx_ctx *c, const int *key, int nItems, int nKeys, int *counts, int *slot, hipStream_t st = nullptr);
 int fill_i32(sbx_ctx *c, int *dst, int value, long long count);
+// frees and deletes a turnover scratch (sbx_turnover.hip); null is a no-op
+void turn_scratch_destroy(TurnScratch *s);
 
 ChainParams chain_params(const float *cp);
 

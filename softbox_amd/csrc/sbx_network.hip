@@ -21,6 +21,7 @@
 
 #include "sbx_device.h"
 #include "sbx_form.h"
+#include "sbx_turnover.h"
 #include "sbx_grid.h"
 #include "sbx_internal.h"
 #include "sbx_links.h"
@@ -154,6 +155,11 @@ struct sbx_net {
     int *formInts = nullptr;    // one allocation behind the pass's int buffers
     float *formFloats = nullptr;
     long long formPasses = 0;
+    // actin turnover every tp.interval steps (turnOn): the caller's state arrays and the cadence's scratch
+    int turnOn = 0;
+    sbx_turnover turn{};
+    sbx_turnover_params tp{};
+    TurnScratch turnScratch;
 };
 
 namespace {
@@ -1571,6 +1577,7 @@ void release(sbx_net *net) {
         if (b) hipFree(b);
     sort_scratch_free(net->sort);
     sort_scratch_free(net->formSort);
+    turn_scratch_free(net->turnScratch);
     for (auto &pe : net->pending) {
         hipEventDestroy(pe.second.first);
         hipEventDestroy(pe.second.second);
@@ -1658,6 +1665,7 @@ int run_formation(sbx_net *net, int step) {
     FormPass &f = net->form;
     f.coord = net->coordB[net->cur];
     f.uVec = net->uB[net->cur];
+    f.filState = net->turnOn ? net->turn.filState : nullptr;
     if ((r = form_pass(c, net->formSort, f, step, net->formFilIDValid != 0, c->stream))) return r;
     net->formFilIDValid = 1;
     hipLaunchKernelGGL(k_form_tally, dim3(1), dim3(1), 0, c->stream, f.stats);
@@ -1666,6 +1674,33 @@ int run_formation(sbx_net *net, int step) {
     net->countsValid = 0;
     net->formPasses++;
     return check_launch(c, "formation");
+}
+
+// One biochem cadence on the current pose buffers (FullSystemDemoHarness.java:650-684), then the step's
+// prepared state that follows the pose, segment lengths and chain topology is marked stale: bounding radii
+// are recomputed, the next step re-bins and re-packs, the formation pass recomputes filament identities.
+int run_turnover(sbx_net *net, int step) {
+    sbx_ctx *c = net->ctx;
+    TurnArrays a{};
+    a.C = net->n;
+    a.coord = net->coordB[net->cur]; a.uVec = net->uB[net->cur]; a.yVec = net->yB[net->cur];
+    a.segLength = net->fil.segLength; a.bTransGam = net->fil.bTransGam; a.bRotGam = net->fil.bRotGam;
+    a.bts = net->fil.brownTransScale; a.brs = net->fil.brownRotScale;
+    a.e1s = net->fil.end1NbrSlot; a.e1d = net->fil.end1NbrSide; a.e2s = net->fil.end2NbrSlot; a.e2d = net->fil.end2NbrSide;
+    a.filState = net->turn.filState; a.monomerCount = net->turn.monomerCount; a.seedNode = net->turn.seedNode;
+    a.nucFrac = net->turn.nucFrac; a.cofFrac = net->turn.cofFrac;
+    a.bTransDiff = net->turn.bTransDiff; a.bRotDiff = net->turn.bRotDiff;
+    a.pool = net->turn.pool; a.ledger = net->turn.ledger; a.stats = net->turn.stats;
+    int r = turnover_cadence(c, a, net->tp, net->turnScratch, step, net->p.rng, net->p.stream, c->stream);
+    if (r) return r;
+    if (net->boundR) {
+        hipLaunchKernelGGL(k_bounding_radius, dim3(blocks_for(net->n, 256)), dim3(256), 0, c->stream,
+                           net->fil.segLength, net->p.actinRadius, net->boundR, net->n);
+        if ((r = check_launch(c, "k_bounding_radius"))) return r;
+    }
+    net->derivedValid = 0;
+    net->formFilIDValid = 0;
+    return SBX_OK;
 }
 
 int net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *xl, const sbx_net_params *p,
@@ -2128,18 +2163,58 @@ int sbx_net_step(sbx_net *net, int step0, int nSteps) {
     sbx_ctx *c = net->ctx;
     SBX_CHECK_CTX(c);
     SBX_REQUIRE(c, nSteps >= 0, "net_step: nSteps < 0");
-    if (!net->formOn) return step_range(net, step0, nSteps);
-    SBX_REQUIRE(c, step0 >= 0, "net_step: formation steps count from 0 (step0 < 0)");
-    const int every = net->p.formInt, end = step0 + nSteps;
+    if (!net->formOn && !net->turnOn) return step_range(net, step0, nSteps);
+    SBX_REQUIRE(c, step0 >= 0, "net_step: cadence steps count from 0 (step0 < 0)");
+    const int end = step0 + nSteps;
     int r = set_device(c);
     if (r) return r;
+    // the cadence passes run between captured/eager step ranges: turnover first, then formation, then
+    // the step (FullSystemDemoHarness.cpuStep order)
     for (int t = step0; t < end;) {
-        if (t % every == 0 && (r = run_formation(net, t))) return r;
-        int next = (t / every + 1) * every;
-        if (next > end) next = end;
+        int next = end;
+        if (net->turnOn) {
+            const int every = net->tp.interval;
+            if (t % every == 0 && (r = run_turnover(net, t))) return r;
+            const int nt = (t / every + 1) * every;
+            if (nt < next) next = nt;
+        }
+        if (net->formOn) {
+            const int every = net->p.formInt;
+            if (t % every == 0 && (r = run_formation(net, t))) return r;
+            const int nt = (t / every + 1) * every;
+            if (nt < next) next = nt;
+        }
         if ((r = step_range(net, t, next - t))) return r;
         t = next;
     }
+    return SBX_OK;
+}
+
+int sbx_net_set_turnover(sbx_net *net, const sbx_turnover *t, const sbx_turnover_params *p) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    if (!t) {
+        net->turnOn = 0;
+        net->form.filState = nullptr;
+        return SBX_OK;
+    }
+    SBX_REQUIRE(c, p, "net_set_turnover: null params");
+    SBX_REQUIRE(c, !net->reps, "net_set_turnover: turnover is not batched");
+    SBX_REQUIRE(c, net->skinMode == SBX_SKIN_REBUILD, "net_set_turnover: not with the skin broad phase");
+    SBX_REQUIRE(c, t->C == net->n, "net_set_turnover: C %d != rods %d", t->C, net->n);
+    SBX_REQUIRE(c, t->filState && t->monomerCount && t->seedNode && t->nucFrac && t->cofFrac && t->pool && t->ledger &&
+                       t->stats,
+                "net_set_turnover: null turnover array");
+    SBX_REQUIRE(c, net->fil.end1NbrSlot && net->fil.end1NbrSide && net->fil.end2NbrSlot && net->fil.end2NbrSide,
+                "net_set_turnover: the network has no chain topology arrays");
+    SBX_REQUIRE(c, p->interval > 0, "net_set_turnover: interval must be > 0");
+    int r = set_device(c);
+    if (!r) r = turn_scratch_reserve(c, net->turnScratch, net->n);
+    if (r) return r;
+    net->turn = *t;
+    net->tp = *p;
+    net->turnOn = 1;
     return SBX_OK;
 }
 
@@ -2223,6 +2298,8 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
                 "net_set_skin: unknown mode %d", mode);
     SBX_REQUIRE(c, !net->hasMotors, "net_set_skin: the skin broad phase covers the filament-only step");
     SBX_REQUIRE(c, mode == SBX_SKIN_REBUILD || !net->reps, "net_set_skin: a batch runs the per-step rebuild");
+    SBX_REQUIRE(c, mode == SBX_SKIN_REBUILD || !net->turnOn,
+                "net_set_skin: turnover changes segment lengths (the skin's reach); not with turnover");
     if (mode != SBX_SKIN_REBUILD) {
         SBX_REQUIRE(c, net->p.gridOn, "net_set_skin: needs the grid (gridOn)");
         SBX_REQUIRE(c, sp, "net_set_skin: null params");

@@ -35,6 +35,14 @@ class Network:
             self._ms = self.mot.struct()
             self._mp = self.mot.params()
             ctx.check(ctx.lib.sbx_net_attach_motors(self.h, C.byref(self._ms), C.byref(self._mp)))
+        self.turn = None
+        if getattr(scene, "turn", None) is not None:
+            from .turnover import DeviceTurnover
+
+            self.turn = DeviceTurnover(ctx, scene.turn, diff=(scene.fs.bTransDiff, scene.fs.bRotDiff))
+            self._ts = self.turn.struct()
+            self._tp = scene.turn.params()
+            ctx.check(ctx.lib.sbx_net_set_turnover(self.h, C.byref(self._ts), C.byref(self._tp)))
 
     def set_skin(self, mode: int, cfg=None, model=None) -> None:
         """Broad-phase mode: SBX_SKIN_REBUILD (the reference's per-step rebuild), SBX_SKIN_FIXED or
@@ -152,6 +160,15 @@ class Network:
         if xl is not None and self.xl is not None:
             self.xl.pull(xl)
         return fs, xl
+
+    def pull_turnover(self, ts, fs=None):
+        """The turnover state (and, with fs, the diffusion coefficients recomputeDrag writes) back into host
+        stores."""
+        self.turn.pull(ts)
+        if fs is not None:
+            fs.bTransDiff[:] = self.ctx.to_host(self.turn.t["bTransDiff"])
+            fs.bRotDiff[:] = self.ctx.to_host(self.turn.t["bRotDiff"])
+        return ts
 
     def pull_motors(self, mot=None, mini=None):
         """Synchronise and copy the motor and backbone state back into host stores."""

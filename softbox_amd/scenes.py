@@ -37,6 +37,7 @@ class Scene:
     seed: int
     n_fil: int = 0
     segs_per_fil: int = 1
+    turn: object = None         # TurnoverStore (actin turnover at its cadence), or None
 
     def net_params(self) -> _lib.SbxNetParams:
         s = _lib.SbxNetParams()
@@ -55,7 +56,8 @@ class Scene:
     def copy(self) -> "Scene":
         return dataclasses.replace(self, fs=self.fs.copy(), xl=None if self.xl is None else self.xl.copy(),
                                    p={k: (list(v) if isinstance(v, (list, tuple, np.ndarray)) else v)
-                                      for k, v in self.p.items()})
+                                      for k, v in self.p.items()},
+                                   turn=None if self.turn is None else self.turn.copy())
 
 
 def _unit_vectors(rng: np.random.Generator, n: int) -> np.ndarray:
@@ -441,3 +443,89 @@ def contractile(n_fil: int = 50000, segs: int = 10, seg_len: float = 0.1, box: f
     p.update(seedMotor=seed + 0x0CDE5, seedBackbone=seed + 0x5C2F11)
     return ContractileScene("contractile", base.fs, base.xl, p, box, seed, n_fil=n_fil, segs_per_fil=segs,
                             mot=mot, dim=dim, mini=mini)
+
+
+def with_turnover(sc: Scene, spare: int, growth_rate: float = 10.0, proxy: bool = True, cofilin_ratio: float = 1.0,
+                  aging: bool = True, interval: int | None = None, seed: int | None = None, speedup: float = 1.0,
+                  eta: float | None = None, kT: float = KT_WORKLOAD) -> Scene:
+    """The scene with actin turnover: `spare` FREE slots appended to the filament store (parked at random
+    positions, inert: Brownian scales 0, no chain links), every segment's monomer count from its length,
+    every chain's barbed tip (end2 free) held as a growing tip, free actin set for `growth_rate` subunits/s at
+    the barbed ends (GrowthSystem: P = kATPOn2WithFormin conc biochemDeltaT), pointed-end depolymerisation
+    (nucleotide-dependent when `proxy`), aging, cofilin severing (off at cofilin_ratio >= 1) and the split
+    allocator, every `interval` steps (the biochem cadence: biochemDeltaT / dt).  `speedup` multiplies every
+    per-cadence probability (test scenes: many events in few steps)."""
+    from .stores import CrosslinkerStore, FilamentStore
+    from .turnover import (FIL_FREE, TurnoverStore, growth_conc_for, monomers_for, pool_for, seed_barbed_tips)
+
+    n0 = sc.fs.n
+    n = n0 + spare
+    g = np.random.default_rng(sc.seed + 7777 if seed is None else seed)
+    fs = FilamentStore(n)
+    for k in FilamentStore.VEC:
+        src = getattr(sc.fs, k).reshape(3, n0)
+        dst = getattr(fs, k).reshape(3, n)
+        dst[:, :n0] = src
+        dst[:, n0:] = src[:, :1]
+    for k in FilamentStore.SCALAR_F:
+        a = getattr(fs, k)
+        a[:n0] = getattr(sc.fs, k)
+        a[n0:] = getattr(sc.fs, k)[0]
+    for k in FilamentStore.SCALAR_I:
+        getattr(fs, k)[:n0] = getattr(sc.fs, k)
+    fs.params[:] = sc.fs.params
+    fs.chainParams[:] = sc.fs.chainParams
+    if spare:
+        lo = 0.0 if sc.p.get("periodic") else -0.5 * sc.box
+        c = fs.plane("coord")
+        c[:, n0:] = lo + sc.box * g.random((3, spare))
+        u = _unit_vectors(g, spare)
+        fs.plane("uVec")[:, n0:] = u
+        fs.plane("yVec")[:, n0:] = _perp(u)
+        fs.brownTransScale[n0:] = 0.0
+        fs.brownRotScale[n0:] = 0.0
+        fs.plane("forceSum")[:, n0:] = 0.0
+        fs.plane("torqueSum")[:, n0:] = 0.0
+        derive_host(fs)
+    xl = None
+    if sc.xl is not None:
+        xl = CrosslinkerStore(sc.xl.nLinks, n)
+        for k, v in sc.xl.__dict__.items():
+            if isinstance(v, np.ndarray) and k not in ("filLinkCt", "activeLinkCount", "counts"):
+                setattr(xl, k, v.copy())
+        xl.filLinkCt[:n0] = sc.xl.filLinkCt
+        xl.activeLinkCount[:n0] = sc.xl.activeLinkCount
+        xl.counts[:] = sc.xl.counts
+        xl.counts[3] = n
+    ts = TurnoverStore(n)
+    ts.filState[n0:] = FIL_FREE
+    ts.monomerCount[:n0] = monomers_for(sc.fs.segLength.astype(np.float64))
+    seed_barbed_tips(fs, ts)
+    dt = float(sc.p["params"][0])
+    ts.interval = int(interval) if interval else max(1, int(round(ts.biochemDeltaT / dt)))
+    ts.seed = int(sc.seed)
+    ts.proxy = 1 if proxy else 0
+    pool_for(growth_conc_for(growth_rate * speedup), sc.box, ts)
+    ts.depolyParams[0] = np.float32(ts.depolyParams[0] * speedup)
+    ts.depolyRateParams[:] = (ts.depolyRateParams.astype(np.float64) * speedup).astype(np.float32)
+    if aging:
+        ts.agingParams[:] = (ts.agingParams.astype(np.float64) * speedup).astype(np.float32)
+    else:
+        ts.agingParams[:] = 0.0
+    ts.cofilinParams[0] = np.float32(ts.cofilinParams[0] * speedup)
+    ts.cofilinParams[1] = np.float32(cofilin_ratio)
+    ts.set_drag(float(eta) if eta is not None else _scene_eta(sc, kT), kT)
+    out = dataclasses.replace(sc, fs=fs, xl=xl, p={k: (list(v) if isinstance(v, (list, tuple, np.ndarray)) else v)
+                                                     for k, v in sc.p.items()}, turn=ts)
+    if out.p.get("formOn"):
+        out.p["formReqCap"] = int(min(n * 256, max(4096, 16 * n)))
+    return out
+
+
+def _scene_eta(sc: Scene, kT: float) -> float:
+    """The viscosity the scene's drag was set from (inverts the parallel drag of its first rod)."""
+    from .physics import rod_drag_si
+
+    L = float(sc.fs.segLength[0])
+    g1 = rod_drag_si(L, ACTIN_RADIUS, 1.0)[0]
+    return float(f"{float(sc.fs.bTransGam[0]) / g1:.6g}")
