@@ -37,9 +37,10 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "rod-steps/sec + HBM GB/s % peak at 1 GPU; learned-skin vs per-step rebuild"
 
 
-def kernel_bytes(sc, name: str, list_entries: int | None = None) -> float | None:
+def kernel_bytes(sc, name: str, list_entries: int | None = None, cand_written: int | None = None) -> float | None:
     """Algorithmic (compulsory) HBM bytes of one launch of a network kernel on this workload:
-    every input read once and every output written once, at its natural width."""
+    every input read once and every output written once, at its natural width.  cand_written: candidate
+    indices the broad phase writes (the used part of every slice), counted when known."""
     n = sc.fs.n
     L = sc.xl.nLinks if sc.xl is not None else 0
     cells = int(sc.p["gridDims"][3])
@@ -63,8 +64,9 @@ def kernel_bytes(sc, name: str, list_entries: int | None = None) -> float | None
         # sorted cell 4); per cell: counts read twice (tile sums, scan) 8 + offsets 4
         return 76.0 * n + 12.0 * cells
     if name == "k_broad_phase":
-        # per body: contents 4 + sorted pose 16 + sorted cell 4 + count 4; per cell: offsets 4
-        return 28.0 * n + 4.0 * cells
+        # per body: contents 4 + sorted pose 16 + sorted cell 4 + count 4; per cell: offsets 4; per candidate
+        # written: its index 4
+        return 28.0 * n + 4.0 * cells + 4.0 * (cand_written or 0)
     if name == "k_skin_pairs" and list_entries:
         # per body: order 4 + centre 12 + radius 4 + cell 4 + list length 4 + count 4; per list entry:
         # index 4 + centre 12 + radius 4 + cell 4
@@ -199,6 +201,8 @@ def main() -> int:
                     help="bp_group,scan_items,rod_threads[,bp_staged]: fixed launch shape instead of the autotuner")
     ap.add_argument("--batch", type=int, default=0,
                     help="N > 0: the batched ensemble instead (N replicas, 2k-40k filaments log-uniform, 5-30 uM)")
+    ap.add_argument("--turnover", type=int, default=1,
+                    help="actomyosin workload: actin turnover at its biochem cadence (1) or none (0)")
     ap.add_argument("--cell-factors", type=str, default="1,1.5,2,3",
                     help="batched ensemble: grid cell multiples the cell-size autotuner times ('' or '1': the "
                          "reference grid)")
@@ -296,6 +300,7 @@ def main() -> int:
         if args.shape or (args.tune_steps and mode == "rebuild"):
             r["shape"] = shape_of(args, None if args.shape else tuned)
         r["digest"] = slices_digest(net, sc)
+        r["cand_written"] = int(np.minimum(r["digest"][0], int(sc.p["maxCand"])).sum())
         if mode != "rebuild":
             r["skin"] = net.skin_stats()
         if args.profile_steps:
@@ -326,9 +331,10 @@ def main() -> int:
     roof = None
     per_kernel = head.get("kernels", {})
     entries = head.get("skin", {}).get("list_entries")
+    cw = head.get("cand_written")
     if per_kernel:
-        dom = max((k for k in per_kernel if kernel_bytes(sc, k, entries)), key=lambda k: per_kernel[k]["avg_us"])
-        b = kernel_bytes(sc, dom, entries)
+        dom = max((k for k in per_kernel if kernel_bytes(sc, k, entries, cw)), key=lambda k: per_kernel[k]["avg_us"])
+        b = kernel_bytes(sc, dom, entries, cw)
         # launch duration: back-to-back launches between two HIP events for the broad phase (a pair of
         # events per launch adds its dispatch and completion signalling), the per-launch timers else
         dur_us, timing = per_kernel[dom]["avg_us"], "HIP events around each launch of 40 eager steps"
@@ -344,7 +350,7 @@ def main() -> int:
                 "valu_issue_frac": kernel_valu_issue(dom) if profiled else None}
         for rr in runs.values():
             for k, v in rr.get("kernels", {}).items():
-                kb = kernel_bytes(sc, k, rr.get("skin", {}).get("list_entries"))
+                kb = kernel_bytes(sc, k, rr.get("skin", {}).get("list_entries"), rr.get("cand_written"))
                 if kb:
                     v["GBs"] = round(kb / (v["avg_us"] * 1e-6) / 1e9, 1)
     cpu = cpu1 = None
@@ -395,10 +401,29 @@ def run_cpu_baseline_actomyosin(sc, step0: int, budget_s: float = 12.0, openmp: 
         if el > budget_s or steps >= 200:
             break
     form = f"OpenMP build, {threads} threads" if openmp else "single thread"
-    return {"value": sc.fs.n * steps / el, "unit": "rod-steps/s (filament rods)", "cores": threads, "kind": "port",
+    rods = int((sc.turn.filState[: sc.fs.n] >= 0).sum()) if getattr(sc, "turn", None) is not None else sc.fs.n
+    return {"value": rods * steps / el, "unit": "rod-steps/s (filament rods)", "cores": threads, "kind": "port",
             "sample": f"oracle C restatement ({form}), the GPU's state at the start of its timed region "
-                      f"({sc.fs.n} filament rods, {sc.mot.nM} motor heads, {sc.xl.nLinks} link slots), steps "
-                      f"{step0}..{step0 + steps - 1}, full actomyosin step incl. grid: {steps} steps in {el:.1f} s"}
+                      f"({rods} filament rods, {sc.mot.nM} motor heads, {sc.xl.nLinks} link slots), steps "
+                      f"{step0}..{step0 + steps - 1}, full actomyosin step incl. grid"
+                      + (" and turnover" if getattr(sc, "turn", None) is not None else "")
+                      + f": {steps} steps in {el:.1f} s"}
+
+
+def actomyosin_turnover(sc, scenes):
+    """The workload's turnover: barbed-end growth at 10 subunits/s at every chain's barbed tip, pointed-end
+    depolymerisation (nucleotide-dependent off-rates, death below 3 monomers) with dead-slot reuse, aging,
+    cofilin severing (dissolve ratio 0.8), splits at 64 monomers into 10% spare slots, every 100 steps.
+    Ragged ends, as in a network that has been turning over: barbed tips hold 33-63 monomers and pointed
+    tips 3-36, uniformly, so splits and deaths fall inside the timed steps."""
+    t = scenes.with_turnover(sc, spare=sc.fs.n // 10, growth_rate=10.0, cofilin_ratio=0.8)
+    ts = t.turn
+    g = np.random.default_rng(2024)
+    tips = np.flatnonzero(ts.seedNode >= 0)
+    ts.monomerCount[tips] = g.integers(33, 64, tips.size)
+    pointed = np.flatnonzero((t.fs.end1NbrSlot < 0) & (ts.filState >= 0) & (ts.seedNode < 0))
+    ts.monomerCount[pointed] = g.integers(3, 37, pointed.size)
+    return t
 
 
 def run_actomyosin(args, log) -> int:
@@ -422,9 +447,12 @@ def run_actomyosin(args, log) -> int:
                             eta=args.eta)
     if args.formation > 0:
         sc.p.update(scenes.formation_params(box, 0.1, sc.fs.n, False, grab=0.04, interval=args.formation))
-    n = sc.fs.n
-    log(f"actomyosin scene: {n} filament rods, {sc.mot.nM} heads, {sc.mini.nBb} backbones, {sc.xl.nLinks} links "
-        f"({time.perf_counter() - t_build:.1f} s)")
+    n_rods = sc.fs.n
+    if args.turnover:
+        sc = actomyosin_turnover(sc, scenes)
+    n = n_rods  # the metric counts the network's filament rods (spare slots excluded)
+    log(f"actomyosin scene: {n} filament rods (+{sc.fs.n - n_rods} free slots), {sc.mot.nM} heads, {sc.mini.nBb} "
+        f"backbones, {sc.xl.nLinks} links ({time.perf_counter() - t_build:.1f} s)")
     ctx = Context(0)
     net = Network(ctx, sc.copy(), keep_forces=False)
     net.set_overlap(bool(args.overlap))
@@ -443,6 +471,8 @@ def run_actomyosin(args, log) -> int:
     start.mot, start.dim, start.mini = sc.mot.copy(), sc.dim.copy(), sc.mini.copy()
     net.pull(start.fs, start.xl)
     net.pull_motors(start.mot, start.mini)
+    if start.turn is not None:
+        net.pull_turnover(start.turn, start.fs)
     links_start = int((start.xl.linkState[: start.xl.nLinks] >= 0).sum())
     bound_start = int((start.mot.boundSeg >= 0).sum())
     formed0 = net.form_stats().get("formed_total", 0) if args.formation > 0 else 0
@@ -450,6 +480,14 @@ def run_actomyosin(args, log) -> int:
     r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "timed_steps": [t0, t0 + args.steps - 1]}
     if args.formation > 0:
         r["links_formed_timed"] = net.form_stats()["formed_total"] - formed0
+    if start.turn is not None:
+        tend = net.pull_turnover(start.turn.copy())
+        r["turnover"] = {"cadences_timed": int(tend.stats[6] - start.turn.stats[6]),
+                         "monomers_grown": int(tend.ledger[0] - start.turn.ledger[0]),
+                         "monomers_returned": int(tend.ledger[1] - start.turn.ledger[1]),
+                         "segments_active_start": int((start.turn.filState >= 0).sum()),
+                         "segments_active_end": int((tend.filState >= 0).sum()),
+                         "free_actin_uM_end": round(float(tend.pool[0]), 4)}
     kernels = profile(net, t0 + args.steps, args.profile_steps) if args.profile_steps else {}
     end = sc.copy()
     end.mot, end.mini = sc.mot.copy(), sc.mini.copy()
@@ -493,6 +531,10 @@ def run_actomyosin(args, log) -> int:
     }
     if "links_formed_timed" in r:
         out["links_formed_timed"] = r["links_formed_timed"]
+    if "turnover" in r:
+        out["turnover"] = r["turnover"]
+        out["config"]["workload"] += (", turnover every 100 steps (barbed growth 10 subunits/s, pointed-end "
+                                      "depolymerisation, aging, cofilin severing, splits into 10% free slots)")
     if kernels:
         out["kernels"] = {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                           for k, v in kernels.items()}
