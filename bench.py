@@ -184,9 +184,9 @@ def main() -> int:
     ap.add_argument("--train-iters", type=int, default=400)
     ap.add_argument("--horizon", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--n-fil", type=int, default=20000)
-    ap.add_argument("--n-links", type=int, default=50000)
-    ap.add_argument("--box", type=float, default=20.0)
+    ap.add_argument("--n-fil", type=int, default=None, help="filaments (network 20000, actomyosin 50000)")
+    ap.add_argument("--n-links", type=int, default=None, help="crosslinker slots (network 50000, actomyosin 100000)")
+    ap.add_argument("--box", type=float, default=None, help="box edge, um (network 20, actomyosin 30)")
     ap.add_argument("--workload", choices=("network", "actomyosin"), default="network",
                     help="network: the crosslinked network (the headline); actomyosin: the full actomyosin "
                          "network (50k filaments, 2k minifilaments of 16 heads, 100k crosslinkers, 30 um box)")
@@ -225,8 +225,15 @@ def main() -> int:
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+    acto = args.workload == "actomyosin"
     if args.eta is None:
-        args.eta = 1.0 if args.workload == "actomyosin" else 0.1
+        args.eta = 1.0 if acto else 0.1
+    if args.n_fil is None:
+        args.n_fil = 50000 if acto else 20000
+    if args.n_links is None:
+        args.n_links = 100000 if acto else 50000
+    if args.box is None:
+        args.box = 30.0 if acto else 20.0
     if args.batch:
         return run_batch(args, log)
     if args.workload == "actomyosin":
@@ -328,7 +335,7 @@ def main() -> int:
         same = (np.array_equal(runs["rebuild"]["digest"][0], runs["learned"]["digest"][0])
                 and runs["rebuild"]["digest"][1] == runs["learned"]["digest"][1])
         assert same, "learned-skin candidate slices differ from the per-step rebuild"
-    roof = None
+    roof = roof_rod = None
     per_kernel = head.get("kernels", {})
     entries = head.get("skin", {}).get("list_entries")
     cw = head.get("cand_written")
@@ -348,6 +355,13 @@ def main() -> int:
                 "traffic": kernel_traffic(dom) if profiled else None, "bytes_per_launch": b,
                 "duration_us": round(dur_us, 2), "timing": timing,
                 "valu_issue_frac": kernel_valu_issue(dom) if profiled else None}
+        if "k_rod" in per_kernel:  # the gather/integrate kernel (chain and link gathers + integrate)
+            kr = kernel_bytes(sc, "k_rod") / (per_kernel["k_rod"]["avg_us"] * 1e-6) / 1e9
+            roof_rod = {"kernel": "k_rod", "achieved": round(kr, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(kr / HBM_PEAK_GBS, 4), "bytes_per_launch": kernel_bytes(sc, "k_rod"),
+                        "duration_us": round(per_kernel["k_rod"]["avg_us"], 2),
+                        "traffic": kernel_traffic("k_rod") if profiled else None,
+                        "note": "duration: HIP events around each launch, which add dispatch and completion"}
         for rr in runs.values():
             for k, v in rr.get("kernels", {}).items():
                 kb = kernel_bytes(sc, k, rr.get("skin", {}).get("list_entries"), rr.get("cand_written"))
@@ -371,7 +385,8 @@ def main() -> int:
                                   if args.formation > 0 else ""),
                    "rods": n, "links": sc.xl.nLinks, "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
                    "broad_phase": bp},
-        "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu1,
+        "roofline": roof, "roofline_gather_integrate": roof_rod, "cpu_baseline": cpu,
+        "cpu_baseline_single_thread": cpu1,
         "links_alive_start": head["links_alive_start"], "links_alive_end": head["links_alive_end"],
     }
     if "rebuild" in runs and "learned" in runs:
@@ -439,9 +454,7 @@ def run_actomyosin(args, log) -> int:
     from softbox_amd.context import Context
     from softbox_amd.network import Network
 
-    n_fil = args.n_fil if args.n_fil != 20000 else 50000
-    n_links = args.n_links if args.n_links != 50000 else 100000
-    box = args.box if args.box != 20.0 else 30.0
+    n_fil, n_links, box = args.n_fil, args.n_links, args.box
     t_build = time.perf_counter()
     sc = scenes.contractile(n_fil=n_fil, segs=10, box=box, n_mini=max(1, n_fil // 25), n_links=n_links, seed=0,
                             eta=args.eta)
@@ -495,7 +508,7 @@ def run_actomyosin(args, log) -> int:
     net.pull_motors(end.mot, end.mini)
     assert np.isfinite(end.fs.coord).all() and np.isfinite(end.mot.body.coord).all(), "non-finite pose"
     net.close()
-    roof = None
+    roof = roof_rod = None
     if kernels:
         known = {k: kernel_bytes(sc, k) for k in kernels}
         dom = max((k for k in kernels if known[k]), key=lambda k: kernels[k]["avg_us"])
@@ -508,6 +521,13 @@ def run_actomyosin(args, log) -> int:
         for k, v in kernels.items():
             if known[k]:
                 v["GBs"] = round(known[k] / (v["avg_us"] * 1e-6) / 1e9, 1)
+        if known.get("k_rod"):  # the gather/integrate kernel (chain, link and motor gathers + integrate)
+            kr = known["k_rod"] / (kernels["k_rod"]["avg_us"] * 1e-6) / 1e9
+            roof_rod = {"kernel": "k_rod", "achieved": round(kr, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(kr / HBM_PEAK_GBS, 4), "bytes_per_launch": known["k_rod"],
+                        "duration_us": round(kernels["k_rod"]["avg_us"], 2),
+                        "note": "bytes: the filament formula (the motor gather's reads left out); duration: HIP "
+                                "events around each launch, which add dispatch and completion"}
     cpu = cpu1 = None
     if not args.no_cpu:
         cpu = run_cpu_baseline_actomyosin(start, t0, openmp=True)
@@ -524,7 +544,8 @@ def run_actomyosin(args, log) -> int:
                    "rods": n, "motor_heads": sc.mot.nM, "backbones": sc.mini.nBb, "links": sc.xl.nLinks,
                    "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
                    "shape": shape_of(args, tuned) if (args.shape or tuned) else None},
-        "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu1,
+        "roofline": roof, "roofline_gather_integrate": roof_rod, "cpu_baseline": cpu,
+        "cpu_baseline_single_thread": cpu1,
         "links_alive_start": links_start, "links_alive_end": int((end.xl.linkState[: end.xl.nLinks] >= 0).sum()),
         "heads_bound_start": bound_start, "heads_bound_end": int((end.mot.boundSeg >= 0).sum()),
         "timed_steps": r["timed_steps"],
