@@ -287,3 +287,26 @@ def test_actomyosin_with_turnover_matches_oracle(ctx):
         t += chunk
         _compare_net(net, orc, sc, f"actomyosin after {t}", motors=True)
     net.close()
+
+
+def test_frames_of_a_gpu_run_equal_the_oracles(ctx, tmp_path):
+    """Frame output of the network with turnover, written at a cadence from the GPU's state, is byte-identical
+    to the frames written from the oracle's state at the same steps."""
+    from softbox_amd.frames import FrameWriter, write_network_frame
+    from softbox_amd.network import Network
+
+    sc = _scene(seed=9)
+    orc = ffi.OracleNet(sc)
+    net = Network(ctx, sc)
+    wg = FrameWriter(str(tmp_path / "gpu"), sc.box, sc.box, sc.box)
+    wo = FrameWriter(str(tmp_path / "orc"), sc.box, sc.box, sc.box)
+    dt = float(sc.p["params"][0])
+    t = 0
+    for chunk in (10, 10, 15):
+        net.step(t, chunk)
+        orc.step(t, chunk)
+        t += chunk
+        a = write_network_frame(wg, net, sc, t * dt)
+        b = wo.write_frame(orc.fs, t * dt, orc.turn.ts)
+        assert open(a).read() == open(b).read(), f"frame at step {t}"
+    net.close()
