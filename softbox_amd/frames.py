@@ -101,7 +101,7 @@ class FrameWriter:
 def write_network_frame(writer: FrameWriter, net, scene, t: float) -> str:
     """Pull a network's current pose (and turnover state) and write one frame."""
     fs = scene.fs.copy()
-    net.pull(fs)
+    net.pull(fs, shape=True)
     turn = None
     if getattr(net, "turn", None) is not None:
         turn = net.pull_turnover(scene.turn.copy())

@@ -150,13 +150,17 @@ class Network:
             out.append(a)
         return out
 
-    def pull(self, fs=None, xl=None):
-        """Synchronise and copy state back into host stores (pose published first)."""
+    SHAPE = ("segLength", "bTransGam", "bRotGam", "end1NbrSlot", "end1NbrSide", "end2NbrSlot", "end2NbrSide")
+
+    def pull(self, fs=None, xl=None, shape=False):
+        """Synchronise and copy state back into host stores (pose published first); with shape, also the
+        segment lengths, drag and chain links that turnover rewrites on the device."""
         self.publish()
         self.ctx.sync()
         if fs is not None:
             self.fil.pull(fs, ("coord", "uVec", "yVec", "zVec", "end1", "end2")
-                          + (("forceSum", "torqueSum") if self.fil.forceSum is not None else ()))
+                          + (("forceSum", "torqueSum") if self.fil.forceSum is not None else ())
+                          + (self.SHAPE if shape else ()))
         if xl is not None and self.xl is not None:
             self.xl.pull(xl)
         return fs, xl

@@ -235,8 +235,8 @@ def _compare_net(net, orc, sc, what, motors=False):
 
     fs = sc.fs.copy()
     xl = sc.xl.copy() if sc.xl is not None else None
-    net.pull(fs, xl)
-    for k in ("coord", "uVec", "yVec", "zVec", "end1", "end2", "forceSum", "torqueSum"):
+    net.pull(fs, xl, shape=True)
+    for k in ("coord", "uVec", "yVec", "zVec", "end1", "end2", "forceSum", "torqueSum") + net.SHAPE:
         assert_same(getattr(fs, k), getattr(orc.fs, k), f"{what}: {k}")
     if xl is not None:
         for k in ("linkState", "linkFilA", "linkFilB", "strainHist", "strainPlace", "xlinkData", "activeLinkCount"):
