@@ -330,6 +330,27 @@ int sbx_mini_tether(sbx_ctx *c, const float *mCoord, const float *mUVec, const f
 /* MiniFilamentSystem.backboneGather (MiniFilamentSystem.java:137-155) */
 int sbx_backbone_gather(sbx_ctx *c, const int *bbDimerOffsets, const int *bbDimerList, const float *miniData,
                         float *bbForceSum, float *bbTorqueSum, int nBb);
+/* NodeSystem.tether (NodeSystem.java:48-127): attachKey = [node(nA) | motor(nA)], radial 3*nA planar body-frame
+ * offsets, attachCoeffK signed (> 0 dimer: force at rod end1 with torque; < 0 singlet: at the rod centre),
+ * nodeData 6*nA node-side reactions, nodeParams[0] = dt.  The per-node sum is sbx_backbone_gather over the CSR
+ * of attachNode (NodeStore.java:27-33). */
+int sbx_node_tether(sbx_ctx *c, const float *mCoord, const float *mUVec, const float *mSegLength,
+                    const float *mBTransGam, float *mForceSum, float *mTorqueSum, const float *nCoord,
+                    const float *nUVec, const float *nYVec, const float *nInvTransY, const int *attachKey,
+                    const float *radial, const float *attachCoeffK, float *nodeData, const float *nodeParams, int nMB,
+                    int nN, int nA);
+/* NodeSystem.tetherBoundStiffen (NodeSystem.java:170-231): a motor whose head is bound uses nodeParams[1] */
+int sbx_node_tether_bound_stiffen(sbx_ctx *c, const float *mCoord, const float *mUVec, const float *mSegLength,
+                                  const float *mBTransGam, float *mForceSum, float *mTorqueSum, const float *nCoord,
+                                  const float *nUVec, const float *nYVec, const float *nInvTransY,
+                                  const int *attachKey, const float *radial, const float *attachCoeffK,
+                                  const int *boundSeg, float *nodeData, const float *nodeParams, int nMB, int nN,
+                                  int nA);
+/* NodeSystem.xbridgeInject (NodeSystem.java:141-166): adds the bound heads' cross-bridge force (and its torque
+ * at the world-frame surface offset) to nodeData */
+int sbx_node_xbridge_inject(sbx_ctx *c, const float *bondData, const float *nUVec, const float *nYVec,
+                            const int *attachKey, const float *radial, const float *attachCoeffK,
+                            const int *boundSeg, float *nodeData, int nN, int nA);
 
 /* ------------------------------------------------------------------ whole network step
  * Replaces the filament block of DenseContractileHarness.buildPlan / cpuStep (DenseContractileHarness.java:
@@ -414,6 +435,10 @@ typedef struct sbx_motors {
     float *bbForceSum, *bbTorqueSum;           /* optional: NULL keeps backbone F/T in registers */
     int *headBackboneSlot, *tetherMotor;
     float *attachAxial, *miniData;
+    /* node shell (sbx_motor_params.nodeMode != 0): the backbone bodies are node spheres and each tethered item
+     * is an attachment: headBackboneSlot = attachNode, tetherMotor = attachMotor, miniData = nodeData,
+     * bbInvDragY = nodeInvTransY (nBb entries); attachAxial unused */
+    float *radial, *attachCoeffK;
 } sbx_motors;
 
 typedef struct sbx_motor_params {
@@ -426,6 +451,8 @@ typedef struct sbx_motor_params {
     float miniParams[3];    /* MiniFilamentStore.miniParams */
     float bbBodyParams[2];  /* MiniFilamentStore.bbBodyParams */
     int seedMotor, seedBackbone;
+    int nodeMode;           /* 0 MiniFilamentSystem.tether; 1 NodeSystem.tether (miniParams[0] = dt);
+                               2 NodeSystem.tetherBoundStiffen (miniParams[1] = boundCoeff) */
 } sbx_motor_params;
 
 int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_params *p);

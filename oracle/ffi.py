@@ -296,7 +296,8 @@ class SboMotors(C.Structure):
                              "bbInvDragY", "headBackboneSlot", "tetherMotor", "attachAxial", "miniData",
                              "bbDimerCount", "bbDimerOffsets", "bbDimerList")]
         + [("miniParams", C.c_float * 3), ("bbBodyParams", C.c_float * 2), ("seedMotor", i32),
-           ("seedBackbone", i32), ("ownerStore", vp), ("ownerSlot", vp)]
+           ("seedBackbone", i32), ("ownerStore", vp), ("ownerSlot", vp), ("radial", vp), ("attachCoeffK", vp),
+           ("nodeMode", i32)]
     )
 
 
@@ -364,6 +365,10 @@ class OracleContractile(OracleNet):
         m.miniData = P(mini.miniData).value
         _fill(m.miniParams, mini.miniParams)
         _fill(m.bbBodyParams, mini.bbBodyParams)
+        if hasattr(mini, "radial"):        # node shell: nodes in the backbone slots (NodeStore)
+            m.radial = P(mini.radial).value
+            m.attachCoeffK = P(mini.attachCoeffK).value
+            m.nodeMode = int(mini.mode)
         m.seedMotor = int(self.p["seedMotor"])
         m.seedBackbone = int(self.p["seedBackbone"])
         self.m = m

@@ -171,6 +171,27 @@ void sbo_mini_tether(int nMB, int nBb, int nD, const float *mCoord, const float 
 void sbo_backbone_gather(int nBb, const int *bbDimerOffsets, const int *bbDimerList, const float *miniData,
                          float *bbForceSum, float *bbTorqueSum);                          /* :137-155 */
 
+/* ---- protein-node shell (softbox_oracle_node.c); attachKey = [node(nA) | motor(nA)] ---- */
+void sbo_node_tether_keys(int nMB, int nN, int nA, const float *mCoord, const float *mUVec, const float *mSegLength,
+                          const float *mBTransGam, float *mForceSum, float *mTorqueSum, const float *nCoord,
+                          const float *nUVec, const float *nYVec, const float *nInvTransY, const int *attachNode,
+                          const int *attachMotor, const float *radial, const float *attachCoeffK, const int *boundSeg,
+                          float *nodeData, const float *nodeParams);
+void sbo_node_tether(int nMB, int nN, int nA, const float *mCoord, const float *mUVec, const float *mSegLength,
+                     const float *mBTransGam, float *mForceSum, float *mTorqueSum, const float *nCoord,
+                     const float *nUVec, const float *nYVec, const float *nInvTransY, const int *attachKey,
+                     const float *radial, const float *attachCoeffK, float *nodeData,
+                     const float *nodeParams);                                            /* NodeSystem.java:48-127 */
+void sbo_node_tether_bound_stiffen(int nMB, int nN, int nA, const float *mCoord, const float *mUVec,
+                                   const float *mSegLength, const float *mBTransGam, float *mForceSum,
+                                   float *mTorqueSum, const float *nCoord, const float *nUVec, const float *nYVec,
+                                   const float *nInvTransY, const int *attachKey, const float *radial,
+                                   const float *attachCoeffK, const int *boundSeg, float *nodeData,
+                                   const float *nodeParams);                              /* :170-231 */
+void sbo_node_xbridge_inject(int nN, int nA, const float *bondData, const float *nUVec, const float *nYVec,
+                             const int *attachKey, const float *radial, const float *attachCoeffK,
+                             const int *boundSeg, float *nodeData);                       /* :141-166 */
+
 /* ---- one whole network step (DenseContractileHarness.java:350-429 filament block, no motors) ---- */
 typedef struct sbo_net {
     int nRod;
@@ -354,6 +375,11 @@ typedef struct sbo_motors {
     float miniParams[3], bbBodyParams[2];
     int32_t seedMotor, seedBackbone;
     int *ownerStore, *ownerSlot;              /* body view back-pointers, cap = nRod + nM */
+    /* node shell: nodeMode 1 (NodeSystem.tether) or 2 (tetherBoundStiffen, boundCoeff miniParams[1]) makes
+     * the backbone bodies node spheres and each tethered item an attachment (headBackboneSlot = node,
+     * tetherMotor = motor, miniData = nodeData, bbInvDragY = nodeInvTransY); 0 keeps the minifilament tether */
+    float *radial, *attachCoeffK;
+    int nodeMode;
 } sbo_motors;
 
 void sbo_contractile_step(sbo_net *net, sbo_motors *mo, int32_t step);

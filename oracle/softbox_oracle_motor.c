@@ -707,9 +707,15 @@ void sbo_contractile_step(sbo_net *net, sbo_motors *mo, int32_t step) {
                      mo->nucleotideState, mo->jointParams);
     sbo_dimer_couple(nB, mo->nD, mo->coord, mo->uVec, mo->segLength, mo->bTransGam, mo->bRotGam, mo->forceSum,
                      mo->torqueSum, mo->motorA, mo->motorB, mo->parallel, mo->dimerParams, mo->boundSeg);
-    sbo_mini_tether(nB, nBb, mo->nMD, mo->coord, mo->uVec, mo->segLength, mo->bTransGam, mo->bRotGam, mo->forceSum,
-                    mo->torqueSum, mo->bbCoord, mo->bbUVec, mo->bbInvDragY, mo->headBackboneSlot, mo->tetherMotor,
-                    mo->attachAxial, mo->miniData, mo->miniParams);
+    if (mo->nodeMode)   /* V2OneXHarness.java:399-400 */
+        sbo_node_tether_keys(nB, nBb, mo->nMD, mo->coord, mo->uVec, mo->segLength, mo->bTransGam, mo->forceSum,
+                             mo->torqueSum, mo->bbCoord, mo->bbUVec, mo->bbYVec, mo->bbInvDragY, mo->headBackboneSlot,
+                             mo->tetherMotor, mo->radial, mo->attachCoeffK, mo->nodeMode == 2 ? mo->boundSeg : NULL,
+                             mo->miniData, mo->miniParams);
+    else
+        sbo_mini_tether(nB, nBb, mo->nMD, mo->coord, mo->uVec, mo->segLength, mo->bTransGam, mo->bRotGam,
+                        mo->forceSum, mo->torqueSum, mo->bbCoord, mo->bbUVec, mo->bbInvDragY, mo->headBackboneSlot,
+                        mo->tetherMotor, mo->attachAxial, mo->miniData, mo->miniParams);
     sbo_csr_build(nBb, mo->nMD, mo->headBackboneSlot, mo->bbDimerCount, mo->bbDimerOffsets, mo->bbDimerList);
     sbo_backbone_gather(nBb, mo->bbDimerOffsets, mo->bbDimerList, mo->miniData, mo->bbForceSum, mo->bbTorqueSum);
     sbo_bond_forces(nM, n, mo->coord, mo->uVec, mo->yVec, mo->bRotGam, net->coord, net->uVec, net->yVec, net->bRotGam,

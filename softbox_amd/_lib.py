@@ -97,7 +97,7 @@ class SbxMotors(C.Structure):
         + [(k, vp) for k in ("bbCoord", "bbUVec", "bbYVec", "bbZVec", "bbEnd1", "bbEnd2", "bbSegLength",
                              "bbTransGam", "bbRotGam", "bbBrownTransScale", "bbBrownRotScale", "bbInvDragY",
                              "bbForceSum", "bbTorqueSum", "headBackboneSlot", "tetherMotor", "attachAxial",
-                             "miniData")]
+                             "miniData", "radial", "attachCoeffK")]
     )
 
 
@@ -113,6 +113,7 @@ class SbxMotorParams(C.Structure):
         ("bbBodyParams", C.c_float * 2),
         ("seedMotor", i32),
         ("seedBackbone", i32),
+        ("nodeMode", i32),
     ]
 
 
@@ -211,6 +212,9 @@ _SIGS = {
     "sbx_dimer_couple": [vp] + [vp] * 10 + [f32p, vp, i32, i32],
     "sbx_mini_tether": [vp] + [vp] * 14 + [f32p, i32, i32, i32],
     "sbx_backbone_gather": [vp] + [vp] * 5 + [i32],
+    "sbx_node_tether": [vp] + [vp] * 14 + [f32p, i32, i32, i32],
+    "sbx_node_tether_bound_stiffen": [vp] + [vp] * 15 + [f32p, i32, i32, i32],
+    "sbx_node_xbridge_inject": [vp] + [vp] * 8 + [i32, i32],
     "sbx_net_create": [vp, C.POINTER(SbxFilaments), C.POINTER(SbxCrosslinks), C.POINTER(SbxNetParams), C.POINTER(vp)],
     "sbx_net_destroy": [vp],
     "sbx_net_attach_motors": [vp, C.POINTER(SbxMotors), C.POINTER(SbxMotorParams)],

@@ -472,6 +472,71 @@ struct MiniParams {
     double dt, fracMove, align;
 };
 
+// NodeSystem.tether body for one attachment (NodeSystem.java:66-126; tetherBoundStiffen :185-230 differs only
+// in the coefficient the caller passes): the rod's end1 pulled toward the node-frame surface point
+// c + ru u + ry y + rz (u x y) by a damping-limited spring; a dimer (atEnd1) also takes the end1 torque on the
+// rod and the torque of the reaction at the body-frame offset (ru, ry, rz) on the node.
+__device__ __forceinline__ void node_tether_one(int a, int nMB, int nN, int nA, int k, int rod, bool atEnd1,
+                                                double coeff, double dt, const float *__restrict__ mC,
+                                                const float *__restrict__ mU, const float *__restrict__ mLen,
+                                                const float *__restrict__ mBTG, float *__restrict__ mF,
+                                                float *__restrict__ mT, const float *__restrict__ nC,
+                                                const float *__restrict__ nU, const float *__restrict__ nY,
+                                                const float *__restrict__ nInv, const float *__restrict__ radial,
+                                                float *__restrict__ nodeData) {
+    double ncx = nC[k], ncy = nC[nN + k], ncz = nC[2 * nN + k];
+    double nux = nU[k], nuy = nU[nN + k], nuz = nU[2 * nN + k];
+    double nyx = nY[k], nyy = nY[nN + k], nyz = nY[2 * nN + k];
+    double nzx = nuy * nyz - nuz * nyy, nzy = nuz * nyx - nux * nyz, nzz = nux * nyy - nuy * nyx;
+    double ru = radial[a], ry = radial[nA + a], rz = radial[2 * nA + a];
+    double pax = ncx + ru * nux + ry * nyx + rz * nzx;
+    double pay = ncy + ru * nuy + ry * nyy + rz * nzy;
+    double paz = ncz + ru * nuz + ry * nyz + rz * nzz;
+    double rcx = mC[rod], rcy = mC[nMB + rod], rcz = mC[2 * nMB + rod];
+    double rux = mU[rod], ruy = mU[nMB + rod], ruz = mU[2 * nMB + rod];
+    double rlen = mLen[rod];
+    double re1x = rcx - 0.5 * rlen * rux, re1y = rcy - 0.5 * rlen * ruy, re1z = rcz - 0.5 * rlen * ruz;
+    double fnx = 0, fny = 0, fnz = 0, tnx = 0, tny = 0, tnz = 0;
+    double dx = pax - re1x, dy = pay - re1y, dz = paz - re1z;
+    double strain = sqrt(dx * dx + dy * dy + dz * dz);
+    if (strain > 0.0) {
+        double inv = 1.0 / strain;
+        double lx = dx * inv, ly = dy * inv, lz = dz * inv;
+        double denom = dt * (1.0 / mBTG[nMB + rod] + nInv[k]);
+        double forceMag = (denom > 0.0) ? (coeff * 1.0e-6 * strain / denom) : 0.0;
+        double Fx = forceMag * lx, Fy = forceMag * ly, Fz = forceMag * lz;
+        mF[rod] = (float)(mF[rod] + Fx);
+        mF[nMB + rod] = (float)(mF[nMB + rod] + Fy);
+        mF[2 * nMB + rod] = (float)(mF[2 * nMB + rod] + Fz);
+        if (atEnd1) {
+            double Rx = -0.5 * rlen * rux * 1.0e-6, Ry = -0.5 * rlen * ruy * 1.0e-6, Rz = -0.5 * rlen * ruz * 1.0e-6;
+            mT[rod] = (float)(mT[rod] + (Ry * Fz - Rz * Fy));
+            mT[nMB + rod] = (float)(mT[nMB + rod] + (Rz * Fx - Rx * Fz));
+            mT[2 * nMB + rod] = (float)(mT[2 * nMB + rod] + (Rx * Fy - Ry * Fx));
+        }
+        fnx = -Fx; fny = -Fy; fnz = -Fz;
+        if (atEnd1) {
+            double Rnx = ru * 1.0e-6, Rny = ry * 1.0e-6, Rnz = rz * 1.0e-6;
+            tnx = Rny * fnz - Rnz * fny;
+            tny = Rnz * fnx - Rnx * fnz;
+            tnz = Rnx * fny - Rny * fnx;
+        }
+    }
+    float *o = nodeData + (size_t)MINI_STRIDE * a;
+    o[0] = (float)fnx; o[1] = (float)fny; o[2] = (float)fnz;
+    o[3] = (float)tnx; o[4] = (float)tny; o[5] = (float)tnz;
+}
+
+// the coefficient of one attachment: |attachCoeffK| (sign = atEnd1), or boundCoeff for a bound motor under
+// tetherBoundStiffen (boundSeg non-null; NodeSystem.java:188-191)
+__device__ __forceinline__ double node_coeff(float cs, int m, const int *__restrict__ boundSeg, double boundCoeff,
+                                             bool &atEnd1) {
+    atEnd1 = cs > 0.0f;
+    double coeff = atEnd1 ? (double)cs : -(double)cs;
+    if (boundSeg && boundSeg[m] >= 0) coeff = boundCoeff;
+    return coeff;
+}
+
 // MiniFilamentSystem.tether body for one tethered dimer (MiniFilamentSystem.java:76-130)
 __device__ __forceinline__ void tether_one(int d, int nMB, int nBb, int bb, int rodA, double ax,
                                            const float *__restrict__ mC, const float *__restrict__ mU,

@@ -241,6 +241,58 @@ __global__ void __launch_bounds__(T) k_tether(const float *__restrict__ mC, cons
                mp);
 }
 
+// attachKey = [node(nA) | motor(nA)]; boundSeg null = NodeSystem.tether, else tetherBoundStiffen
+__global__ void __launch_bounds__(T) k_node_tether(const float *__restrict__ mC, const float *__restrict__ mU,
+                                                   const float *__restrict__ mLen, const float *__restrict__ mBTG,
+                                                   float *__restrict__ mF, float *__restrict__ mT,
+                                                   const float *__restrict__ nC, const float *__restrict__ nU,
+                                                   const float *__restrict__ nY, const float *__restrict__ nInv,
+                                                   const int *__restrict__ attachKey, const float *__restrict__ radial,
+                                                   const float *__restrict__ coeffK, const int *__restrict__ boundSeg,
+                                                   float *__restrict__ nodeData, double dt, double boundCoeff,
+                                                   int nMB, int nN, int nA) {
+    int a = blockIdx.x * T + threadIdx.x;
+    if (a >= nA) return;
+    int m = attachKey[nA + a];
+    bool atEnd1;
+    double coeff = node_coeff(coeffK[a], m, boundSeg, boundCoeff, atEnd1);
+    node_tether_one(a, nMB, nN, nA, attachKey[a], 3 * m, atEnd1, coeff, dt, mC, mU, mLen, mBTG, mF, mT, nC, nU, nY,
+                    nInv, radial, nodeData);
+}
+
+// NodeSystem.xbridgeInject (NodeSystem.java:141-166)
+__global__ void __launch_bounds__(T) k_node_xbridge(const float *__restrict__ bondData, const float *__restrict__ nU,
+                                                    const float *__restrict__ nY, const int *__restrict__ attachKey,
+                                                    const float *__restrict__ radial,
+                                                    const float *__restrict__ coeffK, const int *__restrict__ boundSeg,
+                                                    float *__restrict__ nodeData, int nN, int nA) {
+    int a = blockIdx.x * T + threadIdx.x;
+    if (a >= nA) return;
+    int k = attachKey[a], m = attachKey[nA + a];
+    bool dimer = coeffK[a] > 0.0f;
+    double nux = nU[k], nuy = nU[nN + k], nuz = nU[2 * nN + k];
+    double nyx = nY[k], nyy = nY[nN + k], nyz = nY[2 * nN + k];
+    double nzx = nuy * nyz - nuz * nyy, nzy = nuz * nyx - nux * nyz, nzz = nux * nyy - nuy * nyx;
+    double ru = radial[a], ry = radial[nA + a], rz = radial[2 * nA + a];
+    double Rx = ru * nux + ry * nyx + rz * nzx;
+    double Ry = ru * nuy + ry * nyy + rz * nzy;
+    double Rz = ru * nuz + ry * nyz + rz * nzz;
+    double fx = 0, fy = 0, fz = 0;
+    if (boundSeg[m] >= 0) {
+        const float *b = bondData + (size_t)m * XB_STRIDE;
+        fx += b[0]; fy += b[1]; fz += b[2];
+    }
+    if (dimer && boundSeg[m + 1] >= 0) {
+        const float *b = bondData + (size_t)(m + 1) * XB_STRIDE;
+        fx += b[0]; fy += b[1]; fz += b[2];
+    }
+    double Rmx = Rx * 1.0e-6, Rmy = Ry * 1.0e-6, Rmz = Rz * 1.0e-6;
+    double Tx = Rmy * fz - Rmz * fy, Ty = Rmz * fx - Rmx * fz, Tz = Rmx * fy - Rmy * fx;
+    float *o = nodeData + (size_t)MINI_STRIDE * a;
+    o[0] = (float)(o[0] + fx); o[1] = (float)(o[1] + fy); o[2] = (float)(o[2] + fz);
+    o[3] = (float)(o[3] + Tx); o[4] = (float)(o[4] + Ty); o[5] = (float)(o[5] + Tz);
+}
+
 }  // namespace
 
 #define CHK(c, name) return check_launch(c, name)
@@ -435,6 +487,63 @@ int sbx_mini_tether(sbx_ctx *c, const float *mCoord, const float *mUVec, const f
                        mBTransGam, mBRotGam, mForceSum, mTorqueSum, bbCoord, bbUVec, bbInvDragY, headBackboneSlot,
                        motorA, attachAxial, miniData, mini_params(miniParams), nMB, nBb, nD);
     CHK(c, "k_tether");
+}
+
+static int node_tether_launch(sbx_ctx *c, const char *what, const float *mCoord, const float *mUVec,
+                              const float *mSegLength, const float *mBTransGam, float *mForceSum, float *mTorqueSum,
+                              const float *nCoord, const float *nUVec, const float *nYVec, const float *nInvTransY,
+                              const int *attachKey, const float *radial, const float *attachCoeffK,
+                              const int *boundSeg, float *nodeData, const float *nodeParams, int nMB, int nN, int nA) {
+    SBX_REQUIRE(c, nodeParams && nMB >= 0 && nN >= 0 && nA >= 0, "node_tether: bad params");
+    if (nA == 0) return SBX_OK;
+    SBX_REQUIRE(c, nodeParams[0] != 0.0f, "node_tether: dt must be non-zero");
+    SBX_REQUIRE(c, nN > 0 && nMB > 0, "node_tether: attachments without nodes or motors");
+    SBX_REQUIRE(c, mCoord && mUVec && mSegLength && mBTransGam && mForceSum && mTorqueSum && nCoord && nUVec &&
+                       nYVec && nInvTransY && attachKey && radial && attachCoeffK && nodeData,
+                "node_tether: null array");
+    hipLaunchKernelGGL(k_node_tether, dim3(blocks_for(nA, T)), dim3(T), 0, c->stream, mCoord, mUVec, mSegLength,
+                       mBTransGam, mForceSum, mTorqueSum, nCoord, nUVec, nYVec, nInvTransY, attachKey, radial,
+                       attachCoeffK, boundSeg, nodeData, (double)nodeParams[0],
+                       boundSeg ? (double)nodeParams[1] : 0.0, nMB, nN, nA);
+    CHK(c, what);
+}
+
+int sbx_node_tether(sbx_ctx *c, const float *mCoord, const float *mUVec, const float *mSegLength,
+                    const float *mBTransGam, float *mForceSum, float *mTorqueSum, const float *nCoord,
+                    const float *nUVec, const float *nYVec, const float *nInvTransY, const int *attachKey,
+                    const float *radial, const float *attachCoeffK, float *nodeData, const float *nodeParams, int nMB,
+                    int nN, int nA) {
+    SBX_CHECK_CTX(c);
+    return node_tether_launch(c, "k_node_tether", mCoord, mUVec, mSegLength, mBTransGam, mForceSum, mTorqueSum,
+                              nCoord, nUVec, nYVec, nInvTransY, attachKey, radial, attachCoeffK, nullptr, nodeData,
+                              nodeParams, nMB, nN, nA);
+}
+
+int sbx_node_tether_bound_stiffen(sbx_ctx *c, const float *mCoord, const float *mUVec, const float *mSegLength,
+                                  const float *mBTransGam, float *mForceSum, float *mTorqueSum, const float *nCoord,
+                                  const float *nUVec, const float *nYVec, const float *nInvTransY,
+                                  const int *attachKey, const float *radial, const float *attachCoeffK,
+                                  const int *boundSeg, float *nodeData, const float *nodeParams, int nMB, int nN,
+                                  int nA) {
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, boundSeg || nA == 0, "node_tether_bound_stiffen: null boundSeg");
+    return node_tether_launch(c, "k_node_tether(bound stiffen)", mCoord, mUVec, mSegLength, mBTransGam, mForceSum,
+                              mTorqueSum, nCoord, nUVec, nYVec, nInvTransY, attachKey, radial, attachCoeffK,
+                              boundSeg, nodeData, nodeParams, nMB, nN, nA);
+}
+
+int sbx_node_xbridge_inject(sbx_ctx *c, const float *bondData, const float *nUVec, const float *nYVec,
+                            const int *attachKey, const float *radial, const float *attachCoeffK,
+                            const int *boundSeg, float *nodeData, int nN, int nA) {
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, nN >= 0 && nA >= 0, "node_xbridge_inject: negative counts");
+    if (nA == 0) return SBX_OK;
+    SBX_REQUIRE(c, nN > 0, "node_xbridge_inject: attachments without nodes");
+    SBX_REQUIRE(c, bondData && nUVec && nYVec && attachKey && radial && attachCoeffK && boundSeg && nodeData,
+                "node_xbridge_inject: null array");
+    hipLaunchKernelGGL(k_node_xbridge, dim3(blocks_for(nA, T)), dim3(T), 0, c->stream, bondData, nUVec, nYVec,
+                       attachKey, radial, attachCoeffK, boundSeg, nodeData, nN, nA);
+    CHK(c, "k_node_xbridge");
 }
 
 int sbx_backbone_gather(sbx_ctx *c, const int *bbDimerOffsets, const int *bbDimerList, const float *miniData,

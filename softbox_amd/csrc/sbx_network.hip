@@ -614,9 +614,20 @@ __global__ void __launch_bounds__(256) k_net_dimer(sbx_motors m, DimerParams dp)
               m.forceSum, m.torqueSum, dp);
 }
 
-__global__ void __launch_bounds__(256) k_net_tether(sbx_motors m, MiniParams mp) {
+// the tether of each attachment: MiniFilamentSystem.tether, or with nodeMode NodeSystem.tether (1) /
+// tetherBoundStiffen (2, boundCoeff in miniParams[1]) onto node spheres in the backbone slots
+__global__ void __launch_bounds__(256) k_net_tether(sbx_motors m, MiniParams mp, int nodeMode) {
     int d = blockIdx.x * 256 + threadIdx.x;
     if (d >= m.nMD) return;
+    if (nodeMode) {
+        int mm = m.tetherMotor[d];
+        bool atEnd1;
+        double coeff = node_coeff(m.attachCoeffK[d], mm, nodeMode == 2 ? m.boundSeg : nullptr, mp.fracMove, atEnd1);
+        node_tether_one(d, 3 * m.nM, m.nBb, m.nMD, m.headBackboneSlot[d], 3 * mm, atEnd1, coeff, mp.dt, m.coord,
+                        m.uVec, m.segLength, m.bTransGam, m.forceSum, m.torqueSum, m.bbCoord, m.bbUVec, m.bbYVec,
+                        m.bbInvDragY, m.radial, m.miniData);
+        return;
+    }
     tether_one(d, 3 * m.nM, m.nBb, m.headBackboneSlot[d], 3 * m.tetherMotor[d], m.attachAxial[d], m.coord, m.uVec,
                m.segLength, m.bTransGam, m.bRotGam, m.forceSum, m.torqueSum, m.bbCoord, m.bbUVec, m.bbInvDragY,
                m.miniData, mp);
@@ -1232,7 +1243,7 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
     }
     if (m.nMD > 0) {
         Scope sc(net, T_TETHER);
-        hipLaunchKernelGGL(k_net_tether, dim3(blocks_for(m.nMD, 256)), dim3(256), 0, s, m, net->mip);
+        hipLaunchKernelGGL(k_net_tether, dim3(blocks_for(m.nMD, 256)), dim3(256), 0, s, m, net->mip, net->mp.nodeMode);
     }
     if (m.nBb > 0) {
         Scope sc(net, T_BACKBONE);
@@ -1878,10 +1889,14 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
                 "net_attach_motors: null motor kinetic array");
     SBX_REQUIRE(c, m->nD >= 0 && m->nBb >= 0 && m->nMD >= 0, "net_attach_motors: negative counts");
     if (m->nD > 0) SBX_REQUIRE(c, m->motorA && m->motorB && m->parallel, "net_attach_motors: null dimer array");
+    SBX_REQUIRE(c, p->nodeMode >= 0 && p->nodeMode <= 2, "net_attach_motors: nodeMode must be 0, 1 or 2");
     if (m->nMD > 0) {
-        SBX_REQUIRE(c, m->nBb > 0, "net_attach_motors: tethered dimers without backbones");
-        SBX_REQUIRE(c, m->headBackboneSlot && m->tetherMotor && m->attachAxial && m->miniData,
-                    "net_attach_motors: null tether array");
+        SBX_REQUIRE(c, m->nBb > 0, "net_attach_motors: tethered items without backbones or nodes");
+        SBX_REQUIRE(c, m->headBackboneSlot && m->tetherMotor && m->miniData, "net_attach_motors: null tether array");
+        if (p->nodeMode)
+            SBX_REQUIRE(c, m->radial && m->attachCoeffK, "net_attach_motors: node shell without radial/attachCoeffK");
+        else
+            SBX_REQUIRE(c, m->attachAxial, "net_attach_motors: null attachAxial");
     }
     if (m->nBb > 0)
         SBX_REQUIRE(c, m->bbCoord && m->bbUVec && m->bbYVec && m->bbZVec && m->bbEnd1 && m->bbEnd2 &&

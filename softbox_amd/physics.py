@@ -39,6 +39,15 @@ def rod_drag_si(length_um: float, radius_um: float, aeta: float = AETA_REF) -> t
     return (gtx, gty, gty, grx, gry, gry)
 
 
+def sphere_drag_si(radius_um: float, aeta: float = AETA_REF) -> tuple:
+    """Isotropic sphere drag (DragTensorSystem.sphereDragSI, DragTensorSystem.java:52-58): 6 pi eta R
+    translational and 8 pi eta R^3 rotational, R in metres."""
+    RM = radius_um * 1.0e-6
+    gt = 6 * math.pi * aeta * RM
+    gr = 8 * math.pi * aeta * (RM * RM * RM)
+    return (gt, gt, gt, gr, gr, gr)
+
+
 def set_rod_drag(fs, length_um: float, radius_um: float = ACTIN_RADIUS, aeta: float = AETA_REF,
                  kT: float = KT_REF, idx=None) -> None:
     """Store drag + Einstein diffusion for rods of one length (DragTensorSystem.storeDrag,

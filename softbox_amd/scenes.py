@@ -529,3 +529,199 @@ def _scene_eta(sc: Scene, kT: float) -> float:
     L = float(sc.fs.segLength[0])
     g1 = rod_drag_si(L, ACTIN_RADIUS, 1.0)[0]
     return float(f"{float(sc.fs.bTransGam[0]) / g1:.6g}")
+
+
+class JavaRandom:
+    """java.util.Random as its API specification defines it: a 48-bit linear congruential generator
+    (multiplier 0x5DEECE66D, increment 0xB, the seed scrambled with the multiplier) whose nextDouble is
+    ((next(26) << 27) + next(27)) * 2^-53.  The 1x standard scene draws its node centres and filament
+    poses from it, so the scene here starts from the reference harness's initial condition."""
+
+    MULT, ADD, MASK = 0x5DEECE66D, 0xB, (1 << 48) - 1
+
+    def __init__(self, seed: int):
+        self.s = (seed ^ self.MULT) & self.MASK
+
+    def next(self, bits: int) -> int:
+        self.s = (self.s * self.MULT + self.ADD) & self.MASK
+        return self.s >> (48 - bits)
+
+    def nextDouble(self) -> float:
+        return ((self.next(26) << 27) + self.next(27)) * (1.0 / (1 << 53))
+
+
+def _assemble_articulated(b, m, a, d, brown):
+    """MotorStore.assembleArticulated (MotorStore.java:272-293) in float32 arithmetic: rod, lever and head
+    laid out along d from the anchor a, y = unit(-dy, dx, 0)."""
+    f = np.float32
+    dx, dy, dz = f(d[0]), f(d[1]), f(d[2])
+    px, py, pz = -dy, dx, f(0.0)
+    pm = f(math.sqrt(float(px * px + py * py + pz * pz)))
+    if pm < f(1.0e-4):
+        px, py, pz, pm = f(1.0), f(0.0), f(0.0), f(1.0)
+    px, py, pz = px / pm, py / pm, pz / pm
+    RL, LL, HL = f(0.080), f(0.008), f(0.020)
+    offs = (f(0.5) * RL, RL + f(0.5) * LL, f(0.080 + 0.008) + f(0.5) * HL)
+    ax, ay, az = f(a[0]), f(a[1]), f(a[2])
+    for role, off in enumerate(offs):
+        i = 3 * m + role
+        b.plane("coord")[:, i] = (ax + off * dx, ay + off * dy, az + off * dz)
+        b.plane("uVec")[:, i] = (dx, dy, dz)
+        b.plane("yVec")[:, i] = (px, py, pz)
+    b.brownTransScale[3 * m] = b.brownRotScale[3 * m] = brown
+    b.brownTransScale[3 * m + 1] = b.brownRotScale[3 * m + 1] = 0.0
+    b.brownTransScale[3 * m + 2] = b.brownRotScale[3 * m + 2] = brown
+
+
+def standard_1x(n_nodes: int = 400, n_sing: int = 24, n_fil: int = 1000, segs: int = 10, box_xy: float = 7.071,
+                box_z: float = 0.5, dt: float = 1.0e-5, xlinks: bool = True, xlink_conc: float = 1.0,
+                reach: float = 0.006, brown_off: bool = False, bound_coeff: float | None = None,
+                rng: int = _lib.SBX_RNG_WANG) -> ContractileScene:
+    """The 1x contractility standard (V2OneXHarness.build / buildShells / buildGrid / buildCrosslinkers,
+    V2OneXHarness.java:153-349): n_nodes free protein nodes (radius 50 nm) in a box_xy x box_xy x box_z slab,
+    each with n_sing singlet myosins tethered radially on a Fibonacci sphere; n_fil static filaments of `segs`
+    64-monomer segments placed by the reference's two-point rule; crosslinkers formed every 100 steps
+    (dense preset, kon 40 /uM/s); containment every 1e-4 s; viscosity 0.1 Pa s and the reference kT, no
+    rescale; the reference's seeds (0x310501 motors/filaments/links, 0x315C2F nodes) and java.util.Random
+    initial condition.  bound_coeff switches the node tether to tetherBoundStiffen.  brown_off zeroes every
+    Brownian scale (the harness's -brownoff)."""
+    from .physics import AETA_REF, KT_REF, brownian_force_mag, rod_drag_si, set_rod_drag
+    from .stores import DimerStore, MotorStore, NodeStore
+
+    GOLDEN = 2.399963229728653
+    SEED, SEED_NODE = 0x310501, 0x315C2F
+    ALIGN_TOL, BROWN_TRANS, BROWN_ROT, NODE_BROWN = -0.4, 1.0, 0.3, 0.03
+    FIL_MONO = 64
+    aeta, kT = AETA_REF, KT_REF
+    half, half_z = 0.5 * box_xy, 0.5 * box_z
+    jr = JavaRandom(0x31AC7E5 ^ ((n_nodes << 20) ^ n_fil))
+    R = NodeStore.NODE_RADIUS
+    rl, ll, hl = MotorStore.ROD_LEN, MotorStore.LEVER_LEN, MotorStore.HEAD_LEN
+    marg = R + rl + (ll + hl) + 0.05
+    mz = min(half_z - 0.02, marg)
+    centers = []
+    for _ in range(n_nodes):
+        cx = (jr.nextDouble() - 0.5) * 2 * (half - marg)
+        cy = (jr.nextDouble() - 0.5) * 2 * (half - marg)
+        cz = (jr.nextDouble() - 0.5) * 2 * mz
+        centers.append((cx, cy, cz))
+    # ---- node shells (buildShells, singlets only) ----
+    nM = n_nodes * n_sing
+    mot = MotorStore(nM)
+    dim = DimerStore(0)
+    nd = NodeStore(n_nodes, nM, mode=2 if bound_coeff is not None else 1)
+    b = mot.body
+    for k, (cx, cy, cz) in enumerate(centers):
+        nd.node.plane("coord")[:, k] = (cx, cy, cz)
+        nd.node.plane("uVec")[:, k] = (1.0, 0.0, 0.0)
+        nd.node.plane("yVec")[:, k] = (0.0, 1.0, 0.0)
+        nd.node.brownTransScale[k] = nd.node.brownRotScale[k] = NODE_BROWN
+        for c in range(n_sing):
+            yy = 1.0 - 2.0 * (c + 0.5) / n_sing
+            rr = math.sqrt(max(0.0, 1.0 - yy * yy))
+            phi = c * GOLDEN
+            ux, uy, uz = rr * math.cos(phi), yy, rr * math.sin(phi)
+            m = k * n_sing + c
+            _assemble_articulated(b, m, (cx + R * ux, cy + R * uy, cz + R * uz), (ux, uy, uz), BROWN_TRANS)
+            b.brownRotScale[3 * m] = b.brownRotScale[3 * m + 2] = BROWN_ROT
+            nd.attach(m, k, m, R * ux, R * uy, R * uz, NodeStore.ATTN_FORCE / n_sing, False)
+    idx = np.arange(nM)
+    for role, (L, Rr) in enumerate([(rl, MotorStore.ROD_R), (ll, MotorStore.LEVER_R)]):
+        set_rod_drag(b, L, Rr, aeta, kT, idx=3 * idx + role)
+    gsph = (6 * math.pi * aeta * MotorStore.HEAD_R * 1e-6, 8 * math.pi * aeta * (MotorStore.HEAD_R * 1e-6) ** 3)
+    hidx = 3 * idx + 2
+    b.segLength[hidx] = np.float32(hl)
+    for a in range(3):
+        b.plane("bTransGam")[a][hidx] = np.float32(gsph[0])
+        b.plane("bRotGam")[a][hidx] = np.float32(gsph[1])
+    nd.init_node_drag(aeta, kT)
+    derive_host(b)
+    derive_host(nd.node)
+    bfm = brownian_force_mag(dt, kT)
+    mot.bodyParams[:] = np.array([dt, bfm], np.float32)
+    mot.set_joint_params(dt, 6.0)
+    mot.set_kin_params(reach, ALIGN_TOL, dt, kT)
+    mot.kinParams[12] = 1.0                       # setFaithfulRelease(true, 0): the 12 pN break-force cap
+    mot.set_nuc_params(dt)
+    mot.xbParams[:] = np.array([1.0e-9, 90.0, 0.4, dt, hl, 0.0], np.float32)
+    nd.set_params(dt, kT, 0.0 if bound_coeff is None else bound_coeff)
+    # ---- static filaments (two random points at contour separation) ----
+    n = n_fil * segs
+    fs = FilamentStore(n)
+    L = (FIL_MONO + 1) * ACTIN_MONO_RADIUS
+    span = segs * L
+    for fi in range(n_fil):
+        while True:
+            ax_, ay_, az_ = ((jr.nextDouble() - 0.5) * 2 * half, (jr.nextDouble() - 0.5) * 2 * half,
+                             (jr.nextDouble() - 0.5) * 2 * half_z)
+            bx_, by_, bz_ = ((jr.nextDouble() - 0.5) * 2 * half, (jr.nextDouble() - 0.5) * 2 * half,
+                             (jr.nextDouble() - 0.5) * 2 * half_z)
+            dx, dy, dz = bx_ - ax_, by_ - ay_, bz_ - az_
+            d = math.sqrt(dx * dx + dy * dy + dz * dz)
+            if 0.90 * span <= d <= 1.10 * span:
+                break
+        ux, uy, uz = dx / d, dy / d, dz / d
+        cx, cy, cz = 0.5 * (ax_ + bx_), 0.5 * (ay_ + by_), 0.5 * (az_ + bz_)
+        yx, yy_, yz = -uy, ux, 0.0
+        yn = math.sqrt(yx * yx + yy_ * yy_ + yz * yz)
+        if yn < 1e-9:
+            yx, yy_, yz = 0.0, -uz, uy
+            yn = math.sqrt(yy_ * yy_ + yz * yz)
+        yx, yy_, yz = yx / yn, yy_ / yn, yz / yn
+        e1 = (cx - 0.5 * span * ux, cy - 0.5 * span * uy, cz - 0.5 * span * uz)
+        for i in range(segs):
+            sl = fi * segs + i
+            fs.plane("coord")[:, sl] = (e1[0] + (0.5 + i) * L * ux, e1[1] + (0.5 + i) * L * uy,
+                                        e1[2] + (0.5 + i) * L * uz)
+            fs.plane("uVec")[:, sl] = (ux, uy, uz)
+            fs.plane("yVec")[:, sl] = (yx, yy_, yz)
+            fs.brownTransScale[sl] = 1.0
+            fs.brownRotScale[sl] = 0.5 if (i == 0 or i == segs - 1) else 0.0
+            # chain links exactly as the harness wires them (V2OneXHarness.java:214-215)
+            if i > 0:
+                fs.end2NbrSlot[sl], fs.end2NbrSide[sl] = sl - 1, 0
+            if i < segs - 1:
+                fs.end1NbrSlot[sl], fs.end1NbrSide[sl] = sl + 1, 1
+    set_rod_drag(fs, L, ACTIN_RADIUS, aeta, kT)
+    fs.set_params(dt, bfm)
+    fs.set_chain_params(dt)
+    derive_host(fs)
+    if brown_off:
+        for st in (b, nd.node, fs):
+            st.brownTransScale[:] = 0.0
+            st.brownRotScale[:] = 0.0
+    p = _base_params(dt, kT)
+    p.update(seed=SEED, rng=rng, chainOn=1, seedMotor=SEED, seedBackbone=SEED_NODE)
+    # ---- binding grid over filaments + heads (buildGrid) ----
+    seg_bound = 0.5 * L + ACTIN_RADIUS
+    cutoff = reach + 0.5 * rl
+    cell = 2.0 * seg_bound + cutoff
+    gx, gz = half + 0.5, half_z + 0.3
+    nX, nZ = 1 + int(math.ceil(2 * gx / cell)), 1 + int(math.ceil(2 * gz / cell))
+    p.update(gridOn=1, periodic=0, boxLen=[0.0, 0.0, 0.0], gridParams=[-gx, -gx, -gz, cell, 1.0 / cell, cutoff],
+             gridDims=[nX, nX, nZ, nX * nX * nZ])
+    # ---- containment every 1e-4 s ----
+    check = max(1, int(round(1.0e-4 / dt)))
+    p.update(confineOn=1, boxParams=[1.0e-4, box_xy, box_xy, box_z, ACTIN_RADIUS, 0.5, float(check), dt])
+    xl = None
+    if xlinks:
+        C = max(256, n * 4)
+        xl = CrosslinkerStore(C, n)
+        rest = 0.0125
+        xl.set_params(rest, 0.4, dt)
+        xl.set_off_params(1.0, 1.0, 2.0, dt, rest)
+        xl.set_torsion_params(1.0e-19, True)
+        p.update(xlParams=list(xl.xlParams), offParams=list(xl.offParams), torsionParams=list(xl.torsionParams),
+                 unbindOn=1)
+        grab = 2.0 * ACTIN_MONO_DIAM
+        fcell = 2.0 * (0.5 * L + ACTIN_RADIUS) + grab
+        fgx, fgz = half + fcell, half_z + fcell
+        fnX, fnZ = 1 + int(math.ceil(2 * fgx / fcell)), 1 + int(math.ceil(2 * fgz / fcell))
+        p_form = 1.0 - math.exp(-40.0 * xlink_conc * dt * 100)
+        fp = np.array([0.6, grab * grab, 5.0 * ACTIN_MONO_DIAM, 10, p_form, 2.0 * ACTIN_MONO_DIAM, grab], np.float32)
+        p.update(formOn=1, formInt=100, formParams=[float(v) for v in fp], formMode=0,
+                 formGridParams=[-fgx, -fgx, -fgz, fcell, 1.0 / fcell, grab], formGridDims=[fnX, fnX, fnZ, fnX * fnX * fnZ],
+                 formReqCap=int(min(n * 256, max(4096, 16 * n))))
+    sc = ContractileScene("standard_1x", fs, xl, p, box_xy, SEED, n_fil=n_fil, segs_per_fil=segs, mot=mot, dim=dim,
+                          mini=nd)
+    return sc

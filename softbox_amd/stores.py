@@ -8,6 +8,7 @@ device memory) and produce the C-ABI structs of include/softbox_hip.h.
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import numpy as np
 
@@ -279,6 +280,81 @@ class MiniFilamentStore:
         return o
 
 
+class NodeStore:
+    """NodeStore.java:36-153: protein nodes, rigid spheres with an orientation, each owning motors tethered to
+    body-frame radial points on its surface.  The node bodies sit in the backbone slots of the motor step
+    (the same Brownian / containment / integration / derive and the same CSR gather of the per-attachment
+    reactions); the backbone-named properties below are that view."""
+
+    NODE_RADIUS = 0.05        # um (NodeStore.java:40)
+    ATTN_FORCE = 0.4          # NodeStore.java:41
+    DIMER_FRACMOVE = 0.2      # DimerStore.ROD_FRAC_MOVE (NodeStore.java:42)
+
+    def __init__(self, nNodes: int, nAttach: int, mode: int = 1):
+        self.nNodes, self.nAttach = nNodes, nAttach
+        self.node = FilamentStore(nNodes)
+        self.attachNode = np.zeros(nAttach, np.int32)
+        self.attachMotor = np.zeros(nAttach, np.int32)
+        self.attachAtEnd1 = np.zeros(nAttach, np.int32)
+        self.attachCoeff = np.zeros(nAttach, np.float32)
+        self.radial = np.zeros(3 * nAttach, np.float32)
+        self.attachCoeffK = np.zeros(nAttach, np.float32)
+        self.nodeData = np.zeros(MINI_STRIDE * nAttach, np.float32)
+        self.nodeInvTransY = np.zeros(nNodes, np.float32)
+        self.nodeParams = np.zeros(2, np.float32)       # dt, boundCoeff (tetherBoundStiffen)
+        self.nodeBodyParams = np.zeros(2, np.float32)   # dt, sqrt(2 kT / dt)
+        self.mode = mode                                # 1 tether, 2 tetherBoundStiffen
+        self._axial = np.zeros(nAttach, np.float32)
+
+    @property
+    def attachKey(self) -> np.ndarray:
+        """The packed [node(nA) | motor(nA)] key of the per-System calls."""
+        return np.concatenate([self.attachNode, self.attachMotor]).astype(np.int32)
+
+    def attach(self, a, k, m, ru, ry, rz, coeff, at_end1):           # NodeStore.java:113-123
+        self.attachNode[a], self.attachMotor[a] = k, m
+        self.attachAtEnd1[a] = 1 if at_end1 else 0
+        self.attachCoeff[a] = np.float32(coeff)
+        n = self.nAttach
+        self.radial[a], self.radial[n + a], self.radial[2 * n + a] = np.float32(ru), np.float32(ry), np.float32(rz)
+        self.attachCoeffK[a] = np.float32(coeff if at_end1 else -coeff)
+
+    def init_node_drag(self, aeta: float, kT: float) -> None:        # NodeStore.java:136-151
+        from .physics import sphere_drag_si
+
+        g = sphere_drag_si(self.NODE_RADIUS, aeta)
+        b = self.node
+        b.segLength[:] = 0.0
+        for a in range(3):
+            b.plane("bTransGam")[a][:] = np.float32(g[a])
+            b.plane("bRotGam")[a][:] = np.float32(g[3 + a])
+            b.plane("bTransDiff")[a][:] = np.float32(kT / g[a])
+            b.plane("bRotDiff")[a][:] = np.float32(kT / g[3 + a])
+        self.nodeInvTransY[:] = np.float32(1.0 / g[1])
+
+    def set_params(self, dt: float, kT: float, bound_coeff: float = 0.0) -> None:   # NodeStore.java:125-133
+        self.nodeParams[:] = np.array([dt, bound_coeff], np.float32)
+        self.nodeBodyParams[:] = np.array([dt, math.sqrt(2.0 * kT / dt)], np.float32)
+
+    # the backbone view of the motor step
+    nBb = property(lambda self: self.nNodes)
+    nMD = property(lambda self: self.nAttach)
+    backbone = property(lambda self: self.node)
+    headBackboneSlot = property(lambda self: self.attachNode)
+    motorA = property(lambda self: self.attachMotor)
+    attachAxial = property(lambda self: self._axial)
+    miniData = property(lambda self: self.nodeData)
+    bbInvDragY = property(lambda self: self.nodeInvTransY)
+    bbBodyParams = property(lambda self: self.nodeBodyParams)
+    miniParams = property(lambda self: np.array([self.nodeParams[0], self.nodeParams[1], 0.0], np.float32))
+
+    def copy(self) -> "NodeStore":
+        o = NodeStore.__new__(NodeStore)
+        for k, v in self.__dict__.items():
+            setattr(o, k, v.copy() if hasattr(v, "copy") else v)
+        return o
+
+
 class DeviceMotors:
     """Device mirror of a MotorStore + DimerStore + MiniFilamentStore; `struct()` gives the
     sbx_motors of the C ABI and `params()` the sbx_motor_params."""
@@ -316,6 +392,9 @@ class DeviceMotors:
         self.t["tetherMotor"] = ctx.to_device(mini.motorA) if mini.nMD else None
         self.t["attachAxial"] = ctx.to_device(mini.attachAxial) if mini.nMD else None
         self.t["miniData"] = ctx.to_device(mini.miniData) if mini.nMD else None
+        node = isinstance(mini, NodeStore)
+        self.t["radial"] = ctx.to_device(mini.radial) if (node and mini.nMD) else None
+        self.t["attachCoeffK"] = ctx.to_device(mini.attachCoeffK) if (node and mini.nMD) else None
         p = _lib.SbxMotorParams()
         for k, src in (("bodyParams", mot.bodyParams), ("jointParams", mot.jointParams), ("kinParams", mot.kinParams),
                        ("nucParams", mot.nucParams), ("xbParams", mot.xbParams), ("dimerParams", dim.dimerParams),
@@ -324,6 +403,7 @@ class DeviceMotors:
             for i, v in enumerate(src):
                 arr[i] = float(v)
         p.seedMotor, p.seedBackbone = int(seeds[0]), int(seeds[1])
+        p.nodeMode = int(mini.mode) if node else 0
         self._params = p
 
     def struct(self) -> _lib.SbxMotors:

@@ -370,3 +370,35 @@ class MiniFilamentSystem:
     def backboneGather(ctx: Context, bbDimerOffsets, bbDimerList, miniData, bbForceSum, bbTorqueSum):
         return ctx.check(ctx.lib.sbx_backbone_gather(ctx.h, ptr(bbDimerOffsets), ptr(bbDimerList), ptr(miniData),
                                                      ptr(bbForceSum), ptr(bbTorqueSum), _n3(bbForceSum)))
+
+
+class NodeSystem:
+    """NodeSystem.java:48-231: the radial tether of motor rods to node spheres, its bound-stiffened variant and
+    the cross-bridge load injection.  attachKey = [node(nA) | motor(nA)]; the per-node gather is
+    MiniFilamentSystem.backboneGather over the CSR of attachNode."""
+
+    @staticmethod
+    def tether(ctx: Context, mCoord, mUVec, mSegLength, mBTransGam, mForceSum, mTorqueSum, nCoord, nUVec, nYVec,
+               nInvTransY, attachKey, radial, attachCoeffK, nodeData, nodeParams):
+        npar, npp = _f(nodeParams)
+        return ctx.check(ctx.lib.sbx_node_tether(ctx.h, ptr(mCoord), ptr(mUVec), ptr(mSegLength), ptr(mBTransGam),
+                                                 ptr(mForceSum), ptr(mTorqueSum), ptr(nCoord), ptr(nUVec),
+                                                 ptr(nYVec), ptr(nInvTransY), ptr(attachKey), ptr(radial),
+                                                 ptr(attachCoeffK), ptr(nodeData), npp, _n3(mCoord), _n3(nCoord),
+                                                 attachCoeffK.numel()))
+
+    @staticmethod
+    def tetherBoundStiffen(ctx: Context, mCoord, mUVec, mSegLength, mBTransGam, mForceSum, mTorqueSum, nCoord, nUVec,
+                           nYVec, nInvTransY, attachKey, radial, attachCoeffK, boundSeg, nodeData, nodeParams):
+        npar, npp = _f(nodeParams)
+        return ctx.check(ctx.lib.sbx_node_tether_bound_stiffen(
+            ctx.h, ptr(mCoord), ptr(mUVec), ptr(mSegLength), ptr(mBTransGam), ptr(mForceSum), ptr(mTorqueSum),
+            ptr(nCoord), ptr(nUVec), ptr(nYVec), ptr(nInvTransY), ptr(attachKey), ptr(radial), ptr(attachCoeffK),
+            ptr(boundSeg), ptr(nodeData), npp, _n3(mCoord), _n3(nCoord), attachCoeffK.numel()))
+
+    @staticmethod
+    def xbridgeInject(ctx: Context, bondData, nCoord, nUVec, nYVec, attachKey, radial, attachCoeffK, boundSeg,
+                      nodeData, nodeParams=None):
+        return ctx.check(ctx.lib.sbx_node_xbridge_inject(ctx.h, ptr(bondData), ptr(nUVec), ptr(nYVec), ptr(attachKey),
+                                                         ptr(radial), ptr(attachCoeffK), ptr(boundSeg),
+                                                         ptr(nodeData), _n3(nCoord), attachCoeffK.numel()))
