@@ -187,9 +187,11 @@ def main() -> int:
     ap.add_argument("--n-fil", type=int, default=None, help="filaments (network 20000, actomyosin 50000)")
     ap.add_argument("--n-links", type=int, default=None, help="crosslinker slots (network 50000, actomyosin 100000)")
     ap.add_argument("--box", type=float, default=None, help="box edge, um (network 20, actomyosin 30)")
-    ap.add_argument("--workload", choices=("network", "actomyosin"), default="network",
+    ap.add_argument("--workload", choices=("network", "actomyosin", "standard1x"), default="network",
                     help="network: the crosslinked network (the headline); actomyosin: the full actomyosin "
-                         "network (50k filaments, 2k minifilaments of 16 heads, 100k crosslinkers, 30 um box)")
+                         "network (50k filaments, 2k minifilaments of 16 heads, 100k crosslinkers, 30 um box); "
+                         "standard1x: the reference's 1x contractility standard (400 protein nodes x 24 myosins, "
+                         "1000 filaments x 10 segments, 7.071 x 7.071 x 0.5 um slab)")
     ap.add_argument("--eta", type=float, default=None,
                     help="viscosity, Pa s (default: the reference's Constants.aeta = 0.1 for the network, the "
                          "dense actomyosin fixture's 1.0 for the actomyosin network)")
@@ -238,6 +240,8 @@ def main() -> int:
         return run_batch(args, log)
     if args.workload == "actomyosin":
         return run_actomyosin(args, log)
+    if args.workload == "standard1x":
+        return run_standard(args, log)
     t_build = time.perf_counter()
     sc = scenes.network(n_fil=args.n_fil, segs=10, box=args.box, n_links=args.n_links, seed=0, eta=args.eta)
     if args.formation > 0:  # crosslinker formation at its cadence: the workload's 40 nm binding radius
@@ -556,6 +560,105 @@ def run_actomyosin(args, log) -> int:
         out["turnover"] = r["turnover"]
         out["config"]["workload"] += (", turnover every 100 steps (barbed growth 10 subunits/s, pointed-end "
                                       "depolymerisation, aging, cofilin severing, splits into 10% free slots)")
+    if kernels:
+        out["kernels"] = {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                          for k, v in kernels.items()}
+    print(json.dumps(out))
+    ctx.close()
+    return 0
+
+
+STANDARD_1X_PUBLISHED = {"v1-CPU": 30.4, "v1-GPU": 21.4, "v2-CPU": 15.3, "v2-GPU": 49.5}   # steps/s
+
+
+def run_standard(args, log) -> int:
+    """The reference's 1x contractility standard (V2OneXHarness.java:14-47): 400 free protein nodes with 24
+    singlet myosins each, 1000 static filaments x 10 segments (64 monomers), crosslinker formation every 100
+    steps (dense preset, 1 uM), containment every 1e-4 s, viscosity 0.1 Pa s, dt 1e-5 s, the harness's seeds
+    and java.util.Random initial condition.  Reported in the harness's own unit, steps/s, beside its
+    published four-way figures (v1-CPU 30.4, v1-GPU 21.4, v2-CPU 15.3, v2-GPU 49.5 steps/s on the reference's
+    hardware)."""
+    import torch  # noqa: F401
+
+    from softbox_amd import scenes
+    from softbox_amd.context import Context
+    from softbox_amd.network import Network
+
+    t_build = time.perf_counter()
+    sc = scenes.standard_1x()
+    n = sc.fs.n
+    log(f"1x standard scene: {n} segments, {sc.mot.nM} myosins on {sc.mini.nNodes} nodes, {sc.xl.nLinks} link "
+        f"slots ({time.perf_counter() - t_build:.1f} s)")
+    ctx = Context(0)
+    net = Network(ctx, sc.copy(), keep_forces=False)
+    net.set_overlap(bool(args.overlap))
+    if args.graph:
+        net.use_graph(True)
+    net.step(0, args.warmup)
+    t0 = args.warmup
+    tuned = None
+    if args.shape:
+        net.set_shape(*[int(v) for v in args.shape.split(",")])
+    elif args.tune_steps:
+        tuned = net.tune(t0, args.tune_steps)
+        t0 += tuned["steps"]
+        log(f"tuned launch shape {tuned}")
+    start = sc.copy()
+    start.mot, start.dim, start.mini = sc.mot.copy(), sc.dim.copy(), sc.mini.copy()
+    net.pull(start.fs, start.xl)
+    net.pull_motors(start.mot, start.mini)
+    formed0 = net.form_stats().get("formed_total", 0)
+    el = timed(net, ctx, t0, args.steps)
+    sps = args.steps / el
+    formed = net.form_stats()["formed_total"] - formed0
+    kernels = profile(net, t0 + args.steps, args.profile_steps) if args.profile_steps else {}
+    end = sc.copy()
+    end.mot, end.mini = sc.mot.copy(), sc.mini.copy()
+    net.pull(end.fs, end.xl)
+    net.pull_motors(end.mot, end.mini)
+    assert np.isfinite(end.fs.coord).all() and np.isfinite(end.mot.body.coord).all(), "non-finite pose"
+    assert np.isfinite(end.mini.node.coord).all(), "non-finite node pose"
+    net.close()
+    roof = None
+    if kernels:
+        dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
+        b = kernel_bytes(sc, dom)
+        roof = {"bound": "launch", "kernel": dom, "duration_us": round(kernels[dom]["avg_us"], 2),
+                "note": "at 10k segments and 29k motor bodies every kernel is a few waves deep: the step is "
+                        "bound by launch and dependency latency, not by HBM"}
+        if b:
+            a = b / (kernels[dom]["avg_us"] * 1e-6) / 1e9
+            roof.update(achieved=round(a, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(a / HBM_PEAK_GBS, 4),
+                        bytes_per_launch=b)
+    cpu = cpu1 = None
+    if not args.no_cpu:
+        for openmp in (True, False):
+            c = run_cpu_baseline_actomyosin(start, t0, budget_s=10.0, openmp=openmp)
+            c["steps_per_s"] = round(c["value"] / n, 2)
+            if openmp:
+                cpu = c
+            else:
+                cpu1 = c
+    out = {
+        "metric": "steps/s of the 1x contractility standard at 1 GPU", "value": round(sps, 1), "unit": "steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": round(sps / STANDARD_1X_PUBLISHED["v2-GPU"], 1),
+        "vs_published": {k: round(sps / v, 1) for k, v in STANDARD_1X_PUBLISHED.items()},
+        "rod_steps_per_s": round(n * sps, 1),
+        "dtype": "f32 state, f64 force laws", "data": "synthetic (the harness's seeded initial condition)",
+        "config": {"workload": "1x contractility standard (V2OneXHarness): 400 free nodes x 24 singlet myosins, "
+                               "1000 static filaments x 10 segments (64 monomers), crosslinker formation every 100 "
+                               "steps (1 uM, kon 40 /uM/s), 7.071 x 7.071 x 0.5 um slab with containment, dt 1e-5 s, "
+                               "viscosity 0.1 Pa s, Wang-hash RNG with the harness's seeds",
+                   "segments": n, "myosins": sc.mot.nM, "nodes": sc.mini.nNodes, "link_slots": sc.xl.nLinks,
+                   "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
+                   "shape": shape_of(args, tuned) if (args.shape or tuned) else None},
+        "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu1,
+        "links_formed_timed": formed, "links_alive_end": int((end.xl.linkState[: end.xl.nLinks] >= 0).sum()),
+        "heads_bound_start": int((start.mot.boundSeg >= 0).sum()), "heads_bound_end": int((end.mot.boundSeg >= 0).sum()),
+        "timed_steps": [t0, t0 + args.steps - 1],
+    }
     if kernels:
         out["kernels"] = {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                           for k, v in kernels.items()}
