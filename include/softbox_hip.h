@@ -351,6 +351,36 @@ int sbx_node_tether_bound_stiffen(sbx_ctx *c, const float *mCoord, const float *
 int sbx_node_xbridge_inject(sbx_ctx *c, const float *bondData, const float *nUVec, const float *nYVec,
                             const int *attachKey, const float *radial, const float *attachCoeffK,
                             const int *boundSeg, float *nodeData, int nN, int nA);
+/* NodeNucleationSystem.countBoundFil (NodeNucleationSystem.java:55-62): seeds held by each node */
+int sbx_nuc_count_bound_fil(sbx_ctx *c, const int *seedNode, int *nodeBoundFil, int C, int nN);
+/* NodeNucleationSystem.emit (NodeNucleationSystem.java:66-100): per-node seed requests at stride RC >= nN;
+ * nucParams (host, 3): P_nuc, seed half-length, forminsPerNode; poolOK 0 blocks every request */
+int sbx_nuc_emit(sbx_ctx *c, const float *nodeCoord, const int *nodeBoundFil, int *acceptFlag, float *reqCoord,
+                 float *reqUVec, float *reqYVec, const float *nucParams, int step, int seed, int poolOK, int nN, int RC,
+                 int rng, uint32_t stream);
+/* NodeNucleationSystem.tagSeeds (NodeNucleationSystem.java:104-115): after sbx_birth_alloc, request r's slot
+ * is held by node r */
+int sbx_nuc_tag_seeds(sbx_ctx *c, const int *rankOffsets, const int *freeList, const int *freeOffsets, int *seedNode,
+                      int C, int K);
+/* NodeNucleationSystem.initNewborn (NodeNucleationSystem.java:228-244); seedParams (host, 2): monomers, length */
+int sbx_nuc_init_newborn(sbx_ctx *c, const int *rankOffsets, const int *freeList, const int *freeOffsets,
+                         int *monomerCount, float *segLength, const float *seedParams, int C, int K);
+/* NodeNucleationSystem.seedTether (NodeNucleationSystem.java:117-149); tetherParams (host, 2): fracMove, dt */
+int sbx_nuc_seed_tether(sbx_ctx *c, const float *filCoord, const float *filUVec, const float *filSegLength,
+                        const float *filBTransGam, float *filForceSum, float *filTorqueSum, const float *nodeCoord,
+                        const float *nodeInvTransX, const int *seedNode, const float *tetherParams, int C, int nN);
+/* NodeNucleationSystem.seedTetherNodeReact (NodeNucleationSystem.java:161-187): -F of each node's held seeds,
+ * in slot order, onto the node's forceSum */
+int sbx_nuc_seed_tether_node_react(sbx_ctx *c, const float *filCoord, const float *filUVec, const float *filSegLength,
+                                   const float *filBTransGam, float *nodeForceSum, const float *nodeCoord,
+                                   const float *nodeInvTransX, const int *seedNode, const float *tetherParams, int C,
+                                   int nN);
+/* NodeNucleationSystem.seedAimTorque (NodeNucleationSystem.java:196-211); aimParams (host, 2): coeff, dt */
+int sbx_nuc_seed_aim_torque(sbx_ctx *c, const float *filUVec, const float *filBRotGam, float *filTorqueSum,
+                            const float *aimDir, const float *aimParams, int C);
+/* NodeNucleationSystem.dissolve (NodeNucleationSystem.java:248-257); dissolveParams (host, 1): P_detach */
+int sbx_nuc_dissolve(sbx_ctx *c, int *seedNode, const float *dissolveParams, int step, int seed, int C, int rng,
+                     uint32_t stream);
 
 /* ------------------------------------------------------------------ whole network step
  * Replaces the filament block of DenseContractileHarness.buildPlan / cpuStep (DenseContractileHarness.java:

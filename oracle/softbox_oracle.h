@@ -191,6 +191,28 @@ void sbo_node_tether_bound_stiffen(int nMB, int nN, int nA, const float *mCoord,
 void sbo_node_xbridge_inject(int nN, int nA, const float *bondData, const float *nUVec, const float *nYVec,
                              const int *attachKey, const float *radial, const float *attachCoeffK,
                              const int *boundSeg, float *nodeData);                       /* :141-166 */
+/* ---- node nucleation (NodeNucleationSystem.java); seeds, events and the node bond of each slot ---- */
+void sbo_event_bits4(int rng, uint32_t item, int32_t step, int32_t seed, uint32_t stream, uint32_t salt,
+                     uint32_t out[4]);
+void sbo_nuc_count_bound_fil(int C, int nN, const int *seedNode, int *nodeBoundFil);         /* :55-62 */
+void sbo_nuc_emit(int nN, int RC, const float *nodeCoord, const int *nodeBoundFil, int *acceptFlag, float *reqCoord,
+                  float *reqUVec, float *reqYVec, const float *nucParams, int32_t step, int32_t seed, int poolOK,
+                  int rng, uint32_t stream);                                                 /* :66-100 */
+void sbo_nuc_tag_seeds(int C, int K, const int *rankOffsets, const int *freeList, const int *freeOffsets,
+                       int *seedNode);                                                      /* :104-115 */
+void sbo_nuc_init_newborn(int C, int K, const int *rankOffsets, const int *freeList, const int *freeOffsets,
+                          int *monomerCount, float *segLength, const float *seedParams);    /* :228-244 */
+void sbo_nuc_seed_tether(int C, int nN, const float *filCoord, const float *filUVec, const float *filSegLength,
+                         const float *filBTransGam, float *filForceSum, float *filTorqueSum, const float *nodeCoord,
+                         const float *nodeInvTransX, const int *seedNode, const float *tetherParams); /* :117-149 */
+void sbo_nuc_seed_tether_node_react(int C, int nN, const float *filCoord, const float *filUVec,
+                                    const float *filSegLength, const float *filBTransGam, float *nodeForceSum,
+                                    const float *nodeCoord, const float *nodeInvTransX, const int *seedNode,
+                                    const float *tetherParams);                            /* :161-187 */
+void sbo_nuc_seed_aim_torque(int C, const float *filUVec, const float *filBRotGam, float *filTorqueSum,
+                             const float *aimDir, const float *aimParams);                 /* :196-211 */
+void sbo_nuc_dissolve(int C, int *seedNode, const float *dissolveParams, int32_t step, int32_t seed, int rng,
+                      uint32_t stream);                                                    /* :248-257 */
 
 /* ---- one whole network step (DenseContractileHarness.java:350-429 filament block, no motors) ---- */
 typedef struct sbo_net {

@@ -137,3 +137,200 @@ void sbo_node_xbridge_inject(int nN, int nA, const float *bondData, const float 
         o[3] = (float)(o[3] + Tx); o[4] = (float)(o[4] + Ty); o[5] = (float)(o[5] + Tz);
     }
 }
+
+/* ------------------------------------------------------------------ node nucleation (NodeNucleationSystem.java) */
+#define SALT_NUC1 0x4E554331u   /* "NUC1" */
+#define SALT_NUCD 0x4E554344u   /* "NUCD" */
+
+/* Four words of one event draw.  WANG: the hash of base = key ^ salt and of base ^ 0x9e3779b9 / 0x85ebca6b /
+ * 0xc2b2ae35 (NodeNucleationSystem.java:76-83); PHILOX: the four words of counter (item, step, salt, 2), whose
+ * first word is sbo_event_bits'. */
+void sbo_event_bits4(int rng, uint32_t item, int32_t step, int32_t seed, uint32_t stream, uint32_t salt,
+                     uint32_t out[4]) {
+    if (rng == SBO_RNG_WANG) {
+        uint32_t base = (item * 1000003u) ^ ((uint32_t)step * 999983u) ^ ((uint32_t)seed * 7919u) ^ salt;
+        out[0] = sbo_wang_hash(base);
+        out[1] = sbo_wang_hash(base ^ 0x9e3779b9u);
+        out[2] = sbo_wang_hash(base ^ 0x85ebca6bu);
+        out[3] = sbo_wang_hash(base ^ 0xc2b2ae35u);
+        return;
+    }
+    uint32_t key[2] = {(uint32_t)seed, stream};
+    uint32_t c[4] = {item, (uint32_t)step, salt, 2u};
+    sbo_philox4x32_10(c, key, out);
+}
+
+/* NodeNucleationSystem.countBoundFil (NodeNucleationSystem.java:55-62) */
+void sbo_nuc_count_bound_fil(int C, int nN, const int *seedNode, int *nodeBoundFil) {
+    for (int k = 0; k < nN; k++) nodeBoundFil[k] = 0;
+    for (int s = 0; s < C; s++) {
+        int k = seedNode[s];
+        if (k >= 0) nodeBoundFil[k]++;
+    }
+}
+
+/* NodeNucleationSystem.emit (NodeNucleationSystem.java:66-100): per node, a seed request below the formin cap
+ * when the pool allows it and the draw falls under P_nuc; requests at stride RC; acceptFlag[k] written for
+ * every node.  Float arithmetic as the reference's. */
+void sbo_nuc_emit(int nN, int RC, const float *nodeCoord, const int *nodeBoundFil, int *acceptFlag, float *reqCoord,
+                  float *reqUVec, float *reqYVec, const float *nucParams, int32_t step, int32_t seed, int poolOK,
+                  int rng, uint32_t stream) {
+    float pNuc = nucParams[0], halfLen = nucParams[1];
+    int formins = (int)nucParams[2];
+    for (int k = 0; k < nN; k++) {
+        acceptFlag[k] = 0;
+        if (nodeBoundFil[k] >= formins) continue;
+        if (poolOK == 0) continue;
+        uint32_t h[4];
+        sbo_event_bits4(rng, (uint32_t)k, step, seed, stream, SALT_NUC1, h);
+        float u = sbo_uniform(h[0]);
+        if (u >= pNuc) continue;
+        float dx = sbo_uniform(h[1]) * 2.0f - 1.0f;
+        float dy = sbo_uniform(h[2]) * 2.0f - 1.0f;
+        float dz = sbo_uniform(h[3]) * 2.0f - 1.0f;
+        float m2 = dx * dx + dy * dy + dz * dz;
+        if (m2 < 1.0e-12f) { dx = 1.0f; dy = 0.0f; dz = 0.0f; m2 = 1.0f; }
+        float inv = 1.0f / (float)sqrt((double)m2);
+        dx *= inv; dy *= inv; dz *= inv;
+        float ncx = nodeCoord[k], ncy = nodeCoord[nN + k], ncz = nodeCoord[2 * nN + k];
+        float ex, ey, ez;
+        if (dx < 0.9f && dx > -0.9f) { ex = 1.0f; ey = 0.0f; ez = 0.0f; } else { ex = 0.0f; ey = 1.0f; ez = 0.0f; }
+        float yx = dy * ez - dz * ey, yy = dz * ex - dx * ez, yz = dx * ey - dy * ex;
+        float ym = 1.0f / (float)sqrt((double)(yx * yx + yy * yy + yz * yz));
+        yx *= ym; yy *= ym; yz *= ym;
+        reqCoord[k] = ncx + halfLen * dx; reqCoord[RC + k] = ncy + halfLen * dy; reqCoord[2 * RC + k] = ncz + halfLen * dz;
+        reqUVec[k] = -dx; reqUVec[RC + k] = -dy; reqUVec[2 * RC + k] = -dz;
+        reqYVec[k] = yx; reqYVec[RC + k] = yy; reqYVec[2 * RC + k] = yz;
+        acceptFlag[k] = 1;
+    }
+}
+
+/* NodeNucleationSystem.tagSeeds (NodeNucleationSystem.java:104-115): request r tethers its born slot to node r */
+void sbo_nuc_tag_seeds(int C, int K, const int *rankOffsets, const int *freeList, const int *freeOffsets,
+                       int *seedNode) {
+    int nFree = freeOffsets[C];
+    for (int r = 0; r < K; r++) {
+        int rank = rankOffsets[r];
+        if (!(rankOffsets[r + 1] > rank)) continue;
+        if (rank >= nFree) continue;
+        seedNode[freeList[rank]] = r;
+    }
+}
+
+/* NodeNucleationSystem.initNewborn (NodeNucleationSystem.java:228-244) */
+void sbo_nuc_init_newborn(int C, int K, const int *rankOffsets, const int *freeList, const int *freeOffsets,
+                          int *monomerCount, float *segLength, const float *seedParams) {
+    int nFree = freeOffsets[C];
+    int mon = (int)seedParams[0];
+    float len = seedParams[1];
+    for (int r = 0; r < K; r++) {
+        int rank = rankOffsets[r];
+        if (!(rankOffsets[r + 1] > rank)) continue;
+        if (rank >= nFree) continue;
+        int slot = freeList[rank];
+        monomerCount[slot] = mon;
+        segLength[slot] = len;
+    }
+}
+
+/* the seed tether force of slot s held by node k (NodeNucleationSystem.java:125-146): toward the node centre
+ * from the barbed end2; returns 0 when the end sits on the centre */
+static int seed_tether_force(int s, int k, int C, int nN, const float *filCoord, const float *filUVec,
+                             const float *filSegLength, const float *filBTransGam, const float *nodeCoord,
+                             const float *nodeInvTransX, double fracMove, double dt, double F[3], double *half,
+                             double u[3]) {
+    double cx = filCoord[s], cy = filCoord[C + s], cz = filCoord[2 * C + s];
+    double ux = filUVec[s], uy = filUVec[C + s], uz = filUVec[2 * C + s];
+    double h = 0.5 * filSegLength[s];
+    double e2x = cx + h * ux, e2y = cy + h * uy, e2z = cz + h * uz;
+    double dx = nodeCoord[k] - e2x, dy = nodeCoord[nN + k] - e2y, dz = nodeCoord[2 * nN + k] - e2z;
+    double strain = sqrt(dx * dx + dy * dy + dz * dz);
+    *half = h;
+    u[0] = ux; u[1] = uy; u[2] = uz;
+    if (!(strain > 0.0)) return 0;
+    double inv = 1.0 / strain;
+    double denom = dt * (1.0 / filBTransGam[s] + nodeInvTransX[k]);
+    double fm = (denom > 0.0) ? (fracMove * 1.0e-6 * strain / denom) : 0.0;
+    F[0] = fm * (dx * inv); F[1] = fm * (dy * inv); F[2] = fm * (dz * inv);
+    return 1;
+}
+
+/* NodeNucleationSystem.seedTether (NodeNucleationSystem.java:117-149): force and end2 torque on each held seed */
+void sbo_nuc_seed_tether(int C, int nN, const float *filCoord, const float *filUVec, const float *filSegLength,
+                         const float *filBTransGam, float *filForceSum, float *filTorqueSum, const float *nodeCoord,
+                         const float *nodeInvTransX, const int *seedNode, const float *tetherParams) {
+    double fracMove = tetherParams[0], dt = tetherParams[1];
+    for (int s = 0; s < C; s++) {
+        int k = seedNode[s];
+        if (k < 0) continue;
+        double F[3], half, u[3];
+        if (!seed_tether_force(s, k, C, nN, filCoord, filUVec, filSegLength, filBTransGam, nodeCoord, nodeInvTransX,
+                               fracMove, dt, F, &half, u))
+            continue;
+        filForceSum[s] = (float)(filForceSum[s] + F[0]);
+        filForceSum[C + s] = (float)(filForceSum[C + s] + F[1]);
+        filForceSum[2 * C + s] = (float)(filForceSum[2 * C + s] + F[2]);
+        double Rx = half * u[0] * 1.0e-6, Ry = half * u[1] * 1.0e-6, Rz = half * u[2] * 1.0e-6;
+        filTorqueSum[s] = (float)(filTorqueSum[s] + (Ry * F[2] - Rz * F[1]));
+        filTorqueSum[C + s] = (float)(filTorqueSum[C + s] + (Rz * F[0] - Rx * F[2]));
+        filTorqueSum[2 * C + s] = (float)(filTorqueSum[2 * C + s] + (Rx * F[1] - Ry * F[0]));
+    }
+}
+
+/* NodeNucleationSystem.seedTetherNodeReact (NodeNucleationSystem.java:161-187): each node adds -F of its held
+ * seeds, summed in slot order.  The reference accumulates rx -= fm dx inv; the same products are formed here. */
+void sbo_nuc_seed_tether_node_react(int C, int nN, const float *filCoord, const float *filUVec,
+                                    const float *filSegLength, const float *filBTransGam, float *nodeForceSum,
+                                    const float *nodeCoord, const float *nodeInvTransX, const int *seedNode,
+                                    const float *tetherParams) {
+    double fracMove = tetherParams[0], dt = tetherParams[1];
+    for (int k = 0; k < nN; k++) {
+        double ncx = nodeCoord[k], ncy = nodeCoord[nN + k], ncz = nodeCoord[2 * nN + k];
+        double rx = 0.0, ry = 0.0, rz = 0.0;
+        for (int s = 0; s < C; s++) {
+            if (seedNode[s] != k) continue;
+            double cx = filCoord[s], cy = filCoord[C + s], cz = filCoord[2 * C + s];
+            double ux = filUVec[s], uy = filUVec[C + s], uz = filUVec[2 * C + s];
+            double half = 0.5 * filSegLength[s];
+            double e2x = cx + half * ux, e2y = cy + half * uy, e2z = cz + half * uz;
+            double dx = ncx - e2x, dy = ncy - e2y, dz = ncz - e2z;
+            double strain = sqrt(dx * dx + dy * dy + dz * dz);
+            if (strain > 0.0) {
+                double inv = 1.0 / strain;
+                double denom = dt * (1.0 / filBTransGam[s] + nodeInvTransX[k]);
+                double fm = (denom > 0.0) ? (fracMove * 1.0e-6 * strain / denom) : 0.0;
+                rx -= fm * dx * inv; ry -= fm * dy * inv; rz -= fm * dz * inv;
+            }
+        }
+        nodeForceSum[k] = (float)(nodeForceSum[k] + rx);
+        nodeForceSum[nN + k] = (float)(nodeForceSum[nN + k] + ry);
+        nodeForceSum[2 * nN + k] = (float)(nodeForceSum[2 * nN + k] + rz);
+    }
+}
+
+/* NodeNucleationSystem.seedAimTorque (NodeNucleationSystem.java:196-211) */
+void sbo_nuc_seed_aim_torque(int C, const float *filUVec, const float *filBRotGam, float *filTorqueSum,
+                             const float *aimDir, const float *aimParams) {
+    double aimCoeff = aimParams[0], dt = aimParams[1];
+    for (int s = 0; s < C; s++) {
+        double ax = aimDir[s], ay = aimDir[C + s], az = aimDir[2 * C + s];
+        if (ax * ax + ay * ay + az * az < 0.25) continue;
+        double ux = filUVec[s], uy = filUVec[C + s], uz = filUVec[2 * C + s];
+        double cx = uy * az - uz * ay, cy = uz * ax - ux * az, cz = ux * ay - uy * ax;
+        double k = aimCoeff * filBRotGam[C + s] / dt;
+        filTorqueSum[s] = (float)(filTorqueSum[s] + k * cx);
+        filTorqueSum[C + s] = (float)(filTorqueSum[C + s] + k * cy);
+        filTorqueSum[2 * C + s] = (float)(filTorqueSum[2 * C + s] + k * cz);
+    }
+}
+
+/* NodeNucleationSystem.dissolve (NodeNucleationSystem.java:248-257): a held seed detaches at P_detach */
+void sbo_nuc_dissolve(int C, int *seedNode, const float *dissolveParams, int32_t step, int32_t seed, int rng,
+                      uint32_t stream) {
+    float pDetach = dissolveParams[0];
+    for (int s = 0; s < C; s++) {
+        if (seedNode[s] < 0) continue;
+        float u = sbo_uniform(sbo_event_bits(rng, (uint32_t)s, step, seed, stream, SALT_NUCD));
+        if (u < pDetach) seedNode[s] = -1;
+    }
+}

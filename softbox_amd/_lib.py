@@ -215,6 +215,14 @@ _SIGS = {
     "sbx_node_tether": [vp] + [vp] * 14 + [f32p, i32, i32, i32],
     "sbx_node_tether_bound_stiffen": [vp] + [vp] * 15 + [f32p, i32, i32, i32],
     "sbx_node_xbridge_inject": [vp] + [vp] * 8 + [i32, i32],
+    "sbx_nuc_count_bound_fil": [vp, vp, vp, i32, i32],
+    "sbx_nuc_emit": [vp] + [vp] * 6 + [f32p, i32, i32, i32, i32, i32, i32, u32],
+    "sbx_nuc_tag_seeds": [vp] + [vp] * 4 + [i32, i32],
+    "sbx_nuc_init_newborn": [vp] + [vp] * 5 + [f32p, i32, i32],
+    "sbx_nuc_seed_tether": [vp] + [vp] * 9 + [f32p, i32, i32],
+    "sbx_nuc_seed_tether_node_react": [vp] + [vp] * 8 + [f32p, i32, i32],
+    "sbx_nuc_seed_aim_torque": [vp] + [vp] * 4 + [f32p, i32],
+    "sbx_nuc_dissolve": [vp, vp, f32p, i32, i32, i32, i32, u32],
     "sbx_net_create": [vp, C.POINTER(SbxFilaments), C.POINTER(SbxCrosslinks), C.POINTER(SbxNetParams), C.POINTER(vp)],
     "sbx_net_destroy": [vp],
     "sbx_net_attach_motors": [vp, C.POINTER(SbxMotors), C.POINTER(SbxMotorParams)],

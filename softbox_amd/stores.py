@@ -355,6 +355,60 @@ class NodeStore:
         return o
 
 
+class NodeNucleationStore:
+    """NodeNucleationStore.java:35-110: the node bond of each filament slot (seedNode, -1 free), the per-node
+    count of held seeds and the nucleation, tether and dissolution parameters; the actin pool (ActinPool.java)
+    as a host scalar with its integer ledger."""
+
+    ACTIN_SEED = 3            # Constants.actinSeed
+    K_NODE_NUC = 10.0         # Constants.kNodeNuc, /node/s
+    FRAC_MOVE = 0.5           # Constants.fracMove
+
+    def __init__(self, nNodes: int, filCap: int, pool0: float, box_volume_um3: float):
+        self.nNodes, self.filCap = nNodes, filCap
+        self.seedNode = np.full(filCap, -1, np.int32)
+        self.nodeBoundFil = np.zeros(nNodes, np.int32)
+        self.nucParams = np.zeros(3, np.float32)
+        self.tetherParams = np.zeros(2, np.float32)
+        self.dissolveParams = np.zeros(1, np.float32)
+        self.seedParams = np.array([self.ACTIN_SEED, 0.0], np.float32)
+        self.nucCounts = np.array([nNodes, 0, 0, 0], np.int32)
+        self.conc = float(pool0)
+        self.uMPerMonomer = (1e5 * 1e5 * 1e5 * 1e6) / (box_volume_um3 * 6.022e23)
+        self.taken = self.returned = 0
+
+    def set_nuc_params(self, kNodeNuc, dt, seedLenUm, forminsPerNode):      # NodeNucleationStore.java:85-90
+        self.nucParams[:] = np.array([kNodeNuc * dt, 0.5 * seedLenUm, forminsPerNode], np.float32)
+        self.seedParams[1] = np.float32(seedLenUm)
+
+    def set_tether_params(self, fracMove, dt):                              # :91-94
+        self.tetherParams[:] = np.array([fracMove, dt], np.float32)
+
+    def set_dissolve_params(self, detachRate, dt):                          # :95-97
+        self.dissolveParams[0] = np.float32(detachRate * dt)
+
+    def set_counts(self, step, seed):                                       # :98-101
+        self.nucCounts[1], self.nucCounts[2] = step, seed
+
+    def available(self, monomers: int) -> bool:                            # ActinPool.available
+        return self.conc >= monomers * self.uMPerMonomer
+
+    def take(self, monomers: int) -> None:                                  # ActinPool.take
+        self.conc -= monomers * self.uMPerMonomer
+        if self.conc < 0.0:
+            self.conc = 0.0
+        self.taken += monomers
+
+    def refresh_pool_gate(self) -> None:                                    # :103
+        self.nucCounts[3] = 1 if self.available(self.ACTIN_SEED) else 0
+
+    def deplete_for_births(self, rankOffsets, K: int, freeOffsets, C: int) -> int:   # :107-114
+        births = min(int(rankOffsets[K]), int(freeOffsets[C]))
+        if births > 0:
+            self.take(births * self.ACTIN_SEED)
+        return births
+
+
 class DeviceMotors:
     """Device mirror of a MotorStore + DimerStore + MiniFilamentStore; `struct()` gives the
     sbx_motors of the C ABI and `params()` the sbx_motor_params."""

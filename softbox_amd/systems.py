@@ -402,3 +402,64 @@ class NodeSystem:
         return ctx.check(ctx.lib.sbx_node_xbridge_inject(ctx.h, ptr(bondData), ptr(nUVec), ptr(nYVec), ptr(attachKey),
                                                          ptr(radial), ptr(attachCoeffK), ptr(boundSeg),
                                                          ptr(nodeData), _n3(nCoord), attachCoeffK.numel()))
+
+
+class NodeNucleationSystem:
+    """NodeNucleationSystem.java:55-257: seeds nucleated at nodes, their node bond and tether, its dissolution.
+    nucCounts (host) = [nNodes, step, seed, poolOK]; allocCounts (host) = [C, K] as FilamentBirthSystem's."""
+
+    @staticmethod
+    def countBoundFil(ctx: Context, seedNode, nodeBoundFil, nucCounts):
+        return ctx.check(ctx.lib.sbx_nuc_count_bound_fil(ctx.h, ptr(seedNode), ptr(nodeBoundFil), seedNode.numel(),
+                                                         int(nucCounts[0])))
+
+    @staticmethod
+    def emit(ctx: Context, nodeCoord, nodeBoundFil, acceptFlag, reqCoord, reqUVec, reqYVec, nucParams, nucCounts,
+             rng: int = RNG_WANG, stream: int = 0):
+        npar, npp = _f(nucParams)
+        return ctx.check(ctx.lib.sbx_nuc_emit(ctx.h, ptr(nodeCoord), ptr(nodeBoundFil), ptr(acceptFlag), ptr(reqCoord),
+                                              ptr(reqUVec), ptr(reqYVec), npp, int(nucCounts[1]), int(nucCounts[2]),
+                                              int(nucCounts[3]), int(nucCounts[0]), _n3(reqCoord), rng, stream))
+
+    @staticmethod
+    def tagSeeds(ctx: Context, rankOffsets, freeList, freeOffsets, seedNode, allocCounts):
+        return ctx.check(ctx.lib.sbx_nuc_tag_seeds(ctx.h, ptr(rankOffsets), ptr(freeList), ptr(freeOffsets),
+                                                   ptr(seedNode), int(allocCounts[0]), int(allocCounts[1])))
+
+    @staticmethod
+    def initNewborn(ctx: Context, rankOffsets, freeList, freeOffsets, monomerCount, filSegLength, seedParams,
+                    allocCounts):
+        sp, spp = _f(seedParams)
+        return ctx.check(ctx.lib.sbx_nuc_init_newborn(ctx.h, ptr(rankOffsets), ptr(freeList), ptr(freeOffsets),
+                                                      ptr(monomerCount), ptr(filSegLength), spp, int(allocCounts[0]),
+                                                      int(allocCounts[1])))
+
+    @staticmethod
+    def seedTether(ctx: Context, filCoord, filUVec, filSegLength, filBTransGam, filForceSum, filTorqueSum, nodeCoord,
+                   nodeInvTransX, seedNode, tetherParams):
+        tp, tpp = _f(tetherParams)
+        return ctx.check(ctx.lib.sbx_nuc_seed_tether(ctx.h, ptr(filCoord), ptr(filUVec), ptr(filSegLength),
+                                                     ptr(filBTransGam), ptr(filForceSum), ptr(filTorqueSum),
+                                                     ptr(nodeCoord), ptr(nodeInvTransX), ptr(seedNode), tpp,
+                                                     _n3(filCoord), _n3(nodeCoord)))
+
+    @staticmethod
+    def seedTetherNodeReact(ctx: Context, filCoord, filUVec, filSegLength, filBTransGam, nodeForceSum, nodeCoord,
+                            nodeInvTransX, seedNode, tetherParams):
+        tp, tpp = _f(tetherParams)
+        return ctx.check(ctx.lib.sbx_nuc_seed_tether_node_react(ctx.h, ptr(filCoord), ptr(filUVec), ptr(filSegLength),
+                                                                ptr(filBTransGam), ptr(nodeForceSum), ptr(nodeCoord),
+                                                                ptr(nodeInvTransX), ptr(seedNode), tpp,
+                                                                _n3(filCoord), _n3(nodeCoord)))
+
+    @staticmethod
+    def seedAimTorque(ctx: Context, filUVec, filBRotGam, filTorqueSum, aimDir, aimParams):
+        ap, app = _f(aimParams)
+        return ctx.check(ctx.lib.sbx_nuc_seed_aim_torque(ctx.h, ptr(filUVec), ptr(filBRotGam), ptr(filTorqueSum),
+                                                         ptr(aimDir), app, _n3(filUVec)))
+
+    @staticmethod
+    def dissolve(ctx: Context, seedNode, dissolveParams, nucCounts, rng: int = RNG_WANG, stream: int = 0):
+        dp, dpp = _f(dissolveParams)
+        return ctx.check(ctx.lib.sbx_nuc_dissolve(ctx.h, ptr(seedNode), dpp, int(nucCounts[1]), int(nucCounts[2]),
+                                                  seedNode.numel(), rng, stream))
