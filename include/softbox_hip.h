@@ -292,6 +292,21 @@ int sbx_grid_reachable(sbx_ctx *c, const float *head, const float *uVec, const f
 int sbx_bind_nearest(sbx_ctx *c, const float *head, const float *uVec, const float *rodUVec, const float *segEnd1,
                      const float *segEnd2, const int *motorCandSeg, const int *motorCandCount, int *boundSeg,
                      float *bindArc, const float *kinParams, const int *counts, int nSeg, int maxCand);
+/* BindingDetectionSystem.bindNearestNodeAware (BindingDetectionSystem.java:440-473): bindNearest skipping
+ * segments held by a node (seedNode >= 0) */
+int sbx_bind_nearest_node_aware(sbx_ctx *c, const float *head, const float *uVec, const float *rodUVec,
+                                const float *segEnd1, const float *segEnd2, const int *motorCandSeg,
+                                const int *motorCandCount, int *boundSeg, float *bindArc, const int *seedNode,
+                                const float *kinParams, const int *counts, int nSeg, int maxCand);
+/* BindingDetectionSystem.bruteReachable (BindingDetectionSystem.java:260-283): every segment, in index order */
+int sbx_brute_reachable(sbx_ctx *c, const float *head, const float *uVec, const float *rodUVec, const float *segEnd1,
+                        const float *segEnd2, int *reachSeg, int *reachCount, const float *kinParams,
+                        const int *counts, int nSeg, int maxCand);
+/* BindingDetectionSystem.bruteReachableNodeAware (BindingDetectionSystem.java:410-436) */
+int sbx_brute_reachable_node_aware(sbx_ctx *c, const float *head, const float *uVec, const float *rodUVec,
+                                   const float *segEnd1, const float *segEnd2, int *reachSeg, int *reachCount,
+                                   const int *seedNode, const float *kinParams, const int *counts, int nSeg,
+                                   int maxCand);
 /* NucleotideCycleSystem.catchSlipRelease (NucleotideCycleSystem.java:77-135): unbinding events */
 int sbx_catch_slip_release(sbx_ctx *c, int *boundSeg, const float *forceDotFil, const float *forceMag,
                            int *cooldown, int *stats, int *capStats, const float *kinParams, const int *counts,

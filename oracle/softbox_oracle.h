@@ -142,6 +142,14 @@ void sbo_brute_reachable(int nM, int nSeg, const float *head, const float *uVec,
 void sbo_bind_nearest(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
                       const float *segEnd1, const float *segEnd2, const int *reachSeg, const int *reachCount,
                       int maxCand, int *boundSeg, float *bindArc, const float *kinParams); /* :356-389 */
+void sbo_brute_reachable_node_aware(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
+                                    const float *segEnd1, const float *segEnd2, int maxCand, int *reachSeg,
+                                    int *reachCount, const int *seedNode,
+                                    const float *kinParams);                            /* BindingDetectionSystem.java:410-436 */
+void sbo_bind_nearest_node_aware(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
+                                 const float *segEnd1, const float *segEnd2, const int *reachSeg,
+                                 const int *reachCount, int maxCand, int *boundSeg, float *bindArc,
+                                 const int *seedNode, const float *kinParams);          /* :440-473 */
 void sbo_catch_slip_release(int nM, int *boundSeg, const float *forceDotFil, const float *forceMag, int *cooldown,
                             int *stats, int *capStats, const float *kinParams, int32_t step, int32_t seed, int rng,
                             uint32_t stream);                                             /* NucleotideCycleSystem.java:77-135 */

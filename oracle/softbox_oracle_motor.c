@@ -124,13 +124,14 @@ void sbo_grid_reachable(int nM, int nSeg, const float *head, const float *uVec, 
 }
 
 /* BindingDetectionSystem.bruteReachable, BindingDetectionSystem.java:260-283 */
-void sbo_brute_reachable(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
-                         const float *segEnd1, const float *segEnd2, int maxCand, int *reachSeg, int *reachCount,
-                         const float *kinParams) {
+static void brute_reachable(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
+                            const float *segEnd1, const float *segEnd2, int maxCand, int *reachSeg, int *reachCount,
+                            const int *seedNode, const float *kinParams) {
     float myoColTol = kinParams[7], alignTol = kinParams[8];
     for (int m = 0; m < nM; m++) {
         int out = 0;
         for (int s = 0; s < nSeg; s++) {
+            if (seedNode && seedNode[s] >= 0) continue;   /* node-held tip (BindingDetectionSystem.java:426) */
             float d = reach_test(head[m], head[nM + m], head[2 * nM + m], uVec[m], uVec[nM + m], uVec[2 * nM + m],
                                  rodUVec[m], rodUVec[nM + m], rodUVec[2 * nM + m], segEnd1[s], segEnd1[nSeg + s],
                                  segEnd1[2 * nSeg + s], segEnd2[s], segEnd2[nSeg + s], segEnd2[2 * nSeg + s],
@@ -144,10 +145,45 @@ void sbo_brute_reachable(int nM, int nSeg, const float *head, const float *uVec,
     }
 }
 
+void sbo_brute_reachable(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
+                         const float *segEnd1, const float *segEnd2, int maxCand, int *reachSeg, int *reachCount,
+                         const float *kinParams) {
+    brute_reachable(nM, nSeg, head, uVec, rodUVec, segEnd1, segEnd2, maxCand, reachSeg, reachCount, NULL, kinParams);
+}
+
+/* BindingDetectionSystem.bruteReachableNodeAware (BindingDetectionSystem.java:410-436): segments held by a
+ * node (seedNode >= 0) are never candidates */
+void sbo_brute_reachable_node_aware(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
+                                    const float *segEnd1, const float *segEnd2, int maxCand, int *reachSeg,
+                                    int *reachCount, const int *seedNode, const float *kinParams) {
+    brute_reachable(nM, nSeg, head, uVec, rodUVec, segEnd1, segEnd2, maxCand, reachSeg, reachCount, seedNode,
+                    kinParams);
+}
+
+static void bind_nearest(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
+                         const float *segEnd1, const float *segEnd2, const int *reachSeg, const int *reachCount,
+                         int maxCand, int *boundSeg, float *bindArc, const int *seedNode, const float *kinParams);
+
 /* BindingDetectionSystem.bindNearest, BindingDetectionSystem.java:356-389 */
 void sbo_bind_nearest(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
                       const float *segEnd1, const float *segEnd2, const int *reachSeg, const int *reachCount,
                       int maxCand, int *boundSeg, float *bindArc, const float *kinParams) {
+    bind_nearest(nM, nSeg, head, uVec, rodUVec, segEnd1, segEnd2, reachSeg, reachCount, maxCand, boundSeg, bindArc,
+                 NULL, kinParams);
+}
+
+/* BindingDetectionSystem.bindNearestNodeAware (BindingDetectionSystem.java:440-473) */
+void sbo_bind_nearest_node_aware(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
+                                 const float *segEnd1, const float *segEnd2, const int *reachSeg,
+                                 const int *reachCount, int maxCand, int *boundSeg, float *bindArc,
+                                 const int *seedNode, const float *kinParams) {
+    bind_nearest(nM, nSeg, head, uVec, rodUVec, segEnd1, segEnd2, reachSeg, reachCount, maxCand, boundSeg, bindArc,
+                 seedNode, kinParams);
+}
+
+static void bind_nearest(int nM, int nSeg, const float *head, const float *uVec, const float *rodUVec,
+                         const float *segEnd1, const float *segEnd2, const int *reachSeg, const int *reachCount,
+                         int maxCand, int *boundSeg, float *bindArc, const int *seedNode, const float *kinParams) {
     float myoColTol = kinParams[7], alignTol = kinParams[8];
     for (int m = 0; m < nM; m++) {
         if (boundSeg[m] != FREE_BINDABLE) continue;
@@ -160,6 +196,7 @@ void sbo_bind_nearest(int nM, int nSeg, const float *head, const float *uVec, co
         float bestD = 1.0e30f, bestArc = 0.0f;
         for (int k = 0; k < cnt; k++) {
             int s = reachSeg[(size_t)m * maxCand + k];
+            if (seedNode && seedNode[s] >= 0) continue;
             float e1x = segEnd1[s], e1y = segEnd1[nSeg + s], e1z = segEnd1[2 * nSeg + s];
             float e2x = segEnd2[s], e2y = segEnd2[nSeg + s], e2z = segEnd2[2 * nSeg + s];
             float d = reach_test(mx, my, mz, mux, muy, muz, rux, ruy, ruz, e1x, e1y, e1z, e2x, e2y, e2z, myoColTol,

@@ -215,6 +215,9 @@ _SIGS = {
     "sbx_node_tether": [vp] + [vp] * 14 + [f32p, i32, i32, i32],
     "sbx_node_tether_bound_stiffen": [vp] + [vp] * 15 + [f32p, i32, i32, i32],
     "sbx_node_xbridge_inject": [vp] + [vp] * 8 + [i32, i32],
+    "sbx_bind_nearest_node_aware": [vp] + [vp] * 10 + [f32p, i32p, i32, i32],
+    "sbx_brute_reachable": [vp] + [vp] * 7 + [f32p, i32p, i32, i32],
+    "sbx_brute_reachable_node_aware": [vp] + [vp] * 8 + [f32p, i32p, i32, i32],
     "sbx_nuc_count_bound_fil": [vp, vp, vp, i32, i32],
     "sbx_nuc_emit": [vp] + [vp] * 6 + [f32p, i32, i32, i32, i32, i32, i32, u32],
     "sbx_nuc_tag_seeds": [vp] + [vp] * 4 + [i32, i32],

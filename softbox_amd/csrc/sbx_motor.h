@@ -144,7 +144,8 @@ __device__ __forceinline__ void bind_nearest_one(int m, int nM, int nSeg, const 
                                                  const float *__restrict__ hu, const float *__restrict__ ru,
                                                  const float *__restrict__ e1, const float *__restrict__ e2,
                                                  const int *__restrict__ reachSeg, int reachCount, int maxCand,
-                                                 int &bs, float &arc, float myoColTol, float alignTol) {
+                                                 int &bs, float &arc, float myoColTol, float alignTol,
+                                                 const int *__restrict__ seedNode = nullptr) {
     if (bs != FREE_BINDABLE) return;
     float mx = head[m], my = head[nM + m], mz = head[2 * nM + m];
     float mux = hu[m], muy = hu[nM + m], muz = hu[2 * nM + m];
@@ -154,6 +155,7 @@ __device__ __forceinline__ void bind_nearest_one(int m, int nM, int nSeg, const 
     float bestD = 1.0e30f, bestArc = 0.0f;
     for (int k = 0; k < cnt; k++) {
         int s = reachSeg[(size_t)m * maxCand + k];
+        if (seedNode && seedNode[s] >= 0) continue;   // node-held tip (bindNearestNodeAware)
         float e1x = e1[s], e1y = e1[nSeg + s], e1z = e1[2 * nSeg + s];
         float e2x = e2[s], e2y = e2[nSeg + s], e2z = e2[2 * nSeg + s];
         float d = reach_test(mx, my, mz, mux, muy, muz, rux, ruy, ruz, e1x, e1y, e1z, e2x, e2y, e2z, myoColTol, alignTol);

@@ -98,15 +98,42 @@ __global__ void __launch_bounds__(T) k_bind_nearest(const float *__restrict__ he
                                                     const float *__restrict__ e2, const int *__restrict__ reachSeg,
                                                     const int *__restrict__ reachCount, int maxCand,
                                                     int *__restrict__ boundSeg, float *__restrict__ bindArc, float tol,
-                                                    float align, int nM, int nSeg) {
+                                                    float align, int nM, int nSeg,
+                                                    const int *__restrict__ seedNode) {
     int m = blockIdx.x * T + threadIdx.x;
     if (m >= nM) return;
     int bs = boundSeg[m];
     if (bs != FREE_BINDABLE) return;
     float arc = bindArc[m];
-    bind_nearest_one(m, nM, nSeg, head, hu, ru, e1, e2, reachSeg, reachCount[m], maxCand, bs, arc, tol, align);
+    bind_nearest_one(m, nM, nSeg, head, hu, ru, e1, e2, reachSeg, reachCount[m], maxCand, bs, arc, tol, align,
+                     seedNode);
     boundSeg[m] = bs;
     bindArc[m] = arc;
+}
+
+// BindingDetectionSystem.bruteReachable / bruteReachableNodeAware (BindingDetectionSystem.java:260-283,
+// 410-436): every segment in index order against one head; the segment loads are uniform across the wavefront
+__global__ void __launch_bounds__(T) k_brute_reachable(const float *__restrict__ head, const float *__restrict__ hu,
+                                                       const float *__restrict__ ru, const float *__restrict__ e1,
+                                                       const float *__restrict__ e2, int *__restrict__ reachSeg,
+                                                       int *__restrict__ reachCount, const int *__restrict__ seedNode,
+                                                       float tol, float align, int nM, int nSeg, int maxCand) {
+    int m = blockIdx.x * T + threadIdx.x;
+    if (m >= nM) return;
+    float mx = head[m], my = head[nM + m], mz = head[2 * nM + m];
+    float mux = hu[m], muy = hu[nM + m], muz = hu[2 * nM + m];
+    float rux = ru[m], ruy = ru[nM + m], ruz = ru[2 * nM + m];
+    int out = 0;
+    for (int s = 0; s < nSeg; s++) {
+        if (seedNode && seedNode[s] >= 0) continue;
+        float d = reach_test(mx, my, mz, mux, muy, muz, rux, ruy, ruz, e1[s], e1[nSeg + s], e1[2 * nSeg + s], e2[s],
+                             e2[nSeg + s], e2[2 * nSeg + s], tol, align);
+        if (d >= 0.0f) {
+            if (out < maxCand) reachSeg[(size_t)m * maxCand + out] = s;
+            out++;
+        }
+    }
+    reachCount[m] = out;
 }
 
 __global__ void __launch_bounds__(T) k_catch_slip(int *__restrict__ boundSeg, const float *__restrict__ fdf,
@@ -360,8 +387,57 @@ int sbx_bind_nearest(sbx_ctx *c, const float *head, const float *uVec, const flo
                 "bind_nearest: null array");
     hipLaunchKernelGGL(k_bind_nearest, dim3(blocks_for(nM, T)), dim3(T), 0, c->stream, head, uVec, rodUVec, segEnd1,
                        segEnd2, motorCandSeg, motorCandCount, maxCand, boundSeg, bindArc, kinParams[7], kinParams[8],
-                       nM, nSeg);
+                       nM, nSeg, (const int *)nullptr);
     CHK(c, "k_bind_nearest");
+}
+
+int sbx_bind_nearest_node_aware(sbx_ctx *c, const float *head, const float *uVec, const float *rodUVec,
+                                const float *segEnd1, const float *segEnd2, const int *motorCandSeg,
+                                const int *motorCandCount, int *boundSeg, float *bindArc, const int *seedNode,
+                                const float *kinParams, const int *counts, int nSeg, int maxCand) {
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, kinParams && counts && maxCand > 0 && nSeg >= 0, "bind_nearest_node_aware: bad params");
+    int nM = counts[0];
+    if (nM <= 0) return SBX_OK;
+    SBX_REQUIRE(c, head && uVec && rodUVec && segEnd1 && segEnd2 && motorCandSeg && motorCandCount && boundSeg &&
+                       bindArc && seedNode,
+                "bind_nearest_node_aware: null array");
+    hipLaunchKernelGGL(k_bind_nearest, dim3(blocks_for(nM, T)), dim3(T), 0, c->stream, head, uVec, rodUVec, segEnd1,
+                       segEnd2, motorCandSeg, motorCandCount, maxCand, boundSeg, bindArc, kinParams[7], kinParams[8],
+                       nM, nSeg, seedNode);
+    CHK(c, "k_bind_nearest(node aware)");
+}
+
+static int brute_launch(sbx_ctx *c, const char *what, const float *head, const float *uVec, const float *rodUVec,
+                        const float *segEnd1, const float *segEnd2, int *reachSeg, int *reachCount,
+                        const int *seedNode, const float *kinParams, const int *counts, int nSeg, int maxCand) {
+    SBX_REQUIRE(c, kinParams && counts && maxCand > 0 && nSeg >= 0, "brute_reachable: bad params");
+    int nM = counts[0];
+    if (nM <= 0) return SBX_OK;
+    SBX_REQUIRE(c, head && uVec && rodUVec && reachSeg && reachCount && (nSeg == 0 || (segEnd1 && segEnd2)),
+                "brute_reachable: null array");
+    hipLaunchKernelGGL(k_brute_reachable, dim3(blocks_for(nM, T)), dim3(T), 0, c->stream, head, uVec, rodUVec,
+                       segEnd1, segEnd2, reachSeg, reachCount, seedNode, kinParams[7], kinParams[8], nM, nSeg,
+                       maxCand);
+    CHK(c, what);
+}
+
+int sbx_brute_reachable(sbx_ctx *c, const float *head, const float *uVec, const float *rodUVec, const float *segEnd1,
+                        const float *segEnd2, int *reachSeg, int *reachCount, const float *kinParams,
+                        const int *counts, int nSeg, int maxCand) {
+    SBX_CHECK_CTX(c);
+    return brute_launch(c, "k_brute_reachable", head, uVec, rodUVec, segEnd1, segEnd2, reachSeg, reachCount, nullptr,
+                        kinParams, counts, nSeg, maxCand);
+}
+
+int sbx_brute_reachable_node_aware(sbx_ctx *c, const float *head, const float *uVec, const float *rodUVec,
+                                   const float *segEnd1, const float *segEnd2, int *reachSeg, int *reachCount,
+                                   const int *seedNode, const float *kinParams, const int *counts, int nSeg,
+                                   int maxCand) {
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, seedNode || nSeg == 0, "brute_reachable_node_aware: null seedNode");
+    return brute_launch(c, "k_brute_reachable(node aware)", head, uVec, rodUVec, segEnd1, segEnd2, reachSeg,
+                        reachCount, seedNode, kinParams, counts, nSeg, maxCand);
 }
 
 int sbx_catch_slip_release(sbx_ctx *c, int *boundSeg, const float *forceDotFil, const float *forceMag,

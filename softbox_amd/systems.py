@@ -271,6 +271,38 @@ class BindingDetectionSystem:
                                                   ptr(boundSeg), ptr(bindArc), kpp, cnp, _n3(segEnd1),
                                                   motorCandSeg.numel() // max(1, nM)))
 
+    @staticmethod
+    def bindNearestNodeAware(ctx: Context, head, uVec, rodUVec, segEnd1, segEnd2, motorCandSeg, motorCandCount,
+                             boundSeg, bindArc, seedNode, kinParams, counts):
+        kp, kpp = _f(kinParams)
+        cn, cnp = _i(counts)
+        nM = int(cn[0])
+        return ctx.check(ctx.lib.sbx_bind_nearest_node_aware(ctx.h, ptr(head), ptr(uVec), ptr(rodUVec), ptr(segEnd1),
+                                                             ptr(segEnd2), ptr(motorCandSeg), ptr(motorCandCount),
+                                                             ptr(boundSeg), ptr(bindArc), ptr(seedNode), kpp, cnp,
+                                                             _n3(segEnd1), motorCandSeg.numel() // max(1, nM)))
+
+    @staticmethod
+    def bruteReachable(ctx: Context, head, uVec, rodUVec, segEnd1, segEnd2, bruteReachSeg, bruteReachCount,
+                       kinParams, counts):
+        kp, kpp = _f(kinParams)
+        cn, cnp = _i(counts)
+        nM = int(cn[0])
+        return ctx.check(ctx.lib.sbx_brute_reachable(ctx.h, ptr(head), ptr(uVec), ptr(rodUVec), ptr(segEnd1),
+                                                     ptr(segEnd2), ptr(bruteReachSeg), ptr(bruteReachCount), kpp, cnp,
+                                                     _n3(segEnd1), bruteReachSeg.numel() // max(1, nM)))
+
+    @staticmethod
+    def bruteReachableNodeAware(ctx: Context, head, uVec, rodUVec, segEnd1, segEnd2, bruteReachSeg, bruteReachCount,
+                                seedNode, kinParams, counts):
+        kp, kpp = _f(kinParams)
+        cn, cnp = _i(counts)
+        nM = int(cn[0])
+        return ctx.check(ctx.lib.sbx_brute_reachable_node_aware(ctx.h, ptr(head), ptr(uVec), ptr(rodUVec),
+                                                                ptr(segEnd1), ptr(segEnd2), ptr(bruteReachSeg),
+                                                                ptr(bruteReachCount), ptr(seedNode), kpp, cnp,
+                                                                _n3(segEnd1), bruteReachSeg.numel() // max(1, nM)))
+
 
 class NucleotideCycleSystem:
     """NucleotideCycleSystem.java:33-135: catch-slip unbinding and the ATPase cycle."""

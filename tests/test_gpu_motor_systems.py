@@ -110,6 +110,55 @@ def test_grid_reachable_and_bind_nearest(ctx, state):
     assert_same(H(ctx, darc), arc, "bindArc")
 
 
+def test_brute_and_node_aware_binding(ctx, state):
+    """bruteReachable (segment order) and the node-aware pair (BindingDetectionSystem.java:410-473) with a third of
+    the in-reach segments held by a node: candidates and bindings equal the oracle's."""
+    o = state
+    mot, nM, n = o.mot, o.mot.nM, o.fs.n
+    mc = int(o.p["maxCand"])
+    counts = np.array([nM, 0, 0, n], np.int32)
+    kp = mot.kinParams.copy()
+    kp[7] = np.float32(0.06)                         # a 60 nm reach: many heads with several candidates
+    dhead, dhu, dru = D(ctx, mot.head), D(ctx, mot.uVec), D(ctx, mot.rodUVec)
+    de1, de2 = D(ctx, o.fs.end1), D(ctx, o.fs.end2)
+    rs, rc = np.full(nM * mc, -1, np.int32), np.zeros(nM, np.int32)
+    ffi.call("sbo_brute_reachable", nM, n, mot.head, mot.uVec, mot.rodUVec, o.fs.end1, o.fs.end2, mc, rs, rc,
+             kp)
+    drs, drc = D(ctx, np.full_like(rs, -1)), D(ctx, np.zeros_like(rc))
+    BindingDetectionSystem.bruteReachable(ctx, dhead, dhu, dru, de1, de2, drs, drc, kp, counts)
+    assert rc.sum() > 0
+    assert_same(H(ctx, drc), rc, "brute reachCount")
+    for m in np.flatnonzero(rc):
+        c = min(int(rc[m]), mc)
+        assert_same(H(ctx, drs).reshape(nM, mc)[m, :c], rs.reshape(nM, mc)[m, :c], f"brute reachSeg[{m}]")
+    seedNode = np.full(n, -1, np.int32)
+    held = np.unique(rs[rs >= 0])[::3]
+    seedNode[held] = 0
+    rs2, rc2 = np.full(nM * mc, -1, np.int32), np.zeros(nM, np.int32)
+    ffi.call("sbo_brute_reachable_node_aware", nM, n, mot.head, mot.uVec, mot.rodUVec, o.fs.end1, o.fs.end2, mc, rs2,
+             rc2, seedNode, kp)
+    dsn = D(ctx, seedNode)
+    drs2, drc2 = D(ctx, np.full_like(rs2, -1)), D(ctx, np.zeros_like(rc2))
+    BindingDetectionSystem.bruteReachableNodeAware(ctx, dhead, dhu, dru, de1, de2, drs2, drc2, dsn, kp,
+                                                   counts)
+    assert rc2.sum() < rc.sum()
+    assert_same(H(ctx, drc2), rc2, "node-aware reachCount")
+    for m in np.flatnonzero(rc2):
+        c = min(int(rc2[m]), mc)
+        assert_same(H(ctx, drs2).reshape(nM, mc)[m, :c], rs2.reshape(nM, mc)[m, :c], f"node-aware reachSeg[{m}]")
+    bs0 = np.full(nM, -1, np.int32)
+    arc0 = np.zeros(nM, np.float32)
+    bs, arc = bs0.copy(), arc0.copy()
+    ffi.call("sbo_bind_nearest_node_aware", nM, n, mot.head, mot.uVec, mot.rodUVec, o.fs.end1, o.fs.end2, rs, rc, mc,
+             bs, arc, seedNode, kp)
+    dbs, darc = D(ctx, bs0), D(ctx, arc0)
+    BindingDetectionSystem.bindNearestNodeAware(ctx, dhead, dhu, dru, de1, de2, D(ctx, rs), D(ctx, rc), dbs, darc,
+                                                dsn, kp, counts)
+    assert (bs >= 0).sum() > 0 and not np.isin(bs[bs >= 0], held).any()
+    assert_same(H(ctx, dbs), bs, "node-aware boundSeg")
+    assert_same(H(ctx, darc), arc, "node-aware bindArc")
+
+
 @pytest.mark.parametrize("rng", [RNG_PHILOX, RNG_WANG])
 def test_catch_slip_and_cycle(ctx, state, rng):
     o = state

@@ -319,3 +319,39 @@ def test_force_dot_ring():
     assert sorted(hist.tolist()) == list(range(4, 14))
     ffi.call("sbo_register_force_dot", nM, bond, np.array([-1], np.int32), fdf, fm, hist, place)
     assert place[0] == 0 and fdf[0] == 0 and fm[0] == 0 and not hist.any()
+
+
+def test_node_aware_binding_skips_only_held_segments():
+    """bruteReachableNodeAware = bruteReachable minus the node-held segments, in the same order; bindNearestNodeAware
+    binds as bindNearest does over that filtered list (BindingDetectionSystem.java:410-473)."""
+    g = np.random.default_rng(12)
+    nM, nSeg = 40, 60
+    e1 = g.uniform(-0.05, 0.05, (nSeg, 3))
+    u = np.tile([1.0, 0.0, 0.0], (nSeg, 1))
+    e2 = e1 + 0.1 * u
+    head = e1[g.integers(0, nSeg, nM)] + np.array([0.05, 0.01, 0.0]) + g.normal(0, 0.004, (nM, 3))
+    hu = np.tile([0.6, -0.8, 0.0], (nM, 1))
+    ru = np.tile([1.0, 0.0, 0.0], (nM, 1))
+    mc = 16
+    kp = kin()
+    args = (nM, nSeg, planar(head), planar(hu), planar(ru), planar(e1), planar(e2), mc)
+    rs, rc = np.full(nM * mc, -1, np.int32), np.zeros(nM, np.int32)
+    ffi.call("sbo_brute_reachable", *args, rs, rc, kp)
+    assert rc.sum() > 0
+    seedNode = np.full(nSeg, -1, np.int32)
+    seedNode[::4] = 2
+    rs2, rc2 = np.full(nM * mc, -1, np.int32), np.zeros(nM, np.int32)
+    ffi.call("sbo_brute_reachable_node_aware", *args, rs2, rc2, seedNode, kp)
+    R, R2 = rs.reshape(nM, mc), rs2.reshape(nM, mc)
+    for m in range(nM):
+        full = [s for s in R[m, : min(rc[m], mc)] if seedNode[s] < 0]
+        assert rc2[m] == sum(1 for s in range(nSeg) if seedNode[s] < 0 and s in set(R[m, : min(rc[m], mc)]))
+        assert list(R2[m, : min(rc2[m], mc)]) == full
+    bs, arc = np.full(nM, -1, np.int32), np.zeros(nM, np.float32)
+    ffi.call("sbo_bind_nearest_node_aware", nM, nSeg, planar(head), planar(hu), planar(ru), planar(e1), planar(e2),
+             rs, rc, mc, bs, arc, seedNode, kp)
+    bs2, arc2 = np.full(nM, -1, np.int32), np.zeros(nM, np.float32)
+    ffi.call("sbo_bind_nearest", nM, nSeg, planar(head), planar(hu), planar(ru), planar(e1), planar(e2), rs2, rc2, mc,
+             bs2, arc2, kp)
+    assert np.array_equal(bs, bs2) and np.array_equal(arc, arc2)
+    assert (bs >= 0).any() and not (seedNode[bs[bs >= 0]] >= 0).any()
