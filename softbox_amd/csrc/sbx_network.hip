@@ -128,7 +128,7 @@ struct sbx_net {
     // launch shape (results do not depend on it; sbx_net_tune picks it by timing steps)
     int bpGroup = 4;      // broad-phase lanes per body (2, 4 or 8)
     int bpStaged = 0;     // 1: each block's neighbourhood staged in LDS (4 or 8 lanes)
-    int rodThreads = 256; // k_rod block size (64, 128 or 256)
+    int rodThreads = 256; // k_rod block size (64, 128 or 256), or RODS_PAIRED
     // graph
     int useGraph = 0;
     int graphSteps = 0;  // non-zero while a capture is being recorded
@@ -174,6 +174,10 @@ const char *kTimerNames[T_COUNT_TIMERS] = {"k_body_cell", "grid_sort",     "k_br
                                            "k_bond_head", "k_motor_integ", "motor_csr", "skin_rebuild",
                                            "k_skin_pairs"};
 
+// k_rod launch form selected by rodThreads == 512: 256-thread blocks, two rod groups per block, two waves
+// drawing the noise of the other two (k_rod<*, true>)
+constexpr int RODS_PAIRED = 512;
+
 struct RodArgs {
     const float *coord, *uVec, *yVec;
     float *coordOut, *uOut, *yOut, *zVec, *end1, *end2;
@@ -210,95 +214,145 @@ struct RodArgs {
 // once per rod; lane 0 of a wave only supplies the joint at the end2 of the previous wave's last rod.
 // BATCH: the rods of many replicas, each binned in its replica's grid and drawing its noise under the
 // replica's seed and own rod index (so every replica steps exactly as it would alone).
-template <bool BATCH>
+// PAIRED: blocks of four waves for two rod groups; waves 2 and 3 draw the Brownian noise of the groups of
+// waves 0 and 1 into LDS while those run the chain and the gathers, so the draw (about a third of the
+// kernel's instructions) leaves the critical path of the waves that integrate.  The draw is the same
+// function of (rod, step, seed), so results do not depend on the form.
+template <bool BATCH, bool PAIRED>
 __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
+    __shared__ float noiseLds[PAIRED ? 2 * 64 * 6 : 1];
     const int n = a.n;
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (wave * CHAIN_SPAN - 1 >= n) return;  // the whole wave is past the last rod
+    int wave, slot = 0;
+    bool drawer = false;
+    if constexpr (PAIRED) {
+        const int wib = threadIdx.x >> 6;
+        slot = wib & 1;
+        drawer = wib >= 2;
+        wave = blockIdx.x * 2 + slot;
+    } else {
+        wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    }
+    const bool waveLive = wave * CHAIN_SPAN - 1 < n;
+    if constexpr (!PAIRED) {
+        if (!waveLive) return;  // the whole wave is past the last rod
+    }
     const int i = wave * CHAIN_SPAN + lane - 1;
-    const bool live = i >= 0 && i < n;
+    const bool live = waveLive && i >= 0 && i < n;
     const bool own = live && lane > 0;
     const int ic = live ? i : (i < 0 ? 0 : n - 1);  // in-range index for the loads of idle lanes
-    Pose p;
-    float gT[3], gR[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        p.c[k] = a.coord[k * n + ic];
-        p.u[k] = a.uVec[k * n + ic];
-        p.y[k] = a.yVec[k * n + ic];
-        gT[k] = a.bTransGam[k * n + ic];
-        gR[k] = a.bRotGam[k * n + ic];
-    }
-    const float segLen = a.segLength[ic], tS = a.bts[ic], rS = a.brs[ic];
-    // the crosslink CSR bounds and the step number are loaded with the pose, so their latency runs
-    // under the chain's arithmetic rather than after it
-    int ra0 = 0, ra1 = 0, rb0 = 0, rb1 = 0;
-    if (a.hasLinks) {
-        ra0 = a.offA[ic]; ra1 = a.offA[ic + 1];
-        rb0 = a.offB[ic]; rb1 = a.offB[ic + 1];
-    }
     const int step = *a.dStep + a.stepOff;
-    float f[3] = {0.0f, 0.0f, 0.0f}, t[3] = {0.0f, 0.0f, 0.0f};
-    if (a.chainOn) {
-        double fd[3] = {0.0, 0.0, 0.0}, td[3] = {0.0, 0.0, 0.0};
-        chain_body_wave(n, ic, live, own, lane, p.c, p.u, segLen, gT, gR, a.coord, a.uVec, a.segLength, a.bTransGam,
-                        a.bRotGam, a.e2Slot[ic], a.e2Side[ic], a.e1Slot[ic], a.e1Side[ic], cp, fd, td);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            f[k] = (float)(f[k] + fd[k]);
-            t[k] = (float)(t[k] + td[k]);
-        }
-    }
-    if (!own) return;
-    uint32_t body = (uint32_t)i;
+    uint32_t body = (uint32_t)ic;
     int seed = a.seed, cellBase = 0;
     GridGeom geom = a.geom;
     if constexpr (BATCH) {
-        const RepInfo &R = a.reps[a.rodRep[i]];
-        body = (uint32_t)(i - R.rodBase);
+        const RepInfo &R = a.reps[a.rodRep[ic]];
+        body = (uint32_t)(ic - R.rodBase);
         seed = R.seed;
         cellBase = R.cellBase;
         geom = R.g;
     }
-    if (a.hasLinks) {
-        double fa[3] = {0, 0, 0}, ta[3] = {0, 0, 0};
-        int active = 0;
-        gather_range(ra0, ra1, a.idxA, a.xlinkData, a.linkState, 0, fa, ta, active);
+    float gf[3], gt[3];
+    if constexpr (PAIRED) {
+        if (drawer) {
+            if (own) {
+                brownian_noise(a.rng, body, step, seed, a.stream, gf, gt);
+                float *q = noiseLds + (slot * 64 + lane) * 6;
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            f[k] = (float)(f[k] + fa[k]);
-            t[k] = (float)(t[k] + ta[k]);
-        }
-        double fb[3] = {0, 0, 0}, tb[3] = {0, 0, 0};
-        gather_range(rb0, rb1, a.idxB, a.xlinkData, a.linkState, 6, fb, tb, active);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            f[k] = (float)(f[k] + fb[k]);
-            t[k] = (float)(t[k] + tb[k]);
-        }
-        if (a.activeLinkCount) a.activeLinkCount[i] = active;
-    }
-    if (a.hasMotors) {  // CrossBridgeSystem.segGather (CrossBridgeSystem.java:302-320)
-        double fm[3] = {0, 0, 0}, tm[3] = {0, 0, 0};
-        csr_payload_sum(i, a.mOff, a.mList, a.bondData, XB_STRIDE, 6, fm, tm);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            f[k] = (float)(f[k] + fm[k]);
-            t[k] = (float)(t[k] + tm[k]);
+                for (int k = 0; k < 3; k++) {
+                    q[k] = gf[k];
+                    q[3 + k] = gt[k];
+                }
+            }
+            __syncthreads();
+            return;
         }
     }
-    if (a.confineOn && !(a.checkInt > 1 && (step % a.checkInt) != 0))
-        confine_body(p.c, p.u, segLen, gT[0], a.tau, a.halfX, a.halfY, a.halfZ, a.coeff, f, t);
-    if (a.forceSum) {
+    Pose p;
+    float gT[3], gR[3];
+    float segLen = 0.0f, tS = 0.0f, rS = 0.0f;
+    float f[3] = {0.0f, 0.0f, 0.0f}, t[3] = {0.0f, 0.0f, 0.0f};
+    if (waveLive) {  // wave-uniform
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            a.forceSum[k * n + i] = f[k];
-            a.torqueSum[k * n + i] = t[k];
+            p.c[k] = a.coord[k * n + ic];
+            p.u[k] = a.uVec[k * n + ic];
+            p.y[k] = a.yVec[k * n + ic];
+            gT[k] = a.bTransGam[k * n + ic];
+            gR[k] = a.bRotGam[k * n + ic];
+        }
+        segLen = a.segLength[ic]; tS = a.bts[ic]; rS = a.brs[ic];
+        // the crosslink CSR bounds are loaded with the pose, so their latency runs under the chain's
+        // arithmetic rather than after it
+        int ra0 = 0, ra1 = 0, rb0 = 0, rb1 = 0;
+        if (a.hasLinks) {
+            ra0 = a.offA[ic]; ra1 = a.offA[ic + 1];
+            rb0 = a.offB[ic]; rb1 = a.offB[ic + 1];
+        }
+        if (a.chainOn) {
+            double fd[3] = {0.0, 0.0, 0.0}, td[3] = {0.0, 0.0, 0.0};
+            chain_body_wave(n, ic, live, own, lane, p.c, p.u, segLen, gT, gR, a.coord, a.uVec, a.segLength,
+                            a.bTransGam, a.bRotGam, a.e2Slot[ic], a.e2Side[ic], a.e1Slot[ic], a.e1Side[ic], cp, fd,
+                            td);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                f[k] = (float)(f[k] + fd[k]);
+                t[k] = (float)(t[k] + td[k]);
+            }
+        }
+        if (own) {
+            if (a.hasLinks) {
+                double fa[3] = {0, 0, 0}, ta[3] = {0, 0, 0};
+                int active = 0;
+                gather_range(ra0, ra1, a.idxA, a.xlinkData, a.linkState, 0, fa, ta, active);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    f[k] = (float)(f[k] + fa[k]);
+                    t[k] = (float)(t[k] + ta[k]);
+                }
+                double fb[3] = {0, 0, 0}, tb[3] = {0, 0, 0};
+                gather_range(rb0, rb1, a.idxB, a.xlinkData, a.linkState, 6, fb, tb, active);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    f[k] = (float)(f[k] + fb[k]);
+                    t[k] = (float)(t[k] + tb[k]);
+                }
+                if (a.activeLinkCount) a.activeLinkCount[i] = active;
+            }
+            if (a.hasMotors) {  // CrossBridgeSystem.segGather (CrossBridgeSystem.java:302-320)
+                double fm[3] = {0, 0, 0}, tm[3] = {0, 0, 0};
+                csr_payload_sum(i, a.mOff, a.mList, a.bondData, XB_STRIDE, 6, fm, tm);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    f[k] = (float)(f[k] + fm[k]);
+                    t[k] = (float)(t[k] + tm[k]);
+                }
+            }
+            if (a.confineOn && !(a.checkInt > 1 && (step % a.checkInt) != 0))
+                confine_body(p.c, p.u, segLen, gT[0], a.tau, a.halfX, a.halfY, a.halfZ, a.coeff, f, t);
+            if (a.forceSum) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    a.forceSum[k * n + i] = f[k];
+                    a.torqueSum[k * n + i] = t[k];
+                }
+            }
         }
     }
-    float gf[3], gt[3], rf[3], rt[3];
-    brownian_noise(a.rng, body, step, seed, a.stream, gf, gt);
+    if constexpr (PAIRED) {
+        __syncthreads();
+        if (!own) return;
+        const float *q = noiseLds + (slot * 64 + lane) * 6;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            gf[k] = q[k];
+            gt[k] = q[3 + k];
+        }
+    } else {
+        if (!own) return;
+        brownian_noise(a.rng, body, step, seed, a.stream, gf, gt);
+    }
+    float rf[3], rt[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         rf[k] = brown_amp(tS, a.bfm, gT[k], gf[k]);
@@ -1532,9 +1586,14 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
     {
         Scope sc(net, T_ROD);
         const long long waves = ((long long)n + CHAIN_SPAN - 1) / CHAIN_SPAN;
-        auto kern = net->reps ? k_rod<true> : k_rod<false>;
-        hipLaunchKernelGGL(kern, dim3(blocks_for(waves * 64, net->rodThreads)), dim3(net->rodThreads), 0, s, a,
-                           net->cp);
+        if (net->rodThreads == RODS_PAIRED) {  // two rod groups per block of four waves
+            auto kern = net->reps ? k_rod<true, true> : k_rod<false, true>;
+            hipLaunchKernelGGL(kern, dim3((unsigned)((waves + 1) / 2)), dim3(256), 0, s, a, net->cp);
+        } else {
+            auto kern = net->reps ? k_rod<true, false> : k_rod<false, false>;
+            hipLaunchKernelGGL(kern, dim3(blocks_for(waves * 64, net->rodThreads)), dim3(net->rodThreads), 0, s,
+                               a, net->cp);
+        }
     }
     if (ovl && (r = check_hip(c, hipEventRecord(net->evRod[cb], s), "record evRod"))) return r;
     net->cur = nxt;
@@ -2577,7 +2636,8 @@ int sbx_net_set_shape(sbx_net *net, int bpGroup, int bpStaged, int scanItems, in
     SBX_REQUIRE(c, bpStaged == 0 || (bpStaged == 1 && bpGroup != 2), "net_set_shape: bpStaged 0, or 1 with 4 or 8 "
                    "lanes per body");
     SBX_REQUIRE(c, scanItems == 4 || scanItems == 8 || scanItems == 16, "net_set_shape: scanItems 4, 8 or 16");
-    SBX_REQUIRE(c, rodThreads == 64 || rodThreads == 128 || rodThreads == 256, "net_set_shape: rodThreads 64, 128 or 256");
+    SBX_REQUIRE(c, rodThreads == 64 || rodThreads == 128 || rodThreads == 256 || rodThreads == RODS_PAIRED,
+                "net_set_shape: rodThreads 64, 128, 256 or 512 (the noise-paired form)");
     int r = set_device(c);
     if (!r) r = drop_graph(net);
     if (r) return r;
@@ -2598,7 +2658,7 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if ((r = check_hip(c, hipEventCreate(&e0), "tune event"))) return r;
     if ((r = check_hip(c, hipEventCreate(&e1), "tune event"))) { hipEventDestroy(e0); return r; }
-    const int groups[3] = {4, 2, 8}, items[3] = {8, 4, 16}, threads[2] = {256, 128};
+    const int groups[3] = {4, 2, 8}, items[3] = {8, 4, 16}, threads[3] = {256, 128, RODS_PAIRED};
     const int overlaps = net->side ? 2 : 1;  // with streams: the overlapped and the one-stream step
     int t = step0, best[5] = {net->bpGroup, net->bpStaged, net->sort.scanItems, net->rodThreads, net->overlap};
     float bestMs = -1.0f;
@@ -2606,7 +2666,7 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
     for (int gi = 0; gi < 3 && !r; gi++)
     for (int si = 0; si < 2 && !r; si++)
         for (int ii = 0; ii < 3 && !r; ii++)
-            for (int ti = 0; ti < 2 && !r; ti++) {
+            for (int ti = 0; ti < 3 && !r; ti++) {
                 if (si == 1 && groups[gi] == 2) continue;  // staging takes 4 or 8 lanes per body
                 if ((r = sbx_net_set_shape(net, groups[gi], si, items[ii], threads[ti]))) break;
                 net->overlap = oi == 0 ? 1 : 0;

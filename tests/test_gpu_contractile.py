@@ -61,13 +61,14 @@ def _compare(net, orc, sc, what):
         assert_same(rs[m, :c], want[m, :c], f"{what}: reachSeg[{m}]")
 
 
-@pytest.mark.parametrize("rng", [RNG_PHILOX, RNG_WANG])
-def test_contractile_steps_match_oracle(ctx, rng):
+@pytest.mark.parametrize("rng,rod_threads", [(RNG_PHILOX, 256), (RNG_WANG, 256), (RNG_WANG, 512)])
+def test_contractile_steps_match_oracle(ctx, rng, rod_threads):
     from softbox_amd.network import Network
 
     sc = _scene(seed=3, rng=rng)
     orc = ffi.OracleContractile(sc)
     net = Network(ctx, sc)
+    net.set_shape(4, 8, rod_threads, 0)
     steps = 40
     for t in range(steps):
         orc.step(t, 1)

@@ -328,7 +328,7 @@ def test_time_broad_phase_rewrites_the_same_slices(ctx, graph):
     net.close()
 
 
-@pytest.mark.parametrize("shape", [(2, 8, 256, 0), (4, 8, 128, 1)])
+@pytest.mark.parametrize("shape", [(2, 8, 256, 0), (4, 8, 128, 1), (4, 8, 512, 1)])
 def test_bench_workload_matches_oracle(ctx, shape):
     """The bench's own scene at full size (200k rods, 50k links, 136^3 grid, water), stepped by the
     production path (captured steps, side streams, a launch shape the tuner picks): pose, forces,
