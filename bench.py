@@ -203,6 +203,8 @@ def main() -> int:
                     help="bp_group,scan_items,rod_threads[,bp_staged]: fixed launch shape instead of the autotuner")
     ap.add_argument("--batch", type=int, default=0,
                     help="N > 0: the batched ensemble instead (N replicas, 2k-40k filaments log-uniform, 5-30 uM)")
+    ap.add_argument("--motor-group", type=int, default=0,
+                    help="motor workloads: lanes per motor in the reach search (16, 32, 64; 0: the tuner's or 16)")
     ap.add_argument("--turnover", type=int, default=1,
                     help="actomyosin workload: actin turnover at its biochem cadence (1) or none (0)")
     ap.add_argument("--cell-factors", type=str, default="1,1.5,2,3",
@@ -484,6 +486,8 @@ def run_actomyosin(args, log) -> int:
         tuned = net.tune(t0, args.tune_steps)
         t0 += tuned["steps"]
         log(f"tuned launch shape {tuned}")
+    if args.motor_group:
+        net.set_motor_group(args.motor_group)
     start = sc.copy()
     start.mot, start.dim, start.mini = sc.mot.copy(), sc.dim.copy(), sc.mini.copy()
     net.pull(start.fs, start.xl)
@@ -511,6 +515,7 @@ def run_actomyosin(args, log) -> int:
     net.pull(end.fs, end.xl)
     net.pull_motors(end.mot, end.mini)
     assert np.isfinite(end.fs.coord).all() and np.isfinite(end.mot.body.coord).all(), "non-finite pose"
+    mgroup = net.motor_group()
     net.close()
     roof = roof_rod = None
     if kernels:
@@ -547,7 +552,7 @@ def run_actomyosin(args, log) -> int:
                                + (f", formation every {args.formation} steps" if args.formation > 0 else ""),
                    "rods": n, "motor_heads": sc.mot.nM, "backbones": sc.mini.nBb, "links": sc.xl.nLinks,
                    "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
-                   "shape": shape_of(args, tuned) if (args.shape or tuned) else None},
+                   "shape": shape_of(args, tuned) if (args.shape or tuned) else None, "motor_group": mgroup},
         "roofline": roof, "roofline_gather_integrate": roof_rod, "cpu_baseline": cpu,
         "cpu_baseline_single_thread": cpu1,
         "links_alive_start": links_start, "links_alive_end": int((end.xl.linkState[: end.xl.nLinks] >= 0).sum()),
@@ -603,6 +608,8 @@ def run_standard(args, log) -> int:
         tuned = net.tune(t0, args.tune_steps)
         t0 += tuned["steps"]
         log(f"tuned launch shape {tuned}")
+    if args.motor_group:
+        net.set_motor_group(args.motor_group)
     start = sc.copy()
     start.mot, start.dim, start.mini = sc.mot.copy(), sc.dim.copy(), sc.mini.copy()
     net.pull(start.fs, start.xl)
@@ -618,6 +625,7 @@ def run_standard(args, log) -> int:
     net.pull_motors(end.mot, end.mini)
     assert np.isfinite(end.fs.coord).all() and np.isfinite(end.mot.body.coord).all(), "non-finite pose"
     assert np.isfinite(end.mini.node.coord).all(), "non-finite node pose"
+    mgroup = net.motor_group()
     net.close()
     roof = None
     if kernels:
@@ -653,7 +661,7 @@ def run_standard(args, log) -> int:
                                "viscosity 0.1 Pa s, Wang-hash RNG with the harness's seeds",
                    "segments": n, "myosins": sc.mot.nM, "nodes": sc.mini.nNodes, "link_slots": sc.xl.nLinks,
                    "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
-                   "shape": shape_of(args, tuned) if (args.shape or tuned) else None},
+                   "shape": shape_of(args, tuned) if (args.shape or tuned) else None, "motor_group": mgroup},
         "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu1,
         "links_formed_timed": formed, "links_alive_end": int((end.xl.linkState[: end.xl.nLinks] >= 0).sum()),
         "heads_bound_start": int((start.mot.boundSeg >= 0).sum()), "heads_bound_end": int((end.mot.boundSeg >= 0).sum()),

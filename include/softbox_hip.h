@@ -640,6 +640,11 @@ int sbx_net_set_shape(sbx_net *net, int bpGroup, int bpStaged, int scanItems, in
  * taken; shape[5] = {bpGroup, bpStaged, scanItems, rodThreads, overlap} and *msPerStep (may be NULL) report the
  * choice. */
 int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *shape, float *msPerStep);
+/* No counterpart in the reference: lanes per motor (16, 32 or 64) of the fused reach search + binding; the
+ * results do not depend on it.  sbx_net_tune picks it after the launch shape when motors are attached;
+ * sbx_net_motor_group returns the current value. */
+int sbx_net_set_motor_group(sbx_net *net, int lanes);
+int sbx_net_motor_group(const sbx_net *net);
 
 /* ------------------------------------------------------------------ batched step
  * No counterpart in the reference (its harnesses step one network): nRep independent networks of uneven

@@ -129,6 +129,7 @@ struct sbx_net {
     int bpGroup = 4;      // broad-phase lanes per body (2, 4 or 8)
     int bpStaged = 0;     // 1: each block's neighbourhood staged in LDS (4 or 8 lanes)
     int rodThreads = 256; // k_rod block size (64, 128 or 256), or RODS_PAIRED
+    int mbGroup = 16;     // lanes per motor in k_motor_bind (16, 32 or 64)
     // graph
     int useGraph = 0;
     int graphSteps = 0;  // non-zero while a capture is being recorded
@@ -565,12 +566,14 @@ __global__ void __launch_bounds__(256) k_pub_heads(sbx_motors m) {
 // cell-sorted contents, and the lanes take consecutive positions.  A hit's place in reachSeg is the
 // group's count so far plus the hits at earlier positions of the round (ballot), so the list is the
 // reference's in its order; bindNearest (:367-387) becomes an argmin over (distance, list place) of the
-// first maxCand hits, which is the serial scan's strict-< winner.
-constexpr int MB_GROUP = 16;
+// first maxCand hits, which is the serial scan's strict-< winner.  The group size G (16, 32 or 64 lanes,
+// sbx_net_set_motor_group) trades lanes for rounds: few motors over crowded stencils (heads clustered on
+// nodes) want wide groups, many motors over sparse stencils narrow ones; results do not depend on it.
 constexpr int MB_ROUND = 2;
 
+template <int G>
 __global__ void __launch_bounds__(256) k_motor_bind(MotorArgs a) {
-    constexpr int G = MB_GROUP, U = MB_ROUND;
+    constexpr int U = MB_ROUND;
     const int lane = threadIdx.x & 63, gl = lane % G, grp = lane / G;
     const int k = (blockIdx.x * 256 + threadIdx.x) / G;
     const sbx_motors &m = a.m;
@@ -588,7 +591,7 @@ __global__ void __launch_bounds__(256) k_motor_bind(MotorArgs a) {
     g.periodic = 0;  // the reach stencil is clamped at the faces, periodic or not
     int pre[18], shift[18];
     const int total = stencil_map(cx + cy * rg.nX + cz * rg.nX * rg.nY, g, a.off, pre, shift);
-    const unsigned long long groupBits = ((1ull << G) - 1) << (grp * G);
+    const unsigned long long groupBits = (G == 64 ? ~0ull : ((1ull << G) - 1)) << (grp * G);
     const unsigned long long below = (1ull << lane) - 1;
     int *slice = m.reachSeg + (size_t)k * a.maxCand;
     const float tol = a.kp.myoColTol, align = a.kp.alignTol;
@@ -600,7 +603,9 @@ __global__ void __launch_bounds__(256) k_motor_bind(MotorArgs a) {
         for (int u = 0; u < U; u++) {
             const int q = base + u * G + gl;
             const int j = q < total ? a.cont[stencil_at9(pre, shift, q)] : -1;
-            seg[u] = (j >= 0 && a.ownerStore[j] == 0) ? a.ownerSlot[j] : -1;
+            // the step's body view (k_owner_view): filament segments are bodies [0, nSeg) with ownerSlot = j,
+            // heads come after; so ownerStore[j] == 0 is j < nSeg and needs no load
+            seg[u] = (j >= 0 && j < nSeg) ? j : -1;
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -698,15 +703,15 @@ struct BackboneArgs {
     int stepOff;
 };
 
-// one backbone: zero -> backboneGather -> confine -> Brownian -> integrate -> derive (in place)
+// one backbone (or node): zero -> backboneGather -> confine -> Brownian -> integrate -> derive (in place)
 __global__ void __launch_bounds__(256) k_backbone(BackboneArgs a) {
     int b = blockIdx.x * 256 + threadIdx.x;
     const sbx_motors &m = a.m;
     int nBb = m.nBb;
     if (b >= nBb) return;
-    int step = *a.dStep + a.stepOff;
     double fd[3] = {0, 0, 0}, td[3] = {0, 0, 0};
     csr_payload_sum(b, a.bbOff, a.bbList, m.miniData, MINI_STRIDE, 0, fd, td);
+    int step = *a.dStep + a.stepOff;
     float f[3], t[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -1285,7 +1290,9 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
         a.rg = net->rg; a.kp = net->kp; a.np = net->np;
         a.maxCand = net->p.maxCand; a.nSeg = n; a.seed = net->mp.seedMotor; a.rng = net->p.rng;
         a.stream = net->p.stream; a.dStep = net->dStep; a.stepOff = stepOff;
-        hipLaunchKernelGGL(k_motor_bind, dim3(blocks_for(nM, 256 / MB_GROUP)), dim3(256), 0, s, a);
+        const int G = net->mbGroup;
+        auto kern = G == 64 ? k_motor_bind<64> : (G == 32 ? k_motor_bind<32> : k_motor_bind<16>);
+        hipLaunchKernelGGL(kern, dim3(blocks_for(nM, 256 / G)), dim3(256), 0, s, a);
     }
     {
         Scope sc(net, T_MJOINT);
@@ -2628,6 +2635,20 @@ int drop_graph(sbx_net *net) {
 
 extern "C" {
 
+int sbx_net_set_motor_group(sbx_net *net, int lanes) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, lanes == 16 || lanes == 32 || lanes == 64, "net_set_motor_group: 16, 32 or 64 lanes per motor");
+    int r = set_device(c);
+    if (!r) r = drop_graph(net);
+    if (r) return r;
+    net->mbGroup = lanes;
+    return SBX_OK;
+}
+
+int sbx_net_motor_group(const sbx_net *net) { return net ? net->mbGroup : SBX_ERR_ARG; }
+
 int sbx_net_set_shape(sbx_net *net, int bpGroup, int bpStaged, int scanItems, int rodThreads) {
     if (!net) return SBX_ERR_ARG;
     sbx_ctx *c = net->ctx;
@@ -2694,6 +2715,36 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
     if (r) return r;
     if ((r = sbx_net_set_shape(net, best[0], best[1], best[2], best[3]))) return r;
     net->overlap = best[4];
+    if (net->hasMotors) {  // then the motor stage's group size, at the chosen shape
+        const int gs[3] = {16, 32, 64};
+        int bestG = net->mbGroup;
+        float bestGm = -1.0f;
+        hipEvent_t g0 = nullptr, g1 = nullptr;
+        if ((r = check_hip(c, hipEventCreate(&g0), "tune event"))) return r;
+        if ((r = check_hip(c, hipEventCreate(&g1), "tune event"))) { hipEventDestroy(g0); return r; }
+        for (int gi = 0; gi < 3 && !r; gi++) {
+            if ((r = sbx_net_set_motor_group(net, gs[gi]))) break;
+            if ((r = sbx_net_step(net, t, 2))) break;
+            t += 2;
+            if ((r = join_side(net))) break;
+            if ((r = check_hip(c, hipEventRecord(g0, c->stream), "tune record"))) break;
+            if ((r = sbx_net_step(net, t, trialSteps))) break;
+            t += trialSteps;
+            if ((r = join_side(net))) break;
+            if ((r = check_hip(c, hipEventRecord(g1, c->stream), "tune record"))) break;
+            if ((r = check_hip(c, hipEventSynchronize(g1), "tune sync"))) break;
+            float ms = 0.0f;
+            if ((r = check_hip(c, hipEventElapsedTime(&ms, g0, g1), "tune elapsed"))) break;
+            ms /= (float)trialSteps;
+            if (bestGm < 0.0f || ms < bestGm) { bestGm = ms; bestG = gs[gi]; }
+        }
+        hipEventDestroy(g0);
+        hipEventDestroy(g1);
+        *advanced = t - step0;
+        if (r) return r;
+        if ((r = sbx_net_set_motor_group(net, bestG))) return r;
+        if (bestGm < bestMs) bestMs = bestGm;
+    }
     if (shape)
         for (int k = 0; k < 5; k++) shape[k] = best[k];
     if (msPerStep) *msPerStep = bestMs;

@@ -109,7 +109,17 @@ class Network:
         shape = (C.c_int * 5)()
         ms = C.c_float()
         self.ctx.check(self.ctx.lib.sbx_net_tune(self.h, step0, trial_steps, C.byref(adv), shape, C.byref(ms)))
-        return {"steps": adv.value, "shape": tuple(shape), "ms_per_step": ms.value}
+        out = {"steps": adv.value, "shape": tuple(shape), "ms_per_step": ms.value}
+        if self.mot is not None:
+            out["motor_group"] = self.motor_group()
+        return out
+
+    def set_motor_group(self, lanes: int) -> None:
+        """Lanes per motor of the fused reach search + binding (16, 32 or 64; results do not depend on it)."""
+        self.ctx.check(self.ctx.lib.sbx_net_set_motor_group(self.h, lanes))
+
+    def motor_group(self) -> int:
+        return self.ctx.check(self.ctx.lib.sbx_net_motor_group(self.h))
 
     def set_profiling(self, on: bool) -> None:
         self.ctx.check(self.ctx.lib.sbx_net_set_profiling(self.h, 1 if on else 0))

@@ -107,14 +107,16 @@ def _compare(net, orc, sc, what):
     assert_same(nd.nodeData, orc.mini.nodeData, f"{what}: nodeData")
 
 
-@pytest.mark.parametrize("rng,stiffen,graph", [(RNG_WANG, False, 0), (RNG_PHILOX, True, 1), (RNG_WANG, True, 2)])
-def test_node_shell_step_matches_oracle(ctx, rng, stiffen, graph):
+@pytest.mark.parametrize("rng,stiffen,graph,group", [(RNG_WANG, False, 0, 16), (RNG_PHILOX, True, 1, 64),
+                                                      (RNG_WANG, True, 2, 32)])
+def test_node_shell_step_matches_oracle(ctx, rng, stiffen, graph, group):
     from softbox_amd.network import Network
 
     sc = scenes.standard_1x(n_nodes=30, n_fil=150, box_xy=2.0, xlink_conc=50.0, rng=rng,
                             bound_coeff=0.25 if stiffen else None)
     orc = ffi.OracleContractile(sc)
     net = Network(ctx, sc)
+    net.set_motor_group(group)
     if graph:
         net.use_graph(graph)
     t = 0
@@ -122,7 +124,7 @@ def test_node_shell_step_matches_oracle(ctx, rng, stiffen, graph):
         orc.step(t, chunk)
         net.step(t, chunk)
         t += chunk
-        _compare(net, orc, sc, f"rng={rng} stiffen={stiffen} graph={graph} after {t}")
+        _compare(net, orc, sc, f"rng={rng} stiffen={stiffen} graph={graph} group={group} after {t}")
     assert int((orc.mot.boundSeg >= 0).sum()) > 0 and int((orc.xl.linkState >= 0).sum()) > 0
     assert np.abs(orc.mini.node.coord - sc.mini.node.coord).max() > 1e-4
     net.close()
