@@ -640,6 +640,18 @@ int sbx_net_set_shape(sbx_net *net, int bpGroup, int bpStaged, int scanItems, in
  * taken; shape[5] = {bpGroup, bpStaged, scanItems, rodThreads, overlap} and *msPerStep (may be NULL) report the
  * choice. */
 int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *shape, float *msPerStep);
+/* No counterpart in the reference (TornadoVM's task graphs have no stream capture): the rule the step's
+ * captures follow.  ops = nOps triples (op: 0 record, 1 wait; stream: 0 the capturing stream, 1 the
+ * broad-phase stream, 2 the sort stream; event id).  Returns -1 when every wait on an event recorded inside
+ * the same capture is a join, a fork, a same-stream wait or the broad-phase stream waiting on the sort
+ * stream; the sort stream waiting on the broad-phase stream's in-capture record (the form that ended the
+ * host process in stream capture) returns that triple's index, as does a malformed triple.  Host-only;
+ * needs no device. */
+int sbx_capture_check(const int *ops, int nOps);
+/* The (op, stream, event) triples of the net's last capture, as sbx_capture_check takes them: copies up to
+ * cap ints into out and returns the total number of ints.  A capture that breaks the rule is refused with
+ * SBX_ERR_STATE instead of being replayed. */
+int sbx_net_capture_log(const sbx_net *net, int *out, int cap);
 /* No counterpart in the reference: lanes per motor (16, 32 or 64) of the fused reach search + binding; the
  * results do not depend on it.  sbx_net_tune picks it after the launch shape when motors are attached;
  * sbx_net_motor_group returns the current value. */

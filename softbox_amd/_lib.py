@@ -240,6 +240,8 @@ _SIGS = {
     "sbx_net_set_overlap": [vp, i32],
     "sbx_net_set_shape": [vp, i32, i32, i32, i32],
     "sbx_net_set_motor_group": [vp, i32],
+    "sbx_capture_check": [vp, i32],
+    "sbx_net_capture_log": [vp, vp, i32],
     "sbx_net_motor_group": [vp],
     "sbx_net_tune": [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), f32p],
     "sbx_batch_create": [vp, i32, C.POINTER(SbxFilaments), C.POINTER(SbxCrosslinks), C.POINTER(SbxNetParams),

@@ -106,6 +106,11 @@ struct sbx_net {
     // step); per-step rebuild: the grid sort on a third stream and the grid CSR double-buffered by
     // pose buffer, so the next pose's sort runs while this pose's broad phase still reads its CSR
     hipStream_t side = nullptr, sortS = nullptr;
+    // every event record and stream wait enqueued while a step is being captured, as (op, stream, event)
+    // triples (stream 0 = the capturing stream, 1 = side, 2 = sort; events numbered by first use), checked
+    // against the capture rule before the graph is used (capture_check)
+    std::vector<int> capLog;
+    std::vector<hipEvent_t> capEv;
     hipEvent_t evFork = nullptr, evBuilt = nullptr;
     hipEvent_t evRod[2] = {nullptr, nullptr}, evPlaced[2] = {nullptr, nullptr}, evJoin[2] = {nullptr, nullptr};
     int needFork = 1;  // the side streams must be ordered after the main stream before their next work
@@ -1240,6 +1245,80 @@ struct Scope {
     }
 };
 
+// ---------------------------------------------------------------- the capture rule
+// The step's side streams form a pipeline: the capturing stream forks the sort stream, the sort stream
+// feeds the broad-phase stream, and the capturing stream joins both.  A capture in which the sort stream
+// waited on an event the broad-phase stream recorded inside the same capture (against the pipeline) has
+// ended the host process twice (no GPU fault; the cause is outside this library).  Every event record and
+// stream wait of the step goes through ev_record / ev_wait, which log it while a step is being captured;
+// capture_check accepts the capture only if every wait on an event recorded inside the same capture is a
+// join (by the capturing stream), a fork (of the capturing stream's record), a wait on the same stream, or
+// a wait downstream (the broad-phase stream on the sort stream).  Waits on records made before the capture
+// (the previous graph's) are unrestricted.  Stream ids: 0 capturing, 1 broad phase, 2 sort.
+enum { CAP_RECORD = 0, CAP_WAIT = 1 };
+
+int capture_check(const int *ops, int nOps) {
+    static const int rank[3] = {0, 2, 1};  // pipeline order: capturing, sort, broad phase
+    // recorder[e]: stream whose record of event e is the latest inside this capture, -1 before any
+    std::vector<int> recorder;
+    for (int k = 0; k < nOps; k++) {
+        const int op = ops[3 * k], st = ops[3 * k + 1], ev = ops[3 * k + 2];
+        if (ev < 0 || st < 0 || st > 2 || (op != CAP_RECORD && op != CAP_WAIT)) return k;
+        if ((int)recorder.size() <= ev) recorder.resize(ev + 1, -1);
+        if (op == CAP_RECORD) {
+            recorder[ev] = st;
+        } else {
+            const int rec = recorder[ev];
+            if (st != 0 && rec > 0 && rec != st && rank[rec] > rank[st]) return k;
+        }
+    }
+    return -1;
+}
+
+int stream_id(const sbx_net *net, hipStream_t s) {
+    if (s == net->ctx->stream) return 0;
+    if (s == net->side) return 1;
+    if (s == net->sortS) return 2;
+    return 3;  // not one of the step's streams: capture_check reports it
+}
+
+void cap_log(sbx_net *net, int op, hipStream_t s, hipEvent_t e) {
+    int id = -1;
+    for (size_t k = 0; k < net->capEv.size(); k++)
+        if (net->capEv[k] == e) id = (int)k;
+    if (id < 0) {
+        id = (int)net->capEv.size();
+        net->capEv.push_back(e);
+    }
+    net->capLog.push_back(op);
+    net->capLog.push_back(stream_id(net, s));
+    net->capLog.push_back(id);
+}
+
+int ev_record(sbx_net *net, hipEvent_t e, hipStream_t s, const char *what) {
+    if (net->graphSteps) cap_log(net, CAP_RECORD, s, e);
+    return check_hip(net->ctx, hipEventRecord(e, s), what);
+}
+
+int ev_wait(sbx_net *net, hipStream_t s, hipEvent_t e, const char *what) {
+    if (net->graphSteps) cap_log(net, CAP_WAIT, s, e);
+    return check_hip(net->ctx, hipStreamWaitEvent(s, e, 0), what);
+}
+
+// before a capture begins / after it ends
+void cap_begin(sbx_net *net) {
+    net->capLog.clear();
+    net->capEv.clear();
+}
+
+int cap_verdict(sbx_net *net) {
+    const int bad = capture_check(net->capLog.data(), (int)net->capLog.size() / 3);
+    if (bad < 0) return SBX_OK;
+    return fail(net->ctx, SBX_ERR_STATE, "capture: operation %d (stream %d %s event %d) breaks the capture rule", bad,
+                net->capLog[3 * bad + 1], net->capLog[3 * bad] == CAP_WAIT ? "waits on" : "records",
+                net->capLog[3 * bad + 2]);
+}
+
 int drain_timers(sbx_net *net) {
     if (net->pending.empty()) return SBX_OK;
     int r = check_hip(net->ctx, hipStreamSynchronize(net->ctx->stream), "profiling sync");
@@ -1378,7 +1457,7 @@ int enqueue_skin(sbx_net *net, const float *coord, const float *uVec, hipStream_
         hipLaunchKernelGGL(k_skin_build, dim3(blocks_for(n, BP_THREADS)), dim3(BP_THREADS), 0, s, a);
         if ((r = check_launch(c, "k_skin_build"))) return r;
         // k_rod's guard reads the skins and reference pose of this rebuild
-        if (ovl && (r = check_hip(c, hipEventRecord(net->evBuilt, s), "record evBuilt"))) return r;
+        if (ovl && (r = ev_record(net, net->evBuilt, s, "record evBuilt"))) return r;
     }
     {
         Scope sc(net, T_SKIN_PAIRS);
@@ -1403,7 +1482,7 @@ int settle_events(sbx_net *net) {
     for (hipEvent_t e : {net->evFork, net->evBuilt, net->evRod[0], net->evRod[1], net->evPlaced[0], net->evPlaced[1],
                          net->evJoin[0], net->evJoin[1]}) {
         if (!e) continue;
-        int r = check_hip(c, hipEventRecord(e, c->stream), "settle events");
+        int r = ev_record(net, e, c->stream, "settle events");
         if (r) return r;
     }
     net->evCaptured = 0;
@@ -1425,9 +1504,9 @@ int join_side(sbx_net *net) {
     const bool rebuild = net->skinMode == SBX_SKIN_REBUILD && net->p.gridOn && !net->hasMotors;
     for (int b = 0; b < 2 && !r; b++) {
         if (!net->graphSteps || net->joinLive[b])
-            r = check_hip(c, hipStreamWaitEvent(c->stream, net->evJoin[b], 0), "join");
+            r = ev_wait(net, c->stream, net->evJoin[b], "join");
         if (!r && rebuild && (!net->graphSteps || net->placedLive[b]))
-            r = check_hip(c, hipStreamWaitEvent(c->stream, net->evPlaced[b], 0), "join");
+            r = ev_wait(net, c->stream, net->evPlaced[b], "join");
     }
     net->needFork = 1;
     return r;
@@ -1438,13 +1517,13 @@ int enqueue_bp(sbx_net *net, int b, hipStream_t s2, bool ovl, bool waitPlaced) {
     sbx_ctx *c = net->ctx;
     int r;
     Scope sc(net, T_PAIRS);
-    if (ovl && waitPlaced && (r = check_hip(c, hipStreamWaitEvent(s2, net->evPlaced[b], 0), "wait evPlaced")))
+    if (ovl && waitPlaced && (r = ev_wait(net, s2, net->evPlaced[b], "wait evPlaced")))
         return r;
     r = launch_broad_phase_sorted(c, net->sortedPose[b], net->sortedCell[b], net->csrOff[b], net->csrCont[b],
                                   net->geom, net->reps, net->rodRep, net->bpGroup, net->bpStaged, net->n, net->n,
                                   net->candPartner, net->candCount, net->p.maxCand, s2);
     if (r) return r;
-    if (ovl && (r = check_hip(c, hipEventRecord(net->evJoin[b], s2), "record evJoin"))) return r;
+    if (ovl && (r = ev_record(net, net->evJoin[b], s2, "record evJoin"))) return r;
     if (net->graphSteps) net->joinLive[b] = 1;
     net->lastCsr = b;
     return SBX_OK;
@@ -1484,16 +1563,16 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
     } else if (net->p.gridOn) {
         if (!net->derivedValid) return fail(c, SBX_ERR_STATE, "net step: derived pose data not prepared");
         if (ovl && (net->needFork || capK == 0)) {
-            if ((r = check_hip(c, hipEventRecord(net->evFork, s), "record evFork"))) return r;
-            if ((r = check_hip(c, hipStreamWaitEvent(s2, net->evFork, 0), "wait evFork"))) return r;
-            if ((r = check_hip(c, hipStreamWaitEvent(sS, net->evFork, 0), "wait evFork"))) return r;
+            if ((r = ev_record(net, net->evFork, s, "record evFork"))) return r;
+            if ((r = ev_wait(net, s2, net->evFork, "wait evFork"))) return r;
+            if ((r = ev_wait(net, sS, net->evFork, "wait evFork"))) return r;
             net->needFork = 0;
         }
         if (skinOn) {
-            if (ovl && prev1 && (r = check_hip(c, hipStreamWaitEvent(s2, net->evRod[xb], 0), "wait evRod")))
+            if (ovl && prev1 && (r = ev_wait(net, s2, net->evRod[xb], "wait evRod")))
                 return r;
             if ((r = enqueue_skin(net, coord, uVec, s2, ovl))) return r;
-            if (ovl && (r = check_hip(c, hipEventRecord(net->evJoin[cb], s2), "record evJoin"))) return r;
+            if (ovl && (r = ev_record(net, net->evJoin[cb], s2, "record evJoin"))) return r;
             if (net->graphSteps) net->joinLive[cb] = 1;
         } else {
             if (net->bpPending) {  // the broad phase the previous step left; its sort is ordered before
@@ -1502,9 +1581,9 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
             }
             {
                 Scope sc(net, T_SORT);
-                if (ovl && prev1 && (r = check_hip(c, hipStreamWaitEvent(sS, net->evRod[xb], 0), "wait evRod")))
+                if (ovl && prev1 && (r = ev_wait(net, sS, net->evRod[xb], "wait evRod")))
                     return r;
-                if (ovl && prev2 && (r = check_hip(c, hipStreamWaitEvent(sS, net->evJoin[cb], 0), "wait evJoin")))
+                if (ovl && prev2 && (r = ev_wait(net, sS, net->evJoin[cb], "wait evJoin")))
                     return r;
                 SortPayload pay;
                 pay.in = net->bodyPack[cb];
@@ -1513,7 +1592,7 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
                 r = counting_sort_counted(c, net->sort, net->bodyCell, n, net->p.gridDims[3], n, net->csrOff[cb],
                                           net->csrCont[cb], net->gridCounts[cb], net->gridSlot[cb], sS, pay);
                 if (r) return r;
-                if (ovl && (r = check_hip(c, hipEventRecord(net->evPlaced[cb], sS), "record evPlaced"))) return r;
+                if (ovl && (r = ev_record(net, net->evPlaced[cb], sS, "record evPlaced"))) return r;
                 if (net->graphSteps) net->placedLive[cb] = 1;
                 else net->csrValid[cb] = 1;
             }
@@ -1523,7 +1602,7 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
             } else if ((r = enqueue_bp(net, cb, s2, ovl, true))) {
                 return r;
             }
-            if (ovl && prev1 && (r = check_hip(c, hipStreamWaitEvent(s, net->evPlaced[xb], 0), "wait evPlaced")))
+            if (ovl && prev1 && (r = ev_wait(net, s, net->evPlaced[xb], "wait evPlaced")))
                 return r;
         }
     }
@@ -1589,7 +1668,7 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
     a.halfY = 0.5 * net->p.boxParams[2] - net->p.boxParams[4];
     a.halfZ = 0.5 * net->p.boxParams[3] - net->p.boxParams[4];
     a.coeff = net->p.boxParams[5];
-    if (ovl && skinOn && (r = check_hip(c, hipStreamWaitEvent(s, net->evBuilt, 0), "wait evBuilt"))) return r;
+    if (ovl && skinOn && (r = ev_wait(net, s, net->evBuilt, "wait evBuilt"))) return r;
     {
         Scope sc(net, T_ROD);
         const long long waves = ((long long)n + CHAIN_SPAN - 1) / CHAIN_SPAN;
@@ -1602,7 +1681,7 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
                                a, net->cp);
         }
     }
-    if (ovl && (r = check_hip(c, hipEventRecord(net->evRod[cb], s), "record evRod"))) return r;
+    if (ovl && (r = ev_record(net, net->evRod[cb], s, "record evRod"))) return r;
     net->cur = nxt;
     net->lastCell = net->bodyCell;
     if (a.nextCell) {  // the next step's grid reads the cells k_rod just wrote
@@ -2104,6 +2183,7 @@ int step_range(sbx_net *net, int step0, int nSteps) {
                 int *savedCell = net->bodyCell, *savedNext = net->cellNext, *savedLast = net->lastCell;
                 const int savedLastCsr = net->lastCsr, savedPending = net->bpPending;
                 if ((r = join_side(net))) return r;
+                cap_begin(net);
                 if ((r = check_hip(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
                                    "begin capture")))
                     return r;
@@ -2118,6 +2198,7 @@ int step_range(sbx_net *net, int step0, int nSteps) {
                 hipError_t ee = hipStreamEndCapture(c->stream, &g);
                 net->graphSteps = 0;
                 net->evCaptured = 1;
+                if (!er) er = cap_verdict(net);
                 for (int b = 0; b < 2; b++) net->joinLive[b] = net->placedLive[b] = 0;
                 net->unitLastCell[p] = net->lastCell;
                 net->cur = p;
@@ -2166,6 +2247,7 @@ int step_range(sbx_net *net, int step0, int nSteps) {
             int savedCur = net->cur;
             // side-stream work of earlier eager steps is joined before the graph (which joins its own)
             if ((r = join_side(net))) return r;
+            cap_begin(net);
             if ((r = check_hip(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
                 return r;
             net->graphSteps = G;
@@ -2179,6 +2261,7 @@ int step_range(sbx_net *net, int step0, int nSteps) {
             hipError_t ee = hipStreamEndCapture(c->stream, &g);
             net->graphSteps = 0;
             net->evCaptured = 1;
+            if (!er) er = cap_verdict(net);
             net->cur = savedCur;
             net->graphCur = savedCur;
             if (er) {
@@ -2221,9 +2304,9 @@ int step_range(sbx_net *net, int step0, int nSteps) {
         // (which sorted this CSR) through a fork from the main stream, not through evPlaced
         if ((r = settle_events(net))) return r;
         if (ovl && net->needFork) {
-            if ((r = check_hip(c, hipEventRecord(net->evFork, c->stream), "record evFork"))) return r;
-            if ((r = check_hip(c, hipStreamWaitEvent(net->side, net->evFork, 0), "wait evFork"))) return r;
-            if ((r = check_hip(c, hipStreamWaitEvent(net->sortS, net->evFork, 0), "wait evFork"))) return r;
+            if ((r = ev_record(net, net->evFork, c->stream, "record evFork"))) return r;
+            if ((r = ev_wait(net, net->side, net->evFork, "wait evFork"))) return r;
+            if ((r = ev_wait(net, net->sortS, net->evFork, "wait evFork"))) return r;
             net->needFork = 0;
         }
         if ((r = enqueue_bp(net, net->bpPendingBuf, ovl ? net->side : c->stream, ovl, true))) return r;
@@ -2555,12 +2638,12 @@ int sbx_net_time_broad_phase(sbx_net *net, int reps, float *usPerLaunch) {
         hipEventDestroy(e0);
         return r;
     }
-    r = check_hip(c, hipEventRecord(e0, c->stream), "time bp record");
+    r = ev_record(net, e0, c->stream, "time bp record");
     for (int k = 0; k < reps && !r; k++)
         r = launch_broad_phase_sorted(c, net->sortedPose[b], net->sortedCell[b], net->csrOff[b], net->csrCont[b],
                                       net->geom, net->reps, net->rodRep, net->bpGroup, net->bpStaged, net->n, net->n,
                                       net->candPartner, net->candCount, net->p.maxCand, c->stream);
-    if (!r) r = check_hip(c, hipEventRecord(e1, c->stream), "time bp record");
+    if (!r) r = ev_record(net, e1, c->stream, "time bp record");
     if (!r) r = check_hip(c, hipEventSynchronize(e1), "time bp sync");
     float ms = 0.0f;
     if (!r) r = check_hip(c, hipEventElapsedTime(&ms, e0, e1), "time bp elapsed");
@@ -2635,6 +2718,18 @@ int drop_graph(sbx_net *net) {
 
 extern "C" {
 
+int sbx_capture_check(const int *ops, int nOps) {
+    if (!ops || nOps < 0) return nOps == 0 ? -1 : SBX_ERR_ARG;
+    return capture_check(ops, nOps);
+}
+
+int sbx_net_capture_log(const sbx_net *net, int *out, int cap) {
+    if (!net || cap < 0 || (cap > 0 && !out)) return SBX_ERR_ARG;
+    const int n = (int)net->capLog.size();
+    for (int k = 0; k < n && k < cap; k++) out[k] = net->capLog[k];
+    return n;
+}
+
 int sbx_net_set_motor_group(sbx_net *net, int lanes) {
     if (!net) return SBX_ERR_ARG;
     sbx_ctx *c = net->ctx;
@@ -2694,11 +2789,11 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
                 if ((r = sbx_net_step(net, t, 2))) break;  // warm (and capture, with graphs on)
                 t += 2;
                 if ((r = join_side(net))) break;
-                if ((r = check_hip(c, hipEventRecord(e0, c->stream), "tune record"))) break;
+                if ((r = ev_record(net, e0, c->stream, "tune record"))) break;
                 if ((r = sbx_net_step(net, t, trialSteps))) break;
                 t += trialSteps;
                 if ((r = join_side(net))) break;
-                if ((r = check_hip(c, hipEventRecord(e1, c->stream), "tune record"))) break;
+                if ((r = ev_record(net, e1, c->stream, "tune record"))) break;
                 if ((r = check_hip(c, hipEventSynchronize(e1), "tune sync"))) break;
                 float ms = 0.0f;
                 if ((r = check_hip(c, hipEventElapsedTime(&ms, e0, e1), "tune elapsed"))) break;
@@ -2727,11 +2822,11 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
             if ((r = sbx_net_step(net, t, 2))) break;
             t += 2;
             if ((r = join_side(net))) break;
-            if ((r = check_hip(c, hipEventRecord(g0, c->stream), "tune record"))) break;
+            if ((r = ev_record(net, g0, c->stream, "tune record"))) break;
             if ((r = sbx_net_step(net, t, trialSteps))) break;
             t += trialSteps;
             if ((r = join_side(net))) break;
-            if ((r = check_hip(c, hipEventRecord(g1, c->stream), "tune record"))) break;
+            if ((r = ev_record(net, g1, c->stream, "tune record"))) break;
             if ((r = check_hip(c, hipEventSynchronize(g1), "tune sync"))) break;
             float ms = 0.0f;
             if ((r = check_hip(c, hipEventElapsedTime(&ms, g0, g1), "tune elapsed"))) break;

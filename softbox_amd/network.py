@@ -114,6 +114,14 @@ class Network:
             out["motor_group"] = self.motor_group()
         return out
 
+    def capture_log(self) -> np.ndarray:
+        """The (op, stream, event) triples of the last captured graph (op 0 record, 1 wait; stream 0 the
+        capturing stream, 1 broad phase, 2 sort), as sbx_capture_check takes them."""
+        n = self.ctx.check(self.ctx.lib.sbx_net_capture_log(self.h, None, 0))
+        out = np.zeros(max(1, n), np.int32)
+        self.ctx.check(self.ctx.lib.sbx_net_capture_log(self.h, out.ctypes.data_as(C.c_void_p), n))
+        return out[:n].reshape(-1, 3)
+
     def set_motor_group(self, lanes: int) -> None:
         """Lanes per motor of the fused reach search + binding (16, 32 or 64; results do not depend on it)."""
         self.ctx.check(self.ctx.lib.sbx_net_set_motor_group(self.h, lanes))

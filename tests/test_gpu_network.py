@@ -346,3 +346,38 @@ def test_bench_workload_matches_oracle(ctx, shape):
     net.step(0, steps)
     _compare(net, orc, sc, f"bench workload, shape {shape}, after {steps} steps")
     net.close()
+
+
+@pytest.mark.parametrize("skin", [False, True])
+def test_captures_follow_the_capture_rule(ctx, skin):
+    """The records and waits of real captures (the overlapped per-step rebuild's 1-step graphs, and the
+    skin broad phase's 2-step graphs) pass sbx_capture_check and fork and join the side streams; the steps
+    stay the oracle's."""
+    import ctypes as C
+
+    from softbox_amd import _lib
+    from softbox_amd.network import Network
+    from softbox_amd.skin import SkinConfig
+
+    sc = small_network(seed=12, n_fil=150, links=400)
+    orc = ffi.OracleNet(sc)
+    net = Network(ctx, sc)
+    net.set_overlap(True)
+    if skin:
+        net.set_skin(_lib.SBX_SKIN_FIXED, SkinConfig(skin_default=0.1, list_cap=64))
+    net.use_graph(True)
+    net.step(0, 9)
+    orc.step(0, 9)
+    log = net.capture_log()
+    assert len(log) > 0
+    ops = np.ascontiguousarray(log.ravel())
+    assert ctx.lib.sbx_capture_check(ops.ctypes.data_as(C.c_void_p), len(log)) == -1
+    waits = {(int(s), int(e)) for o, s, e in log if o == 1}
+    recs = {}
+    for o, s, e in log:
+        if o == 0:
+            recs[int(e)] = int(s)
+    assert any(s == 0 and recs.get(e, 0) > 0 for s, e in waits), "no join of a side stream"
+    assert any(s > 0 and recs.get(e, -1) == 0 for s, e in waits), "no fork of a side stream"
+    _compare(net, orc, sc, f"captures (skin={skin}) after 9 steps")
+    net.close()
