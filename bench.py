@@ -174,8 +174,15 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--skin", choices=("compare", "rebuild", "fixed", "learned"), default="compare",
-                    help="compare: per-step rebuild and learned skin on identical trajectories (value = learned)")
+    ap.add_argument("--skin", choices=("compare", "rebuild", "fixed", "learned", "ablation"), default="compare",
+                    help="compare: per-step rebuild and learned skin on identical trajectories (network: the filament "
+                         "skin; actomyosin: the reach skin of the binding grid); ablation (actomyosin): rebuild, "
+                         "fixed and learned reach skins")
+    ap.add_argument("--reach-skin", type=float, default=0.04,
+                    help="actomyosin: the fixed reach skin, um (and the learned mode's first build)")
+    ap.add_argument("--reach-horizon", type=int, default=0,
+                    help="actomyosin: steps between scheduled reach-skin rebuilds (0: fixed mode none, learned mode "
+                         "its training horizon, 50)")
     ap.add_argument("--graph", type=int, default=1, help="1: replay the step as a captured hipGraph")
     ap.add_argument("--overlap", type=int, default=1,
                     help="1: broad phase on a second stream, concurrent with the mechanics kernels")
@@ -447,18 +454,39 @@ def actomyosin_turnover(sc, scenes):
     return t
 
 
+def reach_digest(net, sc) -> dict:
+    """Arrays that must be identical between the per-step rebuild and the reach skin: the reach lists (count
+    and the first maxCand entries), the binding state and the filament and motor poses."""
+    mot, mini = sc.mot.copy(), sc.mini.copy()
+    net.pull_motors(mot, mini)
+    fs = sc.fs.copy()
+    net.pull(fs, sc.xl.copy())
+    mc = int(sc.p["maxCand"])
+    rc = net.ctx.to_host(net.mot.t["reachCount"]).copy()
+    rs = net.ctx.to_host(net.mot.t["reachSeg"]).reshape(-1, mc)
+    lists = np.concatenate([rs[m, : min(int(rc[m]), mc)] for m in np.flatnonzero(rc)] or [np.zeros(0, np.int32)])
+    return {"reachCount": rc, "reachSeg": lists, "boundSeg": mot.boundSeg.copy(), "bindArc": mot.bindArc.copy(),
+            "stats": mot.stats.copy(), "coord": fs.coord.copy(), "motorCoord": mot.body.coord.copy()}
+
+
 def run_actomyosin(args, log) -> int:
     """The full actomyosin network: 50k filaments x 10 rods (0.1 um) in the reference's clamped 30 um box
     with containment walls, 2k minifilaments (4 dimers each end -> 16 heads, stall 4 pN), 100k
     crosslinkers; motor reach grid + binding + catch-slip + cycle + bond forces + joints + dimer
     coupling + tethers + backbones on top of the network step (DenseContractileHarness.cpuStep order).
     Viscosity 1 Pa s, the dense fixture's crowded cytoplasm (DenseContractileHarness.java:58): at water
-    viscosity the placed heads and links fling segments micrometres per step."""
+    viscosity the placed heads and links fling segments micrometres per step.
+
+    --skin: the binding grid's search per step (rebuild: the reference's grid sorted every step), from the
+    reach skin (fixed, learned), or both on the same trajectory (compare: rebuild and learned; ablation:
+    rebuild, fixed and learned).  Every run starts from the same scene and takes the same steps; the runs'
+    reach lists, binding state and poses at the end must be identical."""
     import torch  # noqa: F401  (device runtime first, as in the network workload)
 
-    from softbox_amd import scenes
+    from softbox_amd import _lib, scenes
     from softbox_amd.context import Context
     from softbox_amd.network import Network
+    from softbox_amd.skin import ReachSkinConfig, fit_reach
 
     n_fil, n_links, box = args.n_fil, args.n_links, args.box
     t_build = time.perf_counter()
@@ -473,50 +501,104 @@ def run_actomyosin(args, log) -> int:
     log(f"actomyosin scene: {n} filament rods (+{sc.fs.n - n_rods} free slots), {sc.mot.nM} heads, {sc.mini.nBb} "
         f"backbones, {sc.xl.nLinks} links ({time.perf_counter() - t_build:.1f} s)")
     ctx = Context(0)
-    net = Network(ctx, sc.copy(), keep_forces=False)
-    net.set_overlap(bool(args.overlap))
-    if args.graph:
-        net.use_graph(True)
-    net.step(0, args.warmup)
-    t0 = args.warmup
-    tuned = None
-    if args.shape:
-        net.set_shape(*[int(v) for v in args.shape.split(",")])
-    elif args.tune_steps:
-        tuned = net.tune(t0, args.tune_steps)
-        t0 += tuned["steps"]
-        log(f"tuned launch shape {tuned}")
-    if args.motor_group:
-        net.set_motor_group(args.motor_group)
-    start = sc.copy()
-    start.mot, start.dim, start.mini = sc.mot.copy(), sc.dim.copy(), sc.mini.copy()
-    net.pull(start.fs, start.xl)
-    net.pull_motors(start.mot, start.mini)
-    if start.turn is not None:
-        net.pull_turnover(start.turn, start.fs)
-    links_start = int((start.xl.linkState[: start.xl.nLinks] >= 0).sum())
-    bound_start = int((start.mot.boundSeg >= 0).sum())
-    formed0 = net.form_stats().get("formed_total", 0) if args.formation > 0 else 0
-    el = timed(net, ctx, t0, args.steps)
-    r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "timed_steps": [t0, t0 + args.steps - 1]}
-    if args.formation > 0:
-        r["links_formed_timed"] = net.form_stats()["formed_total"] - formed0
-    if start.turn is not None:
-        tend = net.pull_turnover(start.turn.copy())
-        r["turnover"] = {"cadences_timed": int(tend.stats[6] - start.turn.stats[6]),
-                         "monomers_grown": int(tend.ledger[0] - start.turn.ledger[0]),
-                         "monomers_returned": int(tend.ledger[1] - start.turn.ledger[1]),
-                         "segments_active_start": int((start.turn.filState >= 0).sum()),
-                         "segments_active_end": int((tend.filState >= 0).sum()),
-                         "free_actin_uM_end": round(float(tend.pool[0]), 4)}
-    kernels = profile(net, t0 + args.steps, args.profile_steps) if args.profile_steps else {}
-    end = sc.copy()
-    end.mot, end.mini = sc.mot.copy(), sc.mini.copy()
-    net.pull(end.fs, end.xl)
-    net.pull_motors(end.mot, end.mini)
-    assert np.isfinite(end.fs.coord).all() and np.isfinite(end.mot.body.coord).all(), "non-finite pose"
-    mgroup = net.motor_group()
-    net.close()
+    modes = {"compare": ["rebuild", "learned"], "rebuild": ["rebuild"], "fixed": ["fixed"], "learned": ["learned"],
+             "ablation": ["rebuild", "fixed", "learned"]}[args.skin]
+    fcfg = ReachSkinConfig(skin_default=args.reach_skin, horizon=args.reach_horizon)
+    rcfg = ReachSkinConfig(skin_default=args.reach_skin, horizon=args.reach_horizon)
+    model, train_info = None, None
+    if "learned" in modes:
+        t_tr = time.perf_counter()
+        tnet = Network(ctx, sc.copy(), keep_forces=False)
+        model, hist, (x, _), _ = fit_reach(ctx, tnet, rcfg, snapshots=args.train_snapshots, iters=args.train_iters)
+        tnet.close()
+        train_info = {"samples": int(x.shape[0]), "loss_start": round(hist[0], 4), "loss_end": round(hist[-1], 4),
+                      "margin": round(rcfg.margin, 3), "horizon": rcfg.horizon,
+                      "seconds": round(time.perf_counter() - t_tr, 1)}
+        log(f"reach skin model trained: {train_info}")
+    runs, digests = {}, {}
+    tuned, t0_all, extra, shape = None, None, 0, None
+    for mode in modes:
+        net = Network(ctx, sc.copy(), keep_forces=False)
+        net.set_overlap(bool(args.overlap))
+        if mode != "rebuild":
+            if mode == "learned":
+                net.set_reach_skin(_lib.SBX_SKIN_LEARNED, rcfg, model)
+            else:
+                net.set_reach_skin(_lib.SBX_SKIN_FIXED, fcfg)
+        if args.graph:
+            net.use_graph(True)
+        net.step(0, args.warmup)
+        t0 = args.warmup
+        if args.shape:
+            net.set_shape(*[int(v) for v in args.shape.split(",")])
+        elif args.tune_steps and shape is None:
+            tuned = net.tune(t0, args.tune_steps)  # launch shape and lanes per head, over real steps of the run
+            t0 += tuned["steps"]
+            extra = tuned["steps"]
+            shape = tuned["shape"]
+            log(f"tuned launch shape {tuned} ({mode})")
+        else:
+            if shape is not None:
+                net.set_shape(shape[0], shape[2], shape[3], shape[1])
+            if extra:  # the first run's tuning steps: every run times the same step numbers
+                net.step(t0, extra)
+                t0 += extra
+        if args.motor_group:
+            net.set_motor_group(args.motor_group)
+        elif mode != "rebuild" and (tuned is None or modes[0] == "rebuild"):
+            # lanes per head for the reach skin's search: a few trial steps of each on a copy of this state
+            net.set_motor_group(reach_group(ctx, net, sc, t0))
+        if t0_all is None:
+            t0_all = t0
+            start = sc.copy()
+            start.mot, start.dim, start.mini = sc.mot.copy(), sc.dim.copy(), sc.mini.copy()
+            net.pull(start.fs, start.xl)
+            net.pull_motors(start.mot, start.mini)
+            if start.turn is not None:
+                net.pull_turnover(start.turn, start.fs)
+        links_start = int((start.xl.linkState[: start.xl.nLinks] >= 0).sum())
+        bound_start = int((start.mot.boundSeg >= 0).sum())
+        formed0 = net.form_stats().get("formed_total", 0) if args.formation > 0 else 0
+        turn0 = net.pull_turnover(start.turn.copy()) if start.turn is not None else None
+        el = timed(net, ctx, t0, args.steps)
+        r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps,
+             "timed_steps": [t0, t0 + args.steps - 1], "motor_group": net.motor_group()}
+        if args.formation > 0:
+            r["links_formed_timed"] = net.form_stats()["formed_total"] - formed0
+        if turn0 is not None:
+            tend = net.pull_turnover(start.turn.copy())
+            r["turnover"] = {"cadences_timed": int(tend.stats[6] - turn0.stats[6]),
+                             "monomers_grown": int(tend.ledger[0] - turn0.ledger[0]),
+                             "monomers_returned": int(tend.ledger[1] - turn0.ledger[1]),
+                             "segments_active_start": int((turn0.filState >= 0).sum()),
+                             "segments_active_end": int((tend.filState >= 0).sum()),
+                             "free_actin_uM_end": round(float(tend.pool[0]), 4)}
+        if mode != "rebuild":
+            r["reach_skin"] = net.reach_skin_stats()
+        digests[mode] = reach_digest(net, sc)
+        r["heads_bound_end"] = int((digests[mode]["boundSeg"] >= 0).sum())
+        r["reach_hits_end"] = int(digests[mode]["reachCount"].sum())
+        if args.profile_steps:
+            r["kernels"] = profile(net, t0 + args.steps, args.profile_steps)
+        end = sc.copy()
+        end.mot, end.mini = sc.mot.copy(), sc.mini.copy()
+        net.pull(end.fs, end.xl)
+        net.pull_motors(end.mot, end.mini)
+        assert np.isfinite(end.fs.coord).all() and np.isfinite(end.mot.body.coord).all(), "non-finite pose"
+        r["links_alive_start"] = links_start
+        r["links_alive_end"] = int((end.xl.linkState[: end.xl.nLinks] >= 0).sum())
+        r["heads_bound_start"] = bound_start
+        net.close()
+        runs[mode] = r
+        log(f"{mode}: {r['value']:.4g} rod-steps/s, {r['ms_per_step']:.4f} ms/step")
+    # every form took the same steps: the reach lists, binding events and poses must be identical
+    base = modes[0]
+    for mode in modes[1:]:
+        for k, v in digests[base].items():
+            assert np.array_equal(v, digests[mode][k]), f"{mode} run differs from {base} in {k}"
+    head_mode = "learned" if "learned" in runs else modes[0]
+    head = runs[head_mode]
+    kernels = head.get("kernels", {})
     roof = roof_rod = None
     if kernels:
         known = {k: kernel_bytes(sc, k) for k in kernels}
@@ -527,9 +609,10 @@ def run_actomyosin(args, log) -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": b,
                 "duration_us": round(kernels[dom]["avg_us"], 2),
                 "timing": f"HIP events around each launch of {args.profile_steps} eager steps"}
-        for k, v in kernels.items():
-            if known[k]:
-                v["GBs"] = round(known[k] / (v["avg_us"] * 1e-6) / 1e9, 1)
+        for rr in runs.values():
+            for k, v in rr.get("kernels", {}).items():
+                if known.get(k):
+                    v["GBs"] = round(known[k] / (v["avg_us"] * 1e-6) / 1e9, 1)
         if known.get("k_rod"):  # the gather/integrate kernel (chain, link and motor gathers + integrate)
             kr = known["k_rod"] / (kernels["k_rod"]["avg_us"] * 1e-6) / 1e9
             roof_rod = {"kernel": "k_rod", "achieved": round(kr, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -539,12 +622,14 @@ def run_actomyosin(args, log) -> int:
                                 "events around each launch, which add dispatch and completion"}
     cpu = cpu1 = None
     if not args.no_cpu:
-        cpu = run_cpu_baseline_actomyosin(start, t0, openmp=True)
-        cpu1 = run_cpu_baseline_actomyosin(start, t0, openmp=False)
+        cpu = run_cpu_baseline_actomyosin(start, t0_all, openmp=True)
+        cpu1 = run_cpu_baseline_actomyosin(start, t0_all, openmp=False)
+    bp = {"rebuild": "per-step binding grid (the reference's gridReachable)", "fixed": "reach skin, fixed",
+          "learned": "reach skin, learned"}[head_mode]
     out = {
-        "metric": "rod-steps/sec (filament rods) of the actomyosin step at 1 GPU", "value": round(r["value"], 1),
+        "metric": "rod-steps/sec (filament rods) of the actomyosin step at 1 GPU", "value": round(head["value"], 1),
         "unit": "rod-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(r["ms_per_step"], 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "ms_per_step": round(head["ms_per_step"], 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 state, f64 force laws", "data": "synthetic (seeded scene)",
         "config": {"workload": f"full actomyosin network: {n_fil // 1000}k filaments x 10 rods (0.1 um), "
                                f"{sc.mini.nBb} minifilaments x 16 heads (stall 4 pN), {n_links // 1000}k crosslinker "
@@ -552,25 +637,59 @@ def run_actomyosin(args, log) -> int:
                                + (f", formation every {args.formation} steps" if args.formation > 0 else ""),
                    "rods": n, "motor_heads": sc.mot.nM, "backbones": sc.mini.nBb, "links": sc.xl.nLinks,
                    "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
-                   "shape": shape_of(args, tuned) if (args.shape or tuned) else None, "motor_group": mgroup},
+                   "shape": shape_of(args, tuned) if (args.shape or tuned) else None,
+                   "motor_group": head["motor_group"], "binding_search": bp},
         "roofline": roof, "roofline_gather_integrate": roof_rod, "cpu_baseline": cpu,
         "cpu_baseline_single_thread": cpu1,
-        "links_alive_start": links_start, "links_alive_end": int((end.xl.linkState[: end.xl.nLinks] >= 0).sum()),
-        "heads_bound_start": bound_start, "heads_bound_end": int((end.mot.boundSeg >= 0).sum()),
-        "timed_steps": r["timed_steps"],
+        "links_alive_start": head["links_alive_start"], "links_alive_end": head["links_alive_end"],
+        "heads_bound_start": head["heads_bound_start"], "heads_bound_end": head["heads_bound_end"],
+        "timed_steps": head["timed_steps"],
     }
-    if "links_formed_timed" in r:
-        out["links_formed_timed"] = r["links_formed_timed"]
-    if "turnover" in r:
-        out["turnover"] = r["turnover"]
+    if "rebuild" in runs and len(runs) > 1:
+        for mode in runs:
+            if mode != "rebuild":
+                out[f"{mode}_vs_rebuild"] = round(runs[mode]["value"] / runs["rebuild"]["value"], 3)
+        out["reach_lists_and_binding_identical"] = True
+    if "links_formed_timed" in head:
+        out["links_formed_timed"] = head["links_formed_timed"]
+    if "turnover" in head:
+        out["turnover"] = head["turnover"]
         out["config"]["workload"] += (", turnover every 100 steps (barbed growth 10 subunits/s, pointed-end "
                                       "depolymerisation, aging, cofilin severing, splits into 10% free slots)")
-    if kernels:
-        out["kernels"] = {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()}
-                          for k, v in kernels.items()}
+    out["runs"] = {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for m, r in runs.items()}
+    if train_info:
+        out["reach_skin_training"] = train_info
     print(json.dumps(out))
     ctx.close()
     return 0
+
+
+def reach_group(ctx, net, sc, t0: int, trial: int = 10) -> int:
+    """Lanes per head for the reach skin's search: time `trial` steps of each group on this network from
+    step t0, then restore nothing -- the trial steps run on a throwaway copy of the network's state."""
+    from softbox_amd.network import Network
+
+    tmp = sc.copy()
+    tmp.mot, tmp.dim, tmp.mini = sc.mot.copy(), sc.dim.copy(), sc.mini.copy()
+    net.pull(tmp.fs, tmp.xl)
+    net.pull_motors(tmp.mot, tmp.mini)
+    if tmp.turn is not None:
+        tmp.turn = tmp.turn.copy()
+        net.pull_turnover(tmp.turn, tmp.fs)
+    trial_net = Network(ctx, tmp, keep_forces=False)
+    trial_net.set_reach_skin(*net.reach_mode())
+    trial_net.use_graph(True)
+    best, t = None, t0
+    for g in (4, 8, 16, 32, 64):
+        trial_net.set_motor_group(g)
+        trial_net.step(t, 2)
+        t += 2
+        el = timed(trial_net, ctx, t, trial)
+        t += trial
+        if best is None or el < best[0]:
+            best = (el, g)
+    trial_net.close()
+    return best[1]
 
 
 STANDARD_1X_PUBLISHED = {"v1-CPU": 30.4, "v1-GPU": 21.4, "v2-CPU": 15.3, "v2-GPU": 49.5}   # steps/s

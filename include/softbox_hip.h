@@ -531,6 +531,39 @@ int sbx_net_skin_stats(sbx_net *net, long long *out, int cap);
 int sbx_net_skin_state(sbx_net *net, float **refCoord, float **skin, int **coarseCell, int **coarseOffsets,
                        int **coarseContents, int *nCoarse, float *coarseGrid);
 
+/* ------------------------------------------------------------------ reach skin (the binding grid of the myosin stage)
+ * No counterpart in the reference: BindingDetectionSystem.gridReachable (BindingDetectionSystem.java:143-228)
+ * bins every filament segment and head into the binding grid and sorts it every step.  With a reach skin the
+ * segments sit in a coarse grid built at a rebuild (cell >= 0.5 L + myoColTol + skin, at least cellFloor);
+ * every step each head walks the 27 coarse cells around its current position and keeps the segments that
+ * are in its binding-grid stencil (the cell the per-step grid bins them in) and pass reachTestDistSq, in
+ * the per-step walk's order (stencil position, then index).  A segment that moved more than its skin since
+ * the rebuild is moved by the step's guard (in the filament kernel) onto a list every head also scans; if
+ * that list overflows extraCap, heads scan every segment.  So reachSeg/reachCount and every binding event
+ * are identical to the per-step rebuild's whatever the skins.  Rebuilds are scheduled by the host between
+ * steps (graph replays): on the first step, after turnover, sbx_net_publish or a topology change, every
+ * `horizon` steps (0: never), and once the latest finished step reports more than extraTrigger segments
+ * outside their skins.  While it is on, the binding grid CSR (cellOffsets/cellContents) is not built. */
+typedef struct sbx_reach_skin_params {
+    float skinDefault;   /* um: fixed mode, and the first build of learned mode */
+    float skinMin, skinMax;  /* learned skins are clamped to [skinMin, skinMax] */
+    float margin;        /* learned skin = margin * exp(model out0) */
+    float cellFloor;     /* smallest coarse cell, um (0: the binding grid's cell) */
+    int horizon;         /* steps between scheduled rebuilds (0: only the other triggers) */
+    int extraTrigger;    /* segments outside their skins (reported by the latest finished step) that
+                          * schedule a rebuild */
+    int extraCap;        /* capacity of the list of segments outside their skins */
+} sbx_reach_skin_params;
+/* mode SBX_SKIN_REBUILD turns it off; SBX_SKIN_FIXED: skins from skinDefault; SBX_SKIN_LEARNED: the coarse
+ * cell of each rebuild from the largest per-cell skin the model predicts over the previous build's cells
+ * (modelWeights: device array of sbx_mlp_nparam() floats, copied).  Needs motors attached. */
+int sbx_net_set_reach_skin(sbx_net *net, int mode, const sbx_reach_skin_params *p, const float *modelWeights);
+/* -> out[0..cap): rebuilds (device), segments outside their skins now, steps whose heads scanned every
+ * segment, heads placed by the selection path (more than 4 hits on one lane or more than maxCand hits),
+ * the coarse cell of the last rebuild in nm, rebuilds for: stale state, horizon, outside-skin count;
+ * steps in the mode; returns the number written (<= 9) */
+int sbx_net_reach_skin_stats(sbx_net *net, long long *out, int cap);
+
 /* ------------------------------------------------------------------ skin model (MLP 12 -> 64 -> 64 -> 2, GELU)
  * No counterpart in the reference.  Parameters: one device float array of sbx_mlp_nparam() (layout in
  * softbox_amd/csrc/sbx_mlp.h); x[b*12 + k], y[b*2 + o], target like y.  Loss = mean over samples of
@@ -652,8 +685,9 @@ int sbx_capture_check(const int *ops, int nOps);
  * cap ints into out and returns the total number of ints.  A capture that breaks the rule is refused with
  * SBX_ERR_STATE instead of being replayed. */
 int sbx_net_capture_log(const sbx_net *net, int *out, int cap);
-/* No counterpart in the reference: lanes per motor (16, 32 or 64) of the fused reach search + binding; the
- * results do not depend on it.  sbx_net_tune picks it after the launch shape when motors are attached;
+/* No counterpart in the reference: lanes per motor (4, 8, 16, 32 or 64) of the fused reach search + binding;
+ * the results do not depend on it.  The per-step grid walk takes at least 16 (4 and 8 apply to the reach
+ * skin's search).  sbx_net_tune picks it after the launch shape when motors are attached;
  * sbx_net_motor_group returns the current value. */
 int sbx_net_set_motor_group(sbx_net *net, int lanes);
 int sbx_net_motor_group(const sbx_net *net);

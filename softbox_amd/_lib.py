@@ -127,6 +127,11 @@ class SbxSkinParams(C.Structure):
                 ("listCap", i32), ("blockKeys", i32)]
 
 
+class SbxReachSkinParams(C.Structure):
+    _fields_ = [("skinDefault", C.c_float), ("skinMin", C.c_float), ("skinMax", C.c_float), ("margin", C.c_float),
+                ("cellFloor", C.c_float), ("horizon", i32), ("extraTrigger", i32), ("extraCap", i32)]
+
+
 class SbxTurnover(C.Structure):
     _fields_ = [("C", i32)] + [(name, vp) for name in (
         "filState", "monomerCount", "seedNode", "nucFrac", "cofFrac", "bTransDiff", "bRotDiff", "pool", "ledger",
@@ -253,6 +258,8 @@ _SIGS = {
     "sbx_batch_layout": [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)],
     "sbx_net_set_skin": [vp, i32, C.POINTER(SbxSkinParams), vp],
     "sbx_net_skin_stats": [vp, C.POINTER(C.c_longlong), i32],
+    "sbx_net_set_reach_skin": [vp, i32, C.POINTER(SbxReachSkinParams), vp],
+    "sbx_net_reach_skin_stats": [vp, C.POINTER(C.c_longlong), i32],
     "sbx_net_skin_state": [vp] + [C.POINTER(vp)] * 5 + [C.POINTER(i32), f32p],
     "sbx_skin_features": [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, C.c_float, C.c_float, i32, f32p, vp],
     "sbx_skin_track": [vp, vp, vp, i32, i32, f32p, vp, vp, vp, i32],

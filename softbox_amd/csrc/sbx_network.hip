@@ -14,6 +14,7 @@
 // The crosslink CSR (keyed by linkFilA / linkFilB) depends only on the link topology, so it is built
 // at creation and on sbx_net_topology_changed instead of every step; linkState (which unbinding
 // changes every step) is applied by the gather guard, as in CrosslinkerSystem.segGatherA/B.
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -166,19 +167,42 @@ struct sbx_net {
     sbx_turnover turn{};
     sbx_turnover_params tp{};
     TurnScratch turnScratch;
+    // reach skin (sbx_net_set_reach_skin): gridReachable's candidates from a coarse grid of the segments
+    int rsMode = 0;                          // SBX_SKIN_REBUILD (off), _FIXED or _LEARNED
+    sbx_reach_skin_params rsp{};
+    float rsR0 = 0.0f;                       // 0.5 L + myoColTol of the longest segment at enable (um)
+    float rsFloor = 0.0f;                    // smallest coarse cell (um)
+    long long rsCellsMax = 0;                // coarse cells at the smallest cell (allocation)
+    float *rsRef = nullptr, *rsRefU = nullptr;  // segment centres and axes at the last rebuild
+    float *rsSkin = nullptr;                 // per-segment skin (< 0: on the extra list)
+    int *rsCell = nullptr, *rsOff = nullptr, *rsCont = nullptr;  // coarse cells and CSR of the last rebuild
+    int *rsExtra = nullptr;                  // segments outside their skins (rsp.extraCap)
+    int *rsWords = nullptr;                  // device words (RS_*)
+    float *rsModel = nullptr;                // MLP_NPARAM device floats (learned mode)
+    volatile int *rsHost = nullptr;          // pinned: {generation, outside-skin count} of the latest finished step
+    int *rsHostDev = nullptr;                // its device address
+    int rsValid = 0;                         // the coarse grid describes the current segments
+    int rsGen = 0, rsSince = 0;              // rebuilds enqueued so far; steps since the last one
+    long long rsWhy[3] = {0, 0, 0};          // rebuilds for: stale state, horizon, outside-skin count
+    long long rsSteps = 0;                   // steps in the mode
+    // the host's view of the outside-skin count is at most RS_INFLIGHT replays (or eager steps) old: an event
+    // after each, and the host waits for the oldest before enqueueing past it
+    static constexpr int RS_INFLIGHT = 2;
+    hipEvent_t rsEv[RS_INFLIGHT] = {nullptr, nullptr};
+    int rsEvNext = 0, rsEvLive = 0;
 };
 
 namespace {
 
 enum {
     T_BIN = 0, T_SORT, T_PAIRS, T_UNBIND, T_COUNT, T_LINK, T_ROD, T_MBIND, T_MJOINT, T_DIMER, T_TETHER, T_BACKBONE,
-    T_BOND, T_MINTEG, T_MCSR, T_SKIN_REBUILD, T_SKIN_PAIRS, T_COUNT_TIMERS
+    T_BOND, T_MINTEG, T_MCSR, T_SKIN_REBUILD, T_SKIN_PAIRS, T_RS_REBUILD, T_COUNT_TIMERS
 };
 const char *kTimerNames[T_COUNT_TIMERS] = {"k_body_cell", "grid_sort",     "k_broad_phase", "k_net_unbind",
                                            "k_net_count", "k_net_link",    "k_rod",         "k_motor_bind",
                                            "k_motor_joints", "k_net_dimer", "k_net_tether", "k_backbone",
                                            "k_bond_head", "k_motor_integ", "motor_csr", "skin_rebuild",
-                                           "k_skin_pairs"};
+                                           "k_skin_pairs", "reach_rebuild"};
 
 // k_rod launch form selected by rodThreads == 512: 256-thread blocks, two rod groups per block, two waves
 // drawing the noise of the other two (k_rod<*, true>)
@@ -199,8 +223,12 @@ struct RodArgs {
     int *nextCell;          // bodyCell of the new pose for the next step's grid (null: not fused)
     int *nextCounts, *nextSlot;  // the new pose's grid histogram (null: not taken here)
     float4 *linkPack;       // packed endpoint data of the new pose for the next step's links (or null)
-    const float *skinRef, *skin;  // skin guard: centre at the last rebuild and per-body skin (or null)
-    int *rebuildFlag;             // skin rebuild request word of the new pose's buffer
+    const float *skinRef;         // skin guard: centre at the last rebuild (or null)
+    float *skin;                  // per-body skin
+    int *rebuildFlag;             // skin rebuild request word of the new pose's buffer; reach skin: the
+                                  // outside-skin count
+    int *rsExtra;                 // reach skin: the outside-skin list (null: the filament skin's guard)
+    int rsCap;                    // its capacity
     float4 *bodyPack;             // {new centre, bounding radius} for the next step's broad phase (or null)
     const float *boundR;
     GridGeom geom;
@@ -382,13 +410,22 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
     }
     if (a.skinRef) {  // exact displacement guard: any body beyond its skin forces the next rebuild
         float dx = p.c[0] - a.skinRef[i], dy = p.c[1] - a.skinRef[n + i], dz = p.c[2] - a.skinRef[2 * n + i];
-        if (a.geom.periodic) {
-            dx = min_image(dx, a.geom.Lx, a.geom.iLx);
-            dy = min_image(dy, a.geom.Ly, a.geom.iLy);
-            dz = min_image(dz, a.geom.Lz, a.geom.iLz);
+        if (a.rsExtra) {  // reach skin: raw coordinates (the reach test and the grids use them unwrapped)
+            const float sk = a.skin[i];
+            if (sk >= 0.0f && dx * dx + dy * dy + dz * dz > sk * sk) {  // first time out: onto the list
+                a.skin[i] = -1.0f;
+                const int e = atomicAdd(a.rebuildFlag, 1);
+                if (e < a.rsCap) a.rsExtra[e] = i;
+            }
+        } else {
+            if (a.geom.periodic) {
+                dx = min_image(dx, a.geom.Lx, a.geom.iLx);
+                dy = min_image(dy, a.geom.Ly, a.geom.iLy);
+                dz = min_image(dz, a.geom.Lz, a.geom.iLz);
+            }
+            float sk = a.skin[i];
+            if (dx * dx + dy * dy + dz * dz > sk * sk) *a.rebuildFlag = 1;
         }
-        float sk = a.skin[i];
-        if (dx * dx + dy * dy + dz * dz > sk * sk) *a.rebuildFlag = 1;
     }
     if (a.bodyPack) a.bodyPack[i] = make_float4(p.c[0], p.c[1], p.c[2], a.boundR[i]);
     if (a.linkPack) {
@@ -576,6 +613,29 @@ __global__ void __launch_bounds__(256) k_pub_heads(sbx_motors m) {
 // nodes) want wide groups, many motors over sparse stencils narrow ones; results do not depend on it.
 constexpr int MB_ROUND = 2;
 
+// The rest of the fused motor pass, on one lane per motor: reachCount, catchSlipRelease, bindNearest's
+// winner (bestSeg, the first of the minimal distances in list order; -1: no hit), the nucleotide cycle.
+__device__ __forceinline__ void motor_kinetics(const MotorArgs &a, int k, int cnt, int bestSeg, float mx, float my,
+                                               float mz) {
+    const sbx_motors &m = a.m;
+    const int nSeg = a.nSeg;
+    const int step = *a.dStep + a.stepOff;
+    m.reachCount[k] = cnt;
+    int bs = m.boundSeg[k], cool = m.cooldown[k];
+    catch_slip_one(k, bs, cool, m.stats, m.capStats, m.forceDotFil[k], m.forceMag[k], a.kp, step, a.seed, a.rng,
+                   a.stream);
+    if (bs == FREE_BINDABLE && bestSeg >= 0) {
+        const int sg = bestSeg;
+        m.bindArc[k] = bind_arc(mx, my, mz, a.e1[sg], a.e1[nSeg + sg], a.e1[2 * nSeg + sg], a.e2[sg],
+                                a.e2[nSeg + sg], a.e2[2 * nSeg + sg]);
+        bs = sg;
+    }
+    m.boundSeg[k] = bs;
+    m.cooldown[k] = cool;
+    m.nucleotideState[k] = nucleotide_one(k, m.nucleotideState[k], bs >= 0, m.forceDotHist, a.np, step, a.seed,
+                                          a.rng, a.stream);
+}
+
 template <int G>
 __global__ void __launch_bounds__(256) k_motor_bind(MotorArgs a) {
     constexpr int U = MB_ROUND;
@@ -638,21 +698,307 @@ __global__ void __launch_bounds__(256) k_motor_bind(MotorArgs a) {
         if (od < bestD || (od == bestD && op < bestPos)) { bestD = od; bestPos = op; bestSeg = os; }
     }
     if (gl != 0) return;
-    const int step = *a.dStep + a.stepOff;
-    m.reachCount[k] = cnt;
-    int bs = m.boundSeg[k], cool = m.cooldown[k];
-    catch_slip_one(k, bs, cool, m.stats, m.capStats, m.forceDotFil[k], m.forceMag[k], a.kp, step, a.seed, a.rng,
-                   a.stream);
-    if (bs == FREE_BINDABLE && bestSeg >= 0) {
-        const int sg = bestSeg;
-        m.bindArc[k] = bind_arc(mx, my, mz, a.e1[sg], a.e1[nSeg + sg], a.e1[2 * nSeg + sg], a.e2[sg],
-                                a.e2[nSeg + sg], a.e2[2 * nSeg + sg]);
-        bs = sg;
+    motor_kinetics(a, k, cnt, bestSeg, mx, my, mz);
+}
+
+// ---------------------------------------------------------------- reach skin (sbx_net_set_reach_skin)
+// No counterpart in the reference, which bins every segment and head into the binding grid and sorts it
+// every step for gridReachable (BindingDetectionSystem.java:157-227).  A hit (head h, segment s) has
+// |h - c_s| < 0.5 L_s + myoColTol: the contact point lies on the segment and within myoColTol of the head.
+// So a segment kept in a coarse grid at its centre c0 of the last rebuild, with the skin
+// sk = cell - (0.5 L_s + myoColTol) - 2 slack, lies within one coarse cell of every head it can reach while
+// |c_s - c0| <= sk: the 27 coarse cells around the head's current cell hold all of them (clamping at the
+// faces keeps neighbouring cells neighbours).  Segments past their skin (the guard in k_rod) or too long
+// for the cell are on the outside-skin list, which every head scans too; past its capacity heads scan every
+// segment.  A candidate is a hit if its binding-grid cell (the per-step binning of its current centre) lies
+// in the head's clamped stencil and it passes reachTestDistSq; its place in the per-step list is its key
+// (stencil position << 26 | index), so hits are ranked by key and bindNearest is the argmin over
+// (distance, key) of the first maxCand of them.
+constexpr int RS_EXTRA = 0;      // segments outside their skins since the last rebuild (> extraCap: overflow)
+constexpr int RS_REBUILDS = 1;   // rebuilds so far
+constexpr int RS_MAXSKIN = 2;    // largest learned skin for the rebuild being built (float bits)
+constexpr int RS_GEN = 3;        // generation of the last rebuild (the host's count)
+constexpr int RS_BRUTE = 4;      // steps whose heads scanned every segment (outside-skin list overflow)
+constexpr int RS_SELECT = 5;     // heads placed by the selection path
+constexpr int RS_GEOM = 8;       // 8 words: xMin, yMin, zMin, cell, invCell (float bits), nX, nY, nZ
+constexpr int RS_WORDS = 16;
+constexpr int RS_PER_LANE = 4;   // hits a lane holds in registers (more on any lane: the selection path)
+constexpr int RS_KEY_BITS = 26;
+constexpr unsigned RS_KEY_MASK = (1u << RS_KEY_BITS) - 1u;
+constexpr int RS_MODEL_CELLS = 65536;  // coarse cells the model evaluates per rebuild (at most about)
+constexpr float RS_SLACK = 1.0e-3f;  // um, covers float rounding of the reach bound, the cells and the guard
+
+struct ReachArgs {
+    const float *coord;    // step-start segment centres (planar, stride nSeg)
+    const float *skin;     // per-segment skin (< 0: on the outside-skin list)
+    const int *off, *cont; // coarse CSR of the last rebuild
+    const int *extra;      // the outside-skin list
+    int *words;            // RS_* words
+    int *hostWords;        // pinned {generation, outside-skin count} (device address)
+    int extraCap;
+    GridGeom fine;         // the binding grid as the per-step path bins the segments (k_body_cell_view)
+};
+
+// gridReachable + catchSlipRelease + bindNearest + cycle per motor from the coarse grid (G lanes per head).
+template <int G>
+__global__ void __launch_bounds__(256) k_motor_bind_skin(MotorArgs a, ReachArgs r) {
+    constexpr int U = MB_ROUND;
+    const int lane = threadIdx.x & 63, gl = lane % G, grp = lane / G;
+    const int k = (blockIdx.x * 256 + threadIdx.x) / G;
+    const sbx_motors &m = a.m;
+    const int nExtra = r.words[RS_EXTRA];
+    const bool brute = nExtra > r.extraCap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the host's (lagged) view of the guard
+        r.hostWords[1] = nExtra;
+        r.hostWords[0] = r.words[RS_GEN];
+        if (brute) r.words[RS_BRUTE] += 1;
     }
-    m.boundSeg[k] = bs;
-    m.cooldown[k] = cool;
-    m.nucleotideState[k] = nucleotide_one(k, m.nucleotideState[k], bs >= 0, m.forceDotHist, a.np, step, a.seed,
-                                          a.rng, a.stream);
+    if (k >= m.nM) return;  // whole groups leave together
+    const int nM = m.nM, nSeg = a.nSeg;
+    const float mx = m.head[k], my = m.head[nM + k], mz = m.head[2 * nM + k];
+    const float mux = m.hUVec[k], muy = m.hUVec[nM + k], muz = m.hUVec[2 * nM + k];
+    const float rux = m.rodUVec[k], ruy = m.rodUVec[nM + k], ruz = m.rodUVec[2 * nM + k];
+    const ReachGrid &rg = a.rg;
+    // the head's binding-grid stencil, as the per-step walk clamps it
+    const int cx = clamp_bin((mx - rg.xMin) * rg.invCell, rg.nX);
+    const int cy = clamp_bin((my - rg.yMin) * rg.invCell, rg.nY);
+    const int cz = clamp_bin((mz - rg.zMin) * rg.invCell, rg.nZ);
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < rg.nX ? cx + 1 : rg.nX - 1;
+    const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy + 1 < rg.nY ? cy + 1 : rg.nY - 1;
+    const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz + 1 < rg.nZ ? cz + 1 : rg.nZ - 1;
+    // the coarse cells around the head's current position
+    GridGeom cg{};
+    cg.nX = r.words[RS_GEOM + 5];
+    cg.nY = r.words[RS_GEOM + 6];
+    cg.nZ = r.words[RS_GEOM + 7];
+    cg.periodic = 0;
+    const float cInv = __int_as_float(r.words[RS_GEOM + 4]);
+    const int ccx = clamp_bin((mx - __int_as_float(r.words[RS_GEOM])) * cInv, cg.nX);
+    const int ccy = clamp_bin((my - __int_as_float(r.words[RS_GEOM + 1])) * cInv, cg.nY);
+    const int ccz = clamp_bin((mz - __int_as_float(r.words[RS_GEOM + 2])) * cInv, cg.nZ);
+    int pre[18], shift[18];
+    int nGrid = 0;
+    if (!brute) nGrid = stencil_map(ccx + ccy * cg.nX + ccz * cg.nX * cg.nY, cg, r.off, pre, shift);
+    const int total = brute ? nSeg : nGrid + nExtra;
+    const float tol = a.kp.myoColTol, align = a.kp.alignTol;
+    // q-th candidate, or -1
+    auto cand = [&](int q) -> int {
+        if (q >= total) return -1;
+        if (brute) return q;
+        if (q < nGrid) {
+            const int s = r.cont[stencil_at9(pre, shift, q)];
+            return r.skin[s] < 0.0f ? -1 : s;  // listed segments are taken from the list
+        }
+        return r.extra[q - nGrid];
+    };
+    // reach distance^2 of segment s (< 0: no hit) and its key in the per-step list; the centre and both ends
+    // are loaded together (one round trip)
+    auto test = [&](int s, unsigned &key) -> float {
+        const float px = r.coord[s], py = r.coord[nSeg + s], pz = r.coord[2 * nSeg + s];
+        const float e1x = a.e1[s], e1y = a.e1[nSeg + s], e1z = a.e1[2 * nSeg + s];
+        const float e2x = a.e2[s], e2y = a.e2[nSeg + s], e2z = a.e2[2 * nSeg + s];
+        const int3 b = cell3_of(px, py, pz, r.fine);
+        if (b.x < x0 || b.x > x1 || b.y < y0 || b.y > y1 || b.z < z0 || b.z > z1) return -1.0f;
+        key = ((unsigned)((b.z - z0) * 9 + (b.y - y0) * 3 + (b.x - x0)) << RS_KEY_BITS) | (unsigned)s;
+        return reach_test(mx, my, mz, mux, muy, muz, rux, ruy, ruz, e1x, e1y, e1z, e2x, e2y, e2z, tol, align);
+    };
+    const unsigned long long groupBits = (G == 64 ? ~0ull : ((1ull << G) - 1)) << (grp * G);
+    unsigned hk[RS_PER_LANE];
+    float hd[RS_PER_LANE];
+#pragma unroll
+    for (int q = 0; q < RS_PER_LANE; q++) {
+        hk[q] = 0xffffffffu;
+        hd[q] = 1.0e30f;
+    }
+    int nh = 0, cnt = 0;
+    for (int base = 0; base < total; base += G * U) {
+        int sg[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) sg[u] = cand(base + u * G + gl);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            unsigned key = 0;
+            float d = -1.0f;
+            if (sg[u] >= 0) d = test(sg[u], key);
+            const bool hit = d >= 0.0f;
+            cnt += __popcll(__ballot(hit) & groupBits);
+            if (hit) {
+#pragma unroll
+                for (int q = 0; q < RS_PER_LANE; q++)
+                    if (q == nh) {
+                        hk[q] = key;
+                        hd[q] = d;
+                    }
+                nh++;
+            }
+        }
+    }
+    int *slice = m.reachSeg + (size_t)k * a.maxCand;
+    const bool spill = (__ballot(nh > RS_PER_LANE) & groupBits) != 0;
+    unsigned bestK = 0xffffffffu;
+    float bestD = 1.0e30f;
+    if (!spill && cnt <= a.maxCand) {
+        if (cnt > 0) {
+            // a hit's place = the group's hits with a smaller key (keys are distinct)
+            int rank[RS_PER_LANE] = {0, 0, 0, 0};
+            const int gbase = grp * G;
+            for (int j = 0; j < G; j++) {
+                const int nj = __shfl(nh, gbase + j, 64);
+#pragma unroll
+                for (int p = 0; p < RS_PER_LANE; p++) {
+                    const unsigned kj = __shfl(hk[p], gbase + j, 64);
+#pragma unroll
+                    for (int q = 0; q < RS_PER_LANE; q++) rank[q] += (p < nj && kj < hk[q]) ? 1 : 0;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < RS_PER_LANE; q++)
+                if (q < nh) {
+                    slice[rank[q]] = (int)(hk[q] & RS_KEY_MASK);
+                    if (hd[q] < bestD || (hd[q] == bestD && hk[q] < bestK)) {
+                        bestD = hd[q];
+                        bestK = hk[q];
+                    }
+                }
+#pragma unroll
+            for (int o = G / 2; o >= 1; o >>= 1) {
+                const float od = __shfl_xor(bestD, o, 64);
+                const unsigned ok = __shfl_xor(bestK, o, 64);
+                if (od < bestD || (od == bestD && ok < bestK)) {
+                    bestD = od;
+                    bestK = ok;
+                }
+            }
+        }
+    } else {
+        // selection: the keep smallest keys one per pass over the candidates (dense heads only)
+        const int keep = cnt < a.maxCand ? cnt : a.maxCand;
+        unsigned lo = 0;
+        for (int p = 0; p < keep; p++) {
+            unsigned mk = 0xffffffffu;
+            float md = 1.0e30f;
+            for (int base = 0; base < total; base += G) {
+                const int s = cand(base + gl);
+                if (s >= 0) {
+                    unsigned key = 0;
+                    const float d = test(s, key);
+                    if (d >= 0.0f && key >= lo && key < mk) {
+                        mk = key;
+                        md = d;
+                    }
+                }
+            }
+#pragma unroll
+            for (int o = G / 2; o >= 1; o >>= 1) {
+                const unsigned ok = __shfl_xor(mk, o, 64);
+                const float od = __shfl_xor(md, o, 64);
+                if (ok < mk) {
+                    mk = ok;
+                    md = od;
+                }
+            }
+            if (gl == 0) slice[p] = (int)(mk & RS_KEY_MASK);
+            if (md < bestD) {  // list order is key order: strict < keeps the first of equal distances
+                bestD = md;
+                bestK = mk;
+            }
+            lo = mk + 1u;
+        }
+        if (gl == 0) atomicAdd(&r.words[RS_SELECT], 1);
+    }
+    if (gl != 0) return;
+    motor_kinetics(a, k, cnt, bestK != 0xffffffffu ? (int)(bestK & RS_KEY_MASK) : -1, mx, my, mz);
+}
+
+// A rebuild (enqueued by the host between steps): [learned: per-cell model skins] -> geometry -> bin -> sort.
+struct ReachBuild {
+    int n, extraCap, learned, gen, sampleStride;
+    float since;                     // steps since the previous build (learned features)
+    const float *coord, *uVec, *segLength;
+    const int *links;                // activeLinkCount (learned features; may be null)
+    float *ref, *refU, *skin;
+    int *cell, *extra, *words;
+    const int *off, *cont;           // the previous build's CSR (learned features)
+    const float *model;
+    float tol, R0, floorCell, skinDefault, skinMin, skinMax, margin;
+    float ox, oy, oz, spanX, spanY, spanZ;  // the binding grid's origin and extent
+    long long cellsMax;
+};
+
+// Learned mode: the model's skin for occupied coarse cells of the previous build (features of their members'
+// motion since that build); the largest goes to RS_MAXSKIN.  A thread per segment evaluates the cell it leads
+// (first in the cell's contents) when the segment index is a multiple of sampleStride, so a rebuild evaluates
+// at most n / sampleStride cells whatever the grid's size.
+__global__ void __launch_bounds__(256) k_rs_assign(ReachBuild b) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= b.n || s % b.sampleStride != 0) return;
+    const int c = b.cell[s];
+    if (c < 0 || b.cont[b.off[c]] != s) return;
+    GridGeom g{};  // raw displacements, as the guard takes them
+    float f[SKIN_FEATURES], y[MLP_OUT];
+    skin_cell_features(c, b.off, b.cont, b.n, b.coord, b.ref, b.uVec, b.refU, b.links, b.since,
+                       __int_as_float(b.words[RS_GEOM + 3]), g, f);
+    mlp_forward(b.model, f, y);
+    const float sk = fminf(b.skinMax, fmaxf(b.skinMin, b.margin * expf(y[0])));
+    atomicMax(&b.words[RS_MAXSKIN], __float_as_int(sk));  // positive floats order as ints
+}
+
+// One thread: this build's coarse grid (cell = the reach of the longest segment + the largest skin, at least
+// the floor) over the binding grid's extent; opens the outside-skin list.
+__global__ void k_rs_geom(ReachBuild b) {
+    if (threadIdx.x != 0) return;
+    float maxSkin = b.skinDefault;
+    if (b.learned) {
+        const float ms = __int_as_float(b.words[RS_MAXSKIN]);
+        if (ms > 0.0f) maxSkin = ms;
+    }
+    b.words[RS_MAXSKIN] = 0;
+    const float cell = fmaxf(b.floorCell, b.R0 + maxSkin + 2.0f * RS_SLACK);
+    int nX = max(1, (int)ceilf(b.spanX / cell)), nY = max(1, (int)ceilf(b.spanY / cell));
+    int nZ = max(1, (int)ceilf(b.spanZ / cell));
+    if ((long long)nX * nY * nZ > b.cellsMax) nX = nY = nZ = 1;  // one cell holds every segment: still complete
+    b.words[RS_GEOM] = __float_as_int(b.ox);
+    b.words[RS_GEOM + 1] = __float_as_int(b.oy);
+    b.words[RS_GEOM + 2] = __float_as_int(b.oz);
+    b.words[RS_GEOM + 3] = __float_as_int(cell);
+    b.words[RS_GEOM + 4] = __float_as_int(1.0f / cell);
+    b.words[RS_GEOM + 5] = nX;
+    b.words[RS_GEOM + 6] = nY;
+    b.words[RS_GEOM + 7] = nZ;
+    b.words[RS_EXTRA] = 0;
+    b.words[RS_REBUILDS] += 1;
+    b.words[RS_GEN] = b.gen;
+}
+
+// Per segment: reference centre (and axis), skin, coarse cell; a segment too long for the cell goes on the list.
+__global__ void __launch_bounds__(256) k_rs_bin(ReachBuild b) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= b.n) return;
+    const int n = b.n;
+    const float x = b.coord[s], y = b.coord[n + s], z = b.coord[2 * n + s];
+    b.ref[s] = x;
+    b.ref[n + s] = y;
+    b.ref[2 * n + s] = z;
+    if (b.refU) {
+        b.refU[s] = b.uVec[s];
+        b.refU[n + s] = b.uVec[n + s];
+        b.refU[2 * n + s] = b.uVec[2 * n + s];
+    }
+    const float cell = __int_as_float(b.words[RS_GEOM + 3]), inv = __int_as_float(b.words[RS_GEOM + 4]);
+    const float sk = cell - (0.5f * b.segLength[s] + b.tol) - 2.0f * RS_SLACK;
+    if (sk > 0.0f) {
+        const int nX = b.words[RS_GEOM + 5], nY = b.words[RS_GEOM + 6], nZ = b.words[RS_GEOM + 7];
+        const int ix = clamp_bin((x - __int_as_float(b.words[RS_GEOM])) * inv, nX);
+        const int iy = clamp_bin((y - __int_as_float(b.words[RS_GEOM + 1])) * inv, nY);
+        const int iz = clamp_bin((z - __int_as_float(b.words[RS_GEOM + 2])) * inv, nZ);
+        b.skin[s] = sk;
+        b.cell[s] = ix + nX * (iy + nY * iz);
+    } else {
+        b.skin[s] = -1.0f;
+        b.cell[s] = -1;  // not in the coarse CSR (the sort skips negative keys)
+        const int e = atomicAdd(&b.words[RS_EXTRA], 1);
+        if (e < b.extraCap) b.extra[e] = s;
+    }
 }
 
 // zero + MotorJointSystem.joints for one sub-body: F = (float)(0 + joints)
@@ -1335,6 +1681,81 @@ int drain_timers(sbx_net *net) {
     return SBX_OK;
 }
 
+// A reach-skin rebuild on the main stream (between steps or graph replays, never inside a capture): [learned:
+// the model's skins over the previous build's cells] -> geometry -> bin -> sort of the segments into the
+// coarse grid.  why: 0 stale state, 1 horizon, 2 outside-skin count.
+int rs_rebuild(sbx_net *net, int why) {
+    sbx_ctx *c = net->ctx;
+    hipStream_t s = c->stream;
+    const int n = net->n;
+    ReachBuild b{};
+    b.n = n; b.extraCap = net->rsp.extraCap;
+    b.sampleStride = n > RS_MODEL_CELLS ? (n + RS_MODEL_CELLS - 1) / RS_MODEL_CELLS : 1;
+    b.learned = net->rsMode == SBX_SKIN_LEARNED && net->rsGen > 0 && net->rsModel;
+    b.gen = net->rsGen + 1;
+    b.since = (float)(net->rsSince > 0 ? net->rsSince : 1);
+    b.coord = net->coordB[net->cur]; b.uVec = net->uB[net->cur]; b.segLength = net->fil.segLength;
+    b.links = net->L > 0 ? net->xl.activeLinkCount : nullptr;
+    b.ref = net->rsRef; b.refU = net->rsRefU; b.skin = net->rsSkin;
+    b.cell = net->rsCell; b.extra = net->rsExtra; b.words = net->rsWords;
+    b.off = net->rsOff; b.cont = net->rsCont; b.model = net->rsModel;
+    b.tol = net->kp.myoColTol; b.R0 = net->rsR0; b.floorCell = net->rsFloor;
+    b.skinDefault = net->rsp.skinDefault; b.skinMin = net->rsp.skinMin; b.skinMax = net->rsp.skinMax;
+    b.margin = net->rsp.margin;
+    const float fine = 1.0f / net->rg.invCell;
+    b.ox = net->rg.xMin; b.oy = net->rg.yMin; b.oz = net->rg.zMin;
+    b.spanX = fine * (float)net->rg.nX; b.spanY = fine * (float)net->rg.nY; b.spanZ = fine * (float)net->rg.nZ;
+    b.cellsMax = net->rsCellsMax;
+    int r;
+    {
+        Scope sc(net, T_RS_REBUILD);
+        if (b.learned) hipLaunchKernelGGL(k_rs_assign, dim3(blocks_for(n, 256)), dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_rs_geom, dim3(1), dim3(64), 0, s, b);
+        hipLaunchKernelGGL(k_rs_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, b);
+        if ((r = check_launch(c, "reach skin rebuild"))) return r;
+        if ((r = counting_sort(c, net->sort, net->rsCell, n, (int)net->rsCellsMax, n, net->rsOff, net->rsCont)))
+            return r;
+    }
+    net->rsGen = b.gen;
+    net->rsValid = 1;
+    net->rsSince = 0;
+    net->rsWhy[why]++;
+    return SBX_OK;
+}
+
+// Before a step or a graph replay: rebuild if the coarse grid is stale, the horizon has passed, or the latest
+// finished step reported more than extraTrigger segments outside their skins (read from pinned memory without
+// waiting; a late view only delays the rebuild, the search stays exact).
+int rs_maybe_rebuild(sbx_net *net) {
+    if (!net->rsMode) return SBX_OK;
+    int why = -1;
+    if (!net->rsValid) why = 0;
+    else if (net->rsp.horizon > 0 && net->rsSince >= net->rsp.horizon) why = 1;
+    else if (net->rsHost[0] == net->rsGen && net->rsHost[1] > net->rsp.extraTrigger) why = 2;
+    return why < 0 ? SBX_OK : rs_rebuild(net, why);
+}
+
+// After a step or a replay: count it, and bound how far the host runs ahead of the device (so the
+// outside-skin count it reads is recent).
+int rs_advance(sbx_net *net, int steps) {
+    if (!net->rsMode) return SBX_OK;
+    net->rsSince += steps;
+    net->rsSteps += steps;
+    sbx_ctx *c = net->ctx;
+    const int k = net->rsEvNext;
+    int r = SBX_OK;
+    if (net->rsEvLive == sbx_net::RS_INFLIGHT) {  // slot k holds the oldest record: wait for it
+        r = check_hip(c, hipEventSynchronize(net->rsEv[k]), "reach skin throttle");
+        net->rsEvLive--;
+    }
+    if (!r && !net->rsEv[k]) r = check_hip(c, hipEventCreateWithFlags(&net->rsEv[k], hipEventDisableTiming), "event");
+    if (!r) r = check_hip(c, hipEventRecord(net->rsEv[k], c->stream), "reach skin record");
+    if (r) return r;
+    net->rsEvLive++;
+    net->rsEvNext = (k + 1) % sbx_net::RS_INFLIGHT;
+    return SBX_OK;
+}
+
 // Binding and minifilament mechanics of one step (DenseContractileHarness.cpuStep,
 // DenseContractileHarness.java:350-395), before the filament block.  Every kernel reads the
 // step-start filament pose (the current ping-pong buffer); motor and backbone bodies integrate in
@@ -1349,11 +1770,12 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
     {
         Scope sc(net, T_BIN);
         hipLaunchKernelGGL(k_pub_heads, dim3(blocks_for(nM, 256)), dim3(256), 0, s, m);
-        hipLaunchKernelGGL(k_body_cell_view, dim3(blocks_for(cap, 256)), dim3(256), 0, s, coord, n, m.head, nM,
-                           net->geom, net->bodyCell);
+        if (!net->rsMode)
+            hipLaunchKernelGGL(k_body_cell_view, dim3(blocks_for(cap, 256)), dim3(256), 0, s, coord, n, m.head, nM,
+                               net->geom, net->bodyCell);
         if ((r = check_launch(c, "k_pub_heads/k_body_cell_view"))) return r;
     }
-    {
+    if (!net->rsMode) {
         Scope sc(net, T_SORT);
         if ((r = counting_sort(c, net->sort, net->bodyCell, cap, net->p.gridDims[3], cap, net->cellOffsets,
                                net->cellContents)))
@@ -1370,8 +1792,21 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
         a.maxCand = net->p.maxCand; a.nSeg = n; a.seed = net->mp.seedMotor; a.rng = net->p.rng;
         a.stream = net->p.stream; a.dStep = net->dStep; a.stepOff = stepOff;
         const int G = net->mbGroup;
-        auto kern = G == 64 ? k_motor_bind<64> : (G == 32 ? k_motor_bind<32> : k_motor_bind<16>);
-        hipLaunchKernelGGL(kern, dim3(blocks_for(nM, 256 / G)), dim3(256), 0, s, a);
+        if (net->rsMode) {
+            ReachArgs ra;
+            ra.coord = coord; ra.skin = net->rsSkin; ra.off = net->rsOff; ra.cont = net->rsCont;
+            ra.extra = net->rsExtra; ra.words = net->rsWords; ra.hostWords = net->rsHostDev;
+            ra.extraCap = net->rsp.extraCap; ra.fine = net->geom;
+            auto kern = G == 64 ? k_motor_bind_skin<64>
+                                : (G == 32 ? k_motor_bind_skin<32>
+                                           : (G == 16 ? k_motor_bind_skin<16>
+                                                      : (G == 8 ? k_motor_bind_skin<8> : k_motor_bind_skin<4>)));
+            hipLaunchKernelGGL(kern, dim3(blocks_for(nM, 256 / G)), dim3(256), 0, s, a, ra);
+        } else {
+            const int Gp = G < 16 ? 16 : G;  // the per-step walk takes 16, 32 or 64 lanes
+            auto kern = Gp == 64 ? k_motor_bind<64> : (Gp == 32 ? k_motor_bind<32> : k_motor_bind<16>);
+            hipLaunchKernelGGL(kern, dim3(blocks_for(nM, 256 / Gp)), dim3(256), 0, s, a);
+        }
     }
     {
         Scope sc(net, T_MJOINT);
@@ -1650,6 +2085,15 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
     a.skinRef = skinOn ? net->refC : nullptr;
     a.skin = skinOn ? net->skin : nullptr;
     a.rebuildFlag = skinOn ? net->skinFlags + skf_rebuild(nxt) : nullptr;
+    a.rsExtra = nullptr;
+    a.rsCap = 0;
+    if (net->rsMode) {
+        a.skinRef = net->rsRef;
+        a.skin = net->rsSkin;
+        a.rebuildFlag = net->rsWords + RS_EXTRA;
+        a.rsExtra = net->rsExtra;
+        a.rsCap = net->rsp.extraCap;
+    }
     a.bodyPack = (net->p.gridOn && !net->hasMotors) ? net->bodyPack[nxt] : nullptr;
     a.boundR = net->boundR;
     a.mOff = net->mOff; a.mList = net->mList; a.bondData = net->mo.bondData;
@@ -1728,9 +2172,14 @@ void release(sbx_net *net) {
                     net->cgDev,    net->cCellsDev, net->bodyPack[0], net->bodyPack[1], net->sortedPose[0],
                     net->sortedPose[1], net->sortedCell[0], net->sortedCell[1], net->csrOff[1], net->csrCont[1],
                     net->gridCounts[0], net->gridCounts[1], net->gridSlot[0], net->gridSlot[1],
-                    net->reps,     net->rodRep,    net->linkRep,  net->formInts, net->formFloats};
+                    net->reps,     net->rodRep,    net->linkRep,  net->formInts, net->formFloats,
+                    net->rsRef,    net->rsRefU,    net->rsSkin,   net->rsCell,  net->rsOff,  net->rsCont,
+                    net->rsExtra,  net->rsWords,   net->rsModel};
     for (void *b : bufs)
         if (b) hipFree(b);
+    if (net->rsHost) hipHostFree((void *)net->rsHost);
+    for (hipEvent_t e : net->rsEv)
+        if (e) hipEventDestroy(e);
     sort_scratch_free(net->sort);
     sort_scratch_free(net->formSort);
     turn_scratch_free(net->turnScratch);
@@ -1856,6 +2305,7 @@ int run_turnover(sbx_net *net, int step) {
     }
     net->derivedValid = 0;
     net->formFilIDValid = 0;
+    net->rsValid = 0;  // segment lengths, pose and slots changed
     return SBX_OK;
 }
 
@@ -2104,6 +2554,7 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
     const float *gp = net->p.gridParams;
     net->rg = {gp[0], gp[1], gp[2], gp[4], net->p.gridDims[0], net->p.gridDims[1], net->p.gridDims[2]};
     net->hasMotors = 1;
+    net->rsValid = 0;  // (reach skin) new motor parameters: rebuild before the next search
     hipLaunchKernelGGL(k_owner_view, dim3(blocks_for(cap, 256)), dim3(256), 0, c->stream, ownerStore, ownerSlot, n,
                        nM);
     if ((r = check_launch(c, "k_owner_view"))) return r;
@@ -2274,7 +2725,9 @@ int step_range(sbx_net *net, int step0, int nSteps) {
                 return r;
         }
         if (net->cur != net->graphCur && nSteps - done > 0) {
+            if ((r = rs_maybe_rebuild(net))) return r;
             if ((r = enqueue_step(net, done - advanced))) return r;
+            if ((r = rs_advance(net, 1))) return r;
             done++;
         }
         while (nSteps - done >= G) {
@@ -2284,7 +2737,9 @@ int step_range(sbx_net *net, int step0, int nSteps) {
                 advanced = done;
             }
             if ((r = join_side(net))) return r;
+            if ((r = rs_maybe_rebuild(net))) return r;
             if ((r = check_hip(c, hipGraphLaunch(net->graphExec, c->stream), "graph launch"))) return r;
+            if ((r = rs_advance(net, G))) return r;
             net->needFork = 1;  // the replay's side work is joined into it; later side work forks anew
             net->evCaptured = 1;
             net->lastCsr = net->graphLastCsr;
@@ -2296,7 +2751,9 @@ int step_range(sbx_net *net, int step0, int nSteps) {
         }
     }
     for (; done < nSteps; done++) {
+        if ((r = rs_maybe_rebuild(net))) return r;
         if ((r = enqueue_step(net, done - advanced))) return r;
+        if ((r = rs_advance(net, 1))) return r;
     }
     if (net->bpPending) {  // the last replay's broad phase: the slices of the call's last pose
         const bool ovl = overlapped(net);
@@ -2424,9 +2881,10 @@ int sbx_net_publish(sbx_net *net) {
         }
     }
     // the caller may now edit its pose and link arrays: re-bin, re-pack, recount the active links and
-    // (skin mode) rebuild next step
+    // (skin modes) rebuild next step
     net->derivedValid = 0;
     net->countsValid = 0;
+    net->rsValid = 0;
     if (!r && net->skinFlags) {
         r = fill_i32(c, net->skinFlags + skf_rebuild(net->cur), 1, 1);
         if (!r) r = fill_i32(c, net->skinFlags + skf_rebuild(1 - net->cur), 0, 1);
@@ -2574,6 +3032,119 @@ int sbx_net_skin_stats(sbx_net *net, long long *out, int cap) {
     const int words[] = {skf_rebuild(net->cur), SKF_SINCE, SKF_REBUILDS, SKF_OVERFLOW, SKF_ENTRIES, SKF_STEPS, SKF_HEAVY};
     int k = 0;
     for (; k < (int)(sizeof(words) / sizeof(words[0])) && k < cap; k++) out[k] = f[words[k]];
+    return k;
+}
+
+int sbx_net_set_reach_skin(sbx_net *net, int mode, const sbx_reach_skin_params *sp, const float *modelWeights) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, mode == SBX_SKIN_REBUILD || mode == SBX_SKIN_FIXED || mode == SBX_SKIN_LEARNED,
+                "net_set_reach_skin: unknown mode %d", mode);
+    SBX_REQUIRE(c, net->hasMotors, "net_set_reach_skin: needs motors attached (the binding grid's search)");
+    if (mode != SBX_SKIN_REBUILD) {
+        SBX_REQUIRE(c, sp, "net_set_reach_skin: null params");
+        SBX_REQUIRE(c, sp->skinDefault > 0.0f && sp->skinMin > 0.0f && sp->skinMax >= sp->skinMin &&
+                           sp->skinDefault <= sp->skinMax && sp->margin > 0.0f && sp->cellFloor >= 0.0f,
+                    "net_set_reach_skin: need 0 < skinMin <= skinDefault <= skinMax, margin > 0, cellFloor >= 0");
+        SBX_REQUIRE(c, sp->horizon >= 0 && sp->extraTrigger >= 0 && sp->extraCap > 0,
+                    "net_set_reach_skin: need horizon >= 0, extraTrigger >= 0, extraCap > 0");
+        SBX_REQUIRE(c, mode != SBX_SKIN_LEARNED || modelWeights, "net_set_reach_skin: learned mode needs model weights");
+        SBX_REQUIRE(c, net->n < (1 << RS_KEY_BITS), "net_set_reach_skin: more than 2^26 segments");
+        SBX_REQUIRE(c, !net->reps, "net_set_reach_skin: a batch runs the per-step rebuild");
+    }
+    int r = set_device(c);
+    if (!r) r = check_hip(c, hipStreamSynchronize(c->stream), "net_set_reach_skin sync");
+    if (r) return r;
+    destroy_graphs(net);  // a captured step no longer matches the launch sequence
+    net->rsMode = 0;
+    if (mode == SBX_SKIN_REBUILD) return SBX_OK;
+    const int n = net->n;
+    // the reach of the longest segment now (a segment grown longer later gets a smaller skin, or the list)
+    std::vector<float> len((size_t)n);
+    r = check_hip(c, hipMemcpy(len.data(), net->fil.segLength, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost),
+                  "net_set_reach_skin lengths");
+    if (r) return r;
+    float maxL = 0.0f;
+    for (float v : len)
+        if (v > maxL) maxL = v;
+    const float fine = 1.0f / net->rg.invCell;
+    const float floorCell = sp->cellFloor > 0.0f ? sp->cellFloor : fine;
+    const float spanX = fine * (float)net->rg.nX, spanY = fine * (float)net->rg.nY, spanZ = fine * (float)net->rg.nZ;
+    // the smallest cell a rebuild can take (k_rs_geom), which sizes the CSR
+    const float R0 = 0.5f * maxL + net->kp.myoColTol;
+    const float minSkin = mode == SBX_SKIN_LEARNED ? std::min(sp->skinMin, sp->skinDefault) : sp->skinDefault;
+    const float minCell = std::max(floorCell, R0 + minSkin + 2.0f * RS_SLACK);
+    const long long cells = (long long)std::max(1, (int)ceilf(spanX / minCell)) *
+                            std::max(1, (int)ceilf(spanY / minCell)) * std::max(1, (int)ceilf(spanZ / minCell));
+    SBX_REQUIRE(c, cells < (1LL << 30), "net_set_reach_skin: %lld coarse cells at the smallest cell %g um "
+                "(raise cellFloor or the skins)", cells, minCell);
+    // (re)allocate
+    void *old[] = {net->rsRef, net->rsRefU, net->rsSkin, net->rsCell, net->rsOff, net->rsCont, net->rsExtra,
+                   net->rsWords, net->rsModel};
+    for (void *p : old)
+        if (p) hipFree(p);
+    net->rsRef = net->rsRefU = net->rsSkin = net->rsModel = nullptr;
+    net->rsCell = net->rsOff = net->rsCont = net->rsExtra = net->rsWords = nullptr;
+    do {
+        if ((r = alloc(c, &net->rsRef, 3 * (size_t)n))) break;
+        if ((r = alloc(c, &net->rsRefU, 3 * (size_t)n))) break;
+        if ((r = alloc(c, &net->rsSkin, (size_t)n))) break;
+        if ((r = alloc(c, &net->rsCell, (size_t)n))) break;
+        if ((r = alloc(c, &net->rsOff, (size_t)cells + 1))) break;
+        if ((r = alloc(c, &net->rsCont, (size_t)n))) break;
+        if ((r = alloc(c, &net->rsExtra, (size_t)sp->extraCap))) break;
+        if ((r = alloc(c, &net->rsWords, RS_WORDS))) break;
+        if (mode == SBX_SKIN_LEARNED) {
+            if ((r = alloc(c, &net->rsModel, MLP_NPARAM))) break;
+            r = check_hip(c, hipMemcpyAsync(net->rsModel, modelWeights, sizeof(float) * MLP_NPARAM,
+                                            hipMemcpyDeviceToDevice, c->stream), "net_set_reach_skin model");
+            if (r) break;
+        }
+        if (!net->rsHost) {
+            void *h = nullptr;
+            if ((r = check_hip(c, hipHostMalloc(&h, 2 * sizeof(int), hipHostMallocDefault), "hipHostMalloc"))) break;
+            net->rsHost = (volatile int *)h;
+            void *d = nullptr;
+            if ((r = check_hip(c, hipHostGetDevicePointer(&d, h, 0), "hipHostGetDevicePointer"))) break;
+            net->rsHostDev = (int *)d;
+        }
+        if ((r = fill_i32(c, net->rsWords, 0, RS_WORDS))) break;
+        r = check_hip(c, hipStreamSynchronize(c->stream), "net_set_reach_skin sync");
+    } while (0);
+    if (r) return r;
+    net->rsHost[0] = -1;
+    net->rsHost[1] = 0;
+    net->rsp = *sp;
+    net->rsR0 = R0;
+    net->rsFloor = floorCell;
+    net->rsCellsMax = cells;
+    net->rsMode = mode;
+    net->rsValid = 0;  // the next step rebuilds
+    net->rsGen = 0;
+    net->rsSince = 0;
+    net->rsWhy[0] = net->rsWhy[1] = net->rsWhy[2] = 0;
+    net->rsSteps = 0;
+    return SBX_OK;
+}
+
+int sbx_net_reach_skin_stats(sbx_net *net, long long *out, int cap) {
+    if (!net) return SBX_ERR_ARG;
+    sbx_ctx *c = net->ctx;
+    SBX_CHECK_CTX(c);
+    SBX_REQUIRE(c, out && cap >= 0, "net_reach_skin_stats: null out");
+    if (!net->rsWords) return 0;
+    int w[RS_WORDS];
+    int r = check_hip(c, hipMemcpyAsync(w, net->rsWords, sizeof(w), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (!r) r = check_hip(c, hipStreamSynchronize(c->stream), "sync");
+    if (r) return r;
+    float cell = 0.0f;
+    memcpy(&cell, &w[RS_GEOM + 3], sizeof(cell));
+    const long long v[] = {w[RS_REBUILDS],   w[RS_EXTRA],     w[RS_BRUTE],     w[RS_SELECT],
+                           llroundf(1000.0f * cell),
+                           net->rsWhy[0],    net->rsWhy[1],   net->rsWhy[2],   net->rsSteps};
+    int k = 0;
+    for (; k < (int)(sizeof(v) / sizeof(v[0])) && k < cap; k++) out[k] = v[k];
     return k;
 }
 
@@ -2734,7 +3305,8 @@ int sbx_net_set_motor_group(sbx_net *net, int lanes) {
     if (!net) return SBX_ERR_ARG;
     sbx_ctx *c = net->ctx;
     SBX_CHECK_CTX(c);
-    SBX_REQUIRE(c, lanes == 16 || lanes == 32 || lanes == 64, "net_set_motor_group: 16, 32 or 64 lanes per motor");
+    SBX_REQUIRE(c, lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64,
+                "net_set_motor_group: 4, 8, 16, 32 or 64 lanes per motor");
     int r = set_device(c);
     if (!r) r = drop_graph(net);
     if (r) return r;
@@ -2811,13 +3383,16 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
     if ((r = sbx_net_set_shape(net, best[0], best[1], best[2], best[3]))) return r;
     net->overlap = best[4];
     if (net->hasMotors) {  // then the motor stage's group size, at the chosen shape
-        const int gs[3] = {16, 32, 64};
+        // the reach skin's search also tries narrow groups (few candidates per head)
+        const int gsGrid[3] = {16, 32, 64}, gsSkin[5] = {4, 8, 16, 32, 64};
+        const int *gs = net->rsMode ? gsSkin : gsGrid;
+        const int nGs = net->rsMode ? 5 : 3;
         int bestG = net->mbGroup;
         float bestGm = -1.0f;
         hipEvent_t g0 = nullptr, g1 = nullptr;
         if ((r = check_hip(c, hipEventCreate(&g0), "tune event"))) return r;
         if ((r = check_hip(c, hipEventCreate(&g1), "tune event"))) { hipEventDestroy(g0); return r; }
-        for (int gi = 0; gi < 3 && !r; gi++) {
+        for (int gi = 0; gi < nGs && !r; gi++) {
             if ((r = sbx_net_set_motor_group(net, gs[gi]))) break;
             if ((r = sbx_net_step(net, t, 2))) break;
             t += 2;

@@ -61,6 +61,28 @@ class Network:
                  "skin_steps", "block_sorted")
         return {names[i]: int(out[i]) for i in range(min(k, len(names)))}
 
+    def set_reach_skin(self, mode: int, cfg=None, model=None) -> None:
+        """Binding-grid search of the myosin stage: SBX_SKIN_REBUILD (the reference's per-step grid),
+        SBX_SKIN_FIXED or SBX_SKIN_LEARNED (with a softbox_amd.skin.SkinModel) -- the reach skin."""
+        from .skin import ReachSkinConfig
+
+        cfg = cfg or ReachSkinConfig()
+        self._reach_p = cfg.params()
+        w = ptr(model.w) if model is not None else None
+        self.ctx.check(self.ctx.lib.sbx_net_set_reach_skin(self.h, mode, C.byref(self._reach_p), w))
+        self._reach_mode = (mode, cfg, model)
+
+    def reach_mode(self) -> tuple:
+        """(mode, config, model) of the last set_reach_skin (mode 0: the per-step binding grid)."""
+        return getattr(self, "_reach_mode", (0, None, None))
+
+    def reach_skin_stats(self) -> dict:
+        out = (C.c_longlong * 9)()
+        k = self.ctx.check(self.ctx.lib.sbx_net_reach_skin_stats(self.h, out, 9))
+        names = ("rebuilds", "outside_skin", "brute_steps", "selected_heads", "cell_nm", "rebuilds_stale",
+                 "rebuilds_horizon", "rebuilds_outside", "steps")
+        return {names[i]: int(out[i]) for i in range(min(k, len(names)))}
+
     def form_stats(self) -> dict:
         """Formation counters: passes, candidates and links formed by the last pass, links formed in all."""
         out = (C.c_longlong * 4)()
@@ -123,7 +145,8 @@ class Network:
         return out[:n].reshape(-1, 3)
 
     def set_motor_group(self, lanes: int) -> None:
-        """Lanes per motor of the fused reach search + binding (16, 32 or 64; results do not depend on it)."""
+        """Lanes per motor of the fused reach search + binding (4, 8, 16, 32 or 64; the per-step grid walk takes at
+        least 16; results do not depend on it)."""
         self.ctx.check(self.ctx.lib.sbx_net_set_motor_group(self.h, lanes))
 
     def motor_group(self) -> int:

@@ -39,6 +39,25 @@ class SkinConfig:
                                   self.block_keys)
 
 
+@dataclass
+class ReachSkinConfig:
+    """Parameters of the reach skin (sbx_net_set_reach_skin): the binding grid's search from a coarse grid of
+    the segments rebuilt between steps."""
+    skin_default: float = 0.04    # um: fixed mode, and the first build of learned mode
+    skin_min: float = 0.005
+    skin_max: float = 0.3
+    margin: float = 1.0           # learned skin = margin * exp(out0)
+    cell_floor: float = 0.0       # smallest coarse cell, um (0: the binding grid's cell)
+    horizon: int = 0              # steps between scheduled rebuilds (0: none)
+    extra_trigger: int = 64       # outside-skin segments (lagged) that schedule a rebuild
+    extra_cap: int = 4096         # capacity of the outside-skin list
+    coverage: float = 0.999       # learned: the margin covers this quantile of the cells' largest displacement
+
+    def params(self) -> _lib.SbxReachSkinParams:
+        return _lib.SbxReachSkinParams(self.skin_default, self.skin_min, self.skin_max, self.margin,
+                                       self.cell_floor, self.horizon, self.extra_trigger, self.extra_cap)
+
+
 class SkinModel:
     """Device parameters of the 12 -> 64 -> 64 -> 2 GELU MLP (layout in softbox_amd/csrc/sbx_mlp.h)."""
 
@@ -135,7 +154,9 @@ class TrajectorySampler:
 
     SLACK = 1.0e-3  # um, as the library's list slack
 
-    def __init__(self, ctx: Context, net, cfg: SkinConfig):
+    def __init__(self, ctx: Context, net, cfg: SkinConfig, cell_fn=None):
+        """cell_fn(skin) -> the coarse cell a rebuild with that largest skin uses (default: the filament skin's
+        rebuild grid; reach_cell_fn gives the reach skin's)."""
         import torch
 
         self.ctx, self.net, self.cfg = ctx, net, cfg
@@ -149,6 +170,8 @@ class TrajectorySampler:
         self.fine_dims = [int(v) for v in gd[:3]]
         self.box = float(p["boxLen"][0]) if self.periodic else None
         self.max_r = float((0.5 * sc.fs.segLength + p["actinRadius"]).max())
+        if cell_fn is not None:
+            self._cell = cell_fn
         # coarse cells span what the learned mode can build with: skins in [skin_min, skin_max]
         self.cell_lo = self._cell(cfg.skin_min)
         self.cell_hi = self._cell(cfg.skin_max)
@@ -262,23 +285,24 @@ class TrajectorySampler:
         return torch.cat(xs).contiguous(), torch.cat(ys).contiguous(), torch.cat(ms).contiguous(), t
 
 
-def calibrate_margin(ctx: Context, model: SkinModel, x, m, cells_per_snapshot: float) -> float:
-    """Skin margin from the training samples: the guard rebuilds as soon as ANY body outruns its
-    skin, so margin * exp(out0) must cover each cell's LARGEST member displacement m, over all cells
-    of a rebuild: the (1 - 1/cells) quantile of m / exp(out0) (conformal calibration)."""
+def calibrate_margin(ctx: Context, model: SkinModel, x, m, cells_per_snapshot: float, coverage=None) -> float:
+    """Skin margin from the training samples: the filament skin's guard rebuilds as soon as ANY body
+    outruns its skin, so margin * exp(out0) must cover each cell's LARGEST member displacement m, over all
+    cells of a rebuild: the (1 - 1/cells) quantile of m / exp(out0) (conformal calibration).  `coverage`
+    replaces that quantile (the reach skin keeps the few segments that outrun their skins on a list)."""
     pred = ctx.to_host(model.forward(x))[:, 0]
     ratio = ctx.to_host(m) / np.exp(pred)
-    q = 1.0 - 1.0 / max(2.0, cells_per_snapshot)
+    q = coverage if coverage is not None else 1.0 - 1.0 / max(2.0, cells_per_snapshot)
     return float(max(1.0, np.quantile(ratio, q)))
 
 
 def fit(ctx: Context, net, cfg: SkinConfig, t0: int = 0, snapshots: int = 16, iters: int = 400, seed: int = 0,
-        log_every: int = 0):
+        log_every: int = 0, cell_fn=None, coverage=None):
     """Collect a trajectory of `net` (per-step rebuild) and train a skin model on it; sets
     cfg.margin by calibration; returns (model, loss history, samples, next step)."""
     import torch
 
-    sampler = TrajectorySampler(ctx, net, cfg)
+    sampler = TrajectorySampler(ctx, net, cfg, cell_fn)
     x, y, m, t = sampler.collect(t0, snapshots)
     xh = ctx.to_host(x)
     mean = xh.mean(0)
@@ -286,5 +310,29 @@ def fit(ctx: Context, net, cfg: SkinConfig, t0: int = 0, snapshots: int = 16, it
     model = SkinModel.initial(ctx, mean.astype(np.float32), std.astype(np.float32), seed)
     torch.cuda.synchronize()
     hist = train(ctx, model, x, y, iters=iters, seed=seed, log_every=log_every)
-    cfg.margin = calibrate_margin(ctx, model, x, m, sampler.cells_per_snapshot)
+    cfg.margin = calibrate_margin(ctx, model, x, m, sampler.cells_per_snapshot, coverage)
     return model, hist, (x, y), t
+
+
+def reach_cell_fn(sc, cfg: ReachSkinConfig):
+    """The reach skin's coarse cell for a largest skin (sbx_network.hip k_rs_geom): the reach of the longest
+    segment (0.5 L + myoColTol) + the skin + 2 slack, at least the floor (the binding grid's cell by default)."""
+    r0 = float(0.5 * sc.fs.segLength.max()) + float(sc.mot.kinParams[7])
+    floor = cfg.cell_floor if cfg.cell_floor > 0 else float(sc.p["gridParams"][3])
+    return lambda skin: max(floor, r0 + skin + 2.0e-3)
+
+
+def fit_reach(ctx: Context, net, cfg: ReachSkinConfig, t0: int = 0, snapshots: int = 12, iters: int = 400,
+              seed: int = 0, horizon: int = 0, log_every: int = 0):
+    """The skin model of the reach skin, trained like the filament skin's on a per-step-rebuild trajectory of
+    the actomyosin network `net` (per-segment motion in the reach grid's coarse cells).  The horizon is
+    cfg.horizon (or `horizon`, or 50 steps): the run rebuilds that often and the model sizes skins to last it.
+    Sets cfg.margin and cfg.horizon; returns (model, loss history, samples, next step)."""
+    H = horizon or cfg.horizon or 50
+    scfg = SkinConfig(skin_default=cfg.skin_default, skin_min=cfg.skin_min, skin_max=cfg.skin_max, horizon=H,
+                      window=4 * H)
+    model, hist, samples, t = fit(ctx, net, scfg, t0=t0, snapshots=snapshots, iters=iters, seed=seed,
+                                  log_every=log_every, cell_fn=reach_cell_fn(net.scene, cfg), coverage=cfg.coverage)
+    cfg.margin = scfg.margin
+    cfg.horizon = H
+    return model, hist, samples, t

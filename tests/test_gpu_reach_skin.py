@@ -1,0 +1,217 @@
+"""The reach skin (sbx_net_set_reach_skin): the myosin stage's binding-grid search from a coarse grid of the
+filament segments that is rebuilt only between steps, instead of BindingDetectionSystem.gridReachable's
+grid over every segment and head sorted every step (BindingDetectionSystem.java:143-228).
+
+The per-step rebuild is the judge: after every step (every graph replay) the reach lists (reachCount and
+the first maxCand entries of reachSeg, in the reference's stencil order), every binding event (boundSeg,
+bindArc, cooldown, the catch-slip tallies) and the whole state that follows from them must equal the
+per-step rebuild's bit for bit, and the oracle's.  The cases drive every path of the search: skins that
+outlast the run, skins so small that segments leave them every step (the outside-skin list), a list
+that overflows (heads scan every segment), scheduled rebuilds, more hits than maxCand and more than a lane
+holds (the selection path), captured steps, narrow and wide lane groups, and turnover (rebuilds after
+every cadence).
+"""
+import numpy as np
+import pytest
+
+from _common import assert_same
+from oracle import ffi
+from softbox_amd import scenes
+from softbox_amd.skin import ReachSkinConfig
+from softbox_amd.systems import RNG_PHILOX, RNG_WANG
+
+pytestmark = pytest.mark.gpu
+
+FIXED, LEARNED = 1, 2
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from softbox_amd.context import Context
+
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _scene(seed=3, rng=RNG_PHILOX, n_fil=60, box=3.0, n_mini=12, eta=1.0e-3, max_cand=None, reach=None):
+    sc = scenes.contractile(n_fil=n_fil, segs=10, box=box, n_mini=n_mini, dimers_each_end=4, n_links=150, seed=seed,
+                            mini_near_fil=True, rng=rng, eta=eta)
+    if max_cand is not None:
+        sc.p["maxCand"] = max_cand
+    if reach is not None:  # a wide reach (myoColTol) and no alignment gate: many hits per head
+        sc.mot.kinParams[7] = reach
+        sc.mot.kinParams[8] = -1.0
+    return sc
+
+
+MOTOR_KEYS = ("boundSeg", "bindArc", "cooldown", "stats", "capStats", "nucleotideState", "forceDotFil", "head")
+
+
+def _state(net, sc):
+    fs, xl = sc.fs.copy(), sc.xl.copy()
+    net.pull(fs, xl)
+    mot = sc.mot.copy()
+    net.pull_motors(mot, None)
+    mc = int(sc.p["maxCand"])
+    rc = net.ctx.to_host(net.mot.t["reachCount"]).copy()
+    rs = net.ctx.to_host(net.mot.t["reachSeg"]).reshape(-1, mc).copy()
+    return fs, xl, mot, rc, rs
+
+
+def _compare(a, b, mc, what):
+    fa, xa, ma, rca, rsa = a
+    fb, xb, mb, rcb, rsb = b
+    assert_same(rca, rcb, f"{what}: reachCount")
+    for m in np.flatnonzero(rcb):
+        c = min(int(rcb[m]), mc)
+        assert_same(rsa[m, :c], rsb[m, :c], f"{what}: reachSeg[{m}]")
+    for k in MOTOR_KEYS:
+        assert_same(getattr(ma, k), getattr(mb, k), f"{what}: motor {k}")
+    assert_same(ma.body.coord, mb.body.coord, f"{what}: motor coord")
+    for k in ("coord", "uVec", "end1", "end2", "forceSum"):
+        assert_same(getattr(fa, k), getattr(fb, k), f"{what}: filament {k}")
+    assert_same(xa.linkState, xb.linkState, f"{what}: linkState")
+
+
+def _run_pair(ctx, make, cfg, mode=FIXED, graph=False, group=16, chunks=(1,) * 8 + (2, 3, 2, 5, 20), model=None):
+    """Step a per-step-rebuild network and a reach-skin network of the same scene side by side; compare after
+    every chunk (pulling the state publishes the pose, after which the caller may edit it: the next chunk
+    starts with a rebuild).  Returns the skin network's counters, the total hits seen and the bound heads."""
+    from softbox_amd.network import Network
+
+    sc_r, sc_s = make(), make()
+    ref, net = Network(ctx, sc_r), Network(ctx, sc_s)
+    net.set_reach_skin(mode, cfg, model)
+    net.set_motor_group(group)
+    ref.set_motor_group(max(16, group))
+    if graph:
+        ref.use_graph(2)
+        net.use_graph(2)
+    mc = int(sc_r.p["maxCand"])
+    t, hits = 0, 0
+    for ch in chunks:
+        ref.step(t, ch)
+        net.step(t, ch)
+        t += ch
+        a, b = _state(net, sc_s), _state(ref, sc_r)
+        _compare(a, b, mc, f"after {t} steps")
+        hits += int(b[3].sum())
+    stats = net.reach_skin_stats()
+    bound = int((b[2].boundSeg >= 0).sum())
+    ref.close()
+    net.close()
+    return stats, hits, bound
+
+
+@pytest.mark.parametrize("graph,group", [(False, 4), (False, 16), (True, 8), (True, 64)])
+def test_reach_skin_equals_rebuild_every_step(ctx, graph, group):
+    """Skins that outlast the run (1 Pa s): one rebuild, every step's lists and binding events equal the
+    rebuild's."""
+    stats, hits, bound = _run_pair(ctx, lambda: _scene(seed=3, eta=1.0), ReachSkinConfig(skin_default=0.05),
+                                   graph=graph, group=group)
+    assert hits > 0 and bound > 0, "the scene produced no reach hits or bindings"
+    assert stats["rebuilds"] == stats["rebuilds_stale"] == 13, stats
+    assert stats["steps"] == 40, stats
+
+
+@pytest.mark.parametrize("rng", [RNG_PHILOX, RNG_WANG])
+def test_segments_outside_their_skins_are_still_found(ctx, rng):
+    """Skins of 0.1 nm on a coarse grid of minimal cells: segments leave their skins every step and are
+    found through the outside-skin list (no rebuild is scheduled)."""
+    cfg = ReachSkinConfig(skin_default=1.0e-4, cell_floor=1.0e-3, extra_trigger=1 << 30, extra_cap=1 << 16)
+    stats, hits, _ = _run_pair(ctx, lambda: _scene(seed=5, rng=rng), cfg)
+    assert hits > 0
+    assert stats["rebuilds"] == 13 and stats["outside_skin"] > 100 and stats["brute_steps"] == 0, stats
+
+
+def test_outside_list_overflow_scans_every_segment(ctx):
+    """An outside-skin list of 2 entries overflows at once: heads scan every segment, still exact."""
+    cfg = ReachSkinConfig(skin_default=1.0e-4, cell_floor=1.0e-3, extra_trigger=1 << 30, extra_cap=2)
+    stats, hits, _ = _run_pair(ctx, lambda: _scene(seed=6), cfg, graph=True, group=16)
+    assert hits > 0 and stats["brute_steps"] > 10, stats
+
+
+@pytest.mark.parametrize("horizon,trigger", [(3, 1 << 30), (0, 4)])
+def test_scheduled_rebuilds(ctx, horizon, trigger):
+    """Rebuilds every 3 steps, or whenever more than 4 segments left their skins (read by the host a step or
+    a replay late): every rebuild keeps the lists exact."""
+    cfg = ReachSkinConfig(skin_default=2.0e-3, cell_floor=1.0e-3, horizon=horizon, extra_trigger=trigger)
+    chunks = (1,) * 8 + (2, 3, 2, 5, 20) if horizon else (1, 1, 2, 40, 40)
+    stats, hits, _ = _run_pair(ctx, lambda: _scene(seed=7), cfg, graph=horizon == 0, group=8, chunks=chunks)
+    assert hits > 0
+    if horizon:
+        assert stats["rebuilds_horizon"] >= 7, stats
+    else:  # the host waits for the replay 4 back, so a 20-replay call sees the count a few times
+        assert stats["rebuilds_outside"] >= 4, stats
+
+
+@pytest.mark.parametrize("group,max_cand,reach", [(4, 2, 0.2), (16, 3, 0.2), (64, 1, 0.2), (4, 256, 0.5)])
+def test_more_hits_than_max_cand_or_a_lane_holds(ctx, group, max_cand, reach):
+    """Crowded heads (200 filaments in a 1.2 um box at 1 Pa s, a reach of 0.2-0.5 um with no alignment gate,
+    up to 19 hits per head, some outside the binding-grid stencil): reach lists longer than maxCand are cut
+    to the first maxCand in the reference's order, and heads whose hits do not fit the lanes' registers go
+    through the selection path; both equal the per-step rebuild."""
+    make = lambda: _scene(seed=9, n_fil=200, box=1.2, n_mini=6, eta=1.0, max_cand=max_cand, reach=reach)  # noqa
+    cfg = ReachSkinConfig(skin_default=0.05)
+    stats, hits, _ = _run_pair(ctx, make, cfg, group=group, chunks=(1,) * 6 + (2, 3))
+    assert hits > 0 and stats["selected_heads"] > 0, stats
+
+
+def test_reach_skin_matches_oracle_with_turnover(ctx):
+    """Actomyosin with turnover every 4 steps (growth, depolymerisation, severing, dead-slot reuse): the state
+    after every chunk equals the oracle's; each cadence schedules a rebuild."""
+    from softbox_amd.network import Network
+    from softbox_amd.turnover import TurnoverStore
+
+    sc = scenes.contractile(n_fil=60, segs=10, box=3.0, n_mini=12, dimers_each_end=4, n_links=150, seed=13,
+                            mini_near_fil=True, eta=1.0)
+    sc = scenes.with_turnover(sc, spare=120, growth_rate=10.0, speedup=150.0, cofilin_ratio=0.35, interval=4)
+    orc = ffi.OracleContractile(sc)
+    net = Network(ctx, sc)
+    net.set_reach_skin(FIXED, ReachSkinConfig(skin_default=0.01))
+    net.use_graph(2)
+    t = 0
+    mc = int(sc.p["maxCand"])
+    for chunk in (5, 11, 16, 20):
+        orc.step(t, chunk)
+        net.step(t, chunk)
+        t += chunk
+        fs = sc.fs.copy()
+        net.pull(fs, sc.xl.copy(), shape=True)
+        for k in ("coord", "uVec", "end1", "end2", "forceSum", "torqueSum") + net.SHAPE:
+            assert_same(getattr(fs, k), getattr(orc.fs, k), f"after {t}: {k}")
+        ts = net.pull_turnover(TurnoverStore(sc.fs.n), fs)
+        assert_same(ts.filState, orc.turn.ts.filState, f"after {t}: filState")
+        mot = sc.mot.copy()
+        net.pull_motors(mot, None)
+        for k in MOTOR_KEYS:
+            assert_same(getattr(mot, k), getattr(orc.mot, k), f"after {t}: motor {k}")
+        rc = net.ctx.to_host(net.mot.t["reachCount"])
+        assert_same(rc, orc.mreach["reachCount"], f"after {t}: reachCount")
+        rs = net.ctx.to_host(net.mot.t["reachSeg"]).reshape(-1, mc)
+        want = orc.mreach["reachSeg"].reshape(-1, mc)
+        for m in np.flatnonzero(rc):
+            assert_same(rs[m, : min(int(rc[m]), mc)], want[m, : min(int(rc[m]), mc)], f"after {t}: reachSeg[{m}]")
+    stats = net.reach_skin_stats()
+    assert stats["rebuilds"] >= 13, stats  # the first step and each of the 13 cadences after it
+    assert orc.turn.ledger[0] > 0 and orc.turn.ledger[1] > 0
+    net.close()
+
+
+def test_switching_the_reach_skin_off_and_on(ctx):
+    """Off (per-step grid) -> on -> off -> on within one run: each form continues the same trajectory."""
+    from softbox_amd.network import Network
+
+    sc_r, sc_s = _scene(seed=11), _scene(seed=11)
+    ref, net = Network(ctx, sc_r), Network(ctx, sc_s)
+    mc = int(sc_r.p["maxCand"])
+    t = 0
+    for k, mode in enumerate((FIXED, 0, FIXED, 0, FIXED)):
+        net.set_reach_skin(mode, ReachSkinConfig(skin_default=0.02))
+        ref.step(t, 4)
+        net.step(t, 4)
+        t += 4
+        _compare(_state(net, sc_s), _state(ref, sc_r), mc, f"switch {k} after {t}")
+    ref.close()
+    net.close()
