@@ -80,7 +80,9 @@ struct sbx_net {
     sbx_motors mo{};
     sbx_motor_params mp{};
     int *ownerStore = nullptr, *ownerSlot = nullptr;   // body view back-pointers over cap = n + nM
-    int *mOff = nullptr, *mList = nullptr;             // segment -> bound motors CSR (rebuilt every step)
+    // segment -> bound heads of this step (k_seg_heads): count and up to SEG_HEADS heads per segment; k_rod reads
+    // and clears them (CrossBridgeSystem.csrBuild's inverse, CrossBridgeSystem.java:189-298, without the sort)
+    int *segHeadCnt = nullptr, *segHeads = nullptr;
     int *bbOff = nullptr, *bbList = nullptr;           // backbone -> tethered dimers CSR (static)
     KinParams kp{};
     NucParams np{};
@@ -204,6 +206,53 @@ const char *kTimerNames[T_COUNT_TIMERS] = {"k_body_cell", "grid_sort",     "k_br
                                            "k_bond_head", "k_motor_integ", "motor_csr", "skin_rebuild",
                                            "k_skin_pairs", "reach_rebuild"};
 
+// The bound heads of each segment for this step's motor gather (CrossBridgeSystem.csrBuild, CrossBridgeSystem.java:
+// 189-298, builds a CSR by a counting sort over every segment; few heads are bound, so each bound head takes a
+// slot of its segment instead and k_rod orders them).
+constexpr int SEG_HEADS = 8;
+
+__global__ void __launch_bounds__(256) k_seg_heads(const int *__restrict__ boundSeg, int nM, int n,
+                                                   int *__restrict__ segHeadCnt, int *__restrict__ segHeads) {
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= nM) return;
+    const int s = boundSeg[m];
+    if (s < 0 || s >= n) return;
+    const int slot = atomicAdd(&segHeadCnt[s], 1);
+    if (slot < SEG_HEADS) segHeads[(size_t)s * SEG_HEADS + slot] = m;
+}
+
+// CrossBridgeSystem.segGather for segment s (CrossBridgeSystem.java:302-320): the bound heads' bond forces and
+// torques summed in ascending head order (the CSR's order: a stable sort keeps ascending index within a key),
+// then the segment's count cleared for the next step.  More than SEG_HEADS heads: every head is scanned.
+__device__ __forceinline__ void seg_heads_sum(int s, int *__restrict__ segHeadCnt, const int *__restrict__ segHeads,
+                                              const int *__restrict__ boundSeg, int nM,
+                                              const float *__restrict__ data, double f[3], double t[3]) {
+    const int cnt = segHeadCnt[s];
+    if (cnt == 0) return;
+    auto add = [&](int m) {
+        const float *d = data + (size_t)m * XB_STRIDE + 6;
+        f[0] += d[0]; f[1] += d[1]; f[2] += d[2];
+        t[0] += d[3]; t[1] += d[4]; t[2] += d[5];
+    };
+    if (cnt <= SEG_HEADS) {
+        const int *h = segHeads + (size_t)s * SEG_HEADS;
+        int prev = -1;
+        for (int k = 0; k < cnt; k++) {  // the next larger head: a selection over the (few) slots
+            int next = 0x7fffffff;
+            for (int q = 0; q < cnt; q++) {
+                const int m = h[q];
+                if (m > prev && m < next) next = m;
+            }
+            add(next);
+            prev = next;
+        }
+    } else {
+        for (int m = 0; m < nM; m++)
+            if (boundSeg[m] == s) add(m);
+    }
+    segHeadCnt[s] = 0;
+}
+
 // k_rod launch form selected by rodThreads == 512: 256-thread blocks, two rod groups per block, two waves
 // drawing the noise of the other two (k_rod<*, true>)
 constexpr int RODS_PAIRED = 512;
@@ -216,7 +265,10 @@ struct RodArgs {
     const int *e1Slot, *e1Side, *e2Slot, *e2Side;
     const int *offA, *idxA, *offB, *idxB, *linkState;
     const float *xlinkData;
-    const int *mOff, *mList;
+    int *segHeadCnt;           // bound heads per segment this step (k_rod clears what it reads)
+    const int *segHeads;       // up to SEG_HEADS of them per segment, in any order
+    const int *boundSeg;       // every head's segment (a segment with more than SEG_HEADS heads scans them)
+    int nM;
     const float *bondData;
     int hasMotors;
     int *activeLinkCount;   // written from the gather's active-link tally (may be null)
@@ -355,7 +407,7 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
             }
             if (a.hasMotors) {  // CrossBridgeSystem.segGather (CrossBridgeSystem.java:302-320)
                 double fm[3] = {0, 0, 0}, tm[3] = {0, 0, 0};
-                csr_payload_sum(i, a.mOff, a.mList, a.bondData, XB_STRIDE, 6, fm, tm);
+                seg_heads_sum(i, a.segHeadCnt, a.segHeads, a.boundSeg, a.nM, a.bondData, fm, tm);
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     f[k] = (float)(f[k] + fm[k]);
@@ -716,13 +768,15 @@ __global__ void __launch_bounds__(256) k_motor_bind(MotorArgs a) {
 // (distance, key) of the first maxCand of them.
 constexpr int RS_EXTRA = 0;      // segments outside their skins since the last rebuild (> extraCap: overflow)
 constexpr int RS_REBUILDS = 1;   // rebuilds so far
-constexpr int RS_MAXSKIN = 2;    // largest learned skin for the rebuild being built (float bits)
+constexpr int RS_MAXSKIN = 2;    // learned: 1 once the model has run for the rebuild being built
 constexpr int RS_GEN = 3;        // generation of the last rebuild (the host's count)
 constexpr int RS_BRUTE = 4;      // steps whose heads scanned every segment (outside-skin list overflow)
 constexpr int RS_SELECT = 5;     // heads placed by the selection path
 constexpr int RS_GEOM = 8;       // 8 words: xMin, yMin, zMin, cell, invCell (float bits), nX, nY, nZ
-constexpr int RS_WORDS = 16;
-constexpr int RS_PER_LANE = 4;   // hits a lane holds in registers (more on any lane: the selection path)
+constexpr int RS_HIST = 16;      // learned mode: segments per predicted-skin bin (log bins over [skinMin, skinMax])
+constexpr int RS_HBINS = 64;
+constexpr int RS_WORDS = RS_HIST + RS_HBINS;
+constexpr int RS_PER_LANE = 2;   // hits a lane holds in registers (more on any lane: the selection path)
 constexpr int RS_KEY_BITS = 26;
 constexpr unsigned RS_KEY_MASK = (1u << RS_KEY_BITS) - 1u;
 constexpr int RS_MODEL_CELLS = 65536;  // coarse cells the model evaluates per rebuild (at most about)
@@ -840,7 +894,7 @@ __global__ void __launch_bounds__(256) k_motor_bind_skin(MotorArgs a, ReachArgs 
     if (!spill && cnt <= a.maxCand) {
         if (cnt > 0) {
             // a hit's place = the group's hits with a smaller key (keys are distinct)
-            int rank[RS_PER_LANE] = {0, 0, 0, 0};
+            int rank[RS_PER_LANE] = {};
             const int gbase = grp * G;
             for (int j = 0; j < G; j++) {
                 const int nj = __shfl(nh, gbase + j, 64);
@@ -913,6 +967,7 @@ __global__ void __launch_bounds__(256) k_motor_bind_skin(MotorArgs a, ReachArgs 
 // A rebuild (enqueued by the host between steps): [learned: per-cell model skins] -> geometry -> bin -> sort.
 struct ReachBuild {
     int n, extraCap, learned, gen, sampleStride;
+    long long budget;                // learned: segments the cell may leave outside their predicted skins
     float since;                     // steps since the previous build (learned features)
     const float *coord, *uVec, *segLength;
     const int *links;                // activeLinkCount (learned features; may be null)
@@ -926,12 +981,13 @@ struct ReachBuild {
 };
 
 // Learned mode: the model's skin for occupied coarse cells of the previous build (features of their members'
-// motion since that build); the largest goes to RS_MAXSKIN.  A thread per segment evaluates the cell it leads
-// (first in the cell's contents) when the segment index is a multiple of sampleStride, so a rebuild evaluates
-// at most n / sampleStride cells whatever the grid's size.
+// motion since that build), tallied per skin bin with the cell's member count.  Thread t evaluates the cell
+// that segment t * sampleStride leads (is first in), so a rebuild evaluates at most about n / sampleStride
+// cells whatever the grid's size; the tallies estimate every segment's predicted skin.
 __global__ void __launch_bounds__(256) k_rs_assign(ReachBuild b) {
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= b.n || s % b.sampleStride != 0) return;
+    const long long sl = (long long)(blockIdx.x * 256 + threadIdx.x) * b.sampleStride;
+    if (sl >= b.n) return;
+    const int s = (int)sl;
     const int c = b.cell[s];
     if (c < 0 || b.cont[b.off[c]] != s) return;
     GridGeom g{};  // raw displacements, as the guard takes them
@@ -940,19 +996,31 @@ __global__ void __launch_bounds__(256) k_rs_assign(ReachBuild b) {
                        __int_as_float(b.words[RS_GEOM + 3]), g, f);
     mlp_forward(b.model, f, y);
     const float sk = fminf(b.skinMax, fmaxf(b.skinMin, b.margin * expf(y[0])));
-    atomicMax(&b.words[RS_MAXSKIN], __float_as_int(sk));  // positive floats order as ints
+    const float u = logf(sk / b.skinMin) / logf(b.skinMax / b.skinMin);  // [0, 1]
+    const int bin = min(RS_HBINS - 1, max(0, (int)(u * (float)RS_HBINS)));
+    atomicAdd(&b.words[RS_HIST + bin], b.off[c + 1] - b.off[c]);
+    atomicMax(&b.words[RS_MAXSKIN], 1);  // the model ran
 }
 
-// One thread: this build's coarse grid (cell = the reach of the longest segment + the largest skin, at least
-// the floor) over the binding grid's extent; opens the outside-skin list.
+// One thread: this build's coarse grid (cell = the reach of the longest segment + the skin, at least the floor)
+// over the binding grid's extent; opens the outside-skin list.  Learned mode: the skin is the smallest bin edge
+// that the model predicts all but about `budget` segments (scaled from the sampled cells) to stay within over
+// the horizon; those few will leave their skins and be found on the outside-skin list.
 __global__ void k_rs_geom(ReachBuild b) {
     if (threadIdx.x != 0) return;
     float maxSkin = b.skinDefault;
-    if (b.learned) {
-        const float ms = __int_as_float(b.words[RS_MAXSKIN]);
-        if (ms > 0.0f) maxSkin = ms;
+    if (b.learned && b.words[RS_MAXSKIN] > 0) {
+        long long above = 0;
+        int bin = RS_HBINS - 1;
+        for (; bin > 0; bin--) {  // from the top: stop where the segments above would pass the budget
+            above += (long long)b.words[RS_HIST + bin] * b.sampleStride;
+            if (above > b.budget) break;
+        }
+        maxSkin = b.skinMin * powf(b.skinMax / b.skinMin, (float)(bin + 1) / (float)RS_HBINS);
+        maxSkin = fminf(b.skinMax, maxSkin);
     }
     b.words[RS_MAXSKIN] = 0;
+    for (int k = 0; k < RS_HBINS; k++) b.words[RS_HIST + k] = 0;
     const float cell = fmaxf(b.floorCell, b.R0 + maxSkin + 2.0f * RS_SLACK);
     int nX = max(1, (int)ceilf(b.spanX / cell)), nY = max(1, (int)ceilf(b.spanY / cell));
     int nZ = max(1, (int)ceilf(b.spanZ / cell));
@@ -1691,6 +1759,7 @@ int rs_rebuild(sbx_net *net, int why) {
     ReachBuild b{};
     b.n = n; b.extraCap = net->rsp.extraCap;
     b.sampleStride = n > RS_MODEL_CELLS ? (n + RS_MODEL_CELLS - 1) / RS_MODEL_CELLS : 1;
+    b.budget = net->rsp.extraTrigger / 2;
     b.learned = net->rsMode == SBX_SKIN_LEARNED && net->rsGen > 0 && net->rsModel;
     b.gen = net->rsGen + 1;
     b.since = (float)(net->rsSince > 0 ? net->rsSince : 1);
@@ -1709,7 +1778,9 @@ int rs_rebuild(sbx_net *net, int why) {
     int r;
     {
         Scope sc(net, T_RS_REBUILD);
-        if (b.learned) hipLaunchKernelGGL(k_rs_assign, dim3(blocks_for(n, 256)), dim3(256), 0, s, b);
+        if (b.learned)
+            hipLaunchKernelGGL(k_rs_assign, dim3(blocks_for((n + b.sampleStride - 1) / b.sampleStride, 256)),
+                               dim3(256), 0, s, b);
         hipLaunchKernelGGL(k_rs_geom, dim3(1), dim3(64), 0, s, b);
         hipLaunchKernelGGL(k_rs_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, b);
         if ((r = check_launch(c, "reach skin rebuild"))) return r;
@@ -2096,11 +2167,13 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
     }
     a.bodyPack = (net->p.gridOn && !net->hasMotors) ? net->bodyPack[nxt] : nullptr;
     a.boundR = net->boundR;
-    a.mOff = net->mOff; a.mList = net->mList; a.bondData = net->mo.bondData;
+    a.segHeadCnt = net->segHeadCnt; a.segHeads = net->segHeads; a.boundSeg = net->mo.boundSeg; a.nM = net->mo.nM;
+    a.bondData = net->mo.bondData;
     if (net->hasMotors) {
         Scope sc(net, T_MCSR);
-        if ((r = counting_sort(c, net->sort, net->mo.boundSeg, net->mo.nM, n, net->mo.nM, net->mOff, net->mList)))
-            return r;
+        hipLaunchKernelGGL(k_seg_heads, dim3(blocks_for(net->mo.nM, 256)), dim3(256), 0, s, net->mo.boundSeg,
+                           net->mo.nM, n, net->segHeadCnt, net->segHeads);
+        if ((r = check_launch(c, "k_seg_heads"))) return r;
     }
     a.dStep = net->dStep; a.stepOff = stepOff;
     a.n = n; a.chainOn = net->p.chainOn; a.hasLinks = L > 0;
@@ -2166,7 +2239,7 @@ void release(sbx_net *net) {
                     net->cellOffsets,
                     net->candPartner, net->candCount, net->offA,  net->idxA,        net->offB,
                     net->idxB,     net->dStep,     net->ownerStore, net->ownerSlot,
-                    net->mOff,     net->mList,     net->bbOff,    net->bbList,
+                    net->segHeadCnt, net->segHeads, net->bbOff,  net->bbList,
                     net->cCell,    net->cOff,      net->cCont,    net->refC,    net->refU, net->skin,
                     net->cellSkin, net->cellHorizon, net->vlist,  net->vcount,  net->skinFlags, net->model,
                     net->cgDev,    net->cCellsDev, net->bodyPack[0], net->bodyPack[1], net->sortedPose[0],
@@ -2508,15 +2581,16 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
     if (r) return r;
     const int cap = n + nM;
     // the grid now bins filaments + heads: regrow the per-body grid arrays
-    int *bodyCell = nullptr, *cellContents = nullptr, *ownerStore = nullptr, *ownerSlot = nullptr, *mOff = nullptr,
-        *mList = nullptr, *bbOff = nullptr, *bbList = nullptr;
+    int *bodyCell = nullptr, *cellContents = nullptr, *ownerStore = nullptr, *ownerSlot = nullptr,
+        *segHeadCnt = nullptr, *segHeads = nullptr, *bbOff = nullptr, *bbList = nullptr;
     do {
         if ((r = alloc(c, &bodyCell, cap))) break;
         if ((r = alloc(c, &cellContents, cap))) break;
         if ((r = alloc(c, &ownerStore, cap))) break;
         if ((r = alloc(c, &ownerSlot, cap))) break;
-        if ((r = alloc(c, &mOff, (size_t)n + 1))) break;
-        if ((r = alloc(c, &mList, nM))) break;
+        if ((r = alloc(c, &segHeadCnt, (size_t)n))) break;
+        if ((r = alloc(c, &segHeads, (size_t)n * SEG_HEADS))) break;
+        if ((r = fill_i32(c, segHeadCnt, 0, n))) break;  // k_rod keeps it zero between steps
         if ((r = alloc(c, &bbOff, (size_t)m->nBb + 1))) break;
         if ((r = alloc(c, &bbList, m->nMD))) break;
         long long keys = net->p.gridDims[3];
@@ -2525,7 +2599,7 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
         if ((r = sort_scratch_reserve(c, net->sort, keys, cap))) break;
     } while (0);
     if (r) {
-        void *bufs[] = {bodyCell, cellContents, ownerStore, ownerSlot, mOff, mList, bbOff, bbList};
+        void *bufs[] = {bodyCell, cellContents, ownerStore, ownerSlot, segHeadCnt, segHeads, bbOff, bbList};
         for (void *b : bufs)
             if (b) hipFree(b);
         return r;
@@ -2539,8 +2613,8 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
     net->csrCont[0] = cellContents;
     net->ownerStore = ownerStore;
     net->ownerSlot = ownerSlot;
-    net->mOff = mOff;
-    net->mList = mList;
+    net->segHeadCnt = segHeadCnt;
+    net->segHeads = segHeads;
     net->bbOff = bbOff;
     net->bbList = bbList;
     net->mo = *m;

@@ -215,3 +215,21 @@ def test_switching_the_reach_skin_off_and_on(ctx):
         _compare(_state(net, sc_s), _state(ref, sc_r), mc, f"switch {k} after {t}")
     ref.close()
     net.close()
+
+
+def test_learned_reach_skin_trains_and_stays_exact(ctx):
+    """The skin model trained on a per-step-rebuild trajectory of the scene (the library's own MLP kernels), then
+    the learned reach skin (coarse cell from the model's predicted skins, rebuilt every 8 steps) against the
+    per-step rebuild on the same steps."""
+    from softbox_amd.network import Network
+    from softbox_amd.skin import fit_reach
+
+    make = lambda: _scene(seed=12, eta=0.1)  # noqa: E731
+    cfg = ReachSkinConfig(horizon=8, extra_trigger=16)
+    tnet = Network(ctx, make())
+    model, hist, (x, _), _ = fit_reach(ctx, tnet, cfg, snapshots=6, iters=100)
+    tnet.close()
+    assert x.shape[0] > 0 and hist[-1] < hist[0] and cfg.margin >= 1.0
+    stats, hits, _ = _run_pair(ctx, make, cfg, mode=LEARNED, model=model, graph=True, group=8,
+                               chunks=(1, 1, 2, 20, 24))
+    assert hits > 0 and stats["rebuilds_horizon"] >= 3, stats
