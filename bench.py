@@ -516,7 +516,7 @@ def run_actomyosin(args, log) -> int:
                       "seconds": round(time.perf_counter() - t_tr, 1)}
         log(f"reach skin model trained: {train_info}")
     runs, digests = {}, {}
-    tuned, t0_all, extra, shape = None, None, 0, None
+    tuned, t0_all = None, None
     for mode in modes:
         net = Network(ctx, sc.copy(), keep_forces=False)
         net.set_overlap(bool(args.overlap))
@@ -531,23 +531,16 @@ def run_actomyosin(args, log) -> int:
         t0 = args.warmup
         if args.shape:
             net.set_shape(*[int(v) for v in args.shape.split(",")])
-        elif args.tune_steps and shape is None:
-            tuned = net.tune(t0, args.tune_steps)  # launch shape and lanes per head, over real steps of the run
-            t0 += tuned["steps"]
-            extra = tuned["steps"]
-            shape = tuned["shape"]
-            log(f"tuned launch shape {tuned} ({mode})")
-        else:
-            if shape is not None:
-                net.set_shape(shape[0], shape[2], shape[3], shape[1])
-            if extra:  # the first run's tuning steps: every run times the same step numbers
-                net.step(t0, extra)
-                t0 += extra
+        elif args.tune_steps:
+            # launch shape and lanes per head over real steps of the run; every network takes the same number
+            # of tuning steps, so every run times the same step numbers
+            tn = net.tune(t0, args.tune_steps)
+            t0 += tn["steps"]
+            if tuned is None:
+                tuned = tn
+            log(f"tuned launch shape {tn} ({mode})")
         if args.motor_group:
             net.set_motor_group(args.motor_group)
-        elif mode != "rebuild" and (tuned is None or modes[0] == "rebuild"):
-            # lanes per head for the reach skin's search: a few trial steps of each on a copy of this state
-            net.set_motor_group(reach_group(ctx, net, sc, t0))
         if t0_all is None:
             t0_all = t0
             start = sc.copy()
@@ -561,6 +554,7 @@ def run_actomyosin(args, log) -> int:
         formed0 = net.form_stats().get("formed_total", 0) if args.formation > 0 else 0
         turn0 = net.pull_turnover(start.turn.copy()) if start.turn is not None else None
         el = timed(net, ctx, t0, args.steps)
+        assert t0 == t0_all, "every run times the same steps"
         r = {"value": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps,
              "timed_steps": [t0, t0 + args.steps - 1], "motor_group": net.motor_group()}
         if args.formation > 0:
@@ -662,34 +656,6 @@ def run_actomyosin(args, log) -> int:
     print(json.dumps(out))
     ctx.close()
     return 0
-
-
-def reach_group(ctx, net, sc, t0: int, trial: int = 10) -> int:
-    """Lanes per head for the reach skin's search: time `trial` steps of each group on this network from
-    step t0, then restore nothing -- the trial steps run on a throwaway copy of the network's state."""
-    from softbox_amd.network import Network
-
-    tmp = sc.copy()
-    tmp.mot, tmp.dim, tmp.mini = sc.mot.copy(), sc.dim.copy(), sc.mini.copy()
-    net.pull(tmp.fs, tmp.xl)
-    net.pull_motors(tmp.mot, tmp.mini)
-    if tmp.turn is not None:
-        tmp.turn = tmp.turn.copy()
-        net.pull_turnover(tmp.turn, tmp.fs)
-    trial_net = Network(ctx, tmp, keep_forces=False)
-    trial_net.set_reach_skin(*net.reach_mode())
-    trial_net.use_graph(True)
-    best, t = None, t0
-    for g in (4, 8, 16, 32, 64):
-        trial_net.set_motor_group(g)
-        trial_net.step(t, 2)
-        t += 2
-        el = timed(trial_net, ctx, t, trial)
-        t += trial
-        if best is None or el < best[0]:
-            best = (el, g)
-    trial_net.close()
-    return best[1]
 
 
 STANDARD_1X_PUBLISHED = {"v1-CPU": 30.4, "v1-GPU": 21.4, "v2-CPU": 15.3, "v2-GPU": 49.5}   # steps/s

@@ -80,7 +80,7 @@ struct sbx_net {
     sbx_motors mo{};
     sbx_motor_params mp{};
     int *ownerStore = nullptr, *ownerSlot = nullptr;   // body view back-pointers over cap = n + nM
-    // segment -> bound heads of this step (k_seg_heads): count and up to SEG_HEADS heads per segment; k_rod reads
+    // segment -> bound heads of this step (k_bond_head): count and up to SEG_HEADS heads per segment; k_rod reads
     // and clears them (CrossBridgeSystem.csrBuild's inverse, CrossBridgeSystem.java:189-298, without the sort)
     int *segHeadCnt = nullptr, *segHeads = nullptr;
     int *bbOff = nullptr, *bbList = nullptr;           // backbone -> tethered dimers CSR (static)
@@ -208,18 +208,9 @@ const char *kTimerNames[T_COUNT_TIMERS] = {"k_body_cell", "grid_sort",     "k_br
 
 // The bound heads of each segment for this step's motor gather (CrossBridgeSystem.csrBuild, CrossBridgeSystem.java:
 // 189-298, builds a CSR by a counting sort over every segment; few heads are bound, so each bound head takes a
-// slot of its segment instead and k_rod orders them).
+// slot of its segment in k_bond_head instead and k_rod orders them).
 constexpr int SEG_HEADS = 8;
 
-__global__ void __launch_bounds__(256) k_seg_heads(const int *__restrict__ boundSeg, int nM, int n,
-                                                   int *__restrict__ segHeadCnt, int *__restrict__ segHeads) {
-    const int m = blockIdx.x * 256 + threadIdx.x;
-    if (m >= nM) return;
-    const int s = boundSeg[m];
-    if (s < 0 || s >= n) return;
-    const int slot = atomicAdd(&segHeadCnt[s], 1);
-    if (slot < SEG_HEADS) segHeads[(size_t)s * SEG_HEADS + slot] = m;
-}
 
 // CrossBridgeSystem.segGather for segment s (CrossBridgeSystem.java:302-320): the bound heads' bond forces and
 // torques summed in ascending head order (the CSR's order: a stable sort keeps ascending index within a key),
@@ -779,7 +770,7 @@ constexpr int RS_WORDS = RS_HIST + RS_HBINS;
 constexpr int RS_PER_LANE = 2;   // hits a lane holds in registers (more on any lane: the selection path)
 constexpr int RS_KEY_BITS = 26;
 constexpr unsigned RS_KEY_MASK = (1u << RS_KEY_BITS) - 1u;
-constexpr int RS_MODEL_CELLS = 65536;  // coarse cells the model evaluates per rebuild (at most about)
+constexpr int RS_MODEL_CELLS = 16384;  // coarse cells the model evaluates per rebuild (at most about)
 constexpr float RS_SLACK = 1.0e-3f;  // um, covers float rounding of the reach bound, the cells and the guard
 
 struct ReachArgs {
@@ -809,9 +800,18 @@ __global__ void __launch_bounds__(256) k_motor_bind_skin(MotorArgs a, ReachArgs 
     }
     if (k >= m.nM) return;  // whole groups leave together
     const int nM = m.nM, nSeg = a.nSeg;
-    const float mx = m.head[k], my = m.head[nM + k], mz = m.head[2 * nM + k];
-    const float mux = m.hUVec[k], muy = m.hUVec[nM + k], muz = m.hUVec[2 * nM + k];
-    const float rux = m.rodUVec[k], ruy = m.rodUVec[nM + k], ruz = m.rodUVec[2 * nM + k];
+    // publishHeadFromBody (MotorStore.java:300-314), which k_pub_heads runs for the per-step path
+    const int nB = 3 * nM, hb = 3 * k + 2, rb = 3 * k;
+    const float hl = m.segLength[hb];
+    const float mux = m.uVec[hb], muy = m.uVec[nB + hb], muz = m.uVec[2 * nB + hb];
+    const float mx = m.coord[hb] + 0.5f * hl * mux, my = m.coord[nB + hb] + 0.5f * hl * muy,
+                mz = m.coord[2 * nB + hb] + 0.5f * hl * muz;
+    const float rux = m.uVec[rb], ruy = m.uVec[nB + rb], ruz = m.uVec[2 * nB + rb];
+    if (gl == 0) {
+        m.head[k] = mx; m.head[nM + k] = my; m.head[2 * nM + k] = mz;
+        m.hUVec[k] = mux; m.hUVec[nM + k] = muy; m.hUVec[2 * nM + k] = muz;
+        m.rodUVec[k] = rux; m.rodUVec[nM + k] = ruy; m.rodUVec[2 * nM + k] = ruz;
+    }
     const ReachGrid &rg = a.rg;
     // the head's binding-grid stencil, as the per-step walk clamps it
     const int cx = clamp_bin((mx - rg.xMin) * rg.invCell, rg.nX);
@@ -839,22 +839,23 @@ __global__ void __launch_bounds__(256) k_motor_bind_skin(MotorArgs a, ReachArgs 
     auto cand = [&](int q) -> int {
         if (q >= total) return -1;
         if (brute) return q;
-        if (q < nGrid) {
-            const int s = r.cont[stencil_at9(pre, shift, q)];
-            return r.skin[s] < 0.0f ? -1 : s;  // listed segments are taken from the list
-        }
+        if (q < nGrid) return r.cont[stencil_at9(pre, shift, q)];
         return r.extra[q - nGrid];
     };
-    // reach distance^2 of segment s (< 0: no hit) and its key in the per-step list; the centre and both ends
-    // are loaded together (one round trip)
-    auto test = [&](int s, unsigned &key) -> float {
-        const float px = r.coord[s], py = r.coord[nSeg + s], pz = r.coord[2 * nSeg + s];
+    // reach distance^2 of candidate q's segment s (< 0: no hit) and its key in the per-step list.  A grid entry
+    // that is on the outside-skin list is skipped (the list supplies it); the skin and both ends load together,
+    // the centre (for the binding-grid cell) only for a reach hit.
+    auto test = [&](int q, int s, unsigned &key) -> float {
+        const float sk = (!brute && q < nGrid) ? r.skin[s] : 0.0f;
         const float e1x = a.e1[s], e1y = a.e1[nSeg + s], e1z = a.e1[2 * nSeg + s];
         const float e2x = a.e2[s], e2y = a.e2[nSeg + s], e2z = a.e2[2 * nSeg + s];
-        const int3 b = cell3_of(px, py, pz, r.fine);
+        if (sk < 0.0f) return -1.0f;
+        const float d = reach_test(mx, my, mz, mux, muy, muz, rux, ruy, ruz, e1x, e1y, e1z, e2x, e2y, e2z, tol, align);
+        if (d < 0.0f) return -1.0f;
+        const int3 b = cell3_of(r.coord[s], r.coord[nSeg + s], r.coord[2 * nSeg + s], r.fine);
         if (b.x < x0 || b.x > x1 || b.y < y0 || b.y > y1 || b.z < z0 || b.z > z1) return -1.0f;
         key = ((unsigned)((b.z - z0) * 9 + (b.y - y0) * 3 + (b.x - x0)) << RS_KEY_BITS) | (unsigned)s;
-        return reach_test(mx, my, mz, mux, muy, muz, rux, ruy, ruz, e1x, e1y, e1z, e2x, e2y, e2z, tol, align);
+        return d;
     };
     const unsigned long long groupBits = (G == 64 ? ~0ull : ((1ull << G) - 1)) << (grp * G);
     unsigned hk[RS_PER_LANE];
@@ -873,7 +874,7 @@ __global__ void __launch_bounds__(256) k_motor_bind_skin(MotorArgs a, ReachArgs 
         for (int u = 0; u < U; u++) {
             unsigned key = 0;
             float d = -1.0f;
-            if (sg[u] >= 0) d = test(sg[u], key);
+            if (sg[u] >= 0) d = test(base + u * G + gl, sg[u], key);
             const bool hit = d >= 0.0f;
             cnt += __popcll(__ballot(hit) & groupBits);
             if (hit) {
@@ -935,7 +936,7 @@ __global__ void __launch_bounds__(256) k_motor_bind_skin(MotorArgs a, ReachArgs 
                 const int s = cand(base + gl);
                 if (s >= 0) {
                     unsigned key = 0;
-                    const float d = test(s, key);
+                    const float d = test(base + gl, s, key);
                     if (d >= 0.0f && key >= lo && key < mk) {
                         mk = key;
                         md = d;
@@ -1179,10 +1180,12 @@ __global__ void __launch_bounds__(256) k_backbone(BackboneArgs a) {
 }
 
 // CrossBridgeSystem.bondForces + applyHeadForce for one motor (reads the step-start filament pose)
+// (also gives a bound head its slot in its segment's list for this step's motor gather in k_rod)
 __global__ void __launch_bounds__(256) k_bond_head(sbx_motors m, const float *__restrict__ fc,
                                                    const float *__restrict__ fu, const float *__restrict__ fy,
                                                    const float *__restrict__ fbRG, const float *__restrict__ fLen,
-                                                   XbParams xp, int nSeg) {
+                                                   XbParams xp, int nSeg, int *__restrict__ segHeadCnt,
+                                                   int *__restrict__ segHeads) {
     int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= m.nM) return;
     float out[13];
@@ -1192,6 +1195,10 @@ __global__ void __launch_bounds__(256) k_bond_head(sbx_motors m, const float *__
     if (s >= 0)
         bond_one(k, m.nM, nSeg, s, m.coord, m.uVec, m.yVec, m.bRotGam, fc, fu, fy, fbRG, fLen, m.bindArc[k],
                  m.nucleotideState[k], xp, out);
+    if (s >= 0 && s < nSeg) {
+        const int slot = atomicAdd(&segHeadCnt[s], 1);
+        if (slot < SEG_HEADS) segHeads[(size_t)s * SEG_HEADS + slot] = k;
+    }
     float *d = m.bondData + (size_t)k * XB_STRIDE;
 #pragma unroll
     for (int j = 0; j < 13; j++) d[j] = out[j];
@@ -1839,11 +1846,13 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
     const int nM = m.nM, nB = 3 * nM, cap = n + nM;
     int r;
     {
+        // (the reach skin's search publishes the heads itself)
         Scope sc(net, T_BIN);
-        hipLaunchKernelGGL(k_pub_heads, dim3(blocks_for(nM, 256)), dim3(256), 0, s, m);
-        if (!net->rsMode)
+        if (!net->rsMode) {
+            hipLaunchKernelGGL(k_pub_heads, dim3(blocks_for(nM, 256)), dim3(256), 0, s, m);
             hipLaunchKernelGGL(k_body_cell_view, dim3(blocks_for(cap, 256)), dim3(256), 0, s, coord, n, m.head, nM,
                                net->geom, net->bodyCell);
+        }
         if ((r = check_launch(c, "k_pub_heads/k_body_cell_view"))) return r;
     }
     if (!net->rsMode) {
@@ -1910,7 +1919,7 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
     {
         Scope sc(net, T_BOND);
         hipLaunchKernelGGL(k_bond_head, dim3(blocks_for(nM, 256)), dim3(256), 0, s, m, coord, uVec, yVec,
-                           net->fil.bRotGam, net->fil.segLength, net->xp, n);
+                           net->fil.bRotGam, net->fil.segLength, net->xp, n, net->segHeadCnt, net->segHeads);
     }
     {
         Scope sc(net, T_MINTEG);
@@ -2169,12 +2178,6 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
     a.boundR = net->boundR;
     a.segHeadCnt = net->segHeadCnt; a.segHeads = net->segHeads; a.boundSeg = net->mo.boundSeg; a.nM = net->mo.nM;
     a.bondData = net->mo.bondData;
-    if (net->hasMotors) {
-        Scope sc(net, T_MCSR);
-        hipLaunchKernelGGL(k_seg_heads, dim3(blocks_for(net->mo.nM, 256)), dim3(256), 0, s, net->mo.boundSeg,
-                           net->mo.nM, n, net->segHeadCnt, net->segHeads);
-        if ((r = check_launch(c, "k_seg_heads"))) return r;
-    }
     a.dStep = net->dStep; a.stepOff = stepOff;
     a.n = n; a.chainOn = net->p.chainOn; a.hasLinks = L > 0;
     a.confineOn = net->p.confineOn; a.checkInt = (int)net->p.boxParams[6];
@@ -3457,10 +3460,10 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
     if ((r = sbx_net_set_shape(net, best[0], best[1], best[2], best[3]))) return r;
     net->overlap = best[4];
     if (net->hasMotors) {  // then the motor stage's group size, at the chosen shape
-        // the reach skin's search also tries narrow groups (few candidates per head)
-        const int gsGrid[3] = {16, 32, 64}, gsSkin[5] = {4, 8, 16, 32, 64};
-        const int *gs = net->rsMode ? gsSkin : gsGrid;
-        const int nGs = net->rsMode ? 5 : 3;
+        // the reach skin's search also tries narrow groups (few candidates per head); the per-step grid walk
+        // runs 4 and 8 as 16, so every network takes the same number of tuning steps
+        const int gs[5] = {4, 8, 16, 32, 64};
+        const int nGs = 5;
         int bestG = net->mbGroup;
         float bestGm = -1.0f;
         hipEvent_t g0 = nullptr, g1 = nullptr;
