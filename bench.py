@@ -19,6 +19,11 @@ per launch divided by its mean duration measured with HIP events on the library'
 `cpu_baseline` times the oracle (the C restatement of the reference's sequential CPU runner) on the
 same steps from the same state the GPU's timed region starts from: the OpenMP build on all host
 threads (`cpu_baseline`) and the single-thread build (`cpu_baseline_single_thread`).
+
+Then `learned_skin_at_scale` (--at-scale-steps, default 300): the same recipe scaled to 2M rods with 8k
+minifilaments (128k myosin heads), where the learned skin covers the binding grid of the myosin stage (the
+reach skin): per-step rebuild and learned reach skin on the same steps, their ratio, and a check that the
+reach lists, binding events and poses of the two runs are identical.
 """
 from __future__ import annotations
 
@@ -178,6 +183,9 @@ def main() -> int:
                     help="compare: per-step rebuild and learned skin on identical trajectories (network: the filament "
                          "skin; actomyosin: the reach skin of the binding grid); ablation (actomyosin): rebuild, "
                          "fixed and learned reach skins")
+    ap.add_argument("--at-scale-steps", type=int, default=300,
+                    help="network workload: then the learned-skin comparison at scale (2M rods, 128k heads, the reach "
+                         "skin of the binding grid) over this many steps (0: skip)")
     ap.add_argument("--reach-skin", type=float, default=0.04,
                     help="actomyosin: the fixed reach skin, um (and the learned mode's first build)")
     ap.add_argument("--reach-horizon", type=int, default=0,
@@ -409,9 +417,32 @@ def main() -> int:
                    for m, r in runs.items()}
     if train_info:
         out["skin_training"] = train_info
-    print(json.dumps(out))
     ctx.close()
+    if args.at_scale_steps > 0:
+        out["learned_skin_at_scale"] = learned_skin_at_scale(args, log)
+    print(json.dumps(out))
     return 0
+
+
+def learned_skin_at_scale(args, log) -> dict:
+    """The learned-skin comparison at scale: the crosslinked-network recipe scaled to 200k filaments (2M rods) in
+    a 60 um box with 400k crosslinker slots and 8k minifilaments (128k heads), 1 Pa s, where the skin covers the
+    binding grid of the myosin stage (the reach skin).  Per-step rebuild and learned reach skin on the same steps;
+    their reach lists, binding events and poses at the end are identical (checked)."""
+    import copy
+
+    a2 = copy.copy(args)
+    a2.workload, a2.n_fil, a2.n_links, a2.box, a2.eta, a2.turnover = "actomyosin", 200000, 400000, 60.0, 1.0, 0
+    a2.skin, a2.steps, a2.warmup, a2.no_cpu, a2.profile_steps = "compare", args.at_scale_steps, 10, True, 0
+    sub = run_actomyosin(a2, log, emit=False)
+    runs = sub["runs"]
+    return {"workload": sub["config"]["workload"], "rods": sub["config"]["rods"],
+            "motor_heads": sub["config"]["motor_heads"], "steps": a2.steps,
+            "rebuild_ms_per_step": runs["rebuild"]["ms_per_step"], "learned_ms_per_step": runs["learned"]["ms_per_step"],
+            "learned_rod_steps_per_s": round(runs["learned"]["value"], 1),
+            "learned_vs_rebuild": sub["learned_vs_rebuild"],
+            "reach_lists_and_binding_identical": sub["reach_lists_and_binding_identical"],
+            "reach_skin": runs["learned"].get("reach_skin"), "training": sub.get("reach_skin_training")}
 
 
 def run_cpu_baseline_actomyosin(sc, step0: int, budget_s: float = 12.0, openmp: bool = False) -> dict:
@@ -469,7 +500,7 @@ def reach_digest(net, sc) -> dict:
             "stats": mot.stats.copy(), "coord": fs.coord.copy(), "motorCoord": mot.body.coord.copy()}
 
 
-def run_actomyosin(args, log) -> int:
+def run_actomyosin(args, log, emit: bool = True):
     """The full actomyosin network: 50k filaments x 10 rods (0.1 um) in the reference's clamped 30 um box
     with containment walls, 2k minifilaments (4 dimers each end -> 16 heads, stall 4 pN), 100k
     crosslinkers; motor reach grid + binding + catch-slip + cycle + bond forces + joints + dimer
@@ -653,8 +684,10 @@ def run_actomyosin(args, log) -> int:
     out["runs"] = {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for m, r in runs.items()}
     if train_info:
         out["reach_skin_training"] = train_info
-    print(json.dumps(out))
     ctx.close()
+    if not emit:
+        return out
+    print(json.dumps(out))
     return 0
 
 
