@@ -715,6 +715,13 @@ def run_standard(args, log) -> int:
     ctx = Context(0)
     net = Network(ctx, sc.copy(), keep_forces=False)
     net.set_overlap(bool(args.overlap))
+    reach = args.skin == "fixed"  # the reach skin of the binding grid (the default here: the per-step grid)
+    if reach:
+        from softbox_amd import _lib
+        from softbox_amd.skin import ReachSkinConfig
+
+        net.set_reach_skin(_lib.SBX_SKIN_FIXED, ReachSkinConfig(skin_default=args.reach_skin,
+                                                                horizon=args.reach_horizon))
     if args.graph:
         net.use_graph(True)
     net.step(0, args.warmup)
@@ -744,6 +751,7 @@ def run_standard(args, log) -> int:
     assert np.isfinite(end.fs.coord).all() and np.isfinite(end.mot.body.coord).all(), "non-finite pose"
     assert np.isfinite(end.mini.node.coord).all(), "non-finite node pose"
     mgroup = net.motor_group()
+    reach_stats = net.reach_skin_stats() if reach else None
     net.close()
     roof = None
     if kernels:
@@ -779,7 +787,9 @@ def run_standard(args, log) -> int:
                                "viscosity 0.1 Pa s, Wang-hash RNG with the harness's seeds",
                    "segments": n, "myosins": sc.mot.nM, "nodes": sc.mini.nNodes, "link_slots": sc.xl.nLinks,
                    "grid": list(sc.p["gridDims"][:3]), "graph": bool(args.graph),
-                   "shape": shape_of(args, tuned) if (args.shape or tuned) else None, "motor_group": mgroup},
+                   "shape": shape_of(args, tuned) if (args.shape or tuned) else None, "motor_group": mgroup,
+                   "binding_search": "reach skin, fixed" if reach else "per-step binding grid (the reference's)"},
+        "reach_skin": reach_stats,
         "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu1,
         "links_formed_timed": formed, "links_alive_end": int((end.xl.linkState[: end.xl.nLinks] >= 0).sum()),
         "heads_bound_start": int((start.mot.boundSeg >= 0).sum()), "heads_bound_end": int((end.mot.boundSeg >= 0).sum()),
