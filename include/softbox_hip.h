@@ -505,9 +505,13 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
 /* ------------------------------------------------------------------ skin broad phase
  * No counterpart in the reference (SpatialGrid.broadPhase, SpatialGrid.java:340-404, rebuilds the grid
  * every step).  The candidate slices candPartner/candCount stay identical to the per-step rebuild's:
- * per-body neighbour lists are built with reach + skin_i + skin_j and filtered exactly every step, and
- * an exact displacement guard rebuilds whenever any body moved more than its skin.  The skin only
- * changes the cost.  While a skin mode is on, the fine grid CSR (gridCellOffsets/Contents) is not
+ * per-body neighbour lists are built at a rebuild with reach + skin_i + skin_j and filtered exactly every
+ * step.  A body that moves more than its skin since the rebuild goes (by the step's own guard) onto an
+ * outside-skin list that every body also tests, and finds its own candidates by walking the rebuild's
+ * coarse grid; past extraCap every body tests every other.  Rebuilds are scheduled by the host between
+ * steps (graph replays): on the first step, after sbx_net_publish, every `horizon` steps (0: never), and
+ * once the latest finished step reports more than extraTrigger bodies outside their skins.  The skin
+ * only changes the cost.  While a skin mode is on, the fine grid CSR (gridCellOffsets/Contents) is not
  * rebuilt; bodyCell still is. */
 #define SBX_SKIN_REBUILD 0   /* the reference's per-step grid rebuild + broad phase */
 #define SBX_SKIN_FIXED 1     /* every body gets skinDefault */
@@ -516,15 +520,18 @@ typedef struct sbx_skin_params {
     float skinDefault;   /* um; fixed mode, and the first build of learned mode */
     float skinMin, skinMax;
     float margin;        /* learned skin = margin * exp(model out0) */
-    int listCap;         /* neighbour slots per body (overflowing bodies fall back to an exact coarse walk) */
-    int blockKeys;       /* keys a block buffers for one dense body, 0 = the maximum (4096); a body with
-                          * more is reduced to its first maxCand in chunks (smaller values exercise that) */
+    int listCap;         /* neighbour slots per body (an overflowing body walks the coarse grid instead) */
+    int horizon;         /* steps between scheduled rebuilds (0: only the other triggers) */
+    int extraTrigger;    /* bodies outside their skins (reported by the latest finished step) that schedule
+                          * a rebuild */
+    int extraCap;        /* capacity of the outside-skin list */
 } sbx_skin_params;
 /* modelWeights: device array of sbx_mlp_nparam() floats (learned mode; copied) */
 int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *p, const float *modelWeights);
-/* device counters -> out[0..cap): rebuild pending, steps since rebuild, rebuilds, overflowing bodies,
- * list entries at the last build, skin-mode steps, body-steps done block-wide so far (more than 64
- * candidates, or an overflowed list); returns the number written (<= 7) */
+/* counters -> out[0..cap): rebuilds, bodies outside their skins now, steps in which every body tested every
+ * other (outside-skin list overflow), body-steps done by a whole block (more than 64 candidates), list entries
+ * at the last build, bodies whose list overflowed at the last build, steps since the last rebuild, skin-mode
+ * steps; returns the number written (<= 8) */
 int sbx_net_skin_stats(sbx_net *net, long long *out, int cap);
 /* device state of the last rebuild (for training): reference centres, skins, coarse cells and CSR;
  * coarseGrid[8] = xMin, yMin, zMin, cell, 1/cell, nX, nY, nZ */

@@ -124,7 +124,7 @@ SBX_SKIN_LEARNED = 2
 
 class SbxSkinParams(C.Structure):
     _fields_ = [("skinDefault", C.c_float), ("skinMin", C.c_float), ("skinMax", C.c_float), ("margin", C.c_float),
-                ("listCap", i32), ("blockKeys", i32)]
+                ("listCap", i32), ("horizon", i32), ("extraTrigger", i32), ("extraCap", i32)]
 
 
 class SbxReachSkinParams(C.Structure):

@@ -114,7 +114,7 @@ struct sbx_net {
     // against the capture rule before the graph is used (capture_check)
     std::vector<int> capLog;
     std::vector<hipEvent_t> capEv;
-    hipEvent_t evFork = nullptr, evBuilt = nullptr;
+    hipEvent_t evFork = nullptr;
     hipEvent_t evRod[2] = {nullptr, nullptr}, evPlaced[2] = {nullptr, nullptr}, evJoin[2] = {nullptr, nullptr};
     int needFork = 1;  // the side streams must be ordered after the main stream before their next work
     // 1: the latest record of some event was made inside a stream capture (a captured record stands
@@ -181,12 +181,17 @@ struct sbx_net {
     int *rsExtra = nullptr;                  // segments outside their skins (rsp.extraCap)
     int *rsWords = nullptr;                  // device words (RS_*)
     float *rsModel = nullptr;                // MLP_NPARAM device floats (learned mode)
-    volatile int *rsHost = nullptr;          // pinned: {generation, outside-skin count} of the latest finished step
+    volatile int *rsHost = nullptr;          // pinned: {generation, outside-skin count} of the latest finished step,
+                                             // reach skin [0..1] and filament skin [2..3]
     int *rsHostDev = nullptr;                // its device address
     int rsValid = 0;                         // the coarse grid describes the current segments
     int rsGen = 0, rsSince = 0;              // rebuilds enqueued so far; steps since the last one
     long long rsWhy[3] = {0, 0, 0};          // rebuilds for: stale state, horizon, outside-skin count
     long long rsSteps = 0;                   // steps in the mode
+    // the filament skin's host schedule (sbx_net_set_skin): outside-skin list, validity, rebuild counts
+    int *fsExtra = nullptr;
+    int fsValid = 0, fsGen = 0, fsSince = 0;
+    long long fsWhy[3] = {0, 0, 0};
     // the host's view of the outside-skin count is at most RS_INFLIGHT replays (or eager steps) old: an event
     // after each, and the host waits for the oldest before enqueueing past it
     static constexpr int RS_INFLIGHT = 2;
@@ -267,11 +272,11 @@ struct RodArgs {
     int *nextCounts, *nextSlot;  // the new pose's grid histogram (null: not taken here)
     float4 *linkPack;       // packed endpoint data of the new pose for the next step's links (or null)
     const float *skinRef;         // skin guard: centre at the last rebuild (or null)
-    float *skin;                  // per-body skin
-    int *rebuildFlag;             // skin rebuild request word of the new pose's buffer; reach skin: the
-                                  // outside-skin count
-    int *rsExtra;                 // reach skin: the outside-skin list (null: the filament skin's guard)
+    float *skin;                  // per-body skin (< 0: on the outside-skin list)
+    int *rebuildFlag;             // the outside-skin count
+    int *rsExtra;                 // the outside-skin list (filament skin or reach skin)
     int rsCap;                    // its capacity
+    int minImage;                 // displacements by minimum image (the filament skin in a periodic box)
     float4 *bodyPack;             // {new centre, bounding radius} for the next step's broad phase (or null)
     const float *boundR;
     GridGeom geom;
@@ -451,23 +456,18 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a, ChainParams cp) {
         a.nextCell[i] = cell;
         if (a.nextCounts) a.nextSlot[i] = atomicAdd(&a.nextCounts[cell], 1);
     }
-    if (a.skinRef) {  // exact displacement guard: any body beyond its skin forces the next rebuild
+    if (a.skinRef) {  // exact displacement guard: a body beyond its skin goes onto the outside-skin list
         float dx = p.c[0] - a.skinRef[i], dy = p.c[1] - a.skinRef[n + i], dz = p.c[2] - a.skinRef[2 * n + i];
-        if (a.rsExtra) {  // reach skin: raw coordinates (the reach test and the grids use them unwrapped)
-            const float sk = a.skin[i];
-            if (sk >= 0.0f && dx * dx + dy * dy + dz * dz > sk * sk) {  // first time out: onto the list
-                a.skin[i] = -1.0f;
-                const int e = atomicAdd(a.rebuildFlag, 1);
-                if (e < a.rsCap) a.rsExtra[e] = i;
-            }
-        } else {
-            if (a.geom.periodic) {
-                dx = min_image(dx, a.geom.Lx, a.geom.iLx);
-                dy = min_image(dy, a.geom.Ly, a.geom.iLy);
-                dz = min_image(dz, a.geom.Lz, a.geom.iLz);
-            }
-            float sk = a.skin[i];
-            if (dx * dx + dy * dy + dz * dz > sk * sk) *a.rebuildFlag = 1;
+        if (a.minImage) {  // (the reach skin takes raw coordinates: its test and grids use them unwrapped)
+            dx = min_image(dx, a.geom.Lx, a.geom.iLx);
+            dy = min_image(dy, a.geom.Ly, a.geom.iLy);
+            dz = min_image(dz, a.geom.Lz, a.geom.iLz);
+        }
+        const float sk = a.skin[i];
+        if (sk >= 0.0f && dx * dx + dy * dy + dz * dz > sk * sk) {  // first time out: onto the list
+            a.skin[i] = -1.0f;
+            const int e = atomicAdd(a.rebuildFlag, 1);
+            if (e < a.rsCap) a.rsExtra[e] = i;
         }
     }
     if (a.bodyPack) a.bodyPack[i] = make_float4(p.c[0], p.c[1], p.c[2], a.boundR[i]);
@@ -1276,8 +1276,11 @@ __global__ void __launch_bounds__(256) k_motor_integ(sbx_motors m, float dt, flo
 // small neighbourhood to scan.
 struct SkinArgs {
     int n, mode, listCap, maxCand, cCellsMax;
-    int gate;                // flag word of this pose's rebuild request (skf_rebuild(pose buffer))
-    int blockKeys;           // keys k_skin_pairs buffers for one heavy body (<= TILE_BODIES * TILE_KEYS)
+    int gate;                // flag word that opens the rebuild kernels (set by the host around a rebuild)
+    int gen;                 // generation of the rebuild being built
+    int *extra;              // the outside-skin list
+    int extraCap;
+    int *hostWords;          // pinned {.., .., generation, outside-skin count} (device address)
     const float *coord, *uVec, *boundR;
     const int *links;        // activeLinkCount (may be null)
     GridGeom geom;           // fine grid (the broad phase's)
@@ -1297,7 +1300,6 @@ struct SkinArgs {
 
 constexpr float SKIN_SLACK = 1.0e-3f;  // um, covers float rounding of the list and guard distances
 constexpr int SKF_MAXSKIN = 6;         // flag word: largest skin of the rebuild (float bits)
-constexpr int SKF_HEAVY = 7;           // flag word: bodies k_skin_pairs did block-wide so far
 
 // Skins of this rebuild.  Learned mode after the first rebuild: thread c is a coarse cell of the
 // last rebuild -- features over its members, model, clamp, then the skin of every member;
@@ -1364,6 +1366,8 @@ __global__ void __launch_bounds__(256) k_skin_bin(SkinArgs a) {
         a.flags[SKF_ENTRIES] = 0;
         a.flags[SKF_SINCE] = 0;
         a.flags[SKF_REBUILDS] += 1;
+        a.flags[SKF_EXTRA] = 0;  // a fresh outside-skin list
+        a.flags[SKF_GEN] = a.gen;
     }
     if (i >= a.n) return;
     float4 q = a.pack[i];
@@ -1385,7 +1389,7 @@ __global__ void __launch_bounds__(BP_THREADS) k_skin_build(SkinArgs a) {
     int pre[18], shift[18];
     const int total = stencil_map(a.cCell[i], g, a.cOff, pre, shift);
     int cnt = 0;
-    int *list = a.vlist + (size_t)t * a.listCap;  // per coarse slot: k_skin_pairs reads a tile's lists in one sweep
+    int *list = a.vlist + (size_t)t * a.listCap;  // per coarse slot: k_skin_pairs reads slot t's list
     for (int q = 0; q < total; q += BP_BATCH) {
         int jj[BP_BATCH];
 #pragma unroll
@@ -1451,177 +1455,149 @@ __device__ __forceinline__ int skin_key(int j, int3 ci, float icx, float icy, fl
 }
 
 
-// The 27 coarse cells (of the last rebuild) around body i's coarse cell as contents ranges: thread
-// r < 27 stores range r's start in segLo[r] and its length in segStart[r + 1]; the caller sets
-// segStart[0] = 0 and prefix-sums.  Out-of-grid cells (box) are empty ranges; a periodic coarse
-// grid has >= 3 cells per axis, so the 27 cells are distinct.
-__device__ __forceinline__ void coarse_ranges(const SkinArgs &a, int i, int r, int *segLo, int *segStart) {
-    if (r == 0) segStart[0] = 0;
-    if (r >= 27) return;
-    const GridGeom cgm = *a.cg;
-    const int3 cc = cell_xyz(a.cCell[i], cgm);
-    int xx = cc.x + r % 3 - 1, yy = cc.y + (r / 3) % 3 - 1, zz = cc.z + r / 9 - 1;
-    bool ok = true;
-    if (cgm.periodic) {
-        xx = xx < 0 ? xx + cgm.nX : (xx >= cgm.nX ? xx - cgm.nX : xx);
-        yy = yy < 0 ? yy + cgm.nY : (yy >= cgm.nY ? yy - cgm.nY : yy);
-        zz = zz < 0 ? zz + cgm.nZ : (zz >= cgm.nZ ? zz - cgm.nZ : zz);
-    } else {
-        ok = xx >= 0 && xx < cgm.nX && yy >= 0 && yy < cgm.nY && zz >= 0 && zz < cgm.nZ;
-    }
-    int lo = 0, hi = 0;
-    if (ok) {
-        int c = xx + yy * cgm.nX + zz * cgm.nX * cgm.nY;
-        lo = a.cOff[c];
-        hi = a.cOff[c + 1];
-    }
-    segLo[r] = lo;
-    segStart[r + 1] = hi - lo;
-}
+constexpr int SP_LANES = 4;   // lanes per body in k_skin_pairs
+constexpr int SP_KEYS = 64;   // keys a body buffers in LDS (more: the whole block takes the body)
+static_assert((256 / SP_LANES) * SP_KEYS == TILE_BODIES * TILE_KEYS, "one key buffer for both phases");
 
-// entry k of the concatenated coarse ranges
-__device__ __forceinline__ int coarse_entry(const SkinArgs &a, int k, const int *segLo, const int *segStart) {
-    int r = 0;
-    while (segStart[r + 1] <= k) r++;
-    return a.cCont[segLo[r] + (k - segStart[r])];
-}
+// One body's candidates for k_skin_pairs: its list (every j > i within reach + both skins at the rebuild) --
+// or, if i left its skin or its list overflowed, the 27 coarse cells around its current coarse cell --
+// without the bodies on the outside-skin list, plus every j > i on that list; past the list's capacity,
+// every j > i.
+struct SkinBody {
+    int i, nMain, total;
+    bool walk, brute;
+    float4 qi;
+    int3 ci;
+    const int *list;
+    int pre[18], shift[18];
+    __device__ void init(const SkinArgs &a, int t, int nE) {
+        brute = nE > a.extraCap;
+        i = a.cCont[t];
+        qi = a.pack[i];
+        ci = cell3_of(qi.x, qi.y, qi.z, a.geom);
+        const int cnt = a.vcount[t];
+        walk = !brute && (a.skin[i] < 0.0f || cnt > a.listCap);
+        list = a.vlist + (size_t)t * a.listCap;
+        nMain = cnt;
+        if (walk) {
+            const GridGeom cg = *a.cg;
+            nMain = stencil_map(cell_of(qi.x, qi.y, qi.z, cg), cg, a.cOff, pre, shift);
+        }
+        total = brute ? a.n : nMain + nE;
+    }
+    __device__ int cand(const SkinArgs &a, int q) const {  // the q-th candidate body, or -1
+        if (q >= total) return -1;
+        int j;
+        if (brute) {
+            j = q;
+        } else if (q < nMain) {
+            j = walk ? a.cCont[stencil_at(pre, shift, q)] : list[q];
+            if (a.skin[j] < 0.0f) return -1;  // on the outside-skin list: taken from there
+        } else {
+            j = a.extra[q - nMain];
+        }
+        return j > i ? j : -1;
+    }
+    __device__ int key(const SkinArgs &a, int j, float4 qj) const {
+        return skin_key(j, ci, qi.x, qi.y, qi.z, qi.w, qj, a.geom);
+    }
+};
 
-// Candidate slices of the current pose from the neighbour lists, in the broad phase's order.  A
-// block takes TILE_BODIES consecutive coarse slots and walks the concatenation of their lists with
-// all its threads (so a long list does not hold back a wave), collects each body's keys in LDS and
-// places each key at its rank among its body's keys (keys are distinct).  A body with more than
-// TILE_KEYS candidates, or whose list overflowed at the rebuild, is then done by the whole block.
-__global__ void __launch_bounds__(TILE_THREADS) k_skin_pairs(SkinArgs a) {
-    __shared__ int keyS[TILE_BODIES * TILE_KEYS];
-    __shared__ float4 qS[TILE_BODIES];
-    __shared__ int4 cS[TILE_BODIES];     // fine cell (x, y, z) and body index
-    __shared__ int startS[TILE_BODIES + 1], mS[TILE_BODIES];
-    __shared__ int heavyS[TILE_BODIES], heavyN, foundS;
-    __shared__ int segLo[27], segStart[28];
-    const int tid = threadIdx.x;
-    if (blockIdx.x == 0 && tid == 0) {  // no other thread reads these words in this kernel
-        a.flags[a.gate] = 0;  // the concurrent k_rod sets only the other buffer's word
+// Candidate slices of the current pose (SpatialGrid.broadPhase's output) from the skin state, SP_LANES lanes per
+// coarse slot t (body i = cCont[t], so neighbouring groups hold neighbouring bodies).  Each candidate is kept by
+// the broad phase's own predicate (skin_key, same arithmetic); the group gathers the kept keys (stencil position
+// << 26 | j, distinct) in LDS by ballot and places each at its rank, so the slice is in the broad phase's order.
+// A body with more than SP_KEYS keys is then done by the whole block: keys gathered into the whole buffer and
+// rank-placed, or (more keys than that) the maxCand smallest kept over chunks.
+__global__ void __launch_bounds__(256) k_skin_pairs(SkinArgs a) {
+    constexpr int G = SP_LANES, GPB = 256 / G;
+    __shared__ int keyS[GPB * SP_KEYS];
+    __shared__ int heavyS[GPB], heavyN, foundS;
+    const int lane = threadIdx.x & 63, gl = lane % G, gb = threadIdx.x / G;
+    const int t = blockIdx.x * GPB + gb;
+    const int n = a.n;
+    const int nE = a.flags[SKF_EXTRA];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // no other thread reads these words in this kernel
         a.flags[SKF_SINCE] += 1;
         a.flags[SKF_STEPS] += 1;
-        a.flags[SKF_MAXSKIN] = 0;  // the next rebuild's running max starts here
+        if (nE > a.extraCap) a.flags[SKF_BRUTE] += 1;
+        a.hostWords[3] = nE;  // the host's (lagged) view of the guard
+        a.hostWords[2] = a.flags[SKF_GEN];
     }
-    const int n = a.n;
-    const GridGeom &g = a.geom;
-    const int t0 = xcd_block(blockIdx.x, gridDim.x) * TILE_BODIES;
-    if (tid < TILE_BODIES) {  // wave 0: the tile's bodies and the prefix sum of their list lengths
-        const int t = t0 + tid;
-        int cnt = 0;
-        bool over = false;
-        if (t < n) {
-            cnt = a.vcount[t];
-            const int i = a.cCont[t];
-            const float4 q = a.pack[i];
-            const int3 c3 = cell3_of(q.x, q.y, q.z, g);
-            qS[tid] = q;
-            cS[tid] = make_int4(c3.x, c3.y, c3.z, i);
-            if (cnt > a.listCap) {  // overflowed list: walked below from the coarse cells
-                cnt = 0;
-                over = true;
-            }
-        }
-        mS[tid] = over ? TILE_KEYS + 1 : 0;  // an overflowed body counts as heavy
-        if (tid == 0) heavyN = 0;
-        startS[tid + 1] = wave_inclusive_scan(cnt, tid);
-        if (tid == 0) startS[0] = 0;
-    }
-    __syncthreads();
-    const int total = startS[TILE_BODIES];
-    // runs of TILE_CHUNK consecutive entries per thread: one search for the run's first body, a
-    // forward walk for the rest
-    for (int c0 = tid * TILE_CHUNK; c0 < total; c0 += TILE_THREADS * TILE_CHUNK) {
-        int bb[TILE_CHUNK], jj[TILE_CHUNK];
-        int b = tile_body_of(startS, c0);
+    if (threadIdx.x == 0) heavyN = 0;
+    const bool live = t < n;  // every thread reaches the barriers below
+    SkinBody b;
+    b.init(a, live ? t : 0, nE);
+    if (!live) b.total = 0;
+    const unsigned long long groupBits = ((1ull << G) - 1) << ((lane / G) * G);
+    const unsigned long long below = (1ull << lane) - 1;
+    int *kb = keyS + gb * SP_KEYS;
+    int hits = 0;
+    for (int base = 0; base < b.total; base += 2 * G) {
+        int jj[2];
 #pragma unroll
-        for (int u = 0; u < TILE_CHUNK; u++) {
-            const int e = c0 + u;
-            bb[u] = -1;
-            jj[u] = 0;
-            if (e < total) {
-                while (startS[b + 1] <= e) b++;  // next body with entries left (startS[64] = total)
-                bb[u] = b;
-                jj[u] = a.vlist[(size_t)(t0 + b) * a.listCap + (e - startS[b])];
-            }
-        }
-        float4 qj[TILE_CHUNK];
+        for (int u = 0; u < 2; u++) jj[u] = b.cand(a, base + u * G + gl);
+        float4 qj[2];
 #pragma unroll
-        for (int u = 0; u < TILE_CHUNK; u++) qj[u] = a.pack[jj[u]];
+        for (int u = 0; u < 2; u++) qj[u] = a.pack[jj[u] >= 0 ? jj[u] : b.i];
 #pragma unroll
-        for (int u = 0; u < TILE_CHUNK; u++) {
-            if (bb[u] < 0) continue;
-            const float4 qi = qS[bb[u]];
-            const int4 ci = cS[bb[u]];
-            int key = skin_key(jj[u], make_int3(ci.x, ci.y, ci.z), qi.x, qi.y, qi.z, qi.w, qj[u], g);
-            if (key >= 0) {
-                int k = atomicAdd(&mS[bb[u]], 1);
-                if (k < TILE_KEYS) keyS[bb[u] * TILE_KEYS + k] = key;
-            }
+        for (int u = 0; u < 2; u++) {
+            const int key = jj[u] >= 0 ? b.key(a, jj[u], qj[u]) : -1;
+            const bool hit = key >= 0;
+            const unsigned long long bits = __ballot(hit) & groupBits;
+            const int pos = hits + __popcll(bits & below);
+            if (hit && pos < SP_KEYS) kb[pos] = key;
+            hits += __popcll(bits);
         }
     }
     __syncthreads();
-    // rank placement: key (b, k) goes to slot #{keys of b smaller than it}
-    for (int p = tid; p < TILE_BODIES * TILE_KEYS; p += TILE_THREADS) {
-        const int b = p / TILE_KEYS, k = p - b * TILE_KEYS;
-        const int m = mS[b];
-        if (k >= m || m > TILE_KEYS) continue;
-        const int key = keyS[p];
-        const int *kb = keyS + b * TILE_KEYS;
-        int rank = 0;
-        for (int x = 0; x < m; x++) rank += kb[x] < key ? 1 : 0;
-        if (rank < a.maxCand) a.candPartner[(size_t)cS[b].w * a.maxCand + rank] = key & KEY_INDEX_MASK;
-    }
-    if (tid < TILE_BODIES && t0 + tid < n) {
-        const int m = mS[tid];
-        if (m <= TILE_KEYS) a.candCount[cS[tid].w] = m;
-        else heavyS[atomicAdd(&heavyN, 1)] = tid;
+    if (live) {
+        int *slice = a.candPartner + (size_t)b.i * a.maxCand;
+        if (hits <= SP_KEYS) {
+            for (int k = gl; k < hits; k += G) {
+                const int key = kb[k];
+                int rank = 0;
+                for (int x = 0; x < hits; x++) rank += kb[x] < key ? 1 : 0;
+                if (rank < a.maxCand) slice[rank] = key & KEY_INDEX_MASK;
+            }
+            if (gl == 0) a.candCount[b.i] = hits;
+        } else if (gl == 0) {
+            heavyS[atomicAdd(&heavyN, 1)] = t;
+        }
     }
     __syncthreads();
-    // heavy bodies (more than TILE_KEYS candidates, or an overflowed list), one at a time with the
-    // whole block: keys gathered into the whole key buffer and rank-placed; more than that many
-    // keys goes to k_skin_pairs_big
     const int nHeavy = heavyN;
-    for (int h = 0; h < nHeavy; h++) {
-        const int b = heavyS[h], t = t0 + b;
-        const int i = cS[b].w;
-        const int cnt = a.vcount[t];
-        const bool over = cnt > a.listCap;
-        if (over) coarse_ranges(a, i, tid, segLo, segStart);
-        if (tid == 0) foundS = 0;
+    for (int h = 0; h < nHeavy; h++) {  // heavy bodies, one at a time with the whole block
+        SkinBody hb;
+        hb.init(a, heavyS[h], nE);
+        if (threadIdx.x == 0) foundS = 0;
         __syncthreads();
-        if (over && tid == 0)
-            for (int r = 0; r < 27; r++) segStart[r + 1] += segStart[r];
-        __syncthreads();
-        const int total = over ? segStart[27] : cnt;
-        const float4 qi = qS[b];
-        const int3 ci = make_int3(cS[b].x, cS[b].y, cS[b].z);
-        auto keyAt = [&](int k) -> int {
-            int j = over ? coarse_entry(a, k, segLo, segStart) : a.vlist[(size_t)t * a.listCap + k];
-            if (j <= i) return -1;
-            return skin_key(j, ci, qi.x, qi.y, qi.z, qi.w, a.pack[j], g);
-        };
-        for (int k = tid; k < total; k += TILE_THREADS) {
-            const int key = keyAt(k);
+        for (int q = threadIdx.x; q < hb.total; q += 256) {
+            const int j = hb.cand(a, q);
+            const int key = j >= 0 ? hb.key(a, j, a.pack[j]) : -1;
             if (key >= 0) {
-                int p = atomicAdd(&foundS, 1);
-                if (p < a.blockKeys) keyS[p] = key;
+                const int p = atomicAdd(&foundS, 1);
+                if (p < GPB * SP_KEYS) keyS[p] = key;
             }
         }
         __syncthreads();
         const int m = foundS;
-        int *slice = a.candPartner + (size_t)i * a.maxCand;
-        if (m <= a.blockKeys) {
+        int *slice = a.candPartner + (size_t)hb.i * a.maxCand;
+        if (m <= GPB * SP_KEYS) {
             block_rank_place(keyS, m, a.maxCand, [&](int rank, int key) { slice[rank] = key & KEY_INDEX_MASK; });
         } else {  // more keys than the buffer: keep the maxCand smallest over chunks (exact for any number)
-            block_keep_smallest(total, keyAt, keyS, &foundS);
+            auto keyAt = [&](int q) -> int {
+                const int j = hb.cand(a, q);
+                return j >= 0 ? hb.key(a, j, a.pack[j]) : -1;
+            };
+            block_keep_smallest(hb.total, keyAt, keyS, &foundS);
             const int w = m < a.maxCand ? m : a.maxCand;
-            for (int x = tid; x < w; x += TILE_THREADS) slice[x] = keyS[x] & KEY_INDEX_MASK;
+            for (int x = threadIdx.x; x < w; x += 256) slice[x] = keyS[x] & KEY_INDEX_MASK;
         }
-        if (tid == 0) a.candCount[i] = m;
-        if (tid == 0) atomicAdd(&a.flags[SKF_HEAVY], 1);
+        if (threadIdx.x == 0) {
+            a.candCount[hb.i] = m;
+            atomicAdd(&a.flags[SKF_SELECT], 1);
+        }
         __syncthreads();  // before the next heavy body reuses the buffers
     }
 }
@@ -1756,6 +1732,24 @@ int drain_timers(sbx_net *net) {
     return SBX_OK;
 }
 
+// The pinned words through which the skins' search kernels report the outside-skin count (allocated once).
+int host_words(sbx_net *net) {
+    if (net->rsHost) return SBX_OK;
+    sbx_ctx *c = net->ctx;
+    void *h = nullptr;
+    int r = check_hip(c, hipHostMalloc(&h, 4 * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
+    if (r) return r;
+    void *d = nullptr;
+    if ((r = check_hip(c, hipHostGetDevicePointer(&d, h, 0), "hipHostGetDevicePointer"))) {
+        hipHostFree(h);
+        return r;
+    }
+    net->rsHost = (volatile int *)h;
+    net->rsHostDev = (int *)d;
+    for (int k = 0; k < 4; k++) net->rsHost[k] = k % 2 ? 0 : -1;
+    return SBX_OK;
+}
+
 // A reach-skin rebuild on the main stream (between steps or graph replays, never inside a capture): [learned:
 // the model's skins over the previous build's cells] -> geometry -> bin -> sort of the segments into the
 // coarse grid.  why: 0 stale state, 1 horizon, 2 outside-skin count.
@@ -1816,9 +1810,13 @@ int rs_maybe_rebuild(sbx_net *net) {
 // After a step or a replay: count it, and bound how far the host runs ahead of the device (so the
 // outside-skin count it reads is recent).
 int rs_advance(sbx_net *net, int steps) {
-    if (!net->rsMode) return SBX_OK;
-    net->rsSince += steps;
-    net->rsSteps += steps;
+    const bool fs = net->skinMode != SBX_SKIN_REBUILD && !net->hasMotors;
+    if (!net->rsMode && !fs) return SBX_OK;
+    if (fs) net->fsSince += steps;
+    if (net->rsMode) {
+        net->rsSince += steps;
+        net->rsSteps += steps;
+    }
     sbx_ctx *c = net->ctx;
     const int k = net->rsEvNext;
     int r = SBX_OK;
@@ -1932,9 +1930,9 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
 SkinArgs skin_args(sbx_net *net, const float *coord, const float *uVec) {
     SkinArgs a;
     a.n = net->n; a.mode = net->skinMode; a.listCap = net->sp.listCap; a.maxCand = net->p.maxCand;
-    a.gate = skf_rebuild(net->cur);
-    a.blockKeys = net->sp.blockKeys > 0 && net->sp.blockKeys < TILE_BODIES * TILE_KEYS ? net->sp.blockKeys
-                                                                                  : TILE_BODIES * TILE_KEYS;
+    a.gate = SKF_REBUILD;
+    a.gen = net->fsGen + 1;
+    a.extra = net->fsExtra; a.extraCap = net->sp.extraCap; a.hostWords = net->rsHostDev;
     a.cCellsMax = net->cCellsMax;
     a.coord = coord; a.uVec = uVec; a.boundR = net->boundR;
     a.links = net->L > 0 ? net->xl.activeLinkCount : nullptr;
@@ -1953,15 +1951,19 @@ SkinArgs skin_args(sbx_net *net, const float *coord, const float *uVec) {
     return a;
 }
 
-// skin broad phase of one step on stream s: [gated rebuild: skins (model), geometry + bin, coarse
-// sort, build] -> (evBuilt when overlapping) -> pairs
-int enqueue_skin(sbx_net *net, const float *coord, const float *uVec, hipStream_t s, bool ovl) {
+// A skin rebuild on the main stream (the host's, between steps or graph replays, never inside a capture):
+// skins (model) -> coarse geometry + bin -> coarse sort -> lists; the gate word opens the kernels for it.
+// why: 0 stale state, 1 horizon, 2 outside-skin count.
+int fs_rebuild(sbx_net *net, int why) {
     sbx_ctx *c = net->ctx;
+    hipStream_t s = c->stream;
     const int n = net->n;
-    SkinArgs a = skin_args(net, coord, uVec);
+    SkinArgs a = skin_args(net, net->coordB[net->cur], net->uB[net->cur]);
     int r;
     {
         Scope sc(net, T_SKIN_REBUILD);
+        if ((r = fill_i32(c, net->skinFlags + SKF_REBUILD, 1, 1))) return r;
+        if ((r = fill_i32(c, net->skinFlags + SKF_MAXSKIN, 0, 1))) return r;
         int span = net->cCellsMax > n ? net->cCellsMax : n;
         hipLaunchKernelGGL(k_skin_assign, dim3(blocks_for(span, 256)), dim3(256), 0, s, a, a.model);
         hipLaunchKernelGGL(k_skin_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
@@ -1971,20 +1973,40 @@ int enqueue_skin(sbx_net *net, const float *coord, const float *uVec, hipStream_
             return r;
         hipLaunchKernelGGL(k_skin_build, dim3(blocks_for(n, BP_THREADS)), dim3(BP_THREADS), 0, s, a);
         if ((r = check_launch(c, "k_skin_build"))) return r;
-        // k_rod's guard reads the skins and reference pose of this rebuild
-        if (ovl && (r = ev_record(net, net->evBuilt, s, "record evBuilt"))) return r;
+        if ((r = fill_i32(c, net->skinFlags + SKF_REBUILD, 0, 1))) return r;
     }
-    {
-        Scope sc(net, T_SKIN_PAIRS);
-        hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(n, TILE_BODIES)), dim3(TILE_THREADS), 0, s, a);
-    }
+    net->fsGen = a.gen;
+    net->fsValid = 1;
+    net->fsSince = 0;
+    net->fsWhy[why]++;
+    return SBX_OK;
+}
+
+// Before a step or a graph replay in a skin mode: rebuild on stale state, past the horizon, or when the latest
+// finished step reported more than extraTrigger bodies outside their skins (pinned memory, read without waiting).
+int fs_maybe_rebuild(sbx_net *net) {
+    if (net->skinMode == SBX_SKIN_REBUILD || net->hasMotors) return SBX_OK;
+    int why = -1;
+    if (!net->fsValid) why = 0;
+    else if (net->sp.horizon > 0 && net->fsSince >= net->sp.horizon) why = 1;
+    else if (net->rsHost[2] == net->fsGen && net->rsHost[3] > net->sp.extraTrigger) why = 2;
+    return why < 0 ? SBX_OK : fs_rebuild(net, why);
+}
+
+// the skin broad phase of one step: candidate slices from the lists (main stream)
+int enqueue_skin_pairs(sbx_net *net, const float *coord, const float *uVec) {
+    sbx_ctx *c = net->ctx;
+    SkinArgs a = skin_args(net, coord, uVec);
+    Scope sc(net, T_SKIN_PAIRS);
+    hipLaunchKernelGGL(k_skin_pairs, dim3(blocks_for(net->n, 256 / SP_LANES)), dim3(256), 0, c->stream, a);
     return check_launch(c, "k_skin_pairs");
 }
 
 // Does the step run its broad phase on the side stream?  Only for the filament-only step with the
 // grid on, and not while per-kernel timers (events on the main stream) are recorded.
 bool overlapped(const sbx_net *net) {
-    return net->overlap && net->side && !net->profiling && !net->hasMotors && net->p.gridOn;
+    return net->overlap && net->side && !net->profiling && !net->hasMotors && net->p.gridOn &&
+           net->skinMode == SBX_SKIN_REBUILD;
 }
 
 // Outside a capture: every event whose latest record was captured is recorded again on the main
@@ -1994,7 +2016,7 @@ bool overlapped(const sbx_net *net) {
 int settle_events(sbx_net *net) {
     if (!net->evCaptured || net->graphSteps) return SBX_OK;
     sbx_ctx *c = net->ctx;
-    for (hipEvent_t e : {net->evFork, net->evBuilt, net->evRod[0], net->evRod[1], net->evPlaced[0], net->evPlaced[1],
+    for (hipEvent_t e : {net->evFork, net->evRod[0], net->evRod[1], net->evPlaced[0], net->evPlaced[1],
                          net->evJoin[0], net->evJoin[1]}) {
         if (!e) continue;
         int r = ev_record(net, e, c->stream, "settle events");
@@ -2053,11 +2075,9 @@ int enqueue_bp(sbx_net *net, int b, hipStream_t s2, bool ovl, bool waitPlaced) {
 //   side:  (wait evPlaced[c]) broad phase on CSR[c] -> evJoin[c]
 //   main:  (wait evPlaced[x]: k_rod overwrites the pack, cells and histogram the previous sort read)
 //          unbind -> link -> k_rod -> evRod[c]
-// so the next step's sort runs while this step's broad phase does.  Skin modes, two streams:
-//   side:  (wait evRod[x]) gated skin rebuild -> evBuilt -> pairs -> evJoin[c]
-//   main:  unbind -> link -> (wait evBuilt: the guard reads this step's rebuild) -> k_rod -> evRod[c]
-// (pairs read the packed pose k_rod overwrites a step later; the next rebuild follows them on side).
-// The side streams are forked from the main stream (needFork) at the start of a call, of a capture
+// so the next step's sort runs while this step's broad phase does.  Skin modes run on one stream
+// (pairs -> unbind -> link -> k_rod): the pairs read the skin flags and outside-skin list that the
+// guard in k_rod extends; the host enqueues rebuilds between steps.  The side streams are forked from the main stream (needFork) at the start of a call, of a capture
 // and after a graph replay, so a wait on an event recorded before the fork is only redundant.
 // capK: index of the step in the graph being captured (-1: eager); a capture waits on no event
 // recorded outside it.
@@ -2083,12 +2103,8 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
             if ((r = ev_wait(net, sS, net->evFork, "wait evFork"))) return r;
             net->needFork = 0;
         }
-        if (skinOn) {
-            if (ovl && prev1 && (r = ev_wait(net, s2, net->evRod[xb], "wait evRod")))
-                return r;
-            if ((r = enqueue_skin(net, coord, uVec, s2, ovl))) return r;
-            if (ovl && (r = ev_record(net, net->evJoin[cb], s2, "record evJoin"))) return r;
-            if (net->graphSteps) net->joinLive[cb] = 1;
+        if (skinOn) {  // one stream: the pairs read the flags and list the guard in k_rod writes
+            if ((r = enqueue_skin_pairs(net, coord, uVec))) return r;
         } else {
             if (net->bpPending) {  // the broad phase the previous step left; its sort is ordered before
                 if ((r = enqueue_bp(net, net->bpPendingBuf, s2, ovl, !net->graphSteps))) return r;
@@ -2164,15 +2180,17 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
     a.linkPack = L > 0 ? net->linkPack : nullptr;
     a.skinRef = skinOn ? net->refC : nullptr;
     a.skin = skinOn ? net->skin : nullptr;
-    a.rebuildFlag = skinOn ? net->skinFlags + skf_rebuild(nxt) : nullptr;
-    a.rsExtra = nullptr;
-    a.rsCap = 0;
+    a.rebuildFlag = skinOn ? net->skinFlags + SKF_EXTRA : nullptr;
+    a.rsExtra = skinOn ? net->fsExtra : nullptr;
+    a.rsCap = skinOn ? net->sp.extraCap : 0;
+    a.minImage = skinOn && net->geom.periodic;
     if (net->rsMode) {
         a.skinRef = net->rsRef;
         a.skin = net->rsSkin;
         a.rebuildFlag = net->rsWords + RS_EXTRA;
         a.rsExtra = net->rsExtra;
         a.rsCap = net->rsp.extraCap;
+        a.minImage = 0;
     }
     a.bodyPack = (net->p.gridOn && !net->hasMotors) ? net->bodyPack[nxt] : nullptr;
     a.boundR = net->boundR;
@@ -2188,7 +2206,6 @@ int enqueue_step(sbx_net *net, int stepOff, int capK = -1, bool deferBp = false)
     a.halfY = 0.5 * net->p.boxParams[2] - net->p.boxParams[4];
     a.halfZ = 0.5 * net->p.boxParams[3] - net->p.boxParams[4];
     a.coeff = net->p.boxParams[5];
-    if (ovl && skinOn && (r = ev_wait(net, s, net->evBuilt, "wait evBuilt"))) return r;
     {
         Scope sc(net, T_ROD);
         const long long waves = ((long long)n + CHAIN_SPAN - 1) / CHAIN_SPAN;
@@ -2250,7 +2267,7 @@ void release(sbx_net *net) {
                     net->gridCounts[0], net->gridCounts[1], net->gridSlot[0], net->gridSlot[1],
                     net->reps,     net->rodRep,    net->linkRep,  net->formInts, net->formFloats,
                     net->rsRef,    net->rsRefU,    net->rsSkin,   net->rsCell,  net->rsOff,  net->rsCont,
-                    net->rsExtra,  net->rsWords,   net->rsModel};
+                    net->rsExtra,  net->rsWords,   net->rsModel, net->fsExtra};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (net->rsHost) hipHostFree((void *)net->rsHost);
@@ -2264,7 +2281,7 @@ void release(sbx_net *net) {
         hipEventDestroy(pe.second.second);
     }
     for (hipEvent_t e : net->eventPool) hipEventDestroy(e);
-    for (hipEvent_t e : {net->evFork, net->evBuilt, net->evRod[0], net->evRod[1], net->evPlaced[0], net->evPlaced[1],
+    for (hipEvent_t e : {net->evFork, net->evRod[0], net->evRod[1], net->evPlaced[0], net->evPlaced[1],
                          net->evJoin[0], net->evJoin[1]})
         if (e) hipEventDestroy(e);
     if (net->side) hipStreamDestroy(net->side);
@@ -2468,7 +2485,7 @@ int net_create(sbx_ctx *c, const sbx_filaments *fil, const sbx_crosslinks *xl, c
             // default priority, like the context's stream
             if ((r = check_hip(c, hipStreamCreateWithFlags(&net->side, hipStreamNonBlocking), "side stream"))) break;
             if ((r = check_hip(c, hipStreamCreateWithFlags(&net->sortS, hipStreamNonBlocking), "sort stream"))) break;
-            for (hipEvent_t *e : {&net->evFork, &net->evBuilt, &net->evRod[0], &net->evRod[1], &net->evPlaced[0],
+            for (hipEvent_t *e : {&net->evFork, &net->evRod[0], &net->evRod[1], &net->evPlaced[0],
                                   &net->evPlaced[1], &net->evJoin[0], &net->evJoin[1]})
                 if (!r) r = check_hip(c, hipEventCreateWithFlags(e, hipEventDisableTiming), "event");
             if (r) break;
@@ -2802,7 +2819,7 @@ int step_range(sbx_net *net, int step0, int nSteps) {
                 return r;
         }
         if (net->cur != net->graphCur && nSteps - done > 0) {
-            if ((r = rs_maybe_rebuild(net))) return r;
+            if ((r = rs_maybe_rebuild(net)) || (r = fs_maybe_rebuild(net))) return r;
             if ((r = enqueue_step(net, done - advanced))) return r;
             if ((r = rs_advance(net, 1))) return r;
             done++;
@@ -2814,7 +2831,7 @@ int step_range(sbx_net *net, int step0, int nSteps) {
                 advanced = done;
             }
             if ((r = join_side(net))) return r;
-            if ((r = rs_maybe_rebuild(net))) return r;
+            if ((r = rs_maybe_rebuild(net)) || (r = fs_maybe_rebuild(net))) return r;
             if ((r = check_hip(c, hipGraphLaunch(net->graphExec, c->stream), "graph launch"))) return r;
             if ((r = rs_advance(net, G))) return r;
             net->needFork = 1;  // the replay's side work is joined into it; later side work forks anew
@@ -2828,7 +2845,7 @@ int step_range(sbx_net *net, int step0, int nSteps) {
         }
     }
     for (; done < nSteps; done++) {
-        if ((r = rs_maybe_rebuild(net))) return r;
+        if ((r = rs_maybe_rebuild(net)) || (r = fs_maybe_rebuild(net))) return r;
         if ((r = enqueue_step(net, done - advanced))) return r;
         if ((r = rs_advance(net, 1))) return r;
     }
@@ -2962,10 +2979,7 @@ int sbx_net_publish(sbx_net *net) {
     net->derivedValid = 0;
     net->countsValid = 0;
     net->rsValid = 0;
-    if (!r && net->skinFlags) {
-        r = fill_i32(c, net->skinFlags + skf_rebuild(net->cur), 1, 1);
-        if (!r) r = fill_i32(c, net->skinFlags + skf_rebuild(1 - net->cur), 0, 1);
-    }
+    net->fsValid = 0;
     return r;
 }
 
@@ -3003,9 +3017,10 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
         SBX_REQUIRE(c, net->p.gridOn, "net_set_skin: needs the grid (gridOn)");
         SBX_REQUIRE(c, sp, "net_set_skin: null params");
         SBX_REQUIRE(c, sp->skinDefault > 0.0f && sp->skinMin > 0.0f && sp->skinMax >= sp->skinMin &&
-                           sp->skinDefault <= sp->skinMax && sp->margin > 0.0f && sp->listCap > 0 &&
-                           sp->blockKeys >= 0,
-                    "net_set_skin: need 0 < skinMin <= skinDefault <= skinMax, margin > 0, listCap > 0, blockKeys >= 0");
+                           sp->skinDefault <= sp->skinMax && sp->margin > 0.0f && sp->listCap > 0,
+                    "net_set_skin: need 0 < skinMin <= skinDefault <= skinMax, margin > 0, listCap > 0");
+        SBX_REQUIRE(c, sp->horizon >= 0 && sp->extraTrigger >= 0 && sp->extraCap > 0,
+                    "net_set_skin: need horizon >= 0, extraTrigger >= 0, extraCap > 0");
         SBX_REQUIRE(c, mode != SBX_SKIN_LEARNED || modelWeights, "net_set_skin: learned mode needs model weights");
         SBX_REQUIRE(c, net->n < (1 << 26), "net_set_skin: more than 2^26 bodies");
     }
@@ -3059,10 +3074,11 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
     // (re)allocate the skin state
     void *old[] = {net->cCell, net->cOff,   net->cCont,     net->refC,  net->refU,      net->skin,
                    net->cellSkin, net->cellHorizon, net->vlist, net->vcount, net->skinFlags, net->model,
-                   net->cgDev,  net->cCellsDev};
+                   net->cgDev,  net->cCellsDev, net->fsExtra};
     for (void *b : old)
         if (b) hipFree(b);
     net->cCell = net->cOff = net->cCont = net->vlist = net->vcount = net->skinFlags = net->cCellsDev = nullptr;
+    net->fsExtra = nullptr;
     net->refC = net->refU = net->skin = net->cellSkin = net->cellHorizon = net->model = nullptr;
     net->cgDev = nullptr;
     do {
@@ -3079,6 +3095,8 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
         if ((r = alloc(c, &net->skinFlags, SKF_WORDS))) break;
         if ((r = alloc(c, &net->cgDev, 1))) break;
         if ((r = alloc(c, &net->cCellsDev, 1))) break;
+        if ((r = alloc(c, &net->fsExtra, (size_t)sp->extraCap))) break;
+        if ((r = host_words(net))) break;
         if ((r = fill_i32(c, net->cCellsDev, 0, 1))) break;
         if (mode == SBX_SKIN_LEARNED) {
             if ((r = alloc(c, &net->model, MLP_NPARAM))) break;
@@ -3089,11 +3107,19 @@ int sbx_net_set_skin(sbx_net *net, int mode, const sbx_skin_params *sp, const fl
         long long keys = net->p.gridDims[3] > cc ? (long long)net->p.gridDims[3] : cc;
         if ((r = sort_scratch_reserve(c, net->sort, keys, n))) break;
         if ((r = fill_i32(c, net->skinFlags, 0, SKF_WORDS))) break;
-        if ((r = fill_i32(c, net->skinFlags + skf_rebuild(net->cur), 1, 1))) break;  // the first step builds
         r = sbx_sync(c);
     } while (0);
-    if (r) net->skinMode = SBX_SKIN_REBUILD;
-    return r;
+    if (r) {
+        net->skinMode = SBX_SKIN_REBUILD;
+        return r;
+    }
+    net->rsHost[2] = -1;
+    net->rsHost[3] = 0;
+    net->fsValid = 0;  // the first step builds
+    net->fsGen = 0;
+    net->fsSince = 0;
+    net->fsWhy[0] = net->fsWhy[1] = net->fsWhy[2] = 0;
+    return SBX_OK;
 }
 
 int sbx_net_skin_stats(sbx_net *net, long long *out, int cap) {
@@ -3106,9 +3132,10 @@ int sbx_net_skin_stats(sbx_net *net, long long *out, int cap) {
     int r = check_hip(c, hipMemcpyAsync(f, net->skinFlags, sizeof(f), hipMemcpyDeviceToHost, c->stream), "copy");
     if (!r) r = check_hip(c, hipStreamSynchronize(c->stream), "sync");
     if (r) return r;
-    const int words[] = {skf_rebuild(net->cur), SKF_SINCE, SKF_REBUILDS, SKF_OVERFLOW, SKF_ENTRIES, SKF_STEPS, SKF_HEAVY};
+    const long long v[] = {f[SKF_REBUILDS], f[SKF_EXTRA],    f[SKF_BRUTE], f[SKF_SELECT],
+                           f[SKF_ENTRIES],  f[SKF_OVERFLOW], f[SKF_SINCE], f[SKF_STEPS]};
     int k = 0;
-    for (; k < (int)(sizeof(words) / sizeof(words[0])) && k < cap; k++) out[k] = f[words[k]];
+    for (; k < (int)(sizeof(v) / sizeof(v[0])) && k < cap; k++) out[k] = v[k];
     return k;
 }
 
@@ -3178,14 +3205,7 @@ int sbx_net_set_reach_skin(sbx_net *net, int mode, const sbx_reach_skin_params *
                                             hipMemcpyDeviceToDevice, c->stream), "net_set_reach_skin model");
             if (r) break;
         }
-        if (!net->rsHost) {
-            void *h = nullptr;
-            if ((r = check_hip(c, hipHostMalloc(&h, 2 * sizeof(int), hipHostMallocDefault), "hipHostMalloc"))) break;
-            net->rsHost = (volatile int *)h;
-            void *d = nullptr;
-            if ((r = check_hip(c, hipHostGetDevicePointer(&d, h, 0), "hipHostGetDevicePointer"))) break;
-            net->rsHostDev = (int *)d;
-        }
+        if ((r = host_words(net))) break;
         if ((r = fill_i32(c, net->rsWords, 0, RS_WORDS))) break;
         r = check_hip(c, hipStreamSynchronize(c->stream), "net_set_reach_skin sync");
     } while (0);

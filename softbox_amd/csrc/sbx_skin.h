@@ -6,9 +6,9 @@
 // produced from per-body neighbour lists built at a rebuild with reach + s_i + s_j (s = per-body
 // skin), filtered every step with the exact sphere predicate on the current centres and put in the
 // broad phase's own order (fine-grid stencil position (z, y, x), then ascending index).  A body
-// whose centre moved more than its skin since the rebuild triggers a rebuild before the next
-// step's pairs, so every pair within reach is in the lists: the slices are identical to the
-// per-step rebuild's.
+// whose centre moved more than its skin since the rebuild goes onto an outside-skin list that every
+// body tests, and finds its own candidates from the rebuild's coarse grid, so every pair within
+// reach is found: the slices are identical to the per-step rebuild's.  Rebuilds are the host's.
 #pragma once
 #include "sbx_grid.h"
 
@@ -16,15 +16,17 @@ namespace sbx {
 
 constexpr int SKIN_FEATURES = 12;
 // device flag words of the skin state
-constexpr int SKF_REBUILD = 0;     // 1: rebuild before the pairs of the pose in buffer 0 (guard breach, first step)
-constexpr int SKF_REBUILD1 = 8;    // the same for a pose in buffer 1 (the guard sets the other buffer's word)
+constexpr int SKF_REBUILD = 0;     // 1 while the host's rebuild runs (the rebuild kernels' gate)
 constexpr int SKF_SINCE = 1;       // steps since the last rebuild
 constexpr int SKF_REBUILDS = 2;    // rebuilds so far
 constexpr int SKF_OVERFLOW = 3;    // bodies whose list overflowed at the last rebuild
 constexpr int SKF_ENTRIES = 4;     // list entries at the last rebuild (truncated to 32 bits)
 constexpr int SKF_STEPS = 5;       // steps in skin mode so far
+constexpr int SKF_EXTRA = 8;       // bodies outside their skins since the last rebuild (> extraCap: overflow)
+constexpr int SKF_BRUTE = 9;       // steps in which every body tested every other (outside-skin list overflow)
+constexpr int SKF_SELECT = 10;     // body-steps done by a whole block (more than 64 candidates)
+constexpr int SKF_GEN = 11;        // generation of the last rebuild (the host's count)
 constexpr int SKF_WORDS = 16;
-__host__ __device__ __forceinline__ int skf_rebuild(int buf) { return buf ? SKF_REBUILD1 : SKF_REBUILD; }
 
 // q = c / d (c >= 0, d > 0, q < 2^20) by a float reciprocal and exact integer correction
 __device__ __forceinline__ int div_small(int c, int d, float invD) {

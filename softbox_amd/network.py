@@ -57,8 +57,8 @@ class Network:
     def skin_stats(self) -> dict:
         out = (C.c_longlong * 8)()
         k = self.ctx.check(self.ctx.lib.sbx_net_skin_stats(self.h, out, 8))
-        names = ("rebuild_pending", "steps_since_rebuild", "rebuilds", "overflow_bodies", "list_entries",
-                 "skin_steps", "block_sorted")
+        names = ("rebuilds", "outside_skin", "brute_steps", "block_bodies", "list_entries", "overflow_bodies",
+                 "steps_since_rebuild", "skin_steps")
         return {names[i]: int(out[i]) for i in range(min(k, len(names)))}
 
     def set_reach_skin(self, mode: int, cfg=None, model=None) -> None:

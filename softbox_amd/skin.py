@@ -30,13 +30,14 @@ class SkinConfig:
     skin_max: float = 0.3
     margin: float = 1.0           # learned skin = margin * exp(out0) (set by calibrate_margin)
     list_cap: int = 256           # neighbour slots per body (bundles are ~10x denser than average)
-    horizon: int = 8              # steps a skin is trained to last
+    horizon: int = 8              # steps a skin is trained to last; the run rebuilds that often
     window: int = 32              # steps tracked after a snapshot (censoring of the horizon label)
-    block_keys: int = 0           # keys a block buffers for one dense body (0: the library's maximum)
+    extra_trigger: int = 64       # bodies outside their skins (lagged) that schedule a rebuild
+    extra_cap: int = 8192         # capacity of the outside-skin list
 
     def params(self) -> _lib.SbxSkinParams:
         return _lib.SbxSkinParams(self.skin_default, self.skin_min, self.skin_max, self.margin, self.list_cap,
-                                  self.block_keys)
+                                  self.horizon, self.extra_trigger, self.extra_cap)
 
 
 @dataclass

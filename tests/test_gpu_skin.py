@@ -2,8 +2,10 @@
 
 1. The skin model's HIP forward, loss gradient and Adam step against the numpy oracle.
 2. The skin broad phase (fixed and learned skins, periodic and box grids, list overflow, slice
-   overflow, graph replay) produces candidate slices and counts identical to the per-step rebuild
-   at every step, and the same trajectory.
+   overflow, bodies outside their skins found through the outside-skin list and the coarse walk, a
+   list that overflows (every pair tested), scheduled and count-triggered rebuilds, graph replay)
+   produces candidate slices and counts identical to the per-step rebuild at every step, and the
+   same trajectory.
 3. Training on the per-step-rebuild trajectory lowers the loss; on a scene whose mobility differs
    between regions the trained skins follow the mobility and last the horizon they were calibrated
    for.
@@ -109,8 +111,29 @@ def _run_pair(ctx, sc, mode, cfg, steps, model=None, graph=False, check_every=1)
 def test_fixed_skin_slices_identical_to_rebuild(ctx, periodic):
     sc = small_network(seed=21, n_fil=300, links=600, periodic=periodic)
     stats = _run_pair(ctx, sc, _lib.SBX_SKIN_FIXED, SkinConfig(skin_default=0.15, list_cap=64), steps=40)
-    assert 1 < stats["rebuilds"] < 40, stats          # rebuilds were skipped and the guard fired
+    assert 1 < stats["rebuilds"] < 40, stats          # rebuilds every 8 steps (the horizon), not every step
     assert stats["skin_steps"] == 40
+
+
+@pytest.mark.parametrize("periodic,cap", [(True, 1 << 16), (False, 1 << 16), (True, 2)])
+def test_bodies_outside_their_skins_are_still_found(ctx, periodic, cap):
+    """Skins of 0.1 nm and no scheduled rebuild: bodies leave their skins every step; their partners find
+    them on the outside-skin list and they find theirs by walking the coarse grid.  With a list of 2
+    entries it overflows at once and every body tests every other."""
+    sc = small_network(seed=26, n_fil=300, links=600, periodic=periodic)
+    cfg = SkinConfig(skin_default=1.0e-4, skin_min=1.0e-4, horizon=0, extra_trigger=1 << 30, extra_cap=cap)
+    stats = _run_pair(ctx, sc, _lib.SBX_SKIN_FIXED, cfg, steps=12, check_every=3)
+    assert stats["outside_skin"] > 100, stats
+    assert (stats["brute_steps"] > 0) == (cap == 2), stats
+
+
+def test_rebuilds_triggered_by_the_outside_count(ctx):
+    """No horizon: the host rebuilds once the latest finished step reported more than 8 bodies outside
+    their skins (read two replays late), captured steps."""
+    sc = small_network(seed=27, n_fil=300, links=600)
+    cfg = SkinConfig(skin_default=2.0e-3, skin_min=1.0e-3, horizon=0, extra_trigger=8)
+    stats = _run_pair(ctx, sc, _lib.SBX_SKIN_FIXED, cfg, steps=40, graph=True, check_every=20)
+    assert stats["rebuilds"] >= 3, stats
 
 
 def test_list_and_slice_overflow_stay_exact(ctx):
@@ -120,17 +143,17 @@ def test_list_and_slice_overflow_stay_exact(ctx):
     assert stats["overflow_bodies"] > 0, stats
 
 
-@pytest.mark.parametrize("max_cand,block_keys", [(256, 0), (36, 0), (256, 16), (36, 16)])
-def test_dense_bodies_block_sorted_stay_exact(ctx, max_cand, block_keys):
-    """Bodies with more than 64 candidates are done by the whole block; maxCand 36 keeps only the
-    first 36 of more; block_keys 16 sends them through the chunked keep-smallest reduction."""
+@pytest.mark.parametrize("max_cand", [256, 36])
+def test_dense_bodies_block_ranked_stay_exact(ctx, max_cand):
+    """Bodies with more than 64 candidates (more than a lane group buffers) are ranked by the whole block;
+    maxCand 36 keeps only the first 36 of more."""
     from softbox_amd import scenes
 
     sc = scenes.network(n_fil=400, segs=10, box=1.0, n_links=600, seed=25, periodic=False)
     sc.p["maxCand"] = max_cand
-    cfg = SkinConfig(skin_default=0.15, block_keys=block_keys)
+    cfg = SkinConfig(skin_default=0.15)
     stats = _run_pair(ctx, sc, _lib.SBX_SKIN_FIXED, cfg, steps=12)
-    assert stats["block_sorted"] > 0, stats
+    assert stats["block_bodies"] > 0, stats
 
 
 def test_skin_graph_replay_identical(ctx):
@@ -185,7 +208,7 @@ def test_learned_skins_follow_mobility_and_last_their_horizon(ctx):
     sc = _hetero_scene(31, eta=0.1)
     n = sc.fs.n
     hot = _hot(sc)
-    cfg = SkinConfig(horizon=8, window=32, skin_default=0.04)
+    cfg = SkinConfig(horizon=8, window=32, skin_default=0.04, skin_min=0.002)  # no clamp at the cold skins
     net = Network(ctx, sc.copy())
     model, hist, _, _ = fit(ctx, net, cfg, snapshots=8, iters=300)
     net.close()
@@ -197,8 +220,9 @@ def test_learned_skins_follow_mobility_and_last_their_horizon(ctx):
         net.step(t, 1)
         if net.skin_stats()["rebuilds"] >= 2:  # skins from the model (the first build uses the default)
             sk = _device_floats(ctx, net.skin_state()["skin"], n)
-            ratios.append(sk[hot].mean() / sk[~hot].mean())
-            means.append(sk.mean())
+            inside = sk >= 0.0  # a body that left its skin is marked -1 until the next rebuild
+            ratios.append(sk[hot & inside].mean() / sk[~hot & inside].mean())
+            means.append(sk[inside].mean())
     st_l = net.skin_stats()
     net.close()
     assert len(means) > steps // 2, st_l
@@ -210,6 +234,6 @@ def test_learned_skins_follow_mobility_and_last_their_horizon(ctx):
     st_f = net.skin_stats()
     net.close()
     print(f"hot/cold learned skin {ratio:.3f}; mean skin {np.mean(means):.4f} um; rebuilds learned {st_l['rebuilds']} "
-          f"fixed {st_f['rebuilds']}")
+          f"fixed {st_f['rebuilds']}; outside-skin bodies learned {st_l['outside_skin']} fixed {st_f['outside_skin']}")
     assert ratio > 1.3, ratio
     assert st_l["rebuilds"] <= 2 * steps // cfg.horizon, st_l
