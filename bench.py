@@ -189,8 +189,8 @@ def main() -> int:
     ap.add_argument("--reach-skin", type=float, default=0.04,
                     help="actomyosin: the fixed reach skin, um (and the learned mode's first build)")
     ap.add_argument("--reach-horizon", type=int, default=0,
-                    help="actomyosin: steps between scheduled reach-skin rebuilds (0: fixed mode none, learned mode "
-                         "its training horizon, 50)")
+                    help="actomyosin: steps between scheduled reach-skin rebuilds (0: only on stale state and the "
+                         "outside-skin count; the learned model sizes its skins for 50 steps)")
     ap.add_argument("--graph", type=int, default=1, help="1: replay the step as a captured hipGraph")
     ap.add_argument("--overlap", type=int, default=1,
                     help="1: broad phase on a second stream, concurrent with the mechanics kernels")
@@ -540,10 +540,11 @@ def run_actomyosin(args, log, emit: bool = True):
     if "learned" in modes:
         t_tr = time.perf_counter()
         tnet = Network(ctx, sc.copy(), keep_forces=False)
-        model, hist, (x, _), _ = fit_reach(ctx, tnet, rcfg, snapshots=args.train_snapshots, iters=args.train_iters)
+        model, hist, (x, _), _ = fit_reach(ctx, tnet, rcfg, snapshots=args.train_snapshots, iters=args.train_iters,
+                                           horizon=50)
         tnet.close()
         train_info = {"samples": int(x.shape[0]), "loss_start": round(hist[0], 4), "loss_end": round(hist[-1], 4),
-                      "margin": round(rcfg.margin, 3), "horizon": rcfg.horizon,
+                      "margin": round(rcfg.margin, 3), "trained_horizon": 50, "schedule_horizon": rcfg.horizon,
                       "seconds": round(time.perf_counter() - t_tr, 1)}
         log(f"reach skin model trained: {train_info}")
     runs, digests = {}, {}

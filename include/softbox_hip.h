@@ -563,9 +563,9 @@ typedef struct sbx_reach_skin_params {
 } sbx_reach_skin_params;
 /* mode SBX_SKIN_REBUILD turns it off; SBX_SKIN_FIXED: the coarse cell from skinDefault; SBX_SKIN_LEARNED: the
  * model predicts each (sampled) cell's skin over the previous build's cells, and the coarse cell takes the
- * smallest skin that all but about extraTrigger / 2 segments are predicted to stay within (modelWeights: device
- * array of sbx_mlp_nparam() floats, copied).  Every segment's skin is what the cell leaves it.  Needs motors
- * attached. */
+ * smallest skin that leaves at most extraTrigger segments predicted to outrun it (modelWeights: device array of
+ * sbx_mlp_nparam() floats, copied); a rebuild after turnover or a caller's edit keeps the model's last choice.
+ * Every segment's skin is what the cell leaves it.  Needs motors attached. */
 int sbx_net_set_reach_skin(sbx_net *net, int mode, const sbx_reach_skin_params *p, const float *modelWeights);
 /* -> out[0..cap): rebuilds (device), segments outside their skins now, steps whose heads scanned every
  * segment, heads placed by the selection path (more than 4 hits on one lane or more than maxCand hits),

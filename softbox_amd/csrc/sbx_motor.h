@@ -393,6 +393,74 @@ struct DimerParams {
     double dt, fracMove, leverTorq, leverDeg, rodLen;
 };
 
+// DimerCouplingSystem.couple, one end pairing (DimerCouplingSystem.java:94-160): variant 0 couples rod A's
+// end1 side (endSel -1), variant 1 its end2 side; rod B's end follows the parallel flag.  FA is the force on
+// rod A (rod B takes -FA), TA and TB the torques on rods A and B; false when the two ends coincide (no force).
+__device__ __forceinline__ bool dimer_variant(int variant, int mA, int mB, int par, int nB,
+                                              const float *__restrict__ coord, const float *__restrict__ uVec,
+                                              const float *__restrict__ segLength, const float *__restrict__ bTG,
+                                              const float *__restrict__ bRG, const DimerParams &dp, double FA[3],
+                                              double TA[3], double TB[3]) {
+    int rodA = 3 * mA, rodB = 3 * mB;
+    double endSelA = (variant == 0) ? -1.0 : 1.0;
+    double endSelB = par ? endSelA : -endSelA;
+    double aCx = coord[rodA], aCy = coord[nB + rodA], aCz = coord[2 * nB + rodA];
+    double aUx = uVec[rodA], aUy = uVec[nB + rodA], aUz = uVec[2 * nB + rodA];
+    double bCx = coord[rodB], bCy = coord[nB + rodB], bCz = coord[2 * nB + rodB];
+    double bUx = uVec[rodB], bUy = uVec[nB + rodB], bUz = uVec[2 * nB + rodB];
+    double aLen = segLength[rodA], bLen = segLength[rodB];
+    double pAx = aCx + endSelA * 0.5 * aLen * aUx, pAy = aCy + endSelA * 0.5 * aLen * aUy,
+           pAz = aCz + endSelA * 0.5 * aLen * aUz;
+    double pBx = bCx + endSelB * 0.5 * bLen * bUx, pBy = bCy + endSelB * 0.5 * bLen * bUy,
+           pBz = bCz + endSelB * 0.5 * bLen * bUz;
+    double dx = pBx - pAx, dy = pBy - pAy, dz = pBz - pAz;
+    double strain = sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(strain > 0.0)) return false;
+    double inv = 1.0 / strain;
+    double lx = dx * inv, ly = dy * inv, lz = dz * inv;
+    double mcA = move_c(aUx, aUy, aUz, lx, ly, lz, bTG[rodA], bTG[nB + rodA], bRG[nB + rodA], aLen);
+    double mcB = move_c(bUx, bUy, bUz, lx, ly, lz, bTG[rodB], bTG[nB + rodB], bRG[nB + rodB], bLen);
+    double denom = dp.dt * (mcA + mcB);
+    double forceMag = (denom > 0.0) ? (dp.fracMove * 1.0e-6 * strain / denom) : 0.0;
+    double FAx = forceMag * lx, FAy = forceMag * ly, FAz = forceMag * lz;
+    double RsA = 0.5e-6 * dp.rodLen * endSelA;
+    TA[0] = (RsA * aUy) * FAz - (RsA * aUz) * FAy;
+    TA[1] = (RsA * aUz) * FAx - (RsA * aUx) * FAz;
+    TA[2] = (RsA * aUx) * FAy - (RsA * aUy) * FAx;
+    double RsB = 0.5e-6 * dp.rodLen * endSelB;
+    TB[0] = (RsB * bUy) * (-FAz) - (RsB * bUz) * (-FAy);
+    TB[1] = (RsB * bUz) * (-FAx) - (RsB * bUx) * (-FAz);
+    TB[2] = (RsB * bUx) * (-FAy) - (RsB * bUy) * (-FAx);
+    FA[0] = FAx;
+    FA[1] = FAy;
+    FA[2] = FAz;
+    return true;
+}
+
+// The levers' torsion of a parallel dimer whose heads are not both bound (DimerCouplingSystem.java:162-182):
+// the torque on lever A (lever B takes its negative); false when the levers are parallel.
+__device__ __forceinline__ bool dimer_lever(int mA, int mB, int nB, const float *__restrict__ uVec,
+                                            const float *__restrict__ bRG, const DimerParams &dp, double TL[3]) {
+    int leverA = 3 * mA + 1, leverB = 3 * mB + 1;
+    double u1x = uVec[leverA], u1y = uVec[nB + leverA], u1z = uVec[2 * nB + leverA];
+    double u2x = uVec[leverB], u2y = uVec[nB + leverB], u2z = uVec[2 * nB + leverB];
+    double tvx = u1y * u2z - u1z * u2y, tvy = u1z * u2x - u1x * u2z, tvz = u1x * u2y - u1y * u2x;
+    double tvm2 = tvx * tvx + tvy * tvy + tvz * tvz;
+    if (!(tvm2 > 1.0e-30)) return false;
+    double im = 1.0 / sqrt(tvm2);
+    tvx *= im; tvy *= im; tvz *= im;
+    double dot = u1x * u2x + u1y * u2y + u1z * u2z;
+    if (dot > 1.0) dot = 1.0;
+    if (dot < -1.0) dot = -1.0;
+    double ang = accurate_acos(dot) * 180.0 / SBX_PI_D;
+    double invBRG = 1.0 / bRG[nB + leverA] + 1.0 / bRG[nB + leverB];
+    double torsionMag = dp.leverTorq * (SBX_PI_D / 180.0) * (ang - dp.leverDeg) / (invBRG * dp.dt);
+    TL[0] = tvx * torsionMag;
+    TL[1] = tvy * torsionMag;
+    TL[2] = tvz * torsionMag;
+    return true;
+}
+
 // DimerCouplingSystem.couple body for one dimer (DimerCouplingSystem.java:94-182): read-modify-write
 // of the two rods' and two levers' accumulators (the dimer owns its two motors).
 __device__ __forceinline__ void dimer_one(int mA, int mB, int par, bool bothBound, int nB,
@@ -402,71 +470,22 @@ __device__ __forceinline__ void dimer_one(int mA, int mB, int par, bool bothBoun
                                           const DimerParams &dp) {
     int rodA = 3 * mA, rodB = 3 * mB;
     for (int variant = 0; variant < 2; variant++) {
-        double endSelA = (variant == 0) ? -1.0 : 1.0;
-        double endSelB = par ? endSelA : -endSelA;
-        double aCx = coord[rodA], aCy = coord[nB + rodA], aCz = coord[2 * nB + rodA];
-        double aUx = uVec[rodA], aUy = uVec[nB + rodA], aUz = uVec[2 * nB + rodA];
-        double bCx = coord[rodB], bCy = coord[nB + rodB], bCz = coord[2 * nB + rodB];
-        double bUx = uVec[rodB], bUy = uVec[nB + rodB], bUz = uVec[2 * nB + rodB];
-        double aLen = segLength[rodA], bLen = segLength[rodB];
-        double pAx = aCx + endSelA * 0.5 * aLen * aUx, pAy = aCy + endSelA * 0.5 * aLen * aUy,
-               pAz = aCz + endSelA * 0.5 * aLen * aUz;
-        double pBx = bCx + endSelB * 0.5 * bLen * bUx, pBy = bCy + endSelB * 0.5 * bLen * bUy,
-               pBz = bCz + endSelB * 0.5 * bLen * bUz;
-        double dx = pBx - pAx, dy = pBy - pAy, dz = pBz - pAz;
-        double strain = sqrt(dx * dx + dy * dy + dz * dz);
-        if (strain > 0.0) {
-            double inv = 1.0 / strain;
-            double lx = dx * inv, ly = dy * inv, lz = dz * inv;
-            double mcA = move_c(aUx, aUy, aUz, lx, ly, lz, bTG[rodA], bTG[nB + rodA], bRG[nB + rodA], aLen);
-            double mcB = move_c(bUx, bUy, bUz, lx, ly, lz, bTG[rodB], bTG[nB + rodB], bRG[nB + rodB], bLen);
-            double denom = dp.dt * (mcA + mcB);
-            double forceMag = (denom > 0.0) ? (dp.fracMove * 1.0e-6 * strain / denom) : 0.0;
-            double FAx = forceMag * lx, FAy = forceMag * ly, FAz = forceMag * lz;
-            double RsA = 0.5e-6 * dp.rodLen * endSelA;
-            double TAx = (RsA * aUy) * FAz - (RsA * aUz) * FAy;
-            double TAy = (RsA * aUz) * FAx - (RsA * aUx) * FAz;
-            double TAz = (RsA * aUx) * FAy - (RsA * aUy) * FAx;
-            F[rodA] = (float)(F[rodA] + FAx);
-            F[nB + rodA] = (float)(F[nB + rodA] + FAy);
-            F[2 * nB + rodA] = (float)(F[2 * nB + rodA] + FAz);
-            T[rodA] = (float)(T[rodA] + TAx);
-            T[nB + rodA] = (float)(T[nB + rodA] + TAy);
-            T[2 * nB + rodA] = (float)(T[2 * nB + rodA] + TAz);
-            double RsB = 0.5e-6 * dp.rodLen * endSelB;
-            double TBx = (RsB * bUy) * (-FAz) - (RsB * bUz) * (-FAy);
-            double TBy = (RsB * bUz) * (-FAx) - (RsB * bUx) * (-FAz);
-            double TBz = (RsB * bUx) * (-FAy) - (RsB * bUy) * (-FAx);
-            F[rodB] = (float)(F[rodB] - FAx);
-            F[nB + rodB] = (float)(F[nB + rodB] - FAy);
-            F[2 * nB + rodB] = (float)(F[2 * nB + rodB] - FAz);
-            T[rodB] = (float)(T[rodB] + TBx);
-            T[nB + rodB] = (float)(T[nB + rodB] + TBy);
-            T[2 * nB + rodB] = (float)(T[2 * nB + rodB] + TBz);
+        double FA[3], TA[3], TB[3];
+        if (!dimer_variant(variant, mA, mB, par, nB, coord, uVec, segLength, bTG, bRG, dp, FA, TA, TB)) continue;
+        for (int a = 0; a < 3; a++) {
+            F[a * nB + rodA] = (float)(F[a * nB + rodA] + FA[a]);
+            T[a * nB + rodA] = (float)(T[a * nB + rodA] + TA[a]);
+        }
+        for (int a = 0; a < 3; a++) {
+            F[a * nB + rodB] = (float)(F[a * nB + rodB] - FA[a]);
+            T[a * nB + rodB] = (float)(T[a * nB + rodB] + TB[a]);
         }
     }
-    if (par && !bothBound) {
+    double TL[3];
+    if (par && !bothBound && dimer_lever(mA, mB, nB, uVec, bRG, dp, TL)) {
         int leverA = 3 * mA + 1, leverB = 3 * mB + 1;
-        double u1x = uVec[leverA], u1y = uVec[nB + leverA], u1z = uVec[2 * nB + leverA];
-        double u2x = uVec[leverB], u2y = uVec[nB + leverB], u2z = uVec[2 * nB + leverB];
-        double tvx = u1y * u2z - u1z * u2y, tvy = u1z * u2x - u1x * u2z, tvz = u1x * u2y - u1y * u2x;
-        double tvm2 = tvx * tvx + tvy * tvy + tvz * tvz;
-        if (tvm2 > 1.0e-30) {
-            double im = 1.0 / sqrt(tvm2);
-            tvx *= im; tvy *= im; tvz *= im;
-            double dot = u1x * u2x + u1y * u2y + u1z * u2z;
-            if (dot > 1.0) dot = 1.0;
-            if (dot < -1.0) dot = -1.0;
-            double ang = accurate_acos(dot) * 180.0 / SBX_PI_D;
-            double invBRG = 1.0 / bRG[nB + leverA] + 1.0 / bRG[nB + leverB];
-            double torsionMag = dp.leverTorq * (SBX_PI_D / 180.0) * (ang - dp.leverDeg) / (invBRG * dp.dt);
-            T[leverA] = (float)(T[leverA] + tvx * torsionMag);
-            T[nB + leverA] = (float)(T[nB + leverA] + tvy * torsionMag);
-            T[2 * nB + leverA] = (float)(T[2 * nB + leverA] + tvz * torsionMag);
-            T[leverB] = (float)(T[leverB] - tvx * torsionMag);
-            T[nB + leverB] = (float)(T[nB + leverB] - tvy * torsionMag);
-            T[2 * nB + leverB] = (float)(T[2 * nB + leverB] - tvz * torsionMag);
-        }
+        for (int a = 0; a < 3; a++) T[a * nB + leverA] = (float)(T[a * nB + leverA] + TL[a]);
+        for (int a = 0; a < 3; a++) T[a * nB + leverB] = (float)(T[a * nB + leverB] - TL[a]);
     }
 }
 

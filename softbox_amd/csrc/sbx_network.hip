@@ -84,6 +84,7 @@ struct sbx_net {
     // and clears them (CrossBridgeSystem.csrBuild's inverse, CrossBridgeSystem.java:189-298, without the sort)
     int *segHeadCnt = nullptr, *segHeads = nullptr;
     int *bbOff = nullptr, *bbList = nullptr;           // backbone -> tethered dimers CSR (static)
+    int *dimerOf = nullptr;                            // motor -> its dimer or -1 (static)
     KinParams kp{};
     NucParams np{};
     XbParams xp{};
@@ -763,6 +764,7 @@ constexpr int RS_MAXSKIN = 2;    // learned: 1 once the model has run for the re
 constexpr int RS_GEN = 3;        // generation of the last rebuild (the host's count)
 constexpr int RS_BRUTE = 4;      // steps whose heads scanned every segment (outside-skin list overflow)
 constexpr int RS_SELECT = 5;     // heads placed by the selection path
+constexpr int RS_LASTSKIN = 6;   // learned: the skin the model's last evaluation chose (float bits)
 constexpr int RS_GEOM = 8;       // 8 words: xMin, yMin, zMin, cell, invCell (float bits), nX, nY, nZ
 constexpr int RS_HIST = 16;      // learned mode: segments per predicted-skin bin (log bins over [skinMin, skinMax])
 constexpr int RS_HBINS = 64;
@@ -968,7 +970,8 @@ __global__ void __launch_bounds__(256) k_motor_bind_skin(MotorArgs a, ReachArgs 
 // A rebuild (enqueued by the host between steps): [learned: per-cell model skins] -> geometry -> bin -> sort.
 struct ReachBuild {
     int n, extraCap, learned, gen, sampleStride;
-    long long budget;                // learned: segments the cell may leave outside their predicted skins
+    int keepLearned;                 // learned mode, model not run: keep its last skin
+    int trigger;                     // the outside-skin count that schedules a rebuild
     float since;                     // steps since the previous build (learned features)
     const float *coord, *uVec, *segLength;
     const int *links;                // activeLinkCount (learned features; may be null)
@@ -1004,21 +1007,32 @@ __global__ void __launch_bounds__(256) k_rs_assign(ReachBuild b) {
 }
 
 // One thread: this build's coarse grid (cell = the reach of the longest segment + the skin, at least the floor)
-// over the binding grid's extent; opens the outside-skin list.  Learned mode: the skin is the smallest bin edge
-// that the model predicts all but about `budget` segments (scaled from the sampled cells) to stay within over
-// the horizon; those few will leave their skins and be found on the outside-skin list.
+// over the binding grid's extent; opens the outside-skin list.  Learned mode: the smallest skin (histogram bin
+// edge) whose cell leaves at most extraTrigger segments predicted to outrun their skins over the horizon (scaled
+// from the sampled cells); those few go on the outside-skin list without bringing the next rebuild forward.  So
+// the model grows the cell above the floor only where it predicts more motion than the floor cell's skins hold.
 __global__ void k_rs_geom(ReachBuild b) {
     if (threadIdx.x != 0) return;
     float maxSkin = b.skinDefault;
+    const float last = __int_as_float(b.words[RS_LASTSKIN]);
+    if (b.keepLearned && last > 0.0f) maxSkin = last;  // a stale rebuild keeps the model's last choice
     if (b.learned && b.words[RS_MAXSKIN] > 0) {
-        long long above = 0;
-        int bin = RS_HBINS - 1;
-        for (; bin > 0; bin--) {  // from the top: stop where the segments above would pass the budget
-            above += (long long)b.words[RS_HIST + bin] * b.sampleStride;
-            if (above > b.budget) break;
+        maxSkin = b.skinMax;
+        for (int bin = 0; bin < RS_HBINS; bin++) {
+            const float sk = b.skinMin * powf(b.skinMax / b.skinMin, (float)(bin + 1) / (float)RS_HBINS);
+            const float cell = fmaxf(b.floorCell, b.R0 + sk + 2.0f * RS_SLACK);
+            const float allow = cell - b.R0 - 2.0f * RS_SLACK;  // the skin the cell leaves a segment
+            long long above = 0;  // segments predicted past it: bins whose lower edge is at or above it
+            for (int k = 0; k < RS_HBINS; k++) {
+                const float lo = b.skinMin * powf(b.skinMax / b.skinMin, (float)k / (float)RS_HBINS);
+                if (lo >= allow) above += (long long)b.words[RS_HIST + k] * b.sampleStride;
+            }
+            if (above <= b.trigger) {
+                maxSkin = fminf(b.skinMax, sk);
+                break;
+            }
         }
-        maxSkin = b.skinMin * powf(b.skinMax / b.skinMin, (float)(bin + 1) / (float)RS_HBINS);
-        maxSkin = fminf(b.skinMax, maxSkin);
+        b.words[RS_LASTSKIN] = __float_as_int(maxSkin);
     }
     b.words[RS_MAXSKIN] = 0;
     for (int k = 0; k < RS_HBINS; k++) b.words[RS_HIST + k] = 0;
@@ -1071,27 +1085,64 @@ __global__ void __launch_bounds__(256) k_rs_bin(ReachBuild b) {
 }
 
 // zero + MotorJointSystem.joints for one sub-body: F = (float)(0 + joints)
-__global__ void __launch_bounds__(256) k_motor_joints0(sbx_motors m, JointParams jp) {
+// ... and DimerCouplingSystem.couple (DimerCouplingSystem.java:83-184) for a dimer's rods and levers: each of those
+// sub-bodies' threads evaluates the coupling terms that act on it (from the step-start pose, which the joints
+// also read) and adds them after its joints, in the reference's order (variant 0, variant 1, the levers'
+// torsion), so k_net_dimer's read-modify-write pass is not needed.  dimerOf[m]: the dimer of motor m or -1.
+__global__ void __launch_bounds__(256) k_motor_joints0(sbx_motors m, JointParams jp, DimerParams dp,
+                                                       const int *__restrict__ dimerOf) {
     int s = blockIdx.x * 256 + threadIdx.x;
     int nB = 3 * m.nM;
     if (s >= nB) return;
     double f[3] = {0, 0, 0}, t[3] = {0, 0, 0};
     joints_one(s, nB, m.coord, m.uVec, m.segLength, m.bTransGam, m.bRotGam, m.nucleotideState[s / 3], jp, f, t);
+    float F[3], T[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        m.forceSum[a * nB + s] = (float)(0.0f + f[a]);
-        m.torqueSum[a * nB + s] = (float)(0.0f + t[a]);
+        F[a] = (float)(0.0f + f[a]);
+        T[a] = (float)(0.0f + t[a]);
+    }
+    const int mm = s / 3, role = s - 3 * mm;
+    const int d = (dimerOf && role < 2) ? dimerOf[mm] : -1;
+    if (d >= 0) {
+        const int mA = m.motorA[d], mB = m.motorB[d], par = m.parallel[d] != 0;
+        const bool sideA = mm == mA;
+        if (role == 0) {
+            for (int v = 0; v < 2; v++) {
+                double FA[3], TA[3], TB[3];
+                if (!dimer_variant(v, mA, mB, par, nB, m.coord, m.uVec, m.segLength, m.bTransGam, m.bRotGam, dp, FA,
+                                   TA, TB))
+                    continue;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    F[a] = sideA ? (float)(F[a] + FA[a]) : (float)(F[a] - FA[a]);
+                    T[a] = (float)(T[a] + (sideA ? TA[a] : TB[a]));
+                }
+            }
+        } else if (par) {
+            double TL[3];
+            const bool both = m.boundSeg[mA] >= 0 && m.boundSeg[mB] >= 0;
+            if (!both && dimer_lever(mA, mB, nB, m.uVec, m.bRotGam, dp, TL)) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) T[a] = sideA ? (float)(T[a] + TL[a]) : (float)(T[a] - TL[a]);
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        m.forceSum[a * nB + s] = F[a];
+        m.torqueSum[a * nB + s] = T[a];
     }
 }
 
-__global__ void __launch_bounds__(256) k_net_dimer(sbx_motors m, DimerParams dp) {
+__global__ void __launch_bounds__(256) k_dimer_of(const int *__restrict__ motorA, const int *__restrict__ motorB,
+                                                  int nD, int *__restrict__ dimerOf) {
     int d = blockIdx.x * 256 + threadIdx.x;
-    if (d >= m.nD) return;
-    int mA = m.motorA[d], mB = m.motorB[d];
-    bool both = m.boundSeg[mA] >= 0 && m.boundSeg[mB] >= 0;
-    dimer_one(mA, mB, m.parallel[d] != 0, both, 3 * m.nM, m.coord, m.uVec, m.segLength, m.bTransGam, m.bRotGam,
-              m.forceSum, m.torqueSum, dp);
+    if (d >= nD) return;
+    dimerOf[motorA[d]] = d;
+    dimerOf[motorB[d]] = d;
 }
+
 
 // the tether of each attachment: MiniFilamentSystem.tether, or with nodeMode NodeSystem.tether (1) /
 // tetherBoundStiffen (2, boundCoeff in miniParams[1]) onto node spheres in the backbone slots
@@ -1760,8 +1811,11 @@ int rs_rebuild(sbx_net *net, int why) {
     ReachBuild b{};
     b.n = n; b.extraCap = net->rsp.extraCap;
     b.sampleStride = n > RS_MODEL_CELLS ? (n + RS_MODEL_CELLS - 1) / RS_MODEL_CELLS : 1;
-    b.budget = net->rsp.extraTrigger / 2;
-    b.learned = net->rsMode == SBX_SKIN_LEARNED && net->rsGen > 0 && net->rsModel;
+    // the model reads each segment's motion since the previous build: not after turnover or a caller's edit
+    // (why 0), where segments were born, moved or resized; those rebuilds keep its last skin
+    b.learned = net->rsMode == SBX_SKIN_LEARNED && net->rsGen > 0 && net->rsModel && why != 0;
+    b.keepLearned = net->rsMode == SBX_SKIN_LEARNED && !b.learned;
+    b.trigger = net->rsp.extraTrigger;
     b.gen = net->rsGen + 1;
     b.since = (float)(net->rsSince > 0 ? net->rsSince : 1);
     b.coord = net->coordB[net->cur]; b.uVec = net->uB[net->cur]; b.segLength = net->fil.segLength;
@@ -1888,11 +1942,9 @@ int enqueue_myosin(sbx_net *net, const float *coord, const float *uVec, const fl
     }
     {
         Scope sc(net, T_MJOINT);
-        hipLaunchKernelGGL(k_motor_joints0, dim3(blocks_for(nB, 256)), dim3(256), 0, s, m, net->jp);
-    }
-    if (m.nD > 0) {
-        Scope sc(net, T_DIMER);
-        hipLaunchKernelGGL(k_net_dimer, dim3(blocks_for(m.nD, 256)), dim3(256), 0, s, m, net->dp);
+        // with the dimer coupling of each dimer's rods and levers
+        hipLaunchKernelGGL(k_motor_joints0, dim3(blocks_for(nB, 256)), dim3(256), 0, s, m, net->jp, net->dp,
+                           m.nD > 0 ? net->dimerOf : nullptr);
     }
     if (m.nMD > 0) {
         Scope sc(net, T_TETHER);
@@ -2259,7 +2311,7 @@ void release(sbx_net *net) {
                     net->cellOffsets,
                     net->candPartner, net->candCount, net->offA,  net->idxA,        net->offB,
                     net->idxB,     net->dStep,     net->ownerStore, net->ownerSlot,
-                    net->segHeadCnt, net->segHeads, net->bbOff,  net->bbList,
+                    net->segHeadCnt, net->segHeads, net->bbOff,  net->bbList, net->dimerOf,
                     net->cCell,    net->cOff,      net->cCont,    net->refC,    net->refU, net->skin,
                     net->cellSkin, net->cellHorizon, net->vlist,  net->vcount,  net->skinFlags, net->model,
                     net->cgDev,    net->cCellsDev, net->bodyPack[0], net->bodyPack[1], net->sortedPose[0],
@@ -2602,7 +2654,7 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
     const int cap = n + nM;
     // the grid now bins filaments + heads: regrow the per-body grid arrays
     int *bodyCell = nullptr, *cellContents = nullptr, *ownerStore = nullptr, *ownerSlot = nullptr,
-        *segHeadCnt = nullptr, *segHeads = nullptr, *bbOff = nullptr, *bbList = nullptr;
+        *segHeadCnt = nullptr, *segHeads = nullptr, *bbOff = nullptr, *bbList = nullptr, *dimerOf = nullptr;
     do {
         if ((r = alloc(c, &bodyCell, cap))) break;
         if ((r = alloc(c, &cellContents, cap))) break;
@@ -2613,13 +2665,20 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
         if ((r = fill_i32(c, segHeadCnt, 0, n))) break;  // k_rod keeps it zero between steps
         if ((r = alloc(c, &bbOff, (size_t)m->nBb + 1))) break;
         if ((r = alloc(c, &bbList, m->nMD))) break;
+        if ((r = alloc(c, &dimerOf, nM))) break;
+        if ((r = fill_i32(c, dimerOf, -1, nM))) break;
+        if (m->nD > 0) {
+            hipLaunchKernelGGL(k_dimer_of, dim3(blocks_for(m->nD, 256)), dim3(256), 0, c->stream, m->motorA,
+                               m->motorB, m->nD, dimerOf);
+            if ((r = check_launch(c, "k_dimer_of"))) break;
+        }
         long long keys = net->p.gridDims[3];
         if (keys < n) keys = n;
         if (keys < m->nBb) keys = m->nBb;
         if ((r = sort_scratch_reserve(c, net->sort, keys, cap))) break;
     } while (0);
     if (r) {
-        void *bufs[] = {bodyCell, cellContents, ownerStore, ownerSlot, segHeadCnt, segHeads, bbOff, bbList};
+        void *bufs[] = {bodyCell, cellContents, ownerStore, ownerSlot, segHeadCnt, segHeads, bbOff, bbList, dimerOf};
         for (void *b : bufs)
             if (b) hipFree(b);
         return r;
@@ -2637,6 +2696,7 @@ int sbx_net_attach_motors(sbx_net *net, const sbx_motors *m, const sbx_motor_par
     net->segHeads = segHeads;
     net->bbOff = bbOff;
     net->bbList = bbList;
+    net->dimerOf = dimerOf;
     net->mo = *m;
     net->mp = *p;
     net->kp = kin_params(p->kinParams);

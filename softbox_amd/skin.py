@@ -326,14 +326,13 @@ def reach_cell_fn(sc, cfg: ReachSkinConfig):
 def fit_reach(ctx: Context, net, cfg: ReachSkinConfig, t0: int = 0, snapshots: int = 12, iters: int = 400,
               seed: int = 0, horizon: int = 0, log_every: int = 0):
     """The skin model of the reach skin, trained like the filament skin's on a per-step-rebuild trajectory of
-    the actomyosin network `net` (per-segment motion in the reach grid's coarse cells).  The horizon is
-    cfg.horizon (or `horizon`, or 50 steps): the run rebuilds that often and the model sizes skins to last it.
-    Sets cfg.margin and cfg.horizon; returns (model, loss history, samples, next step)."""
+    the actomyosin network `net` (per-segment motion in the reach grid's coarse cells).  The model sizes skins
+    to last `horizon` steps (or cfg.horizon, or 50); the run's schedule (cfg.horizon, the outside-skin
+    trigger) is the caller's.  Sets cfg.margin; returns (model, loss history, samples, next step)."""
     H = horizon or cfg.horizon or 50
     scfg = SkinConfig(skin_default=cfg.skin_default, skin_min=cfg.skin_min, skin_max=cfg.skin_max, horizon=H,
                       window=4 * H)
     model, hist, samples, t = fit(ctx, net, scfg, t0=t0, snapshots=snapshots, iters=iters, seed=seed,
                                   log_every=log_every, cell_fn=reach_cell_fn(net.scene, cfg), coverage=cfg.coverage)
     cfg.margin = scfg.margin
-    cfg.horizon = H
     return model, hist, samples, t
