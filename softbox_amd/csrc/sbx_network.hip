@@ -203,14 +203,13 @@ struct sbx_net {
 namespace {
 
 enum {
-    T_BIN = 0, T_SORT, T_PAIRS, T_UNBIND, T_COUNT, T_LINK, T_ROD, T_MBIND, T_MJOINT, T_DIMER, T_TETHER, T_BACKBONE,
-    T_BOND, T_MINTEG, T_MCSR, T_SKIN_REBUILD, T_SKIN_PAIRS, T_RS_REBUILD, T_COUNT_TIMERS
+    T_BIN = 0, T_SORT, T_PAIRS, T_UNBIND, T_COUNT, T_LINK, T_ROD, T_MBIND, T_MJOINT, T_TETHER, T_BACKBONE,
+    T_BOND, T_MINTEG, T_SKIN_REBUILD, T_SKIN_PAIRS, T_RS_REBUILD, T_COUNT_TIMERS
 };
-const char *kTimerNames[T_COUNT_TIMERS] = {"k_body_cell", "grid_sort",     "k_broad_phase", "k_net_unbind",
-                                           "k_net_count", "k_net_link",    "k_rod",         "k_motor_bind",
-                                           "k_motor_joints", "k_net_dimer", "k_net_tether", "k_backbone",
-                                           "k_bond_head", "k_motor_integ", "motor_csr", "skin_rebuild",
-                                           "k_skin_pairs", "reach_rebuild"};
+const char *kTimerNames[T_COUNT_TIMERS] = {"k_body_cell",    "grid_sort",    "k_broad_phase", "k_net_unbind",
+                                           "k_net_count",    "k_net_link",   "k_rod",         "k_motor_bind",
+                                           "k_motor_joints", "k_net_tether", "k_backbone",    "k_bond_head",
+                                           "k_motor_integ",  "skin_rebuild", "k_skin_pairs",  "reach_rebuild"};
 
 // The bound heads of each segment for this step's motor gather (CrossBridgeSystem.csrBuild, CrossBridgeSystem.java:
 // 189-298, builds a CSR by a counting sort over every segment; few heads are bound, so each bound head takes a
@@ -1088,7 +1087,7 @@ __global__ void __launch_bounds__(256) k_rs_bin(ReachBuild b) {
 // ... and DimerCouplingSystem.couple (DimerCouplingSystem.java:83-184) for a dimer's rods and levers: each of those
 // sub-bodies' threads evaluates the coupling terms that act on it (from the step-start pose, which the joints
 // also read) and adds them after its joints, in the reference's order (variant 0, variant 1, the levers'
-// torsion), so k_net_dimer's read-modify-write pass is not needed.  dimerOf[m]: the dimer of motor m or -1.
+// torsion), so no separate read-modify-write pass over the dimers is needed.  dimerOf[m]: the dimer of motor m or -1.
 __global__ void __launch_bounds__(256) k_motor_joints0(sbx_motors m, JointParams jp, DimerParams dp,
                                                        const int *__restrict__ dimerOf) {
     int s = blockIdx.x * 256 + threadIdx.x;
