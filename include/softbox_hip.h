@@ -672,7 +672,8 @@ int sbx_net_use_graph(sbx_net *net, int on);
  * stream either way.  Off while per-kernel timers are on. */
 int sbx_net_set_overlap(sbx_net *net, int on);
 /* Launch shape of the step: broad-phase lanes per body (2, 4 or 8), the broad phase's neighbourhoods
- * staged in LDS per block (bpStaged 1, with 4 or 8 lanes), scan keys per thread (4, 8 or 16: scan tiles of
+ * staged in LDS per block (bpStaged 1, with 4 or 8 lanes) or split by stencil row over a body's lanes
+ * (bpStaged 2, with 4 or 8 lanes: each lane walks whole rows; for sparse grids), scan keys per thread (4, 8 or 16: scan tiles of
  * 256 * scanItems cells) and k_rod block size (64, 128 or 256).  No counterpart in the reference; results
  * do not depend on it. */
 int sbx_net_set_shape(sbx_net *net, int bpGroup, int bpStaged, int scanItems, int rodThreads);

@@ -342,6 +342,130 @@ void launch_staged(SortedPose pose, Geo geo, const int *cellOffsets, const int *
                        cellOffsets, cellContents, cap, S, candPartner, candCount, maxCand);
 }
 
+// Row-split form of the same query (SpatialGrid.java:340-406), for sparse grids.  The G lanes of a
+// body take whole stencil rows -- lane l the rows l, l + G, ... of the 9 (dz, dy) rows, each the three
+// x-cells of one row: one contents range, two at a periodic wrap -- so a lane loads only its own rows'
+// bounds and walks them in order, with no position-to-range map.  Pass 1 counts every row's hits; the
+// nine counts are exchanged inside the group, which places each row in the slice (rows in (z, y)
+// order, contents positions ascending within a row: the broad phase's order); pass 2 walks again only
+// the rows that hold hits and writes them.  candCount is the full count, as in k_broad_phase.
+template <typename Geo, int G>
+__global__ void __launch_bounds__(BP_THREADS) k_broad_phase_rows(SortedPose pose, Geo geo,
+                                                                 const int *__restrict__ cellOffsets,
+                                                                 const int *__restrict__ cellContents, int cap,
+                                                                 int nSorted, int *__restrict__ candPartner,
+                                                                 int *__restrict__ candCount, int maxCand) {
+    static_assert(G == 4 || G == 8, "4 or 8 lanes per body");
+    constexpr int TB = BP_THREADS / G;      // bodies per block
+    constexpr int NR = (9 + G - 1) / G;     // rows per lane
+    const int tid = threadIdx.x, gl = (tid & 63) % G;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * TB + tid / G;
+    if (t >= cap) return;  // whole groups leave together: the shuffles below stay inside live groups
+    if (t >= nSorted) {
+        if (gl == 0) candCount[t] = 0;
+        return;
+    }
+    const int i = cellContents[t];
+    const int c = pose.cell(t, i);
+    if (c < 0) {
+        if (gl == 0) candCount[i] = 0;
+        return;
+    }
+    const float4 qi = pose.own(t, i);
+    const BodyGeo bg = geo.of(i);
+    const GridGeom &g = bg.g;
+    const int *off = cellOffsets + bg.cellBase;
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ, nXY = nX * nY;
+    const int cl = c - bg.cellBase;
+    const int cz = cl / nXY;
+    const int rem = cl - cz * nXY;
+    const int cy = rem / nX;
+    const int cx = rem - cy * nX;
+    int xa0, xa1, xb0 = 0, xb1 = -1;  // the row's x-cells: [xa0, xa1] then (periodic wrap) [xb0, xb1]
+    if (!g.periodic) {
+        xa0 = cx > 0 ? cx - 1 : 0;
+        xa1 = cx + 1 < nX ? cx + 1 : nX - 1;
+    } else if (cx == 0) {
+        xa0 = nX - 1; xa1 = nX - 1; xb0 = 0; xb1 = 1;
+    } else if (cx == nX - 1) {
+        xa0 = nX - 2; xa1 = nX - 1; xb0 = 0; xb1 = 0;
+    } else {
+        xa0 = cx - 1; xa1 = cx + 1;
+    }
+    const bool twoSeg = xb1 >= xb0;
+    int lo0[NR], hi0[NR], lo1[NR], hi1[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int r = gl + k * G;
+        int zz = cz + r / 3 - 1, yy = cy + r % 3 - 1;
+        bool ok = r < 9;
+        if (g.periodic) {
+            zz = zz < 0 ? zz + nZ : (zz >= nZ ? zz - nZ : zz);
+            yy = yy < 0 ? yy + nY : (yy >= nY ? yy - nY : yy);
+        } else {
+            ok = ok && zz >= 0 && zz < nZ && yy >= 0 && yy < nY;
+        }
+        const int base = zz * nXY + yy * nX;
+        lo0[k] = ok ? off[base + xa0] : 0;
+        hi0[k] = ok ? off[base + xa1 + 1] : 0;
+        lo1[k] = ok && twoSeg ? off[base + xb0] : 0;
+        hi1[k] = ok && twoSeg ? off[base + xb1 + 1] : 0;
+    }
+    // pass 1: hits per row
+    int cnt[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        int h = 0;
+        for (int q = lo0[k]; q < hi0[k]; q++) {
+            const int j = cellContents[q];
+            const float4 qj = pose.at(q, j);
+            h += (j > i && bp_hit(qi, qj, g)) ? 1 : 0;
+        }
+        for (int q = lo1[k]; q < hi1[k]; q++) {
+            const int j = cellContents[q];
+            const float4 qj = pose.at(q, j);
+            h += (j > i && bp_hit(qi, qj, g)) ? 1 : 0;
+        }
+        cnt[k] = h;
+    }
+    // each row's place in the slice: row r is counted by lane r % G of the group, in its slot r / G
+    int start[NR];
+    int total = 0;
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        const int v = __shfl(cnt[r / G], r % G, G);
+        if (r % G == gl) start[r / G] = total;
+        total += v;
+    }
+#pragma unroll
+    for (int r = 9; r < NR * G; r++)
+        if (r % G == gl) start[r / G] = total;  // rows past the ninth hold no hit
+    // pass 2: the rows with hits write them in order
+    int *slice = candPartner + (size_t)i * (size_t)maxCand;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        if (cnt[k] == 0 || start[k] >= maxCand) continue;
+        int slot = start[k];
+        for (int q = lo0[k]; q < hi0[k] && slot < maxCand; q++) {
+            const int j = cellContents[q];
+            if (j > i && bp_hit(qi, pose.at(q, j), g)) slice[slot++] = j;
+        }
+        for (int q = lo1[k]; q < hi1[k] && slot < maxCand; q++) {
+            const int j = cellContents[q];
+            if (j > i && bp_hit(qi, pose.at(q, j), g)) slice[slot++] = j;
+        }
+    }
+    if (gl == 0) candCount[i] = total;
+}
+
+template <typename Geo, int G>
+void launch_rows(SortedPose pose, Geo geo, const int *cellOffsets, const int *cellContents, int cap, int S,
+                 int *candPartner, int *candCount, int maxCand, hipStream_t st) {
+    auto kern = k_broad_phase_rows<Geo, G>;
+    hipLaunchKernelGGL(kern, dim3(blocks_for((long long)cap * G, BP_THREADS)), dim3(BP_THREADS), 0, st, pose, geo,
+                       cellOffsets, cellContents, cap, S, candPartner, candCount, maxCand);
+}
+
 template <typename Geo, int G>
 void launch_sorted(sbx_ctx *c, SortedPose pose, Geo geo, const int *cellOffsets, const int *cellContents, int cap,
                    int S, int *candPartner, int *candCount, int maxCand, hipStream_t st) {
@@ -357,6 +481,23 @@ int launch_broad_phase_sorted(sbx_ctx *c, const float4 *sortedPose, const int *s
     SBX_REQUIRE(c, group == 2 || group == 4 || group == 8, "broad phase: %d lanes per body (2, 4 or 8)", group);
     SortedPose pose{sortedPose, sortedCell};
     hipStream_t q = st ? st : c->stream;
+    if (staged == 2) {
+        SBX_REQUIRE(c, group == 4 || group == 8, "row-split broad phase: 4 or 8 lanes per body");
+        if (reps) {
+            const BatchGeo geo{reps, rodRep};
+            if (group == 8) launch_rows<BatchGeo, 8>(pose, geo, cellOffsets, cellContents, cap, S, candPartner,
+                                                     candCount, maxCand, q);
+            else launch_rows<BatchGeo, 4>(pose, geo, cellOffsets, cellContents, cap, S, candPartner, candCount,
+                                          maxCand, q);
+        } else {
+            const SingleGeo geo{g};
+            if (group == 8) launch_rows<SingleGeo, 8>(pose, geo, cellOffsets, cellContents, cap, S, candPartner,
+                                                      candCount, maxCand, q);
+            else launch_rows<SingleGeo, 4>(pose, geo, cellOffsets, cellContents, cap, S, candPartner, candCount,
+                                           maxCand, q);
+        }
+        return check_launch(c, "k_broad_phase_rows");
+    }
     if (staged) {
         SBX_REQUIRE(c, group == 4 || group == 8, "staged broad phase: 4 or 8 lanes per body");
         if (reps) {

@@ -136,7 +136,7 @@ struct sbx_net {
     int *rodRep = nullptr, *linkRep = nullptr;        // replica of each rod / link
     // launch shape (results do not depend on it; sbx_net_tune picks it by timing steps)
     int bpGroup = 4;      // broad-phase lanes per body (2, 4 or 8)
-    int bpStaged = 0;     // 1: each block's neighbourhood staged in LDS (4 or 8 lanes)
+    int bpStaged = 0;     // 1: each block's neighbourhood staged in LDS; 2: rows split over the lanes (4 or 8)
     int rodThreads = 256; // k_rod block size (64, 128 or 256), or RODS_PAIRED
     int mbGroup = 16;     // lanes per motor in k_motor_bind (16, 32 or 64)
     // graph
@@ -3477,8 +3477,8 @@ int sbx_net_set_shape(sbx_net *net, int bpGroup, int bpStaged, int scanItems, in
     sbx_ctx *c = net->ctx;
     SBX_CHECK_CTX(c);
     SBX_REQUIRE(c, bpGroup == 2 || bpGroup == 4 || bpGroup == 8, "net_set_shape: bpGroup 2, 4 or 8");
-    SBX_REQUIRE(c, bpStaged == 0 || (bpStaged == 1 && bpGroup != 2), "net_set_shape: bpStaged 0, or 1 with 4 or 8 "
-                   "lanes per body");
+    SBX_REQUIRE(c, bpStaged == 0 || ((bpStaged == 1 || bpStaged == 2) && bpGroup != 2),
+                "net_set_shape: bpStaged 0, or 1 (LDS-staged) or 2 (row-split) with 4 or 8 lanes per body");
     SBX_REQUIRE(c, scanItems == 4 || scanItems == 8 || scanItems == 16, "net_set_shape: scanItems 4, 8 or 16");
     SBX_REQUIRE(c, rodThreads == 64 || rodThreads == 128 || rodThreads == 256 || rodThreads == RODS_PAIRED,
                 "net_set_shape: rodThreads 64, 128, 256 or 512 (the noise-paired form)");
@@ -3508,10 +3508,10 @@ int sbx_net_tune(sbx_net *net, int step0, int trialSteps, int *advanced, int *sh
     float bestMs = -1.0f;
     for (int oi = 0; oi < overlaps && !r; oi++)
     for (int gi = 0; gi < 3 && !r; gi++)
-    for (int si = 0; si < 2 && !r; si++)
+    for (int si = 0; si < 3 && !r; si++)
         for (int ii = 0; ii < 3 && !r; ii++)
             for (int ti = 0; ti < 3 && !r; ti++) {
-                if (si == 1 && groups[gi] == 2) continue;  // staging takes 4 or 8 lanes per body
+                if (si >= 1 && groups[gi] == 2) continue;  // staged and row-split forms take 4 or 8 lanes
                 if ((r = sbx_net_set_shape(net, groups[gi], si, items[ii], threads[ti]))) break;
                 net->overlap = oi == 0 ? 1 : 0;
                 if ((r = sbx_net_step(net, t, 2))) break;  // warm (and capture, with graphs on)

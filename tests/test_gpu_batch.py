@@ -73,7 +73,8 @@ def _check_replica(bt, r, sc, want, what):
     assert_same(cell, wcell, f"{what}: replica {r} bodyCell")
 
 
-@pytest.mark.parametrize("graph,overlap,staged", [(False, False, 0), (False, True, 0), (True, True, 0), (True, True, 1)])
+@pytest.mark.parametrize("graph,overlap,staged", [(False, False, 0), (False, True, 0), (True, True, 0), (True, True, 1),
+                                                  (True, True, 2)])
 def test_batch_replicas_equal_stepping_alone(ctx, graph, overlap, staged):
     from softbox_amd.batch import Batch
 
@@ -83,7 +84,7 @@ def test_batch_replicas_equal_stepping_alone(ctx, graph, overlap, staged):
     bt = Batch(ctx, [sc.copy() for sc in reps])
     bt.net.set_overlap(overlap)
     if staged:
-        bt.net.set_shape(4, 8, 256, 1)
+        bt.net.set_shape(4, 8, 256, staged)
     if graph:
         bt.net.use_graph(2)
     bt.step(0, 5)

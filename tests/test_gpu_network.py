@@ -99,11 +99,14 @@ def test_link_edits_between_steps_are_recounted(ctx, graph):
     net.close()
 
 
+@pytest.mark.parametrize("staged", [1, 2])
 @pytest.mark.parametrize("lanes", [4, 8])
 @pytest.mark.parametrize("periodic,dense", [(True, False), (True, True), (False, True)])
-def test_staged_broad_phase_matches_oracle(ctx, lanes, periodic, dense):
+def test_staged_broad_phase_matches_oracle(ctx, lanes, periodic, dense, staged):
     """The LDS-staged broad phase (each block's 9 neighbourhood ranges in LDS; ranges outside them,
-    at periodic wraps and grid edges, read global memory) gives the reference's slices and grid."""
+    at periodic wraps and grid edges, read global memory) and the row-split one (each lane walks whole
+    stencil rows, the rows placed in the slice by the group's counts) give the reference's slices and
+    grid."""
     from softbox_amd.network import Network
 
     if dense:  # ~2 bodies per fine cell: staged ranges of hundreds of entries, some blocks over capacity
@@ -113,11 +116,31 @@ def test_staged_broad_phase_matches_oracle(ctx, lanes, periodic, dense):
         sc = small_network(seed=13, n_fil=300, links=600, periodic=periodic)
     orc = ffi.OracleNet(sc)
     net = Network(ctx, sc)
-    net.set_shape(lanes, 8, 256, 1)
+    net.set_shape(lanes, 8, 256, staged)
     for t in range(6):
         orc.step(t, 1)
     net.step(0, 6)
-    _compare(net, orc, sc, f"staged lanes={lanes} periodic={periodic} dense={dense}")
+    _compare(net, orc, sc, f"staged={staged} lanes={lanes} periodic={periodic} dense={dense}")
+    net.close()
+
+
+@pytest.mark.parametrize("staged", [1, 2])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_broad_phase_slice_overflow_matches_oracle(ctx, periodic, staged):
+    """maxCand 3 on a dense scene: slices keep the first 3 hits in the broad phase's order and the counts
+    are the full counts, in both block-level forms."""
+    from softbox_amd.network import Network
+
+    sc = scenes.network(n_fil=500, segs=10, box=1.6 if periodic else 1.2, n_links=800, seed=14,
+                        periodic=periodic, confine=False)
+    sc.p["maxCand"] = 3
+    orc = ffi.OracleNet(sc)
+    net = Network(ctx, sc)
+    net.set_shape(4, 8, 256, staged)
+    for t in range(4):
+        orc.step(t, 1)
+    net.step(0, 4)
+    _compare(net, orc, sc, f"maxCand 3 staged={staged} periodic={periodic}")
     net.close()
 
 
