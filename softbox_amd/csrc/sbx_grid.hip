@@ -6,6 +6,7 @@
 // a wavefront's 27-cell neighbourhoods overlap and the offsets/contents/centre reads stay in L2.
 #include "sbx_device.h"
 #include "sbx_grid.h"
+#include <type_traits>
 #include "sbx_internal.h"
 
 namespace sbx {
@@ -276,7 +277,8 @@ __global__ void __launch_bounds__(BP_THREADS) k_broad_phase_staged(SortedPose po
     const BodyGeo bg = geo.of(i);
     const GridGeom &g = bg.g;
     int pre[18], shift[18];
-    const int total = stencil_map(c - bg.cellBase, g, cellOffsets + bg.cellBase, pre, shift);
+    bool wraps = false;
+    const int total = stencil_map(c - bg.cellBase, g, cellOffsets + bg.cellBase, pre, shift, &wraps);
     // ranges inside their row's staged range read LDS: their shift then maps a walk position to an
     // LDS index (bit r of inLds)
     unsigned inLds = 0;
@@ -295,33 +297,44 @@ __global__ void __launch_bounds__(BP_THREADS) k_broad_phase_staged(SortedPose po
     int *slice = candPartner + (size_t)i * (size_t)maxCand;
     const unsigned long long groupBits = G == 64 ? ~0ull : ((1ull << G) - 1) << ((lane / G) * G);
     const unsigned long long below = (1ull << lane) - 1;
-    int cnt = 0;
-    for (int base = 0; base < total; base += G * U) {
-        int at[U], jj[U];
-        bool fromLds[U];
+    int cnt = 0, base = 0;
+    // No row of any body in the wave wraps (cells off the periodic x faces): the 9 odd ranges are empty
+    // and own no position, so the even ranges alone map a position.
+    const bool anyWrap = __ballot(wraps) != 0ull;
+    // One round: positions base + k G + gl (k < UU) of every body, in ascending position order.
+    auto round = [&](auto width) {
+        constexpr int UU = decltype(width)::value;
+        int at[UU], jj[UU];
+        bool fromLds[UU];
 #pragma unroll
-        for (int k = 0; k < U; k++) {
+        for (int k = 0; k < UU; k++) {
             const int q = base + k * G + gl;
             int sh = shift[0], rr = 0;
+            if (anyWrap) {
 #pragma unroll
-            for (int r = 1; r < 18; r++)
-                if (q >= pre[r]) { sh = shift[r]; rr = r; }
+                for (int r = 1; r < 18; r++)
+                    if (q >= pre[r]) { sh = shift[r]; rr = r; }
+            } else {
+#pragma unroll
+                for (int r = 2; r < 18; r += 2)
+                    if (q >= pre[r]) { sh = shift[r]; rr = r; }
+            }
             fromLds[k] = q < total && ((inLds >> rr) & 1u);
             at[k] = q < total ? q + sh : t;
         }
-        float4 qj[U];
+        float4 qj[UU];
 #pragma unroll
-        for (int k = 0; k < U; k++) {
+        for (int k = 0; k < UU; k++) {
             const bool valid = base + k * G + gl < total;
             jj[k] = !valid ? -1 : (fromLds[k] ? sJ[at[k]] : cellContents[at[k]]);
         }
 #pragma unroll
-        for (int k = 0; k < U; k++) {
+        for (int k = 0; k < UU; k++) {
             const bool use = jj[k] > i;
             qj[k] = !use ? qi : (fromLds[k] ? sP[at[k]] : pose.at(at[k], jj[k]));
         }
 #pragma unroll
-        for (int k = 0; k < U; k++) {
+        for (int k = 0; k < UU; k++) {
             const bool hit = jj[k] > i && bp_hit(qi, qj[k], g);
             const unsigned long long m = __ballot(hit) & groupBits;
             if (hit) {
@@ -330,6 +343,12 @@ __global__ void __launch_bounds__(BP_THREADS) k_broad_phase_staged(SortedPose po
             }
             cnt += __popcll(m);
         }
+        base += G * UU;
+    };
+    // full rounds while some body of the wave has more than G positions left, then single ones
+    while (base < total) {
+        if (__ballot(base + G < total) != 0ull) round(std::integral_constant<int, U>{});
+        else round(std::integral_constant<int, 1>{});
     }
     if (gl == 0) candCount[i] = cnt;
 }

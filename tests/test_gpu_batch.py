@@ -128,12 +128,13 @@ def test_launch_shapes_and_tuner_change_only_time(ctx):
     reps = _replicas(seed=5)
     bt = Batch(ctx, [sc.copy() for sc in reps])
     t = 0
-    for shape in ((2, 4, 64, 0), (4, 16, 128, 1), (8, 8, 256, 0), (8, 4, 256, 1), (4, 8, 512, 0), (4, 8, 256, 0)):
+    for shape in ((2, 4, 64, 0), (4, 16, 128, 1), (8, 8, 256, 0), (8, 4, 256, 1), (4, 8, 512, 0), (4, 8, 256, 0),
+                  (4, 8, 128, 2), (8, 16, 256, 2)):
         bt.net.set_shape(*shape)
         bt.step(t, 3)
         t += 3
-    res = bt.net.tune(t, trial_steps=1)  # (1 + 2 + 2) x 3 x 3 shapes, overlapped and not, 2 + 1 steps each
-    assert res["steps"] == 90 * 3 and res["shape"][0] in (2, 4, 8) and res["ms_per_step"] > 0, res
+    res = bt.net.tune(t, trial_steps=1)  # (1 + 3 + 3) x 3 x 3 shapes, overlapped and not, 2 + 1 steps each
+    assert res["steps"] == 126 * 3 and res["shape"][0] in (2, 4, 8) and res["ms_per_step"] > 0, res
     t += res["steps"]
     bt.step(t, 2)
     t += 2
