@@ -120,7 +120,8 @@ class Network:
         self.ctx.check(self.ctx.lib.sbx_net_set_overlap(self.h, 1 if on else 0))
 
     def set_shape(self, bp_group: int = 4, scan_items: int = 8, rod_threads: int = 256, bp_staged: int = 0) -> None:
-        """Launch shape of the step (results do not depend on it)."""
+        """Launch shape of the step (results do not depend on it).  bp_staged: 0 global walk, 1 LDS-staged
+        neighbourhoods, 2 stencil rows split over the body's lanes (1 and 2 take 4 or 8 lanes)."""
         self.ctx.check(self.ctx.lib.sbx_net_set_shape(self.h, bp_group, bp_staged, scan_items, rod_threads))
 
     def tune(self, step0: int, trial_steps: int = 20) -> dict:
