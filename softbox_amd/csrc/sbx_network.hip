@@ -988,16 +988,20 @@ struct ReachBuild {
 // that segment t * sampleStride leads (is first in), so a rebuild evaluates at most about n / sampleStride
 // cells whatever the grid's size; the tallies estimate every segment's predicted skin.
 __global__ void __launch_bounds__(256) k_rs_assign(ReachBuild b) {
+    __shared__ float ws[MLP_NPARAM];  // the model: 21 KB, more than the scalar cache holds
     const long long sl = (long long)(blockIdx.x * 256 + threadIdx.x) * b.sampleStride;
-    if (sl >= b.n) return;
-    const int s = (int)sl;
-    const int c = b.cell[s];
-    if (c < 0 || b.cont[b.off[c]] != s) return;
+    const int s = sl < b.n ? (int)sl : 0;
+    const int c = sl < b.n ? b.cell[s] : -1;
+    const bool leads = c >= 0 && b.cont[b.off[c]] == s;
+    if (!__syncthreads_or(leads)) return;  // the whole block: no cell to evaluate
+    for (int q = threadIdx.x; q < MLP_NPARAM; q += 256) ws[q] = b.model[q];
+    __syncthreads();
+    if (!leads) return;
     GridGeom g{};  // raw displacements, as the guard takes them
     float f[SKIN_FEATURES], y[MLP_OUT];
     skin_cell_features(c, b.off, b.cont, b.n, b.coord, b.ref, b.uVec, b.refU, b.links, b.since,
                        __int_as_float(b.words[RS_GEOM + 3]), g, f);
-    mlp_forward(b.model, f, y);
+    mlp_forward(ws, f, y);
     const float sk = fminf(b.skinMax, fmaxf(b.skinMin, b.margin * expf(y[0])));
     const float u = logf(sk / b.skinMin) / logf(b.skinMax / b.skinMin);  // [0, 1]
     const int bin = min(RS_HBINS - 1, max(0, (int)(u * (float)RS_HBINS)));
@@ -1355,11 +1359,22 @@ constexpr int SKF_MAXSKIN = 6;         // flag word: largest skin of the rebuild
 // last rebuild -- features over its members, model, clamp, then the skin of every member;
 // otherwise thread i is a body and gets the default skin.  Tracks the largest skin (flag word).
 __global__ void __launch_bounds__(256) k_skin_assign(SkinArgs a, const float *__restrict__ model) {
+    __shared__ float ws[MLP_NPARAM];  // the model: 21 KB, more than the scalar cache holds
     if (a.flags[a.gate] == 0) return;
     int k = blockIdx.x * 256 + threadIdx.x;
-    if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && model) {
-        if (k >= *a.cCells) return;
-        const int p0 = a.cOff[k], p1 = a.cOff[k + 1];
+    if (a.mode == SBX_SKIN_LEARNED && a.flags[SKF_REBUILDS] > 0 && model) {  // uniform over the grid
+        const int nC = *a.cCells;
+        const int p0 = k < nC ? a.cOff[k] : 0, p1 = k < nC ? a.cOff[k + 1] : 0;
+        if (!__syncthreads_or(p1 > p0)) {  // the whole block: empty cells or none
+            if (k < nC) {
+                a.cellSkin[k] = 0.0f;
+                a.cellHorizon[k] = 0.0f;
+            }
+            return;
+        }
+        for (int q = threadIdx.x; q < MLP_NPARAM; q += 256) ws[q] = model[q];
+        __syncthreads();
+        if (k >= nC) return;
         if (p1 == p0) {  // an empty cell has no member to give a skin: no model evaluation
             a.cellSkin[k] = 0.0f;
             a.cellHorizon[k] = 0.0f;
@@ -1369,7 +1384,7 @@ __global__ void __launch_bounds__(256) k_skin_assign(SkinArgs a, const float *__
         float H = (float)(a.flags[SKF_SINCE] > 0 ? a.flags[SKF_SINCE] : 1);
         skin_cell_features(k, a.cOff, a.cCont, a.n, a.coord, a.refC, a.uVec, a.refU, a.links, H,
                            1.0f / a.cg->invCell, a.geom, f);
-        mlp_forward(model, f, y);  // every lane reads the same weight: scalar loads, no LDS round trips
+        mlp_forward(ws, f, y);  // every lane reads the same weight: LDS broadcasts
         float sk = fminf(a.skinMax, fmaxf(a.skinMin, a.margin * expf(y[0])));
         a.cellSkin[k] = sk;
         a.cellHorizon[k] = expf(y[1]);
